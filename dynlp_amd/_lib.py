@@ -1,0 +1,83 @@
+"""ctypes binding of libdynlp_hip.so (the C ABI declared in include/dynlp.h).
+
+The library is loaded from this package's folder.  There is no fallback: if it is missing
+or fails to load, every call raises, naming the path and the build command.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdynlp_hip.so")
+
+F32, F64 = 0, 1
+_lib = None
+
+P = ct.c_void_p
+I32 = ct.c_int32
+I64 = ct.c_int64
+DBL = ct.c_double
+SZ = ct.c_size_t
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "dlp_last_error": (ct.c_char_p, []),
+    "dlp_abi_version": (ct.c_int, []),
+    "dlp_spmm_axpy": (ct.c_int, [I32, I32, P, P, P, P, I64, P, DBL, P, I64, I32, DBL, P]),
+    "dlp_row_operator": (ct.c_int, [I32, P, I32, P, P, P, P, I32, P, P, I64, I32, P, DBL, P, I64,
+                                    P, I64, I32, I32, DBL, P]),
+    "dlp_rho_cols": (ct.c_int, [I32, P, I32, P, I64, I32, P, P, P]),
+    "dlp_violating_rows": (ct.c_int, [I32, P, I32, P, I64, I32, P, P, P, P, P]),
+    "dlp_frontier_jacobi": (ct.c_int, [I32, P, I32, P, P, P, P, P, P, DBL, P, P, P, I32, DBL, P]),
+    "dlp_push_scatter": (ct.c_int, [I32, P, I32, P, P, P, P, P, P, P, I32, DBL, P]),
+    "dlp_mark_neighbors": (ct.c_int, [I32, P, P, P, P, P, P]),
+    "dlp_flatnonzero_u8": (ct.c_int, [I32, P, P, P, P, SZ, P]),
+    "dlp_scan_workspace_bytes": (SZ, [I32]),
+    "dlp_fused_push_round": (ct.c_int, [P, P, P, P, P, P, P, I32, P, P, P, P, P, P, I32, I32, P]),
+    "dlp_compact_frontier": (ct.c_int, [P, P, P, P, I32, P, P, P, P, P, I32, I32, P]),
+    "dlp_cc_edges": (ct.c_int, [I32, I64, P, P, P, P, P, SZ, P]),
+    "dlp_cc_csr": (ct.c_int, [I32, P, P, P, P, P, P, P, SZ, P]),
+    "dlp_cc_workspace_bytes": (SZ, [I32]),
+}
+
+
+class DLPError(RuntimeError):
+    pass
+
+
+def load():
+    """The loaded library (raises DLPError if it is not built or does not load)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DLPError(f"{LIB_PATH} is missing: build it with `make -C {_HERE}/csrc` "
+                       "(or __graft_entry__.build()); there is no fallback path")
+    try:
+        lib = ct.CDLL(LIB_PATH)
+    except OSError as e:
+        raise DLPError(f"cannot load {LIB_PATH}: {e}") from e
+    for name, (res, args) in SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+SIGS = _SIGS
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def call(name, *args):
+    """Call a status-returning entry point; raise DLPError with the library's message."""
+    fn = getattr(load(), name)
+    rc = fn(*args)
+    if rc != 0:
+        msg = load().dlp_last_error().decode(errors="replace")
+        raise DLPError(f"{name} failed ({rc}): {msg}")
+    return rc

@@ -1,0 +1,164 @@
+"""Device backend: torch-ROCm tensors as device buffers, HIP kernels through the C ABI.
+
+Mirrors dynlp/backend.py of the reference (backend.py:1-56): the array namespace, CSR
+container, unbuffered scatters, device sync, memory use and a synchronising Timer.  The
+reference bound `xp` to CuPy unconditionally; here DYNLP_BACKEND selects the backend and the
+only product backend is "hip" (one MI355X, gfx950).  Any other value raises: the FP64 CPU
+restatement lives outside this package (it is test infrastructure), so there is no CPU
+fallback to fall into.
+"""
+from __future__ import annotations
+
+import os
+import time
+
+import torch
+
+from . import _lib
+
+BACKEND = os.environ.get("DYNLP_BACKEND", "hip")
+if BACKEND != "hip":
+    raise ImportError(f"DYNLP_BACKEND={BACKEND!r}: this package runs on the HIP backend only")
+
+xp = torch
+float32, float64, int32, int64, uint8 = torch.float32, torch.float64, torch.int32, torch.int64, torch.uint8
+
+
+def device():
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream():
+    """Current torch stream as a raw hipStream_t (what the C ABI takes)."""
+    return torch.cuda.current_stream().cuda_stream
+
+
+def ptr(t):
+    return None if t is None else t.data_ptr()
+
+
+def dtype_code(dt):
+    if dt == torch.float32:
+        return _lib.F32
+    if dt == torch.float64:
+        return _lib.F64
+    raise TypeError(f"unsupported dtype {dt}")
+
+
+def asarray(x, dtype=None):
+    """Device tensor from a host array / tensor (copies host data to the device)."""
+    if isinstance(x, torch.Tensor):
+        t = x.to(device())
+    else:
+        t = torch.as_tensor(x).to(device())
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.contiguous()
+
+
+def zeros(*shape, dtype=torch.float32):
+    return torch.zeros(*shape, dtype=dtype, device=device())
+
+
+def empty(*shape, dtype=torch.float32):
+    return torch.empty(*shape, dtype=dtype, device=device())
+
+
+def scatter_add(a, idx, v):
+    """a[idx] += v, unbuffered (repeated indices accumulate) (backend.py:19-20)."""
+    a.index_put_((idx.long(),), torch.as_tensor(v, dtype=a.dtype, device=a.device).expand(idx.shape[0], *a.shape[1:]),
+                 accumulate=True)
+
+
+def scatter_max(a, idx, v):
+    """a[idx] = max(a[idx], v), unbuffered (backend.py:23-24)."""
+    a.scatter_reduce_(0, idx.long(), torch.as_tensor(v, dtype=a.dtype, device=a.device).expand(idx.shape[0]),
+                      reduce="amax", include_self=True)
+
+
+def scatter_min(a, idx, v):
+    """a[idx] = min(a[idx], v), unbuffered (backend.py:27-28)."""
+    a.scatter_reduce_(0, idx.long(), torch.as_tensor(v, dtype=a.dtype, device=a.device).expand(idx.shape[0]),
+                      reduce="amin", include_self=True)
+
+
+def sync():
+    torch.cuda.synchronize()
+
+
+def mem_used_gb() -> float:
+    return torch.cuda.memory_allocated() / 1e9
+
+
+class Timer:
+    """Wall-clock timer that synchronizes the device on enter and exit (backend.py:43-56)."""
+
+    def __init__(self):
+        self.ms = 0.0
+
+    def __enter__(self):
+        sync()
+        self._t = time.perf_counter()
+        return self
+
+    def __exit__(self, *exc):
+        sync()
+        self.ms += (time.perf_counter() - self._t) * 1e3
+
+
+class CSR:
+    """Device CSR (int32 indptr/indices, FP32 or FP64 data).  `rowlen` (optional) makes it a
+    slack CSR: row r holds rowlen[r] live entries from indptr[r]."""
+
+    def __init__(self, data, indices, indptr, shape, rowlen=None):
+        self.data = data
+        self.indices = indices
+        self.indptr = indptr
+        self.shape = tuple(int(s) for s in shape)
+        self.rowlen = rowlen
+        self._nnz = None
+
+    @property
+    def nnz(self):
+        if self._nnz is None:
+            self._nnz = int(self.rowlen.sum()) if self.rowlen is not None else int(self.indptr[-1])
+        return self._nnz
+
+    @property
+    def avg_deg(self):
+        return self.nnz / max(self.shape[0], 1)
+
+    @property
+    def dtype(self):
+        return self.data.dtype
+
+    def astype(self, dt):
+        if dt == self.data.dtype:
+            return self
+        return CSR(self.data.to(dt), self.indices, self.indptr, self.shape, self.rowlen)
+
+    def __matmul__(self, X):
+        from .kernels import spmm_axpy
+        if X.dim() == 1:
+            return spmm_axpy(self, X[:, None].contiguous())[:, 0]
+        return spmm_axpy(self, X)
+
+    def row_ids(self):
+        """Row index of every stored entry (graphs.csr_rows)."""
+        counts = (self.indptr[1:] - self.indptr[:-1]).long()
+        return torch.repeat_interleave(torch.arange(self.shape[0], device=self.indptr.device, dtype=torch.int32),
+                                       counts)
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.data.cpu().numpy(), self.indices.cpu().numpy(), self.indptr.cpu().numpy()),
+                             shape=self.shape)
+
+
+def csr(data, indices, indptr, shape):
+    """(backend.py:35-36)."""
+    return CSR(data, indices, indptr, shape)
+
+
+def csr_from_host(indptr, indices, data, n, dtype=torch.float32):
+    return CSR(asarray(data, dtype), asarray(indices, torch.int32), asarray(indptr, torch.int32), (n, n))

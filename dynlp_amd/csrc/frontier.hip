@@ -1,0 +1,343 @@
+// Frontier primitives: Jacobi on a vertex subset, residual scatter, neighbour marking, and the
+// asynchronous fused push (take residual, update, scatter, compact).
+#include "rowgather.h"
+
+namespace dlp {
+
+// Y[g] = (rhs[f] + sum w X[col]) / diag[f]; changed[g] = any |Y - X[f]| > delta.
+template <typename T, int CPL>
+__global__ __launch_bounds__(256) void k_frontier_jacobi(
+    int nf, const int32_t* __restrict__ frontier, int C, int g_shift, int lc_shift,
+    const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+    const T* __restrict__ w, const T* __restrict__ X, const T* __restrict__ rhs,
+    const T* __restrict__ diag, T delta, T* __restrict__ Y, uint8_t* __restrict__ changed,
+    unsigned long long* __restrict__ edges) {
+  const Lanes L(g_shift, lc_shift);
+  const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> g_shift;
+  const bool valid = g < nf;
+  const int r = valid ? frontier[g] : 0;
+  const int beg = valid ? indptr[r] : 0, end = valid ? indptr[r + 1] : 0;
+  T acc[CPL];
+#pragma unroll
+  for (int k = 0; k < CPL; ++k) acc[k] = 0;
+  row_gather<T, T, T, CPL>(L, beg, end, indices, w, nullptr, X, C, C, acc);
+  group_reduce<T, CPL>(L, acc);
+  bool ch = false;
+  if (valid && L.e == 0) {
+    const T inv = (T)1 / diag[r];
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) {
+      const int c = L.l + k * L.LC;
+      if (c < C) {
+        const T y = (rhs[(int64_t)r * C + c] + acc[k]) * inv;
+        Y[g * C + c] = y;
+        ch |= fabs(y - X[(int64_t)r * C + c]) > delta;
+      }
+    }
+  }
+  const unsigned long long bal = __ballot(ch);
+  const int wl = threadIdx.x & 63;
+  const unsigned long long gm = (L.G == 64 ? ~0ull : ((1ull << L.G) - 1)) << (wl & ~(L.G - 1));
+  if (valid && L.lane == 0) changed[g] = (bal & gm) != 0;
+  if (edges) {
+    const unsigned long long s = wave_sum_u64((valid && L.lane == 0) ? (unsigned long long)(end - beg) : 0ull);
+    if (wl == 0 && s) atomicAdd(edges, s);
+  }
+}
+
+// R[col] += w * D[g] and mark[col] = 1 over the rows of the frontier.
+template <typename T, int CPL>
+__global__ __launch_bounds__(256) void k_push_scatter(
+    int nf, const int32_t* __restrict__ frontier, int C, int g_shift, int lc_shift,
+    const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+    const T* __restrict__ w, const T* __restrict__ D, T* __restrict__ R,
+    uint8_t* __restrict__ mark, unsigned long long* __restrict__ edges) {
+  const Lanes L(g_shift, lc_shift);
+  const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> g_shift;
+  const bool valid = g < nf;
+  const int r = valid ? frontier[g] : 0;
+  const int beg = valid ? indptr[r] : 0, end = valid ? indptr[r + 1] : 0;
+  T d[CPL];
+#pragma unroll
+  for (int k = 0; k < CPL; ++k) {
+    const int c = L.l + k * L.LC;
+    d[k] = (valid && c < C) ? D[g * C + c] : (T)0;
+  }
+  for (int j = beg + L.e; j < end; j += L.LE) {
+    const int v = indices[j];
+    const T wv = w[j];
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) {
+      const int c = L.l + k * L.LC;
+      if (c < C) atomic_add(&R[(int64_t)v * C + c], wv * d[k]);
+    }
+    if (L.l == 0) mark[v] = 1;
+  }
+  if (edges) {
+    const unsigned long long s = wave_sum_u64((valid && L.lane == 0) ? (unsigned long long)(end - beg) : 0ull);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(edges, s);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mark_neighbors(int nr, const int32_t* __restrict__ rows,
+                                                        const int32_t* __restrict__ indptr,
+                                                        const int32_t* __restrict__ indices,
+                                                        uint8_t* __restrict__ mark,
+                                                        unsigned long long* __restrict__ edges) {
+  const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+  const int lane = threadIdx.x & 7;
+  const bool valid = g < nr;
+  const int r = valid ? rows[g] : 0;
+  const int beg = valid ? indptr[r] : 0, end = valid ? indptr[r + 1] : 0;
+  for (int j = beg + lane; j < end; j += 8) mark[indices[j]] = 1;
+  if (edges) {
+    const unsigned long long s = wave_sum_u64((valid && lane == 0) ? (unsigned long long)(end - beg) : 0ull);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(edges, s);
+  }
+}
+
+// One asynchronous push round (chaotic relaxation): the residual of every frontier vertex is
+// read-and-zeroed atomically so pushes landing during the round are kept for the next one.
+template <typename T, int CPL>
+__global__ __launch_bounds__(256) void k_fused_push_round(
+    const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+    const T* __restrict__ w, const T* __restrict__ diag, const int32_t* __restrict__ front,
+    const int32_t* __restrict__ fcount, int32_t* __restrict__ fcount_next, int C, int g_shift,
+    int lc_shift, T* __restrict__ Z, T* __restrict__ R, int32_t* __restrict__ mark,
+    int32_t* __restrict__ touched, int32_t* __restrict__ tcount,
+    unsigned long long* __restrict__ edges) {
+  const Lanes L(g_shift, lc_shift);
+  const int nf = *fcount;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *fcount_next = 0;
+  const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> g_shift;
+  const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> g_shift;
+  const int64_t wave_first = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) >> g_shift;
+  unsigned long long my_edges = 0;
+  // iterate in wave-uniform steps so the cross-lane reductions see whole waves
+  for (int64_t base = wave_first; base < nf; base += groups) {
+    const int64_t g = base + (g0 - wave_first);
+    const bool valid = g < nf;
+    const int u = valid ? front[g] : 0;
+    T d[CPL];
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) d[k] = 0;
+    if (valid && L.e == 0) {
+      const T inv = (T)1 / diag[u];
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) {
+        const int c = L.l + k * L.LC;
+        if (c < C) {
+          d[k] = atomic_take(&R[(int64_t)u * C + c]) * inv;
+          Z[(int64_t)u * C + c] += d[k];
+        }
+      }
+    }
+    // broadcast the column values from edge lane 0 to the other edge lanes of the group
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) d[k] = __shfl(d[k], L.l, L.G);
+    if (valid) {
+      const int beg = indptr[u], end = indptr[u + 1];
+      if (L.lane == 0) my_edges += (unsigned long long)(end - beg);
+      for (int j = beg + L.e; j < end; j += L.LE) {
+        const T wv = w[j];
+        if (wv == (T)0) continue;  // inert entries (outside the solved set)
+        const int v = indices[j];
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) {
+          const int c = L.l + k * L.LC;
+          if (c < C) atomic_add(&R[(int64_t)v * C + c], wv * d[k]);
+        }
+        if (L.l == 0 && atomicExch(&mark[v], 1) == 0) touched[atomicAdd(tcount, 1)] = v;
+      }
+    }
+  }
+  if (edges) {
+    const unsigned long long s = wave_sum_u64(my_edges);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(edges, s);
+  }
+}
+
+// Touched vertices are unmarked; violators form the next frontier.
+template <typename T>
+__global__ __launch_bounds__(256) void k_compact_frontier(
+    const int32_t* __restrict__ touched, const int32_t* __restrict__ tcount,
+    int32_t* __restrict__ tcount_next, int32_t* __restrict__ mark, int C, int lc_shift,
+    const T* __restrict__ R, const T* __restrict__ diag, const T* __restrict__ eps,
+    int32_t* __restrict__ out, int32_t* __restrict__ fcount_out) {
+  const int LC = 1 << lc_shift;
+  const int nt = *tcount;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *tcount_next = 0;
+  const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> lc_shift;
+  const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> lc_shift;
+  const int64_t wave_first = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) >> lc_shift;
+  const int l = threadIdx.x & (LC - 1);
+  const int wl = threadIdx.x & 63;
+  const unsigned long long gm = (LC == 64 ? ~0ull : ((1ull << LC) - 1)) << (wl & ~(LC - 1));
+  for (int64_t base = wave_first; base < nt; base += groups) {
+    const int64_t i = base + (g0 - wave_first);
+    const bool valid = i < nt;
+    const int v = valid ? touched[i] : 0;
+    bool viol = false;
+    if (valid) {
+      if (l == 0) mark[v] = 0;
+      const T dv = diag[v];
+      for (int c = l; c < C; c += LC) viol |= fabs(R[(int64_t)v * C + c]) > eps[c] * dv;
+    }
+    const unsigned long long bal = __ballot(viol);
+    if (valid && l == 0 && (bal & gm)) out[atomicAdd(fcount_out, 1)] = v;
+  }
+}
+
+}  // namespace dlp
+
+using namespace dlp;
+
+#define DLP_DISPATCH_CPL(CPL, MACRO) \
+  switch (CPL) {                     \
+    case 1: MACRO(1); break;         \
+    case 2: MACRO(2); break;         \
+    case 3: MACRO(3); break;         \
+    default: MACRO(4); break;        \
+  }
+
+template <typename T>
+static int frontier_jacobi_t(int nf, const int32_t* frontier, int C, const int32_t* indptr,
+                             const int32_t* indices, const void* w, const void* X,
+                             const void* rhs, const void* diag, double delta, void* Y,
+                             uint8_t* changed, unsigned long long* edges, double avg_deg,
+                             hipStream_t st) {
+  RowMap m = choose_map(C, avg_deg);
+  dim3 grid(grid_for((long long)nf << m.g_shift)), block(kBlock);
+#define K(CPLV)                                                                               \
+  hipLaunchKernelGGL((k_frontier_jacobi<T, CPLV>), grid, block, 0, st, nf, frontier, C,       \
+                     m.g_shift, m.lc_shift, indptr, indices, (const T*)w, (const T*)X,        \
+                     (const T*)rhs, (const T*)diag, (T)delta, (T*)Y, changed, edges)
+  DLP_DISPATCH_CPL(m.cpl, K)
+#undef K
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+extern "C" int dlp_frontier_jacobi(int nf, const int32_t* frontier, int C, const int32_t* indptr,
+                                   const int32_t* indices, const void* w, const void* X,
+                                   const void* rhs, const void* diag, double delta, void* Y,
+                                   uint8_t* changed, unsigned long long* edges, int dtype,
+                                   double avg_deg, void* stream) {
+  DLP_CHECK_ARG(nf >= 0 && C >= 1 && C <= 256, "frontier_jacobi: bad nf=%d C=%d", nf, C);
+  if (nf == 0) return DLP_OK;
+  DLP_CHECK_ARG(frontier && indptr && indices && w && X && rhs && diag && Y && changed,
+                "frontier_jacobi: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == DLP_F32)
+    return frontier_jacobi_t<float>(nf, frontier, C, indptr, indices, w, X, rhs, diag, delta, Y,
+                                    changed, edges, avg_deg, st);
+  if (dtype == DLP_F64)
+    return frontier_jacobi_t<double>(nf, frontier, C, indptr, indices, w, X, rhs, diag, delta, Y,
+                                     changed, edges, avg_deg, st);
+  DLP_CHECK_ARG(false, "frontier_jacobi: bad dtype %d", dtype);
+  return DLP_ERR_ARG;
+}
+
+template <typename T>
+static int push_scatter_t(int nf, const int32_t* frontier, int C, const int32_t* indptr,
+                          const int32_t* indices, const void* w, const void* D, void* R,
+                          uint8_t* mark, unsigned long long* edges, double avg_deg,
+                          hipStream_t st) {
+  RowMap m = choose_map(C, avg_deg);
+  dim3 grid(grid_for((long long)nf << m.g_shift)), block(kBlock);
+#define K(CPLV)                                                                              \
+  hipLaunchKernelGGL((k_push_scatter<T, CPLV>), grid, block, 0, st, nf, frontier, C,         \
+                     m.g_shift, m.lc_shift, indptr, indices, (const T*)w, (const T*)D, (T*)R, \
+                     mark, edges)
+  DLP_DISPATCH_CPL(m.cpl, K)
+#undef K
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+extern "C" int dlp_push_scatter(int nf, const int32_t* frontier, int C, const int32_t* indptr,
+                                const int32_t* indices, const void* w, const void* D, void* R,
+                                uint8_t* mark, unsigned long long* edges, int dtype,
+                                double avg_deg, void* stream) {
+  DLP_CHECK_ARG(nf >= 0 && C >= 1 && C <= 256, "push_scatter: bad nf=%d C=%d", nf, C);
+  if (nf == 0) return DLP_OK;
+  DLP_CHECK_ARG(frontier && indptr && indices && w && D && R && mark, "push_scatter: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == DLP_F32)
+    return push_scatter_t<float>(nf, frontier, C, indptr, indices, w, D, R, mark, edges, avg_deg, st);
+  if (dtype == DLP_F64)
+    return push_scatter_t<double>(nf, frontier, C, indptr, indices, w, D, R, mark, edges, avg_deg, st);
+  DLP_CHECK_ARG(false, "push_scatter: bad dtype %d", dtype);
+  return DLP_ERR_ARG;
+}
+
+extern "C" int dlp_mark_neighbors(int nr, const int32_t* rows, const int32_t* indptr,
+                                  const int32_t* indices, uint8_t* mark,
+                                  unsigned long long* edges, void* stream) {
+  DLP_CHECK_ARG(nr >= 0, "mark_neighbors: bad nr=%d", nr);
+  if (nr == 0) return DLP_OK;
+  DLP_CHECK_ARG(rows && indptr && indices && mark, "mark_neighbors: null pointer");
+  hipLaunchKernelGGL(k_mark_neighbors, dim3(grid_for((long long)nr * 8)), dim3(kBlock), 0,
+                     (hipStream_t)stream, nr, rows, indptr, indices, mark, edges);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+extern "C" int dlp_fused_push_round(const int32_t* indptr, const int32_t* indices, const void* w,
+                                    const void* diag, const int32_t* front,
+                                    const int32_t* fcount, int32_t* fcount_next, int C, void* Z,
+                                    void* R, int32_t* mark, int32_t* touched, int32_t* tcount,
+                                    unsigned long long* edges, int dtype, int grid,
+                                    void* stream) {
+  DLP_CHECK_ARG(C >= 1 && C <= 256 && grid >= 1, "fused_push_round: bad C=%d grid=%d", C, grid);
+  DLP_CHECK_ARG(indptr && indices && w && diag && front && fcount && fcount_next && Z && R &&
+                    mark && touched && tcount,
+                "fused_push_round: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  // lanes over columns only (the scatter is atomic per column): G = LC
+  RowMap m = choose_map(C, 1.0);
+  m.g_shift = m.lc_shift < 2 ? 2 : m.lc_shift;
+  if (dtype == DLP_F32) {
+#define K(CPLV)                                                                                \
+  hipLaunchKernelGGL((k_fused_push_round<float, CPLV>), dim3(grid), dim3(kBlock), 0, st, indptr, \
+                     indices, (const float*)w, (const float*)diag, front, fcount, fcount_next, C, \
+                     m.g_shift, m.lc_shift, (float*)Z, (float*)R, mark, touched, tcount, edges)
+    DLP_DISPATCH_CPL(m.cpl, K)
+#undef K
+  } else if (dtype == DLP_F64) {
+#define K(CPLV)                                                                                   \
+  hipLaunchKernelGGL((k_fused_push_round<double, CPLV>), dim3(grid), dim3(kBlock), 0, st, indptr,   \
+                     indices, (const double*)w, (const double*)diag, front, fcount, fcount_next, C, \
+                     m.g_shift, m.lc_shift, (double*)Z, (double*)R, mark, touched, tcount, edges)
+    DLP_DISPATCH_CPL(m.cpl, K)
+#undef K
+  } else {
+    DLP_CHECK_ARG(false, "fused_push_round: bad dtype %d", dtype);
+  }
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+extern "C" int dlp_compact_frontier(const int32_t* touched, const int32_t* tcount,
+                                    int32_t* tcount_next, int32_t* mark, int C, const void* R,
+                                    const void* diag, const void* eps, int32_t* out,
+                                    int32_t* fcount_out, int dtype, int grid, void* stream) {
+  DLP_CHECK_ARG(C >= 1 && grid >= 1, "compact_frontier: bad C=%d grid=%d", C, grid);
+  DLP_CHECK_ARG(touched && tcount && tcount_next && mark && R && diag && eps && out && fcount_out,
+                "compact_frontier: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  int ls = C <= 1 ? 0 : ceil_log2(C);
+  if (ls > 6) ls = 6;
+  if (dtype == DLP_F32)
+    hipLaunchKernelGGL(k_compact_frontier<float>, dim3(grid), dim3(kBlock), 0, st, touched, tcount,
+                       tcount_next, mark, C, ls, (const float*)R, (const float*)diag,
+                       (const float*)eps, out, fcount_out);
+  else if (dtype == DLP_F64)
+    hipLaunchKernelGGL(k_compact_frontier<double>, dim3(grid), dim3(kBlock), 0, st, touched,
+                       tcount, tcount_next, mark, C, ls, (const double*)R, (const double*)diag,
+                       (const double*)eps, out, fcount_out);
+  else
+    DLP_CHECK_ARG(false, "compact_frontier: bad dtype %d", dtype);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}

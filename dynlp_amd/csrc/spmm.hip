@@ -1,0 +1,206 @@
+// Row-major SpMM with fused axpy, the masked row operator, and residual reductions.
+#include "rowgather.h"
+
+namespace dlp {
+
+// out[o] = add[r] + a[r]*X[r] + b * sum_{j, mask[col_j]} w_j X[col_j]   (o = i or r)
+template <typename TW, typename TX, typename TO, int CPL>
+__global__ __launch_bounds__(256) void k_row_operator(
+    int nrows, const int32_t* __restrict__ rows, int C, int g_shift, int lc_shift,
+    const int32_t* __restrict__ indptr, const int32_t* __restrict__ rowlen,
+    const int32_t* __restrict__ indices, const TW* __restrict__ w,
+    const uint8_t* __restrict__ mask, const TX* __restrict__ X, int64_t ldx,
+    const TO* __restrict__ a, TO b, const TO* __restrict__ add, int64_t ldadd,
+    TO* __restrict__ out, int64_t ldo, int compact_out) {
+  const Lanes L(g_shift, lc_shift);
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> g_shift;
+  const bool valid = i < nrows;
+  const int r = valid ? (rows ? rows[i] : (int)i) : 0;
+  const bool live = valid && (mask == nullptr || mask[r] != 0);
+  int beg = 0, end = 0;
+  if (live) {
+    beg = indptr[r];
+    end = rowlen ? beg + rowlen[r] : indptr[r + 1];
+  }
+  TO acc[CPL];
+#pragma unroll
+  for (int k = 0; k < CPL; ++k) acc[k] = 0;
+  row_gather<TW, TX, TO, CPL>(L, beg, end, indices, w, mask, X, ldx, C, acc);
+  group_reduce<TO, CPL>(L, acc);
+  if (valid && L.e == 0) {
+    const TO ar = a ? a[r] : (TO)0;
+    const int64_t o = compact_out ? i : (int64_t)r;
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) {
+      const int c = L.l + k * L.LC;
+      if (c < C) {
+        TO y = b * acc[k];
+        if (a) y += ar * (TO)X[(int64_t)r * ldx + c];
+        if (add) y += add[(int64_t)r * ldadd + c];
+        out[o * ldo + c] = y;
+      }
+    }
+  }
+}
+
+template <typename TW, typename TX, typename TO>
+static int launch_row_operator(int nrows, const int32_t* rows, int C, const int32_t* indptr,
+                               const int32_t* rowlen, const int32_t* indices, const void* w,
+                               const uint8_t* mask, const void* X, int64_t ldx, const void* a,
+                               double b, const void* add, int64_t ldadd, void* out, int64_t ldo,
+                               int compact_out, double avg_deg, hipStream_t st) {
+  RowMap m = choose_map(C, avg_deg);
+  const long long threads = (long long)nrows << m.g_shift;
+  dim3 grid(grid_for(threads)), block(kBlock);
+#define DLP_ROWOP(CPLV)                                                                        \
+  hipLaunchKernelGGL((k_row_operator<TW, TX, TO, CPLV>), grid, block, 0, st, nrows, rows, C,   \
+                     m.g_shift, m.lc_shift, indptr, rowlen, indices, (const TW*)w, mask,       \
+                     (const TX*)X, ldx, (const TO*)a, (TO)b, (const TO*)add, ldadd, (TO*)out,  \
+                     ldo, compact_out)
+  switch (m.cpl) {
+    case 1: DLP_ROWOP(1); break;
+    case 2: DLP_ROWOP(2); break;
+    case 3: DLP_ROWOP(3); break;
+    default: DLP_ROWOP(4); break;
+  }
+#undef DLP_ROWOP
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+// rho[c] = max |R[r,c]| / diag[r] over listed rows; one block reduces in LDS, then one
+// atomic max per column per block.
+template <typename TR>
+__global__ __launch_bounds__(256) void k_rho_cols(int nrows, const int32_t* __restrict__ rows,
+                                                  int C, const TR* __restrict__ R, int64_t ldr,
+                                                  const double* __restrict__ diag,
+                                                  double* __restrict__ rho) {
+  extern __shared__ double sm[];  // C doubles
+  for (int c = threadIdx.x; c < C; c += blockDim.x) sm[c] = 0.0;
+  __syncthreads();
+  // a block iteration covers rpi whole rows: thread t takes row t / C, column t % C
+  const int rpi = C <= (int)blockDim.x ? (int)blockDim.x / C : 1;
+  const int sub = C <= (int)blockDim.x ? (int)threadIdx.x / C : 0;
+  const int c0 = C <= (int)blockDim.x ? (int)threadIdx.x - sub * C : (int)threadIdx.x;
+  const int cstep = C <= (int)blockDim.x ? C : (int)blockDim.x;
+  if (sub < rpi) {
+    double local = 0.0;
+    int lastc = c0;
+    for (int64_t i = (int64_t)blockIdx.x * rpi + sub; i < nrows; i += (int64_t)gridDim.x * rpi) {
+      const int r = rows ? rows[i] : (int)i;
+      const double inv = 1.0 / diag[r];
+      for (int c = c0; c < C; c += cstep) {
+        const double v = fabs((double)R[(int64_t)r * ldr + c]) * inv;
+        if (c == c0) local = fmax(local, v);
+        else atomic_max_nonneg(&sm[c], v);
+      }
+    }
+    (void)lastc;
+    atomic_max_nonneg(&sm[c0], local);
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += blockDim.x) atomic_max_nonneg(&rho[c], sm[c]);
+}
+
+// Rows with any |R[r,c]| > eps[c]*diag[r] appended to out.  Group of LC lanes per row.
+template <typename TR>
+__global__ __launch_bounds__(256) void k_violating(int nrows, const int32_t* __restrict__ rows,
+                                                   int C, int lc_shift, const TR* __restrict__ R,
+                                                   int64_t ldr, const double* __restrict__ diag,
+                                                   const double* __restrict__ eps,
+                                                   int32_t* __restrict__ out,
+                                                   int32_t* __restrict__ count) {
+  const int LC = 1 << lc_shift;
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> lc_shift;
+  const int l = threadIdx.x & (LC - 1);
+  const bool valid = i < nrows;
+  const int r = valid ? (rows ? rows[i] : (int)i) : 0;
+  bool viol = false;
+  if (valid) {
+    const double d = diag[r];
+    for (int c = l; c < C; c += LC) viol |= fabs((double)R[(int64_t)r * ldr + c]) > eps[c] * d;
+  }
+  const unsigned long long bal = __ballot(viol);
+  const int lane = threadIdx.x & 63;
+  const unsigned long long gmask = (LC == 64 ? ~0ull : ((1ull << LC) - 1)) << (lane & ~(LC - 1));
+  if (valid && l == 0 && (bal & gmask)) out[atomicAdd(count, 1)] = r;
+}
+
+}  // namespace dlp
+
+using namespace dlp;
+
+extern "C" int dlp_row_operator(int nrows, const int32_t* rows, int C, const int32_t* indptr,
+                                const int32_t* rowlen, const int32_t* indices, const void* w,
+                                int w_dtype, const uint8_t* mask, const void* X, int64_t ldx,
+                                int x_dtype, const void* a, double b, const void* add,
+                                int64_t ldadd, void* out, int64_t ldo, int out_dtype,
+                                int compact_out, double avg_deg, void* stream) {
+  DLP_CHECK_ARG(nrows >= 0 && C >= 1 && C <= 256, "row_operator: bad nrows=%d or C=%d", nrows, C);
+  if (nrows == 0) return DLP_OK;
+  DLP_CHECK_ARG(indptr && indices && w && X && out, "row_operator: null pointer");
+  DLP_CHECK_ARG(ldx >= C && ldo >= C && (!add || ldadd >= C), "row_operator: leading dim < C");
+  hipStream_t st = (hipStream_t)stream;
+#define DLP_RO(TW, TX, TO)                                                                   \
+  return launch_row_operator<TW, TX, TO>(nrows, rows, C, indptr, rowlen, indices, w, mask, X, \
+                                         ldx, a, b, add, ldadd, out, ldo, compact_out,        \
+                                         avg_deg, st)
+  if (w_dtype == DLP_F32 && x_dtype == DLP_F32 && out_dtype == DLP_F32) DLP_RO(float, float, float);
+  if (w_dtype == DLP_F64 && x_dtype == DLP_F64 && out_dtype == DLP_F64) DLP_RO(double, double, double);
+  if (w_dtype == DLP_F32 && x_dtype == DLP_F64 && out_dtype == DLP_F64) DLP_RO(float, double, double);
+  if (w_dtype == DLP_F32 && x_dtype == DLP_F32 && out_dtype == DLP_F64) DLP_RO(float, float, double);
+#undef DLP_RO
+  DLP_CHECK_ARG(false, "row_operator: unsupported dtype combination w=%d x=%d out=%d", w_dtype,
+                x_dtype, out_dtype);
+  return DLP_ERR_ARG;
+}
+
+extern "C" int dlp_spmm_axpy(int n, int C, const int32_t* indptr, const int32_t* indices,
+                             const void* w, const void* X, int64_t ldx, const void* a, double b,
+                             void* Y, int64_t ldy, int dtype, double avg_deg, void* stream) {
+  DLP_CHECK_ARG(dtype == DLP_F32 || dtype == DLP_F64, "spmm_axpy: bad dtype %d", dtype);
+  return dlp_row_operator(n, nullptr, C, indptr, nullptr, indices, w, dtype, nullptr, X, ldx,
+                          dtype, a, b, nullptr, 0, Y, ldy, dtype, 0, avg_deg, stream);
+}
+
+extern "C" int dlp_rho_cols(int nrows, const int32_t* rows, int C, const void* R, int64_t ldr,
+                            int r_dtype, const double* diag, double* rho, void* stream) {
+  DLP_CHECK_ARG(nrows >= 0 && C >= 1 && C <= 4096, "rho_cols: bad sizes");
+  DLP_CHECK_ARG(R && diag && rho, "rho_cols: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  DLP_CHECK_HIP(hipMemsetAsync(rho, 0, sizeof(double) * C, st));
+  if (nrows == 0) return DLP_OK;
+  const long long total = (long long)nrows * C;
+  unsigned grid = grid_for(total);
+  if (grid > 2048) grid = 2048;
+  const size_t shm = sizeof(double) * C;
+  if (r_dtype == DLP_F32)
+    hipLaunchKernelGGL(k_rho_cols<float>, dim3(grid), dim3(kBlock), shm, st, nrows, rows, C,
+                       (const float*)R, ldr, diag, rho);
+  else
+    hipLaunchKernelGGL(k_rho_cols<double>, dim3(grid), dim3(kBlock), shm, st, nrows, rows, C,
+                       (const double*)R, ldr, diag, rho);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+extern "C" int dlp_violating_rows(int nrows, const int32_t* rows, int C, const void* R,
+                                  int64_t ldr, int r_dtype, const double* diag,
+                                  const double* eps, int32_t* out, int32_t* count, void* stream) {
+  DLP_CHECK_ARG(nrows >= 0 && C >= 1, "violating_rows: bad sizes");
+  DLP_CHECK_ARG(R && diag && eps && out && count, "violating_rows: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  DLP_CHECK_HIP(hipMemsetAsync(count, 0, sizeof(int32_t), st));
+  if (nrows == 0) return DLP_OK;
+  int ls = C <= 1 ? 0 : ceil_log2(C);
+  if (ls > 6) ls = 6;
+  const long long threads = (long long)nrows << ls;
+  if (r_dtype == DLP_F32)
+    hipLaunchKernelGGL(k_violating<float>, dim3(grid_for(threads)), dim3(kBlock), 0, st, nrows,
+                       rows, C, ls, (const float*)R, ldr, diag, eps, out, count);
+  else
+    hipLaunchKernelGGL(k_violating<double>, dim3(grid_for(threads)), dim3(kBlock), 0, st, nrows,
+                       rows, C, ls, (const double*)R, ldr, diag, eps, out, count);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}

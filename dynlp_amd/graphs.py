@@ -1,0 +1,252 @@
+"""Synthetic graph inputs and host-side CSR conventions.
+
+A graph here is an undirected, loop-free, weighted edge set over vertices 0..n-1, held on the
+host as three parallel arrays (u < v, w > 0).  The CSR form stores both directions with sorted
+columns per row, duplicates merged and self-loops dropped, following the canonical form of
+graphs.py:36-48 in the reference (`coalesce`, `symmetric_from_pairs`), except that a pair
+drawn twice keeps its first weight instead of summing both draws (unit-weight ER stays unit).
+
+Weights are drawn in FP64 and rounded to FP32 once, at generation: the FP32 value is THE
+weight, so the FP32 working operator and the FP64 certification operator are the same matrix
+(an FP32 weight is exact in FP64).
+
+Seeds follow one convention: graph S, labels S+1, update stream S+2, arrivals S+3.
+All generators use numpy's PCG64 (`numpy.random.default_rng`).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class Graph:
+    """Undirected weighted graph with labels; edges are unique pairs u < v."""
+    name: str
+    n: int
+    u: np.ndarray          # int64[m], u < v
+    v: np.ndarray          # int64[m]
+    w: np.ndarray          # float32[m] > 0
+    labeled: np.ndarray    # bool[n]
+    y: np.ndarray          # int32[n] label of every vertex (ground truth; used where labeled)
+    K: int                 # classes
+    block: np.ndarray | None = None   # SBM block of each vertex
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def m(self) -> int:
+        return int(self.u.shape[0])
+
+    def csr(self):
+        """(indptr int32[n+1], indices int32[2m], data float32[2m]) with sorted columns."""
+        return csr_from_pairs(self.n, self.u, self.v, self.w)
+
+
+def canonical_pairs(n, a, b, w):
+    """Unique undirected pairs (u < v), self-loops dropped, first occurrence's weight kept."""
+    a = np.asarray(a, dtype=np.int64)
+    b = np.asarray(b, dtype=np.int64)
+    w = np.asarray(w, dtype=np.float32)
+    keep = a != b
+    a, b, w = a[keep], b[keep], w[keep]
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    key = lo * n + hi
+    _, first = np.unique(key, return_index=True)
+    first.sort()
+    return lo[first], hi[first], w[first]
+
+
+def csr_from_pairs(n, u, v, w):
+    """Symmetric CSR of unique pairs: both directions, columns sorted within each row."""
+    r = np.concatenate([u, v]).astype(np.int64)
+    c = np.concatenate([v, u]).astype(np.int64)
+    d = np.concatenate([w, w]).astype(np.float32)
+    order = np.lexsort((c, r))
+    r, c, d = r[order], c[order], d[order]
+    indptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(np.bincount(r, minlength=n), out=indptr[1:])
+    if indptr[-1] >= 2**31:
+        raise ValueError("graph too large for int32 CSR offsets")
+    return indptr.astype(np.int32), c.astype(np.int32), d
+
+
+def csr_rows(indptr):
+    """Row index of every stored entry (graphs.py:51-64 `expand_counts`/`csr_rows`)."""
+    indptr = np.asarray(indptr)
+    return np.repeat(np.arange(indptr.shape[0] - 1, dtype=np.int32), np.diff(indptr))
+
+
+def _labels(n, frac, seed, min_count=1):
+    rng = np.random.default_rng(seed)
+    cnt = max(min_count, int(round(frac * n)))
+    lab = np.zeros(n, dtype=bool)
+    lab[rng.choice(n, size=cnt, replace=False)] = True
+    return lab
+
+
+def erdos_renyi(n, deg, K, label_frac=0.01, seed=0, unit=True, name=None) -> Graph:
+    """ER: m = n*deg/2 uniform pairs, loops dropped, duplicates merged; labels uniform over K."""
+    rng = np.random.default_rng(seed)
+    m = int(n * deg / 2)
+    a = rng.integers(0, n, size=m, dtype=np.int64)
+    b = rng.integers(0, n, size=m, dtype=np.int64)
+    w = np.ones(m, dtype=np.float32) if unit else rng.uniform(0.5, 1.5, size=m).astype(np.float32)
+    u, v, w = canonical_pairs(n, a, b, w)
+    lab = _labels(n, label_frac, seed + 1, min_count=K)
+    y = np.random.default_rng(seed + 1).integers(0, K, size=n).astype(np.int32)
+    return Graph(name or f"er-n{n}-d{deg:g}-k{K}", n, u, v, w, lab, y, K,
+                 meta=dict(kind="er", deg=deg, seed=seed))
+
+
+def sbm(n, blocks, deg, p_in, K, label_frac=0.01, seed=0, wlo=0.5, whi=1.5, unit=False, name=None) -> Graph:
+    """SBM with equal blocks block(v) = v*blocks // n; a p_in share of the n*deg/2 drawn pairs
+    is intra-block (uniform inside the block of a uniform endpoint), the rest joins two
+    different blocks uniformly.  Weights U[wlo, whi] rounded to FP32 (or 1).  Labels: block mod K."""
+    rng = np.random.default_rng(seed)
+    m = int(n * deg / 2)
+    blk = (np.arange(n, dtype=np.int64) * blocks) // n
+    start = (np.arange(blocks + 1, dtype=np.int64) * n + blocks - 1) // blocks
+    # start[b] = first vertex with v*blocks//n == b  (ceil(b*n/blocks))
+    size = np.diff(start)
+    a = rng.integers(0, n, size=m, dtype=np.int64)
+    intra = rng.random(m) < p_in
+    ba = blk[a]
+    b_in = start[ba] + (rng.random(m) * size[ba]).astype(np.int64)
+    other = rng.integers(0, blocks - 1, size=m, dtype=np.int64)
+    other = other + (other >= ba)              # a block different from a's
+    b_out = start[other] + (rng.random(m) * size[other]).astype(np.int64)
+    b = np.where(intra, b_in, b_out)
+    w = np.ones(m, dtype=np.float32) if unit else rng.uniform(wlo, whi, size=m).astype(np.float32)
+    u, v, w = canonical_pairs(n, a, b, w)
+    lab = _labels(n, label_frac, seed + 1, min_count=K)
+    y = (blk % K).astype(np.int32)
+    return Graph(name or f"sbm-n{n}-b{blocks}-d{deg:g}-k{K}", n, u, v, w, lab, y, K, block=blk,
+                 meta=dict(kind="sbm", blocks=blocks, deg=deg, p_in=p_in, seed=seed))
+
+
+# ---------------------------------------------------------------------------------------------
+# The five workload configurations (sizes as stated; `scale` shrinks n for tests).
+
+def config_graph(cfg: int, scale: float = 1.0, seed: int = 0) -> Graph:
+    """Graph of workload configuration cfg (1..4); n multiplied by `scale`."""
+    if cfg == 1:   # CPU case: ER n=1e5, deg 10, unit weights, 1% labelled, k=10
+        return erdos_renyi(int(100_000 * scale), 10, 10, 0.01, seed, unit=True)
+    if cfg == 2:   # ER n=1e6, deg 16, unit, 1%, k=16
+        return erdos_renyi(int(1_000_000 * scale), 16, 16, 0.01, seed, unit=True)
+    if cfg == 3:   # SBM n=1e6, 64 blocks, deg 20, 80% intra, w~U[0.5,1.5], 0.5% labelled, k=64
+        return sbm(int(1_000_000 * scale), 64, 20, 0.8, 64, 0.005, seed)
+    if cfg == 4:   # SBM n=3e5, 32 blocks, deg 8, 90% intra, 1% labelled, k=32 (unit weights)
+        return sbm(int(300_000 * scale), 32, 8, 0.9, 32, 0.01, seed, unit=True)
+    raise ValueError(f"no graph for configuration {cfg}")
+
+
+# ---------------------------------------------------------------------------------------------
+# Edge-update streams.
+
+@dataclass
+class EdgeBatch:
+    """One request: edge insertions (u, v, w) and deletions (u, v); pairs distinct in a batch."""
+    ins_u: np.ndarray
+    ins_v: np.ndarray
+    ins_w: np.ndarray
+    del_u: np.ndarray
+    del_v: np.ndarray
+
+    @property
+    def size(self) -> int:
+        return int(self.ins_u.shape[0] + self.del_u.shape[0])
+
+
+class EdgeSet:
+    """Host mirror of a graph's edge set as sorted int64 keys u*n+v (u < v), for drawing
+    updates (deletions need existing edges, insertions need non-edges)."""
+
+    def __init__(self, g: Graph):
+        self.n = g.n
+        key = g.u * g.n + g.v
+        order = np.argsort(key)
+        self.key = key[order]
+        self.w = g.w[order]
+
+    def contains(self, key):
+        pos = np.searchsorted(self.key, key)
+        pos = np.minimum(pos, self.key.shape[0] - 1)
+        return (self.key.shape[0] > 0) & (self.key[pos] == key)
+
+    def apply(self, b: EdgeBatch):
+        n = self.n
+        dk = np.minimum(b.del_u, b.del_v) * n + np.maximum(b.del_u, b.del_v)
+        if dk.shape[0]:
+            keep = ~np.isin(self.key, dk)
+            self.key, self.w = self.key[keep], self.w[keep]
+        ik = np.minimum(b.ins_u, b.ins_v) * n + np.maximum(b.ins_u, b.ins_v)
+        if ik.shape[0]:
+            have = self.contains(ik)
+            if have.any():   # upsert: an existing pair takes the new weight
+                pos = np.searchsorted(self.key, ik[have])
+                self.w[pos] = b.ins_w[have]
+            nk, nw = ik[~have], b.ins_w[~have]
+            key = np.concatenate([self.key, nk])
+            w = np.concatenate([self.w, nw])
+            order = np.argsort(key, kind="stable")
+            self.key, self.w = key[order], w[order]
+
+    def pairs(self):
+        return self.key // self.n, self.key % self.n, self.w
+
+
+def draw_batch(es: EdgeSet, size, ins_frac=0.5, rng=None, wlo=0.5, whi=1.5, unit=False,
+               del_pool=None) -> EdgeBatch:
+    """`size` updates: round(size*ins_frac) insertions of uniform new non-edge pairs (rejection
+    sampled) and the rest deletions of uniform existing edges (without replacement).
+    del_pool: optional bool mask over es.key restricting which edges may be deleted; when it
+    runs out, deletions fall back to any edge."""
+    n = es.n
+    n_ins = int(round(size * ins_frac))
+    n_del = size - n_ins
+    # deletions
+    if del_pool is not None and del_pool.sum() >= n_del:
+        cand = np.flatnonzero(del_pool)
+    else:
+        cand = None
+    if cand is not None:
+        pick = cand[rng.choice(cand.shape[0], size=n_del, replace=False)]
+    else:
+        pick = rng.choice(es.key.shape[0], size=min(n_del, es.key.shape[0]), replace=False)
+    dk = es.key[pick]
+    # insertions: rejection-sample pairs that are neither edges nor already drawn
+    got = np.zeros(0, dtype=np.int64)
+    while got.shape[0] < n_ins:
+        need = n_ins - got.shape[0]
+        a = rng.integers(0, n, size=2 * need + 8, dtype=np.int64)
+        b = rng.integers(0, n, size=2 * need + 8, dtype=np.int64)
+        ok = a != b
+        k = np.minimum(a, b)[ok] * n + np.maximum(a, b)[ok]
+        k = k[~es.contains(k)]
+        k = k[~np.isin(k, dk)]
+        _, first = np.unique(np.concatenate([got, k]), return_index=True)
+        allk = np.concatenate([got, k])
+        first.sort()
+        got = allk[first][:n_ins] if first.shape[0] >= n_ins else allk[first]
+    w = np.ones(n_ins, dtype=np.float32) if unit else rng.uniform(wlo, whi, size=n_ins).astype(np.float32)
+    return EdgeBatch(got // n, got % n, w, dk // n, dk % n)
+
+
+def update_stream(g: Graph, n_batches, batch_size, ins_frac=0.5, seed=0, unit=None,
+                  inter_first=False):
+    """Pre-drawn batches for graph g (seed S+2).  inter_first: deletions are drawn from
+    inter-block edges while any remain, then from all edges (fragmenting stream)."""
+    rng = np.random.default_rng(seed + 2)
+    es = EdgeSet(g)
+    unit = bool(np.all(g.w == 1)) if unit is None else unit
+    out = []
+    for _ in range(n_batches):
+        pool = None
+        if inter_first and g.block is not None:
+            uu, vv = es.key // es.n, es.key % es.n
+            pool = g.block[uu] != g.block[vv]
+        b = draw_batch(es, batch_size, ins_frac, rng, unit=unit, del_pool=pool)
+        es.apply(b)
+        out.append(b)
+    return out, es

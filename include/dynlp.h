@@ -1,0 +1,135 @@
+/*
+ * dynlp.h — C ABI of the MI355X (gfx950) label-propagation hot path.
+ *
+ * Every entry point takes device pointers owned by the caller, sizes as int32/int64,
+ * dtype codes, and a hipStream_t passed as void* (NULL = default stream).  Kernels are
+ * enqueued on that stream; no entry point synchronises the host unless its comment says so.
+ * Return value: 0 on success, a negative DLP_ERR_* code otherwise; dlp_last_error() gives
+ * the message (thread-local).  Scratch memory comes from caller workspaces whose size the
+ * matching *_workspace_bytes() function reports; no entry point allocates device memory
+ * except the dlp_push_plan_* / dlp_graph_* handles, which own their scratch until destroyed.
+ *
+ * CSR: indptr int32[n+1] holds row starts; when rowlen (int32[n]) is non-NULL the row r
+ * holds rowlen[r] live entries from indptr[r] (slack CSR: capacity indptr[r+1]-indptr[r]),
+ * otherwise indptr[r+1]-indptr[r].  Column indices int32.  Weights FP32 or FP64.
+ * mask (uint8[n], may be NULL): vertices with mask 0 are outside the solved set; their rows
+ * carry no off-diagonal term and they contribute nothing to other rows (W_UU restriction).
+ * Dense blocks are row-major with a leading dimension (elements).
+ *
+ * Each declaration cites the reference interface it replaces (dynlp/<file>:<line>).
+ */
+#ifndef DYNLP_H
+#define DYNLP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { DLP_F32 = 0, DLP_F64 = 1 };
+enum { DLP_OK = 0, DLP_ERR_ARG = -1, DLP_ERR_HIP = -2, DLP_ERR_CAPACITY = -3, DLP_ERR_STATE = -4 };
+
+/* Last error message of the calling thread (empty string if none). */
+const char* dlp_last_error(void);
+/* ABI version (incremented on any signature change). */
+int dlp_abi_version(void);
+
+/* ---- Row-major SpMM -------------------------------------------------------------------
+ * Y = a .* X + b * (W X), all n rows.  Replaces kernels.spmm_axpy (kernels.py:377-393; CUDA
+ * kernels.py:140-166, 240-266).  X, Y: n x C, leading dims ldx, ldy.  a: length n or NULL
+ * (treated as 0).  dtype applies to w, X, a and Y alike (the reference requires W and X to
+ * share a dtype, solvers.py:36-37).  avg_deg picks the lane layout (0 = compute it). */
+int dlp_spmm_axpy(int n, int C, const int32_t* indptr, const int32_t* indices, const void* w,
+                  const void* X, int64_t ldx, const void* a, double b, void* Y, int64_t ldy,
+                  int dtype, double avg_deg, void* stream);
+
+/* ---- Masked row operator (the building block of apply_A / residual refresh) -----------
+ * For each listed row r (rows[i], or r = i when rows is NULL, i < nrows):
+ *   out[o] = add[r] + a[r] * X[r] + b * sum_{j in row r, mask[col_j]} w_j X[col_j]
+ * where o = i when compact_out != 0, else o = r; rows with mask[r] == 0 skip the sum.
+ * add and a may be NULL.  Types: w_dtype for w; x_dtype for X; out_dtype for out, add and
+ * the accumulation; a is FP64 when out is FP64, else FP32.
+ * Replaces solvers.apply_A (solvers.py:46-49) and the changed-row refresh
+ * R[Q] = rhs[Q] - diag[Q] Z[Q] + W[Q] Z (solvers.py:404-407). */
+int dlp_row_operator(int nrows, const int32_t* rows, int C, const int32_t* indptr,
+                     const int32_t* rowlen, const int32_t* indices, const void* w, int w_dtype,
+                     const uint8_t* mask, const void* X, int64_t ldx, int x_dtype,
+                     const void* a, double b, const void* add, int64_t ldadd, void* out,
+                     int64_t ldo, int out_dtype, int compact_out, double avg_deg, void* stream);
+
+/* ---- Column residual ratios ------------------------------------------------------------
+ * rho[c] = max_{listed rows r} |R[r,c]| / diag[r]  (FP64 out[C], overwritten).  R dtype
+ * r_dtype, diag FP64.  Replaces solvers.rho_cols (solvers.py:52-54). */
+int dlp_rho_cols(int nrows, const int32_t* rows, int C, const void* R, int64_t ldr, int r_dtype,
+                 const double* diag, double* rho, void* stream);
+
+/* Rows violating |R[r,c]| > eps[c] * diag[r] for some c, appended to out (order arbitrary,
+ * set exact); *count (device int32) receives the number.  Scans the listed rows (or all).
+ * Replaces the `violating` filter of solvers.push (solvers.py:105-107, 115-116). */
+int dlp_violating_rows(int nrows, const int32_t* rows, int C, const void* R, int64_t ldr,
+                       int r_dtype, const double* diag, const double* eps, int32_t* out,
+                       int32_t* count, void* stream);
+
+/* ---- Frontier primitives (FrontierOps, kernels.py:436-492) ----------------------------- */
+/* Y[g] = (rhs[f_g] + sum_j w_j X[col_j]) / diag[f_g], changed[g] = any_c |Y[g,c]-X[f_g,c]| > delta
+ * for g < nf (f = frontier); adds the frontier's edge count to *edges (device uint64, may be
+ * NULL).  dtype for w, X, rhs, diag, Y.  Replaces FrontierOps.jacobi (kernels.py:464-474;
+ * CUDA kernels.py:31-102, 186-220). */
+int dlp_frontier_jacobi(int nf, const int32_t* frontier, int C, const int32_t* indptr,
+                        const int32_t* indices, const void* w, const void* X, const void* rhs,
+                        const void* diag, double delta, void* Y, uint8_t* changed,
+                        unsigned long long* edges, int dtype, double avg_deg, void* stream);
+
+/* R[col_j] += w_j * D[g] (atomic) for every entry of row f_g, and mark[col_j] = 1.
+ * Replaces FrontierOps.push (kernels.py:476-486; CUDA kernels.py:105-126, 222-238). */
+int dlp_push_scatter(int nf, const int32_t* frontier, int C, const int32_t* indptr,
+                     const int32_t* indices, const void* w, const void* D, void* R,
+                     uint8_t* mark, unsigned long long* edges, int dtype, double avg_deg,
+                     void* stream);
+
+/* mark[col_j] = 1 for every entry of the listed rows.  Replaces FrontierOps.mark_neighbors
+ * (kernels.py:488-492; CUDA kernels.py:128-138). */
+int dlp_mark_neighbors(int nr, const int32_t* rows, const int32_t* indptr, const int32_t* indices,
+                       uint8_t* mark, unsigned long long* edges, void* stream);
+
+/* Indices i < n with mask[i] != 0, in increasing order, to out; *count (device int32).
+ * Replaces xp.flatnonzero (solvers.py:115, 227, 238).  Workspace: dlp_scan_workspace_bytes(n). */
+int dlp_flatnonzero_u8(int n, const uint8_t* mask, int32_t* out, int32_t* count, void* ws,
+                       size_t ws_bytes, void* stream);
+size_t dlp_scan_workspace_bytes(int n);
+
+/* ---- Fused asynchronous push (FusedPush, kernels.py:396-433) -------------------------
+ * One round: every frontier vertex u (front[0..*fcount)) takes R[u] atomically, adds
+ * R[u]/diag[u] to Z[u] and scatters w*d to its neighbours (entries with w == 0 skipped);
+ * first touches are appended to touched (count in tcount); fcount_next is reset.  CUDA
+ * kernels.py:287-331. */
+int dlp_fused_push_round(const int32_t* indptr, const int32_t* indices, const void* w,
+                         const void* diag, const int32_t* front, const int32_t* fcount,
+                         int32_t* fcount_next, int C, void* Z, void* R, int32_t* mark,
+                         int32_t* touched, int32_t* tcount, unsigned long long* edges,
+                         int dtype, int grid, void* stream);
+/* Compaction: touched vertices are unmarked and those violating |R| > eps*diag appended to
+ * out (count in fcount_out); tcount_next is reset.  CUDA kernels.py:333-350. */
+int dlp_compact_frontier(const int32_t* touched, const int32_t* tcount, int32_t* tcount_next,
+                         int32_t* mark, int C, const void* R, const void* diag, const void* eps,
+                         int32_t* out, int32_t* fcount_out, int dtype, int grid, void* stream);
+
+/* ---- Connected components (cc.connected_components, cc.py:8-23) ----------------------
+ * labels[v] = rank of the minimum vertex id of v's component (int32, bit-exact with the
+ * reference's hook/jump + unique); *n_comp (device int32).  Edges (rows[e], cols[e]), any
+ * direction, duplicates and self-loops allowed.  Workspace: dlp_cc_workspace_bytes(n). */
+int dlp_cc_edges(int n, int64_t m, const int32_t* rows, const int32_t* cols, int32_t* labels,
+                 int32_t* n_comp, void* ws, size_t ws_bytes, void* stream);
+/* Same over a (slack) CSR; with mask non-NULL only vertices with mask != 0 and the entries
+ * between them take part (others become singleton components). */
+int dlp_cc_csr(int n, const int32_t* indptr, const int32_t* rowlen, const int32_t* indices,
+               const uint8_t* mask, int32_t* labels, int32_t* n_comp, void* ws, size_t ws_bytes,
+               void* stream);
+size_t dlp_cc_workspace_bytes(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DYNLP_H */

@@ -1,0 +1,165 @@
+"""HIP kernels vs the FP64 oracle on seeded inputs (patterns of the reference's
+tests/test_dynlp.py:195-222, 246-254 and tests/test_incremental.py:66-81)."""
+import numpy as np
+import pytest
+
+from dynlp_amd import graphs
+from oracle import lp as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(n=3000, deg=12, seed=8, dtype="float32"):
+    import torch
+    from dynlp_amd import backend as B
+    g = graphs.sbm(n, 2, deg, 0.85, 2, 0.02, seed)
+    ip, ix, d = g.csr()
+    dt = torch.float32 if dtype == "float32" else torch.float64
+    W = B.csr_from_host(ip, ix, d, n, dt)
+    Ws = O.csr(ip, ix, d, n)
+    return g, W, Ws, dt
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("C", [1, 2, 8, 17, 40, 64, 65, 130])
+def test_spmm_axpy_matches_oracle(dtype, C):
+    import torch
+    from dynlp_amd.kernels import spmm_axpy
+    g, W, Ws, dt = _setup(dtype=dtype)
+    rs = np.random.default_rng(1)
+    X = rs.random((g.n, C)).astype(dtype)
+    a = rs.random(g.n).astype(dtype)
+    Xd = torch.as_tensor(X, device="cuda")
+    ad = torch.as_tensor(a, device="cuda")
+    tol = 1e-5 if dtype == "float32" else 1e-12
+    Y = spmm_axpy(W, Xd).cpu().numpy()
+    np.testing.assert_allclose(Y, O.spmm_axpy(Ws, X), rtol=tol, atol=tol)
+    Y2 = spmm_axpy(W, Xd, a=ad, b=-1.0).cpu().numpy()
+    np.testing.assert_allclose(Y2, O.spmm_axpy(Ws, X, a, -1.0), rtol=tol, atol=tol)
+
+
+@pytest.mark.parametrize("group", [1, 2, 8, 32, "auto"])
+def test_spmm_lane_layouts_agree(group):
+    import torch
+    from dynlp_amd.kernels import spmm_axpy
+    g, W, Ws, dt = _setup(dtype="float64")
+    X = np.random.default_rng(2).random((g.n, 3))
+    Y = spmm_axpy(W, torch.as_tensor(X, device="cuda"), group=group).cpu().numpy()
+    np.testing.assert_allclose(Y, O.spmm_axpy(Ws, X), rtol=1e-12)
+
+
+def test_row_operator_masked_subset_mixed_precision():
+    import torch
+    from dynlp_amd import _lib
+    from dynlp_amd.backend import ptr, stream
+    g, W, Ws, _ = _setup(dtype="float32")
+    n, C = g.n, 5
+    rs = np.random.default_rng(3)
+    mask = (rs.random(n) < 0.8).astype(np.uint8)
+    rows = np.sort(rs.choice(n, 700, replace=False)).astype(np.int32)
+    X = rs.random((n, C))                        # FP64 block, FP32 weights
+    a = rs.random(n)
+    add = rs.random((n, C))
+    out = torch.zeros((700, C), dtype=torch.float64, device="cuda")
+    Xd, ad, addd = (torch.as_tensor(v, device="cuda") for v in (X, a, add))
+    md, rd = torch.as_tensor(mask, device="cuda"), torch.as_tensor(rows, device="cuda")
+    _lib.call("dlp_row_operator", 700, ptr(rd), C, ptr(W.indptr), None, ptr(W.indices), ptr(W.data), 0, ptr(md),
+              ptr(Xd), C, 1, ptr(ad), -2.0, ptr(addd), C, ptr(out), C, 1, 1, 0.0, stream())
+    Wm = Ws.multiply(mask[:, None]).multiply(mask[None, :]).tocsr()
+    ref = add[rows] + a[rows, None] * X[rows] - 2.0 * (Wm @ X)[rows]
+    np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("C", [1, 3, 8, 40, 65])
+def test_frontier_kernels_match_oracle(dtype, C):
+    import torch
+    from dynlp_amd.kernels import FrontierOps
+    g, W, Ws, dt = _setup(dtype=dtype)
+    n = g.n
+    rs = np.random.default_rng(0)
+    frontier = np.sort(rs.choice(n, 700, replace=False)).astype(np.int32)
+    X, rhs = rs.random((n, C)).astype(dtype), rs.random((n, C)).astype(dtype)
+    diag = (np.asarray(Ws.sum(axis=1)).ravel() + 1).astype(dtype)
+    ops = FrontierOps(W, C)
+    fd = torch.as_tensor(frontier, device="cuda")
+    Y, ch = ops.jacobi(fd, torch.as_tensor(X, device="cuda"), torch.as_tensor(rhs, device="cuda"),
+                       torch.as_tensor(diag, device="cuda"), 1e-2)
+    Yo, cho = O.frontier_jacobi(Ws, frontier, X.astype(np.float64), rhs.astype(np.float64),
+                                diag.astype(np.float64), 1e-2)
+    rtol = 1e-5 if dtype == "float32" else 1e-12
+    np.testing.assert_allclose(Y.cpu().numpy(), Yo, rtol=rtol)
+    # flags agree except within rounding of the threshold
+    ambiguous = np.abs(np.abs(Yo - X[frontier]).max(axis=1) - 1e-2) < 1e-5
+    assert np.array_equal(ch.cpu().numpy()[~ambiguous], cho[~ambiguous])
+    D = rs.standard_normal((frontier.size, C)).astype(dtype)
+    R = torch.zeros((n, C), dtype=dt, device="cuda")
+    mark = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    ops.push(fd, torch.as_tensor(D, device="cuda"), R, mark)
+    Ro, mo = np.zeros((n, C)), np.zeros(n, dtype=np.uint8)
+    O.push_scatter(Ws, frontier, D.astype(np.float64), Ro, mo)
+    np.testing.assert_allclose(R.cpu().numpy(), Ro, rtol=rtol, atol=1e-5 if dtype == "float32" else 1e-12)
+    assert np.array_equal(mark.cpu().numpy(), mo)
+    mark.zero_()
+    mo[:] = 0
+    ops.mark_neighbors(fd, mark)
+    O.mark_neighbors(Ws, frontier, mo)
+    assert np.array_equal(mark.cpu().numpy(), mo)
+    assert int(ops.edges) == 3 * O.edge_visits(Ws, frontier)
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("C", [2, 4, 65])
+def test_fused_push_reaches_tolerance(dtype, C):
+    """tests/test_incremental.py:66-81: converges, kept R == rhs - A Z, max |R|/diag <= eps."""
+    import torch
+    from dynlp_amd.kernels import FusedPush
+    n = 4000
+    g = graphs.sbm(n, 2, 8.0, 0.85, 2, 0.03, 9)
+    sysm = O.build_system(n, g.u, g.v, g.w, g.labeled, g.y, 2)
+    from dynlp_amd import backend as B
+    Wc = sysm.W.tocsr()
+    W = B.csr_from_host(Wc.indptr, Wc.indices, Wc.data, n, torch.float32 if dtype == "float32" else torch.float64)
+    rs = np.random.default_rng(0)
+    rhs = (rs.random((n, C)) * sysm.solved[:, None]).astype(dtype)
+    Z = np.zeros((n, C), dtype=dtype)
+    diag = sysm.diag.astype(dtype)
+    R = rhs.astype(np.float64) - O.apply_A(sysm.W, sysm.diag, Z.astype(np.float64))
+    Zd = torch.as_tensor(Z, device="cuda")
+    Rd = torch.as_tensor(R.astype(dtype), device="cuda")
+    dd = torch.as_tensor(diag, device="cuda")
+    eps = np.full(C, 1e-5 if dtype == "float64" else 1e-4)
+    cand = torch.as_tensor(np.flatnonzero((np.abs(R) > 0).any(axis=1)).astype(np.int32), device="cuda")
+    rounds, ok, edges = FusedPush(W, C).run(Zd, Rd, dd, eps, cand, 100_000)
+    assert ok and rounds > 0 and edges > 0
+    Zh = Zd.cpu().numpy().astype(np.float64)
+    R_true = rhs.astype(np.float64) - O.apply_A(sysm.W, sysm.diag, Zh)
+    np.testing.assert_allclose(Rd.cpu().numpy(), R_true, atol=1e-4 if dtype == "float32" else 1e-10)
+    assert (np.abs(R_true) / sysm.diag[:, None]).max() / eps[0] <= 1.0 + (0.05 if dtype == "float32" else 1e-6)
+
+
+def test_connected_components_match_union_find():
+    """tests/test_dynlp.py:41-46, plus the closed form (bit-exact labels)."""
+    import random
+    from dynlp_amd.cc import connected_components
+    rnd = random.Random(0)
+    for n, m in [(400, 300), (5000, 4000), (20000, 30000), (1, 0)]:
+        pairs = [(rnd.randrange(n), rnd.randrange(n)) for _ in range(m)]
+        r = np.array([a for a, _ in pairs], dtype=np.int32)
+        c = np.array([b for _, b in pairs], dtype=np.int32)
+        lab, nc = connected_components(n, r, c)
+        ref, ncr = O.connected_components(n, r, c)
+        assert nc == ncr
+        assert np.array_equal(lab.cpu().numpy(), ref)
+
+
+def test_cc_csr_masked_matches_oracle():
+    import torch
+    from dynlp_amd.cc import connected_components_csr
+    g, W, Ws, _ = _setup(n=20000, deg=2.5, seed=3)
+    mask = (np.random.default_rng(5).random(g.n) < 0.7).astype(np.uint8)
+    lab, nc = connected_components_csr(W, torch.as_tensor(mask, device="cuda"))
+    coo = Ws.tocoo()
+    keep = (mask[coo.row] > 0) & (mask[coo.col] > 0)
+    ref, ncr = O.connected_components(g.n, coo.row[keep], coo.col[keep])
+    assert nc == ncr and np.array_equal(lab.cpu().numpy(), ref)
