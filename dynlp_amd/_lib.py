@@ -39,7 +39,30 @@ _SIGS = {
     "dlp_cc_edges": (ct.c_int, [I32, I64, P, P, P, P, P, SZ, P]),
     "dlp_cc_csr": (ct.c_int, [I32, P, P, P, P, P, P, P, SZ, P]),
     "dlp_cc_workspace_bytes": (SZ, [I32]),
+    "dlp_lp_create": (ct.c_int, [P, P]),
+    "dlp_lp_destroy": (ct.c_int, [P]),
+    "dlp_lp_set_store": (ct.c_int, [P, P, P, P, P]),
+    "dlp_lp_set_eps": (ct.c_int, [P, P, DBL, P]),
+    "dlp_lp_begin_batch": (ct.c_int, [P, P]),
+    "dlp_lp_apply": (ct.c_int, [P, I32, P, P, I32, P, P, P, P]),
+    "dlp_lp_regrow_plan": (ct.c_int, [P, I32, DBL, P, P, P]),
+    "dlp_lp_regrow_commit": (ct.c_int, [P, P, P, P, I32, P, P, P, P, P]),
+    "dlp_lp_ground": (ct.c_int, [P, P]),
+    "dlp_lp_refresh": (ct.c_int, [P, I32, I32, P]),
+    "dlp_lp_push": (ct.c_int, [P, I32, I32, I32, P, P]),
+    "dlp_lp_push64_round": (ct.c_int, [P, I32, P, P]),
+    "dlp_lp_push64_next": (ct.c_int, [P, P, P, P]),
+    "dlp_lp_violators": (ct.c_int, [P, P, P]),
+    "dlp_lp_status": (ct.c_int, [P, P, P, P]),
 }
+
+
+class LPDesc(ct.Structure):
+    """dlp_lp_desc of include/dynlp.h."""
+    _fields_ = [("n", I32), ("C1", I32), ("K2", I32), ("pad_", I32), ("eta", DBL), ("prior", DBL),
+                ("indptr", P), ("rowlen", P), ("col", P), ("w", P), ("labeled", P), ("y", P),
+                ("solved", P), ("labcnt", P), ("Z32", P), ("R32", P), ("ld", I64), ("diag64", P),
+                ("diag32", P), ("Z64", P), ("R64", P)]
 
 
 class DLPError(RuntimeError):

@@ -4,7 +4,17 @@
 #include <stdint.h>
 #include <stdio.h>
 #include <stdarg.h>
+#include <string.h>
 #include "../../include/dynlp.h"
+
+// Dispatch on the column-chunk count of a row group (template parameter CPL).
+#define DLP_DISPATCH_CPL(CPL, MACRO) \
+  switch (CPL) {                     \
+    case 1: MACRO(1); break;         \
+    case 2: MACRO(2); break;         \
+    case 3: MACRO(3); break;         \
+    default: MACRO(4); break;        \
+  }
 
 namespace dlp {
 

@@ -192,13 +192,6 @@ __global__ __launch_bounds__(256) void k_compact_frontier(
 
 using namespace dlp;
 
-#define DLP_DISPATCH_CPL(CPL, MACRO) \
-  switch (CPL) {                     \
-    case 1: MACRO(1); break;         \
-    case 2: MACRO(2); break;         \
-    case 3: MACRO(3); break;         \
-    default: MACRO(4); break;        \
-  }
 
 template <typename T>
 static int frontier_jacobi_t(int nf, const int32_t* frontier, int C, const int32_t* indptr,

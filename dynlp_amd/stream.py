@@ -1,0 +1,275 @@
+"""Streaming protocol and the per-batch linear system, resident on the device.
+
+Per batch, the harmonic system on the active unlabeled set U (reference stream.py:1-6):
+    (diag - W_UU) F_U = rhs,   diag = s + rowsum(W_UU),   s = w(u, L) + eta
+with h (A h = diag) riding along as one extra column for the certificate |F - F*| <= rho*h
+(solvers.py:10-12).
+
+The reference streams vertex arrivals over a fixed graph and patches edge weights in place
+(`IncrementalStream`, stream.py:180-346).  This module streams edge insertions and deletions:
+`EdgeStreamLP` keeps the graph in a slack CSR on the device (global vertex ids, rows outside
+U inert), applies each batch in place, re-grounds, refreshes the changed rows' FP64 residual
+and repairs it by the round-synchronous push; the FP64 plain path (`fp64=True`) repairs with
+the reference's host-driven push loop (solvers.py:100-125) instead.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from .backend import device, ptr, stream
+from .graphs import EdgeBatch, Graph
+
+
+@dataclass
+class Stats:
+    """Per-batch statistics (solvers.py:25-33 `Stats`, plus latency parts)."""
+    iters: int = 0              # push rounds
+    edges: int = 0              # edge visits
+    ms: float = 0.0
+    setup_ms: float = 0.0
+    path: str = ""
+    cert: float = float("nan")
+    hmax: float = float("nan")
+    rho: float = float("nan")   # max_c<C |R_uc|/diag_u over rows refreshed at the end of the batch
+    refine: int = 0             # FP64 verification rounds that found violators
+    touched: int = 0            # rows touched by pushes
+    changed: int = 0            # rows whose equation changed
+    toggled: int = 0            # vertices that changed groundedness
+    extra: dict = field(default_factory=dict)
+
+
+def slack_layout(indptr, indices, data, slack_min=4, slack_frac=0.25):
+    """Host ingest: a CSR expanded to per-row capacity deg + max(slack_min, slack_frac*deg)."""
+    n = indptr.shape[0] - 1
+    deg = np.diff(indptr).astype(np.int64)
+    cap = deg + np.maximum(slack_min, np.ceil(slack_frac * deg).astype(np.int64))
+    new_ptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(cap, out=new_ptr[1:])
+    if new_ptr[-1] >= 2**31:
+        raise ValueError("slack store exceeds int32 offsets")
+    rows = np.repeat(np.arange(n, dtype=np.int64), deg)
+    pos = new_ptr[rows] + (np.arange(indices.shape[0], dtype=np.int64) - indptr[:-1].astype(np.int64)[rows])
+    col = np.full(int(new_ptr[-1]), -1, dtype=np.int32)
+    w = np.zeros(int(new_ptr[-1]), dtype=np.float32)
+    col[pos] = indices
+    w[pos] = data
+    return new_ptr.astype(np.int32), deg.astype(np.int32), col, w
+
+
+class EdgeStreamLP:
+    """One graph's resident label-propagation state on the device.
+
+    eps: stop threshold on the label columns, |R_uc| <= eps * diag_u (rho of solvers.py:52-54);
+    eps_h: the h column's (solvers.py:247 default 0.05).  fp64=False is the mixed-precision
+    service path (FP32 Z/R and push, FP64 residual verification and refinement); fp64=True keeps
+    Z/R in FP64 and repairs with the host-driven push rounds of the reference."""
+
+    def __init__(self, g: Graph, eps=1e-6, eps_h=0.05, fp64=False, eta=0.0, slack_min=4,
+                 slack_frac=0.25, push_scale=0.9, rounds_per_launch=16, use_graph=True, max_rounds=200_000):
+        self.g, self.n, self.K = g, g.n, g.K
+        self.C = 1 if g.K == 2 else g.K
+        self.C1 = self.C + 1
+        self.eps, self.eps_h, self.fp64, self.eta = float(eps), float(eps_h), bool(fp64), float(eta)
+        self.rounds_per_launch, self.use_graph, self.max_rounds = rounds_per_launch, use_graph, max_rounds
+        self.push_scale = push_scale
+        dev = device()
+        ip, ix, d = g.csr()
+        sp_, rl, col, w = slack_layout(ip, ix, d, slack_min, slack_frac)
+        labcnt = np.zeros(self.n, dtype=np.int32)
+        np.add.at(labcnt, np.repeat(np.arange(self.n), np.diff(ip)), g.labeled[ix].astype(np.int32))
+        t = lambda a, dt: torch.as_tensor(a).to(dev, dt).contiguous()  # noqa: E731
+        self.indptr, self.rowlen = t(sp_, torch.int32), t(rl, torch.int32)
+        self.col, self.w = t(col, torch.int32), t(w, torch.float32)
+        self.labeled = t(g.labeled.astype(np.uint8), torch.uint8)
+        self.y = t(g.y, torch.int32)
+        self.solved = torch.zeros(self.n, dtype=torch.uint8, device=dev)
+        self.labcnt = t(labcnt, torch.int32)
+        self.ld = self.C1
+        self.prior = 0.5 if self.K == 2 else 1.0 / self.K
+        Z = torch.full((self.n, self.C1), self.prior, dtype=torch.float32, device=dev)
+        Z[:, self.C] = 1.0
+        self.Z32 = Z
+        self.R32 = torch.zeros((self.n, self.C1), dtype=torch.float32, device=dev)
+        self.diag64 = torch.ones(self.n, dtype=torch.float64, device=dev)
+        self.diag32 = torch.ones(self.n, dtype=torch.float32, device=dev)
+        self.Z64 = self.R64 = None
+        if self.fp64:
+            self.Z64 = Z.double()
+            self.R64 = torch.zeros((self.n, self.C1), dtype=torch.float64, device=dev)
+            self.cand = torch.zeros(self.n, dtype=torch.int32, device=dev)
+            self.cand_n = torch.zeros(1, dtype=torch.int32, device=dev)
+        self._desc = _lib.LPDesc(n=self.n, C1=self.C1, K2=int(self.K == 2), pad_=0, eta=self.eta,
+                                 prior=self.prior, indptr=ptr(self.indptr), rowlen=ptr(self.rowlen),
+                                 col=ptr(self.col), w=ptr(self.w), labeled=ptr(self.labeled), y=ptr(self.y),
+                                 solved=ptr(self.solved), labcnt=ptr(self.labcnt), Z32=ptr(self.Z32),
+                                 R32=ptr(self.R32), ld=self.ld, diag64=ptr(self.diag64), diag32=ptr(self.diag32),
+                                 Z64=ptr(self.Z64), R64=ptr(self.R64))
+        h = ct.c_void_p()
+        _lib.call("dlp_lp_create", ct.byref(self._desc), ct.byref(h))
+        self.h = h
+        self._set_eps()
+        self._status = np.zeros(16, dtype=np.int64)
+        self._rho = np.zeros(self.C1, dtype=np.float64)
+        self.batches = 0
+        self.scratch3 = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) is not None and self.h.value:
+                _lib.call("dlp_lp_destroy", self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def _set_eps(self):
+        eps = np.array([self.eps] * self.C + [self.eps_h], dtype=np.float64)
+        _lib.call("dlp_lp_set_eps", self.h, eps.ctypes.data, float(self.push_scale), stream())
+
+    # -- device steps ---------------------------------------------------------------------------
+    def status(self):
+        _lib.call("dlp_lp_status", self.h, self._status.ctypes.data, self._rho.ctypes.data, stream())
+        s = self._status
+        return dict(changed=int(s[0]), missing=int(s[1]), overflow=int(s[2]), toggled=int(s[3]), viol=int(s[4]),
+                    touched=int(s[6]), round=int(s[7]), edges=int(s[8]), cand_total=int(s[9]),
+                    push_total=int(s[10]), gather_total=int(s[11]), push_scan_total=int(s[12]),
+                    rounds_launched=int(s[13]), push_us=int(s[14]))
+
+    def _push(self):
+        r = ct.c_int64(0)
+        _lib.call("dlp_lp_push", self.h, self.max_rounds, self.rounds_per_launch, int(self.use_graph),
+                  ct.byref(r), stream())
+        return int(r.value)
+
+    def _push64(self):
+        """Host-driven FP64 rounds (solvers.py:108-116); one host sync per round."""
+        lst, cnt = ct.c_void_p(), ct.c_void_p()
+        _lib.call("dlp_lp_violators", self.h, ct.byref(lst), ct.byref(cnt))
+        st = self.status()
+        nc, src, rounds = st["viol"], lst.value, 0
+        bufs = [self.cand, torch.empty_like(self.cand)]
+        while nc and rounds < self.max_rounds:
+            _lib.call("dlp_lp_push64_round", self.h, nc, src, stream())
+            dst = bufs[rounds & 1]
+            _lib.call("dlp_lp_push64_next", self.h, ptr(dst), ptr(self.cand_n), stream())
+            nc = int(self.cand_n.item())
+            src = dst.data_ptr()
+            rounds += 1
+        return rounds
+
+    def _repair(self, st: Stats):
+        """Push until clean, then verify the touched rows in FP64 (refinement) and repeat."""
+        if self.fp64:
+            st.iters += self._push64()
+            return self.status()
+        st.iters += self._push()
+        for it in range(8):
+            _lib.call("dlp_lp_refresh", self.h, 1, 0, stream())
+            s = self.status()
+            if s["viol"] == 0:
+                return s
+            st.refine += 1
+            st.iters += self._push()
+        return self.status()
+
+    def _upload(self, b: EdgeBatch):
+        dev = device()
+        f = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a)).to(dev, dt, non_blocking=False)  # noqa: E731
+        return (f(b.del_u, torch.int32), f(b.del_v, torch.int32), f(b.ins_u, torch.int32),
+                f(b.ins_v, torch.int32), f(b.ins_w, torch.float32))
+
+    def _regrow(self, ins, extra):
+        iu, iv, iw = ins
+        new_ptr = torch.empty(self.n + 1, dtype=torch.int32, device=device())
+        total = ct.c_int64(0)
+        _lib.call("dlp_lp_regrow_plan", self.h, int(extra), 0.25, ptr(new_ptr), ct.byref(total), stream())
+        col = torch.empty(int(total.value), dtype=torch.int32, device=device())
+        w = torch.empty(int(total.value), dtype=torch.float32, device=device())
+        if self.scratch3 is None:
+            self.scratch3 = torch.empty(3 << 20, dtype=torch.int32, device=device())
+        _lib.call("dlp_lp_regrow_commit", self.h, ptr(new_ptr), ptr(col), ptr(w), int(iu.shape[0]), ptr(iu),
+                  ptr(iv), ptr(iw), ptr(self.scratch3), stream())
+        self.indptr, self.col, self.w = new_ptr, col, w
+
+    # -- public ---------------------------------------------------------------------------------
+    def initial_solve(self):
+        """Ground, refresh every row and push to eps from F = prior, h = 1."""
+        st = Stats(path="initial")
+        t0 = time.perf_counter()
+        _lib.call("dlp_lp_begin_batch", self.h, stream())
+        _lib.call("dlp_lp_ground", self.h, stream())
+        prec = 1 if self.fp64 else 0
+        _lib.call("dlp_lp_refresh", self.h, 2, prec, stream())
+        s = self._repair(st)
+        if not self.fp64:
+            _lib.call("dlp_lp_refresh", self.h, 2, 0, stream())
+            s = self.status()
+            tries = 0
+            while s["viol"]:
+                tries += 1
+                if tries > 20:
+                    raise RuntimeError(f"initial solve: {s['viol']} rows still violate after refinement")
+                st.iters += self._push()
+                _lib.call("dlp_lp_refresh", self.h, 2, 0, stream())
+                s = self.status()
+        st.ms = (time.perf_counter() - t0) * 1e3
+        self._finish(st, s)
+        return st
+
+    def apply_and_solve(self, b: EdgeBatch) -> Stats:
+        """Apply one batch of edge updates and repair the solution to eps (certified in FP64)."""
+        st = Stats(path="fp64-push" if self.fp64 else "push")
+        t0 = time.perf_counter()
+        du, dv, iu, iv, iw = self._upload(b)
+        _lib.call("dlp_lp_begin_batch", self.h, stream())
+        _lib.call("dlp_lp_apply", self.h, int(du.shape[0]), ptr(du), ptr(dv), int(iu.shape[0]), ptr(iu), ptr(iv),
+                  ptr(iw), stream())
+        _lib.call("dlp_lp_ground", self.h, stream())
+        prec = 1 if self.fp64 else 0
+        _lib.call("dlp_lp_refresh", self.h, 0, prec, stream())
+        s = self._repair(st)
+        if s["overflow"]:
+            # room for every insertion of this batch in any one row, so one regrow suffices
+            per_row = np.bincount(np.concatenate([b.ins_u, b.ins_v]), minlength=1).max()
+            self._regrow((iu, iv, iw), max(8, int(per_row)))
+            _lib.call("dlp_lp_ground", self.h, stream())   # re-inserted edges can re-ground rows
+            _lib.call("dlp_lp_refresh", self.h, 0, prec, stream())
+            s = self._repair(st)
+        st.ms = (time.perf_counter() - t0) * 1e3
+        if s["missing"]:
+            st.extra["missing_deletions"] = s["missing"]
+        self._finish(st, s)
+        self.batches += 1
+        return st
+
+    def _finish(self, st, s):
+        st.edges, st.touched, st.changed, st.toggled = s["edges"], s["touched"], s["changed"], s["toggled"]
+        st.rho = float(self._rho[: self.C].max()) if self.C else 0.0
+
+    def full_check(self):
+        """FP64 residual of every row: (rho per column, violators) — the periodic full refresh
+        of solvers.py:370-373, which also resyncs the kept residual."""
+        prec = 1 if self.fp64 else 0
+        _lib.call("dlp_lp_refresh", self.h, 2, prec, stream())
+        s = self.status()
+        return self._rho.copy(), s["viol"]
+
+    def scores_with_h(self):
+        Z = self.Z64 if self.fp64 else self.Z32
+        return Z[:, : self.C1].double().cpu().numpy()
+
+    def solved_mask(self):
+        return self.solved.cpu().numpy().astype(bool)
+
+    def components(self):
+        """Canonical CC labels (int32, cc.py:8-23) of the current whole graph."""
+        from . import cc as _cc
+        from .backend import CSR
+        W = CSR(self.w, self.col, self.indptr, (self.n, self.n), rowlen=self.rowlen)
+        lab, _ = _cc.connected_components_csr(W)
+        return lab.cpu().numpy()

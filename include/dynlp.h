@@ -129,6 +129,92 @@ int dlp_cc_csr(int n, const int32_t* indptr, const int32_t* rowlen, const int32_
                void* stream);
 size_t dlp_cc_workspace_bytes(int n);
 
+
+/* ---- Resident incremental state of one graph (the service's per-graph handle) ----------
+ * Replaces IncrementalStream's in-place batch apply (stream.py:180-346) and IncrementalLP's
+ * resident Z/R with changed-row refresh and push (solvers.py:346-434), for edge-level updates
+ * on a slack CSR.  All arrays are caller-owned device memory except the handle's scratch.
+ * Columns: C1 = C + 1 (label columns then h, the absorption column of solvers.py:10-12). */
+typedef struct dlp_lp_desc {
+  int32_t n;              /* vertices (global ids; rows outside U are inert) */
+  int32_t C1;             /* solved columns including h */
+  int32_t K2;             /* 1: binary problem, only label 1 feeds column 0 (stream.py:167) */
+  int32_t pad_;
+  double eta;             /* dongle weight added to s (stream.py:165) */
+  double prior;           /* prior value per label column (eta*prior joins rhs) */
+  const int32_t* indptr;  /* slack store: row segments */
+  int32_t* rowlen;        /* live entries per row */
+  int32_t* col;           /* column of each entry, -1 in unused slots */
+  float* w;               /* FP32 weight of each entry (the exact operator weight) */
+  const uint8_t* labeled; /* labeled vertices (never solved) */
+  const int32_t* y;       /* label of each vertex (read where labeled) */
+  uint8_t* solved;        /* U mask: unlabeled and grounded (maintained by dlp_lp_ground) */
+  int32_t* labcnt;        /* labeled-neighbour count per vertex (maintained by dlp_lp_apply) */
+  float* Z32;             /* n x ld working solution [F, h] */
+  float* R32;             /* n x ld kept residual */
+  int64_t ld;
+  double* diag64;         /* diag = s + rowsum(W_UU), FP64 (inert rows 1) */
+  float* diag32;
+  double* Z64;            /* FP64 state of the plain per-batch path (may be NULL) */
+  double* R64;
+} dlp_lp_desc;
+
+int dlp_lp_create(const dlp_lp_desc* desc, void** handle);
+int dlp_lp_destroy(void* handle);
+/* Switch the handle to another store (after a regrow). */
+int dlp_lp_set_store(void* handle, const int32_t* indptr, int32_t* rowlen, int32_t* col, float* w);
+/* Stop thresholds: |R_c| <= eps[c]*diag (host array of C1, FP64); the FP32 push uses
+ * eps*push_scale.  Synchronises the stream. */
+int dlp_lp_set_eps(void* handle, const double* eps, double push_scale, void* stream);
+/* New batch: clears the changed-row list, counters and the batch-touched list. */
+int dlp_lp_begin_batch(void* handle, void* stream);
+/* Deletions (du, dv) then insertions (iu, iv, iw) of undirected pairs (distinct in a call):
+ * both stored directions change; endpoints join the changed-row list.  Missing deletions and
+ * insertions that found a full row are counted (dlp_lp_status); overflowed insertions are
+ * re-inserted by dlp_lp_regrow_commit.  Replaces IncrementalStream._toggle/_set_w
+ * (stream.py:202-228) and the weight/row-sum/class-weight updates (stream.py:221-235). */
+int dlp_lp_apply(void* handle, int nd, const int32_t* du, const int32_t* dv, int ni,
+                 const int32_t* iu, const int32_t* iv, const float* iw, void* stream);
+/* New row offsets with room to grow (rowlen + max(extra, growth*rowlen)); *total = capacity
+ * (host; synchronises). */
+int dlp_lp_regrow_plan(void* handle, int extra, double growth, int32_t* new_indptr, int64_t* total,
+                       void* stream);
+/* Copy into the new store, switch to it, re-insert the last apply's overflowed entries
+ * (scratch3: 3 * 2^20 int32 of device memory). */
+int dlp_lp_regrow_commit(void* handle, const int32_t* new_indptr, int32_t* new_col, float* new_w,
+                         int ni, const int32_t* iu, const int32_t* iv, const float* iw,
+                         int32_t* scratch3, void* stream);
+/* Grounding: U = unlabeled vertices whose component in the unlabeled subgraph has a vertex
+ * with a labeled neighbour (stream.py:148-158, replacing the ball/full ground checks of
+ * stream.py:237-294); toggled vertices and their neighbours join the changed-row list. */
+int dlp_lp_ground(void* handle, void* stream);
+/* Row refresh from scratch: s, diag, rhs and R = rhs - diag*Z + W_UU Z in FP64 for the rows of
+ * `which` (0 changed list, 1 rows touched by pushes in this batch, 2 all rows); inert rows
+ * parked (diag 1, F 0, h 1, R 0).  prec 0: Z32 -> R32 (rounded), prec 1: Z64 -> R64.
+ * Violators (|R_c| > eps_c*diag) are listed for the next push; column maxima of |R|/diag go to
+ * the status.  Replaces solvers.py:404-407 (refresh) and :370-373 (full residual). */
+int dlp_lp_refresh(void* handle, int which, int prec, void* stream);
+/* Round-synchronous FP32 push from the current violators until no vertex violates or
+ * max_rounds; rounds are replayed from a captured hipGraph of rounds_per_launch rounds (even)
+ * when use_graph != 0.  Synchronises once per launch.  Replaces FusedPush.run
+ * (kernels.py:415-433) as the service's repair step. */
+int dlp_lp_push(void* handle, int max_rounds, int rounds_per_launch, int use_graph,
+                int64_t* rounds_out, void* stream);
+/* Plain FP64 path, one host-driven round of solvers.push (solvers.py:108-116): candidates
+ * take D = R/diag, Z += D, R = 0, scatter W D with FP64 atomics and mark N(cand) and cand. */
+int dlp_lp_push64_round(void* handle, int nc, const int32_t* cand, void* stream);
+/* Marked vertices (ascending) that violate in FP64 -> cand_out, count (device); marks cleared. */
+int dlp_lp_push64_next(void* handle, int32_t* cand_out, int32_t* count_out_dev, void* stream);
+/* Device list and count of the last refresh's violators. */
+int dlp_lp_violators(void* handle, int32_t** list, int32_t** count);
+/* Counters (host; synchronises): [0] changed rows, [1] missing deletions, [2] overflowed
+ * insertions, [3] toggled vertices, [4] violators, [5] FP64-path candidates, [6] rows touched
+ * by pushes, [7] round number, [8] edge visits of this batch, cumulative push work [9]
+ * candidates, [10] pushes, [11] frontier entries gathered, [12] entries scanned by pushes,
+ * [13] push rounds launched, [14] their summed duration in microseconds (HIP events);
+ * rho (C1, may be NULL): column maxima of the last refresh.  `out` holds 16 entries. */
+int dlp_lp_status(void* handle, int64_t* out, double* rho, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
