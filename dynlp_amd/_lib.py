@@ -49,11 +49,14 @@ _SIGS = {
     "dlp_lp_regrow_commit": (ct.c_int, [P, P, P, P, I32, P, P, P, P, P]),
     "dlp_lp_ground": (ct.c_int, [P, P]),
     "dlp_lp_refresh": (ct.c_int, [P, I32, I32, P]),
-    "dlp_lp_push": (ct.c_int, [P, I32, I32, I32, P, P]),
+    "dlp_lp_push": (ct.c_int, [P, I32, I32, I32, I64, P, P, P]),
     "dlp_lp_push64_round": (ct.c_int, [P, I32, P, P]),
     "dlp_lp_push64_next": (ct.c_int, [P, P, P, P]),
     "dlp_lp_violators": (ct.c_int, [P, P, P]),
     "dlp_lp_status": (ct.c_int, [P, P, P, P]),
+    "dlp_lp_pcg": (ct.c_int, [P, I32, I32, I32, I32, P, P, P]),
+    "dlp_lp_pcg_stats": (ct.c_int, [P, P]),
+    "dlp_lp_time_operator": (ct.c_int, [P, I32, P, P]),
 }
 
 

@@ -1,0 +1,213 @@
+// Fused BLAS-1 passes of the column-wise PCG (solvers.py:66-97) over row-major n x C blocks.
+//
+// Every pass maps column lanes over a row (LC = 2^lc_shift lanes, CPL column chunks per lane)
+// and rows over the grid, so each row is one coalesced access.  Column reductions accumulate
+// in FP64: per lane, then per workgroup in LDS, then one FP64 atomic per column per workgroup.
+#include "blas.h"
+
+namespace dlp {
+
+struct ColMap {
+  int lc_shift, cpl;
+};
+
+inline ColMap col_map(int C) {
+  ColMap m;
+  int lc = C <= 1 ? 0 : ceil_log2(C);
+  if (lc > 6) lc = 6;
+  m.lc_shift = lc;
+  m.cpl = (C + (1 << lc) - 1) >> lc;
+  return m;
+}
+
+constexpr int kMaxCols = 256;
+
+__device__ __forceinline__ void lds_add(double* p, double v) { atomicAdd(p, v); }
+
+// out[c] += sum_r X[r,c] * Y[r,c] * (scale_by_diag ? 1/diag[r] : 1)
+template <int CPL>
+__global__ __launch_bounds__(256) void k_coldot(int n, int C, int lc_shift, const float* __restrict__ X,
+                                                const float* __restrict__ Y, int64_t ld,
+                                                double* __restrict__ out) {
+  __shared__ double sm[kMaxCols];
+  for (int c = threadIdx.x; c < C; c += blockDim.x) sm[c] = 0.0;
+  __syncthreads();
+  const int LC = 1 << lc_shift;
+  const int l = threadIdx.x & (LC - 1);
+  double acc[CPL];
+#pragma unroll
+  for (int k = 0; k < CPL; ++k) acc[k] = 0.0;
+  const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> lc_shift;
+  for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> lc_shift; r < n; r += groups) {
+    const int64_t o = r * ld;
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) {
+      const int c = l + k * LC;
+      if (c < C) acc[k] += (double)X[o + c] * (double)Y[o + c];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < CPL; ++k) {
+    const int c = l + k * LC;
+    if (c < C) lds_add(&sm[c], acc[k]);
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&out[c], sm[c]);
+}
+
+// PCG step over all rows: Z += a P, R -= a AP; then Y = R/diag and
+// rz_new[c] = sum R*Y, viol = max |R|/(eps_c*diag) (float bits, atomic max).
+template <int CPL>
+__global__ __launch_bounds__(256) void k_pcg_update(int n, int C, int lc_shift, float* __restrict__ Z,
+                                                    float* __restrict__ R, const float* __restrict__ P,
+                                                    const float* __restrict__ AP, int64_t ld,
+                                                    const double* __restrict__ alpha,
+                                                    const float* __restrict__ diag,
+                                                    const float* __restrict__ eps,
+                                                    double* __restrict__ rz_new, float* __restrict__ viol) {
+  __shared__ double sm[kMaxCols];
+  __shared__ float vmax;
+  for (int c = threadIdx.x; c < C; c += blockDim.x) sm[c] = 0.0;
+  if (threadIdx.x == 0) vmax = 0.f;
+  __syncthreads();
+  const int LC = 1 << lc_shift;
+  const int l = threadIdx.x & (LC - 1);
+  float a[CPL], ie[CPL];
+  double acc[CPL];
+#pragma unroll
+  for (int k = 0; k < CPL; ++k) {
+    const int c = l + k * LC;
+    a[k] = c < C ? (float)alpha[c] : 0.f;
+    ie[k] = c < C ? 1.f / eps[c] : 0.f;
+    acc[k] = 0.0;
+  }
+  float vm = 0.f;
+  const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> lc_shift;
+  for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> lc_shift; r < n; r += groups) {
+    const int64_t o = r * ld;
+    const float inv = 1.f / diag[r];
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) {
+      const int c = l + k * LC;
+      if (c < C) {
+        Z[o + c] += a[k] * P[o + c];
+        const float rv = R[o + c] - a[k] * AP[o + c];
+        R[o + c] = rv;
+        acc[k] += (double)rv * (double)rv * (double)inv;
+        vm = fmaxf(vm, fabsf(rv) * inv * ie[k]);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < CPL; ++k) {
+    const int c = l + k * LC;
+    if (c < C) lds_add(&sm[c], acc[k]);
+  }
+  for (int off = 32; off > 0; off >>= 1) vm = fmaxf(vm, __shfl_xor(vm, off, 64));
+  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&vmax, vm);
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&rz_new[c], sm[c]);
+  if (threadIdx.x == 0) atomic_max_nonneg(viol, vmax);
+}
+
+// P = R/diag + beta P   (P = R/diag when first != 0)
+template <int CPL>
+__global__ __launch_bounds__(256) void k_pcg_dir(int n, int C, int lc_shift, float* __restrict__ P,
+                                                 const float* __restrict__ R, int64_t ld,
+                                                 const float* __restrict__ diag,
+                                                 const double* __restrict__ beta, int first) {
+  const int LC = 1 << lc_shift;
+  const int l = threadIdx.x & (LC - 1);
+  float b[CPL];
+#pragma unroll
+  for (int k = 0; k < CPL; ++k) {
+    const int c = l + k * LC;
+    b[k] = (c < C && !first) ? (float)beta[c] : 0.f;
+  }
+  const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> lc_shift;
+  for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> lc_shift; r < n; r += groups) {
+    const int64_t o = r * ld;
+    const float inv = 1.f / diag[r];
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) {
+      const int c = l + k * LC;
+      if (c < C) P[o + c] = R[o + c] * inv + (first ? 0.f : b[k] * P[o + c]);
+    }
+  }
+}
+
+// Scalars of one iteration (one workgroup): mode 0: alpha = rz/pAp (0 where pAp <= 0), then
+// clear rz_new and viol for the update pass; mode 1: beta = rz_new/rz (0 where rz <= 0),
+// rz = rz_new, then clear pAp for the next iteration.
+__global__ void k_pcg_scalars(int C, int mode, double* __restrict__ rz, double* __restrict__ rz_new,
+                              double* __restrict__ pAp, double* __restrict__ alpha,
+                              double* __restrict__ beta, float* __restrict__ viol) {
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    if (mode == 0) {
+      const double p = pAp[c];
+      alpha[c] = p > 0.0 ? rz[c] / p : 0.0;
+      rz_new[c] = 0.0;
+    } else {
+      const double r = rz[c];
+      beta[c] = r > 0.0 ? rz_new[c] / r : 0.0;
+      rz[c] = rz_new[c];
+      pAp[c] = 0.0;
+    }
+  }
+  if (mode == 0 && threadIdx.x == 0) *viol = 0.f;
+}
+
+// Clears n doubles (a fill kernel, so it can sit in a captured graph).
+__global__ void k_zero_f64(int n, double* __restrict__ p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+}  // namespace dlp
+
+namespace dlp {
+
+static unsigned pass_grid(int n, int lc_shift) {
+  long long threads = (long long)n << lc_shift;
+  unsigned g = grid_for(threads);
+  return g > 2048 ? 2048 : g;
+}
+
+int pcg_start(const PcgBufs& b, hipStream_t st) {
+  DLP_CHECK_ARG(b.C >= 1 && b.C <= kMaxCols, "pcg: C=%d outside [1, %d]", b.C, kMaxCols);
+  const ColMap m = col_map(b.C);
+  const unsigned g = pass_grid(b.n, m.lc_shift);
+  hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, b.C, b.rz);
+  hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, b.C, b.pAp);
+#define K(CPLV)                                                                                     \
+  hipLaunchKernelGGL((k_pcg_dir<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.P, b.R, \
+                     b.ld, b.diag, b.beta, 1);                                                     \
+  hipLaunchKernelGGL((k_coldot<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.R, b.P,  \
+                     b.ld, b.rz);
+  DLP_DISPATCH_CPL(m.cpl, K)
+#undef K
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+int pcg_finish_iteration(const PcgBufs& b, hipStream_t st) {
+  const ColMap m = col_map(b.C);
+  const unsigned g = pass_grid(b.n, m.lc_shift);
+#define K(CPLV)                                                                                       \
+  hipLaunchKernelGGL((k_coldot<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.P, b.AP,   \
+                     b.ld, b.pAp);                                                                   \
+  hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 0, b.rz, b.rz_new, b.pAp,        \
+                     b.alpha, b.beta, b.viol);                                                       \
+  hipLaunchKernelGGL((k_pcg_update<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.Z, b.R, \
+                     b.P, b.AP, b.ld, b.alpha, b.diag, b.eps, b.rz_new, b.viol);                     \
+  hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 1, b.rz, b.rz_new, b.pAp,        \
+                     b.alpha, b.beta, b.viol);                                                       \
+  hipLaunchKernelGGL((k_pcg_dir<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.P, b.R,   \
+                     b.ld, b.diag, b.beta, 0);
+  DLP_DISPATCH_CPL(m.cpl, K)
+#undef K
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+}  // namespace dlp

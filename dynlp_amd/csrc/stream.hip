@@ -9,6 +9,8 @@
 // operator, so W_UU never has to be materialised.
 #include "rowgather.h"
 #include "scan.h"
+#include "blas.h"
+#include "rowgather.h"
 
 namespace dlp {
 
@@ -270,7 +272,7 @@ struct RefreshArgs {
 
 template <typename TZ, typename TR, int CPL>
 __device__ __forceinline__ void refresh_row(const RefreshArgs& A, const Lanes& L, int64_t i, int nr,
-                                            TZ* __restrict__ Z, TR* __restrict__ R) {
+                                            TZ* __restrict__ Z, TR* __restrict__ R, double (&rmax)[CPL]) {
   const bool valid = i < nr;
   const int r = valid ? (A.rows ? A.rows[i] : (int)i) : 0;
   const bool live = valid && A.solved[r];
@@ -382,7 +384,7 @@ __device__ __forceinline__ void refresh_row(const RefreshArgs& A, const Lanes& L
           R[zo + c] = (TR)res;
           const double ratio = fabs(res) / dg;
           viol |= ratio > A.eps[c];
-          if (A.rho) atomic_max_nonneg(A.rho + c, ratio);
+          rmax[k] = fmax(rmax[k], ratio);
         }
       }
     }
@@ -402,21 +404,44 @@ __global__ __launch_bounds__(256) void k_refresh(RefreshArgs A, int g_shift, int
   const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> g_shift;
   const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> g_shift;
   const int64_t wave_first = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) >> g_shift;
+  double rmax[CPL];
+#pragma unroll
+  for (int k = 0; k < CPL; ++k) rmax[k] = 0.0;
   for (int64_t base = wave_first; base < nr; base += groups)
-    refresh_row<TZ, TR, CPL>(A, L, base + (g0 - wave_first), nr, Z, R);
+    refresh_row<TZ, TR, CPL>(A, L, base + (g0 - wave_first), nr, Z, R, rmax);
+  if (!A.rho) return;
+  // column maxima: across the wave's row groups, then the workgroup (LDS), then one atomic
+  // per column per workgroup
+  __shared__ double sm[256];
+  for (int c = threadIdx.x; c < A.C1; c += blockDim.x) sm[c] = 0.0;
+  for (int off = L.G; off < 64; off <<= 1) {
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) rmax[k] = fmax(rmax[k], __shfl_xor(rmax[k], off, 64));
+  }
+  __syncthreads();
+  if ((threadIdx.x & 63) < L.G && L.e == 0) {
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) {
+      const int c = L.l + k * L.LC;
+      if (c < A.C1) atomic_max_nonneg(&sm[c], rmax[k]);
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < A.C1; c += blockDim.x) atomic_max_nonneg(A.rho + c, sm[c]);
 }
 
 // ---------------------------------------------------------------------------------------
-// Round-synchronous push (the service's push; deterministic).
+// Asynchronous scatter push (the service's repair step; chaotic relaxation as FusedPush,
+// kernels.py:269-351, with the compaction fused into the next round).
 //
-// Round t: F_t (stamped tag[t%2][u] == t, D[t%2][u] = its increment) has been pushed
-// (Z_u += D_u, R_u = 0).  Candidates N(F_t) (list t%3, each once) gather
-//   R_v += sum_{u in N(v), tag[t%2][u] == t} w_vu D_u
-// and, if now violating, are pushed at once for round t+1 (stamp t+1, D[(t+1)%2]) and append
-// their neighbours to list (t+1)%3.  Each wave owns one candidate's R/Z rows, and every value
-// read from other vertices belongs to the other parity, so no two waves race.  The round
-// number lives on the device (rnd[node parity]) so one captured graph replays any number of
-// times; counts are a ring of three (read t%3, append (t+1)%3, reset (t+2)%3).
+// Round t reads candidate list t%3 (vertices whose residual changed in round t-1, each listed
+// once).  A candidate that violates |R_c| > eps_c*diag takes its residual atomically
+// (concurrent pushes landing later are kept), adds R/diag to Z and scatters w*R/diag into its
+// solved neighbours' residuals with hardware FP32 atomics; every neighbour whose residual it
+// changed is appended once to list (t+1)%3.  A round with an empty list sets `done`.  The
+// round number lives on the device (rnd[node parity]) so one captured graph of rounds replays
+// any number of times; counts are a ring of three (read t%3, append (t+1)%3, reset (t+2)%3).
+// All list appends are aggregated per wavefront (one atomic per wave).
 
 struct PushState {
   int n, C1;
@@ -430,104 +455,30 @@ struct PushState {
   const float* eps;      // FP32 thresholds, C1
   float* Z;
   float* R;
-  float* D[2];
-  int32_t* tag[2];
-  int32_t* ctag;         // candidate stamp (round of the list a vertex was appended to)
+  int32_t* ctag;         // candidate stamp: round of the list a vertex was last appended to
   int32_t* list[3];
   int32_t* count;        // [3]
   int32_t* rnd;          // [2] round number by node parity
   int32_t* done;         // set when a round finds no candidates
-  int32_t* btag;         // batch stamp: vertex touched during the current batch
+  int32_t* btag;         // batch stamp: vertex pushed during the current batch
   int32_t* blist;
   int32_t* bcount;
   const int32_t* bstamp; // device: current batch stamp
   unsigned long long* edges;
-  unsigned long long* stats;  // [0] candidates, [1] pushes, [2] frontier entries gathered, [3] entries scanned by pushes
+  unsigned long long* stats;  // [0] candidates, [1] pushes, [2] scatter entries, [3] unused
 };
 
-template <int CPL>
-__device__ __forceinline__ void push_vertex(const PushState& S, const Lanes& L, int v, int tq,
-                                            float (&r)[CPL], int32_t* next, int32_t* ncount) {
-  // r holds R_v (columns of this lane); push it: D = R/diag, Z += D, R = 0
-  const float inv = 1.f / S.diag[v];
-  const int64_t o = (int64_t)v * S.ld;
-  const int q = tq & 1;
-  if (L.e == 0) {
-#pragma unroll
-    for (int k = 0; k < CPL; ++k) {
-      const int c = L.l + k * L.LC;
-      if (c < S.C1) {
-        const float d = r[k] * inv;
-        S.Z[o + c] += d;
-        S.R[o + c] = 0.f;
-        S.D[q][o + c] = d;
-      }
-    }
-  }
-  if (L.lane == 0) S.tag[q][v] = tq;
-  const int beg = S.indptr[v], end = beg + S.rowlen[v];
-  if (L.lane == 0) {
-    atomicAdd(S.stats + 1, 1ull);
-    atomicAdd(S.stats + 3, (unsigned long long)(end - beg));
-  }
-  for (int j = beg + L.lane; j < end; j += L.G) {
-    const int x = S.col[j];
-    if (x >= 0 && S.solved[x]) {
-      if (atomicExch(S.ctag + x, tq) != tq) next[atomicAdd(ncount, 1)] = x;
-    }
-  }
-}
-
-template <int CPL>
-__device__ __forceinline__ bool violates(const PushState& S, const Lanes& L, int v, const float (&r)[CPL]) {
-  const float dv = S.diag[v];
-  bool viol = false;
-#pragma unroll
-  for (int k = 0; k < CPL; ++k) {
-    const int c = L.l + k * L.LC;
-    if (c < S.C1) viol |= fabsf(r[k]) > S.eps[c] * dv;
-  }
-  const unsigned long long bal = __ballot(viol);
-  const int wl = threadIdx.x & 63;
-  const unsigned long long gm = (L.G == 64 ? ~0ull : ((1ull << L.G) - 1)) << (wl & ~(L.G - 1));
-  return (bal & gm) != 0;
-}
-
-// Seed: push the listed violators as F_{t0} (t0 = rnd[0]), candidates into list t0%3.
-template <int CPL>
-__device__ __forceinline__ void seed_one(const PushState& S, const Lanes& L, int v, int t0) {
-  float r[CPL];
-  const int64_t o = (int64_t)v * S.ld;
-#pragma unroll
-  for (int k = 0; k < CPL; ++k) {
-    const int c = L.l + k * L.LC;
-    r[k] = c < S.C1 ? S.R[o + c] : 0.f;
-  }
-  const int bs = *S.bstamp;
-  if (L.lane == 0 && atomicExch(S.btag + v, bs) != bs) S.blist[atomicAdd(S.bcount, 1)] = v;
-  push_vertex<CPL>(S, L, v, t0, r, S.list[t0 % 3], S.count + (t0 % 3));
-}
-
-template <int CPL>
-__global__ __launch_bounds__(256) void k_push_seed(PushState S, const int32_t* __restrict__ seeds,
-                                                   const int32_t* __restrict__ nseeds, int g_shift,
-                                                   int lc_shift) {
-  const Lanes L(g_shift, lc_shift);
-  const int t0 = S.rnd[0];
-  const int ns = *nseeds;
-  const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> g_shift;
-  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> g_shift; i < ns; i += groups)
-    seed_one<CPL>(S, L, seeds[i], t0);
-}
-
-// Reset for a new push phase: counts, done flag; rnd[1] mirrors rnd[0].
-__global__ void k_push_begin(PushState S) {
-  if (threadIdx.x == 0) {
-    const int t0 = S.rnd[0];
-    S.count[0] = S.count[1] = S.count[2] = 0;
-    S.rnd[1] = t0;
-    *S.done = 0;
-  }
+// Append val for every active lane with pred, one atomic per wavefront.
+__device__ __forceinline__ void wave_append(bool pred, int val, int32_t* __restrict__ list,
+                                            int32_t* __restrict__ count) {
+  const unsigned long long m = __ballot(pred);
+  if (m == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(m));
+  base = __shfl(base, leader, 64);
+  if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = val;
 }
 
 template <int CPL>
@@ -541,7 +492,6 @@ __global__ __launch_bounds__(256) void k_push_round(PushState S, int parity, int
     if (nc == 0) *S.done = 1;
   }
   if (nc == 0) return;
-  const int p = t & 1;
   const int tq = t + 1;
   const int32_t* cand = S.list[t % 3];
   int32_t* next = S.list[tq % 3];
@@ -550,91 +500,109 @@ __global__ __launch_bounds__(256) void k_push_round(PushState S, int parity, int
   const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> g_shift;
   const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> g_shift;
   const int64_t wave_first = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) >> g_shift;
-  unsigned long long my_edges = 0, my_cand = 0, my_dg = 0;
+  const int wl = threadIdx.x & 63;
+  const unsigned long long gm = (L.G == 64 ? ~0ull : ((1ull << L.G) - 1)) << (wl & ~(L.G - 1));
+  unsigned long long my_cand = 0, my_push = 0, my_sc = 0;
   for (int64_t base = wave_first; base < nc; base += groups) {
     const int64_t gi = base + (g0 - wave_first);
     const bool valid = gi < nc;
     const int v = valid ? cand[gi] : 0;
+    const int64_t o = (int64_t)v * S.ld;
+    // violation test on the current residual (edge lane 0 of the group reads the row)
+    bool viol = false;
+    if (valid && L.e == 0) {
+      const float dv = S.diag[v];
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) {
+        const int c = L.l + k * L.LC;
+        if (c < S.C1) viol |= fabsf(S.R[o + c]) > S.eps[c] * dv;
+      }
+    }
+    const bool push = valid && (__ballot(viol) & gm) != 0ull;
     if (valid && L.lane == 0) ++my_cand;
+    // take the residual and update Z
+    float d[CPL];
     int beg = 0, end = 0;
-    if (valid) {
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) d[k] = 0.f;
+    if (push) {
+      if (L.e == 0) {
+        const float inv = 1.f / S.diag[v];
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) {
+          const int c = L.l + k * L.LC;
+          if (c < S.C1) {
+            d[k] = atomic_take(S.R + o + c) * inv;
+            S.Z[o + c] += d[k];
+          }
+        }
+      }
       beg = S.indptr[v];
       end = beg + S.rowlen[v];
       if (L.lane == 0) {
-        my_edges += (unsigned long long)(end - beg);
-        if (atomicExch(S.btag + v, bs) != bs) S.blist[atomicAdd(S.bcount, 1)] = v;
+        ++my_push;
+        my_sc += (unsigned long long)(end - beg);
       }
     }
-    // gather the increments of this round's frontier neighbours
-    float acc[CPL];
 #pragma unroll
-    for (int k = 0; k < CPL; ++k) acc[k] = 0.f;
-    for (int b0 = beg; b0 < end; b0 += L.G) {
-      const int j = b0 + L.lane;
-      int myv = 0;
-      float myw = 0.f;
+    for (int k = 0; k < CPL; ++k) d[k] = __shfl(d[k], L.l, L.G);
+    // scatter w*d into the solved neighbours (no-return atomics)
+    for (int j = beg + L.e; j < end; j += L.LE) {
+      const int x = S.col[j];
+      if (x < 0 || !S.solved[x]) continue;
+      const float wv = S.w[j];
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) {
+        const int c = L.l + k * L.LC;
+        if (c < S.C1) atomic_add(S.R + (int64_t)x * S.ld + c, wv * d[k]);
+      }
+    }
+    // neighbours join the next list once (all lanes of the group over the entries)
+    for (int j0 = beg; j0 < end; j0 += L.G) {
+      const int j = j0 + L.lane;
+      bool app = false;
+      int x = 0;
       if (j < end) {
-        myv = S.col[j];
-        if (myv >= 0 && S.tag[p][myv] == t && S.solved[myv]) myw = S.w[j];
-        if (myw != 0.f) ++my_dg;
+        x = S.col[j];
+        app = x >= 0 && S.solved[x] && atomicExch(S.ctag + x, tq) != tq;
       }
-      const int cnt = min(L.G, end - b0);
-      for (int t0 = 0; t0 < cnt; t0 += 4 * L.LE) {
-        int vv[4];
-        float ww[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int tt = t0 + u * L.LE + L.e;
-          vv[u] = __shfl(myv, tt & (L.G - 1), L.G);
-          ww[u] = __shfl(myw, tt & (L.G - 1), L.G);
-          if (tt >= cnt) ww[u] = 0.f;
-        }
-        float xv[4][CPL];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const float* dr = S.D[p] + (int64_t)vv[u] * S.ld;
-#pragma unroll
-          for (int k = 0; k < CPL; ++k) {
-            const int c = L.l + k * L.LC;
-            xv[u][k] = (ww[u] != 0.f && c < S.C1) ? dr[c] : 0.f;
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int k = 0; k < CPL; ++k) acc[k] += ww[u] * xv[u][k];
-      }
+      wave_append(app, x, next, ncount);
     }
-    for (int off = L.LC; off < L.G; off <<= 1) {
-#pragma unroll
-      for (int k = 0; k < CPL; ++k) acc[k] += __shfl_xor(acc[k], off, 64);
-    }
-    // R_v += acc (edge lane 0 holds the sums; broadcast to the group)
-    float r[CPL];
-    const int64_t o = (int64_t)v * S.ld;
-#pragma unroll
-    for (int k = 0; k < CPL; ++k) {
-      const int c = L.l + k * L.LC;
-      float val = 0.f;
-      if (valid && L.e == 0 && c < S.C1) {
-        val = S.R[o + c] + acc[k];
-        S.R[o + c] = val;
-      }
-      r[k] = __shfl(val, L.l, L.G);
-    }
-    const bool vio = violates<CPL>(S, L, valid ? v : 0, r) && valid;
-    if (vio) push_vertex<CPL>(S, L, v, tq, r, next, ncount);
+    // every candidate's residual changed in this batch: it joins the FP64 verification list
+    const bool first = valid && L.lane == 0 && atomicExch(S.btag + v, bs) != bs;
+    wave_append(first, v, S.blist, S.bcount);
   }
-  const unsigned long long se = wave_sum_u64(my_edges);
   const unsigned long long sc = wave_sum_u64(my_cand);
-  const unsigned long long sd = wave_sum_u64(my_dg);
-  if ((threadIdx.x & 63) == 0) {
-    if (se) atomicAdd(S.edges, se);
+  const unsigned long long sp = wave_sum_u64(my_push);
+  const unsigned long long ss = wave_sum_u64(my_sc);
+  if (wl == 0) {
     if (sc) atomicAdd(S.stats + 0, sc);
-    if (sd) atomicAdd(S.stats + 2, sd);
+    if (sp) atomicAdd(S.stats + 1, sp);
+    if (ss) {
+      atomicAdd(S.stats + 2, ss);
+      atomicAdd(S.edges, ss);
+    }
   }
 }
 
+// New push phase: the current violators become the candidate list of round t0 = rnd[0].
+__global__ void k_push_begin(PushState S, const int32_t* __restrict__ viol, const int32_t* __restrict__ nviol) {
+  const int t0 = S.rnd[0];
+  const int nv = *nviol;
+  int32_t* dst = S.list[t0 % 3];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += gridDim.x * blockDim.x) {
+    const int v = viol[i];
+    dst[i] = v;
+    S.ctag[v] = t0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S.count[t0 % 3] = nv;
+    S.count[(t0 + 1) % 3] = 0;
+    S.count[(t0 + 2) % 3] = 0;
+    S.rnd[1] = t0;
+    *S.done = 0;
+  }
+}
 
 // ---------------------------------------------------------------------------------------
 // Host-driven FP64 push round of the plain per-batch path (solvers.py:100-125 with the
@@ -735,6 +703,15 @@ struct LP {
   RowMap map;                   // lane layout for C1 columns
   RowMap map_seed;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // global solve (column-wise PCG on the FP32 state)
+  float* P = nullptr;
+  float* AP = nullptr;
+  double* pcg_scal = nullptr;   // rz, rz_new, pAp, alpha, beta (C1 each)
+  float* pcg_viol = nullptr;
+  hipGraphExec_t pcg_exec = nullptr;
+  int pcg_graph_iters = 0;
+  double pcg_ms = 0.0;
+  int64_t pcg_iters = 0;
   double push_ms = 0.0;         // summed duration of push-round launches (events)
   int64_t push_launched = 0;    // rounds launched
 };
@@ -750,7 +727,7 @@ int dev_alloc(T** p, size_t count) {
 int lp_free(LP* h) {
   void* ptrs[] = {h->stamp, h->rows, h->cnt, h->overflow, h->toggled, h->parent, h->grounded,
                   h->viol, h->eps64, h->rho, h->mark, h->dsnap, h->cand64, h->scan_ws,
-                  h->S.D[0], h->S.D[1], h->S.tag[0], h->S.tag[1], h->S.ctag, h->S.list[0],
+                  h->S.ctag, h->S.list[0],
                   h->S.list[1], h->S.list[2], h->S.count, h->S.rnd, h->S.done, h->S.btag,
                   h->S.blist, h->S.bcount, (void*)h->S.bstamp, (void*)h->S.eps, h->S.edges,
                   h->S.stats};
@@ -759,6 +736,11 @@ int lp_free(LP* h) {
   if (h->host) (void)hipHostFree(h->host);
   if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->P) (void)hipFree(h->P);
+  if (h->AP) (void)hipFree(h->AP);
+  if (h->pcg_scal) (void)hipFree(h->pcg_scal);
+  if (h->pcg_viol) (void)hipFree(h->pcg_viol);
+  if (h->pcg_exec) (void)hipGraphExecDestroy(h->pcg_exec);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   delete h;
   return DLP_OK;
@@ -821,7 +803,6 @@ extern "C" int dlp_lp_create(const dlp_lp_desc* d, void** handle) {
   A_(h->stamp, n); A_(h->rows, n); A_(h->cnt, CNT_N); A_(h->overflow, 1 << 20);
   A_(h->toggled, n); A_(h->parent, n); A_(h->grounded, n); A_(h->viol, n);
   A_(h->eps64, d->C1); A_(h->rho, d->C1); A_(h->mark, n); A_(h->cand64, n);
-  A_(h->S.D[0], nz); A_(h->S.D[1], nz); A_(h->S.tag[0], n); A_(h->S.tag[1], n);
   A_(h->S.ctag, n); A_(h->S.list[0], n); A_(h->S.list[1], n); A_(h->S.list[2], n);
   A_(h->S.count, 3); A_(h->S.rnd, 2); A_(h->S.done, 1); A_(h->S.btag, n); A_(h->S.blist, n);
   A_(h->S.bcount, 1); A_(h->S.edges, 1); A_(h->S.stats, 4);
@@ -842,8 +823,6 @@ extern "C" int dlp_lp_create(const dlp_lp_desc* d, void** handle) {
   }
   // initial contents: stamps/tags -1 (never equal to a live stamp), counters 0
   (void)hipMemset(h->stamp, 0xff, sizeof(int32_t) * n);
-  (void)hipMemset(h->S.tag[0], 0xff, sizeof(int32_t) * n);
-  (void)hipMemset(h->S.tag[1], 0xff, sizeof(int32_t) * n);
   (void)hipMemset(h->S.ctag, 0xff, sizeof(int32_t) * n);
   (void)hipMemset(h->S.btag, 0xff, sizeof(int32_t) * n);
   (void)hipMemset(h->cnt, 0, sizeof(int32_t) * CNT_N);
@@ -903,6 +882,10 @@ extern "C" int dlp_lp_set_store(void* handle, const int32_t* indptr, int32_t* ro
   if (h->gexec) {  // the captured rounds hold the old pointers
     (void)hipGraphExecDestroy(h->gexec);
     h->gexec = nullptr;
+  }
+  if (h->pcg_exec) {
+    (void)hipGraphExecDestroy(h->pcg_exec);
+    h->pcg_exec = nullptr;
   }
   return DLP_OK;
 }
@@ -1143,30 +1126,22 @@ static int capture_rounds(LP* h, int rounds, hipStream_t st) {
 // parity 0,1,0,1,... so it must start on rnd[0].  A push phase begins with rnd[1] = rnd[0]
 // (k_push_begin) and a replay of an even number of rounds ends with the round in rnd[0].
 extern "C" int dlp_lp_push(void* handle, int max_rounds, int rounds_per_launch, int use_graph,
-                           int64_t* rounds_out, void* stream) {
+                           int64_t budget_edges, int64_t* rounds_out, int32_t* converged_out,
+                           void* stream) {
   DLP_CHECK_ARG(handle && max_rounds >= 0 && rounds_per_launch >= 2 && (rounds_per_launch % 2) == 0,
                 "lp_push: bad argument (rounds_per_launch must be even)");
   LP* h = (LP*)handle;
   hipStream_t st = (hipStream_t)stream;
   const RowMap& m = h->map;
-  const RowMap& ms = h->map;
-  hipLaunchKernelGGL(k_push_begin, dim3(1), dim3(64), 0, st, h->S);
+  hipLaunchKernelGGL(k_push_begin, dim3(256), dim3(kBlock), 0, st, h->S, h->viol, h->cnt + CNT_VIOL);
   DLP_CHECK_LAUNCH();
-  {
-    const unsigned grid = persistent_grid(h->d.n, ms.g_shift);
-#define K(CPLV)                                                                                   \
-  hipLaunchKernelGGL((k_push_seed<CPLV>), dim3(grid), dim3(kBlock), 0, st, h->S, h->viol, h->cnt + CNT_VIOL, \
-                     ms.g_shift, ms.lc_shift);
-    DLP_DISPATCH_CPL(ms.cpl, K)
-#undef K
-    DLP_CHECK_LAUNCH();
-  }
   if (use_graph) {
     int rc = capture_rounds(h, rounds_per_launch, st);
     if (rc) return rc;
   }
   const unsigned grid = persistent_grid(h->d.n, m.g_shift);
   int64_t rounds = 0;
+  int32_t converged = 0;
   while (rounds < max_rounds) {
     DLP_CHECK_HIP(hipEventRecord(h->ev0, st));
     if (use_graph) {
@@ -1184,13 +1159,19 @@ extern "C" int dlp_lp_push(void* handle, int max_rounds, int rounds_per_launch, 
     rounds += rounds_per_launch;
     DLP_CHECK_HIP(hipEventRecord(h->ev1, st));
     DLP_CHECK_HIP(hipMemcpyAsync(h->host, h->S.done, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    DLP_CHECK_HIP(hipMemcpyAsync(h->host + 1, h->S.edges, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     DLP_CHECK_HIP(hipStreamSynchronize(st));
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess) h->push_ms += ms;
     h->push_launched += rounds_per_launch;
-    if (*(volatile int32_t*)h->host) break;
+    if (*(volatile int32_t*)h->host) {
+      converged = 1;
+      break;
+    }
+    if (budget_edges > 0 && ((volatile int64_t*)h->host)[1] > budget_edges) break;
   }
   if (rounds_out) *rounds_out = rounds;
+  if (converged_out) *converged_out = converged;
   return DLP_OK;
 }
 
@@ -1276,4 +1257,171 @@ extern "C" int dlp_lp_status(void* handle, int64_t* out, double* rho, void* stre
   out[CNT_N + 7] = h->push_launched;
   out[CNT_N + 8] = (int64_t)(h->push_ms * 1000.0);  // microseconds
   return DLP_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Global solve: column-wise PCG with the Jacobi preconditioner on the FP32 state, from the
+// kept residual R32 (solvers.py:66-97, the DynLP+ global path solvers.py:268-288).
+
+static PcgBufs pcg_bufs(LP* h) {
+  PcgBufs b;
+  b.n = h->d.n;
+  b.C = h->d.C1;
+  b.ld = h->d.ld;
+  b.Z = h->d.Z32;
+  b.R = h->d.R32;
+  b.P = h->P;
+  b.AP = h->AP;
+  b.diag = h->d.diag32;
+  b.eps = h->S.eps;
+  const int C1 = h->d.C1;
+  b.rz = h->pcg_scal;
+  b.rz_new = h->pcg_scal + C1;
+  b.pAp = h->pcg_scal + 2 * C1;
+  b.alpha = h->pcg_scal + 3 * C1;
+  b.beta = h->pcg_scal + 4 * C1;
+  b.viol = h->pcg_viol;
+  return b;
+}
+
+static int pcg_iteration(LP* h, const PcgBufs& b, hipStream_t st) {
+  // AP = diag*P - W_UU P
+  int rc = dlp_row_operator(b.n, nullptr, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, DLP_F32,
+                            h->d.solved, b.P, b.ld, DLP_F32, b.diag, -1.0, nullptr, 0, b.AP, b.ld, DLP_F32,
+                            0, 20.0, st);
+  if (rc) return rc;
+  return pcg_finish_iteration(b, st);
+}
+
+static int capture_pcg(LP* h, int iters) {
+  if (h->pcg_exec && h->pcg_graph_iters == iters) return DLP_OK;
+  if (h->pcg_exec) {
+    (void)hipGraphExecDestroy(h->pcg_exec);
+    h->pcg_exec = nullptr;
+  }
+  hipStream_t cs;
+  DLP_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  const PcgBufs b = pcg_bufs(h);
+  hipGraph_t graph;
+  int rc = DLP_OK;
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    for (int i = 0; i < iters && rc == DLP_OK; ++i) rc = pcg_iteration(h, b, cs);
+    e = hipStreamEndCapture(cs, &graph);
+  }
+  if (e == hipSuccess && rc == DLP_OK) {
+    e = hipGraphInstantiate(&h->pcg_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+  }
+  (void)hipStreamDestroy(cs);
+  if (e != hipSuccess || rc != DLP_OK) {
+    set_error("pcg graph capture failed: %s", hipGetErrorString(e));
+    h->pcg_exec = nullptr;
+    return DLP_ERR_HIP;
+  }
+  h->pcg_graph_iters = iters;
+  return DLP_OK;
+}
+
+// Runs PCG iterations (in captured groups of iters_per_launch when use_graph) until the FP32
+// recurrence residual meets the push thresholds (max |R|/(eps*diag) <= 1), the violation
+// stalls for `patience` launches, or max_iter.  *iters_out: iterations run; *viol_out: last
+// violation ratio.
+extern "C" int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int use_graph, int patience,
+                          int64_t* iters_out, double* viol_out, void* stream) {
+  DLP_CHECK_ARG(handle && max_iter >= 0 && iters_per_launch >= 1, "lp_pcg: bad argument");
+  LP* h = (LP*)handle;
+  hipStream_t st = (hipStream_t)stream;
+  const int C1 = h->d.C1;
+  const size_t nz = (size_t)h->d.n * h->d.ld;
+  if (!h->P) {
+    DLP_CHECK_HIP(hipMalloc((void**)&h->P, nz * sizeof(float)));
+    DLP_CHECK_HIP(hipMalloc((void**)&h->AP, nz * sizeof(float)));
+    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_scal, 5 * C1 * sizeof(double)));
+    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_viol, sizeof(float)));
+    DLP_CHECK_HIP(hipMemsetAsync(h->P, 0, nz * sizeof(float), st));
+    DLP_CHECK_HIP(hipMemsetAsync(h->AP, 0, nz * sizeof(float), st));
+  }
+  const PcgBufs b = pcg_bufs(h);
+  int rc = pcg_start(b, st);
+  if (rc) return rc;
+  if (use_graph) {
+    rc = capture_pcg(h, iters_per_launch);
+    if (rc) return rc;
+  }
+  int64_t it = 0;
+  float v = 0.f, best = 3.4e38f;
+  int stall = 0;
+  while (it < max_iter) {
+    DLP_CHECK_HIP(hipEventRecord(h->ev0, st));
+    if (use_graph) {
+      DLP_CHECK_HIP(hipGraphLaunch(h->pcg_exec, st));
+    } else {
+      for (int i = 0; i < iters_per_launch; ++i) {
+        rc = pcg_iteration(h, b, st);
+        if (rc) return rc;
+      }
+    }
+    it += iters_per_launch;
+    DLP_CHECK_HIP(hipEventRecord(h->ev1, st));
+    DLP_CHECK_HIP(hipMemcpyAsync(h->host, h->pcg_viol, sizeof(float), hipMemcpyDeviceToHost, st));
+    DLP_CHECK_HIP(hipStreamSynchronize(st));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess) h->pcg_ms += ms;
+    h->pcg_iters += iters_per_launch;
+    v = *(volatile float*)h->host;
+    if (v <= 1.f) break;
+    if (v < 0.9f * best) {
+      best = v;
+      stall = 0;
+    } else if (++stall >= patience) {
+      break;
+    }
+  }
+  if (iters_out) *iters_out = it;
+  if (viol_out) *viol_out = v;
+  return DLP_OK;
+}
+
+// Cumulative timing of the global solve: [0] iterations, [1] microseconds (HIP events).
+extern "C" int dlp_lp_pcg_stats(void* handle, int64_t* out) {
+  DLP_CHECK_ARG(handle && out, "lp_pcg_stats: null argument");
+  LP* h = (LP*)handle;
+  out[0] = h->pcg_iters;
+  out[1] = (int64_t)(h->pcg_ms * 1000.0);
+  return DLP_OK;
+}
+
+// Average duration (ms, HIP events on `stream`) of the global solve's operator launch
+// AP = diag*P - W_UU P on the live state, over reps launches.
+extern "C" int dlp_lp_time_operator(void* handle, int reps, double* ms_out, void* stream) {
+  DLP_CHECK_ARG(handle && reps >= 1 && ms_out, "lp_time_operator: bad argument");
+  LP* h = (LP*)handle;
+  DLP_CHECK_ARG(h->P && h->AP, "lp_time_operator: run dlp_lp_pcg first");
+  hipStream_t st = (hipStream_t)stream;
+  const PcgBufs b = pcg_bufs(h);
+  int rc = dlp_row_operator(b.n, nullptr, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, DLP_F32,
+                            h->d.solved, b.P, b.ld, DLP_F32, b.diag, -1.0, nullptr, 0, b.AP, b.ld, DLP_F32, 0,
+                            20.0, st);
+  if (rc) return rc;
+  DLP_CHECK_HIP(hipEventRecord(h->ev0, st));
+  for (int i = 0; i < reps; ++i) {
+    rc = dlp_row_operator(b.n, nullptr, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, DLP_F32, h->d.solved,
+                          b.P, b.ld, DLP_F32, b.diag, -1.0, nullptr, 0, b.AP, b.ld, DLP_F32, 0, 20.0, st);
+    if (rc) return rc;
+  }
+  DLP_CHECK_HIP(hipEventRecord(h->ev1, st));
+  DLP_CHECK_HIP(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  DLP_CHECK_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  *ms_out = ms / reps;
+  return DLP_OK;
+}
+
+// Live store sizes: out[0] = sum of rowlen (stored entries), out[1] = capacity.
+__global__ void k_sum_i32(int n, const int32_t* __restrict__ a, unsigned long long* __restrict__ out) {
+  unsigned long long s = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += (unsigned)a[i];
+  s = dlp::wave_sum_u64(s);
+  if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, s);
 }

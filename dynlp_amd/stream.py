@@ -71,13 +71,17 @@ class EdgeStreamLP:
     Z/R in FP64 and repairs with the host-driven push rounds of the reference."""
 
     def __init__(self, g: Graph, eps=1e-6, eps_h=0.05, fp64=False, eta=0.0, slack_min=4,
-                 slack_frac=0.25, push_scale=0.9, rounds_per_launch=16, use_graph=True, max_rounds=200_000):
+                 slack_frac=0.25, push_scale=0.9, rounds_per_launch=4, use_graph=True, max_rounds=200_000,
+                 push_frac=0.05, push_max_rows=4096, push_budget=1.0, pcg_iters_per_launch=8,
+                 pcg_max_iter=20_000):
         self.g, self.n, self.K = g, g.n, g.K
         self.C = 1 if g.K == 2 else g.K
         self.C1 = self.C + 1
         self.eps, self.eps_h, self.fp64, self.eta = float(eps), float(eps_h), bool(fp64), float(eta)
         self.rounds_per_launch, self.use_graph, self.max_rounds = rounds_per_launch, use_graph, max_rounds
         self.push_scale = push_scale
+        self.push_frac, self.push_budget, self.push_max_rows = push_frac, push_budget, push_max_rows
+        self.pcg_iters_per_launch, self.pcg_max_iter = pcg_iters_per_launch, pcg_max_iter
         dev = device()
         ip, ix, d = g.csr()
         sp_, rl, col, w = slack_layout(ip, ix, d, slack_min, slack_frac)
@@ -104,6 +108,7 @@ class EdgeStreamLP:
             self.R64 = torch.zeros((self.n, self.C1), dtype=torch.float64, device=dev)
             self.cand = torch.zeros(self.n, dtype=torch.int32, device=dev)
             self.cand_n = torch.zeros(1, dtype=torch.int32, device=dev)
+            self._rho_dev = torch.zeros(self.C1, dtype=torch.float64, device=dev)
         self._desc = _lib.LPDesc(n=self.n, C1=self.C1, K2=int(self.K == 2), pad_=0, eta=self.eta,
                                  prior=self.prior, indptr=ptr(self.indptr), rowlen=ptr(self.rowlen),
                                  col=ptr(self.col), w=ptr(self.w), labeled=ptr(self.labeled), y=ptr(self.y),
@@ -118,6 +123,7 @@ class EdgeStreamLP:
         self._rho = np.zeros(self.C1, dtype=np.float64)
         self.batches = 0
         self.scratch3 = None
+        self.nnz = int(ix.shape[0])
 
     def __del__(self):
         try:
@@ -137,14 +143,25 @@ class EdgeStreamLP:
         s = self._status
         return dict(changed=int(s[0]), missing=int(s[1]), overflow=int(s[2]), toggled=int(s[3]), viol=int(s[4]),
                     touched=int(s[6]), round=int(s[7]), edges=int(s[8]), cand_total=int(s[9]),
-                    push_total=int(s[10]), gather_total=int(s[11]), push_scan_total=int(s[12]),
+                    push_total=int(s[10]), scatter_total=int(s[11]),
                     rounds_launched=int(s[13]), push_us=int(s[14]))
 
-    def _push(self):
-        r = ct.c_int64(0)
+    def _push(self, budget=0):
+        r, ok = ct.c_int64(0), ct.c_int32(0)
         _lib.call("dlp_lp_push", self.h, self.max_rounds, self.rounds_per_launch, int(self.use_graph),
-                  ct.byref(r), stream())
-        return int(r.value)
+                  int(budget), ct.byref(r), ct.byref(ok), stream())
+        return int(r.value), bool(ok.value)
+
+    def _pcg(self):
+        it, v = ct.c_int64(0), ct.c_double(0.0)
+        _lib.call("dlp_lp_pcg", self.h, self.pcg_max_iter, self.pcg_iters_per_launch, int(self.use_graph), 50,
+                  ct.byref(it), ct.byref(v), stream())
+        return int(it.value), float(v.value)
+
+    def pcg_stats(self):
+        out = np.zeros(2, dtype=np.int64)
+        _lib.call("dlp_lp_pcg_stats", self.h, out.ctypes.data)
+        return dict(pcg_iters=int(out[0]), pcg_us=int(out[1]))
 
     def _push64(self):
         """Host-driven FP64 rounds (solvers.py:108-116); one host sync per round."""
@@ -162,19 +179,41 @@ class EdgeStreamLP:
             rounds += 1
         return rounds
 
-    def _repair(self, st: Stats):
-        """Push until clean, then verify the touched rows in FP64 (refinement) and repeat."""
+    def _repair(self, st: Stats, force_global=False):
+        """Mixed path (DynLP+ policy, solvers.py:272-288 with refinement :322-337): push while
+        the violating set is small and within the edge budget, else the global PCG; then FP64
+        verification of every row the repair touched, pushing again until it certifies.
+        FP64 path: the reference's incremental push to completion (solvers.py:375-389)."""
         if self.fp64:
             st.iters += self._push64()
-            return self.status()
-        st.iters += self._push()
+            s = self.status()
+            # the FP64 kept residual is exact up to rounding: read its column maxima
+            _lib.call("dlp_rho_cols", self.n, None, self.C1, ptr(self.R64), self.ld, _lib.F64, ptr(self.diag64),
+                      ptr(self._rho_dev), stream())
+            self._rho[:] = self._rho_dev.cpu().numpy()
+            return s
+        s = self.status()
+        ok = False
+        if not force_global and s["viol"] <= min(self.push_frac * self.n, self.push_max_rows):
+            r, ok = self._push(budget=int(self.push_budget * max(self.nnz, 1)))
+            st.iters += r
+        if not ok:
+            st.path = "push+pcg" if not force_global else "pcg"
+            it, _ = self._pcg()
+            st.extra["pcg_iters"] = st.extra.get("pcg_iters", 0) + it
+            _lib.call("dlp_lp_refresh", self.h, 2, 0, stream())     # FP64 residual of every row
+            s = self.status()
+            if s["viol"]:
+                r, ok = self._push()
+                st.iters += r
         for it in range(8):
             _lib.call("dlp_lp_refresh", self.h, 1, 0, stream())
             s = self.status()
             if s["viol"] == 0:
                 return s
             st.refine += 1
-            st.iters += self._push()
+            r, _ = self._push()
+            st.iters += r
         return self.status()
 
     def _upload(self, b: EdgeBatch):
@@ -198,26 +237,41 @@ class EdgeStreamLP:
 
     # -- public ---------------------------------------------------------------------------------
     def initial_solve(self):
-        """Ground, refresh every row and push to eps from F = prior, h = 1."""
+        """Ground, refresh every row, solve from F = prior, h = 1: global PCG then push (mixed
+        path), or the FP64 push (FP64 path); finishes with an FP64 residual of every row."""
         st = Stats(path="initial")
         t0 = time.perf_counter()
         _lib.call("dlp_lp_begin_batch", self.h, stream())
         _lib.call("dlp_lp_ground", self.h, stream())
         prec = 1 if self.fp64 else 0
         _lib.call("dlp_lp_refresh", self.h, 2, prec, stream())
-        s = self._repair(st)
+        s = self._repair(st, force_global=not self.fp64)
         if not self.fp64:
-            _lib.call("dlp_lp_refresh", self.h, 2, 0, stream())
-            s = self.status()
-            tries = 0
-            while s["viol"]:
-                tries += 1
-                if tries > 20:
-                    raise RuntimeError(f"initial solve: {s['viol']} rows still violate after refinement")
-                st.iters += self._push()
+            for tries in range(21):
                 _lib.call("dlp_lp_refresh", self.h, 2, 0, stream())
                 s = self.status()
+                if not s["viol"]:
+                    break
+                if tries == 20:
+                    raise RuntimeError(f"initial solve: {s['viol']} rows still violate after refinement")
+                st.iters += self._push()[0]
         st.ms = (time.perf_counter() - t0) * 1e3
+        self._finish(st, s)
+        return st
+
+    def load_state(self, Z):
+        """Start from a given [F, h] block (host or device, n x C1) and re-solve to eps: the FP64
+        residual of every row, then the repair.  Used to hand one path's solution to another."""
+        Zt = torch.as_tensor(Z).to(device())
+        if self.fp64:
+            self.Z64.copy_(Zt.double())
+        else:
+            self.Z32.copy_(Zt.float())
+        st = Stats(path="load")
+        _lib.call("dlp_lp_begin_batch", self.h, stream())
+        _lib.call("dlp_lp_ground", self.h, stream())
+        _lib.call("dlp_lp_refresh", self.h, 2, 1 if self.fp64 else 0, stream())
+        s = self._repair(st)
         self._finish(st, s)
         return st
 
@@ -249,7 +303,26 @@ class EdgeStreamLP:
 
     def _finish(self, st, s):
         st.edges, st.touched, st.changed, st.toggled = s["edges"], s["touched"], s["changed"], s["toggled"]
-        st.rho = float(self._rho[: self.C].max()) if self.C else 0.0
+        st.rho = self.rho()
+
+    def rho(self):
+        """max over label columns and all rows of |R_uc|/diag_u of the kept residual (every row
+        of it was last written by an FP64 refresh or, on the FP64 path, kept in FP64)."""
+        R = self.R64 if self.fp64 else self.R32
+        if not hasattr(self, "_rho_dev"):
+            self._rho_dev = torch.zeros(self.C1, dtype=torch.float64, device=device())
+        _lib.call("dlp_rho_cols", self.n, None, self.C1, ptr(R), self.ld, _lib.F64 if self.fp64 else _lib.F32,
+                  ptr(self.diag64), ptr(self._rho_dev), stream())
+        return float(self._rho_dev[: self.C].max().item())
+
+    def time_operator(self, reps=20):
+        """Average ms of the global solve's A*P launch (HIP events on the current stream)."""
+        ms = ct.c_double(0.0)
+        _lib.call("dlp_lp_time_operator", self.h, int(reps), ct.byref(ms), stream())
+        return float(ms.value)
+
+    def stored_entries(self):
+        return int(self.rowlen.sum().item())
 
     def full_check(self):
         """FP64 residual of every row: (rho per column, violators) — the periodic full refresh

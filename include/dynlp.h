@@ -194,12 +194,28 @@ int dlp_lp_ground(void* handle, void* stream);
  * Violators (|R_c| > eps_c*diag) are listed for the next push; column maxima of |R|/diag go to
  * the status.  Replaces solvers.py:404-407 (refresh) and :370-373 (full residual). */
 int dlp_lp_refresh(void* handle, int which, int prec, void* stream);
-/* Round-synchronous FP32 push from the current violators until no vertex violates or
- * max_rounds; rounds are replayed from a captured hipGraph of rounds_per_launch rounds (even)
- * when use_graph != 0.  Synchronises once per launch.  Replaces FusedPush.run
- * (kernels.py:415-433) as the service's repair step. */
+/* Asynchronous FP32 push (chaotic relaxation, hardware float atomics) from the current
+ * violators until no vertex violates or max_rounds; one kernel per round (the violation test
+ * of round t+1 is fused with its pushes), replayed from a captured hipGraph of
+ * rounds_per_launch rounds (even) when use_graph != 0.  Synchronises once per launch.  Stops
+ * early once this batch's scattered entries exceed budget_edges (> 0), the push budget of
+ * solvers.py:283 (DynLP+ then falls back to the global solve).  *converged_out = 1 when no
+ * vertex violates.  Replaces FusedPush.run (kernels.py:415-433) as the service's repair step. */
 int dlp_lp_push(void* handle, int max_rounds, int rounds_per_launch, int use_graph,
-                int64_t* rounds_out, void* stream);
+                int64_t budget_edges, int64_t* rounds_out, int32_t* converged_out, void* stream);
+/* Global solve: column-wise PCG (Jacobi preconditioner, FP32 vectors, FP64 column dots) on
+ * the FP32 state from the kept residual, until max |R|/(eps*diag) <= 1 in the recurrence,
+ * a stall of `patience` launches, or max_iter; iterations are replayed from a captured
+ * hipGraph of iters_per_launch when use_graph != 0.  Synchronises once per launch.  The
+ * caller verifies with an FP64 refresh.  Replaces solvers.pcg (solvers.py:66-97) as used by
+ * DynLPPlus._global (solvers.py:268-270). */
+int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int use_graph, int patience,
+               int64_t* iters_out, double* viol_out, void* stream);
+/* Cumulative global-solve timing: out[0] iterations, out[1] microseconds (HIP events). */
+int dlp_lp_pcg_stats(void* handle, int64_t* out);
+/* Average duration in ms (HIP events on `stream`, synchronises) of the global solve's operator
+ * launch AP = diag*P - W_UU P on the live state, over reps launches (after dlp_lp_pcg). */
+int dlp_lp_time_operator(void* handle, int reps, double* ms_out, void* stream);
 /* Plain FP64 path, one host-driven round of solvers.push (solvers.py:108-116): candidates
  * take D = R/diag, Z += D, R = 0, scatter W D with FP64 atomics and mark N(cand) and cand. */
 int dlp_lp_push64_round(void* handle, int nc, const int32_t* cand, void* stream);
@@ -210,7 +226,7 @@ int dlp_lp_violators(void* handle, int32_t** list, int32_t** count);
 /* Counters (host; synchronises): [0] changed rows, [1] missing deletions, [2] overflowed
  * insertions, [3] toggled vertices, [4] violators, [5] FP64-path candidates, [6] rows touched
  * by pushes, [7] round number, [8] edge visits of this batch, cumulative push work [9]
- * candidates, [10] pushes, [11] frontier entries gathered, [12] entries scanned by pushes,
+ * candidates, [10] pushes, [11] entries scattered by pushes, [12] reserved,
  * [13] push rounds launched, [14] their summed duration in microseconds (HIP events);
  * rho (C1, may be NULL): column maxima of the last refresh.  `out` holds 16 entries. */
 int dlp_lp_status(void* handle, int64_t* out, double* rho, void* stream);

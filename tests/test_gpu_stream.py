@@ -97,8 +97,9 @@ def test_overflow_regrow_keeps_results():
     assert int(lp.indptr[-1]) > cap0
 
 
-def test_service_result_is_deterministic():
-    """The round-synchronous push has no atomics on values: two runs are bit-identical."""
+def test_repeat_runs_agree_within_certificate():
+    """The asynchronous push is order-dependent (float atomics): two runs may differ in the
+    last bits, but both are certified, so they agree within twice the certified bound."""
     from dynlp_amd.stream import EdgeStreamLP
     g = graphs.sbm(3000, 8, 10, 0.8, 8, 0.02, seed=4)
     batches, _ = graphs.update_stream(g, 2, 300, seed=4)
@@ -109,4 +110,10 @@ def test_service_result_is_deterministic():
         for b in batches:
             lp.apply_and_solve(b)
         outs.append(lp.scores_with_h())
-    assert np.array_equal(outs[0], outs[1])
+    es = edge_dict(g)
+    for b in batches:
+        O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
+                           zip(b.del_u.tolist(), b.del_v.tolist()))
+    sysm, _ = oracle_system(g, es)
+    hmax = O.solve_system(sysm)[:, sysm.C].max()
+    assert np.abs(outs[0] - outs[1])[:, : sysm.C].max() <= 2 * 1e-6 * hmax * 1.05

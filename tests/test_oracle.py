@@ -247,3 +247,17 @@ def test_planted_sign_error_is_caught():
     Dfull[F] = D
     R -= sysm.W @ Dfull          # wrong sign
     assert np.abs(R - (sysm.rhs - O.apply_A(sysm.W, sysm.diag, Z))).max() > 1e-3
+
+
+def test_cpu_incremental_port_certifies_each_batch():
+    from oracle.incremental import CPUIncremental
+    g = graphs.sbm(1500, 6, 8, 0.8, 6, 0.03, seed=12)
+    cpu = CPUIncremental(g.n, g.u, g.v, g.w, g.labeled, g.y, g.K, eps=1e-6, push_max_rows=50)
+    batches, es = graphs.update_stream(g, 3, 200, seed=12)
+    for b in batches:
+        out = cpu.solve_batch(b.ins_u, b.ins_v, b.ins_w, b.del_u, b.del_v)
+        assert out["rho"] <= 1e-6
+    u, v, w = es.pairs()
+    sysm = O.build_system(g.n, u, v, w, g.labeled, g.y, g.K)
+    R = O.residual(sysm, cpu.Z)
+    assert O.rho_cols(sysm.diag, R)[: sysm.C].max() <= 1e-6 * 1.0001
