@@ -57,6 +57,7 @@ _SIGS = {
     "dlp_lp_pcg": (ct.c_int, [P, I32, I32, I32, I32, P, P, P]),
     "dlp_lp_pcg_stats": (ct.c_int, [P, P]),
     "dlp_lp_time_operator": (ct.c_int, [P, I32, P, P]),
+    "dlp_tune_set": (ct.c_int, [I32, I32]),
 }
 
 

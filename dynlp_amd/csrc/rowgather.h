@@ -27,7 +27,7 @@ struct Lanes {
 
 // acc[k] += sum over entries j in [beg, end) with mask[col_j] != 0 of w_j * X[col_j, l + k*LC].
 // All lanes of the group must call this with the same beg/end.
-template <typename TW, typename TX, typename TA, int CPL>
+template <typename TW, typename TX, typename TA, int CPL, int U = 4>
 __device__ __forceinline__ void row_gather(const Lanes& L, int beg, int end,
                                            const int32_t* __restrict__ indices,
                                            const TW* __restrict__ w,
@@ -44,19 +44,19 @@ __device__ __forceinline__ void row_gather(const Lanes& L, int beg, int end,
       if (mask != nullptr && mask[myv] == 0) myw = 0;
     }
     const int cnt = min(L.G, end - base);
-    for (int t0 = 0; t0 < cnt; t0 += 4 * L.LE) {
-      int vv[4];
-      TA ww[4];
+    for (int t0 = 0; t0 < cnt; t0 += U * L.LE) {
+      int vv[U];
+      TA ww[U];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < U; ++u) {
         const int t = t0 + u * L.LE + L.e;
         vv[u] = __shfl(myv, t & (L.G - 1), L.G);
         ww[u] = __shfl(myw, t & (L.G - 1), L.G);
         if (t >= cnt) ww[u] = 0;
       }
-      TA xv[4][CPL];
+      TA xv[U][CPL];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < U; ++u) {
         const TX* xr = X + (int64_t)vv[u] * ldx;
 #pragma unroll
         for (int k = 0; k < CPL; ++k) {
@@ -65,12 +65,67 @@ __device__ __forceinline__ void row_gather(const Lanes& L, int beg, int end,
         }
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+      for (int u = 0; u < U; ++u)
 #pragma unroll
         for (int k = 0; k < CPL; ++k) acc[k] += ww[u] * xv[u][k];
     }
   }
 }
+
+// FP32 gather with 16-byte loads: lane l of the column lanes covers columns 4l..4l+3 (X rows
+// 16-byte aligned: ldx % 4 == 0), so a 65-column row (ld 68) is 17 lanes of one instruction.
+template <int U>
+__device__ __forceinline__ void row_gather_v4(const Lanes& L, int beg, int end,
+                                              const int32_t* __restrict__ indices,
+                                              const float* __restrict__ w,
+                                              const uint8_t* __restrict__ mask,
+                                              const float* __restrict__ X, int64_t ldx, int C,
+                                              float4& acc) {
+  const bool lane_on = 4 * L.l < C;
+  for (int base = beg; base < end; base += L.G) {
+    const int j = base + L.lane;
+    int myv = 0;
+    float myw = 0.f;
+    if (j < end) {
+      myv = indices[j];
+      myw = w[j];
+      if (mask != nullptr && mask[myv] == 0) myw = 0.f;
+    }
+    const int cnt = min(L.G, end - base);
+    for (int t0 = 0; t0 < cnt; t0 += U * L.LE) {
+      int vv[U];
+      float ww[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int t = t0 + u * L.LE + L.e;
+        vv[u] = __shfl(myv, t & (L.G - 1), L.G);
+        ww[u] = __shfl(myw, t & (L.G - 1), L.G);
+        if (t >= cnt) ww[u] = 0.f;
+      }
+      float4 xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const float4* xr = reinterpret_cast<const float4*>(X + (int64_t)vv[u] * ldx) + L.l;
+        xv[u] = (ww[u] != 0.f && lane_on) ? *xr : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        acc.x += ww[u] * xv[u].x;
+        acc.y += ww[u] * xv[u].y;
+        acc.z += ww[u] * xv[u].z;
+        acc.w += ww[u] * xv[u].w;
+      }
+    }
+  }
+}
+
+// Process-wide tuning knobs (dlp_tune_set): unroll factor of the scalar gather (4 or 8) and
+// whether FP32 operators may use 16-byte loads when the layout allows it.
+struct Tuning {
+  int unroll = 8;
+  int vec4 = 1;
+};
+Tuning& tuning();
 
 // Sum acc over the edge lanes of the group (xor butterflies inside the group).
 template <typename TA, int CPL>

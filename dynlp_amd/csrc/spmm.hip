@@ -1,10 +1,11 @@
 // Row-major SpMM with fused axpy, the masked row operator, and residual reductions.
 #include "rowgather.h"
+#include <type_traits>
 
 namespace dlp {
 
 // out[o] = add[r] + a[r]*X[r] + b * sum_{j, mask[col_j]} w_j X[col_j]   (o = i or r)
-template <typename TW, typename TX, typename TO, int CPL>
+template <typename TW, typename TX, typename TO, int CPL, int U>
 __global__ __launch_bounds__(256) void k_row_operator(
     int nrows, const int32_t* __restrict__ rows, int C, int g_shift, int lc_shift,
     const int32_t* __restrict__ indptr, const int32_t* __restrict__ rowlen,
@@ -25,7 +26,7 @@ __global__ __launch_bounds__(256) void k_row_operator(
   TO acc[CPL];
 #pragma unroll
   for (int k = 0; k < CPL; ++k) acc[k] = 0;
-  row_gather<TW, TX, TO, CPL>(L, beg, end, indices, w, mask, X, ldx, C, acc);
+  row_gather<TW, TX, TO, CPL, U>(L, beg, end, indices, w, mask, X, ldx, C, acc);
   group_reduce<TO, CPL>(L, acc);
   if (valid && L.e == 0) {
     const TO ar = a ? a[r] : (TO)0;
@@ -43,17 +44,95 @@ __global__ __launch_bounds__(256) void k_row_operator(
   }
 }
 
+// FP32 operator with 16-byte loads (see row_gather_v4); LC lanes cover ceil(C/4) float4s.
+template <int U>
+__global__ __launch_bounds__(256) void k_row_operator_v4(
+    int nrows, const int32_t* __restrict__ rows, int C, int g_shift, int lc_shift,
+    const int32_t* __restrict__ indptr, const int32_t* __restrict__ rowlen,
+    const int32_t* __restrict__ indices, const float* __restrict__ w,
+    const uint8_t* __restrict__ mask, const float* __restrict__ X, int64_t ldx,
+    const float* __restrict__ a, float b, const float* __restrict__ add, int64_t ldadd,
+    float* __restrict__ out, int64_t ldo, int compact_out) {
+  const Lanes L(g_shift, lc_shift);
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> g_shift;
+  const bool valid = i < nrows;
+  const int r = valid ? (rows ? rows[i] : (int)i) : 0;
+  const bool live = valid && (mask == nullptr || mask[r] != 0);
+  int beg = 0, end = 0;
+  if (live) {
+    beg = indptr[r];
+    end = rowlen ? beg + rowlen[r] : indptr[r + 1];
+  }
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  row_gather_v4<U>(L, beg, end, indices, w, mask, X, ldx, C, acc);
+  for (int off = L.LC; off < L.G; off <<= 1) {
+    acc.x += __shfl_xor(acc.x, off, 64);
+    acc.y += __shfl_xor(acc.y, off, 64);
+    acc.z += __shfl_xor(acc.z, off, 64);
+    acc.w += __shfl_xor(acc.w, off, 64);
+  }
+  if (valid && L.e == 0 && 4 * L.l < C) {
+    const float ar = a ? a[r] : 0.f;
+    const int64_t o = compact_out ? i : (int64_t)r;
+    const float accs[4] = {acc.x, acc.y, acc.z, acc.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = 4 * L.l + q;
+      if (c < C) {
+        float y = b * accs[q];
+        if (a) y += ar * X[(int64_t)r * ldx + c];
+        if (add) y += add[(int64_t)r * ldadd + c];
+        out[o * ldo + c] = y;
+      }
+    }
+  }
+}
+
+Tuning& tuning() {
+  static Tuning t;
+  return t;
+}
+
 template <typename TW, typename TX, typename TO>
 static int launch_row_operator(int nrows, const int32_t* rows, int C, const int32_t* indptr,
                                const int32_t* rowlen, const int32_t* indices, const void* w,
                                const uint8_t* mask, const void* X, int64_t ldx, const void* a,
                                double b, const void* add, int64_t ldadd, void* out, int64_t ldo,
                                int compact_out, double avg_deg, hipStream_t st) {
+  const bool v4 = std::is_same<TW, float>::value && std::is_same<TX, float>::value &&
+                  std::is_same<TO, float>::value && tuning().vec4 && (ldx % 4) == 0 &&
+                  (((uintptr_t)X) & 15) == 0 && C <= 256;
+  if (v4) {
+    // column lanes: next power of two >= ceil(C/4); edge lanes fill the wavefront
+    int lc = ceil_log2((C + 3) / 4);
+    if (lc > 6) lc = 6;
+    RowMap m4;
+    m4.lc_shift = lc;
+    m4.g_shift = 6;
+    m4.cpl = 1;
+    const long long threads = (long long)nrows << m4.g_shift;
+    dim3 grid(grid_for(threads)), block(kBlock);
+    if (tuning().unroll >= 8)
+      hipLaunchKernelGGL((k_row_operator_v4<8>), grid, block, 0, st, nrows, rows, C, m4.g_shift, m4.lc_shift,
+                         indptr, rowlen, indices, (const float*)w, mask, (const float*)X, ldx, (const float*)a,
+                         (float)b, (const float*)add, ldadd, (float*)out, ldo, compact_out);
+    else
+      hipLaunchKernelGGL((k_row_operator_v4<4>), grid, block, 0, st, nrows, rows, C, m4.g_shift, m4.lc_shift,
+                         indptr, rowlen, indices, (const float*)w, mask, (const float*)X, ldx, (const float*)a,
+                         (float)b, (const float*)add, ldadd, (float*)out, ldo, compact_out);
+    DLP_CHECK_LAUNCH();
+    return DLP_OK;
+  }
   RowMap m = choose_map(C, avg_deg);
   const long long threads = (long long)nrows << m.g_shift;
   dim3 grid(grid_for(threads)), block(kBlock);
+  const bool u8 = tuning().unroll >= 8;
 #define DLP_ROWOP(CPLV)                                                                        \
-  hipLaunchKernelGGL((k_row_operator<TW, TX, TO, CPLV>), grid, block, 0, st, nrows, rows, C,   \
+  if (u8) hipLaunchKernelGGL((k_row_operator<TW, TX, TO, CPLV, 8>), grid, block, 0, st, nrows, rows, C, \
+                     m.g_shift, m.lc_shift, indptr, rowlen, indices, (const TW*)w, mask,       \
+                     (const TX*)X, ldx, (const TO*)a, (TO)b, (const TO*)add, ldadd, (TO*)out,  \
+                     ldo, compact_out);                                                        \
+  else hipLaunchKernelGGL((k_row_operator<TW, TX, TO, CPLV, 4>), grid, block, 0, st, nrows, rows, C,   \
                      m.g_shift, m.lc_shift, indptr, rowlen, indices, (const TW*)w, mask,       \
                      (const TX*)X, ldx, (const TO*)a, (TO)b, (const TO*)add, ldadd, (TO*)out,  \
                      ldo, compact_out)
@@ -202,5 +281,18 @@ extern "C" int dlp_violating_rows(int nrows, const int32_t* rows, int C, const v
     hipLaunchKernelGGL(k_violating<double>, dim3(grid_for(threads)), dim3(kBlock), 0, st, nrows,
                        rows, C, ls, (const double*)R, ldr, diag, eps, out, count);
   DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+extern "C" int dlp_tune_set(int key, int value) {
+  // key 0: unroll factor of the gather loops (4 or 8); key 1: 16-byte FP32 loads on/off
+  if (key == 0) {
+    DLP_CHECK_ARG(value == 4 || value == 8, "tune: unroll must be 4 or 8");
+    tuning().unroll = value;
+  } else if (key == 1) {
+    tuning().vec4 = value ? 1 : 0;
+  } else {
+    DLP_CHECK_ARG(false, "tune: unknown key %d", key);
+  }
   return DLP_OK;
 }

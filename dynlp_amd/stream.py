@@ -72,7 +72,7 @@ class EdgeStreamLP:
 
     def __init__(self, g: Graph, eps=1e-6, eps_h=0.05, fp64=False, eta=0.0, slack_min=4,
                  slack_frac=0.25, push_scale=0.9, rounds_per_launch=4, use_graph=True, max_rounds=200_000,
-                 push_frac=0.05, push_max_rows=4096, push_budget=1.0, pcg_iters_per_launch=8,
+                 push_frac=0.05, push_max_rows=4096, push_budget=1.0, pcg_iters_per_launch=4,
                  pcg_max_iter=20_000):
         self.g, self.n, self.K = g, g.n, g.K
         self.C = 1 if g.K == 2 else g.K
@@ -94,18 +94,19 @@ class EdgeStreamLP:
         self.y = t(g.y, torch.int32)
         self.solved = torch.zeros(self.n, dtype=torch.uint8, device=dev)
         self.labcnt = t(labcnt, torch.int32)
-        self.ld = self.C1
+        self.ld = (self.C1 + 3) // 4 * 4    # 16-byte aligned rows: 128-bit neighbour-row gathers
         self.prior = 0.5 if self.K == 2 else 1.0 / self.K
-        Z = torch.full((self.n, self.C1), self.prior, dtype=torch.float32, device=dev)
+        Z = torch.zeros((self.n, self.ld), dtype=torch.float32, device=dev)
+        Z[:, : self.C] = self.prior
         Z[:, self.C] = 1.0
         self.Z32 = Z
-        self.R32 = torch.zeros((self.n, self.C1), dtype=torch.float32, device=dev)
+        self.R32 = torch.zeros((self.n, self.ld), dtype=torch.float32, device=dev)
         self.diag64 = torch.ones(self.n, dtype=torch.float64, device=dev)
         self.diag32 = torch.ones(self.n, dtype=torch.float32, device=dev)
         self.Z64 = self.R64 = None
         if self.fp64:
             self.Z64 = Z.double()
-            self.R64 = torch.zeros((self.n, self.C1), dtype=torch.float64, device=dev)
+            self.R64 = torch.zeros((self.n, self.ld), dtype=torch.float64, device=dev)
             self.cand = torch.zeros(self.n, dtype=torch.int32, device=dev)
             self.cand_n = torch.zeros(1, dtype=torch.int32, device=dev)
             self._rho_dev = torch.zeros(self.C1, dtype=torch.float64, device=dev)
@@ -262,11 +263,11 @@ class EdgeStreamLP:
     def load_state(self, Z):
         """Start from a given [F, h] block (host or device, n x C1) and re-solve to eps: the FP64
         residual of every row, then the repair.  Used to hand one path's solution to another."""
-        Zt = torch.as_tensor(Z).to(device())
+        Zt = torch.as_tensor(Z).to(device())[:, : self.C1]
         if self.fp64:
-            self.Z64.copy_(Zt.double())
+            self.Z64[:, : self.C1].copy_(Zt.double())
         else:
-            self.Z32.copy_(Zt.float())
+            self.Z32[:, : self.C1].copy_(Zt.float())
         st = Stats(path="load")
         _lib.call("dlp_lp_begin_batch", self.h, stream())
         _lib.call("dlp_lp_ground", self.h, stream())

@@ -59,6 +59,11 @@ int dlp_row_operator(int nrows, const int32_t* rows, int C, const int32_t* indpt
                      const void* a, double b, const void* add, int64_t ldadd, void* out,
                      int64_t ldo, int out_dtype, int compact_out, double avg_deg, void* stream);
 
+/* Process-wide tuning of the gather loops: key 0 = unroll factor (4 or 8, default 8); key 1 =
+ * 16-byte FP32 loads when the row layout allows them (1 default, 0 off).  No reference
+ * counterpart (the reference templated lanes per row at JIT time, kernels.py:365-374). */
+int dlp_tune_set(int key, int value);
+
 /* ---- Column residual ratios ------------------------------------------------------------
  * rho[c] = max_{listed rows r} |R[r,c]| / diag[r]  (FP64 out[C], overwritten).  R dtype
  * r_dtype, diag FP64.  Replaces solvers.rho_cols (solvers.py:52-54). */
