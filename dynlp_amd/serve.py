@@ -1,0 +1,254 @@
+"""Continuous-batching update service over many device-resident graphs.
+
+No reference counterpart (the reference times one solver per batch, bench.py:93-118); this is
+the serving layer the workload asks for:
+
+- Residency and admission.  Every graph lives on the device as an `EdgeStreamLP` (slack CSR,
+  FP32 scores and kept residual, FP64 diag).  A graph is admitted when its estimated footprint
+  fits the HBM budget; otherwise it waits, and idle graphs are evicted least-recently-used to
+  make room (an evicted graph keeps its host-side edge set and is re-admitted with a fresh
+  solve).
+- Continuous batching.  Requests (edge-update batches for one graph) queue per graph.  A worker
+  takes *all* requests queued for a graph at that moment and fuses them into one batch (per
+  pair, the last operation wins, which is what applying them in arrival order gives), so
+  requests that arrive while the graph is busy are served by its next repair.
+- Overlap.  `slots` workers run concurrently, each on its own HIP stream (default priority);
+  the C ABI releases the GIL, so repairs of different graphs overlap on the device.
+- Latency is arrival -> certified result; throughput counts edge updates.
+"""
+from __future__ import annotations
+
+import threading
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .graphs import EdgeBatch, Graph
+from .stream import EdgeStreamLP
+
+
+@dataclass
+class Request:
+    gid: int
+    batch: EdgeBatch
+    arrival: float                 # seconds (service clock)
+    done: float = float("nan")
+    rho: float = float("nan")
+    fused_with: int = 1
+
+    @property
+    def latency_ms(self):
+        return (self.done - self.arrival) * 1e3
+
+
+def fuse(batches):
+    """One batch equivalent to applying `batches` in order: per undirected pair the last
+    operation wins (deletion, or insertion with its weight)."""
+    if len(batches) == 1:
+        return batches[0]
+    keys, ops, ws = [], [], []
+    n = max(int(max(b.ins_u.max(initial=0), b.ins_v.max(initial=0), b.del_u.max(initial=0),
+                    b.del_v.max(initial=0))) for b in batches) + 1
+    for t, b in enumerate(batches):
+        lo, hi = np.minimum(b.del_u, b.del_v), np.maximum(b.del_u, b.del_v)
+        keys.append(lo.astype(np.int64) * n + hi)
+        ops.append(np.zeros(lo.shape[0], dtype=np.int8))
+        ws.append(np.zeros(lo.shape[0], dtype=np.float32))
+        lo, hi = np.minimum(b.ins_u, b.ins_v), np.maximum(b.ins_u, b.ins_v)
+        keys.append(lo.astype(np.int64) * n + hi)
+        ops.append(np.ones(lo.shape[0], dtype=np.int8))
+        ws.append(b.ins_w.astype(np.float32))
+    k, o, w = np.concatenate(keys), np.concatenate(ops), np.concatenate(ws)
+    # last occurrence per key: reverse, unique first-index, map back
+    rk = k[::-1]
+    _, first = np.unique(rk, return_index=True)
+    last = k.shape[0] - 1 - first
+    k, o, w = k[last], o[last], w[last]
+    ins, dele = o == 1, o == 0
+    return EdgeBatch(k[ins] // n, k[ins] % n, w[ins], k[dele] // n, k[dele] % n)
+
+
+def footprint_bytes(g: Graph, fp64=False):
+    """Device bytes of one resident graph (store with slack, state, PCG buffers, scratch)."""
+    n, C1 = g.n, (1 if g.K == 2 else g.K) + 1
+    ld = (C1 + 3) // 4 * 4
+    entries = 2 * g.m
+    cap = int(entries * 1.25) + 4 * n
+    store = cap * 8 + n * (4 + 4 + 1 + 1 + 4 + 4)
+    state = n * ld * 4 * 4 + n * 12          # Z, R, P, AP (FP32) + diag
+    if fp64:
+        state += n * ld * 8 * 3
+    scratch = n * 4 * 14 + (1 << 22)
+    return store + state + scratch
+
+
+@dataclass
+class Slot:
+    gid: int
+    graph: Graph
+    lp: EdgeStreamLP | None = None
+    queue: deque = field(default_factory=deque)
+    busy: bool = False
+    last_used: float = 0.0
+    bytes: int = 0
+    served: int = 0
+    solves: int = 0
+
+
+class Service:
+    """Request loop over resident graphs (see the module docstring)."""
+
+    def __init__(self, graphs, hbm_budget_bytes, slots=4, eps=1e-6, max_fuse=64, lp_kwargs=None):
+        self.slots_n = int(slots)
+        self.budget = int(hbm_budget_bytes)
+        self.eps = eps
+        self.max_fuse = max_fuse
+        self.lp_kwargs = dict(lp_kwargs or {})
+        self.graphs = {i: Slot(i, g) for i, g in enumerate(graphs)}
+        for s in self.graphs.values():
+            s.bytes = footprint_bytes(s.graph)
+            if s.bytes > self.budget:
+                raise ValueError(f"graph {s.gid} needs {s.bytes / 1e9:.2f} GB > budget {self.budget / 1e9:.2f} GB")
+        self.resident = 0
+        self.lock = threading.Lock()
+        self.cv = threading.Condition(self.lock)
+        self.done = []
+        self.stats = {"admissions": 0, "evictions": 0, "fused_requests": 0, "repairs": 0}
+        self._t0 = None
+        self._stop = False
+
+    # -- admission ------------------------------------------------------------------------
+    def _admit(self, s: Slot):
+        """Called with the lock held; returns True when s is resident (may evict idle graphs)."""
+        if s.lp is not None:
+            return True
+        while self.resident + s.bytes > self.budget:
+            idle = [x for x in self.graphs.values() if x.lp is not None and not x.busy and not x.queue]
+            if not idle:
+                return False
+            victim = min(idle, key=lambda x: x.last_used)
+            victim.lp = None        # frees its device buffers (handle destroyed with the object)
+            self.resident -= victim.bytes
+            self.stats["evictions"] += 1
+        self.resident += s.bytes
+        s.busy = True               # reserved; the worker builds the state outside the lock
+        self.stats["admissions"] += 1
+        return "build"
+
+    # -- worker ----------------------------------------------------------------------------
+    def _worker(self, wid):
+        stream = torch.cuda.Stream()
+        with torch.cuda.stream(stream):
+            while True:
+                with self.cv:
+                    job = None
+                    while job is None:
+                        if self._stop and not any(x.queue for x in self.graphs.values()):
+                            return
+                        cands = [x for x in self.graphs.values() if x.queue and not x.busy]
+                        cands.sort(key=lambda x: x.queue[0].arrival)
+                        for s in cands:
+                            a = self._admit(s)
+                            if a:
+                                job = (s, a == "build")
+                                break
+                        if job is None:
+                            self.cv.wait(timeout=0.05)
+                    s, build = job
+                    s.busy = True
+                    reqs = [s.queue.popleft() for _ in range(min(len(s.queue), self.max_fuse))]
+                if build:
+                    s.lp = EdgeStreamLP(s.graph, eps=self.eps, **self.lp_kwargs)
+                    s.lp.initial_solve()
+                    s.solves += 1
+                    # an evicted graph re-enters with the updates it had already served
+                    if getattr(s, "history", None):
+                        s.lp.apply_and_solve(fuse(s.history))
+                b = fuse([r.batch for r in reqs])
+                st = s.lp.apply_and_solve(b)
+                stream.synchronize()
+                now = time.perf_counter() - self._t0
+                with self.cv:
+                    for r in reqs:
+                        r.done, r.rho, r.fused_with = now, st.rho, len(reqs)
+                    s.history = getattr(s, "history", []) + [b]
+                    self.done.extend(reqs)
+                    s.served += len(reqs)
+                    s.busy = False
+                    s.last_used = now
+                    self.stats["fused_requests"] += len(reqs)
+                    self.stats["repairs"] += 1
+                    self.cv.notify_all()
+
+    # -- driver ----------------------------------------------------------------------------
+    def run(self, requests, preload=True):
+        """Serve `requests` (list of Request with arrival times in seconds from start, sorted)
+        open-loop; returns the completed requests.  preload: admit and solve every graph that
+        fits before the clock starts."""
+        if preload:
+            for s in sorted(self.graphs.values(), key=lambda x: x.bytes):
+                if self.resident + s.bytes <= self.budget:
+                    s.lp = EdgeStreamLP(s.graph, eps=self.eps, **self.lp_kwargs)
+                    s.lp.initial_solve()
+                    s.solves += 1
+                    self.resident += s.bytes
+                    self.stats["admissions"] += 1
+        torch.cuda.synchronize()
+        self._t0 = time.perf_counter()
+        self._stop = False
+        workers = [threading.Thread(target=self._worker, args=(i,), daemon=True) for i in range(self.slots_n)]
+        for w in workers:
+            w.start()
+        pending = sorted(requests, key=lambda r: r.arrival)
+        i = 0
+        while i < len(pending):
+            now = time.perf_counter() - self._t0
+            r = pending[i]
+            if r.arrival > now:
+                time.sleep(min(r.arrival - now, 0.002))
+                continue
+            with self.cv:
+                while i < len(pending) and pending[i].arrival <= now:
+                    self.graphs[pending[i].gid].queue.append(pending[i])
+                    i += 1
+                self.cv.notify_all()
+        with self.cv:
+            self._stop = True
+            self.cv.notify_all()
+        for w in workers:
+            w.join()
+        self.wall = time.perf_counter() - self._t0
+        return self.done
+
+    def report(self):
+        lat = np.array([r.latency_ms for r in self.done])
+        edges = sum(r.batch.size for r in self.done)
+        return {"requests": len(self.done), "edge_updates": edges,
+                "edge_updates_per_s": edges / self.wall if self.wall > 0 else float("nan"),
+                "p50_ms": float(np.percentile(lat, 50)) if lat.size else float("nan"),
+                "p99_ms": float(np.percentile(lat, 99)) if lat.size else float("nan"),
+                "max_rho": max((r.rho for r in self.done), default=float("nan")),
+                "mean_fused": float(np.mean([r.fused_with for r in self.done])) if self.done else 0.0,
+                **self.stats}
+
+
+def poisson_requests(graphs, rate, duration, size_lo=100, size_hi=100_000, seed=0, max_requests=None):
+    """Open-loop arrivals at `rate` req/s over `duration` s (seed S+3): graph uniform, batch size
+    log-uniform in [size_lo, size_hi], 50/50 insert/delete, drawn from each graph's own edge set."""
+    from .graphs import EdgeSet, draw_batch
+    rng = np.random.default_rng(seed + 3)
+    sets = [EdgeSet(g) for g in graphs]
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration or (max_requests and len(out) >= max_requests):
+            break
+        gid = int(rng.integers(0, len(graphs)))
+        size = int(np.exp(rng.uniform(np.log(size_lo), np.log(size_hi))))
+        b = draw_batch(sets[gid], size, 0.5, rng, unit=bool(np.all(graphs[gid].w == 1)))
+        sets[gid].apply(b)
+        out.append(Request(gid, b, t))
+    return out

@@ -1,0 +1,82 @@
+"""Serving: each graph's result is the same alone and in the mix (within the certified bound),
+admission under an HBM budget with eviction, fused requests."""
+import numpy as np
+import pytest
+
+from dynlp_amd import graphs
+from oracle import lp as O
+
+pytestmark = pytest.mark.gpu
+
+EPS = 1e-6
+
+
+def _graphs():
+    return [graphs.sbm(2000, 4, 8, 0.8, 4, 0.03, seed=21),
+            graphs.erdos_renyi(5000, 10, 8, 0.02, seed=22, unit=True),
+            graphs.sbm(12000, 16, 12, 0.8, 16, 0.01, seed=23)]
+
+
+def _requests(gs, n=36, rate=4000.0):
+    from dynlp_amd.serve import poisson_requests
+    return poisson_requests(gs, rate=rate, duration=10.0, size_lo=20, size_hi=400, seed=5, max_requests=n)
+
+
+def _alone(g, reqs):
+    from dynlp_amd.stream import EdgeStreamLP
+    lp = EdgeStreamLP(g, eps=EPS)
+    lp.initial_solve()
+    for r in reqs:
+        st = lp.apply_and_solve(r.batch)
+        assert st.rho <= EPS * 1.0001
+    return lp.scores_with_h()
+
+
+def _oracle_hmax(g, reqs):
+    es = {(int(a), int(b)): float(w) for a, b, w in zip(g.u, g.v, g.w)}
+    for r in reqs:
+        b = r.batch
+        O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
+                           zip(b.del_u.tolist(), b.del_v.tolist()))
+    u = np.array([k[0] for k in es]); v = np.array([k[1] for k in es]); w = np.array(list(es.values()))
+    sysm = O.build_system(g.n, u, v, w, g.labeled, g.y, g.K)
+    Fs = O.solve_system(sysm)
+    return sysm, Fs
+
+
+@pytest.mark.parametrize("tight", [False, True])
+def test_alone_equals_mixed(tight):
+    from dynlp_amd.serve import Service, footprint_bytes
+    gs = _graphs()
+    reqs = _requests(gs)
+    sizes = [footprint_bytes(g) for g in gs]
+    budget = sum(sizes) + (1 << 20) if not tight else sizes[2] + max(sizes[0], sizes[1]) + (1 << 20)
+    svc = Service(gs, budget, slots=3, eps=EPS)
+    done = svc.run(reqs)
+    rep = svc.report()
+    assert rep["requests"] == len(reqs)
+    assert rep["max_rho"] <= EPS * 1.0001
+    if tight:
+        assert rep["evictions"] > 0
+    for gid, g in enumerate(gs):
+        mine = [r for r in reqs if r.gid == gid]
+        sysm, Fs = _oracle_hmax(g, mine)
+        hmax = Fs[:, sysm.C].max()
+        Zm = svc.graphs[gid].lp.scores_with_h() if svc.graphs[gid].lp is not None else None
+        Za = _alone(g, mine)
+        # both certified against the exact solution of the final edge set
+        assert np.abs(Za[:, : sysm.C] - Fs[:, : sysm.C]).max() <= EPS * hmax * 1.05
+        if Zm is not None:
+            assert np.abs(Zm[:, : sysm.C] - Fs[:, : sysm.C]).max() <= EPS * hmax * 1.05
+            assert np.abs(Zm - Za)[:, : sysm.C].max() <= 2 * EPS * hmax * 1.05
+
+
+def test_fuse_last_operation_wins():
+    from dynlp_amd.graphs import EdgeBatch
+    from dynlp_amd.serve import fuse
+    a = EdgeBatch(np.array([1, 2]), np.array([5, 6]), np.array([1.0, 2.0], np.float32), np.array([3]), np.array([4]))
+    b = EdgeBatch(np.array([4]), np.array([3]), np.array([7.0], np.float32), np.array([5]), np.array([1]))
+    f = fuse([a, b])
+    ins = {(min(x, y), max(x, y)): w for x, y, w in zip(f.ins_u, f.ins_v, f.ins_w)}
+    dels = {(min(x, y), max(x, y)) for x, y in zip(f.del_u, f.del_v)}
+    assert ins == {(2, 6): 2.0, (3, 4): 7.0} and dels == {(1, 5)}
