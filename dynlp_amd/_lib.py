@@ -58,6 +58,15 @@ _SIGS = {
     "dlp_lp_pcg_stats": (ct.c_int, [P, P]),
     "dlp_lp_time_operator": (ct.c_int, [P, I32, P, P]),
     "dlp_tune_set": (ct.c_int, [I32, I32]),
+    "dlp_coldot": (ct.c_int, [I32, I32, P, I64, P, I64, P, P, I32, P]),
+    "dlp_colop": (ct.c_int, [I32, I32, I32, DBL, P, P, I64, P, P, I64, I32, P]),
+    "dlp_amg_match": (ct.c_int, [I32, P, P, P, I32, I32, P, P, P, P, P, SZ, P, P]),
+    "dlp_amg_compose": (ct.c_int, [I32, P, P, P, P]),
+    "dlp_amg_coarsen_fill": (ct.c_int, [I32, I32, P, P, P, I32, P, P, P, P, P, P, SZ, P]),
+    "dlp_amg_coarsen_merge": (ct.c_int, [I32, P, P, P, I32, P, P, SZ, P, P]),
+    "dlp_amg_coarsen_compact": (ct.c_int, [I32, P, P, P, I32, P, P, P, P]),
+    "dlp_amg_restrict": (ct.c_int, [I32, I32, P, P, I64, P, I64, I32, P]),
+    "dlp_amg_prolong_add": (ct.c_int, [I32, I32, P, P, I64, P, I64, I32, P]),
 }
 
 

@@ -1,0 +1,312 @@
+// Aggregation AMG on the device (reference amg.py:1-134): heavy-edge handshake matching,
+// Galerkin coarsening by aggregates (W_c = off-diagonal part of P^T W P, s_c = P^T s) and the
+// restriction / prolongation of the V-cycle.
+//
+// Matching choices are made row-locally over CSR rows (a row is scanned by one thread), which
+// gives the reference's scatter_max/scatter_min result (heaviest weight, ties -> smallest
+// neighbour id, amg.py:38-44) without atomics.  Coarse rows are built by a counting sort on
+// the aggregate id followed by a per-row insertion sort and duplicate merge.
+#include "common.h"
+#include "scan.h"
+
+namespace dlp {
+
+// mode 0: entries with both ends unmatched (amg.py:47-50); mode 1: r unmatched, c matched
+// (amg.py:56-58).  choice[r] = best column or n.
+template <typename T>
+__global__ void k_best_choice(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+                              const T* __restrict__ w, const uint8_t* __restrict__ matched, int mode,
+                              int32_t* __restrict__ choice) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int best_c = n;
+  T best_w = 0;
+  const bool mr = matched[r] != 0;
+  if (!mr) {
+    for (int j = indptr[r]; j < indptr[r + 1]; ++j) {
+      const int c = indices[j];
+      const bool ok = mode == 0 ? !matched[c] : matched[c] != 0;
+      if (!ok) continue;
+      const T wv = w[j];
+      if (best_c == n || wv > best_w || (wv == best_w && c < best_c)) {
+        best_w = wv;
+        best_c = c;
+      }
+    }
+  }
+  choice[r] = best_c;
+}
+
+__global__ void k_mutual(int n, const int32_t* __restrict__ choice, uint8_t* __restrict__ matched,
+                         int32_t* __restrict__ agg) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int p = choice[r];
+  if (p < n && choice[p] == r) {
+    matched[r] = 1;
+    agg[r] = min(r, p);
+  }
+}
+
+__global__ void k_attach(int n, const int32_t* __restrict__ choice, int32_t* __restrict__ agg) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int p = choice[r];
+  if (p < n) agg[r] = agg[p];
+}
+
+__global__ void k_agg_init(int n, int32_t* __restrict__ agg, uint8_t* __restrict__ matched) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  agg[r] = -1;
+  matched[r] = 0;
+}
+
+__global__ void k_left_flag(int n, int32_t* __restrict__ agg, int32_t* __restrict__ flag) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int a = agg[r];
+  if (a < 0) {
+    a = r;
+    agg[r] = r;
+  }
+  flag[a] = 1;
+}
+
+__global__ void k_zero_i32(int n, int32_t* __restrict__ p) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) p[r] = 0;
+}
+
+__global__ void k_relabel(int n, int32_t* __restrict__ agg, const int32_t* __restrict__ rank) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) agg[r] = rank[agg[r]];
+}
+
+__global__ void k_compose(int n, const int32_t* __restrict__ prev, const int32_t* __restrict__ a2,
+                          int32_t* __restrict__ out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) out[r] = a2[prev[r]];
+}
+
+__global__ void k_coarse_count(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+                               const int32_t* __restrict__ agg, int32_t* __restrict__ cnt) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int ar = agg[r];
+  int k = 0;
+  for (int j = indptr[r]; j < indptr[r + 1]; ++j) k += agg[indices[j]] != ar;
+  if (k) atomicAdd(cnt + ar, k);
+}
+
+template <typename T>
+__global__ void k_coarse_fill(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+                              const T* __restrict__ w, const int32_t* __restrict__ agg,
+                              int32_t* __restrict__ cursor, int32_t* __restrict__ ci, T* __restrict__ cw) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int ar = agg[r];
+  for (int j = indptr[r]; j < indptr[r + 1]; ++j) {
+    const int ac = agg[indices[j]];
+    if (ac != ar) {
+      const int pos = atomicAdd(cursor + ar, 1);
+      ci[pos] = ac;
+      cw[pos] = w[j];
+    }
+  }
+}
+
+// Sort a coarse row's entries by column and merge duplicates (summing weights); ucnt = unique.
+template <typename T>
+__global__ void k_coarse_merge(int nc, const int32_t* __restrict__ off, int32_t* __restrict__ ci,
+                               T* __restrict__ cw, int32_t* __restrict__ ucnt) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nc) return;
+  const int beg = off[r], end = off[r + 1];
+  for (int i = beg + 1; i < end; ++i) {
+    const int key = ci[i];
+    const T val = cw[i];
+    int j = i - 1;
+    while (j >= beg && ci[j] > key) {
+      ci[j + 1] = ci[j];
+      cw[j + 1] = cw[j];
+      --j;
+    }
+    ci[j + 1] = key;
+    cw[j + 1] = val;
+  }
+  int o = beg;
+  for (int i = beg; i < end; ++i) {
+    if (o > beg && ci[o - 1] == ci[i]) {
+      cw[o - 1] += cw[i];
+    } else {
+      ci[o] = ci[i];
+      cw[o] = cw[i];
+      ++o;
+    }
+  }
+  ucnt[r] = o - beg;
+}
+
+template <typename T>
+__global__ void k_coarse_compact(int nc, const int32_t* __restrict__ off, const int32_t* __restrict__ ci,
+                                 const T* __restrict__ cw, const int32_t* __restrict__ optr,
+                                 int32_t* __restrict__ oi, T* __restrict__ ow) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nc) return;
+  const int b = off[r], o = optr[r], k = optr[r + 1] - o;
+  for (int i = 0; i < k; ++i) {
+    oi[o + i] = ci[b + i];
+    ow[o + i] = cw[b + i];
+  }
+}
+
+template <typename T>
+__global__ void k_scatter_add_rows(int n, int C, const int32_t* __restrict__ agg, const T* __restrict__ X,
+                                   int64_t ldx, T* __restrict__ Y, int64_t ldy) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * C) return;
+  const int64_t r = t / C;
+  const int c = (int)(t - r * C);
+  atomic_add(Y + (int64_t)agg[r] * ldy + c, X[r * ldx + c]);
+}
+
+template <typename T>
+__global__ void k_gather_add_rows(int n, int C, const int32_t* __restrict__ agg, const T* __restrict__ Xc,
+                                  int64_t ldc, T* __restrict__ X, int64_t ldx) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * C) return;
+  const int64_t r = t / C;
+  const int c = (int)(t - r * C);
+  X[r * ldx + c] += Xc[(int64_t)agg[r] * ldc + c];
+}
+
+}  // namespace dlp
+
+using namespace dlp;
+
+// Handshake matching (amg.py:32-64) over the CSR of one level: agg (out, int32[n]) holds the
+// renumbered aggregate of every row; *nc_out (host) the aggregate count.  Workspace ints:
+// choice(n) + flag(n) + scan workspace; matched bytes(n).
+extern "C" int dlp_amg_match(int n, const int32_t* indptr, const int32_t* indices, const void* w, int dtype,
+                             int rounds, int32_t* agg, int32_t* choice, int32_t* flag, uint8_t* matched,
+                             void* scan_ws, size_t scan_bytes, int32_t* nc_dev, void* stream) {
+  DLP_CHECK_ARG(n >= 1 && indptr && indices && w && agg && choice && flag && matched && nc_dev,
+                "amg_match: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 g(grid_for(n)), b(kBlock);
+  hipLaunchKernelGGL(k_agg_init, g, b, 0, st, n, agg, matched);
+  for (int r = 0; r < rounds; ++r) {
+    if (dtype == DLP_F32)
+      hipLaunchKernelGGL(k_best_choice<float>, g, b, 0, st, n, indptr, indices, (const float*)w, matched, 0, choice);
+    else
+      hipLaunchKernelGGL(k_best_choice<double>, g, b, 0, st, n, indptr, indices, (const double*)w, matched, 0,
+                         choice);
+    hipLaunchKernelGGL(k_mutual, g, b, 0, st, n, choice, matched, agg);
+  }
+  if (dtype == DLP_F32)
+    hipLaunchKernelGGL(k_best_choice<float>, g, b, 0, st, n, indptr, indices, (const float*)w, matched, 1, choice);
+  else
+    hipLaunchKernelGGL(k_best_choice<double>, g, b, 0, st, n, indptr, indices, (const double*)w, matched, 1, choice);
+  hipLaunchKernelGGL(k_attach, g, b, 0, st, n, choice, agg);
+  hipLaunchKernelGGL(k_zero_i32, g, b, 0, st, n, flag);
+  hipLaunchKernelGGL(k_left_flag, g, b, 0, st, n, agg, flag);
+  DLP_CHECK_LAUNCH();
+  int rc = exclusive_scan_i32(n, flag, flag, nc_dev, scan_ws, scan_bytes, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_relabel, g, b, 0, st, n, agg, flag);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+extern "C" int dlp_amg_compose(int n, const int32_t* prev, const int32_t* a2, int32_t* out, void* stream) {
+  DLP_CHECK_ARG(n >= 0, "amg_compose: bad n");
+  if (n == 0) return DLP_OK;
+  hipLaunchKernelGGL(k_compose, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, n, prev, a2, out);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+// Coarse operator, phase 1: per coarse row counts (cnt, nc+1 ints zeroed here) and the
+// unmerged entries (ci/cw of capacity nnz) in row segments off (exclusive scan, nc+1).
+// Phase 2 (dlp_amg_coarsen_merge): sort + merge per row, unique counts into ucnt.
+extern "C" int dlp_amg_coarsen_fill(int n, int nc, const int32_t* indptr, const int32_t* indices, const void* w,
+                                    int dtype, const int32_t* agg, int32_t* off, int32_t* cursor, int32_t* ci,
+                                    void* cw, void* scan_ws, size_t scan_bytes, void* stream) {
+  DLP_CHECK_ARG(n >= 1 && nc >= 1, "amg_coarsen_fill: bad sizes");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(nc + 1)), dim3(kBlock), 0, st, nc + 1, off);
+  hipLaunchKernelGGL(k_coarse_count, dim3(grid_for(n)), dim3(kBlock), 0, st, n, indptr, indices, agg, off);
+  DLP_CHECK_LAUNCH();
+  int rc = exclusive_scan_i32(nc + 1, off, off, nullptr, scan_ws, scan_bytes, st);
+  if (rc) return rc;
+  DLP_CHECK_HIP(hipMemcpyAsync(cursor, off, sizeof(int32_t) * (nc + 1), hipMemcpyDeviceToDevice, st));
+  if (dtype == DLP_F32)
+    hipLaunchKernelGGL(k_coarse_fill<float>, dim3(grid_for(n)), dim3(kBlock), 0, st, n, indptr, indices,
+                       (const float*)w, agg, cursor, ci, (float*)cw);
+  else
+    hipLaunchKernelGGL(k_coarse_fill<double>, dim3(grid_for(n)), dim3(kBlock), 0, st, n, indptr, indices,
+                       (const double*)w, agg, cursor, ci, (double*)cw);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+extern "C" int dlp_amg_coarsen_merge(int nc, const int32_t* off, int32_t* ci, void* cw, int dtype, int32_t* ucnt,
+                                     void* scan_ws, size_t scan_bytes, int32_t* total_dev, void* stream) {
+  DLP_CHECK_ARG(nc >= 1, "amg_coarsen_merge: bad nc");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == DLP_F32)
+    hipLaunchKernelGGL(k_coarse_merge<float>, dim3(grid_for(nc)), dim3(kBlock), 0, st, nc, off, ci, (float*)cw, ucnt);
+  else
+    hipLaunchKernelGGL(k_coarse_merge<double>, dim3(grid_for(nc)), dim3(kBlock), 0, st, nc, off, ci, (double*)cw,
+                       ucnt);
+  DLP_CHECK_LAUNCH();
+  // ucnt has nc+1 slots: scan to row pointers, total in total_dev
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, 1, ucnt + nc);
+  int rc = exclusive_scan_i32(nc + 1, ucnt, ucnt, total_dev, scan_ws, scan_bytes, st);
+  return rc;
+}
+
+extern "C" int dlp_amg_coarsen_compact(int nc, const int32_t* off, const int32_t* ci, const void* cw, int dtype,
+                                       const int32_t* optr, int32_t* oi, void* ow, void* stream) {
+  DLP_CHECK_ARG(nc >= 1, "amg_coarsen_compact: bad nc");
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == DLP_F32)
+    hipLaunchKernelGGL(k_coarse_compact<float>, dim3(grid_for(nc)), dim3(kBlock), 0, st, nc, off, ci,
+                       (const float*)cw, optr, oi, (float*)ow);
+  else
+    hipLaunchKernelGGL(k_coarse_compact<double>, dim3(grid_for(nc)), dim3(kBlock), 0, st, nc, off, ci,
+                       (const double*)cw, optr, oi, (double*)ow);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+// Y[agg[r]] += X[r] (restriction P^T, amg.py:72,129) — Y zeroed by the caller.
+extern "C" int dlp_amg_restrict(int n, int C, const int32_t* agg, const void* X, int64_t ldx, void* Y, int64_t ldy,
+                                int dtype, void* stream) {
+  DLP_CHECK_ARG(n >= 0 && C >= 1, "amg_restrict: bad sizes");
+  if (n == 0) return DLP_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 g(grid_for((long long)n * C)), b(kBlock);
+  if (dtype == DLP_F32)
+    hipLaunchKernelGGL(k_scatter_add_rows<float>, g, b, 0, st, n, C, agg, (const float*)X, ldx, (float*)Y, ldy);
+  else
+    hipLaunchKernelGGL(k_scatter_add_rows<double>, g, b, 0, st, n, C, agg, (const double*)X, ldx, (double*)Y, ldy);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+// X[r] += Xc[agg[r]] (prolongation, amg.py:130).
+extern "C" int dlp_amg_prolong_add(int n, int C, const int32_t* agg, const void* Xc, int64_t ldc, void* X,
+                                   int64_t ldx, int dtype, void* stream) {
+  DLP_CHECK_ARG(n >= 0 && C >= 1, "amg_prolong: bad sizes");
+  if (n == 0) return DLP_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 g(grid_for((long long)n * C)), b(kBlock);
+  if (dtype == DLP_F32)
+    hipLaunchKernelGGL(k_gather_add_rows<float>, g, b, 0, st, n, C, agg, (const float*)Xc, ldc, (float*)X, ldx);
+  else
+    hipLaunchKernelGGL(k_gather_add_rows<double>, g, b, 0, st, n, C, agg, (const double*)Xc, ldc, (double*)X, ldx);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}

@@ -211,3 +211,117 @@ int pcg_finish_iteration(const PcgBufs& b, hipStream_t st) {
 }
 
 }  // namespace dlp
+
+// ---------------------------------------------------------------------------------------
+// Generic column-wise BLAS-1 over row-major blocks (FP32 or FP64), for the reference-API
+// solvers (solvers.py:66-97): column dots, column-scaled axpy, P = X + beta P, X / diag.
+
+namespace dlp {
+
+template <typename T, int CPL>
+__global__ __launch_bounds__(256) void k_coldot_g(int n, int C, int lc_shift, const T* __restrict__ X, int64_t ldx,
+                                                  const T* __restrict__ Y, int64_t ldy,
+                                                  const T* __restrict__ d, double* __restrict__ out) {
+  __shared__ double sm[kMaxCols];
+  for (int c = threadIdx.x; c < C; c += blockDim.x) sm[c] = 0.0;
+  __syncthreads();
+  const int LC = 1 << lc_shift;
+  const int l = threadIdx.x & (LC - 1);
+  double acc[CPL];
+#pragma unroll
+  for (int k = 0; k < CPL; ++k) acc[k] = 0.0;
+  const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> lc_shift;
+  for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> lc_shift; r < n; r += groups) {
+    const double s = d ? 1.0 / (double)d[r] : 1.0;
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) {
+      const int c = l + k * LC;
+      if (c < C) acc[k] += (double)X[r * ldx + c] * (double)Y[r * ldy + c] * s;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < CPL; ++k) {
+    const int c = l + k * LC;
+    if (c < C) atomicAdd(&sm[c], acc[k]);
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&out[c], sm[c]);
+}
+
+// mode 0: Y += sign*a_c*X;  mode 1: Y = X + a_c*Y;  mode 2: Y = X / d[r]
+template <typename T, int CPL>
+__global__ __launch_bounds__(256) void k_colop_g(int n, int C, int lc_shift, int mode, double sign,
+                                                 const double* __restrict__ a, const T* __restrict__ X,
+                                                 int64_t ldx, const T* __restrict__ d, T* __restrict__ Y,
+                                                 int64_t ldy) {
+  const int LC = 1 << lc_shift;
+  const int l = threadIdx.x & (LC - 1);
+  T av[CPL];
+#pragma unroll
+  for (int k = 0; k < CPL; ++k) {
+    const int c = l + k * LC;
+    av[k] = (a && c < C) ? (T)(sign * a[c]) : (T)0;
+  }
+  const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> lc_shift;
+  for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> lc_shift; r < n; r += groups) {
+    const T inv = (mode == 2) ? (T)1 / d[r] : (T)0;
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) {
+      const int c = l + k * LC;
+      if (c < C) {
+        const T x = X[r * ldx + c];
+        T* y = Y + r * ldy + c;
+        if (mode == 0) *y += av[k] * x;
+        else if (mode == 1) *y = x + av[k] * *y;
+        else *y = x * inv;
+      }
+    }
+  }
+}
+
+}  // namespace dlp
+
+using namespace dlp;
+
+extern "C" int dlp_coldot(int n, int C, const void* X, int64_t ldx, const void* Y, int64_t ldy,
+                          const void* d, double* out, int dtype, void* stream) {
+  DLP_CHECK_ARG(n >= 0 && C >= 1 && C <= kMaxCols && out, "coldot: bad arguments");
+  DLP_CHECK_ARG(n == 0 || (X && Y), "coldot: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, C, out);
+  if (n == 0) return DLP_OK;
+  const ColMap m = col_map(C);
+  const unsigned g = pass_grid(n, m.lc_shift);
+#define K(CPLV)                                                                                       \
+  if (dtype == DLP_F32)                                                                               \
+    hipLaunchKernelGGL((k_coldot_g<float, CPLV>), dim3(g), dim3(kBlock), 0, st, n, C, m.lc_shift,       \
+                       (const float*)X, ldx, (const float*)Y, ldy, (const float*)d, out);              \
+  else                                                                                                \
+    hipLaunchKernelGGL((k_coldot_g<double, CPLV>), dim3(g), dim3(kBlock), 0, st, n, C, m.lc_shift,      \
+                       (const double*)X, ldx, (const double*)Y, ldy, (const double*)d, out);
+  DLP_DISPATCH_CPL(m.cpl, K)
+#undef K
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+extern "C" int dlp_colop(int n, int C, int mode, double sign, const double* a, const void* X, int64_t ldx,
+                         const void* d, void* Y, int64_t ldy, int dtype, void* stream) {
+  DLP_CHECK_ARG(n >= 0 && C >= 1 && C <= kMaxCols && mode >= 0 && mode <= 2, "colop: bad arguments");
+  if (n == 0) return DLP_OK;
+  DLP_CHECK_ARG(X && Y && (mode == 2 ? d != nullptr : a != nullptr), "colop: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const ColMap m = col_map(C);
+  const unsigned g = pass_grid(n, m.lc_shift);
+#define K(CPLV)                                                                                        \
+  if (dtype == DLP_F32)                                                                                \
+    hipLaunchKernelGGL((k_colop_g<float, CPLV>), dim3(g), dim3(kBlock), 0, st, n, C, m.lc_shift, mode,   \
+                       sign, a, (const float*)X, ldx, (const float*)d, (float*)Y, ldy);                \
+  else                                                                                                 \
+    hipLaunchKernelGGL((k_colop_g<double, CPLV>), dim3(g), dim3(kBlock), 0, st, n, C, m.lc_shift, mode,  \
+                       sign, a, (const double*)X, ldx, (const double*)d, (double*)Y, ldy);
+  DLP_DISPATCH_CPL(m.cpl, K)
+#undef K
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}

@@ -77,6 +77,42 @@ int dlp_violating_rows(int nrows, const int32_t* rows, int C, const void* R, int
                        int r_dtype, const double* diag, const double* eps, int32_t* out,
                        int32_t* count, void* stream);
 
+/* ---- Column-wise BLAS-1 of the reference PCG (solvers.py:66-97) ---------------------
+ * out[c] = sum_r X[r,c]*Y[r,c] (/ d[r] when d != NULL), FP64 accumulation, out overwritten
+ * (the (R*Y).sum(axis=0), (P*AP).sum(axis=0) of solvers.py:76,79,94). */
+int dlp_coldot(int n, int C, const void* X, int64_t ldx, const void* Y, int64_t ldy, const void* d,
+               double* out, int dtype, void* stream);
+/* mode 0: Y += sign*a[c]*X (solvers.py:81-82); mode 1: Y = X + a[c]*Y (:95); mode 2:
+ * Y = X / d[r] (jacobi_M, :62-63).  a: device FP64[C]. */
+int dlp_colop(int n, int C, int mode, double sign, const double* a, const void* X, int64_t ldx,
+              const void* d, void* Y, int64_t ldy, int dtype, void* stream);
+
+/* ---- Aggregation AMG (amg.AMG, amg.py:77-134) ------------------------------------------
+ * Handshake matching of one level (amg.py:32-64): agg[r] = renumbered aggregate, *nc_dev =
+ * count.  choice/flag: int32[n] workspaces, matched: uint8[n]; scan workspace
+ * dlp_scan_workspace_bytes(n). */
+int dlp_amg_match(int n, const int32_t* indptr, const int32_t* indices, const void* w, int dtype,
+                  int rounds, int32_t* agg, int32_t* choice, int32_t* flag, uint8_t* matched,
+                  void* scan_ws, size_t scan_bytes, int32_t* nc_dev, void* stream);
+/* out[r] = a2[prev[r]] (composition of two matching passes, amg.py:92). */
+int dlp_amg_compose(int n, const int32_t* prev, const int32_t* a2, int32_t* out, void* stream);
+/* Galerkin coarse operator W_c = offdiag(P^T W P) (amg.py:67-74) in three steps: fill
+ * (counting sort by aggregate into segments off[nc+1], entries ci/cw of capacity nnz), merge
+ * (per-row sort + duplicate sum; optr[nc+1] row pointers, *total_dev entries), compact. */
+int dlp_amg_coarsen_fill(int n, int nc, const int32_t* indptr, const int32_t* indices, const void* w,
+                         int dtype, const int32_t* agg, int32_t* off, int32_t* cursor, int32_t* ci,
+                         void* cw, void* scan_ws, size_t scan_bytes, void* stream);
+int dlp_amg_coarsen_merge(int nc, const int32_t* off, int32_t* ci, void* cw, int dtype, int32_t* optr,
+                          void* scan_ws, size_t scan_bytes, int32_t* total_dev, void* stream);
+int dlp_amg_coarsen_compact(int nc, const int32_t* off, const int32_t* ci, const void* cw, int dtype,
+                            const int32_t* optr, int32_t* oi, void* ow, void* stream);
+/* Y[agg[r]] += X[r] (restriction P^T, amg.py:72,129; Y zeroed by the caller) and
+ * X[r] += Xc[agg[r]] (prolongation, amg.py:130). */
+int dlp_amg_restrict(int n, int C, const int32_t* agg, const void* X, int64_t ldx, void* Y, int64_t ldy,
+                     int dtype, void* stream);
+int dlp_amg_prolong_add(int n, int C, const int32_t* agg, const void* Xc, int64_t ldc, void* X,
+                        int64_t ldx, int dtype, void* stream);
+
 /* ---- Frontier primitives (FrontierOps, kernels.py:436-492) ----------------------------- */
 /* Y[g] = (rhs[f_g] + sum_j w_j X[col_j]) / diag[f_g], changed[g] = any_c |Y[g,c]-X[f_g,c]| > delta
  * for g < nf (f = frontier); adds the frontier's edge count to *edges (device uint64, may be
