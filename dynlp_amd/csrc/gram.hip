@@ -1,0 +1,317 @@
+// Block PCG building blocks: the tall-skinny FP64 Gram product G += X^T diag(s) Y on the
+// matrix cores (v_mfma_f64_16x16x4f64), an FP64 Cholesky solve of the k x k systems in one
+// workgroup, and the row-local block updates (n x k times k x k).
+//
+// Gram: a workgroup stages 32 rows of X and Y in LDS (converted to FP64, columns padded to a
+// multiple of 16), and each of its four wavefronts accumulates a share of the 16 x 16 output
+// tiles with MFMA over 4-row k-steps.  Operand layout of the 16x16x4 FP64 MFMA: lane l holds
+// A[l % 16][l / 16] and B[l / 16][l % 16]; the result tile has 4 values per lane,
+// D[l / 16 + 4 * v][l % 16], v = 0..3 (pinned by the Gram test).  Partial tiles are added to G
+// with FP64 atomics.
+#include "common.h"
+
+namespace dlp {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kGramRows = 32;   // rows per LDS stage (2 x 32 x 80 doubles = 40 KB)
+constexpr int kGramMaxT = 5;    // up to 80 columns (5 tiles of 16)
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gram_mfma(int n, int C, const T* __restrict__ X, int64_t ldx,
+                                                   const T* __restrict__ Y, int64_t ldy,
+                                                   const T* __restrict__ rowdiv, double* __restrict__ G) {
+  const int nt = (C + 15) >> 4;          // tiles per side
+  const int CP = nt << 4;                // padded columns
+  __shared__ double xs[kGramRows * kGramMaxT * 16];
+  __shared__ double ys[kGramRows * kGramMaxT * 16];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ntiles = nt * nt;
+  // each wave owns tiles wave, wave+4, ... (at most 7 of 25)
+  d4_t acc[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) acc[q] = (d4_t){0.0, 0.0, 0.0, 0.0};
+  const int li = lane & 15, lk = lane >> 4;
+  for (int64_t r0 = (int64_t)blockIdx.x * kGramRows; r0 < n; r0 += (int64_t)gridDim.x * kGramRows) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < kGramRows * CP; t += blockDim.x) {
+      const int rr = t / CP, c = t - rr * CP;
+      const int64_t r = r0 + rr;
+      double xv = 0.0, yv = 0.0;
+      if (r < n && c < C) {
+        xv = (double)X[r * ldx + c];
+        yv = (double)Y[r * ldy + c];
+        if (rowdiv) yv /= (double)rowdiv[r];
+      }
+      xs[rr * CP + c] = xv;
+      ys[rr * CP + c] = yv;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+      const int tile = wave + 4 * q;
+      if (tile < ntiles) {
+        const int ti = tile / nt, tj = tile - ti * nt;
+        d4_t a4 = acc[q];
+        for (int k = 0; k < kGramRows; k += 4) {
+          const double a = xs[(k + lk) * CP + ti * 16 + li];
+          const double b = ys[(k + lk) * CP + tj * 16 + li];
+          a4 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, a4, 0, 0, 0);
+        }
+        acc[q] = a4;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    const int tile = wave + 4 * q;
+    if (tile < ntiles) {
+      const int ti = tile / nt, tj = tile - ti * nt;
+      const int j = tj * 16 + li;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = ti * 16 + lk + 4 * v;
+        if (i < C && j < C) atomicAdd(&G[i * C + j], acc[q][v]);
+      }
+    }
+  }
+}
+
+// S = A^{-1} B for SPD A (k x k, FP64, row-major), B k x k; one workgroup.  The system is
+// symmetrically scaled to unit diagonal; a pivot below tol (relative) marks a direction that
+// has collapsed (its column of the block has converged or become dependent): that row and
+// column are dropped (deflation), and the matching rows of S are zero.
+__global__ __launch_bounds__(256) void k_chol_solve(int k, const double* __restrict__ A, const double* __restrict__ B,
+                                                    double* __restrict__ S, double tol) {
+  extern __shared__ double sh[];
+  double* L = sh;              // k*k
+  double* X = sh + k * k;      // k*k (right-hand sides, then solution)
+  double* dsc = X + k * k;     // k scaling
+  int* alive = (int*)(dsc + k);
+  for (int t = threadIdx.x; t < k; t += blockDim.x) {
+    const double d = A[t * k + t];
+    dsc[t] = d > 0.0 ? 1.0 / sqrt(d) : 0.0;
+    alive[t] = d > 0.0;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < k * k; t += blockDim.x) {
+    const int i = t / k, j = t - (t / k) * k;
+    L[t] = A[t] * dsc[i] * dsc[j];
+    X[t] = B[t] * dsc[i];
+  }
+  __syncthreads();
+  // right-looking Cholesky, lower triangle
+  for (int c = 0; c < k; ++c) {
+    __shared__ double piv;
+    if (threadIdx.x == 0) {
+      double p = alive[c] ? L[c * k + c] : 0.0;
+      if (!(p > tol)) {
+        alive[c] = 0;
+        piv = 0.0;
+      } else {
+        piv = sqrt(p);
+      }
+    }
+    __syncthreads();
+    const double pv = piv;
+    for (int i = c + threadIdx.x; i < k; i += blockDim.x) {
+      if (pv > 0.0) L[i * k + c] = (i == c) ? pv : L[i * k + c] / pv;
+      else L[i * k + c] = (i == c) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (pv > 0.0) {
+      for (int t = threadIdx.x; t < (k - c - 1) * (k - c - 1); t += blockDim.x) {
+        const int i = c + 1 + t / (k - c - 1), j = c + 1 + t % (k - c - 1);
+        if (j <= i) L[i * k + j] -= L[i * k + c] * L[j * k + c];
+      }
+    }
+    __syncthreads();
+  }
+  // forward and back substitution, one thread per right-hand side column
+  for (int col = threadIdx.x; col < k; col += blockDim.x) {
+    for (int i = 0; i < k; ++i) {
+      if (!alive[i]) {
+        X[i * k + col] = 0.0;
+        continue;
+      }
+      double s = X[i * k + col];
+      for (int j = 0; j < i; ++j) s -= L[i * k + j] * X[j * k + col];
+      X[i * k + col] = s / L[i * k + i];
+    }
+    for (int i = k - 1; i >= 0; --i) {
+      if (!alive[i]) {
+        X[i * k + col] = 0.0;
+        continue;
+      }
+      double s = X[i * k + col];
+      for (int j = i + 1; j < k; ++j) s -= L[j * k + i] * X[j * k + col];
+      X[i * k + col] = s / L[i * k + i];
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < k * k; t += blockDim.x) {
+    const int i = t / k;
+    S[t] = X[t] * dsc[i];
+  }
+}
+
+// Row-local block updates with the k x k matrix in LDS (FP32 copy of FP64 S):
+// mode 0: Z += P S, R -= AP S, viol = max |R|/(eps*diag);  mode 1: P = R/diag + P S.
+template <int CPL>
+__global__ __launch_bounds__(256) void k_block_update(int n, int C, int lc_shift, int mode, float* __restrict__ Z,
+                                                      float* __restrict__ R, float* __restrict__ P,
+                                                      const float* __restrict__ AP, int64_t ld,
+                                                      const double* __restrict__ S, const float* __restrict__ diag,
+                                                      const float* __restrict__ eps, float* __restrict__ viol) {
+  __shared__ float s[kGramMaxT * 16 * kGramMaxT * 16];
+  __shared__ float vmax;
+  for (int t = threadIdx.x; t < C * C; t += blockDim.x) s[t] = (float)S[t];
+  if (threadIdx.x == 0) vmax = 0.f;
+  __syncthreads();
+  const int LC = 1 << lc_shift;
+  const int l = threadIdx.x & (LC - 1);
+  const int lane = threadIdx.x & 63;
+  const int gbase = lane & ~(LC - 1);     // first lane of this row group in the wavefront
+  float ie[CPL];
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) {
+    const int c = l + q * LC;
+    ie[q] = (c < C && eps) ? 1.f / eps[c] : 0.f;
+  }
+  float vm = 0.f;
+  const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> lc_shift;
+  const int64_t wave_first = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) >> lc_shift;
+  const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> lc_shift;
+  for (int64_t base = wave_first; base < n; base += groups) {
+    const int64_t r = base + (g0 - wave_first);
+    const bool valid = r < n;
+    const int64_t o = (valid ? r : 0) * ld;
+    float p[CPL], ap[CPL], outp[CPL], outa[CPL];
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+      const int c = l + q * LC;
+      p[q] = (valid && c < C) ? P[o + c] : 0.f;
+      ap[q] = (valid && c < C && mode == 0) ? AP[o + c] : 0.f;
+      outp[q] = outa[q] = 0.f;
+    }
+    // out[j] = sum_i in[i] * S[i][j]: broadcast in[i] from the lane that holds column i
+    for (int i = 0; i < C; ++i) {
+      const int src = gbase + (i & (LC - 1));
+      const int qi = i >> lc_shift;
+      float pi = 0.f, ai = 0.f;
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) {
+        const float tp = __shfl(p[q], src, 64);
+        const float ta = __shfl(ap[q], src, 64);
+        if (q == qi) {
+          pi = tp;
+          ai = ta;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) {
+        const int j = l + q * LC;
+        if (j < C) {
+          const float sij = s[i * C + j];
+          outp[q] += pi * sij;
+          outa[q] += ai * sij;
+        }
+      }
+    }
+    if (valid) {
+      const float inv = 1.f / diag[r];
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) {
+        const int c = l + q * LC;
+        if (c < C) {
+          if (mode == 0) {
+            Z[o + c] += outp[q];
+            const float rv = R[o + c] - outa[q];
+            R[o + c] = rv;
+            vm = fmaxf(vm, fabsf(rv) * inv * ie[q]);
+          } else {
+            P[o + c] = R[o + c] * inv + outp[q];
+          }
+        }
+      }
+    }
+  }
+  if (mode == 0) {
+    for (int off = 32; off > 0; off >>= 1) vm = fmaxf(vm, __shfl_xor(vm, off, 64));
+    if (lane == 0) atomic_max_nonneg(&vmax, vm);
+    __syncthreads();
+    if (threadIdx.x == 0) atomic_max_nonneg(viol, vmax);
+  }
+}
+
+__global__ void k_zero_f64b(int n, double* __restrict__ p) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+__global__ void k_zero_f32b(float* __restrict__ p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0.f;
+}
+
+__global__ void k_copy_f64(int n, const double* __restrict__ a, double* __restrict__ b) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = a[i];
+}
+
+}  // namespace dlp
+
+using namespace dlp;
+
+// G = X^T diag(1/rowdiv) Y (FP64, C x C row-major, overwritten).  X, Y: n x C, FP32 or FP64.
+extern "C" int dlp_gram(int n, int C, const void* X, int64_t ldx, const void* Y, int64_t ldy, const void* rowdiv,
+                        double* G, int dtype, void* stream) {
+  DLP_CHECK_ARG(n >= 0 && C >= 1 && C <= 16 * kGramMaxT && G, "gram: C=%d outside [1, %d]", C, 16 * kGramMaxT);
+  DLP_CHECK_ARG(n == 0 || (X && Y), "gram: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_f64b, dim3(8), dim3(kBlock), 0, st, C * C, G);
+  if (n > 0) {
+    unsigned grid = (unsigned)((n + kGramRows - 1) / kGramRows);
+    if (grid > 1024) grid = 1024;
+    if (dtype == DLP_F32)
+      hipLaunchKernelGGL(k_gram_mfma<float>, dim3(grid), dim3(kBlock), 0, st, n, C, (const float*)X, ldx,
+                         (const float*)Y, ldy, (const float*)rowdiv, G);
+    else
+      hipLaunchKernelGGL(k_gram_mfma<double>, dim3(grid), dim3(kBlock), 0, st, n, C, (const double*)X, ldx,
+                         (const double*)Y, ldy, (const double*)rowdiv, G);
+  }
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+// S = A^{-1} B (k x k FP64, device), A SPD; deflates collapsed directions (relative pivot tol).
+extern "C" int dlp_chol_solve(int k, const double* A, const double* B, double* S, double tol, void* stream) {
+  DLP_CHECK_ARG(k >= 1 && k <= 128 && A && B && S, "chol_solve: bad arguments (k=%d)", k);
+  const size_t shm = sizeof(double) * (2 * k * k + k) + sizeof(int) * k;
+  hipLaunchKernelGGL(k_chol_solve, dim3(1), dim3(kBlock), shm, (hipStream_t)stream, k, A, B, S, tol);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+namespace dlp {
+int copy_f64(int n, const double* a, double* b, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy_f64, dim3(8), dim3(kBlock), 0, st, n, a, b);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+int block_update(int n, int C, int mode, float* Z, float* R, float* P, const float* AP, int64_t ld, const double* S,
+                 const float* diag, const float* eps, float* viol, hipStream_t st) {
+  int lc = C <= 1 ? 0 : ceil_log2(C);
+  if (lc > 6) lc = 6;
+  const int cpl = (C + (1 << lc) - 1) >> lc;
+  long long threads = (long long)n << lc;
+  unsigned g = grid_for(threads);
+  if (g > 2048) g = 2048;
+  if (mode == 0) hipLaunchKernelGGL(k_zero_f32b, dim3(1), dim3(1), 0, st, viol);
+#define K(CPLV)                                                                                     \
+  hipLaunchKernelGGL((k_block_update<CPLV>), dim3(g), dim3(kBlock), 0, st, n, C, lc, mode, Z, R, P, AP, ld, S, \
+                     diag, eps, viol);
+  DLP_DISPATCH_CPL(cpl, K)
+#undef K
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+}  // namespace dlp

@@ -712,6 +712,9 @@ struct LP {
   int pcg_graph_iters = 0;
   double pcg_ms = 0.0;
   int64_t pcg_iters = 0;
+  double* bk = nullptr;         // block PCG k x k matrices: G, PAP, Gn, alpha, beta
+  hipGraphExec_t bpcg_exec = nullptr;
+  int bpcg_graph_iters = 0;
   double push_ms = 0.0;         // summed duration of push-round launches (events)
   int64_t push_launched = 0;    // rounds launched
 };
@@ -741,6 +744,8 @@ int lp_free(LP* h) {
   if (h->pcg_scal) (void)hipFree(h->pcg_scal);
   if (h->pcg_viol) (void)hipFree(h->pcg_viol);
   if (h->pcg_exec) (void)hipGraphExecDestroy(h->pcg_exec);
+  if (h->bk) (void)hipFree(h->bk);
+  if (h->bpcg_exec) (void)hipGraphExecDestroy(h->bpcg_exec);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   delete h;
   return DLP_OK;
@@ -886,6 +891,10 @@ extern "C" int dlp_lp_set_store(void* handle, const int32_t* indptr, int32_t* ro
   if (h->pcg_exec) {
     (void)hipGraphExecDestroy(h->pcg_exec);
     h->pcg_exec = nullptr;
+  }
+  if (h->bpcg_exec) {
+    (void)hipGraphExecDestroy(h->bpcg_exec);
+    h->bpcg_exec = nullptr;
   }
   return DLP_OK;
 }
@@ -1424,4 +1433,126 @@ __global__ void k_sum_i32(int n, const int32_t* __restrict__ a, unsigned long lo
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += (unsigned)a[i];
   s = dlp::wave_sum_u64(s);
   if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, s);
+}
+
+// ---------------------------------------------------------------------------------------
+// Block PCG (one search space shared by all C1 columns): alpha = (P^T A P)^{-1} (R^T D^{-1} R),
+// beta = (R^T D^{-1} R)_old^{-1} (R^T D^{-1} R)_new, with the k x k Grams on the FP64 matrix
+// cores and their Cholesky solves in FP64 (gram.hip); vectors FP32.
+
+extern "C" int dlp_gram(int n, int C, const void* X, int64_t ldx, const void* Y, int64_t ldy, const void* rowdiv,
+                        double* G, int dtype, void* stream);
+extern "C" int dlp_chol_solve(int k, const double* A, const double* B, double* S, double tol, void* stream);
+namespace dlp {
+int block_update(int n, int C, int mode, float* Z, float* R, float* P, const float* AP, int64_t ld, const double* S,
+                 const float* diag, const float* eps, float* viol, hipStream_t st);
+int copy_f64(int n, const double* a, double* b, hipStream_t st);
+}
+
+static int bpcg_iteration(LP* h, const PcgBufs& b, hipStream_t st) {
+  const int C = b.C;
+  double* G = h->bk;
+  double* PAP = G + C * C;
+  double* Gn = PAP + C * C;
+  double* alpha = Gn + C * C;
+  double* beta = alpha + C * C;
+  int rc = dlp_row_operator(b.n, nullptr, C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, DLP_F32, h->d.solved,
+                            b.P, b.ld, DLP_F32, b.diag, -1.0, nullptr, 0, b.AP, b.ld, DLP_F32, 0, 20.0, st);
+  if (!rc) rc = dlp_gram(b.n, C, b.P, b.ld, b.AP, b.ld, nullptr, PAP, DLP_F32, st);
+  if (!rc) rc = dlp_chol_solve(C, PAP, G, alpha, 1e-8, st);
+  if (!rc) rc = block_update(b.n, C, 0, b.Z, b.R, b.P, b.AP, b.ld, alpha, b.diag, b.eps, b.viol, st);
+  if (!rc) rc = dlp_gram(b.n, C, b.R, b.ld, b.R, b.ld, b.diag, Gn, DLP_F32, st);
+  if (!rc) rc = dlp_chol_solve(C, G, Gn, beta, 1e-8, st);
+  if (!rc) rc = block_update(b.n, C, 1, b.Z, b.R, b.P, b.AP, b.ld, beta, b.diag, b.eps, b.viol, st);
+  if (!rc) rc = copy_f64(C * C, Gn, G, st);
+  return rc;
+}
+
+static int capture_bpcg(LP* h, int iters) {
+  if (h->bpcg_exec && h->bpcg_graph_iters == iters) return DLP_OK;
+  if (h->bpcg_exec) {
+    (void)hipGraphExecDestroy(h->bpcg_exec);
+    h->bpcg_exec = nullptr;
+  }
+  hipStream_t cs;
+  DLP_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  const PcgBufs b = pcg_bufs(h);
+  hipGraph_t graph;
+  int rc = DLP_OK;
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    for (int i = 0; i < iters && rc == DLP_OK; ++i) rc = bpcg_iteration(h, b, cs);
+    e = hipStreamEndCapture(cs, &graph);
+  }
+  if (e == hipSuccess && rc == DLP_OK) {
+    e = hipGraphInstantiate(&h->bpcg_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+  }
+  (void)hipStreamDestroy(cs);
+  if (e != hipSuccess || rc != DLP_OK) {
+    set_error("block pcg graph capture failed: %s", hipGetErrorString(e));
+    h->bpcg_exec = nullptr;
+    return DLP_ERR_HIP;
+  }
+  h->bpcg_graph_iters = iters;
+  return DLP_OK;
+}
+
+extern "C" int dlp_lp_bpcg(void* handle, int max_iter, int iters_per_launch, int use_graph, int patience,
+                           int64_t* iters_out, double* viol_out, void* stream) {
+  DLP_CHECK_ARG(handle && max_iter >= 0 && iters_per_launch >= 1, "lp_bpcg: bad argument");
+  LP* h = (LP*)handle;
+  DLP_CHECK_ARG(h->d.C1 <= 80, "lp_bpcg: block PCG takes at most 80 columns (C1=%d)", h->d.C1);
+  hipStream_t st = (hipStream_t)stream;
+  const int C1 = h->d.C1;
+  const size_t nz = (size_t)h->d.n * h->d.ld;
+  if (!h->P) {
+    DLP_CHECK_HIP(hipMalloc((void**)&h->P, nz * sizeof(float)));
+    DLP_CHECK_HIP(hipMalloc((void**)&h->AP, nz * sizeof(float)));
+    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_scal, 5 * C1 * sizeof(double)));
+    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_viol, sizeof(float)));
+    DLP_CHECK_HIP(hipMemsetAsync(h->P, 0, nz * sizeof(float), st));
+    DLP_CHECK_HIP(hipMemsetAsync(h->AP, 0, nz * sizeof(float), st));
+  }
+  if (!h->bk) DLP_CHECK_HIP(hipMalloc((void**)&h->bk, 5 * C1 * C1 * sizeof(double)));
+  const PcgBufs b = pcg_bufs(h);
+  int rc = pcg_start(b, st);                       // P = R/diag
+  if (!rc) rc = dlp_gram(b.n, C1, b.R, b.ld, b.R, b.ld, b.diag, h->bk, DLP_F32, st);   // G = R^T D^-1 R
+  if (rc) return rc;
+  if (use_graph) {
+    rc = capture_bpcg(h, iters_per_launch);
+    if (rc) return rc;
+  }
+  int64_t it = 0;
+  float v = 0.f, best = 3.4e38f;
+  int stall = 0;
+  while (it < max_iter) {
+    DLP_CHECK_HIP(hipEventRecord(h->ev0, st));
+    if (use_graph) {
+      DLP_CHECK_HIP(hipGraphLaunch(h->bpcg_exec, st));
+    } else {
+      for (int i = 0; i < iters_per_launch; ++i) {
+        rc = bpcg_iteration(h, b, st);
+        if (rc) return rc;
+      }
+    }
+    it += iters_per_launch;
+    DLP_CHECK_HIP(hipEventRecord(h->ev1, st));
+    DLP_CHECK_HIP(hipMemcpyAsync(h->host, h->pcg_viol, sizeof(float), hipMemcpyDeviceToHost, st));
+    DLP_CHECK_HIP(hipStreamSynchronize(st));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess) h->pcg_ms += ms;
+    h->pcg_iters += iters_per_launch;
+    v = *(volatile float*)h->host;
+    if (v <= 1.f) break;
+    if (v < 0.9f * best) {
+      best = v;
+      stall = 0;
+    } else if (++stall >= patience) {
+      break;
+    }
+  }
+  if (iters_out) *iters_out = it;
+  if (viol_out) *viol_out = v;
+  return DLP_OK;
 }

@@ -73,7 +73,7 @@ class EdgeStreamLP:
     def __init__(self, g: Graph, eps=1e-6, eps_h=0.05, fp64=False, eta=0.0, slack_min=4,
                  slack_frac=0.25, push_scale=0.9, rounds_per_launch=4, use_graph=True, max_rounds=200_000,
                  push_frac=0.05, push_max_rows=4096, push_budget=1.0, pcg_iters_per_launch=4,
-                 pcg_max_iter=20_000):
+                 pcg_max_iter=20_000, pcg_mode="column"):
         self.g, self.n, self.K = g, g.n, g.K
         self.C = 1 if g.K == 2 else g.K
         self.C1 = self.C + 1
@@ -82,6 +82,9 @@ class EdgeStreamLP:
         self.push_scale = push_scale
         self.push_frac, self.push_budget, self.push_max_rows = push_frac, push_budget, push_max_rows
         self.pcg_iters_per_launch, self.pcg_max_iter = pcg_iters_per_launch, pcg_max_iter
+        if pcg_mode not in ("column", "block"):
+            raise ValueError(f"pcg_mode {pcg_mode!r}: 'column' or 'block'")
+        self.pcg_mode = pcg_mode
         dev = device()
         ip, ix, d = g.csr()
         sp_, rl, col, w = slack_layout(ip, ix, d, slack_min, slack_frac)
@@ -155,7 +158,8 @@ class EdgeStreamLP:
 
     def _pcg(self):
         it, v = ct.c_int64(0), ct.c_double(0.0)
-        _lib.call("dlp_lp_pcg", self.h, self.pcg_max_iter, self.pcg_iters_per_launch, int(self.use_graph), 50,
+        fn = "dlp_lp_bpcg" if self.pcg_mode == "block" else "dlp_lp_pcg"
+        _lib.call(fn, self.h, self.pcg_max_iter, self.pcg_iters_per_launch, int(self.use_graph), 50,
                   ct.byref(it), ct.byref(v), stream())
         return int(it.value), float(v.value)
 

@@ -252,6 +252,19 @@ int dlp_lp_push(void* handle, int max_rounds, int rounds_per_launch, int use_gra
  * DynLPPlus._global (solvers.py:268-270). */
 int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int use_graph, int patience,
                int64_t* iters_out, double* viol_out, void* stream);
+/* Block PCG variant of dlp_lp_pcg: one search space for all C1 (<= 80) columns, alpha =
+ * (P^T A P)^{-1} (R^T D^{-1} R), beta from the ratio of successive R^T D^{-1} R, the k x k
+ * Grams in FP64 on the matrix cores and their Cholesky solves in FP64 (collapsed directions
+ * deflated).  The block form of solvers.py:66-97 (the reference uses only the Gram diagonals). */
+int dlp_lp_bpcg(void* handle, int max_iter, int iters_per_launch, int use_graph, int patience,
+                int64_t* iters_out, double* viol_out, void* stream);
+/* G = X^T diag(1/rowdiv) Y (C x C FP64, overwritten; rowdiv may be NULL), C <= 80, on the FP64
+ * matrix cores (v_mfma_f64_16x16x4f64). */
+int dlp_gram(int n, int C, const void* X, int64_t ldx, const void* Y, int64_t ldy, const void* rowdiv,
+             double* G, int dtype, void* stream);
+/* S = A^{-1} B for SPD A (k x k FP64, k <= 128) by Cholesky in one workgroup, scaled to unit
+ * diagonal; pivots below tol deflate their direction (row of S = 0). */
+int dlp_chol_solve(int k, const double* A, const double* B, double* S, double tol, void* stream);
 /* Cumulative global-solve timing: out[0] iterations, out[1] microseconds (HIP events). */
 int dlp_lp_pcg_stats(void* handle, int64_t* out);
 /* Average duration in ms (HIP events on `stream`, synchronises) of the global solve's operator

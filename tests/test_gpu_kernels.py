@@ -163,3 +163,49 @@ def test_cc_csr_masked_matches_oracle():
     keep = (mask[coo.row] > 0) & (mask[coo.col] > 0)
     ref, ncr = O.connected_components(g.n, coo.row[keep], coo.col[keep])
     assert nc == ncr and np.array_equal(lab.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("C", [1, 16, 17, 65, 80])
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_gram_on_matrix_cores_matches_numpy(C, dtype):
+    """X^T diag(1/d) Y on v_mfma_f64_16x16x4f64 (also pins the operand/result lane layout)."""
+    import torch
+    from dynlp_amd import _lib
+    from dynlp_amd.backend import ptr, stream
+    rs = np.random.default_rng(C)
+    n = 1037
+    X = rs.standard_normal((n, C)).astype(dtype)
+    Y = rs.standard_normal((n, C)).astype(dtype)
+    d = rs.uniform(1, 3, n).astype(dtype)
+    G = torch.empty((C, C), dtype=torch.float64, device="cuda")
+    Xd, Yd, dd = (torch.as_tensor(a, device="cuda") for a in (X, Y, d))
+    dt = 0 if dtype == "float32" else 1
+    _lib.call("dlp_gram", n, C, ptr(Xd), C, ptr(Yd), C, ptr(dd), ptr(G), dt, stream())
+    ref = X.astype(np.float64).T @ (Y.astype(np.float64) / d.astype(np.float64)[:, None])
+    np.testing.assert_allclose(G.cpu().numpy(), ref, rtol=1e-11, atol=1e-11)
+    _lib.call("dlp_gram", n, C, ptr(Xd), C, ptr(Yd), C, None, ptr(G), dt, stream())
+    np.testing.assert_allclose(G.cpu().numpy(), X.astype(np.float64).T @ Y.astype(np.float64), rtol=1e-11, atol=1e-11)
+
+
+def test_cholesky_solve_and_deflation():
+    import torch
+    from dynlp_amd import _lib
+    from dynlp_amd.backend import ptr, stream
+    rs = np.random.default_rng(0)
+    k = 65
+    M = rs.standard_normal((k, 3 * k))
+    A = M @ M.T + k * np.eye(k)
+    B = rs.standard_normal((k, k))
+    S = torch.empty((k, k), dtype=torch.float64, device="cuda")
+    Ad, Bd = torch.as_tensor(A, device="cuda"), torch.as_tensor(B, device="cuda")
+    _lib.call("dlp_chol_solve", k, ptr(Ad), ptr(Bd), ptr(S), 1e-12, stream())
+    np.testing.assert_allclose(S.cpu().numpy(), np.linalg.solve(A, B), rtol=1e-9, atol=1e-12)
+    # a collapsed direction (zero row/column) is dropped; the rest is the reduced solve
+    A2 = A.copy()
+    A2[7, :] = 0
+    A2[:, 7] = 0
+    _lib.call("dlp_chol_solve", k, ptr(torch.as_tensor(A2, device="cuda")), ptr(Bd), ptr(S), 1e-12, stream())
+    Sh = S.cpu().numpy()
+    keep = np.arange(k) != 7
+    assert np.all(Sh[7] == 0)
+    np.testing.assert_allclose(Sh[keep], np.linalg.solve(A[np.ix_(keep, keep)], B[keep]), rtol=1e-9, atol=1e-12)

@@ -117,3 +117,22 @@ def test_repeat_runs_agree_within_certificate():
     sysm, _ = oracle_system(g, es)
     hmax = O.solve_system(sysm)[:, sysm.C].max()
     assert np.abs(outs[0] - outs[1])[:, : sysm.C].max() <= 2 * 1e-6 * hmax * 1.05
+
+
+@pytest.mark.parametrize("K", [2, 8])
+def test_block_pcg_repair_certifies(K):
+    """The block PCG global solve (FP64 Grams on the matrix cores) repairs large batches to the
+    same certified tolerance."""
+    from dynlp_amd.stream import EdgeStreamLP
+    g = graphs.sbm(3000, 8, 10, 0.8, K, 0.02, seed=7)
+    lp = EdgeStreamLP(g, eps=1e-6, pcg_mode="block", push_max_rows=50)
+    st0 = lp.initial_solve()
+    es = edge_dict(g)
+    check_against_oracle(lp, g, es, 1e-6, "initial")
+    batches, _ = graphs.update_stream(g, 3, 400, seed=7)
+    for t, b in enumerate(batches):
+        st = lp.apply_and_solve(b)
+        assert "pcg" in st.path
+        O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
+                           zip(b.del_u.tolist(), b.del_v.tolist()))
+        check_against_oracle(lp, g, es, 1e-6, f"batch {t}")
