@@ -108,7 +108,7 @@ def main(argv=None):
     stored = lp.stored_entries()
     nbytes = operator_bytes(g.n, lp.C1, stored)
     achieved = nbytes / (op_ms * 1e-3) / 1e9
-    roofline = {"kernel": "k_row_operator<float,float,float,2> (A*P of the global solve)",
+    roofline = {"kernel": "k_row_operator_v4_dot<8> (A*P with fused p^T A p, the global solve)",
                 "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "avg_launch_ms": round(op_ms, 4), "algorithmic_bytes": int(nbytes)}

@@ -48,6 +48,8 @@ _SIGS = {
     "dlp_lp_regrow_plan": (ct.c_int, [P, I32, DBL, P, P, P]),
     "dlp_lp_regrow_commit": (ct.c_int, [P, P, P, P, I32, P, P, P, P, P]),
     "dlp_lp_ground": (ct.c_int, [P, P]),
+    "dlp_lp_ground_incremental": (ct.c_int, [P, I32, P, P, I32, P]),
+    "dlp_lp_ground_flags": (ct.c_int, [P, P, P]),
     "dlp_lp_refresh": (ct.c_int, [P, I32, I32, P]),
     "dlp_lp_push": (ct.c_int, [P, I32, I32, I32, I64, P, P, P]),
     "dlp_lp_push64_round": (ct.c_int, [P, I32, P, P]),

@@ -23,7 +23,13 @@ struct PcgBufs {
 
 // P = R/diag, rz = sum R*R/diag, pAp = 0.
 int pcg_start(const PcgBufs& b, hipStream_t st);
-// After AP = A P has been formed: pAp, alpha, Z/R update (rz_new, viol), beta, new P.
-int pcg_finish_iteration(const PcgBufs& b, hipStream_t st);
+// After AP = A P has been formed: pAp (unless have_pap: the operator accumulated it), alpha,
+// Z/R update (rz_new, viol), beta, new P.
+int pcg_finish_iteration(const PcgBufs& b, hipStream_t st, bool have_pap = false);
+
+// Operator with the fused p^T A p epilogue (spmm.hip).
+int row_operator_dot(int n, int C, const int32_t* indptr, const int32_t* rowlen, const int32_t* indices,
+                     const float* w, const uint8_t* mask, const float* X, int64_t ldx, const float* a, float b,
+                     float* out, int64_t ldo, double* dot, hipStream_t st);
 
 }  // namespace dlp

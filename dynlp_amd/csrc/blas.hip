@@ -190,12 +190,13 @@ int pcg_start(const PcgBufs& b, hipStream_t st) {
   return DLP_OK;
 }
 
-int pcg_finish_iteration(const PcgBufs& b, hipStream_t st) {
+int pcg_finish_iteration(const PcgBufs& b, hipStream_t st, bool have_pap) {
   const ColMap m = col_map(b.C);
   const unsigned g = pass_grid(b.n, m.lc_shift);
 #define K(CPLV)                                                                                       \
-  hipLaunchKernelGGL((k_coldot<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.P, b.AP,   \
-                     b.ld, b.pAp);                                                                   \
+  if (!have_pap)                                                                                      \
+    hipLaunchKernelGGL((k_coldot<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.P, b.AP, \
+                       b.ld, b.pAp);                                                                 \
   hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 0, b.rz, b.rz_new, b.pAp,        \
                      b.alpha, b.beta, b.viol);                                                       \
   hipLaunchKernelGGL((k_pcg_update<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.Z, b.R, \

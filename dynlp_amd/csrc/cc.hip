@@ -2,8 +2,8 @@
 //
 // Hooking always links the larger root under the smaller one (compare-and-swap on the
 // larger root's parent), so every tree's root is its minimum vertex id and parent ids
-// strictly decrease along any path.  After all unions, full path compression gives each
-// vertex its component minimum; flags (root or not) are scanned so that a component's label
+// strictly decrease along any path.  After all unions a read-only pass gives each vertex its
+// root, the component minimum; flags (root or not) are scanned so that a component's label
 // is the rank of its minimum among all component minima, which is the labelling the
 // reference's hook/jump + unique produces (cc.py:12-23).
 #include "common.h"
@@ -83,19 +83,32 @@ __global__ __launch_bounds__(256) void k_cc_hook_csr(int n, const int32_t* __res
   }
 }
 
-__global__ void k_cc_compress(int n, int32_t* P, int32_t* __restrict__ flag) {
+// Root of x without path compression.  The final pass must not store into other vertices'
+// entries: a halving store by one thread could overwrite the root another thread has just
+// written for its own vertex, leaving a non-root there.
+__device__ __forceinline__ int uf_root(const int32_t* P, int x) {
+  int p = ld_relaxed(P + x);
+  while (p != x) {
+    x = p;
+    p = ld_relaxed(P + x);
+  }
+  return x;
+}
+
+// roots[i] = root of i (its component's minimum id), flag[i] = (i is a root).
+__global__ void k_cc_compress(int n, const int32_t* P, int32_t* __restrict__ roots, int32_t* __restrict__ flag) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
-    const int r = uf_find(P, i);
-    P[i] = r;
+    const int r = uf_root(P, i);
+    roots[i] = r;
     flag[i] = (r == i) ? 1 : 0;
   }
 }
 
-__global__ void k_cc_label(int n, const int32_t* __restrict__ P, const int32_t* __restrict__ rank,
-                           int32_t* __restrict__ labels) {
+// labels[i] = rank[roots[i]]; labels may alias roots (each thread reads then writes its own i).
+__global__ void k_cc_label(int n, const int32_t* roots, const int32_t* __restrict__ rank, int32_t* labels) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) labels[i] = rank[P[i]];
+  if (i < n) labels[i] = rank[roots[i]];
 }
 
 struct CCWork {
@@ -123,12 +136,11 @@ static int cc_carve(int n, void* ws, size_t ws_bytes, CCWork& w) {
 }
 
 static int cc_finish(int n, CCWork& w, int32_t* labels, int32_t* n_comp, hipStream_t st) {
-  hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(n)), dim3(kBlock), 0, st, n, w.parent, w.flag);
+  hipLaunchKernelGGL(k_cc_compress, dim3(grid_for(n)), dim3(kBlock), 0, st, n, w.parent, labels, w.flag);
   DLP_CHECK_LAUNCH();
   int rc = exclusive_scan_i32(n, w.flag, w.flag, n_comp, w.scan_ws, w.scan_bytes, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_cc_label, dim3(grid_for(n)), dim3(kBlock), 0, st, n, w.parent, w.flag,
-                     labels);
+  hipLaunchKernelGGL(k_cc_label, dim3(grid_for(n)), dim3(kBlock), 0, st, n, labels, w.flag, labels);
   DLP_CHECK_LAUNCH();
   return DLP_OK;
 }

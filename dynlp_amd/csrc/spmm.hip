@@ -88,9 +88,89 @@ __global__ __launch_bounds__(256) void k_row_operator_v4(
   }
 }
 
+// Grid-stride form of the FP32 operator that also accumulates dot[c] += X[r,c] * out[r,c] in
+// FP64 (the p^T A p column products of PCG, fused into the operator's epilogue).
+template <int U>
+__global__ __launch_bounds__(256) void k_row_operator_v4_dot(
+    int nrows, int C, int g_shift, int lc_shift, const int32_t* __restrict__ indptr,
+    const int32_t* __restrict__ rowlen, const int32_t* __restrict__ indices, const float* __restrict__ w,
+    const uint8_t* __restrict__ mask, const float* __restrict__ X, int64_t ldx, const float* __restrict__ a,
+    float b, float* __restrict__ out, int64_t ldo, double* __restrict__ dot) {
+  const Lanes L(g_shift, lc_shift);
+  __shared__ double sm[256];
+  for (int c = threadIdx.x; c < C; c += blockDim.x) sm[c] = 0.0;
+  double dacc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> g_shift;
+  const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> g_shift;
+  const int64_t wave_first = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) >> g_shift;
+  for (int64_t base = wave_first; base < nrows; base += groups) {
+    const int64_t i = base + (g0 - wave_first);
+    const bool valid = i < nrows;
+    const int r = valid ? (int)i : 0;
+    const bool live = valid && (mask == nullptr || mask[r] != 0);
+    int beg = 0, end = 0;
+    if (live) {
+      beg = indptr[r];
+      end = rowlen ? beg + rowlen[r] : indptr[r + 1];
+    }
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    row_gather_v4<U>(L, beg, end, indices, w, mask, X, ldx, C, acc);
+    for (int off = L.LC; off < L.G; off <<= 1) {
+      acc.x += __shfl_xor(acc.x, off, 64);
+      acc.y += __shfl_xor(acc.y, off, 64);
+      acc.z += __shfl_xor(acc.z, off, 64);
+      acc.w += __shfl_xor(acc.w, off, 64);
+    }
+    if (valid && L.e == 0 && 4 * L.l < C) {
+      const float ar = a ? a[r] : 0.f;
+      const float accs[4] = {acc.x, acc.y, acc.z, acc.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c = 4 * L.l + q;
+        if (c < C) {
+          const float xr = X[(int64_t)r * ldx + c];
+          const float y = b * accs[q] + ar * xr;
+          out[(int64_t)r * ldo + c] = y;
+          dacc[q] += (double)xr * (double)y;
+        }
+      }
+    }
+  }
+  for (int off = L.G; off < 64; off <<= 1) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dacc[q] += __shfl_xor(dacc[q], off, 64);
+  }
+  __syncthreads();
+  if ((threadIdx.x & 63) < L.G && L.e == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = 4 * L.l + q;
+      if (c < C) atomicAdd(&sm[c], dacc[q]);
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&dot[c], sm[c]);
+}
+
 Tuning& tuning() {
   static Tuning t;
   return t;
+}
+
+// out = a .* X + b * W_mask X over all rows, and dot[c] += sum_r X[r,c] * out[r,c] (dot not
+// cleared here).  FP32, 16-byte rows required; returns DLP_ERR_ARG when the layout does not fit.
+int row_operator_dot(int n, int C, const int32_t* indptr, const int32_t* rowlen, const int32_t* indices,
+                     const float* w, const uint8_t* mask, const float* X, int64_t ldx, const float* a, float b,
+                     float* out, int64_t ldo, double* dot, hipStream_t st) {
+  DLP_CHECK_ARG((ldx % 4) == 0 && (((uintptr_t)X) & 15) == 0 && C <= 256, "row_operator_dot: layout");
+  int lc = ceil_log2((C + 3) / 4);
+  if (lc > 6) lc = 6;
+  unsigned grid = grid_for((long long)n << 6);
+  if (grid > 4096) grid = 4096;
+  hipLaunchKernelGGL((k_row_operator_v4_dot<8>), dim3(grid), dim3(kBlock), 0, st, n, C, 6, lc, indptr, rowlen,
+                     indices, w, mask, X, ldx, a, b, out, ldo, dot);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
 }
 
 template <typename TW, typename TX, typename TO>

@@ -172,7 +172,9 @@ __device__ __forceinline__ void uf_union2(int32_t* P, int a, int b) {
   }
 }
 
-__global__ void k_ground_init(int n, int32_t* __restrict__ P, uint8_t* __restrict__ grounded) {
+__global__ void k_ground_init(int n, int32_t* __restrict__ P, uint8_t* __restrict__ grounded,
+                              const int32_t* __restrict__ need) {
+  if (need && *need == 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     P[i] = i;
@@ -183,7 +185,9 @@ __global__ void k_ground_init(int n, int32_t* __restrict__ P, uint8_t* __restric
 __global__ __launch_bounds__(256) void k_ground_hook(int n, const int32_t* __restrict__ indptr,
                                                      const int32_t* __restrict__ rowlen,
                                                      const int32_t* __restrict__ col,
-                                                     const uint8_t* __restrict__ labeled, int32_t* P) {
+                                                     const uint8_t* __restrict__ labeled, int32_t* P,
+                                                     const int32_t* __restrict__ need) {
+  if (need && *need == 0) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int r = (int)(t >> 3);
   const int lane = (int)(t & 7);
@@ -195,29 +199,115 @@ __global__ __launch_bounds__(256) void k_ground_hook(int n, const int32_t* __res
   }
 }
 
-__global__ void k_ground_roots(int n, int32_t* P, const uint8_t* __restrict__ labeled,
-                               const int32_t* __restrict__ labcnt, uint8_t* __restrict__ grounded) {
+// Roots without path compression (see cc.hip: a halving store in this pass could overwrite
+// another vertex's final root); roots[i] is written for unlabeled i only.
+__global__ void k_ground_roots(int n, const int32_t* P, const uint8_t* __restrict__ labeled,
+                               const int32_t* __restrict__ labcnt, uint8_t* __restrict__ grounded,
+                               int32_t* __restrict__ roots, const int32_t* __restrict__ need) {
+  if (need && *need == 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || labeled[i]) return;
-  const int r = uf_find2(P, i);
-  P[i] = r;
-  if (labcnt[i] > 0) grounded[r] = 1;
+  int x = i, p = ld_rlx(P + i);
+  while (p != x) {
+    x = p;
+    p = ld_rlx(P + x);
+  }
+  roots[i] = x;
+  if (labcnt[i] > 0) grounded[x] = 1;
 }
 
 // New solved flag; vertices that toggle are listed (for the neighbour pass) and stamped changed.
-__global__ void k_ground_toggle(int n, const int32_t* __restrict__ P, const uint8_t* __restrict__ labeled,
+__global__ void k_ground_toggle(int n, const int32_t* __restrict__ roots, const uint8_t* __restrict__ labeled,
                                 const uint8_t* __restrict__ grounded, uint8_t* __restrict__ solved,
                                 int32_t* __restrict__ toggled, int32_t* __restrict__ ntoggled,
                                 int32_t* __restrict__ stamp, int stampv, int32_t* __restrict__ rows,
-                                int32_t* __restrict__ nrows) {
+                                int32_t* __restrict__ nrows, const int32_t* __restrict__ need,
+                                int32_t* __restrict__ ninert) {
+  if (need && *need == 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint8_t now = (!labeled[i] && grounded[P[i]]) ? 1 : 0;
+  const uint8_t now = (!labeled[i] && grounded[roots[i]]) ? 1 : 0;
+  if (!labeled[i] && !now) atomicAdd(ninert, 1);
   if (now != solved[i]) {
     solved[i] = now;
     toggled[atomicAdd(ntoggled, 1)] = i;
     stamp_append(i, stamp, stampv, rows, nrows);
   }
+}
+
+// Exact local grounding test (no inert vertex before the batch): a component that loses its
+// labeled contact must contain an endpoint of a deleted edge, so it suffices that every
+// unlabeled deletion endpoint still reaches, through unlabeled vertices of the updated graph,
+// a vertex with a labeled neighbour.  One wavefront per endpoint runs a bounded BFS with its
+// queue and visited set in LDS; an endpoint that exhausts the budget (or any insertion while
+// inert vertices exist) requests the full connected-components pass (need[0] = 1).
+constexpr int kBallCap = 256;      // vertices a check may visit
+constexpr int kBallHash = 1024;    // open-addressing slots (power of two)
+
+__global__ __launch_bounds__(256) void k_ground_ball(int nd, const int32_t* __restrict__ du, const int32_t* __restrict__ dv,
+                                                     int ni, const int32_t* __restrict__ indptr,
+                                                     const int32_t* __restrict__ rowlen, const int32_t* __restrict__ col,
+                                                     const uint8_t* __restrict__ labeled,
+                                                     const int32_t* __restrict__ labcnt, int32_t* __restrict__ flags) {
+  __shared__ int32_t queue[4][kBallCap];
+  __shared__ int32_t hset[4][kBallHash];
+  __shared__ int32_t qlen[4];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    atomicAdd(flags + 2, 1);
+    if (ni > 0 && flags[1] > 0) flags[0] = 1;
+  }
+  const int64_t e = (int64_t)blockIdx.x * 4 + wave;
+  if (e >= 2 * (int64_t)nd) return;
+  const int x = e < nd ? du[e] : dv[e - nd];
+  if (labeled[x]) return;
+  if (labcnt[x] > 0) return;              // grounded directly
+  for (int t = lane; t < kBallHash; t += 64) hset[wave][t] = -1;
+  if (lane == 0) {
+    queue[wave][0] = x;
+    qlen[wave] = 1;
+    hset[wave][(x * 2654435761u) & (kBallHash - 1)] = x;
+  }
+  // wavefront-local LDS hand-off (each wave owns its own queue and set; waves may have left)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  bool found = false;
+  int head = 0;
+  while (!found && head < qlen[wave]) {
+    const int v = queue[wave][head++];
+    const int beg = indptr[v], end = beg + rowlen[v];
+    for (int j0 = beg; j0 < end && !found; j0 += 64) {
+      const int j = j0 + lane;
+      bool hit = false;
+      if (j < end) {
+        const int c = col[j];
+        if (c >= 0 && !labeled[c]) {
+          if (labcnt[c] > 0) {
+            hit = true;
+          } else {
+            unsigned hsh = (unsigned)(c * 2654435761u) & (kBallHash - 1);
+            for (int probe = 0; probe < kBallHash; ++probe) {
+              const int old = atomicCAS(&hset[wave][hsh], -1, c);
+              if (old == -1) {
+                const int q = atomicAdd(&qlen[wave], 1);
+                if (q < kBallCap) queue[wave][q] = c;
+                break;
+              }
+              if (old == c) break;
+              hsh = (hsh + 1) & (kBallHash - 1);
+            }
+          }
+        }
+      }
+      found = __ballot(hit) != 0ull;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (qlen[wave] > kBallCap) break;
+  }
+  if (!found && lane == 0) atomicExch(flags, 1);
 }
 
 __global__ void k_mark_neighbours_changed(const int32_t* __restrict__ list, const int32_t* __restrict__ count,
@@ -393,6 +483,157 @@ __device__ __forceinline__ void refresh_row(const RefreshArgs& A, const Lanes& L
   const int wl = threadIdx.x & 63;
   const unsigned long long gm = (L.G == 64 ? ~0ull : ((1ull << L.G) - 1)) << (wl & ~(L.G - 1));
   if (valid && L.lane == 0 && (bal & gm) && A.viol) A.viol[atomicAdd(A.nviol, 1)] = r;
+}
+
+// FP32 state with 16-byte rows (ld % 4 == 0): lane l of the LC = 2^lc_shift column lanes covers
+// columns 4l..4l+3 with double accumulators; LE = 64/LC edge lanes.  Labeled neighbours (rare)
+// add their weight to the rhs of the lane owning their class column.
+__device__ __forceinline__ void refresh_row_v4(const RefreshArgs& A, const Lanes& L, int64_t i, int nr,
+                                               float* __restrict__ Z, float* __restrict__ R, double (&rmax)[4]) {
+  const bool valid = i < nr;
+  const int r = valid ? (A.rows ? A.rows[i] : (int)i) : 0;
+  const bool live = valid && A.solved[r];
+  const int C = A.C1 - 1;
+  int beg = 0, end = 0;
+  if (live) {
+    beg = A.indptr[r];
+    end = beg + A.rowlen[r];
+  }
+  double acc[4] = {0.0, 0.0, 0.0, 0.0}, rhs[4] = {0.0, 0.0, 0.0, 0.0};
+  double s = 0.0, rs = 0.0;
+  const bool lane_on = 4 * L.l < A.C1;
+  for (int base = beg; base < end; base += L.G) {
+    const int j = base + L.lane;
+    int myv = 0, kind = 0, mycol = -1;
+    float myw = 0.f;
+    if (j < end) {
+      myv = A.col[j];
+      myw = A.w[j];
+      if (myv >= 0) {
+        if (A.labeled[myv]) {
+          kind = 1;
+          const int yl = A.y[myv];
+          mycol = A.K2 ? (yl == 1 ? 0 : -1) : yl;
+        } else if (A.solved[myv]) {
+          kind = 2;
+        }
+      }
+    }
+    if (kind == 1) s += myw;
+    if (kind == 2) rs += myw;
+    const int cnt = min(L.G, end - base);
+    for (int t0 = 0; t0 < cnt; t0 += 4 * L.LE) {
+      int vv[4], kk[4], cc[4];
+      float ww[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = t0 + u * L.LE + L.e;
+        const int src = t & (L.G - 1);
+        vv[u] = __shfl(myv, src, L.G);
+        kk[u] = __shfl(kind, src, L.G);
+        cc[u] = __shfl(mycol, src, L.G);
+        ww[u] = __shfl(myw, src, L.G);
+        if (t >= cnt) kk[u] = 0;
+      }
+      float4 xv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float4* zr = reinterpret_cast<const float4*>(Z + (int64_t)vv[u] * A.ld) + L.l;
+        xv[u] = (kk[u] == 2 && lane_on) ? *zr : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double wd = (double)ww[u];
+        acc[0] += wd * (double)xv[u].x;
+        acc[1] += wd * (double)xv[u].y;
+        acc[2] += wd * (double)xv[u].z;
+        acc[3] += wd * (double)xv[u].w;
+        if (kk[u] == 1 && (cc[u] >> 2) == L.l) rhs[cc[u] & 3] += wd;
+      }
+    }
+  }
+  for (int off = L.LC; off < L.G; off <<= 1) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      acc[q] += __shfl_xor(acc[q], off, 64);
+      rhs[q] += __shfl_xor(rhs[q], off, 64);
+    }
+  }
+  for (int off = 1; off < L.G; off <<= 1) {
+    s += __shfl_xor(s, off, 64);
+    rs += __shfl_xor(rs, off, 64);
+  }
+  bool viol = false;
+  if (valid && L.e == 0 && lane_on) {
+    const int64_t zo = (int64_t)r * A.ld;
+    if (!live) {
+      if (L.l == 0) {
+        A.diag64[r] = 1.0;
+        A.diag32[r] = 1.f;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c = 4 * L.l + q;
+        if (c < A.C1) {
+          Z[zo + c] = (c == C) ? 1.f : 0.f;
+          R[zo + c] = 0.f;
+        }
+      }
+    } else {
+      const double dg = s + A.eta + rs;
+      if (L.l == 0) {
+        A.diag64[r] = dg;
+        A.diag32[r] = (float)dg;
+      }
+      const float4 zr = *(reinterpret_cast<const float4*>(Z + zo) + L.l);
+      const float zv[4] = {zr.x, zr.y, zr.z, zr.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c = 4 * L.l + q;
+        if (c < A.C1) {
+          const double rh = c == C ? dg : rhs[q] + A.eta * A.prior;
+          const double res = rh - dg * (double)zv[q] + acc[q];
+          R[zo + c] = (float)res;
+          const double ratio = fabs(res) / dg;
+          viol |= ratio > A.eps[c];
+          rmax[q] = fmax(rmax[q], ratio);
+        }
+      }
+    }
+  }
+  const unsigned long long bal = __ballot(viol);
+  const int wl = threadIdx.x & 63;
+  const unsigned long long gm = (L.G == 64 ? ~0ull : ((1ull << L.G) - 1)) << (wl & ~(L.G - 1));
+  if (valid && L.lane == 0 && (bal & gm) && A.viol) A.viol[atomicAdd(A.nviol, 1)] = r;
+}
+
+__global__ __launch_bounds__(256) void k_refresh_v4(RefreshArgs A, int g_shift, int lc_shift, float* __restrict__ Z,
+                                                    float* __restrict__ R) {
+  const Lanes L(g_shift, lc_shift);
+  const int nr = A.rows_count ? *A.rows_count : A.nrows;
+  const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> g_shift;
+  const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> g_shift;
+  const int64_t wave_first = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) >> g_shift;
+  double rmax[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t base = wave_first; base < nr; base += groups)
+    refresh_row_v4(A, L, base + (g0 - wave_first), nr, Z, R, rmax);
+  if (!A.rho) return;
+  __shared__ double sm[256];
+  for (int c = threadIdx.x; c < A.C1; c += blockDim.x) sm[c] = 0.0;
+  for (int off = L.G; off < 64; off <<= 1) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) rmax[q] = fmax(rmax[q], __shfl_xor(rmax[q], off, 64));
+  }
+  __syncthreads();
+  if ((threadIdx.x & 63) < L.G && L.e == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = 4 * L.l + q;
+      if (c < A.C1) atomic_max_nonneg(&sm[c], rmax[q]);
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < A.C1; c += blockDim.x) atomic_max_nonneg(A.rho + c, sm[c]);
 }
 
 // Grid-stride over the listed rows in wave-uniform steps (the row count may live on the device).
@@ -686,6 +927,7 @@ struct LP {
   int32_t* overflow = nullptr;
   int32_t* toggled = nullptr;
   int32_t* parent = nullptr;
+  int32_t* roots = nullptr;
   uint8_t* grounded = nullptr;
   int32_t* viol = nullptr;
   double* eps64 = nullptr;
@@ -713,6 +955,7 @@ struct LP {
   double pcg_ms = 0.0;
   int64_t pcg_iters = 0;
   double* bk = nullptr;         // block PCG k x k matrices: G, PAP, Gn, alpha, beta
+  int32_t* gflags = nullptr;    // [0] need full CC, [1] inert unlabeled vertices, [2] checks run
   hipGraphExec_t bpcg_exec = nullptr;
   int bpcg_graph_iters = 0;
   double push_ms = 0.0;         // summed duration of push-round launches (events)
@@ -728,7 +971,7 @@ int dev_alloc(T** p, size_t count) {
 }
 
 int lp_free(LP* h) {
-  void* ptrs[] = {h->stamp, h->rows, h->cnt, h->overflow, h->toggled, h->parent, h->grounded,
+  void* ptrs[] = {h->stamp, h->rows, h->cnt, h->overflow, h->toggled, h->parent, h->roots, h->grounded,
                   h->viol, h->eps64, h->rho, h->mark, h->dsnap, h->cand64, h->scan_ws,
                   h->S.ctag, h->S.list[0],
                   h->S.list[1], h->S.list[2], h->S.count, h->S.rnd, h->S.done, h->S.btag,
@@ -745,6 +988,7 @@ int lp_free(LP* h) {
   if (h->pcg_viol) (void)hipFree(h->pcg_viol);
   if (h->pcg_exec) (void)hipGraphExecDestroy(h->pcg_exec);
   if (h->bk) (void)hipFree(h->bk);
+  if (h->gflags) (void)hipFree(h->gflags);
   if (h->bpcg_exec) (void)hipGraphExecDestroy(h->bpcg_exec);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   delete h;
@@ -806,14 +1050,14 @@ extern "C" int dlp_lp_create(const dlp_lp_desc* d, void** handle) {
   int rc = DLP_OK;
 #define A_(p, c) if (!rc) rc = dev_alloc(&(p), (c))
   A_(h->stamp, n); A_(h->rows, n); A_(h->cnt, CNT_N); A_(h->overflow, 1 << 20);
-  A_(h->toggled, n); A_(h->parent, n); A_(h->grounded, n); A_(h->viol, n);
+  A_(h->toggled, n); A_(h->parent, n); A_(h->roots, n); A_(h->grounded, n); A_(h->viol, n);
   A_(h->eps64, d->C1); A_(h->rho, d->C1); A_(h->mark, n); A_(h->cand64, n);
   A_(h->S.ctag, n); A_(h->S.list[0], n); A_(h->S.list[1], n); A_(h->S.list[2], n);
   A_(h->S.count, 3); A_(h->S.rnd, 2); A_(h->S.done, 1); A_(h->S.btag, n); A_(h->S.blist, n);
   A_(h->S.bcount, 1); A_(h->S.edges, 1); A_(h->S.stats, 4);
   int32_t* bst = nullptr;
   float* eps32 = nullptr;
-  A_(bst, 1); A_(eps32, d->C1);
+  A_(bst, 1); A_(eps32, d->C1); A_(h->gflags, 4);
 #undef A_
   h->S.bstamp = bst;
   h->S.eps = eps32;
@@ -839,6 +1083,7 @@ extern "C" int dlp_lp_create(const dlp_lp_desc* d, void** handle) {
   (void)hipMemset(h->S.edges, 0, sizeof(unsigned long long));
   (void)hipMemset(h->S.stats, 0, 4 * sizeof(unsigned long long));
   (void)hipMemset(h->mark, 0, n);
+  (void)hipMemset(h->gflags, 0, 4 * sizeof(int32_t));
   h->S.n = n;
   h->S.C1 = d->C1;
   h->S.ld = d->ld;
@@ -1046,28 +1291,68 @@ extern "C" int dlp_lp_regrow_commit(void* handle, const int32_t* new_indptr, int
   return DLP_OK;
 }
 
-extern "C" int dlp_lp_ground(void* handle, void* stream) {
-  DLP_CHECK_ARG(handle, "lp_ground: null handle");
-  LP* h = (LP*)handle;
+__global__ void k_ground_prep(int32_t* flags, int force) {
+  if (threadIdx.x == 0) {
+    flags[0] = force ? 1 : 0;
+  }
+}
+__global__ void k_ground_inert_reset(int32_t* flags) {
+  if (threadIdx.x == 0 && flags[0]) flags[1] = 0;
+}
+
+static int ground_impl(LP* h, int force, int nd, const int32_t* du, const int32_t* dv, int ni, hipStream_t st) {
   const int n = h->d.n;
-  hipStream_t st = (hipStream_t)stream;
+  int32_t* need = h->gflags;
   DLP_CHECK_HIP(hipMemsetAsync(h->cnt + CNT_TOGGLED, 0, sizeof(int32_t), st));
-  hipLaunchKernelGGL(k_ground_init, dim3(grid_for(n)), dim3(kBlock), 0, st, n, h->parent, h->grounded);
+  hipLaunchKernelGGL(k_ground_prep, dim3(1), dim3(64), 0, st, h->gflags, force);
+  if (!force && nd > 0) {
+    hipLaunchKernelGGL(k_ground_ball, dim3((unsigned)((2LL * nd + 3) / 4)), dim3(kBlock), 0, st, nd, du, dv, ni,
+                       h->d.indptr, h->d.rowlen, h->d.col, h->d.labeled, h->d.labcnt, h->gflags);
+  } else if (!force) {
+    // insertions only: they matter when inert vertices exist
+    hipLaunchKernelGGL(k_ground_ball, dim3(1), dim3(kBlock), 0, st, 0, du, dv, ni, h->d.indptr, h->d.rowlen,
+                       h->d.col, h->d.labeled, h->d.labcnt, h->gflags);
+  }
+  DLP_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_ground_inert_reset, dim3(1), dim3(64), 0, st, h->gflags);
+  hipLaunchKernelGGL(k_ground_init, dim3(grid_for(n)), dim3(kBlock), 0, st, n, h->parent, h->grounded, need);
   DLP_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_ground_hook, dim3(grid_for((long long)n * 8)), dim3(kBlock), 0, st, n, h->d.indptr,
-                     h->d.rowlen, h->d.col, h->d.labeled, h->parent);
+                     h->d.rowlen, h->d.col, h->d.labeled, h->parent, need);
   DLP_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_ground_roots, dim3(grid_for(n)), dim3(kBlock), 0, st, n, h->parent, h->d.labeled,
-                     h->d.labcnt, h->grounded);
+                     h->d.labcnt, h->grounded, h->roots, need);
   DLP_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_ground_toggle, dim3(grid_for(n)), dim3(kBlock), 0, st, n, h->parent, h->d.labeled,
+  hipLaunchKernelGGL(k_ground_toggle, dim3(grid_for(n)), dim3(kBlock), 0, st, n, h->roots, h->d.labeled,
                      h->grounded, h->d.solved, h->toggled, h->cnt + CNT_TOGGLED, h->stamp, h->stampv,
-                     h->rows, h->cnt + CNT_ROWS);
+                     h->rows, h->cnt + CNT_ROWS, need, h->gflags + 1);
   DLP_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_mark_neighbours_changed, dim3(grid_for((long long)n * 8)), dim3(kBlock), 0, st,
                      h->toggled, h->cnt + CNT_TOGGLED, h->d.indptr, h->d.rowlen, h->d.col, h->stamp,
                      h->stampv, h->rows, h->cnt + CNT_ROWS);
   DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+extern "C" int dlp_lp_ground(void* handle, void* stream) {
+  DLP_CHECK_ARG(handle, "lp_ground: null handle");
+  return ground_impl((LP*)handle, 1, 0, nullptr, nullptr, 0, (hipStream_t)stream);
+}
+
+// Incremental grounding after an apply of nd deletions (du, dv: that call's arrays) and ni
+// insertions: the exact local test decides on the device whether the full pass runs.
+extern "C" int dlp_lp_ground_incremental(void* handle, int nd, const int32_t* du, const int32_t* dv, int ni,
+                                         void* stream) {
+  DLP_CHECK_ARG(handle && nd >= 0 && ni >= 0 && (nd == 0 || (du && dv)), "lp_ground_incremental: bad argument");
+  return ground_impl((LP*)handle, 0, nd, du, dv, ni, (hipStream_t)stream);
+}
+
+// [0] full pass requested by the last grounding, [1] inert unlabeled vertices, [2] local checks run.
+extern "C" int dlp_lp_ground_flags(void* handle, int32_t* out, void* stream) {
+  DLP_CHECK_ARG(handle && out, "lp_ground_flags: null argument");
+  LP* h = (LP*)handle;
+  DLP_CHECK_HIP(hipMemcpyAsync(out, h->gflags, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, (hipStream_t)stream));
+  DLP_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream));
   return DLP_OK;
 }
 
@@ -1080,7 +1365,17 @@ extern "C" int dlp_lp_refresh(void* handle, int which, int prec, void* stream) {
   DLP_CHECK_HIP(hipMemsetAsync(h->rho, 0, sizeof(double) * h->d.C1, st));
   RefreshArgs A = refresh_args(h, which);
   const RowMap& m = h->map;
-  const unsigned grid = persistent_grid(h->d.n, m.g_shift);
+  // listed rows: the changed list is O(batch); the touched list can reach n
+  const unsigned cap = which == 0 ? 512u : (which == 1 ? 2048u : 4096u);
+  unsigned grid = persistent_grid(h->d.n, m.g_shift);
+  if (grid > cap) grid = cap;
+  if (prec == 0 && (h->d.ld % 4) == 0 && tuning().vec4 && h->d.C1 <= 256) {
+    int lc = ceil_log2((h->d.C1 + 3) / 4);
+    if (lc > 6) lc = 6;
+    hipLaunchKernelGGL(k_refresh_v4, dim3(grid), dim3(kBlock), 0, st, A, 6, lc, h->d.Z32, h->d.R32);
+    DLP_CHECK_LAUNCH();
+    return DLP_OK;
+  }
 #define K(CPLV)                                                                                    \
   if (prec == 0)                                                                                   \
     hipLaunchKernelGGL((k_refresh<float, float, CPLV>), dim3(grid), dim3(kBlock), 0, st, A, m.g_shift, \
@@ -1294,7 +1589,13 @@ static PcgBufs pcg_bufs(LP* h) {
 }
 
 static int pcg_iteration(LP* h, const PcgBufs& b, hipStream_t st) {
-  // AP = diag*P - W_UU P
+  // AP = diag*P - W_UU P, with p^T A p accumulated in the same pass when the layout allows
+  if ((b.ld % 4) == 0 && tuning().vec4) {
+    int rc = row_operator_dot(b.n, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.solved, b.P, b.ld,
+                              b.diag, -1.f, b.AP, b.ld, b.pAp, st);
+    if (rc) return rc;
+    return pcg_finish_iteration(b, st, true);
+  }
   int rc = dlp_row_operator(b.n, nullptr, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, DLP_F32,
                             h->d.solved, b.P, b.ld, DLP_F32, b.diag, -1.0, nullptr, 0, b.AP, b.ld, DLP_F32,
                             0, 20.0, st);
@@ -1409,14 +1710,21 @@ extern "C" int dlp_lp_time_operator(void* handle, int reps, double* ms_out, void
   DLP_CHECK_ARG(h->P && h->AP, "lp_time_operator: run dlp_lp_pcg first");
   hipStream_t st = (hipStream_t)stream;
   const PcgBufs b = pcg_bufs(h);
-  int rc = dlp_row_operator(b.n, nullptr, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, DLP_F32,
-                            h->d.solved, b.P, b.ld, DLP_F32, b.diag, -1.0, nullptr, 0, b.AP, b.ld, DLP_F32, 0,
-                            20.0, st);
+  // the launch the global solve makes: the 16-byte-row operator with the fused p^T A p
+  // epilogue when the layout allows it, else the plain row operator
+  const bool fused = (b.ld % 4) == 0 && tuning().vec4;
+  auto launch = [&]() -> int {
+    if (fused)
+      return row_operator_dot(b.n, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.solved, b.P, b.ld,
+                              b.diag, -1.f, b.AP, b.ld, b.pAp, st);
+    return dlp_row_operator(b.n, nullptr, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, DLP_F32, h->d.solved,
+                            b.P, b.ld, DLP_F32, b.diag, -1.0, nullptr, 0, b.AP, b.ld, DLP_F32, 0, 20.0, st);
+  };
+  int rc = launch();
   if (rc) return rc;
   DLP_CHECK_HIP(hipEventRecord(h->ev0, st));
   for (int i = 0; i < reps; ++i) {
-    rc = dlp_row_operator(b.n, nullptr, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, DLP_F32, h->d.solved,
-                          b.P, b.ld, DLP_F32, b.diag, -1.0, nullptr, 0, b.AP, b.ld, DLP_F32, 0, 20.0, st);
+    rc = launch();
     if (rc) return rc;
   }
   DLP_CHECK_HIP(hipEventRecord(h->ev1, st));
@@ -1425,14 +1733,6 @@ extern "C" int dlp_lp_time_operator(void* handle, int reps, double* ms_out, void
   DLP_CHECK_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
   *ms_out = ms / reps;
   return DLP_OK;
-}
-
-// Live store sizes: out[0] = sum of rowlen (stored entries), out[1] = capacity.
-__global__ void k_sum_i32(int n, const int32_t* __restrict__ a, unsigned long long* __restrict__ out) {
-  unsigned long long s = 0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += (unsigned)a[i];
-  s = dlp::wave_sum_u64(s);
-  if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, s);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -73,7 +73,7 @@ class EdgeStreamLP:
     def __init__(self, g: Graph, eps=1e-6, eps_h=0.05, fp64=False, eta=0.0, slack_min=4,
                  slack_frac=0.25, push_scale=0.9, rounds_per_launch=4, use_graph=True, max_rounds=200_000,
                  push_frac=0.05, push_max_rows=4096, push_budget=1.0, pcg_iters_per_launch=4,
-                 pcg_max_iter=20_000, pcg_mode="column"):
+                 pcg_max_iter=20_000, pcg_mode="column", incremental_ground=True):
         self.g, self.n, self.K = g, g.n, g.K
         self.C = 1 if g.K == 2 else g.K
         self.C1 = self.C + 1
@@ -85,6 +85,7 @@ class EdgeStreamLP:
         if pcg_mode not in ("column", "block"):
             raise ValueError(f"pcg_mode {pcg_mode!r}: 'column' or 'block'")
         self.pcg_mode = pcg_mode
+        self.incremental_ground = bool(incremental_ground)
         dev = device()
         ip, ix, d = g.csr()
         sp_, rl, col, w = slack_layout(ip, ix, d, slack_min, slack_frac)
@@ -288,7 +289,11 @@ class EdgeStreamLP:
         _lib.call("dlp_lp_begin_batch", self.h, stream())
         _lib.call("dlp_lp_apply", self.h, int(du.shape[0]), ptr(du), ptr(dv), int(iu.shape[0]), ptr(iu), ptr(iv),
                   ptr(iw), stream())
-        _lib.call("dlp_lp_ground", self.h, stream())
+        if self.incremental_ground:
+            _lib.call("dlp_lp_ground_incremental", self.h, int(du.shape[0]), ptr(du), ptr(dv), int(iu.shape[0]),
+                      stream())
+        else:
+            _lib.call("dlp_lp_ground", self.h, stream())
         prec = 1 if self.fp64 else 0
         _lib.call("dlp_lp_refresh", self.h, 0, prec, stream())
         s = self._repair(st)
@@ -325,6 +330,11 @@ class EdgeStreamLP:
         ms = ct.c_double(0.0)
         _lib.call("dlp_lp_time_operator", self.h, int(reps), ct.byref(ms), stream())
         return float(ms.value)
+
+    def ground_flags(self):
+        out = np.zeros(3, dtype=np.int32)
+        _lib.call("dlp_lp_ground_flags", self.h, out.ctypes.data, stream())
+        return dict(full_pass=int(out[0]), inert=int(out[1]), calls=int(out[2]))
 
     def stored_entries(self):
         return int(self.rowlen.sum().item())

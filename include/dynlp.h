@@ -229,6 +229,17 @@ int dlp_lp_regrow_commit(void* handle, const int32_t* new_indptr, int32_t* new_c
  * with a labeled neighbour (stream.py:148-158, replacing the ball/full ground checks of
  * stream.py:237-294); toggled vertices and their neighbours join the changed-row list. */
 int dlp_lp_ground(void* handle, void* stream);
+/* Incremental grounding after an apply (du, dv: its deletion arrays; ni its insertion count):
+ * when no inert vertex existed, a component can only lose its labeled contact if it contains an
+ * endpoint of a deleted edge, so a bounded BFS from every unlabeled deletion endpoint to a
+ * vertex with a labeled neighbour decides it exactly; otherwise (or when a BFS exhausts its
+ * budget, or insertions meet inert vertices) the full pass of dlp_lp_ground runs, decided on
+ * the device.  Replaces the ball check of stream.py:237-273 and the full check :275-280. */
+int dlp_lp_ground_incremental(void* handle, int nd, const int32_t* du, const int32_t* dv, int ni,
+                              void* stream);
+/* out[0] = full pass requested by the last grounding, out[1] = inert unlabeled vertices (after
+ * the last full pass), out[2] = grounding calls (host copy; synchronises). */
+int dlp_lp_ground_flags(void* handle, int32_t* out, void* stream);
 /* Row refresh from scratch: s, diag, rhs and R = rhs - diag*Z + W_UU Z in FP64 for the rows of
  * `which` (0 changed list, 1 rows touched by pushes in this batch, 2 all rows); inert rows
  * parked (diag 1, F 0, h 1, R 0).  prec 0: Z32 -> R32 (rounded), prec 1: Z64 -> R64.

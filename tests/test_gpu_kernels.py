@@ -209,3 +209,17 @@ def test_cholesky_solve_and_deflation():
     keep = np.arange(k) != 7
     assert np.all(Sh[7] == 0)
     np.testing.assert_allclose(Sh[keep], np.linalg.solve(A[np.ix_(keep, keep)], B[keep]), rtol=1e-9, atol=1e-12)
+
+
+def test_cc_labels_stable_over_repeats():
+    """Regression for the final-pass race (a halving store overwriting another vertex's root):
+    repeated runs on larger random inputs must all give the canonical labels."""
+    from dynlp_amd.cc import connected_components
+    rs = np.random.default_rng(11)
+    n, m = 200_000, 150_000
+    r = rs.integers(0, n, m).astype(np.int32)
+    c = rs.integers(0, n, m).astype(np.int32)
+    ref, ncr = O.connected_components(n, r, c)
+    for _ in range(10):
+        lab, nc = connected_components(n, r, c)
+        assert nc == ncr and np.array_equal(lab.cpu().numpy(), ref)
