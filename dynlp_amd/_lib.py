@@ -18,6 +18,7 @@ P = ct.c_void_p
 I32 = ct.c_int32
 I64 = ct.c_int64
 DBL = ct.c_double
+FLT = ct.c_float
 SZ = ct.c_size_t
 
 # name -> (restype, argtypes)
@@ -63,6 +64,7 @@ _SIGS = {
     "dlp_chol_solve": (ct.c_int, [I32, P, P, P, DBL, P]),
     "dlp_lp_time_operator": (ct.c_int, [P, I32, P, P]),
     "dlp_tune_set": (ct.c_int, [I32, I32]),
+    "dlp_row_operator_dot": (ct.c_int, [I32, I32, P, P, P, P, P, P, I64, P, FLT, P, I64, P, P]),
     "dlp_coldot": (ct.c_int, [I32, I32, P, I64, P, I64, P, P, I32, P]),
     "dlp_colop": (ct.c_int, [I32, I32, I32, DBL, P, P, I64, P, P, I64, I32, P]),
     "dlp_amg_match": (ct.c_int, [I32, P, P, P, I32, I32, P, P, P, P, P, SZ, P, P]),

@@ -19,6 +19,9 @@ struct PcgBufs {
   double* alpha;
   double* beta;
   float* viol;
+  // [0] done (set once the violation is <= 1: every later pass of a captured group returns at
+  // once), [1] iterations completed; null disables the device-side exit
+  int32_t* ctl;
 };
 
 // P = R/diag, rz = sum R*R/diag, pAp = 0.
@@ -30,6 +33,9 @@ int pcg_finish_iteration(const PcgBufs& b, hipStream_t st, bool have_pap = false
 // Operator with the fused p^T A p epilogue (spmm.hip).
 int row_operator_dot(int n, int C, const int32_t* indptr, const int32_t* rowlen, const int32_t* indices,
                      const float* w, const uint8_t* mask, const float* X, int64_t ldx, const float* a, float b,
-                     float* out, int64_t ldo, double* dot, hipStream_t st);
+                     float* out, int64_t ldo, double* dot, hipStream_t st, const int32_t* skip = nullptr);
+
+// Blocks of a 256-thread kernel that are resident on the current device at once (spmm.hip).
+int resident_grid(const void* kernel);
 
 }  // namespace dlp

@@ -4,6 +4,7 @@
 // and rows over the grid, so each row is one coalesced access.  Column reductions accumulate
 // in FP64: per lane, then per workgroup in LDS, then one FP64 atomic per column per workgroup.
 #include "blas.h"
+#include "rowgather.h"
 
 namespace dlp {
 
@@ -64,7 +65,9 @@ __global__ __launch_bounds__(256) void k_pcg_update(int n, int C, int lc_shift, 
                                                     const double* __restrict__ alpha,
                                                     const float* __restrict__ diag,
                                                     const float* __restrict__ eps,
-                                                    double* __restrict__ rz_new, float* __restrict__ viol) {
+                                                    double* __restrict__ rz_new, float* __restrict__ viol,
+                                                    const int32_t* __restrict__ skip) {
+  if (skip && *skip) return;
   __shared__ double sm[kMaxCols];
   __shared__ float vmax;
   for (int c = threadIdx.x; c < C; c += blockDim.x) sm[c] = 0.0;
@@ -115,7 +118,9 @@ template <int CPL>
 __global__ __launch_bounds__(256) void k_pcg_dir(int n, int C, int lc_shift, float* __restrict__ P,
                                                  const float* __restrict__ R, int64_t ld,
                                                  const float* __restrict__ diag,
-                                                 const double* __restrict__ beta, int first) {
+                                                 const double* __restrict__ beta, int first,
+                                                 const int32_t* __restrict__ skip) {
+  if (skip && *skip) return;
   const int LC = 1 << lc_shift;
   const int l = threadIdx.x & (LC - 1);
   float b[CPL];
@@ -136,12 +141,156 @@ __global__ __launch_bounds__(256) void k_pcg_dir(int n, int C, int lc_shift, flo
   }
 }
 
+// ---- Flat 16-byte passes (ld % 4 == 0, 16-byte aligned blocks) ---------------------------
+// The n x ld block is a flat array of n*Q4 float4 (Q4 = ld/4).  A workgroup uses Q4 * (256/Q4)
+// of its 256 threads and the grid stride is a multiple of Q4, so every thread keeps one column quad
+// (threadIdx % Q4) for the whole pass, consecutive threads read consecutive 16-byte words, and
+// no lane idles on a power-of-two column layout.  Quad components at columns >= C are left
+// unchanged and excluded from the reductions.
+
+// pcg start: P = R/diag, rz[c] += sum R*R/diag.
+__global__ __launch_bounds__(256) void k_pcg_start4(int n, int Q4, int C, float4* __restrict__ P,
+                                                    const float4* __restrict__ R, const float* __restrict__ diag,
+                                                    double* __restrict__ rz) {
+  __shared__ double sm[kMaxCols + 4];
+  for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) sm[c] = 0.0;
+  __syncthreads();
+  const Quad Qd = quad_of(Q4, C);
+  const int64_t nq = (int64_t)n * Q4;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (Qd.active && Qd.r0 < n) {
+    int64_t r = Qd.r0;
+    for (int64_t i = Qd.i0; i < nq; i += Qd.step, r += Qd.rstep) {
+      const float inv = 1.f / diag[r];
+      const float4 rv = R[i];
+      float4 p = P[i];
+      p.x = Qd.cnt > 0 ? rv.x * inv : p.x;
+      p.y = Qd.cnt > 1 ? rv.y * inv : p.y;
+      p.z = Qd.cnt > 2 ? rv.z * inv : p.z;
+      p.w = Qd.cnt > 3 ? rv.w * inv : p.w;
+      P[i] = p;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (j < Qd.cnt) acc[j] += (double)quad_comp(rv, j) * (double)quad_comp(p, j);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if (j < Qd.cnt) lds_add(&sm[4 * Qd.q4 + j], acc[j]);
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&rz[c], sm[c]);
+}
+
+// Z += a P, R -= a AP; rz_new[c] += sum R*R/diag; viol = max |R|/(eps_c*diag).
+__global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float4* __restrict__ Z,
+                                                     float4* __restrict__ R, const float4* __restrict__ P,
+                                                     const float4* __restrict__ AP,
+                                                     const double* __restrict__ alpha,
+                                                     const float* __restrict__ diag, const float* __restrict__ eps,
+                                                     double* __restrict__ rz_new, float* __restrict__ viol,
+                                                     const int32_t* __restrict__ skip) {
+  if (skip && *skip) return;
+  __shared__ double sm[kMaxCols + 4];
+  __shared__ float vmax;
+  for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) sm[c] = 0.0;
+  if (threadIdx.x == 0) vmax = 0.f;
+  __syncthreads();
+  const Quad Qd = quad_of(Q4, C);
+  float a[4], ie[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int c = 4 * Qd.q4 + j;
+    a[j] = j < Qd.cnt ? (float)alpha[c] : 0.f;
+    ie[j] = j < Qd.cnt ? 1.f / eps[c] : 0.f;
+  }
+  const int64_t nq = (int64_t)n * Q4;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  float vm = 0.f;
+  if (Qd.active && Qd.r0 < n) {
+    int64_t r = Qd.r0;
+    for (int64_t i = Qd.i0; i < nq; i += Qd.step, r += Qd.rstep) {
+      const float inv = 1.f / diag[r];
+      float4 z = Z[i], rv = R[i];
+      const float4 p = P[i], ap = AP[i];
+      if (Qd.cnt > 0) { z.x += a[0] * p.x; rv.x -= a[0] * ap.x; }
+      if (Qd.cnt > 1) { z.y += a[1] * p.y; rv.y -= a[1] * ap.y; }
+      if (Qd.cnt > 2) { z.z += a[2] * p.z; rv.z -= a[2] * ap.z; }
+      if (Qd.cnt > 3) { z.w += a[3] * p.w; rv.w -= a[3] * ap.w; }
+      Z[i] = z;
+      R[i] = rv;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (j < Qd.cnt) {
+          const float x = quad_comp(rv, j);
+          acc[j] += (double)x * (double)x * (double)inv;
+          vm = fmaxf(vm, fabsf(x) * inv * ie[j]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if (j < Qd.cnt) lds_add(&sm[4 * Qd.q4 + j], acc[j]);
+  for (int off = 32; off > 0; off >>= 1) vm = fmaxf(vm, __shfl_xor(vm, off, 64));
+  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&vmax, vm);
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&rz_new[c], sm[c]);
+  if (threadIdx.x == 0) atomic_max_nonneg(viol, vmax);
+}
+
+// P = R/diag + beta P.
+__global__ __launch_bounds__(256) void k_pcg_dir4(int n, int Q4, int C, float4* __restrict__ P,
+                                                  const float4* __restrict__ R, const float* __restrict__ diag,
+                                                  const double* __restrict__ beta, const int32_t* __restrict__ skip) {
+  if (skip && *skip) return;
+  const Quad Qd = quad_of(Q4, C);
+  float b[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) b[j] = j < Qd.cnt ? (float)beta[4 * Qd.q4 + j] : 0.f;
+  const int64_t nq = (int64_t)n * Q4;
+  if (!Qd.active || Qd.r0 >= n) return;
+  int64_t r = Qd.r0;
+  for (int64_t i = Qd.i0; i < nq; i += Qd.step, r += Qd.rstep) {
+    const float inv = 1.f / diag[r];
+    const float4 rv = R[i];
+    float4 p = P[i];
+    if (Qd.cnt > 0) p.x = rv.x * inv + b[0] * p.x;
+    if (Qd.cnt > 1) p.y = rv.y * inv + b[1] * p.y;
+    if (Qd.cnt > 2) p.z = rv.z * inv + b[2] * p.z;
+    if (Qd.cnt > 3) p.w = rv.w * inv + b[3] * p.w;
+    P[i] = p;
+  }
+}
+
+struct FlatLaunch {
+  bool ok;
+  int Q4;
+  unsigned block, grid;
+};
+
+static FlatLaunch flat_launch(const PcgBufs& b) {
+  FlatLaunch f{false, 0, 0, 0};
+  const uintptr_t al = (uintptr_t)b.Z | (uintptr_t)b.R | (uintptr_t)b.P | (uintptr_t)b.AP;
+  if ((b.ld % 4) != 0 || (al & 15) != 0 || b.ld / 4 > 64 || !tuning().vec4) return f;
+  f.Q4 = (int)(b.ld / 4);
+  f.block = kBlock;
+  const long long used = f.Q4 * (kBlock / f.Q4);
+  const long long nq = (long long)b.n * f.Q4;
+  long long g = (nq + used - 1) / used;
+  if (g > 2048) g = 2048;
+  f.grid = (unsigned)(g < 1 ? 1 : g);
+  f.ok = true;
+  return f;
+}
+
 // Scalars of one iteration (one workgroup): mode 0: alpha = rz/pAp (0 where pAp <= 0), then
 // clear rz_new and viol for the update pass; mode 1: beta = rz_new/rz (0 where rz <= 0),
 // rz = rz_new, then clear pAp for the next iteration.
 __global__ void k_pcg_scalars(int C, int mode, double* __restrict__ rz, double* __restrict__ rz_new,
                               double* __restrict__ pAp, double* __restrict__ alpha,
-                              double* __restrict__ beta, float* __restrict__ viol) {
+                              double* __restrict__ beta, float* __restrict__ viol, int32_t* ctl) {
+  // every thread reads the flag before thread 0 may set it below (after the barrier)
+  if (ctl && ctl[0]) return;
   for (int c = threadIdx.x; c < C; c += blockDim.x) {
     if (mode == 0) {
       const double p = pAp[c];
@@ -155,6 +304,13 @@ __global__ void k_pcg_scalars(int C, int mode, double* __restrict__ rz, double* 
     }
   }
   if (mode == 0 && threadIdx.x == 0) *viol = 0.f;
+  if (mode == 1 && ctl) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      ctl[1] += 1;
+      if (*viol <= 1.f) ctl[0] = 1;
+    }
+  }
 }
 
 // Clears n doubles (a fill kernel, so it can sit in a captured graph).
@@ -179,9 +335,17 @@ int pcg_start(const PcgBufs& b, hipStream_t st) {
   const unsigned g = pass_grid(b.n, m.lc_shift);
   hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, b.C, b.rz);
   hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, b.C, b.pAp);
+  if (b.ctl) hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, 1, (double*)b.ctl);
+  const FlatLaunch f = flat_launch(b);
+  if (f.ok) {
+    hipLaunchKernelGGL(k_pcg_start4, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (float4*)b.P,
+                       (const float4*)b.R, b.diag, b.rz);
+    DLP_CHECK_LAUNCH();
+    return DLP_OK;
+  }
 #define K(CPLV)                                                                                     \
   hipLaunchKernelGGL((k_pcg_dir<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.P, b.R, \
-                     b.ld, b.diag, b.beta, 1);                                                     \
+                     b.ld, b.diag, b.beta, 1, nullptr);                                                     \
   hipLaunchKernelGGL((k_coldot<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.R, b.P,  \
                      b.ld, b.rz);
   DLP_DISPATCH_CPL(m.cpl, K)
@@ -193,18 +357,39 @@ int pcg_start(const PcgBufs& b, hipStream_t st) {
 int pcg_finish_iteration(const PcgBufs& b, hipStream_t st, bool have_pap) {
   const ColMap m = col_map(b.C);
   const unsigned g = pass_grid(b.n, m.lc_shift);
+  const FlatLaunch f = flat_launch(b);
+  if (f.ok) {
+    if (!have_pap) {
+#define K(CPLV)                                                                                       \
+  hipLaunchKernelGGL((k_coldot<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.P, b.AP, b.ld, \
+                     b.pAp);
+      DLP_DISPATCH_CPL(m.cpl, K)
+#undef K
+    }
+    hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 0, b.rz, b.rz_new, b.pAp, b.alpha,
+                       b.beta, b.viol, b.ctl);
+    hipLaunchKernelGGL(k_pcg_update4, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (float4*)b.Z,
+                       (float4*)b.R, (const float4*)b.P, (const float4*)b.AP, b.alpha, b.diag, b.eps, b.rz_new,
+                       b.viol, b.ctl);
+    hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 1, b.rz, b.rz_new, b.pAp, b.alpha,
+                       b.beta, b.viol, b.ctl);
+    hipLaunchKernelGGL(k_pcg_dir4, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (float4*)b.P,
+                       (const float4*)b.R, b.diag, b.beta, b.ctl);
+    DLP_CHECK_LAUNCH();
+    return DLP_OK;
+  }
 #define K(CPLV)                                                                                       \
   if (!have_pap)                                                                                      \
     hipLaunchKernelGGL((k_coldot<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.P, b.AP, \
                        b.ld, b.pAp);                                                                 \
   hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 0, b.rz, b.rz_new, b.pAp,        \
-                     b.alpha, b.beta, b.viol);                                                       \
+                     b.alpha, b.beta, b.viol, b.ctl);                                                \
   hipLaunchKernelGGL((k_pcg_update<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.Z, b.R, \
-                     b.P, b.AP, b.ld, b.alpha, b.diag, b.eps, b.rz_new, b.viol);                     \
+                     b.P, b.AP, b.ld, b.alpha, b.diag, b.eps, b.rz_new, b.viol, b.ctl);              \
   hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 1, b.rz, b.rz_new, b.pAp,        \
-                     b.alpha, b.beta, b.viol);                                                       \
+                     b.alpha, b.beta, b.viol, b.ctl);                                                \
   hipLaunchKernelGGL((k_pcg_dir<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.P, b.R,   \
-                     b.ld, b.diag, b.beta, 0);
+                     b.ld, b.diag, b.beta, 0, b.ctl);
   DLP_DISPATCH_CPL(m.cpl, K)
 #undef K
   DLP_CHECK_LAUNCH();

@@ -1,5 +1,6 @@
 // Row-major SpMM with fused axpy, the masked row operator, and residual reductions.
 #include "rowgather.h"
+#include <mutex>
 #include <type_traits>
 
 namespace dlp {
@@ -52,9 +53,9 @@ __global__ __launch_bounds__(256) void k_row_operator_v4(
     const int32_t* __restrict__ indices, const float* __restrict__ w,
     const uint8_t* __restrict__ mask, const float* __restrict__ X, int64_t ldx,
     const float* __restrict__ a, float b, const float* __restrict__ add, int64_t ldadd,
-    float* __restrict__ out, int64_t ldo, int compact_out) {
+    float* __restrict__ out, int64_t ldo, int compact_out, int xcd_per) {
   const Lanes L(g_shift, lc_shift);
-  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> g_shift;
+  const int64_t i = (block_order(xcd_per) * blockDim.x + threadIdx.x) >> g_shift;
   const bool valid = i < nrows;
   const int r = valid ? (rows ? rows[i] : (int)i) : 0;
   const bool live = valid && (mask == nullptr || mask[r] != 0);
@@ -88,60 +89,116 @@ __global__ __launch_bounds__(256) void k_row_operator_v4(
   }
 }
 
-// Grid-stride form of the FP32 operator that also accumulates dot[c] += X[r,c] * out[r,c] in
-// FP64 (the p^T A p column products of PCG, fused into the operator's epilogue).
+// Packed-lane FP32 operator, one row per wavefront (LanesP).
 template <int U>
-__global__ __launch_bounds__(256) void k_row_operator_v4_dot(
-    int nrows, int C, int g_shift, int lc_shift, const int32_t* __restrict__ indptr,
+__global__ __launch_bounds__(256) void k_row_operator_v4p(
+    int nrows, const int32_t* __restrict__ rows, int C, const int32_t* __restrict__ indptr,
     const int32_t* __restrict__ rowlen, const int32_t* __restrict__ indices, const float* __restrict__ w,
-    const uint8_t* __restrict__ mask, const float* __restrict__ X, int64_t ldx, const float* __restrict__ a,
-    float b, float* __restrict__ out, int64_t ldo, double* __restrict__ dot) {
-  const Lanes L(g_shift, lc_shift);
+    const uint8_t* __restrict__ mask, const float* __restrict__ X, int64_t ldx,
+    const float* __restrict__ a, float b, const float* __restrict__ add, int64_t ldadd,
+    float* __restrict__ out, int64_t ldo, int compact_out, int xcd_per) {
+  const LanesP L(C);
+  const int64_t i = (block_order(xcd_per) * blockDim.x + threadIdx.x) >> 6;
+  const bool valid = i < nrows;
+  const int r = valid ? (rows ? rows[i] : (int)i) : 0;
+  const bool live = valid && (mask == nullptr || mask[r] != 0);
+  int beg = 0, end = 0;
+  if (live) {
+    beg = indptr[r];
+    end = rowlen ? beg + rowlen[r] : indptr[r + 1];
+  }
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  row_gather_v4p<U>(L, beg, end, indices, w, mask, X, ldx, acc);
+  acc = reduce_v4p(L, acc);
+  if (valid && L.e == 0) {
+    const float ar = a ? a[r] : 0.f;
+    const int64_t o = compact_out ? i : (int64_t)r;
+    const float accs[4] = {acc.x, acc.y, acc.z, acc.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = 4 * L.l + q;
+      if (c < C) {
+        float y = b * accs[q];
+        if (a) y += ar * X[(int64_t)r * ldx + c];
+        if (add) y += add[(int64_t)r * ldadd + c];
+        out[o * ldo + c] = y;
+      }
+    }
+  }
+}
+
+// Grid-stride FP32 operator over all rows (one row per wavefront per step) that also
+// accumulates dot[c] += X[r,c] * out[r,c] in FP64: the p^T A p column products of PCG, fused
+// into the operator's epilogue.  With xcd set (grid a multiple of 8) the blocks an XCD runs
+// (blockIdx % 8) walk one contiguous eighth of the rows, so each XCD's L2 serves the
+// neighbourhoods of one row range.
+template <int U, bool PACK>
+__global__ __launch_bounds__(256) void k_row_operator_v4_dot(
+    int nrows, int C, const int32_t* __restrict__ indptr, const int32_t* __restrict__ rowlen,
+    const int32_t* __restrict__ indices, const float* __restrict__ w, const uint8_t* __restrict__ mask,
+    const float* __restrict__ X, int64_t ldx, const float* __restrict__ a, float b, float* __restrict__ out,
+    int64_t ldo, double* __restrict__ dot, int xcd, const int32_t* __restrict__ skip) {
+  if (skip && *skip) return;
   __shared__ double sm[256];
   for (int c = threadIdx.x; c < C; c += blockDim.x) sm[c] = 0.0;
+  const int wave = threadIdx.x >> 6;
+  int64_t lo = 0, hi = nrows, wx, nwx;
+  if (xcd) {
+    const int64_t chunk = ((int64_t)nrows + 7) >> 3;
+    lo = (blockIdx.x & 7) * chunk;
+    hi = min((int64_t)nrows, lo + chunk);
+    wx = (int64_t)(blockIdx.x >> 3) * 4 + wave;
+    nwx = (int64_t)(gridDim.x >> 3) * 4;
+  } else {
+    wx = (int64_t)blockIdx.x * 4 + wave;
+    nwx = (int64_t)gridDim.x * 4;
+  }
+  const LanesP L(C, !PACK);
+  const bool owner = L.e == 0 && 4 * L.l < C;
   double dacc[4] = {0.0, 0.0, 0.0, 0.0};
-  const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> g_shift;
-  const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> g_shift;
-  const int64_t wave_first = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) >> g_shift;
-  for (int64_t base = wave_first; base < nrows; base += groups) {
-    const int64_t i = base + (g0 - wave_first);
-    const bool valid = i < nrows;
-    const int r = valid ? (int)i : 0;
-    const bool live = valid && (mask == nullptr || mask[r] != 0);
+  for (int64_t i = lo + wx; i < hi; i += nwx) {
+    const int r = (int)i;
+    const bool live = mask == nullptr || mask[r] != 0;
     int beg = 0, end = 0;
     if (live) {
       beg = indptr[r];
       end = rowlen ? beg + rowlen[r] : indptr[r + 1];
     }
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    row_gather_v4<U>(L, beg, end, indices, w, mask, X, ldx, C, acc);
-    for (int off = L.LC; off < L.G; off <<= 1) {
-      acc.x += __shfl_xor(acc.x, off, 64);
-      acc.y += __shfl_xor(acc.y, off, 64);
-      acc.z += __shfl_xor(acc.z, off, 64);
-      acc.w += __shfl_xor(acc.w, off, 64);
-    }
-    if (valid && L.e == 0 && 4 * L.l < C) {
+    row_gather_v4p<U>(L, beg, end, indices, w, mask, X, ldx, acc);
+    acc = reduce_v4p(L, acc);
+    if (owner) {
       const float ar = a ? a[r] : 0.f;
-      const float accs[4] = {acc.x, acc.y, acc.z, acc.w};
+      const int c0 = 4 * L.l;
+      if (c0 + 3 < C) {
+        const float4 xr = *(reinterpret_cast<const float4*>(X + (int64_t)r * ldx) + L.l);
+        float4 y;
+        y.x = b * acc.x + ar * xr.x;
+        y.y = b * acc.y + ar * xr.y;
+        y.z = b * acc.z + ar * xr.z;
+        y.w = b * acc.w + ar * xr.w;
+        *(reinterpret_cast<float4*>(out + (int64_t)r * ldo) + L.l) = y;
+        dacc[0] += (double)xr.x * (double)y.x;
+        dacc[1] += (double)xr.y * (double)y.y;
+        dacc[2] += (double)xr.z * (double)y.z;
+        dacc[3] += (double)xr.w * (double)y.w;
+      } else {
+        const float accs[4] = {acc.x, acc.y, acc.z, acc.w};
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int c = 4 * L.l + q;
-        if (c < C) {
-          const float xr = X[(int64_t)r * ldx + c];
-          const float y = b * accs[q] + ar * xr;
-          out[(int64_t)r * ldo + c] = y;
-          dacc[q] += (double)xr * (double)y;
+        for (int q = 0; q < 4; ++q) {
+          const int c = c0 + q;
+          if (c < C) {
+            const float xr = X[(int64_t)r * ldx + c];
+            const float y = b * accs[q] + ar * xr;
+            out[(int64_t)r * ldo + c] = y;
+            dacc[q] += (double)xr * (double)y;
+          }
         }
       }
     }
   }
-  for (int off = L.G; off < 64; off <<= 1) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) dacc[q] += __shfl_xor(dacc[q], off, 64);
-  }
   __syncthreads();
-  if ((threadIdx.x & 63) < L.G && L.e == 0) {
+  if (owner) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int c = 4 * L.l + q;
@@ -157,18 +214,62 @@ Tuning& tuning() {
   return t;
 }
 
+// Blocks of `kernel` (256 threads) that fit on the current device at once, cached per
+// (kernel, device).
+int resident_grid(const void* kernel) {
+  struct Entry {
+    const void* k;
+    int dev, blocks;
+  };
+  static Entry cache[64];
+  static int used = 0;
+  static std::mutex mu;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  std::lock_guard<std::mutex> lock(mu);
+  for (int i = 0; i < used; ++i)
+    if (cache[i].k == kernel && cache[i].dev == dev) return cache[i].blocks;
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || per_cu < 1)
+    per_cu = 4;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+  const int g = per_cu * cus;
+  if (used < 64) cache[used++] = Entry{kernel, dev, g};
+  return g;
+}
+
 // out = a .* X + b * W_mask X over all rows, and dot[c] += sum_r X[r,c] * out[r,c] (dot not
 // cleared here).  FP32, 16-byte rows required; returns DLP_ERR_ARG when the layout does not fit.
 int row_operator_dot(int n, int C, const int32_t* indptr, const int32_t* rowlen, const int32_t* indices,
                      const float* w, const uint8_t* mask, const float* X, int64_t ldx, const float* a, float b,
-                     float* out, int64_t ldo, double* dot, hipStream_t st) {
-  DLP_CHECK_ARG((ldx % 4) == 0 && (((uintptr_t)X) & 15) == 0 && C <= 256, "row_operator_dot: layout");
-  int lc = ceil_log2((C + 3) / 4);
-  if (lc > 6) lc = 6;
-  unsigned grid = grid_for((long long)n << 6);
-  if (grid > 4096) grid = 4096;
-  hipLaunchKernelGGL((k_row_operator_v4_dot<8>), dim3(grid), dim3(kBlock), 0, st, n, C, 6, lc, indptr, rowlen,
-                     indices, w, mask, X, ldx, a, b, out, ldo, dot);
+                     float* out, int64_t ldo, double* dot, hipStream_t st, const int32_t* skip) {
+  DLP_CHECK_ARG((ldx % 4) == 0 && (ldo % 4) == 0 && (((uintptr_t)X) & 15) == 0 && (((uintptr_t)out) & 15) == 0 &&
+                    C <= 256,
+                "row_operator_dot: layout");
+  if (n <= 0) return DLP_OK;
+  const Tuning& T = tuning();
+  const bool pack = T.pack != 0;
+  const bool u8 = T.unroll == 8;   // auto (0): 4, the measured best of the grid-stride form
+  const void* kfn = pack ? (u8 ? reinterpret_cast<const void*>(&k_row_operator_v4_dot<8, true>)
+                               : reinterpret_cast<const void*>(&k_row_operator_v4_dot<4, true>))
+                         : (u8 ? reinterpret_cast<const void*>(&k_row_operator_v4_dot<8, false>)
+                               : reinterpret_cast<const void*>(&k_row_operator_v4_dot<4, false>));
+  long long grid = T.dot_grid > 0 ? T.dot_grid : resident_grid(kfn);
+  const long long need = ((long long)n + 3) / 4;
+  if (grid > need) grid = need;
+  const int xcd = T.xcd != 0;
+  if (xcd) grid = grid < 8 ? 8 : grid & ~7LL;
+#define DLP_DOT(UV, PV)                                                                                      \
+  hipLaunchKernelGGL((k_row_operator_v4_dot<UV, PV>), dim3((unsigned)grid), dim3(kBlock), 0, st, n, C, indptr, \
+                     rowlen, indices, w, mask, X, ldx, a, b, out, ldo, dot, xcd, skip)
+  if (pack) {
+    if (u8) DLP_DOT(8, true);
+    else DLP_DOT(4, true);
+  } else {
+    if (u8) DLP_DOT(8, false);
+    else DLP_DOT(4, false);
+  }
+#undef DLP_DOT
   DLP_CHECK_LAUNCH();
   return DLP_OK;
 }
@@ -183,30 +284,42 @@ static int launch_row_operator(int nrows, const int32_t* rows, int C, const int3
                   std::is_same<TO, float>::value && tuning().vec4 && (ldx % 4) == 0 &&
                   (((uintptr_t)X) & 15) == 0 && C <= 256;
   if (v4) {
-    // column lanes: next power of two >= ceil(C/4); edge lanes fill the wavefront
+    // one row per wavefront; column lanes either packed (ceil(C/4) lanes, 64/that edges) or
+    // the next power of two >= ceil(C/4) with edge lanes filling the wavefront
     int lc = ceil_log2((C + 3) / 4);
     if (lc > 6) lc = 6;
-    RowMap m4;
-    m4.lc_shift = lc;
-    m4.g_shift = 6;
-    m4.cpl = 1;
-    const long long threads = (long long)nrows << m4.g_shift;
-    dim3 grid(grid_for(threads)), block(kBlock);
-    if (tuning().unroll >= 8)
-      hipLaunchKernelGGL((k_row_operator_v4<8>), grid, block, 0, st, nrows, rows, C, m4.g_shift, m4.lc_shift,
-                         indptr, rowlen, indices, (const float*)w, mask, (const float*)X, ldx, (const float*)a,
-                         (float)b, (const float*)add, ldadd, (float*)out, ldo, compact_out);
+    const long long threads = (long long)nrows << 6;
+    unsigned g = grid_for(threads);
+    int xcd_per = 0;
+    if (tuning().xcd) {
+      g = (g + 7) & ~7u;
+      xcd_per = (int)(g >> 3);
+    }
+    dim3 grid(g), block(kBlock);
+    const float* wf = (const float*)w;
+    const float* Xf = (const float*)X;
+    const float* af = (const float*)a;
+    const float* addf = (const float*)add;
+    float* of = (float*)out;
+    if (tuning().pack && tuning().unroll == 8)
+      hipLaunchKernelGGL((k_row_operator_v4p<8>), grid, block, 0, st, nrows, rows, C, indptr, rowlen, indices, wf,
+                         mask, Xf, ldx, af, (float)b, addf, ldadd, of, ldo, compact_out, xcd_per);
+    else if (tuning().pack)
+      hipLaunchKernelGGL((k_row_operator_v4p<4>), grid, block, 0, st, nrows, rows, C, indptr, rowlen, indices, wf,
+                         mask, Xf, ldx, af, (float)b, addf, ldadd, of, ldo, compact_out, xcd_per);
+    else if (tuning().unroll != 4)
+      hipLaunchKernelGGL((k_row_operator_v4<8>), grid, block, 0, st, nrows, rows, C, 6, lc, indptr, rowlen,
+                         indices, wf, mask, Xf, ldx, af, (float)b, addf, ldadd, of, ldo, compact_out, xcd_per);
     else
-      hipLaunchKernelGGL((k_row_operator_v4<4>), grid, block, 0, st, nrows, rows, C, m4.g_shift, m4.lc_shift,
-                         indptr, rowlen, indices, (const float*)w, mask, (const float*)X, ldx, (const float*)a,
-                         (float)b, (const float*)add, ldadd, (float*)out, ldo, compact_out);
+      hipLaunchKernelGGL((k_row_operator_v4<4>), grid, block, 0, st, nrows, rows, C, 6, lc, indptr, rowlen,
+                         indices, wf, mask, Xf, ldx, af, (float)b, addf, ldadd, of, ldo, compact_out, xcd_per);
     DLP_CHECK_LAUNCH();
     return DLP_OK;
   }
   RowMap m = choose_map(C, avg_deg);
   const long long threads = (long long)nrows << m.g_shift;
   dim3 grid(grid_for(threads)), block(kBlock);
-  const bool u8 = tuning().unroll >= 8;
+  const bool u8 = tuning().unroll != 4;
 #define DLP_ROWOP(CPLV)                                                                        \
   if (u8) hipLaunchKernelGGL((k_row_operator<TW, TX, TO, CPLV, 8>), grid, block, 0, st, nrows, rows, C, \
                      m.g_shift, m.lc_shift, indptr, rowlen, indices, (const TW*)w, mask,       \
@@ -314,12 +427,50 @@ extern "C" int dlp_row_operator(int nrows, const int32_t* rows, int C, const int
   return DLP_ERR_ARG;
 }
 
+extern "C" int dlp_row_operator_dot(int n, int C, const int32_t* indptr, const int32_t* rowlen,
+                                    const int32_t* indices, const float* w, const uint8_t* mask,
+                                    const float* X, int64_t ldx, const float* a, float b, float* out,
+                                    int64_t ldo, double* dot, void* stream) {
+  DLP_CHECK_ARG(n >= 0 && C >= 1 && C <= 256, "row_operator_dot: bad n=%d or C=%d", n, C);
+  if (n == 0) return DLP_OK;
+  DLP_CHECK_ARG(indptr && indices && w && X && out && dot, "row_operator_dot: null pointer");
+  DLP_CHECK_ARG(ldx >= C && ldo >= C, "row_operator_dot: leading dim < C");
+  return row_operator_dot(n, C, indptr, rowlen, indices, w, mask, X, ldx, a, b, out, ldo, dot,
+                          (hipStream_t)stream, nullptr);
+}
+
 extern "C" int dlp_spmm_axpy(int n, int C, const int32_t* indptr, const int32_t* indices,
                              const void* w, const void* X, int64_t ldx, const void* a, double b,
                              void* Y, int64_t ldy, int dtype, double avg_deg, void* stream) {
   DLP_CHECK_ARG(dtype == DLP_F32 || dtype == DLP_F64, "spmm_axpy: bad dtype %d", dtype);
   return dlp_row_operator(n, nullptr, C, indptr, nullptr, indices, w, dtype, nullptr, X, ldx,
                           dtype, a, b, nullptr, 0, Y, ldy, dtype, 0, avg_deg, stream);
+}
+
+// rho[c] = max_r |R[r,c]| / diag[r] over all rows of a 16-byte-aligned FP32 block (flat quads).
+__global__ __launch_bounds__(256) void k_rho_cols4(int n, int Q4, int C, const float4* __restrict__ R,
+                                                   const double* __restrict__ diag, double* __restrict__ rho) {
+  __shared__ double sm[260];
+  for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) sm[c] = 0.0;
+  __syncthreads();
+  const Quad Qd = quad_of(Q4, C);
+  double m[4] = {0.0, 0.0, 0.0, 0.0};
+  if (Qd.active && Qd.r0 < n) {
+    const int64_t nq = (int64_t)n * Q4;
+    int64_t r = Qd.r0;
+    for (int64_t i = Qd.i0; i < nq; i += Qd.step, r += Qd.rstep) {
+      const double inv = 1.0 / diag[r];
+      const float4 v = R[i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (j < Qd.cnt) m[j] = fmax(m[j], fabs((double)quad_comp(v, j)) * inv);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if (j < Qd.cnt) atomic_max_nonneg(&sm[4 * Qd.q4 + j], m[j]);
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += blockDim.x) atomic_max_nonneg(rho + c, sm[c]);
 }
 
 extern "C" int dlp_rho_cols(int nrows, const int32_t* rows, int C, const void* R, int64_t ldr,
@@ -329,6 +480,17 @@ extern "C" int dlp_rho_cols(int nrows, const int32_t* rows, int C, const void* R
   hipStream_t st = (hipStream_t)stream;
   DLP_CHECK_HIP(hipMemsetAsync(rho, 0, sizeof(double) * C, st));
   if (nrows == 0) return DLP_OK;
+  if (!rows && r_dtype == DLP_F32 && (ldr % 4) == 0 && ldr <= 256 && (((uintptr_t)R) & 15) == 0 &&
+      tuning().vec4) {
+    const int Q4 = (int)(ldr / 4);
+    const long long used = Q4 * (kBlock / Q4);
+    long long g = ((long long)nrows * Q4 + used - 1) / used;
+    if (g > 2048) g = 2048;
+    hipLaunchKernelGGL(k_rho_cols4, dim3((unsigned)g), dim3(kBlock), 0, st, nrows, Q4, C, (const float4*)R, diag,
+                       rho);
+    DLP_CHECK_LAUNCH();
+    return DLP_OK;
+  }
   const long long total = (long long)nrows * C;
   unsigned grid = grid_for(total);
   if (grid > 2048) grid = 2048;
@@ -365,12 +527,24 @@ extern "C" int dlp_violating_rows(int nrows, const int32_t* rows, int C, const v
 }
 
 extern "C" int dlp_tune_set(int key, int value) {
-  // key 0: unroll factor of the gather loops (4 or 8); key 1: 16-byte FP32 loads on/off
+  // key 0: unroll factor of the gather loops (0 = per-kernel default, 4 or 8); key 1: 16-byte
+  // FP32 loads on/off;
+  // key 2: packed lane groups; key 3: XCD-aware row order; key 4: blocks of the fused-dot
+  // operator (0 = resident blocks); key 5: fuse p^T A p into the global solve's operator
   if (key == 0) {
-    DLP_CHECK_ARG(value == 4 || value == 8, "tune: unroll must be 4 or 8");
+    DLP_CHECK_ARG(value == 0 || value == 4 || value == 8, "tune: unroll must be 0, 4 or 8");
     tuning().unroll = value;
   } else if (key == 1) {
     tuning().vec4 = value ? 1 : 0;
+  } else if (key == 2) {
+    tuning().pack = value ? 1 : 0;
+  } else if (key == 3) {
+    tuning().xcd = value ? 1 : 0;
+  } else if (key == 4) {
+    DLP_CHECK_ARG(value >= 0 && value <= (1 << 20), "tune: dot grid out of range");
+    tuning().dot_grid = value;
+  } else if (key == 5) {
+    tuning().fuse_dot = value ? 1 : 0;
   } else {
     DLP_CHECK_ARG(false, "tune: unknown key %d", key);
   }

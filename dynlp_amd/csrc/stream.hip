@@ -7,6 +7,7 @@
 // that finds the row full is recorded as overflow and retried after the host regrows the
 // store.  Labeled vertices stay in the store; `solved` (U: unlabeled and grounded) masks the
 // operator, so W_UU never has to be materialised.
+#include <cstring>
 #include "rowgather.h"
 #include "scan.h"
 #include "blas.h"
@@ -485,10 +486,11 @@ __device__ __forceinline__ void refresh_row(const RefreshArgs& A, const Lanes& L
   if (valid && L.lane == 0 && (bal & gm) && A.viol) A.viol[atomicAdd(A.nviol, 1)] = r;
 }
 
-// FP32 state with 16-byte rows (ld % 4 == 0): lane l of the LC = 2^lc_shift column lanes covers
-// columns 4l..4l+3 with double accumulators; LE = 64/LC edge lanes.  Labeled neighbours (rare)
-// add their weight to the rhs of the lane owning their class column.
-__device__ __forceinline__ void refresh_row_v4(const RefreshArgs& A, const Lanes& L, int64_t i, int nr,
+// FP32 state with 16-byte rows (ld % 4 == 0), one row per wavefront: column lane l covers
+// columns 4l..4l+3 with double accumulators, LE edge lanes (LanesP: packed ceil(C1/4) column
+// lanes, or a power of two).  Labeled neighbours (rare) add their weight to the rhs of the lane
+// owning their class column.
+__device__ __forceinline__ void refresh_row_v4(const RefreshArgs& A, const LanesP& L, int64_t i, int nr,
                                                float* __restrict__ Z, float* __restrict__ R, double (&rmax)[4]) {
   const bool valid = i < nr;
   const int r = valid ? (A.rows ? A.rows[i] : (int)i) : 0;
@@ -501,8 +503,7 @@ __device__ __forceinline__ void refresh_row_v4(const RefreshArgs& A, const Lanes
   }
   double acc[4] = {0.0, 0.0, 0.0, 0.0}, rhs[4] = {0.0, 0.0, 0.0, 0.0};
   double s = 0.0, rs = 0.0;
-  const bool lane_on = 4 * L.l < A.C1;
-  for (int base = beg; base < end; base += L.G) {
+  for (int base = beg; base < end; base += 64) {
     const int j = base + L.lane;
     int myv = 0, kind = 0, mycol = -1;
     float myw = 0.f;
@@ -521,25 +522,25 @@ __device__ __forceinline__ void refresh_row_v4(const RefreshArgs& A, const Lanes
     }
     if (kind == 1) s += myw;
     if (kind == 2) rs += myw;
-    const int cnt = min(L.G, end - base);
+    const int cnt = min(64, end - base);
     for (int t0 = 0; t0 < cnt; t0 += 4 * L.LE) {
       int vv[4], kk[4], cc[4];
       float ww[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int t = t0 + u * L.LE + L.e;
-        const int src = t & (L.G - 1);
-        vv[u] = __shfl(myv, src, L.G);
-        kk[u] = __shfl(kind, src, L.G);
-        cc[u] = __shfl(mycol, src, L.G);
-        ww[u] = __shfl(myw, src, L.G);
-        if (t >= cnt) kk[u] = 0;
+        const int src = t & 63;
+        vv[u] = __shfl(myv, src, 64);
+        kk[u] = __shfl(kind, src, 64);
+        cc[u] = __shfl(mycol, src, 64);
+        ww[u] = __shfl(myw, src, 64);
+        if (t >= cnt || !L.on) kk[u] = 0;
       }
       float4 xv[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const float4* zr = reinterpret_cast<const float4*>(Z + (int64_t)vv[u] * A.ld) + L.l;
-        xv[u] = (kk[u] == 2 && lane_on) ? *zr : make_float4(0.f, 0.f, 0.f, 0.f);
+        xv[u] = kk[u] == 2 ? *zr : make_float4(0.f, 0.f, 0.f, 0.f);
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -552,19 +553,34 @@ __device__ __forceinline__ void refresh_row_v4(const RefreshArgs& A, const Lanes
       }
     }
   }
-  for (int off = L.LC; off < L.G; off <<= 1) {
+  {
+    // edge-lane partials of column lane l are at lanes l + k*LC
+    double tacc[4], trhs[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      acc[q] += __shfl_xor(acc[q], off, 64);
-      rhs[q] += __shfl_xor(rhs[q], off, 64);
+      tacc[q] = acc[q];
+      trhs[q] = rhs[q];
+    }
+    for (int k = 1; k < L.LE; ++k) {
+      const int src = L.lane + k * L.LC;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        tacc[q] += __shfl(acc[q], src, 64);
+        trhs[q] += __shfl(rhs[q], src, 64);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      acc[q] = tacc[q];
+      rhs[q] = trhs[q];
     }
   }
-  for (int off = 1; off < L.G; off <<= 1) {
+  for (int off = 1; off < 64; off <<= 1) {
     s += __shfl_xor(s, off, 64);
     rs += __shfl_xor(rs, off, 64);
   }
   bool viol = false;
-  if (valid && L.e == 0 && lane_on) {
+  if (valid && L.e == 0 && 4 * L.l < A.C1) {
     const int64_t zo = (int64_t)r * A.ld;
     if (!live) {
       if (L.l == 0) {
@@ -602,30 +618,34 @@ __device__ __forceinline__ void refresh_row_v4(const RefreshArgs& A, const Lanes
     }
   }
   const unsigned long long bal = __ballot(viol);
-  const int wl = threadIdx.x & 63;
-  const unsigned long long gm = (L.G == 64 ? ~0ull : ((1ull << L.G) - 1)) << (wl & ~(L.G - 1));
-  if (valid && L.lane == 0 && (bal & gm) && A.viol) A.viol[atomicAdd(A.nviol, 1)] = r;
+  if (valid && L.lane == 0 && bal && A.viol) A.viol[atomicAdd(A.nviol, 1)] = r;
 }
 
-__global__ __launch_bounds__(256) void k_refresh_v4(RefreshArgs A, int g_shift, int lc_shift, float* __restrict__ Z,
+__global__ __launch_bounds__(256) void k_refresh_v4(RefreshArgs A, int pow2, int xcd, float* __restrict__ Z,
                                                     float* __restrict__ R) {
-  const Lanes L(g_shift, lc_shift);
+  const LanesP L(A.C1, pow2 != 0);
   const int nr = A.rows_count ? *A.rows_count : A.nrows;
-  const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> g_shift;
-  const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> g_shift;
-  const int64_t wave_first = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) >> g_shift;
+  // with xcd (grid a multiple of 8) the blocks one XCD runs walk one contiguous eighth of the
+  // (listed) rows
+  int64_t lo = 0, hi = nr, w0, waves;
+  const int wave = threadIdx.x >> 6;
+  if (xcd) {
+    const int64_t chunk = ((int64_t)nr + 7) >> 3;
+    lo = (blockIdx.x & 7) * chunk;
+    hi = min((int64_t)nr, lo + chunk);
+    w0 = (int64_t)(blockIdx.x >> 3) * 4 + wave;
+    waves = (int64_t)(gridDim.x >> 3) * 4;
+  } else {
+    w0 = (int64_t)blockIdx.x * 4 + wave;
+    waves = (int64_t)gridDim.x * 4;
+  }
   double rmax[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int64_t base = wave_first; base < nr; base += groups)
-    refresh_row_v4(A, L, base + (g0 - wave_first), nr, Z, R, rmax);
+  for (int64_t i = lo + w0; i < hi; i += waves) refresh_row_v4(A, L, i, nr, Z, R, rmax);
   if (!A.rho) return;
   __shared__ double sm[256];
   for (int c = threadIdx.x; c < A.C1; c += blockDim.x) sm[c] = 0.0;
-  for (int off = L.G; off < 64; off <<= 1) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) rmax[q] = fmax(rmax[q], __shfl_xor(rmax[q], off, 64));
-  }
   __syncthreads();
-  if ((threadIdx.x & 63) < L.G && L.e == 0) {
+  if (L.e == 0 && 4 * L.l < A.C1) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int c = 4 * L.l + q;
@@ -938,7 +958,8 @@ struct LP {
   void* scan_ws = nullptr;
   size_t scan_bytes = 0;
   PushState S;
-  int64_t* host = nullptr;      // pinned status mirror
+  int64_t* host = nullptr;      // pinned mirror of stat_dev (kStatWords words)
+  int64_t* stat_dev = nullptr;  // status words gathered on the device for one copy
   hipGraphExec_t gexec = nullptr;
   int graph_rounds = 0;
   hipStream_t graph_stream = nullptr;
@@ -963,6 +984,36 @@ struct LP {
 };
 
 enum { CNT_ROWS = 0, CNT_MISSING, CNT_OVERFLOW, CNT_TOGGLED, CNT_VIOL, CNT_CAND64, CNT_N };
+// status words: [0, 32) counters, [32, 32 + 256) rho as FP64 bits
+constexpr int kStatRho = 32;
+constexpr int kStatWords = kStatRho + 256;
+
+// Gathers the host-visible status into one buffer so the host reads it with a single copy:
+// counters, push list count and round, edge visits, push statistics, and (if rho) rho[C1].
+__global__ void k_status_gather(const int32_t* __restrict__ cnt, const int32_t* __restrict__ bcount,
+                                const int32_t* __restrict__ rnd, const unsigned long long* __restrict__ edges,
+                                const unsigned long long* __restrict__ stats, const double* __restrict__ rho,
+                                int C1, int64_t* __restrict__ out) {
+  const int t = threadIdx.x;
+  if (t < CNT_N) out[t] = cnt[t];
+  if (t == 0) {
+    out[CNT_N] = *bcount;
+    out[CNT_N + 1] = rnd[0];
+    out[CNT_N + 2] = (int64_t)*edges;
+  }
+  if (t < 4) out[CNT_N + 3 + t] = (int64_t)stats[t];
+  if (rho)
+    for (int c = t; c < C1; c += blockDim.x) reinterpret_cast<double*>(out + kStatRho)[c] = rho[c];
+}
+
+// out[0] = *done, out[1] = *edges (the push loop's per-launch check).
+__global__ void k_push_probe(const int32_t* __restrict__ done, const unsigned long long* __restrict__ edges,
+                             int64_t* __restrict__ out) {
+  if (threadIdx.x == 0) {
+    out[0] = *done;
+    out[1] = (int64_t)*edges;
+  }
+}
 
 template <typename T>
 int dev_alloc(T** p, size_t count) {
@@ -980,6 +1031,7 @@ int lp_free(LP* h) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->host) (void)hipHostFree(h->host);
+  if (h->stat_dev) (void)hipFree(h->stat_dev);
   if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->P) (void)hipFree(h->P);
@@ -1064,7 +1116,8 @@ extern "C" int dlp_lp_create(const dlp_lp_desc* d, void** handle) {
   if (!rc && d->Z64) rc = dev_alloc(&h->dsnap, nz);
   h->scan_bytes = dlp_scan_workspace_bytes(n);
   if (!rc && hipMalloc(&h->scan_ws, h->scan_bytes) != hipSuccess) rc = DLP_ERR_HIP;
-  if (!rc && hipHostMalloc((void**)&h->host, 64 * sizeof(int64_t)) != hipSuccess) rc = DLP_ERR_HIP;
+  if (!rc && hipHostMalloc((void**)&h->host, kStatWords * sizeof(int64_t)) != hipSuccess) rc = DLP_ERR_HIP;
+  if (!rc) rc = dev_alloc(&h->stat_dev, kStatWords);
   if (rc) {
     set_error("lp_create: device allocation failed");
     lp_free(h);
@@ -1370,9 +1423,15 @@ extern "C" int dlp_lp_refresh(void* handle, int which, int prec, void* stream) {
   unsigned grid = persistent_grid(h->d.n, m.g_shift);
   if (grid > cap) grid = cap;
   if (prec == 0 && (h->d.ld % 4) == 0 && tuning().vec4 && h->d.C1 <= 256) {
-    int lc = ceil_log2((h->d.C1 + 3) / 4);
-    if (lc > 6) lc = 6;
-    hipLaunchKernelGGL(k_refresh_v4, dim3(grid), dim3(kBlock), 0, st, A, 6, lc, h->d.Z32, h->d.R32);
+    // one row per wavefront: the grid is capped at the resident blocks (no tail of late blocks)
+    const unsigned res = (unsigned)resident_grid(reinterpret_cast<const void*>(&k_refresh_v4));
+    unsigned g4 = grid_for((long long)h->d.n << 6);
+    if (g4 > res) g4 = res;
+    if (g4 > cap) g4 = cap;
+    const int xcd = tuning().xcd != 0;
+    if (xcd) g4 = g4 < 8 ? 8 : g4 & ~7u;
+    hipLaunchKernelGGL(k_refresh_v4, dim3(g4), dim3(kBlock), 0, st, A, tuning().pack ? 0 : 1, xcd, h->d.Z32,
+                       h->d.R32);
     DLP_CHECK_LAUNCH();
     return DLP_OK;
   }
@@ -1462,13 +1521,14 @@ extern "C" int dlp_lp_push(void* handle, int max_rounds, int rounds_per_launch, 
     }
     rounds += rounds_per_launch;
     DLP_CHECK_HIP(hipEventRecord(h->ev1, st));
-    DLP_CHECK_HIP(hipMemcpyAsync(h->host, h->S.done, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    DLP_CHECK_HIP(hipMemcpyAsync(h->host + 1, h->S.edges, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(k_push_probe, dim3(1), dim3(64), 0, st, h->S.done, h->S.edges, h->stat_dev);
+    DLP_CHECK_LAUNCH();
+    DLP_CHECK_HIP(hipMemcpyAsync(h->host, h->stat_dev, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
     DLP_CHECK_HIP(hipStreamSynchronize(st));
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess) h->push_ms += ms;
     h->push_launched += rounds_per_launch;
-    if (*(volatile int32_t*)h->host) {
+    if (((volatile int64_t*)h->host)[0]) {
       converged = 1;
       break;
     }
@@ -1544,20 +1604,16 @@ extern "C" int dlp_lp_status(void* handle, int64_t* out, double* rho, void* stre
   DLP_CHECK_ARG(handle && out, "lp_status: null argument");
   LP* h = (LP*)handle;
   hipStream_t st = (hipStream_t)stream;
-  int32_t tmp[CNT_N + 8];
-  DLP_CHECK_HIP(hipMemcpyAsync(tmp, h->cnt, sizeof(int32_t) * CNT_N, hipMemcpyDeviceToHost, st));
-  DLP_CHECK_HIP(hipMemcpyAsync(tmp + CNT_N, h->S.bcount, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  DLP_CHECK_HIP(hipMemcpyAsync(tmp + CNT_N + 1, h->S.rnd, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, st));
-  unsigned long long edges = 0, pst[4] = {0, 0, 0, 0};
-  DLP_CHECK_HIP(hipMemcpyAsync(&edges, h->S.edges, sizeof(edges), hipMemcpyDeviceToHost, st));
-  DLP_CHECK_HIP(hipMemcpyAsync(pst, h->S.stats, sizeof(pst), hipMemcpyDeviceToHost, st));
-  if (rho) DLP_CHECK_HIP(hipMemcpyAsync(rho, h->rho, sizeof(double) * h->d.C1, hipMemcpyDeviceToHost, st));
+  const int C1 = h->d.C1;
+  hipLaunchKernelGGL(k_status_gather, dim3(1), dim3(kBlock), 0, st, h->cnt, h->S.bcount, h->S.rnd, h->S.edges,
+                     h->S.stats, rho ? h->rho : nullptr, C1, h->stat_dev);
+  DLP_CHECK_LAUNCH();
+  const size_t words = rho ? (size_t)kStatRho + C1 : (size_t)CNT_N + 7;
+  DLP_CHECK_HIP(hipMemcpyAsync(h->host, h->stat_dev, words * sizeof(int64_t), hipMemcpyDeviceToHost, st));
   DLP_CHECK_HIP(hipStreamSynchronize(st));
-  for (int i = 0; i < CNT_N; ++i) out[i] = tmp[i];
-  out[CNT_N] = tmp[CNT_N];
-  out[CNT_N + 1] = tmp[CNT_N + 1];
-  out[CNT_N + 2] = (int64_t)edges;
-  for (int i = 0; i < 4; ++i) out[CNT_N + 3 + i] = (int64_t)pst[i];
+  const volatile int64_t* hs = h->host;
+  for (int i = 0; i < CNT_N + 7; ++i) out[i] = hs[i];
+  if (rho) std::memcpy(rho, (const void*)(h->host + kStatRho), sizeof(double) * C1);
   out[CNT_N + 7] = h->push_launched;
   out[CNT_N + 8] = (int64_t)(h->push_ms * 1000.0);  // microseconds
   return DLP_OK;
@@ -1585,14 +1641,15 @@ static PcgBufs pcg_bufs(LP* h) {
   b.alpha = h->pcg_scal + 3 * C1;
   b.beta = h->pcg_scal + 4 * C1;
   b.viol = h->pcg_viol;
+  b.ctl = h->pcg_viol ? reinterpret_cast<int32_t*>(h->pcg_viol) + 2 : nullptr;
   return b;
 }
 
 static int pcg_iteration(LP* h, const PcgBufs& b, hipStream_t st) {
   // AP = diag*P - W_UU P, with p^T A p accumulated in the same pass when the layout allows
-  if ((b.ld % 4) == 0 && tuning().vec4) {
+  if ((b.ld % 4) == 0 && tuning().vec4 && tuning().fuse_dot) {
     int rc = row_operator_dot(b.n, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.solved, b.P, b.ld,
-                              b.diag, -1.f, b.AP, b.ld, b.pAp, st);
+                              b.diag, -1.f, b.AP, b.ld, b.pAp, st, b.ctl);
     if (rc) return rc;
     return pcg_finish_iteration(b, st, true);
   }
@@ -1648,7 +1705,7 @@ extern "C" int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int 
     DLP_CHECK_HIP(hipMalloc((void**)&h->P, nz * sizeof(float)));
     DLP_CHECK_HIP(hipMalloc((void**)&h->AP, nz * sizeof(float)));
     DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_scal, 5 * C1 * sizeof(double)));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_viol, sizeof(float)));
+    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_viol, 4 * sizeof(float)));   // viol, pad, done, iterations
     DLP_CHECK_HIP(hipMemsetAsync(h->P, 0, nz * sizeof(float), st));
     DLP_CHECK_HIP(hipMemsetAsync(h->AP, 0, nz * sizeof(float), st));
   }
@@ -1659,10 +1716,12 @@ extern "C" int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int 
     rc = capture_pcg(h, iters_per_launch);
     if (rc) return rc;
   }
-  int64_t it = 0;
+  // Iterations after the one that meets the thresholds return at once (device flag), so a
+  // group may be longer than the solve needs; the device counts the iterations that ran.
+  int64_t launched = 0, it = 0;
   float v = 0.f, best = 3.4e38f;
   int stall = 0;
-  while (it < max_iter) {
+  while (launched < max_iter) {
     DLP_CHECK_HIP(hipEventRecord(h->ev0, st));
     if (use_graph) {
       DLP_CHECK_HIP(hipGraphLaunch(h->pcg_exec, st));
@@ -1672,15 +1731,20 @@ extern "C" int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int 
         if (rc) return rc;
       }
     }
-    it += iters_per_launch;
+    launched += iters_per_launch;
     DLP_CHECK_HIP(hipEventRecord(h->ev1, st));
-    DLP_CHECK_HIP(hipMemcpyAsync(h->host, h->pcg_viol, sizeof(float), hipMemcpyDeviceToHost, st));
+    DLP_CHECK_HIP(hipMemcpyAsync(h->host, h->pcg_viol, 4 * sizeof(float), hipMemcpyDeviceToHost, st));
     DLP_CHECK_HIP(hipStreamSynchronize(st));
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess) h->pcg_ms += ms;
-    h->pcg_iters += iters_per_launch;
-    v = *(volatile float*)h->host;
-    if (v <= 1.f) break;
+    const volatile float* hv = reinterpret_cast<volatile float*>(h->host);
+    const volatile int32_t* hc = reinterpret_cast<volatile int32_t*>(h->host) + 2;
+    v = hv[0];
+    const int done = hc[0];
+    const int64_t dev_it = hc[1];
+    h->pcg_iters += dev_it - it;
+    it = dev_it;
+    if (done || v <= 1.f) break;
     if (v < 0.9f * best) {
       best = v;
       stall = 0;
@@ -1712,7 +1776,7 @@ extern "C" int dlp_lp_time_operator(void* handle, int reps, double* ms_out, void
   const PcgBufs b = pcg_bufs(h);
   // the launch the global solve makes: the 16-byte-row operator with the fused p^T A p
   // epilogue when the layout allows it, else the plain row operator
-  const bool fused = (b.ld % 4) == 0 && tuning().vec4;
+  const bool fused = (b.ld % 4) == 0 && tuning().vec4 && tuning().fuse_dot;
   auto launch = [&]() -> int {
     if (fused)
       return row_operator_dot(b.n, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.solved, b.P, b.ld,
@@ -1810,7 +1874,7 @@ extern "C" int dlp_lp_bpcg(void* handle, int max_iter, int iters_per_launch, int
     DLP_CHECK_HIP(hipMalloc((void**)&h->P, nz * sizeof(float)));
     DLP_CHECK_HIP(hipMalloc((void**)&h->AP, nz * sizeof(float)));
     DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_scal, 5 * C1 * sizeof(double)));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_viol, sizeof(float)));
+    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_viol, 4 * sizeof(float)));   // viol, pad, done, iterations
     DLP_CHECK_HIP(hipMemsetAsync(h->P, 0, nz * sizeof(float), st));
     DLP_CHECK_HIP(hipMemsetAsync(h->AP, 0, nz * sizeof(float), st));
   }

@@ -72,7 +72,7 @@ class EdgeStreamLP:
 
     def __init__(self, g: Graph, eps=1e-6, eps_h=0.05, fp64=False, eta=0.0, slack_min=4,
                  slack_frac=0.25, push_scale=0.9, rounds_per_launch=4, use_graph=True, max_rounds=200_000,
-                 push_frac=0.05, push_max_rows=4096, push_budget=1.0, pcg_iters_per_launch=4,
+                 push_frac=0.05, push_max_rows=4096, push_budget=1.0, pcg_iters_per_launch=6,
                  pcg_max_iter=20_000, pcg_mode="column", incremental_ground=True):
         self.g, self.n, self.K = g, g.n, g.K
         self.C = 1 if g.K == 2 else g.K
@@ -128,6 +128,7 @@ class EdgeStreamLP:
         self._rho = np.zeros(self.C1, dtype=np.float64)
         self.batches = 0
         self.scratch3 = None
+        self._stage = self._stage_dev = None
         self.nnz = int(ix.shape[0])
 
     def __del__(self):
@@ -223,10 +224,24 @@ class EdgeStreamLP:
         return self.status()
 
     def _upload(self, b: EdgeBatch):
-        dev = device()
-        f = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a)).to(dev, dt, non_blocking=False)  # noqa: E731
-        return (f(b.del_u, torch.int32), f(b.del_v, torch.int32), f(b.ins_u, torch.int32),
-                f(b.ins_v, torch.int32), f(b.ins_w, torch.float32))
+        """The batch's five arrays packed into one pinned staging buffer and sent with one copy
+        (the previous request has finished with both buffers: every request ends in a sync)."""
+        nd, ni = len(b.del_u), len(b.ins_u)
+        total = 2 * nd + 3 * ni
+        if self._stage is None or self._stage.numel() < total:
+            cap = max(total, 1 << 16)
+            self._stage = torch.empty(cap, dtype=torch.int32, pin_memory=True)
+            self._stage_dev = torch.empty(cap, dtype=torch.int32, device=device())
+        h = self._stage.numpy()
+        h[:nd] = b.del_u
+        h[nd:2 * nd] = b.del_v
+        o = 2 * nd
+        h[o:o + ni] = b.ins_u
+        h[o + ni:o + 2 * ni] = b.ins_v
+        h[o + 2 * ni:total] = np.asarray(b.ins_w, dtype=np.float32).view(np.int32)
+        d = self._stage_dev
+        d[:total].copy_(self._stage[:total], non_blocking=True)
+        return (d[:nd], d[nd:2 * nd], d[o:o + ni], d[o + ni:o + 2 * ni], d[o + 2 * ni:total].view(torch.float32))
 
     def _regrow(self, ins, extra):
         iu, iv, iw = ins

@@ -59,9 +59,25 @@ int dlp_row_operator(int nrows, const int32_t* rows, int C, const int32_t* indpt
                      const void* a, double b, const void* add, int64_t ldadd, void* out,
                      int64_t ldo, int out_dtype, int compact_out, double avg_deg, void* stream);
 
-/* Process-wide tuning of the gather loops: key 0 = unroll factor (4 or 8, default 8); key 1 =
- * 16-byte FP32 loads when the row layout allows them (1 default, 0 off).  No reference
- * counterpart (the reference templated lanes per row at JIT time, kernels.py:365-374). */
+/* FP32 operator over all rows with the column dot fused into its epilogue:
+ *   out[r] = a[r] * X[r] + b * sum_{j in row r, mask[col_j]} w_j X[col_j]   (every row r < n)
+ *   dot[c] += sum_r X[r,c] * out[r,c]   (FP64, accumulated: dot is not cleared)
+ * With a = diag, b = -1 this is AP = apply_A(P) and pAp = (P * AP).sum(axis=0) of the
+ * reference PCG loop (solvers.py:78-79) in one pass.  Needs ldx % 4 == 0, ldo % 4 == 0 and
+ * 16-byte aligned X and out (DLP_ERR_ARG otherwise); C <= 256. */
+int dlp_row_operator_dot(int n, int C, const int32_t* indptr, const int32_t* rowlen,
+                         const int32_t* indices, const float* w, const uint8_t* mask,
+                         const float* X, int64_t ldx, const float* a, float b, float* out,
+                         int64_t ldo, double* dot, void* stream);
+
+/* Process-wide tuning of the row kernels (no reference counterpart; the reference templated
+ * lanes per row at JIT time, kernels.py:365-374):
+ *   key 0 = unroll factor of the gather loops (0 default = per kernel, 4 or 8);
+ *   key 1 = 16-byte FP32 loads when the row layout allows them (1 default, 0 off);
+ *   key 2 = packed lane groups, ceil(C/4) column lanes per row (1 default) or a power of two;
+ *   key 3 = XCD-aware row order (each XCD walks one contiguous eighth of the rows; 1 default);
+ *   key 4 = blocks of the fused-dot operator (0 default = the resident blocks of the device);
+ *   key 5 = fuse p^T A p into the global solve's operator launch (1 default). */
 int dlp_tune_set(int key, int value);
 
 /* ---- Column residual ratios ------------------------------------------------------------

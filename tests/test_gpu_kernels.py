@@ -70,6 +70,72 @@ def test_row_operator_masked_subset_mixed_precision():
     np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-12, atol=1e-12)
 
 
+TUNE_DEFAULTS = {0: 0, 1: 1, 2: 1, 3: 1, 4: 0, 5: 1}
+
+
+@pytest.fixture
+def tune():
+    from dynlp_amd import _lib
+
+    def set_(**kv):
+        keys = {"unroll": 0, "vec4": 1, "pack": 2, "xcd": 3, "dot_grid": 4, "fuse_dot": 5}
+        for k, v in kv.items():
+            _lib.call("dlp_tune_set", keys[k], int(v))
+    yield set_
+    for k, v in TUNE_DEFAULTS.items():
+        _lib.call("dlp_tune_set", k, v)
+
+
+@pytest.mark.parametrize("pack", [0, 1])
+@pytest.mark.parametrize("xcd", [0, 1])
+@pytest.mark.parametrize("unroll", [4, 8])
+@pytest.mark.parametrize("C", [4, 17, 65, 130])
+def test_fp32_row_layouts_match_oracle(tune, pack, xcd, unroll, C):
+    """16-byte FP32 operator and the fused operator + column dot, every lane layout and row
+    order, on a 16-byte-aligned slice (ld = C rounded up to 4) with a solved mask."""
+    import torch
+    from dynlp_amd import _lib
+    from dynlp_amd.backend import ptr, stream
+    tune(pack=pack, xcd=xcd, unroll=unroll)
+    g, W, Ws, _ = _setup(n=4001, dtype="float32")
+    n, ld = g.n, (C + 3) // 4 * 4
+    rs = np.random.default_rng(C)
+    mask = (rs.random(n) < 0.9).astype(np.uint8)
+    X = np.zeros((n, ld), np.float32)
+    X[:, :C] = rs.standard_normal((n, C))
+    a = (rs.random(n) + 1).astype(np.float32)
+    Xd, ad, md = (torch.as_tensor(v, device="cuda") for v in (X, a, mask))
+    Wm = Ws.multiply(mask[:, None]).multiply(mask[None, :]).tocsr()
+    Xc = X[:, :C].astype(np.float64)
+    ref = a[:, None].astype(np.float64) * Xc - (Wm @ Xc)
+    ref[mask == 0] = a[mask == 0, None] * Xc[mask == 0]
+    out = torch.zeros((n, ld), dtype=torch.float32, device="cuda")
+    _lib.call("dlp_row_operator", n, None, C, ptr(W.indptr), None, ptr(W.indices), ptr(W.data), 0, ptr(md),
+              ptr(Xd), ld, 0, ptr(ad), -1.0, None, 0, ptr(out), ld, 0, 0, 0.0, stream())
+    np.testing.assert_allclose(out.cpu().numpy()[:, :C], ref, rtol=2e-5, atol=2e-5)
+    out.zero_()
+    dot = torch.full((C,), 0.5, dtype=torch.float64, device="cuda")   # accumulated into
+    _lib.call("dlp_row_operator_dot", n, C, ptr(W.indptr), None, ptr(W.indices), ptr(W.data), ptr(md), ptr(Xd), ld,
+              ptr(ad), -1.0, ptr(out), ld, ptr(dot), stream())
+    o = out.cpu().numpy()
+    np.testing.assert_allclose(o[:, :C], ref, rtol=2e-5, atol=2e-5)
+    assert not o[:, C:].any()                      # padding columns untouched
+    np.testing.assert_allclose(dot.cpu().numpy(), 0.5 + (Xc * o[:, :C].astype(np.float64)).sum(axis=0),
+                               rtol=1e-9, atol=1e-9)
+
+
+def test_row_operator_dot_rejects_unaligned_rows():
+    import torch
+    from dynlp_amd import _lib
+    from dynlp_amd.backend import ptr, stream
+    g, W, Ws, _ = _setup(n=200, dtype="float32")
+    X = torch.zeros((200, 6), dtype=torch.float32, device="cuda")
+    dot = torch.zeros(5, dtype=torch.float64, device="cuda")
+    with pytest.raises(_lib.DLPError):
+        _lib.call("dlp_row_operator_dot", 200, 5, ptr(W.indptr), None, ptr(W.indices), ptr(W.data), None, ptr(X), 6,
+                  None, -1.0, ptr(X), 6, ptr(dot), stream())
+
+
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
 @pytest.mark.parametrize("C", [1, 3, 8, 40, 65])
 def test_frontier_kernels_match_oracle(dtype, C):

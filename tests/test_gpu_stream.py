@@ -60,6 +60,56 @@ def test_stream_matches_rebuilt_system(fp64, K):
         check_against_oracle(lp, g, es, eps, f"batch {t}")
 
 
+@pytest.mark.parametrize("pack,xcd,vec4", [(0, 0, 1), (0, 1, 1), (1, 0, 1), (1, 1, 0)])
+def test_stream_lane_layouts_and_row_orders(pack, xcd, vec4):
+    """The FP32 service path (refresh, operator with fused dot, flat PCG passes) under the
+    power-of-two lane layout, the plain row order and scalar loads; the defaults (packed lanes,
+    XCD-aware row order, 16-byte loads) run in the tests above."""
+    from dynlp_amd import _lib
+    from dynlp_amd.stream import EdgeStreamLP
+    _lib.call("dlp_tune_set", 1, vec4)
+    _lib.call("dlp_tune_set", 2, pack)
+    _lib.call("dlp_tune_set", 3, xcd)
+    try:
+        g = graphs.sbm(3000, 8, 10, 0.8, 8, 0.02, seed=4)
+        lp = EdgeStreamLP(g, eps=1e-6)
+        lp.initial_solve()
+        es = edge_dict(g)
+        batches, _ = graphs.update_stream(g, 3, 300, seed=4)
+        for t, b in enumerate(batches):
+            lp.apply_and_solve(b)
+            O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
+                               zip(b.del_u.tolist(), b.del_v.tolist()))
+        st = lp.load_state(np.zeros((g.n, lp.C1), np.float32))   # a global solve from zero too
+        assert "pcg" in st.path, st.path
+        check_against_oracle(lp, g, es, 1e-6, f"pack={pack} xcd={xcd} vec4={vec4}")
+    finally:
+        _lib.call("dlp_tune_set", 1, 1)
+        _lib.call("dlp_tune_set", 2, 1)
+        _lib.call("dlp_tune_set", 3, 1)
+
+
+def test_pcg_group_length_does_not_change_the_solve():
+    """Iterations past convergence inside a captured group return at once (device flag): group
+    lengths 1 and 7 run the same iterations and give the same state."""
+    from dynlp_amd.stream import EdgeStreamLP
+    g = graphs.sbm(3000, 8, 10, 0.8, 8, 0.02, seed=6)
+    out = []
+    for per in (1, 7):
+        lp = EdgeStreamLP(g, eps=1e-6, pcg_iters_per_launch=per)
+        lp.initial_solve()
+        before = lp.pcg_stats()["pcg_iters"]
+        st = lp.load_state(np.zeros((g.n, lp.C1), np.float32))
+        assert "pcg" in st.path
+        out.append((lp.pcg_stats()["pcg_iters"] - before, lp.scores_with_h()))
+    (i1, Z1), (i7, Z7) = out
+    assert abs(i1 - i7) <= 1 and i1 > 1, (i1, i7)
+    # both certified to rho <= 1e-6: each within rho * hmax of the exact solution
+    C = Z1.shape[1] - 1
+    hmax = max(Z1[:, C].max(), Z7[:, C].max())
+    np.testing.assert_allclose(Z1[:, :C], Z7[:, :C], rtol=0, atol=2.1e-6 * hmax)
+
+
 def test_fragmenting_stream_regrounds():
     """Sparse SBM with deletions of inter-block edges first: components split off and lose
     (and regain) their labeled contact, so rows toggle between solved and inert."""
