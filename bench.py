@@ -42,6 +42,26 @@ def operator_bytes(n, C1, stored):
     return 4 * (n + 1) + 4 * n + 8 * stored + n + 4 * n * C1 + 4 * n + 4 * n * C1
 
 
+def pmc_probe(g):
+    """Launch sequence for hardware-counter runs (rocprofv3 --pmc ...): the counters of the 5
+    A*P launches are read against those of one plain copy of known size."""
+    import torch
+    from dynlp_amd.stream import EdgeStreamLP
+    lp = EdgeStreamLP(g, eps=1e-6)
+    lp.initial_solve()
+    torch.cuda.synchronize()
+    op_ms = lp.time_operator(reps=4)           # 1 + 4 launches
+    nbytes = operator_bytes(g.n, lp.C1, lp.stored_entries())
+    src = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"pmc probe: 5 A*P launches ({op_ms:.3f} ms each, {nbytes} algorithmic bytes each); "
+        f"copy of {src.numel() * 4} bytes")
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,6 +74,9 @@ def main(argv=None):
     ap.add_argument("--cpu-batches", type=int, default=1, help="requests timed on the CPU port")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="counter probe: initial solve, 5 A*P launches, then a device copy of 512 MiB "
+                         "(reads 512 MiB, writes 512 MiB) that calibrates FETCH_SIZE / WRITE_SIZE")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {a.gpus}); "
@@ -72,6 +95,8 @@ def main(argv=None):
     t = time.perf_counter()
     g = graphs.config_graph(3, scale=a.scale, seed=a.seed)
     log(f"graph n={g.n:,} m={g.m:,} labelled={int(g.labeled.sum()):,} ({time.perf_counter() - t:.1f}s)")
+    if a.pmc_probe:
+        return pmc_probe(g)
     t = time.perf_counter()
     nreq = a.warmup + a.steps
     batches, _ = graphs.update_stream(g, nreq, a.batch_edges, seed=a.seed)

@@ -35,6 +35,12 @@ int row_operator_dot(int n, int C, const int32_t* indptr, const int32_t* rowlen,
                      const float* w, const uint8_t* mask, const float* X, int64_t ldx, const float* a, float b,
                      float* out, int64_t ldo, double* dot, hipStream_t st, const int32_t* skip = nullptr);
 
+// The global solve's operator: pipelined row gathers, rows outside rowmask skipped, neighbour
+// rows unmasked (P = 0 outside the solved set), fused p^T A p; skip = device exit flag (spmm.hip).
+int pcg_operator(int n, int C, const int32_t* indptr, const int32_t* rowlen, const int32_t* indices, const float* w,
+                 const uint8_t* rowmask, const float* X, int64_t ldx, const float* a, float b, float* out,
+                 int64_t ldo, double* dot, hipStream_t st, const int32_t* skip);
+
 // Blocks of a 256-thread kernel that are resident on the current device at once (spmm.hip).
 int resident_grid(const void* kernel);
 

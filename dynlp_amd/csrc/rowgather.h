@@ -243,6 +243,7 @@ struct Tuning {
   int xcd = 1;        // XCD-aware row order: each XCD walks one contiguous eighth of the rows
   int dot_grid = 0;   // blocks of the grid-stride operator with fused dot (0: resident blocks)
   int fuse_dot = 1;   // fuse p^T A p into the operator launch of the global solve
+  int pipe = 4;       // gathers per lane in flight in the pipelined global-solve operator (0: off)
 };
 Tuning& tuning();
 

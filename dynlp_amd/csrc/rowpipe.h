@@ -1,0 +1,172 @@
+// Software-pipelined row gathers over every row of a row-major FP32 block, one row per
+// wavefront step (grid-stride, optionally XCD-aware), shared by the global solve's operator and
+// the full residual verification.
+//
+// Lanes are packed (LanesP): ceil(C/4) column lanes of 16 bytes times LE edge lanes.  While row
+// i's neighbour rows are gathered (S gathers per lane issued before the first multiply-add), row
+// i+1's range and its first 64 (column, weight) pairs are already in flight.  Neighbour rows are
+// read with raw buffer loads at 32-bit byte offsets v*row_bytes + 16*l (24-bit multiply-add);
+// lanes without an edge get an out-of-range offset, for which the buffer load returns zeros, so
+// no lane masking or select sits on the gather path.  Rows outside rowmask are skipped; neighbour
+// rows are not masked, so the block must hold the values the caller wants summed for every
+// neighbour (zero where a neighbour must not count).  Callers guarantee n < 2^24 and
+// n*row_bytes < 2^31.
+#pragma once
+#include "rowgather.h"
+
+namespace dlp {
+
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float4 buf_load4(__amdgpu_buffer_rsrc_t rs, unsigned off) {
+  const u4v r = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0);
+  return make_float4(__uint_as_float(r.x), __uint_as_float(r.y), __uint_as_float(r.z), __uint_as_float(r.w));
+}
+
+// Rows of this wavefront in a grid-stride pass: [lo, hi) stepping nwx from lo + wx.  With
+// xcd set (grid a multiple of 8) the blocks one XCD runs (blockIdx % 8) take one contiguous
+// eighth of the rows.
+__device__ __forceinline__ void xcd_rows(int nrows, int xcd, int64_t& lo, int64_t& hi, int64_t& wx, int64_t& nwx) {
+  const int wave = threadIdx.x >> 6;
+  lo = 0;
+  hi = nrows;
+  if (xcd) {
+    const int64_t chunk = ((int64_t)nrows + 7) >> 3;
+    lo = (blockIdx.x & 7) * chunk;
+    hi = min((int64_t)nrows, lo + chunk);
+    wx = (int64_t)(blockIdx.x >> 3) * 4 + wave;
+    nwx = (int64_t)(gridDim.x >> 3) * 4;
+  } else {
+    wx = (int64_t)blockIdx.x * 4 + wave;
+    nwx = (int64_t)gridDim.x * 4;
+  }
+}
+
+struct RowPipe {
+  LanesP L;
+  int64_t lo, hi, wx, nwx;
+  const uint8_t* __restrict__ rowmask;
+  const int32_t* __restrict__ indptr;
+  const int32_t* __restrict__ rowlen;
+  const int32_t* __restrict__ indices;
+  const float* __restrict__ w;
+  __amdgpu_buffer_rsrc_t rx;
+  unsigned rowb, laneq;
+
+  __device__ __forceinline__ RowPipe(int nrows, int C, int xcd, const uint8_t* rm, const int32_t* ip,
+                                     const int32_t* rl, const int32_t* ix, const float* wv, const float* X,
+                                     int64_t ldx, int xbytes)
+      : L(C), rowmask(rm), indptr(ip), rowlen(rl), indices(ix), w(wv) {
+    xcd_rows(nrows, xcd, lo, hi, wx, nwx);
+    rx = __builtin_amdgcn_make_buffer_rsrc((void*)X, (short)0, xbytes, 0x00020000);
+    rowb = (unsigned)(ldx * 4);
+    laneq = 16u * (unsigned)L.l;
+  }
+};
+
+template <typename TA>
+__device__ __forceinline__ void fma4(TA (&acc)[4], float wv, const float4& x) {
+  acc[0] += (TA)wv * (TA)x.x;
+  acc[1] += (TA)wv * (TA)x.y;
+  acc[2] += (TA)wv * (TA)x.z;
+  acc[3] += (TA)wv * (TA)x.w;
+}
+
+// Runs every row of the pass: epi.pre(r) before row r's gathers are issued (own-row loads),
+// epi.post(r, live, acc) with live = rowmask[r] and acc = sum over the row's entries of
+// w_j * X[col_j, 4l..4l+3] (zero when !live; valid in edge lane 0), called by every lane of the
+// wavefront.
+template <int S, typename TA, typename E>
+__device__ __forceinline__ void run_rows(const RowPipe& G, E& epi) {
+  const LanesP& L = G.L;
+  constexpr unsigned kOOB = 0x80000000u;
+  auto range = [&](int64_t ii, int& beg, int& end, bool& live) {
+    beg = end = 0;
+    live = false;
+    if (ii < G.hi) {
+      const int ri = __builtin_amdgcn_readfirstlane((int)ii);
+      live = G.rowmask[ri] != 0;
+      if (live) {
+        beg = __builtin_amdgcn_readfirstlane(G.indptr[ri]);
+        end = beg + __builtin_amdgcn_readfirstlane(G.rowlen[ri]);
+      }
+    }
+  };
+  // cnt_l: pairs this lane may read (0 on idle edge lanes); the select is branch-free
+  auto gather = [&](int v, float wv, int t, int cnt_l, float4& xv, float& ww) {
+    const int vv = __shfl(v, t & 63, 64);
+    ww = __shfl(wv, t & 63, 64);
+    const unsigned o = __umul24((unsigned)vv, G.rowb) + G.laneq;
+    xv = buf_load4(G.rx, t < cnt_l ? o : kOOB);
+  };
+  auto steps = [&](int v, float wv, int t0, int cnt, TA (&acc)[4]) {
+    const int cnt_l = L.on ? cnt : 0;
+    for (; t0 < cnt; t0 += 4 * L.LE) {
+      float4 yv[4];
+      float yw[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) gather(v, wv, t0 + u * L.LE + L.e, cnt_l, yv[u], yw[u]);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) fma4(acc, yw[u], yv[u]);
+    }
+  };
+  int64_t i = G.lo + G.wx;
+  int beg0, end0;
+  bool live0;
+  range(i, beg0, end0, live0);
+  int v0 = 0;
+  float w0 = 0.f;
+  if (beg0 + L.lane < end0) {
+    v0 = G.indices[beg0 + L.lane];
+    w0 = G.w[beg0 + L.lane];
+  }
+  for (; i < G.hi; i += G.nwx) {
+    int beg1, end1;
+    bool live1;
+    range(i + G.nwx, beg1, end1, live1);
+    const int r = __builtin_amdgcn_readfirstlane((int)i);
+    epi.pre(r);
+    const int cnt = min(64, end0 - beg0);
+    const int cnt_l = L.on ? cnt : 0;
+    float4 xv[S];
+    float ww[S];
+#pragma unroll
+    for (int u = 0; u < S; ++u) gather(v0, w0, u * L.LE + L.e, cnt_l, xv[u], ww[u]);
+    // next row's pairs go out while this row's gathers are in flight
+    int v1 = 0;
+    float w1 = 0.f;
+    if (beg1 + L.lane < end1) {
+      v1 = G.indices[beg1 + L.lane];
+      w1 = G.w[beg1 + L.lane];
+    }
+    TA acc[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int u = 0; u < S; ++u) fma4(acc, ww[u], xv[u]);
+    steps(v0, w0, S * L.LE, cnt, acc);
+    // rows longer than 64 entries: further chunks of 64 pairs
+    for (int base = beg0 + 64; base < end0; base += 64) {
+      int vc = 0;
+      float wc = 0.f;
+      if (base + L.lane < end0) {
+        vc = G.indices[base + L.lane];
+        wc = G.w[base + L.lane];
+      }
+      steps(vc, wc, 0, min(64, end0 - base), acc);
+    }
+    // edge-lane partials of column lane l are at lanes l + k*LC
+    TA tot[4] = {acc[0], acc[1], acc[2], acc[3]};
+    for (int k = 1; k < L.LE; ++k) {
+      const int src = L.lane + k * L.LC;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) tot[q] += __shfl(acc[q], src, 64);
+    }
+    epi.post(r, live0, tot);
+    v0 = v1;
+    w0 = w1;
+    beg0 = beg1;
+    end0 = end1;
+    live0 = live1;
+  }
+}
+
+}  // namespace dlp

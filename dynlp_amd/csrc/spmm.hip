@@ -1,5 +1,5 @@
 // Row-major SpMM with fused axpy, the masked row operator, and residual reductions.
-#include "rowgather.h"
+#include "rowpipe.h"
 #include <mutex>
 #include <type_traits>
 
@@ -127,6 +127,51 @@ __global__ __launch_bounds__(256) void k_row_operator_v4p(
   }
 }
 
+// out[r, 4l..4l+3] = b*acc + ar*xr (columns < C) and dacc += xr .* out.
+__device__ __forceinline__ void dot_epilogue(const LanesP& L, int C, int r, const float4& acc, const float4& xr,
+                                             float ar, float b, float* __restrict__ out, int64_t ldo,
+                                             double (&dacc)[4]) {
+  float4 y;
+  y.x = b * acc.x + ar * xr.x;
+  y.y = b * acc.y + ar * xr.y;
+  y.z = b * acc.z + ar * xr.z;
+  y.w = b * acc.w + ar * xr.w;
+  const int c0 = 4 * L.l;
+  float* op = out + (int64_t)r * ldo + c0;
+  if (c0 + 3 < C) {
+    *reinterpret_cast<float4*>(op) = y;
+    dacc[0] += (double)xr.x * (double)y.x;
+    dacc[1] += (double)xr.y * (double)y.y;
+    dacc[2] += (double)xr.z * (double)y.z;
+    dacc[3] += (double)xr.w * (double)y.w;
+  } else {
+    const float ys[4] = {y.x, y.y, y.z, y.w};
+    const float xs[4] = {xr.x, xr.y, xr.z, xr.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (c0 + q < C) {
+        op[q] = ys[q];
+        dacc[q] += (double)xs[q] * (double)ys[q];
+      }
+    }
+  }
+}
+
+// Column sums of dacc: workgroup (LDS) then one FP64 atomic per column per workgroup.
+__device__ __forceinline__ void dot_flush(const LanesP& L, int C, bool owner, const double (&dacc)[4], double* sm,
+                                          double* __restrict__ dot) {
+  __syncthreads();
+  if (owner) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = 4 * L.l + q;
+      if (c < C) atomicAdd(&sm[c], dacc[q]);
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&dot[c], sm[c]);
+}
+
 // Grid-stride FP32 operator over all rows (one row per wavefront per step) that also
 // accumulates dot[c] += X[r,c] * out[r,c] in FP64: the p^T A p column products of PCG, fused
 // into the operator's epilogue.  With xcd set (grid a multiple of 8) the blocks an XCD runs
@@ -141,18 +186,8 @@ __global__ __launch_bounds__(256) void k_row_operator_v4_dot(
   if (skip && *skip) return;
   __shared__ double sm[256];
   for (int c = threadIdx.x; c < C; c += blockDim.x) sm[c] = 0.0;
-  const int wave = threadIdx.x >> 6;
-  int64_t lo = 0, hi = nrows, wx, nwx;
-  if (xcd) {
-    const int64_t chunk = ((int64_t)nrows + 7) >> 3;
-    lo = (blockIdx.x & 7) * chunk;
-    hi = min((int64_t)nrows, lo + chunk);
-    wx = (int64_t)(blockIdx.x >> 3) * 4 + wave;
-    nwx = (int64_t)(gridDim.x >> 3) * 4;
-  } else {
-    wx = (int64_t)blockIdx.x * 4 + wave;
-    nwx = (int64_t)gridDim.x * 4;
-  }
+  int64_t lo, hi, wx, nwx;
+  xcd_rows(nrows, xcd, lo, hi, wx, nwx);
   const LanesP L(C, !PACK);
   const bool owner = L.e == 0 && 4 * L.l < C;
   double dacc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -169,44 +204,53 @@ __global__ __launch_bounds__(256) void k_row_operator_v4_dot(
     acc = reduce_v4p(L, acc);
     if (owner) {
       const float ar = a ? a[r] : 0.f;
-      const int c0 = 4 * L.l;
-      if (c0 + 3 < C) {
-        const float4 xr = *(reinterpret_cast<const float4*>(X + (int64_t)r * ldx) + L.l);
-        float4 y;
-        y.x = b * acc.x + ar * xr.x;
-        y.y = b * acc.y + ar * xr.y;
-        y.z = b * acc.z + ar * xr.z;
-        y.w = b * acc.w + ar * xr.w;
-        *(reinterpret_cast<float4*>(out + (int64_t)r * ldo) + L.l) = y;
-        dacc[0] += (double)xr.x * (double)y.x;
-        dacc[1] += (double)xr.y * (double)y.y;
-        dacc[2] += (double)xr.z * (double)y.z;
-        dacc[3] += (double)xr.w * (double)y.w;
-      } else {
-        const float accs[4] = {acc.x, acc.y, acc.z, acc.w};
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int c = c0 + q;
-          if (c < C) {
-            const float xr = X[(int64_t)r * ldx + c];
-            const float y = b * accs[q] + ar * xr;
-            out[(int64_t)r * ldo + c] = y;
-            dacc[q] += (double)xr * (double)y;
-          }
-        }
-      }
+      const float4 xr = *(reinterpret_cast<const float4*>(X + (int64_t)r * ldx) + L.l);
+      dot_epilogue(L, C, r, acc, xr, ar, b, out, ldo, dacc);
     }
   }
-  __syncthreads();
-  if (owner) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int c = 4 * L.l + q;
-      if (c < C) atomicAdd(&sm[c], dacc[q]);
-    }
+  dot_flush(L, C, owner, dacc, sm, dot);
+}
+
+// The global solve's operator AP = diag*P - W_UU P with the fused p^T A p, on the pipelined
+// row gathers of rowpipe.h.  Neighbour rows are not masked: every row outside the solved set
+// holds P = 0 during the solve (its residual is parked at 0 by the refresh, so P = R/diag +
+// beta P and AP = diag*P keep it 0), and its term vanishes.
+struct PcgEpilogue {
+  const LanesP& L;
+  int C;
+  bool owner;
+  const float* __restrict__ X;
+  int64_t ldx;
+  const float* __restrict__ a;
+  float b;
+  float* __restrict__ out;
+  int64_t ldo;
+  double dacc[4] = {0.0, 0.0, 0.0, 0.0};
+  float4 xr;
+  float ar;
+  __device__ __forceinline__ void pre(int r) {
+    xr = owner ? *(reinterpret_cast<const float4*>(X + (int64_t)r * ldx) + L.l) : make_float4(0.f, 0.f, 0.f, 0.f);
+    ar = a[r];
   }
-  __syncthreads();
-  for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&dot[c], sm[c]);
+  // rows outside the mask get AP = diag*P (no neighbour sum), as the plain operator writes them
+  __device__ __forceinline__ void post(int r, bool, const float (&acc)[4]) {
+    if (owner) dot_epilogue(L, C, r, make_float4(acc[0], acc[1], acc[2], acc[3]), xr, ar, b, out, ldo, dacc);
+  }
+};
+
+template <int S>
+__global__ __launch_bounds__(256) void k_pcg_operator(
+    int nrows, int C, const int32_t* __restrict__ indptr, const int32_t* __restrict__ rowlen,
+    const int32_t* __restrict__ indices, const float* __restrict__ w, const uint8_t* __restrict__ rowmask,
+    const float* __restrict__ X, int64_t ldx, int xbytes, const float* __restrict__ a, float b,
+    float* __restrict__ out, int64_t ldo, double* __restrict__ dot, int xcd, const int32_t* __restrict__ skip) {
+  if (skip && *skip) return;
+  __shared__ double sm[256];
+  for (int c = threadIdx.x; c < C; c += blockDim.x) sm[c] = 0.0;
+  const RowPipe G(nrows, C, xcd, rowmask, indptr, rowlen, indices, w, X, ldx, xbytes);
+  PcgEpilogue E{G.L, C, G.L.e == 0 && 4 * G.L.l < C, X, ldx, a, b, out, ldo};
+  run_rows<S, float>(G, E);
+  dot_flush(G.L, C, E.owner, E.dacc, sm, dot);
 }
 
 Tuning& tuning() {
@@ -270,6 +314,39 @@ int row_operator_dot(int n, int C, const int32_t* indptr, const int32_t* rowlen,
     else DLP_DOT(4, false);
   }
 #undef DLP_DOT
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+// The global solve's operator (k_pcg_operator): rowmask = the solved set, rowlen required,
+// 16-byte rows, C <= 256.  Falls back to row_operator_dot when tuning disables the pipelined
+// form (key 6 = 0).
+int pcg_operator(int n, int C, const int32_t* indptr, const int32_t* rowlen, const int32_t* indices, const float* w,
+                 const uint8_t* rowmask, const float* X, int64_t ldx, const float* a, float b, float* out,
+                 int64_t ldo, double* dot, hipStream_t st, const int32_t* skip) {
+  const Tuning& T = tuning();
+  const long long xbytes = (long long)n * ldx * 4;
+  if (!T.pipe || !rowlen || !rowmask || !a || n >= (1 << 24) || xbytes >= (1LL << 31))
+    return row_operator_dot(n, C, indptr, rowlen, indices, w, rowmask, X, ldx, a, b, out, ldo, dot, st, skip);
+  DLP_CHECK_ARG((ldx % 4) == 0 && (ldo % 4) == 0 && (((uintptr_t)X) & 15) == 0 && (((uintptr_t)out) & 15) == 0 &&
+                    C <= 256,
+                "pcg_operator: layout");
+  if (n <= 0) return DLP_OK;
+  const void* kfn = T.pipe == 8   ? reinterpret_cast<const void*>(&k_pcg_operator<8>)
+                    : T.pipe == 6 ? reinterpret_cast<const void*>(&k_pcg_operator<6>)
+                                  : reinterpret_cast<const void*>(&k_pcg_operator<4>);
+  long long grid = T.dot_grid > 0 ? T.dot_grid : resident_grid(kfn);
+  const long long need = ((long long)n + 3) / 4;
+  if (grid > need) grid = need;
+  const int xcd = T.xcd != 0;
+  if (xcd) grid = grid < 8 ? 8 : grid & ~7LL;
+#define DLP_PIPE(SV)                                                                                           \
+  hipLaunchKernelGGL((k_pcg_operator<SV>), dim3((unsigned)grid), dim3(kBlock), 0, st, n, C, indptr, rowlen, indices, \
+                     w, rowmask, X, ldx, (int)xbytes, a, b, out, ldo, dot, xcd, skip)
+  if (T.pipe == 8) DLP_PIPE(8);
+  else if (T.pipe == 6) DLP_PIPE(6);
+  else DLP_PIPE(4);
+#undef DLP_PIPE
   DLP_CHECK_LAUNCH();
   return DLP_OK;
 }
@@ -530,7 +607,9 @@ extern "C" int dlp_tune_set(int key, int value) {
   // key 0: unroll factor of the gather loops (0 = per-kernel default, 4 or 8); key 1: 16-byte
   // FP32 loads on/off;
   // key 2: packed lane groups; key 3: XCD-aware row order; key 4: blocks of the fused-dot
-  // operator (0 = resident blocks); key 5: fuse p^T A p into the global solve's operator
+  // operator (0 = resident blocks); key 5: fuse p^T A p into the global solve's operator;
+  // key 6: gathers in flight per lane in the pipelined global-solve operator (4, 6 or 8; 0 =
+  // the plain fused operator)
   if (key == 0) {
     DLP_CHECK_ARG(value == 0 || value == 4 || value == 8, "tune: unroll must be 0, 4 or 8");
     tuning().unroll = value;
@@ -545,6 +624,10 @@ extern "C" int dlp_tune_set(int key, int value) {
     tuning().dot_grid = value;
   } else if (key == 5) {
     tuning().fuse_dot = value ? 1 : 0;
+  } else if (key == 6) {
+    DLP_CHECK_ARG(value == 0 || value == 4 || value == 6 || value == 8,
+                  "tune: pipelined operator depth must be 0, 4, 6 or 8");
+    tuning().pipe = value;
   } else {
     DLP_CHECK_ARG(false, "tune: unknown key %d", key);
   }

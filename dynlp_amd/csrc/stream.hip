@@ -8,10 +8,9 @@
 // store.  Labeled vertices stay in the store; `solved` (U: unlabeled and grounded) masks the
 // operator, so W_UU never has to be materialised.
 #include <cstring>
-#include "rowgather.h"
+#include "rowpipe.h"
 #include "scan.h"
 #include "blas.h"
-#include "rowgather.h"
 
 namespace dlp {
 
@@ -359,11 +358,31 @@ struct RefreshArgs {
   int32_t* viol;
   int32_t* nviol;
   double* rho;             // C1 column maxima, or NULL
+  double* mass;            // sum over violating rows of max_c |R_c|/(eps_c*diag), or NULL
 };
+
+// Rows outside the solved set are parked so that an unmasked neighbour sum over every entry of
+// a row is exactly what its residual needs: a labeled row holds the indicator of its class
+// column (so W Z adds the labeled-neighbour weights of the right-hand side) and h = 0; any
+// other row holds zeros.  R = 0 and diag = 1 on all of them.  Returns the column that holds 1,
+// or -1.
+__device__ __forceinline__ int parked_class(const RefreshArgs& A, int r) {
+  if (!A.labeled[r]) return -1;
+  const int yl = A.y[r];
+  return A.K2 ? (yl == 1 ? 0 : -1) : yl;
+}
+
+// One FP64 atomic per wavefront: the wave's sum of per-row mass terms.
+__device__ __forceinline__ void flush_mass(double* dst, double v) {
+  if (!dst) return;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0 && v > 0.0) atomicAdd(dst, v);
+}
 
 template <typename TZ, typename TR, int CPL>
 __device__ __forceinline__ void refresh_row(const RefreshArgs& A, const Lanes& L, int64_t i, int nr,
-                                            TZ* __restrict__ Z, TR* __restrict__ R, double (&rmax)[CPL]) {
+                                            TZ* __restrict__ Z, TR* __restrict__ R, double (&rmax)[CPL],
+                                            double& mass) {
   const bool valid = i < nr;
   const int r = valid ? (A.rows ? A.rows[i] : (int)i) : 0;
   const bool live = valid && A.solved[r];
@@ -443,6 +462,7 @@ __device__ __forceinline__ void refresh_row(const RefreshArgs& A, const Lanes& L
     s += __shfl_xor(s, off, 64);
     rs += __shfl_xor(rs, off, 64);
   }
+  double m = 0.0;  // max_c ratio/eps_c of this lane
   bool viol = false;
   if (valid && L.e == 0) {
     const int64_t zo = (int64_t)r * A.ld;
@@ -451,11 +471,12 @@ __device__ __forceinline__ void refresh_row(const RefreshArgs& A, const Lanes& L
         A.diag64[r] = 1.0;
         A.diag32[r] = 1.f;
       }
+      const int pc = parked_class(A, r);
 #pragma unroll
       for (int k = 0; k < CPL; ++k) {
         const int c = L.l + k * L.LC;
         if (c < A.C1) {
-          Z[zo + c] = (TZ)(c == C ? 1 : 0);
+          Z[zo + c] = (TZ)(c == pc ? 1 : 0);
           R[zo + c] = (TR)0;
         }
       }
@@ -476,6 +497,7 @@ __device__ __forceinline__ void refresh_row(const RefreshArgs& A, const Lanes& L
           const double ratio = fabs(res) / dg;
           viol |= ratio > A.eps[c];
           rmax[k] = fmax(rmax[k], ratio);
+          m = fmax(m, ratio / A.eps[c]);
         }
       }
     }
@@ -484,6 +506,10 @@ __device__ __forceinline__ void refresh_row(const RefreshArgs& A, const Lanes& L
   const int wl = threadIdx.x & 63;
   const unsigned long long gm = (L.G == 64 ? ~0ull : ((1ull << L.G) - 1)) << (wl & ~(L.G - 1));
   if (valid && L.lane == 0 && (bal & gm) && A.viol) A.viol[atomicAdd(A.nviol, 1)] = r;
+  if (A.mass) {
+    for (int off = 1; off < L.G; off <<= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if (valid && L.lane == 0 && m > 1.0) mass += m;
+  }
 }
 
 // FP32 state with 16-byte rows (ld % 4 == 0), one row per wavefront: column lane l covers
@@ -491,7 +517,8 @@ __device__ __forceinline__ void refresh_row(const RefreshArgs& A, const Lanes& L
 // lanes, or a power of two).  Labeled neighbours (rare) add their weight to the rhs of the lane
 // owning their class column.
 __device__ __forceinline__ void refresh_row_v4(const RefreshArgs& A, const LanesP& L, int64_t i, int nr,
-                                               float* __restrict__ Z, float* __restrict__ R, double (&rmax)[4]) {
+                                               float* __restrict__ Z, float* __restrict__ R, double (&rmax)[4],
+                                               double& mass) {
   const bool valid = i < nr;
   const int r = valid ? (A.rows ? A.rows[i] : (int)i) : 0;
   const bool live = valid && A.solved[r];
@@ -579,6 +606,7 @@ __device__ __forceinline__ void refresh_row_v4(const RefreshArgs& A, const Lanes
     s += __shfl_xor(s, off, 64);
     rs += __shfl_xor(rs, off, 64);
   }
+  double m = 0.0;  // max_c ratio/eps_c of this lane
   bool viol = false;
   if (valid && L.e == 0 && 4 * L.l < A.C1) {
     const int64_t zo = (int64_t)r * A.ld;
@@ -587,11 +615,12 @@ __device__ __forceinline__ void refresh_row_v4(const RefreshArgs& A, const Lanes
         A.diag64[r] = 1.0;
         A.diag32[r] = 1.f;
       }
+      const int pc = parked_class(A, r);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int c = 4 * L.l + q;
         if (c < A.C1) {
-          Z[zo + c] = (c == C) ? 1.f : 0.f;
+          Z[zo + c] = (c == pc) ? 1.f : 0.f;
           R[zo + c] = 0.f;
         }
       }
@@ -613,12 +642,17 @@ __device__ __forceinline__ void refresh_row_v4(const RefreshArgs& A, const Lanes
           const double ratio = fabs(res) / dg;
           viol |= ratio > A.eps[c];
           rmax[q] = fmax(rmax[q], ratio);
+          m = fmax(m, ratio / A.eps[c]);
         }
       }
     }
   }
   const unsigned long long bal = __ballot(viol);
   if (valid && L.lane == 0 && bal && A.viol) A.viol[atomicAdd(A.nviol, 1)] = r;
+  if (A.mass) {
+    for (int off = 1; off < 64; off <<= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if (valid && L.lane == 0 && m > 1.0) mass += m;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_refresh_v4(RefreshArgs A, int pow2, int xcd, float* __restrict__ Z,
@@ -640,7 +674,9 @@ __global__ __launch_bounds__(256) void k_refresh_v4(RefreshArgs A, int pow2, int
     waves = (int64_t)gridDim.x * 4;
   }
   double rmax[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int64_t i = lo + w0; i < hi; i += waves) refresh_row_v4(A, L, i, nr, Z, R, rmax);
+  double mass = 0.0;
+  for (int64_t i = lo + w0; i < hi; i += waves) refresh_row_v4(A, L, i, nr, Z, R, rmax, mass);
+  flush_mass(A.mass, mass);
   if (!A.rho) return;
   __shared__ double sm[256];
   for (int c = threadIdx.x; c < A.C1; c += blockDim.x) sm[c] = 0.0;
@@ -650,6 +686,70 @@ __global__ __launch_bounds__(256) void k_refresh_v4(RefreshArgs A, int pow2, int
     for (int q = 0; q < 4; ++q) {
       const int c = 4 * L.l + q;
       if (c < A.C1) atomic_max_nonneg(&sm[c], rmax[q]);
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < A.C1; c += blockDim.x) atomic_max_nonneg(A.rho + c, sm[c]);
+}
+
+// Residual verification of every solved row after a global solve (refresh mode 3): the row
+// equations (diag, parking) are unchanged since the last refresh of the changed rows, so
+// R = rhs - diag*Z + W Z needs no per-entry look-up of the neighbour's kind: parked rows hold
+// exactly the values an unmasked neighbour sum needs (class indicator on labeled rows, zero
+// elsewhere), and the labeled-neighbour part of rhs arrives through W Z.  FP64 accumulation on
+// the pipelined gathers of rowpipe.h; writes R (FP32), the violator list and column maxima.
+struct VerifyEpilogue {
+  const LanesP& L;
+  const RefreshArgs& A;
+  const float* __restrict__ Z;
+  float* __restrict__ R;
+  double rmax[4] = {0.0, 0.0, 0.0, 0.0};
+  float4 zr;
+  __device__ __forceinline__ bool owner() const { return L.e == 0 && 4 * L.l < A.C1; }
+  __device__ __forceinline__ void pre(int r) {
+    zr = owner() ? *(reinterpret_cast<const float4*>(Z + (int64_t)r * A.ld) + L.l) : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  // rows outside the solved set keep their parked state (R = 0)
+  __device__ __forceinline__ void post(int r, bool live, const double (&acc)[4]) {
+    if (!live) return;  // wave-uniform
+    bool viol = false;
+    if (owner()) {
+      const double dg = A.diag64[r];
+      const int C = A.C1 - 1;
+      const float zv[4] = {zr.x, zr.y, zr.z, zr.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c = 4 * L.l + q;
+        if (c < A.C1) {
+          const double rh = c == C ? dg : A.eta * A.prior;
+          const double res = rh - dg * (double)zv[q] + acc[q];
+          R[(int64_t)r * A.ld + c] = (float)res;
+          const double ratio = fabs(res) / dg;
+          viol |= ratio > A.eps[c];
+          rmax[q] = fmax(rmax[q], ratio);
+        }
+      }
+    }
+    const unsigned long long bal = __ballot(viol);
+    if (L.lane == 0 && bal && A.viol) A.viol[atomicAdd(A.nviol, 1)] = r;
+  }
+};
+
+template <int S>
+__global__ __launch_bounds__(256) void k_verify(RefreshArgs A, int xcd, int xbytes, const float* __restrict__ Z,
+                                                float* __restrict__ R) {
+  const RowPipe G(A.nrows, A.C1, xcd, A.solved, A.indptr, A.rowlen, A.col, A.w, Z, A.ld, xbytes);
+  VerifyEpilogue E{G.L, A, Z, R};
+  run_rows<S, double>(G, E);
+  if (!A.rho) return;
+  __shared__ double sm[256];
+  for (int c = threadIdx.x; c < A.C1; c += blockDim.x) sm[c] = 0.0;
+  __syncthreads();
+  if (E.owner()) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = 4 * G.L.l + q;
+      if (c < A.C1) atomic_max_nonneg(&sm[c], E.rmax[q]);
     }
   }
   __syncthreads();
@@ -668,8 +768,10 @@ __global__ __launch_bounds__(256) void k_refresh(RefreshArgs A, int g_shift, int
   double rmax[CPL];
 #pragma unroll
   for (int k = 0; k < CPL; ++k) rmax[k] = 0.0;
+  double mass = 0.0;
   for (int64_t base = wave_first; base < nr; base += groups)
-    refresh_row<TZ, TR, CPL>(A, L, base + (g0 - wave_first), nr, Z, R, rmax);
+    refresh_row<TZ, TR, CPL>(A, L, base + (g0 - wave_first), nr, Z, R, rmax, mass);
+  flush_mass(A.mass, mass);
   if (!A.rho) return;
   // column maxima: across the wave's row groups, then the workgroup (LDS), then one atomic
   // per column per workgroup
@@ -952,6 +1054,7 @@ struct LP {
   int32_t* viol = nullptr;
   double* eps64 = nullptr;
   double* rho = nullptr;
+  double* mass = nullptr;       // refresh of the changed rows: sum of violators' max ratio/eps
   uint8_t* mark = nullptr;      // FP64 path marks
   double* dsnap = nullptr;      // FP64 path increments of the current candidates
   int32_t* cand64 = nullptr;
@@ -992,14 +1095,15 @@ constexpr int kStatWords = kStatRho + 256;
 // counters, push list count and round, edge visits, push statistics, and (if rho) rho[C1].
 __global__ void k_status_gather(const int32_t* __restrict__ cnt, const int32_t* __restrict__ bcount,
                                 const int32_t* __restrict__ rnd, const unsigned long long* __restrict__ edges,
-                                const unsigned long long* __restrict__ stats, const double* __restrict__ rho,
-                                int C1, int64_t* __restrict__ out) {
+                                const unsigned long long* __restrict__ stats, const double* __restrict__ mass,
+                                const double* __restrict__ rho, int C1, int64_t* __restrict__ out) {
   const int t = threadIdx.x;
   if (t < CNT_N) out[t] = cnt[t];
   if (t == 0) {
     out[CNT_N] = *bcount;
     out[CNT_N + 1] = rnd[0];
     out[CNT_N + 2] = (int64_t)*edges;
+    out[CNT_N + 9] = __double_as_longlong(*mass);
   }
   if (t < 4) out[CNT_N + 3 + t] = (int64_t)stats[t];
   if (rho)
@@ -1041,6 +1145,7 @@ int lp_free(LP* h) {
   if (h->pcg_exec) (void)hipGraphExecDestroy(h->pcg_exec);
   if (h->bk) (void)hipFree(h->bk);
   if (h->gflags) (void)hipFree(h->gflags);
+  if (h->mass) (void)hipFree(h->mass);
   if (h->bpcg_exec) (void)hipGraphExecDestroy(h->bpcg_exec);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   delete h;
@@ -1077,6 +1182,7 @@ RefreshArgs refresh_args(LP* h, int which) {
   A.viol = h->viol;
   A.nviol = h->cnt + CNT_VIOL;
   A.rho = h->rho;
+  A.mass = which == 0 ? h->mass : nullptr;
   return A;
 }
 
@@ -1109,7 +1215,7 @@ extern "C" int dlp_lp_create(const dlp_lp_desc* d, void** handle) {
   A_(h->S.bcount, 1); A_(h->S.edges, 1); A_(h->S.stats, 4);
   int32_t* bst = nullptr;
   float* eps32 = nullptr;
-  A_(bst, 1); A_(eps32, d->C1); A_(h->gflags, 4);
+  A_(bst, 1); A_(eps32, d->C1); A_(h->gflags, 4); A_(h->mass, 1);
 #undef A_
   h->S.bstamp = bst;
   h->S.eps = eps32;
@@ -1410,12 +1516,33 @@ extern "C" int dlp_lp_ground_flags(void* handle, int32_t* out, void* stream) {
 }
 
 extern "C" int dlp_lp_refresh(void* handle, int which, int prec, void* stream) {
-  DLP_CHECK_ARG(handle && which >= 0 && which <= 2 && (prec == 0 || prec == 1), "lp_refresh: bad argument");
+  DLP_CHECK_ARG(handle && which >= 0 && which <= 3 && (prec == 0 || prec == 1), "lp_refresh: bad argument");
   LP* h = (LP*)handle;
   DLP_CHECK_ARG(prec == 0 || (h->d.Z64 && h->d.R64), "lp_refresh: FP64 state not attached");
   hipStream_t st = (hipStream_t)stream;
   DLP_CHECK_HIP(hipMemsetAsync(h->cnt + CNT_VIOL, 0, sizeof(int32_t), st));
   DLP_CHECK_HIP(hipMemsetAsync(h->rho, 0, sizeof(double) * h->d.C1, st));
+  if (which == 0) DLP_CHECK_HIP(hipMemsetAsync(h->mass, 0, sizeof(double), st));
+  const long long xbytes = (long long)h->d.n * h->d.ld * 4;
+  if (which == 3) {
+    // verification of every solved row with unchanged equations: the pipelined kernel when the
+    // FP32 layout allows it, else the full refresh
+    if (prec == 0 && (h->d.ld % 4) == 0 && h->d.C1 <= 256 && tuning().vec4 && tuning().pipe &&
+        h->d.n < (1 << 24) && xbytes < (1LL << 31)) {
+      RefreshArgs A = refresh_args(h, 2);
+      const int xcd = tuning().xcd != 0;
+      const void* kfn = reinterpret_cast<const void*>(&k_verify<4>);
+      long long g = resident_grid(kfn);
+      const long long need = ((long long)h->d.n + 3) / 4;
+      if (g > need) g = need;
+      if (xcd) g = g < 8 ? 8 : g & ~7LL;
+      hipLaunchKernelGGL((k_verify<4>), dim3((unsigned)g), dim3(kBlock), 0, st, A, xcd, (int)xbytes, h->d.Z32,
+                         h->d.R32);
+      DLP_CHECK_LAUNCH();
+      return DLP_OK;
+    }
+    which = 2;
+  }
   RefreshArgs A = refresh_args(h, which);
   const RowMap& m = h->map;
   // listed rows: the changed list is O(batch); the touched list can reach n
@@ -1606,13 +1733,14 @@ extern "C" int dlp_lp_status(void* handle, int64_t* out, double* rho, void* stre
   hipStream_t st = (hipStream_t)stream;
   const int C1 = h->d.C1;
   hipLaunchKernelGGL(k_status_gather, dim3(1), dim3(kBlock), 0, st, h->cnt, h->S.bcount, h->S.rnd, h->S.edges,
-                     h->S.stats, rho ? h->rho : nullptr, C1, h->stat_dev);
+                     h->S.stats, h->mass, rho ? h->rho : nullptr, C1, h->stat_dev);
   DLP_CHECK_LAUNCH();
-  const size_t words = rho ? (size_t)kStatRho + C1 : (size_t)CNT_N + 7;
+  const size_t words = rho ? (size_t)kStatRho + C1 : (size_t)CNT_N + 10;
   DLP_CHECK_HIP(hipMemcpyAsync(h->host, h->stat_dev, words * sizeof(int64_t), hipMemcpyDeviceToHost, st));
   DLP_CHECK_HIP(hipStreamSynchronize(st));
   const volatile int64_t* hs = h->host;
   for (int i = 0; i < CNT_N + 7; ++i) out[i] = hs[i];
+  out[CNT_N + 9] = hs[CNT_N + 9];
   if (rho) std::memcpy(rho, (const void*)(h->host + kStatRho), sizeof(double) * C1);
   out[CNT_N + 7] = h->push_launched;
   out[CNT_N + 8] = (int64_t)(h->push_ms * 1000.0);  // microseconds
@@ -1648,8 +1776,8 @@ static PcgBufs pcg_bufs(LP* h) {
 static int pcg_iteration(LP* h, const PcgBufs& b, hipStream_t st) {
   // AP = diag*P - W_UU P, with p^T A p accumulated in the same pass when the layout allows
   if ((b.ld % 4) == 0 && tuning().vec4 && tuning().fuse_dot) {
-    int rc = row_operator_dot(b.n, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.solved, b.P, b.ld,
-                              b.diag, -1.f, b.AP, b.ld, b.pAp, st, b.ctl);
+    int rc = pcg_operator(b.n, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.solved, b.P, b.ld, b.diag,
+                          -1.f, b.AP, b.ld, b.pAp, st, b.ctl);
     if (rc) return rc;
     return pcg_finish_iteration(b, st, true);
   }
@@ -1757,6 +1885,33 @@ extern "C" int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int 
   return DLP_OK;
 }
 
+// Reported scores [F, h] in FP64: solved rows as held, rows outside the solved set as F = 0,
+// h = 1 (whatever their parked state).
+template <typename TZ>
+__global__ void k_scores(int n, int C1, const TZ* __restrict__ Z, int64_t ld, const uint8_t* __restrict__ solved,
+                         double* __restrict__ out, int64_t ldo) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n * C1) return;
+  const int r = (int)(i / C1), c = (int)(i - (int64_t)r * C1);
+  out[(int64_t)r * ldo + c] = solved[r] ? (double)Z[(int64_t)r * ld + c] : (c == C1 - 1 ? 1.0 : 0.0);
+}
+
+extern "C" int dlp_lp_scores(void* handle, double* out, int64_t ldo, void* stream) {
+  DLP_CHECK_ARG(handle && out, "lp_scores: null argument");
+  LP* h = (LP*)handle;
+  DLP_CHECK_ARG(ldo >= h->d.C1, "lp_scores: ldo < C1");
+  hipStream_t st = (hipStream_t)stream;
+  const long long total = (long long)h->d.n * h->d.C1;
+  if (h->d.Z64)
+    hipLaunchKernelGGL(k_scores<double>, dim3(grid_for(total)), dim3(kBlock), 0, st, h->d.n, h->d.C1, h->d.Z64,
+                       h->d.ld, h->d.solved, out, ldo);
+  else
+    hipLaunchKernelGGL(k_scores<float>, dim3(grid_for(total)), dim3(kBlock), 0, st, h->d.n, h->d.C1, h->d.Z32,
+                       h->d.ld, h->d.solved, out, ldo);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
 // Cumulative timing of the global solve: [0] iterations, [1] microseconds (HIP events).
 extern "C" int dlp_lp_pcg_stats(void* handle, int64_t* out) {
   DLP_CHECK_ARG(handle && out, "lp_pcg_stats: null argument");
@@ -1779,8 +1934,8 @@ extern "C" int dlp_lp_time_operator(void* handle, int reps, double* ms_out, void
   const bool fused = (b.ld % 4) == 0 && tuning().vec4 && tuning().fuse_dot;
   auto launch = [&]() -> int {
     if (fused)
-      return row_operator_dot(b.n, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.solved, b.P, b.ld,
-                              b.diag, -1.f, b.AP, b.ld, b.pAp, st);
+      return pcg_operator(b.n, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.solved, b.P, b.ld, b.diag,
+                          -1.f, b.AP, b.ld, b.pAp, st, nullptr);
     return dlp_row_operator(b.n, nullptr, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, DLP_F32, h->d.solved,
                             b.P, b.ld, DLP_F32, b.diag, -1.0, nullptr, 0, b.AP, b.ld, DLP_F32, 0, 20.0, st);
   };

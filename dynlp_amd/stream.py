@@ -72,7 +72,7 @@ class EdgeStreamLP:
 
     def __init__(self, g: Graph, eps=1e-6, eps_h=0.05, fp64=False, eta=0.0, slack_min=4,
                  slack_frac=0.25, push_scale=0.9, rounds_per_launch=4, use_graph=True, max_rounds=200_000,
-                 push_frac=0.05, push_max_rows=4096, push_budget=1.0, pcg_iters_per_launch=6,
+                 push_frac=0.05, push_max_rows=4096, push_budget=1.0, push_mass=0.15, pcg_iters_per_launch=6,
                  pcg_max_iter=20_000, pcg_mode="column", incremental_ground=True):
         self.g, self.n, self.K = g, g.n, g.K
         self.C = 1 if g.K == 2 else g.K
@@ -81,6 +81,7 @@ class EdgeStreamLP:
         self.rounds_per_launch, self.use_graph, self.max_rounds = rounds_per_launch, use_graph, max_rounds
         self.push_scale = push_scale
         self.push_frac, self.push_budget, self.push_max_rows = push_frac, push_budget, push_max_rows
+        self.push_mass = float(push_mass)
         self.pcg_iters_per_launch, self.pcg_max_iter = pcg_iters_per_launch, pcg_max_iter
         if pcg_mode not in ("column", "block"):
             raise ValueError(f"pcg_mode {pcg_mode!r}: 'column' or 'block'")
@@ -150,7 +151,7 @@ class EdgeStreamLP:
         return dict(changed=int(s[0]), missing=int(s[1]), overflow=int(s[2]), toggled=int(s[3]), viol=int(s[4]),
                     touched=int(s[6]), round=int(s[7]), edges=int(s[8]), cand_total=int(s[9]),
                     push_total=int(s[10]), scatter_total=int(s[11]),
-                    rounds_launched=int(s[13]), push_us=int(s[14]))
+                    rounds_launched=int(s[13]), push_us=int(s[14]), mass=float(s[15:16].view(np.float64)[0]))
 
     def _push(self, budget=0):
         r, ok = ct.c_int64(0), ct.c_int32(0)
@@ -201,14 +202,22 @@ class EdgeStreamLP:
             return s
         s = self.status()
         ok = False
-        if not force_global and s["viol"] <= min(self.push_frac * self.n, self.push_max_rows):
+        # push only when it can finish cheaply: few violators, and a residual mass (sum of the
+        # violators' max |R|/(eps*diag), about the row pushes a push-only repair takes) that
+        # is small next to the graph (a global solve costs a few dozen passes over all rows)
+        tried = False
+        if (not force_global and s["viol"] <= min(self.push_frac * self.n, self.push_max_rows)
+                and s["mass"] <= self.push_mass * self.n):
+            tried = True
             r, ok = self._push(budget=int(self.push_budget * max(self.nnz, 1)))
             st.iters += r
         if not ok:
-            st.path = "push+pcg" if not force_global else "pcg"
+            st.path = "push+pcg" if tried else "pcg"
             it, _ = self._pcg()
             st.extra["pcg_iters"] = st.extra.get("pcg_iters", 0) + it
-            _lib.call("dlp_lp_refresh", self.h, 2, 0, stream())     # FP64 residual of every row
+            # FP64 residual of every solved row; their equations are those of the changed-row
+            # refresh, so the verification pass (mode 3) suffices
+            _lib.call("dlp_lp_refresh", self.h, 3, 0, stream())
             s = self.status()
             if s["viol"]:
                 r, ok = self._push()
@@ -363,8 +372,10 @@ class EdgeStreamLP:
         return self._rho.copy(), s["viol"]
 
     def scores_with_h(self):
-        Z = self.Z64 if self.fp64 else self.Z32
-        return Z[:, : self.C1].double().cpu().numpy()
+        """[F, h] of every vertex (FP64, host): rows outside the solved set as F = 0, h = 1."""
+        out = torch.empty((self.n, self.C1), dtype=torch.float64, device=device())
+        _lib.call("dlp_lp_scores", self.h, ptr(out), self.C1, stream())
+        return out.cpu().numpy()
 
     def solved_mask(self):
         return self.solved.cpu().numpy().astype(bool)

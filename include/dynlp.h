@@ -77,7 +77,9 @@ int dlp_row_operator_dot(int n, int C, const int32_t* indptr, const int32_t* row
  *   key 2 = packed lane groups, ceil(C/4) column lanes per row (1 default) or a power of two;
  *   key 3 = XCD-aware row order (each XCD walks one contiguous eighth of the rows; 1 default);
  *   key 4 = blocks of the fused-dot operator (0 default = the resident blocks of the device);
- *   key 5 = fuse p^T A p into the global solve's operator launch (1 default). */
+ *   key 5 = fuse p^T A p into the global solve's operator launch (1 default);
+ *   key 6 = gathers in flight per lane in the global solve's pipelined operator (4, 6 or 8, 4
+ *           default; 0 = the plain fused operator). */
 int dlp_tune_set(int key, int value);
 
 /* ---- Column residual ratios ------------------------------------------------------------
@@ -257,11 +259,20 @@ int dlp_lp_ground_incremental(void* handle, int nd, const int32_t* du, const int
  * the last full pass), out[2] = grounding calls (host copy; synchronises). */
 int dlp_lp_ground_flags(void* handle, int32_t* out, void* stream);
 /* Row refresh from scratch: s, diag, rhs and R = rhs - diag*Z + W_UU Z in FP64 for the rows of
- * `which` (0 changed list, 1 rows touched by pushes in this batch, 2 all rows); inert rows
- * parked (diag 1, F 0, h 1, R 0).  prec 0: Z32 -> R32 (rounded), prec 1: Z64 -> R64.
- * Violators (|R_c| > eps_c*diag) are listed for the next push; column maxima of |R|/diag go to
- * the status.  Replaces solvers.py:404-407 (refresh) and :370-373 (full residual). */
+ * `which` (0 changed list, 1 rows touched by pushes in this batch, 2 all rows, 3 all solved
+ * rows whose equations are unchanged since their last refresh: the verification after a
+ * global solve, same result as 2 without re-deriving s and diag); rows outside the solved set
+ * are parked with diag 1, R 0 and Z = the indicator of the class column on labeled rows, zeros
+ * elsewhere (h 0) -- dlp_lp_scores reports them as F 0, h 1.  prec 0: Z32 -> R32 (rounded),
+ * prec 1: Z64 -> R64.  Violators (|R_c| > eps_c*diag) are listed for the next push; column
+ * maxima of |R|/diag go to the status.  Replaces solvers.py:404-407 (refresh) and :370-373
+ * (full residual). */
 int dlp_lp_refresh(void* handle, int which, int prec, void* stream);
+/* Scores [F, h] of every vertex in FP64 (n x C1, leading dimension ldo): solved rows from the
+ * state (Z64 when attached, else Z32), every other row F = 0, h = 1 -- the layout the oracle's
+ * solve_system reports.  The reference reports F on the solved rows (Solver.scores,
+ * solvers.py:186-187). */
+int dlp_lp_scores(void* handle, double* out, int64_t ldo, void* stream);
 /* Asynchronous FP32 push (chaotic relaxation, hardware float atomics) from the current
  * violators until no vertex violates or max_rounds; one kernel per round (the violation test
  * of round t+1 is fused with its pushes), replayed from a captured hipGraph of
@@ -308,8 +319,11 @@ int dlp_lp_violators(void* handle, int32_t** list, int32_t** count);
  * insertions, [3] toggled vertices, [4] violators, [5] FP64-path candidates, [6] rows touched
  * by pushes, [7] round number, [8] edge visits of this batch, cumulative push work [9]
  * candidates, [10] pushes, [11] entries scattered by pushes, [12] reserved,
- * [13] push rounds launched, [14] their summed duration in microseconds (HIP events);
- * rho (C1, may be NULL): column maxima of the last refresh.  `out` holds 16 entries. */
+ * [13] push rounds launched, [14] their summed duration in microseconds (HIP events),
+ * [15] the bits of an FP64 "push mass": over the rows the last changed-row refresh found
+ * violating, the sum of max_c |R_c|/(eps_c*diag) (about the row pushes a repair by push alone
+ * would take); rho (C1, may be NULL): column maxima of the last refresh.  `out` holds 16
+ * entries. */
 int dlp_lp_status(void* handle, int64_t* out, double* rho, void* stream);
 
 #ifdef __cplusplus

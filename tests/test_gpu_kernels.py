@@ -70,7 +70,7 @@ def test_row_operator_masked_subset_mixed_precision():
     np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-12, atol=1e-12)
 
 
-TUNE_DEFAULTS = {0: 0, 1: 1, 2: 1, 3: 1, 4: 0, 5: 1}
+TUNE_DEFAULTS = {0: 0, 1: 1, 2: 1, 3: 1, 4: 0, 5: 1, 6: 4}
 
 
 @pytest.fixture
@@ -78,7 +78,7 @@ def tune():
     from dynlp_amd import _lib
 
     def set_(**kv):
-        keys = {"unroll": 0, "vec4": 1, "pack": 2, "xcd": 3, "dot_grid": 4, "fuse_dot": 5}
+        keys = {"unroll": 0, "vec4": 1, "pack": 2, "xcd": 3, "dot_grid": 4, "fuse_dot": 5, "pipe": 6}
         for k, v in kv.items():
             _lib.call("dlp_tune_set", keys[k], int(v))
     yield set_

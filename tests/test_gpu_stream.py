@@ -60,16 +60,18 @@ def test_stream_matches_rebuilt_system(fp64, K):
         check_against_oracle(lp, g, es, eps, f"batch {t}")
 
 
-@pytest.mark.parametrize("pack,xcd,vec4", [(0, 0, 1), (0, 1, 1), (1, 0, 1), (1, 1, 0)])
-def test_stream_lane_layouts_and_row_orders(pack, xcd, vec4):
-    """The FP32 service path (refresh, operator with fused dot, flat PCG passes) under the
-    power-of-two lane layout, the plain row order and scalar loads; the defaults (packed lanes,
-    XCD-aware row order, 16-byte loads) run in the tests above."""
+@pytest.mark.parametrize("pack,xcd,vec4,pipe", [(0, 0, 1, 0), (0, 1, 1, 0), (1, 0, 1, 4), (1, 1, 1, 6),
+                                               (1, 1, 0, 8)])
+def test_stream_lane_layouts_and_row_orders(pack, xcd, vec4, pipe):
+    """The FP32 service path (refresh, global-solve operator, PCG passes) under the power-of-two
+    lane layout, the plain row order, scalar loads, the plain fused operator and every depth of
+    the pipelined one; the defaults run in the tests above."""
     from dynlp_amd import _lib
     from dynlp_amd.stream import EdgeStreamLP
     _lib.call("dlp_tune_set", 1, vec4)
     _lib.call("dlp_tune_set", 2, pack)
     _lib.call("dlp_tune_set", 3, xcd)
+    _lib.call("dlp_tune_set", 6, pipe)
     try:
         g = graphs.sbm(3000, 8, 10, 0.8, 8, 0.02, seed=4)
         lp = EdgeStreamLP(g, eps=1e-6)
@@ -82,8 +84,9 @@ def test_stream_lane_layouts_and_row_orders(pack, xcd, vec4):
                                zip(b.del_u.tolist(), b.del_v.tolist()))
         st = lp.load_state(np.zeros((g.n, lp.C1), np.float32))   # a global solve from zero too
         assert "pcg" in st.path, st.path
-        check_against_oracle(lp, g, es, 1e-6, f"pack={pack} xcd={xcd} vec4={vec4}")
+        check_against_oracle(lp, g, es, 1e-6, f"pack={pack} xcd={xcd} vec4={vec4} pipe={pipe}")
     finally:
+        _lib.call("dlp_tune_set", 6, 4)
         _lib.call("dlp_tune_set", 1, 1)
         _lib.call("dlp_tune_set", 2, 1)
         _lib.call("dlp_tune_set", 3, 1)
@@ -128,6 +131,40 @@ def test_fragmenting_stream_regrounds():
                            zip(b.del_u.tolist(), b.del_v.tolist()))
         check_against_oracle(lp, g, es, eps, f"batch {t}")
     assert toggles > 0
+
+
+def test_verify_pass_matches_full_refresh():
+    """The verification pass (refresh mode 3, no per-entry kind look-up, parked rows summed
+    unmasked) gives the residual, violators and column maxima of the full refresh (mode 2), on a
+    fragmenting graph with inert and labeled rows; parked rows report F = 0, h = 1."""
+    from dynlp_amd import _lib
+    from dynlp_amd.backend import stream
+    from dynlp_amd.stream import EdgeStreamLP
+    g = graphs.sbm(4000, 16, 3.0, 0.9, 16, 0.01, seed=3, unit=True)
+    lp = EdgeStreamLP(g, eps=1e-6)
+    lp.initial_solve()
+    batches, _ = graphs.update_stream(g, 3, 400, ins_frac=0.3, seed=3, inter_first=True)
+    for t, b in enumerate(batches):
+        lp.apply_and_solve(b)
+        # perturb the solved rows so the residual is far from zero, then compare the two passes
+        import torch
+        sm = lp.solved.bool()
+        noise = torch.as_tensor(np.random.default_rng(t).normal(0, 1e-3, (g.n, lp.C1)), dtype=torch.float32,
+                                device=lp.Z32.device)
+        lp.Z32[:, : lp.C1] += noise * sm[:, None]
+        out = []
+        for mode in (3, 2):
+            _lib.call("dlp_lp_refresh", lp.h, mode, 0, stream())
+            s = lp.status()
+            out.append((lp.R32[:, : lp.C1].double().cpu().numpy(), lp._rho.copy(), s["viol"]))
+        (R3, rho3, v3), (R2, rho2, v2) = out
+        solved = lp.solved_mask()
+        assert (~solved).any()
+        np.testing.assert_allclose(R3, R2, rtol=1e-6, atol=1e-12)
+        np.testing.assert_allclose(rho3, rho2, rtol=1e-6)
+        assert v3 == v2 and v3 > 0
+        Zs = lp.scores_with_h()
+        assert np.all(Zs[~solved, : lp.C] == 0) and np.all(Zs[~solved, lp.C] == 1)
 
 
 def test_overflow_regrow_keeps_results():
