@@ -7,7 +7,8 @@ with h).  After the initial solve, a stream of requests, each a batch of `--batc
 updates (50% insertions of new uniform pairs, 50% deletions of existing edges; seed S+2).  A step
 is one request: apply on the device-resident slack CSR, re-ground (connected components),
 refresh the changed rows' FP64 residual, repair to max |R_uc|/diag_u <= 1e-6 (push or global PCG
-on FP32 vectors, replayed from hipGraphs) and verify in FP64.  Latency is submit -> certified.
+on FP32 vectors with BF16 search directions, replayed from hipGraphs) and verify in FP64.  Latency
+is submit -> certified.
 
 Printed on stdout: one JSON line.  Progress goes to stderr.
 """
@@ -36,10 +37,11 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if xs else float("nan")
 
 
-def operator_bytes(n, C1, stored):
+def operator_bytes(n, C1, stored, p_bytes=4):
     """Algorithmic bytes of one A*P launch (each distinct element once): row starts and
-    lengths, column/weight of every stored entry, the solved mask, P read, diag, AP written."""
-    return 4 * (n + 1) + 4 * n + 8 * stored + n + 4 * n * C1 + 4 * n + 4 * n * C1
+    lengths, column/weight of every stored entry, the solved mask, P read (p_bytes per value:
+    2 for BF16 directions), diag, AP written (FP32)."""
+    return 4 * (n + 1) + 4 * n + 8 * stored + n + p_bytes * n * C1 + 4 * n + 4 * n * C1
 
 
 def pmc_probe(g):
@@ -51,7 +53,7 @@ def pmc_probe(g):
     lp.initial_solve()
     torch.cuda.synchronize()
     op_ms = lp.time_operator(reps=4)           # 1 + 4 launches
-    nbytes = operator_bytes(g.n, lp.C1, lp.stored_entries())
+    nbytes = operator_bytes(g.n, lp.C1, lp.stored_entries(), 2 if lp.pcg_directions == "bf16" else 4)
     src = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
@@ -131,13 +133,17 @@ def main(argv=None):
     # ---- roofline of the dominant kernel (the global solve's A*P launch) --------------------
     op_ms = lp.time_operator(reps=20)
     stored = lp.stored_entries()
-    nbytes = operator_bytes(g.n, lp.C1, stored)
+    bf = lp.pcg_directions == "bf16"
+    nbytes = operator_bytes(g.n, lp.C1, stored, 2 if bf else 4)
     achieved = nbytes / (op_ms * 1e-3) / 1e9
-    roofline = {"kernel": "k_row_operator_v4_dot<8> (A*P with fused p^T A p, the global solve)",
+    kname = ("k_pcg_operator<4, BF16x4> (A*P on BF16 directions with fused p^T A p, the global solve)" if bf
+             else "k_pcg_operator<4, F32x4> (A*P with fused p^T A p, the global solve)")
+    roofline = {"kernel": kname,
                 "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "avg_launch_ms": round(op_ms, 4), "algorithmic_bytes": int(nbytes)}
     log(f"A*P launch: {op_ms:.3f} ms, {nbytes / 1e6:.0f} MB -> {achieved:.0f} GB/s")
+    pcg_dirs = lp.pcg_directions
     pcg_iters = pcg1["pcg_iters"] - pcg0["pcg_iters"]
     pcg_ms = (pcg1["pcg_us"] - pcg0["pcg_us"]) / 1e3
     del lp
@@ -191,7 +197,8 @@ def main(argv=None):
         "data": "synthetic",
         "config": {"workload": "sbm-1m-k64 edge-update stream" if a.scale == 1.0 else f"sbm-{g.n}-k64 edge-update stream",
                    "n": g.n, "blocks": 64, "deg": 20, "p_in": 0.8, "k": 64, "label_frac": 0.005,
-                   "batch_edges": a.batch_edges, "ins_frac": 0.5, "eps": 1e-6, "seed": a.seed},
+                   "batch_edges": a.batch_edges, "ins_frac": 0.5, "eps": 1e-6, "seed": a.seed,
+                   "pcg_directions": pcg_dirs},
         "p50_ms": round(percentile(lat, 50), 3), "p99_ms": round(percentile(lat, 99), 3),
         "rho_max": max(rhos), "paths": paths,
         "pcg_iters_per_request": round(pcg_iters / a.steps, 2), "pcg_ms_per_request": round(pcg_ms / a.steps, 3),

@@ -22,6 +22,10 @@ struct PcgBufs {
   // [0] done (set once the violation is <= 1: every later pass of a captured group returns at
   // once), [1] iterations completed; null disables the device-side exit
   int32_t* ctl;
+  // BF16 directions (null: P in FP32): n x ldh values, ldh % 8 == 0; rp = r.p per column
+  uint16_t* P16;
+  int64_t ldh;
+  double* rp;
 };
 
 // P = R/diag, rz = sum R*R/diag, pAp = 0.
@@ -38,8 +42,8 @@ int row_operator_dot(int n, int C, const int32_t* indptr, const int32_t* rowlen,
 // The global solve's operator: pipelined row gathers, rows outside rowmask skipped, neighbour
 // rows unmasked (P = 0 outside the solved set), fused p^T A p; skip = device exit flag (spmm.hip).
 int pcg_operator(int n, int C, const int32_t* indptr, const int32_t* rowlen, const int32_t* indices, const float* w,
-                 const uint8_t* rowmask, const float* X, int64_t ldx, const float* a, float b, float* out,
-                 int64_t ldo, double* dot, hipStream_t st, const int32_t* skip);
+                 const uint8_t* rowmask, const float* X, int64_t ldx, const uint16_t* X16, int64_t ldh,
+                 const float* a, float b, float* out, int64_t ldo, double* dot, hipStream_t st, const int32_t* skip);
 
 // Blocks of a 256-thread kernel that are resident on the current device at once (spmm.hip).
 int resident_grid(const void* kernel);

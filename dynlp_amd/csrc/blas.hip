@@ -4,7 +4,7 @@
 // and rows over the grid, so each row is one coalesced access.  Column reductions accumulate
 // in FP64: per lane, then per workgroup in LDS, then one FP64 atomic per column per workgroup.
 #include "blas.h"
-#include "rowgather.h"
+#include "rowpipe.h"
 
 namespace dlp {
 
@@ -148,42 +148,85 @@ __global__ __launch_bounds__(256) void k_pcg_dir(int n, int C, int lc_shift, flo
 // no lane idles on a power-of-two column layout.  Quad components at columns >= C are left
 // unchanged and excluded from the reductions.
 
-// pcg start: P = R/diag, rz[c] += sum R*R/diag.
-__global__ __launch_bounds__(256) void k_pcg_start4(int n, int Q4, int C, float4* __restrict__ P,
-                                                    const float4* __restrict__ R, const float* __restrict__ diag,
-                                                    double* __restrict__ rz) {
+// The direction block P of the flat passes: FP32 (the n x Q4 float4 layout of Z and R) or BF16
+// (rows of H4 groups of 4 values, 8 bytes each; stores round to nearest even).
+template <bool BF>
+struct PBlock {
+  void* base;
+  int64_t H4;
+  __device__ __forceinline__ float4 load(int64_t i, int64_t r, int q4) const {
+    if constexpr (BF) {
+      const uint2 u = reinterpret_cast<const uint2*>(base)[r * H4 + q4];
+      return make_float4(bf16_lo(u.x), bf16_hi(u.x), bf16_lo(u.y), bf16_hi(u.y));
+    } else {
+      return reinterpret_cast<const float4*>(base)[i];
+    }
+  }
+  // stores v and returns the value held (rounded to BF16 in the BF16 block)
+  __device__ __forceinline__ float4 store(int64_t i, int64_t r, int q4, const float4& v) const {
+    if constexpr (BF) {
+      uint2 u;
+      u.x = bf16_pack(v.x, v.y);
+      u.y = bf16_pack(v.z, v.w);
+      reinterpret_cast<uint2*>(base)[r * H4 + q4] = u;
+      return make_float4(bf16_lo(u.x), bf16_hi(u.x), bf16_lo(u.y), bf16_hi(u.y));
+    } else {
+      reinterpret_cast<float4*>(base)[i] = v;
+      return v;
+    }
+  }
+};
+
+// Column sums of per-thread quads: LDS, then one FP64 atomic per column per workgroup.
+__device__ __forceinline__ void flush_quads(const Quad& Qd, int Q4, int C, const double (&acc)[4], double* sm,
+                                            double* __restrict__ out) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if (j < Qd.cnt) lds_add(&sm[4 * Qd.q4 + j], acc[j]);
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&out[c], sm[c]);
+}
+
+// pcg start: P = R/diag, rz[c] += sum R*R/diag; with a BF16 block also rp[c] += sum R*P as held.
+template <bool BF>
+__global__ __launch_bounds__(256) void k_pcg_start4(int n, int Q4, int C, PBlock<BF> P, const float4* __restrict__ R,
+                                                    const float* __restrict__ diag, double* __restrict__ rz,
+                                                    double* __restrict__ rp) {
   __shared__ double sm[kMaxCols + 4];
-  for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) sm[c] = 0.0;
+  __shared__ double sm2[kMaxCols + 4];
+  for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) sm[c] = sm2[c] = 0.0;
   __syncthreads();
   const Quad Qd = quad_of(Q4, C);
   const int64_t nq = (int64_t)n * Q4;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double acc[4] = {0.0, 0.0, 0.0, 0.0}, acc2[4] = {0.0, 0.0, 0.0, 0.0};
   if (Qd.active && Qd.r0 < n) {
     int64_t r = Qd.r0;
     for (int64_t i = Qd.i0; i < nq; i += Qd.step, r += Qd.rstep) {
       const float inv = 1.f / diag[r];
       const float4 rv = R[i];
-      float4 p = P[i];
-      p.x = Qd.cnt > 0 ? rv.x * inv : p.x;
-      p.y = Qd.cnt > 1 ? rv.y * inv : p.y;
-      p.z = Qd.cnt > 2 ? rv.z * inv : p.z;
-      p.w = Qd.cnt > 3 ? rv.w * inv : p.w;
-      P[i] = p;
+      float4 y = P.load(i, r, Qd.q4);
+      y.x = Qd.cnt > 0 ? rv.x * inv : y.x;
+      y.y = Qd.cnt > 1 ? rv.y * inv : y.y;
+      y.z = Qd.cnt > 2 ? rv.z * inv : y.z;
+      y.w = Qd.cnt > 3 ? rv.w * inv : y.w;
+      const float4 p = P.store(i, r, Qd.q4, y);
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (j < Qd.cnt) acc[j] += (double)quad_comp(rv, j) * (double)quad_comp(p, j);
+      for (int j = 0; j < 4; ++j) {
+        if (j < Qd.cnt) {
+          acc[j] += (double)quad_comp(rv, j) * (double)quad_comp(y, j);
+          if constexpr (BF) acc2[j] += (double)quad_comp(rv, j) * (double)quad_comp(p, j);
+        }
+      }
     }
   }
-#pragma unroll
-  for (int j = 0; j < 4; ++j)
-    if (j < Qd.cnt) lds_add(&sm[4 * Qd.q4 + j], acc[j]);
-  __syncthreads();
-  for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&rz[c], sm[c]);
+  flush_quads(Qd, Q4, C, acc, sm, rz);
+  if constexpr (BF) flush_quads(Qd, Q4, C, acc2, sm2, rp);
 }
 
 // Z += a P, R -= a AP; rz_new[c] += sum R*R/diag; viol = max |R|/(eps_c*diag).
+template <bool BF>
 __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float4* __restrict__ Z,
-                                                     float4* __restrict__ R, const float4* __restrict__ P,
+                                                     float4* __restrict__ R, PBlock<BF> P,
                                                      const float4* __restrict__ AP,
                                                      const double* __restrict__ alpha,
                                                      const float* __restrict__ diag, const float* __restrict__ eps,
@@ -211,7 +254,7 @@ __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float
     for (int64_t i = Qd.i0; i < nq; i += Qd.step, r += Qd.rstep) {
       const float inv = 1.f / diag[r];
       float4 z = Z[i], rv = R[i];
-      const float4 p = P[i], ap = AP[i];
+      const float4 p = P.load(i, r, Qd.q4), ap = AP[i];
       if (Qd.cnt > 0) { z.x += a[0] * p.x; rv.x -= a[0] * ap.x; }
       if (Qd.cnt > 1) { z.y += a[1] * p.y; rv.y -= a[1] * ap.y; }
       if (Qd.cnt > 2) { z.z += a[2] * p.z; rv.z -= a[2] * ap.z; }
@@ -238,28 +281,43 @@ __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float
   if (threadIdx.x == 0) atomic_max_nonneg(viol, vmax);
 }
 
-// P = R/diag + beta P.
-__global__ __launch_bounds__(256) void k_pcg_dir4(int n, int Q4, int C, float4* __restrict__ P,
-                                                  const float4* __restrict__ R, const float* __restrict__ diag,
-                                                  const double* __restrict__ beta, const int32_t* __restrict__ skip) {
+// P = R/diag + beta P; with a BF16 block also rp[c] += sum R*P as held (the next step length's
+// numerator, exact for the rounded direction).
+template <bool BF>
+__global__ __launch_bounds__(256) void k_pcg_dir4(int n, int Q4, int C, PBlock<BF> P, const float4* __restrict__ R,
+                                                  const float* __restrict__ diag, const double* __restrict__ beta,
+                                                  double* __restrict__ rp, const int32_t* __restrict__ skip) {
   if (skip && *skip) return;
+  __shared__ double sm[kMaxCols + 4];
+  if constexpr (BF) {
+    for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) sm[c] = 0.0;
+    __syncthreads();
+  }
   const Quad Qd = quad_of(Q4, C);
   float b[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) b[j] = j < Qd.cnt ? (float)beta[4 * Qd.q4 + j] : 0.f;
   const int64_t nq = (int64_t)n * Q4;
-  if (!Qd.active || Qd.r0 >= n) return;
-  int64_t r = Qd.r0;
-  for (int64_t i = Qd.i0; i < nq; i += Qd.step, r += Qd.rstep) {
-    const float inv = 1.f / diag[r];
-    const float4 rv = R[i];
-    float4 p = P[i];
-    if (Qd.cnt > 0) p.x = rv.x * inv + b[0] * p.x;
-    if (Qd.cnt > 1) p.y = rv.y * inv + b[1] * p.y;
-    if (Qd.cnt > 2) p.z = rv.z * inv + b[2] * p.z;
-    if (Qd.cnt > 3) p.w = rv.w * inv + b[3] * p.w;
-    P[i] = p;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (Qd.active && Qd.r0 < n) {
+    int64_t r = Qd.r0;
+    for (int64_t i = Qd.i0; i < nq; i += Qd.step, r += Qd.rstep) {
+      const float inv = 1.f / diag[r];
+      const float4 rv = R[i];
+      float4 p = P.load(i, r, Qd.q4);
+      if (Qd.cnt > 0) p.x = rv.x * inv + b[0] * p.x;
+      if (Qd.cnt > 1) p.y = rv.y * inv + b[1] * p.y;
+      if (Qd.cnt > 2) p.z = rv.z * inv + b[2] * p.z;
+      if (Qd.cnt > 3) p.w = rv.w * inv + b[3] * p.w;
+      const float4 held = P.store(i, r, Qd.q4, p);
+      if constexpr (BF) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (j < Qd.cnt) acc[j] += (double)quad_comp(rv, j) * (double)quad_comp(held, j);
+      }
+    }
   }
+  if constexpr (BF) flush_quads(Qd, Q4, C, acc, sm, rp);
 }
 
 struct FlatLaunch {
@@ -270,8 +328,10 @@ struct FlatLaunch {
 
 static FlatLaunch flat_launch(const PcgBufs& b) {
   FlatLaunch f{false, 0, 0, 0};
-  const uintptr_t al = (uintptr_t)b.Z | (uintptr_t)b.R | (uintptr_t)b.P | (uintptr_t)b.AP;
-  if ((b.ld % 4) != 0 || (al & 15) != 0 || b.ld / 4 > 64 || !tuning().vec4) return f;
+  const uintptr_t al = (uintptr_t)b.Z | (uintptr_t)b.R | (uintptr_t)(b.P16 ? (void*)b.P16 : (void*)b.P) | (uintptr_t)b.AP;
+  // the tuning switch for 16-byte loads covers FP32 directions; BF16 directions exist only in
+  // this layout (checked when the mode is set)
+  if ((b.ld % 4) != 0 || (al & 15) != 0 || b.ld / 4 > 64 || (!tuning().vec4 && !b.P16)) return f;
   f.Q4 = (int)(b.ld / 4);
   f.block = kBlock;
   const long long used = f.Q4 * (kBlock / f.Q4);
@@ -288,19 +348,22 @@ static FlatLaunch flat_launch(const PcgBufs& b) {
 // rz = rz_new, then clear pAp for the next iteration.
 __global__ void k_pcg_scalars(int C, int mode, double* __restrict__ rz, double* __restrict__ rz_new,
                               double* __restrict__ pAp, double* __restrict__ alpha,
-                              double* __restrict__ beta, float* __restrict__ viol, int32_t* ctl) {
+                              double* __restrict__ beta, float* __restrict__ viol, int32_t* ctl,
+                              double* __restrict__ rp) {
   // every thread reads the flag before thread 0 may set it below (after the barrier)
   if (ctl && ctl[0]) return;
   for (int c = threadIdx.x; c < C; c += blockDim.x) {
     if (mode == 0) {
+      // step length along P as held: r.p / p.Ap (r.p = rz for an exact direction)
       const double p = pAp[c];
-      alpha[c] = p > 0.0 ? rz[c] / p : 0.0;
+      alpha[c] = p > 0.0 ? (rp ? rp[c] : rz[c]) / p : 0.0;
       rz_new[c] = 0.0;
     } else {
       const double r = rz[c];
       beta[c] = r > 0.0 ? rz_new[c] / r : 0.0;
       rz[c] = rz_new[c];
       pAp[c] = 0.0;
+      if (rp) rp[c] = 0.0;
     }
   }
   if (mode == 0 && threadIdx.x == 0) *viol = 0.f;
@@ -337,9 +400,17 @@ int pcg_start(const PcgBufs& b, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, b.C, b.pAp);
   if (b.ctl) hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, 1, (double*)b.ctl);
   const FlatLaunch f = flat_launch(b);
+  DLP_CHECK_ARG(f.ok || !b.P16, "pcg: BF16 directions need 16-byte FP32 rows");
+  if (b.P16) {
+    hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, b.C, b.rp);
+    hipLaunchKernelGGL(k_pcg_start4<true>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
+                       PBlock<true>{b.P16, b.ldh / 4}, (const float4*)b.R, b.diag, b.rz, b.rp);
+    DLP_CHECK_LAUNCH();
+    return DLP_OK;
+  }
   if (f.ok) {
-    hipLaunchKernelGGL(k_pcg_start4, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (float4*)b.P,
-                       (const float4*)b.R, b.diag, b.rz);
+    hipLaunchKernelGGL(k_pcg_start4<false>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
+                       PBlock<false>{b.P, 0}, (const float4*)b.R, b.diag, b.rz, nullptr);
     DLP_CHECK_LAUNCH();
     return DLP_OK;
   }
@@ -366,28 +437,39 @@ int pcg_finish_iteration(const PcgBufs& b, hipStream_t st, bool have_pap) {
       DLP_DISPATCH_CPL(m.cpl, K)
 #undef K
     }
+    double* rp = b.P16 ? b.rp : nullptr;
     hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 0, b.rz, b.rz_new, b.pAp, b.alpha,
-                       b.beta, b.viol, b.ctl);
-    hipLaunchKernelGGL(k_pcg_update4, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (float4*)b.Z,
-                       (float4*)b.R, (const float4*)b.P, (const float4*)b.AP, b.alpha, b.diag, b.eps, b.rz_new,
-                       b.viol, b.ctl);
+                       b.beta, b.viol, b.ctl, rp);
+    if (b.P16)
+      hipLaunchKernelGGL(k_pcg_update4<true>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (float4*)b.Z,
+                         (float4*)b.R, PBlock<true>{b.P16, b.ldh / 4}, (const float4*)b.AP, b.alpha, b.diag, b.eps,
+                         b.rz_new, b.viol, b.ctl);
+    else
+      hipLaunchKernelGGL(k_pcg_update4<false>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (float4*)b.Z,
+                         (float4*)b.R, PBlock<false>{b.P, 0}, (const float4*)b.AP, b.alpha, b.diag, b.eps,
+                         b.rz_new, b.viol, b.ctl);
     hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 1, b.rz, b.rz_new, b.pAp, b.alpha,
-                       b.beta, b.viol, b.ctl);
-    hipLaunchKernelGGL(k_pcg_dir4, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (float4*)b.P,
-                       (const float4*)b.R, b.diag, b.beta, b.ctl);
+                       b.beta, b.viol, b.ctl, rp);
+    if (b.P16)
+      hipLaunchKernelGGL(k_pcg_dir4<true>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
+                         PBlock<true>{b.P16, b.ldh / 4}, (const float4*)b.R, b.diag, b.beta, b.rp, b.ctl);
+    else
+      hipLaunchKernelGGL(k_pcg_dir4<false>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
+                         PBlock<false>{b.P, 0}, (const float4*)b.R, b.diag, b.beta, nullptr, b.ctl);
     DLP_CHECK_LAUNCH();
     return DLP_OK;
   }
+  DLP_CHECK_ARG(!b.P16, "pcg: BF16 directions need 16-byte FP32 rows");
 #define K(CPLV)                                                                                       \
   if (!have_pap)                                                                                      \
     hipLaunchKernelGGL((k_coldot<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.P, b.AP, \
                        b.ld, b.pAp);                                                                 \
   hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 0, b.rz, b.rz_new, b.pAp,        \
-                     b.alpha, b.beta, b.viol, b.ctl);                                                \
+                     b.alpha, b.beta, b.viol, b.ctl, nullptr);                                       \
   hipLaunchKernelGGL((k_pcg_update<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.Z, b.R, \
                      b.P, b.AP, b.ld, b.alpha, b.diag, b.eps, b.rz_new, b.viol, b.ctl);              \
   hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 1, b.rz, b.rz_new, b.pAp,        \
-                     b.alpha, b.beta, b.viol, b.ctl);                                                \
+                     b.alpha, b.beta, b.viol, b.ctl, nullptr);                                       \
   hipLaunchKernelGGL((k_pcg_dir<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.P, b.R,   \
                      b.ld, b.diag, b.beta, 0, b.ctl);
   DLP_DISPATCH_CPL(m.cpl, K)

@@ -244,6 +244,7 @@ struct Tuning {
   int dot_grid = 0;   // blocks of the grid-stride operator with fused dot (0: resident blocks)
   int fuse_dot = 1;   // fuse p^T A p into the operator launch of the global solve
   int pipe = 4;       // gathers per lane in flight in the pipelined global-solve operator (0: off)
+  int bf16_word = 4;  // BF16 direction values per lane in the global-solve operator (4 or 8)
 };
 Tuning& tuning();
 

@@ -1,16 +1,17 @@
-// Software-pipelined row gathers over every row of a row-major FP32 block, one row per
-// wavefront step (grid-stride, optionally XCD-aware), shared by the global solve's operator and
-// the full residual verification.
+// Software-pipelined row gathers over every row of a row-major block, one row per wavefront step
+// (grid-stride, optionally XCD-aware), shared by the global solve's operator and the full
+// residual verification.  Rows are FP32 (W = 4 columns per 16-byte lane word) or BF16 (W = 8 per
+// 16-byte word, or W = 4 per 8-byte word).
 //
-// Lanes are packed (LanesP): ceil(C/4) column lanes of 16 bytes times LE edge lanes.  While row
+// Lanes are packed (LanesP): ceil(C/W) column lanes times LE edge lanes.  While row
 // i's neighbour rows are gathered (S gathers per lane issued before the first multiply-add), row
 // i+1's range and its first 64 (column, weight) pairs are already in flight.  Neighbour rows are
-// read with raw buffer loads at 32-bit byte offsets v*row_bytes + 16*l (24-bit multiply-add);
+// read with raw buffer loads at 32-bit byte offsets v*row_bytes + word*l (24-bit multiply-add);
 // lanes without an edge get an out-of-range offset, for which the buffer load returns zeros, so
 // no lane masking or select sits on the gather path.  Rows outside rowmask are skipped; neighbour
 // rows are not masked, so the block must hold the values the caller wants summed for every
-// neighbour (zero where a neighbour must not count).  Callers guarantee n < 2^24 and
-// n*row_bytes < 2^31.
+// neighbour (zero where a neighbour must not count).  Entries [indptr, indptr + rowlen) of a row
+// are all live (deletions compact the row).  Callers guarantee n < 2^24 and n*row_bytes < 2^31.
 #pragma once
 #include "rowgather.h"
 
@@ -21,6 +22,68 @@ typedef unsigned int u4v __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float4 buf_load4(__amdgpu_buffer_rsrc_t rs, unsigned off) {
   const u4v r = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0);
   return make_float4(__uint_as_float(r.x), __uint_as_float(r.y), __uint_as_float(r.z), __uint_as_float(r.w));
+}
+
+// BF16 <-> FP32: a BF16 value is the upper half of an FP32 word; rounding to nearest even.
+__device__ __forceinline__ float bf16_lo(unsigned u) { return __uint_as_float(u << 16); }
+__device__ __forceinline__ float bf16_hi(unsigned u) { return __uint_as_float(u & 0xffff0000u); }
+__device__ __forceinline__ unsigned bf16_bits(float x) {
+  const unsigned u = __float_as_uint(x);
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ unsigned bf16_pack(float lo, float hi) { return bf16_bits(lo) | (bf16_bits(hi) << 16); }
+__device__ __forceinline__ float bf16_round(float x) { return __uint_as_float(bf16_bits(x) << 16); }
+
+// Lane words of the gathered rows: FP32 x 4 (16 bytes), BF16 x 8 (16 bytes) or BF16 x 4
+// (8 bytes).  W values per lane, column order.
+enum class Word { F32x4, BF16x8, BF16x4 };
+
+template <Word K> struct WordT;
+template <> struct WordT<Word::F32x4> {
+  static constexpr int W = 4, bytes = 16;
+  using raw = u4v;
+};
+template <> struct WordT<Word::BF16x8> {
+  static constexpr int W = 8, bytes = 16;
+  using raw = u4v;
+};
+template <> struct WordT<Word::BF16x4> {
+  static constexpr int W = 4, bytes = 8;
+  using raw = unsigned int __attribute__((ext_vector_type(2)));
+};
+
+template <Word K>
+__device__ __forceinline__ typename WordT<K>::raw load_word(__amdgpu_buffer_rsrc_t rs, unsigned off) {
+  if constexpr (WordT<K>::bytes == 16) return __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0);
+  else return __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, 0);
+}
+template <Word K>
+__device__ __forceinline__ typename WordT<K>::raw load_word(const void* p) {
+  return *reinterpret_cast<const typename WordT<K>::raw*>(p);
+}
+
+template <Word K>
+__device__ __forceinline__ void unpack(const typename WordT<K>::raw& u, float (&x)[WordT<K>::W]) {
+  if constexpr (K == Word::F32x4) {
+    x[0] = __uint_as_float(u.x);
+    x[1] = __uint_as_float(u.y);
+    x[2] = __uint_as_float(u.z);
+    x[3] = __uint_as_float(u.w);
+  } else if constexpr (K == Word::BF16x8) {
+    x[0] = bf16_lo(u.x);
+    x[1] = bf16_hi(u.x);
+    x[2] = bf16_lo(u.y);
+    x[3] = bf16_hi(u.y);
+    x[4] = bf16_lo(u.z);
+    x[5] = bf16_hi(u.z);
+    x[6] = bf16_lo(u.w);
+    x[7] = bf16_hi(u.w);
+  } else {
+    x[0] = bf16_lo(u.x);
+    x[1] = bf16_hi(u.x);
+    x[2] = bf16_lo(u.y);
+    x[3] = bf16_hi(u.y);
+  }
 }
 
 // Rows of this wavefront in a grid-stride pass: [lo, hi) stepping nwx from lo + wx.  With
@@ -53,31 +116,36 @@ struct RowPipe {
   __amdgpu_buffer_rsrc_t rx;
   unsigned rowb, laneq;
 
-  __device__ __forceinline__ RowPipe(int nrows, int C, int xcd, const uint8_t* rm, const int32_t* ip,
-                                     const int32_t* rl, const int32_t* ix, const float* wv, const float* X,
-                                     int64_t ldx, int xbytes)
-      : L(C), rowmask(rm), indptr(ip), rowlen(rl), indices(ix), w(wv) {
+  // C columns of W per lane word of word_bytes; X rows of row_bytes bytes (a multiple of the
+  // word), the block xbytes long
+  __device__ __forceinline__ RowPipe(int nrows, int C, int W, int word_bytes, int xcd, const uint8_t* rm,
+                                     const int32_t* ip, const int32_t* rl, const int32_t* ix, const float* wv,
+                                     const void* X, unsigned row_bytes, int xbytes)
+      : L(W == 4 ? C : (C + 1) / 2), rowmask(rm), indptr(ip), rowlen(rl), indices(ix), w(wv) {
     xcd_rows(nrows, xcd, lo, hi, wx, nwx);
-    rx = __builtin_amdgcn_make_buffer_rsrc((void*)X, (short)0, xbytes, 0x00020000);
-    rowb = (unsigned)(ldx * 4);
-    laneq = 16u * (unsigned)L.l;
+    rx = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(X), (short)0, xbytes, 0x00020000);
+    rowb = row_bytes;
+    laneq = (unsigned)word_bytes * (unsigned)L.l;
   }
 };
 
-template <typename TA>
-__device__ __forceinline__ void fma4(TA (&acc)[4], float wv, const float4& x) {
-  acc[0] += (TA)wv * (TA)x.x;
-  acc[1] += (TA)wv * (TA)x.y;
-  acc[2] += (TA)wv * (TA)x.z;
-  acc[3] += (TA)wv * (TA)x.w;
+template <Word K, typename TA>
+__device__ __forceinline__ void fmaw(TA (&acc)[WordT<K>::W], float wv, const typename WordT<K>::raw& u) {
+  constexpr int W = WordT<K>::W;
+  float x[W];
+  unpack<K>(u, x);
+#pragma unroll
+  for (int q = 0; q < W; ++q) acc[q] += (TA)wv * (TA)x[q];
 }
 
 // Runs every row of the pass: epi.pre(r) before row r's gathers are issued (own-row loads),
 // epi.post(r, live, acc) with live = rowmask[r] and acc = sum over the row's entries of
-// w_j * X[col_j, 4l..4l+3] (zero when !live; valid in edge lane 0), called by every lane of the
-// wavefront.
-template <int S, typename TA, typename E>
+// w_j * X[col_j, W*l .. W*l+W-1] (zero when !live; valid in edge lane 0), called by every lane
+// of the wavefront.
+template <int S, Word K, typename TA, typename E>
 __device__ __forceinline__ void run_rows(const RowPipe& G, E& epi) {
+  constexpr int W = WordT<K>::W;
+  using raw = typename WordT<K>::raw;
   const LanesP& L = G.L;
   constexpr unsigned kOOB = 0x80000000u;
   auto range = [&](int64_t ii, int& beg, int& end, bool& live) {
@@ -93,21 +161,21 @@ __device__ __forceinline__ void run_rows(const RowPipe& G, E& epi) {
     }
   };
   // cnt_l: pairs this lane may read (0 on idle edge lanes); the select is branch-free
-  auto gather = [&](int v, float wv, int t, int cnt_l, float4& xv, float& ww) {
+  auto gather = [&](int v, float wv, int t, int cnt_l, raw& xv, float& ww) {
     const int vv = __shfl(v, t & 63, 64);
     ww = __shfl(wv, t & 63, 64);
     const unsigned o = __umul24((unsigned)vv, G.rowb) + G.laneq;
-    xv = buf_load4(G.rx, t < cnt_l ? o : kOOB);
+    xv = load_word<K>(G.rx, t < cnt_l ? o : kOOB);
   };
-  auto steps = [&](int v, float wv, int t0, int cnt, TA (&acc)[4]) {
+  auto steps = [&](int v, float wv, int t0, int cnt, TA (&acc)[W]) {
     const int cnt_l = L.on ? cnt : 0;
     for (; t0 < cnt; t0 += 4 * L.LE) {
-      float4 yv[4];
+      raw yv[4];
       float yw[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) gather(v, wv, t0 + u * L.LE + L.e, cnt_l, yv[u], yw[u]);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) fma4(acc, yw[u], yv[u]);
+      for (int u = 0; u < 4; ++u) fmaw<K>(acc, yw[u], yv[u]);
     }
   };
   int64_t i = G.lo + G.wx;
@@ -128,7 +196,7 @@ __device__ __forceinline__ void run_rows(const RowPipe& G, E& epi) {
     epi.pre(r);
     const int cnt = min(64, end0 - beg0);
     const int cnt_l = L.on ? cnt : 0;
-    float4 xv[S];
+    raw xv[S];
     float ww[S];
 #pragma unroll
     for (int u = 0; u < S; ++u) gather(v0, w0, u * L.LE + L.e, cnt_l, xv[u], ww[u]);
@@ -139,9 +207,11 @@ __device__ __forceinline__ void run_rows(const RowPipe& G, E& epi) {
       v1 = G.indices[beg1 + L.lane];
       w1 = G.w[beg1 + L.lane];
     }
-    TA acc[4] = {0, 0, 0, 0};
+    TA acc[W];
 #pragma unroll
-    for (int u = 0; u < S; ++u) fma4(acc, ww[u], xv[u]);
+    for (int q = 0; q < W; ++q) acc[q] = 0;
+#pragma unroll
+    for (int u = 0; u < S; ++u) fmaw<K>(acc, ww[u], xv[u]);
     steps(v0, w0, S * L.LE, cnt, acc);
     // rows longer than 64 entries: further chunks of 64 pairs
     for (int base = beg0 + 64; base < end0; base += 64) {
@@ -154,11 +224,13 @@ __device__ __forceinline__ void run_rows(const RowPipe& G, E& epi) {
       steps(vc, wc, 0, min(64, end0 - base), acc);
     }
     // edge-lane partials of column lane l are at lanes l + k*LC
-    TA tot[4] = {acc[0], acc[1], acc[2], acc[3]};
+    TA tot[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) tot[q] = acc[q];
     for (int k = 1; k < L.LE; ++k) {
       const int src = L.lane + k * L.LC;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) tot[q] += __shfl(acc[q], src, 64);
+      for (int q = 0; q < W; ++q) tot[q] += __shfl(acc[q], src, 64);
     }
     epi.post(r, live0, tot);
     v0 = v1;

@@ -212,45 +212,84 @@ __global__ __launch_bounds__(256) void k_row_operator_v4_dot(
 }
 
 // The global solve's operator AP = diag*P - W_UU P with the fused p^T A p, on the pipelined
-// row gathers of rowpipe.h.  Neighbour rows are not masked: every row outside the solved set
-// holds P = 0 during the solve (its residual is parked at 0 by the refresh, so P = R/diag +
-// beta P and AP = diag*P keep it 0), and its term vanishes.
+// row gathers of rowpipe.h, with P in FP32 (W = 4) or BF16 (W = 8; AP stays FP32).  Neighbour
+// rows are not masked: every row outside the solved set holds P = 0 during the solve (its
+// residual is parked at 0 by the refresh, so P = R/diag + beta P and AP = diag*P keep it 0),
+// and its term vanishes.
+template <Word K>
 struct PcgEpilogue {
+  static constexpr int W = WordT<K>::W;
   const LanesP& L;
   int C;
   bool owner;
-  const float* __restrict__ X;
-  int64_t ldx;
+  const char* __restrict__ X;  // P rows of row_bytes bytes
+  unsigned row_bytes;
   const float* __restrict__ a;
   float b;
   float* __restrict__ out;
   int64_t ldo;
-  double dacc[4] = {0.0, 0.0, 0.0, 0.0};
-  float4 xr;
+  double dacc[W] = {};
+  float xr[W];
   float ar;
   __device__ __forceinline__ void pre(int r) {
-    xr = owner ? *(reinterpret_cast<const float4*>(X + (int64_t)r * ldx) + L.l) : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (owner) {
+      unpack<K>(load_word<K>(X + (int64_t)r * row_bytes + WordT<K>::bytes * L.l), xr);
+    } else {
+#pragma unroll
+      for (int q = 0; q < W; ++q) xr[q] = 0.f;
+    }
     ar = a[r];
   }
-  // rows outside the mask get AP = diag*P (no neighbour sum), as the plain operator writes them
-  __device__ __forceinline__ void post(int r, bool, const float (&acc)[4]) {
-    if (owner) dot_epilogue(L, C, r, make_float4(acc[0], acc[1], acc[2], acc[3]), xr, ar, b, out, ldo, dacc);
+  // out[r, W*l ..] = b*acc + ar*P (columns < C), dacc += P .* out; rows outside the mask get
+  // AP = diag*P (no neighbour sum), as the plain operator writes them
+  __device__ __forceinline__ void post(int r, bool, const float (&acc)[W]) {
+    if (!owner) return;
+    const int c0 = W * L.l;
+    float y[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) y[q] = b * acc[q] + ar * xr[q];
+    float* op = out + (int64_t)r * ldo + c0;
+    if (c0 + W - 1 < C) {
+#pragma unroll
+      for (int k = 0; k < W / 4; ++k)
+        reinterpret_cast<float4*>(op)[k] = make_float4(y[4 * k], y[4 * k + 1], y[4 * k + 2], y[4 * k + 3]);
+#pragma unroll
+      for (int q = 0; q < W; ++q) dacc[q] += (double)xr[q] * (double)y[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < W; ++q) {
+        if (c0 + q < C) {
+          op[q] = y[q];
+          dacc[q] += (double)xr[q] * (double)y[q];
+        }
+      }
+    }
   }
 };
 
-template <int S>
+template <int S, Word K>
 __global__ __launch_bounds__(256) void k_pcg_operator(
     int nrows, int C, const int32_t* __restrict__ indptr, const int32_t* __restrict__ rowlen,
     const int32_t* __restrict__ indices, const float* __restrict__ w, const uint8_t* __restrict__ rowmask,
-    const float* __restrict__ X, int64_t ldx, int xbytes, const float* __restrict__ a, float b,
+    const void* __restrict__ X, unsigned row_bytes, int xbytes, const float* __restrict__ a, float b,
     float* __restrict__ out, int64_t ldo, double* __restrict__ dot, int xcd, const int32_t* __restrict__ skip) {
   if (skip && *skip) return;
   __shared__ double sm[256];
   for (int c = threadIdx.x; c < C; c += blockDim.x) sm[c] = 0.0;
-  const RowPipe G(nrows, C, xcd, rowmask, indptr, rowlen, indices, w, X, ldx, xbytes);
-  PcgEpilogue E{G.L, C, G.L.e == 0 && 4 * G.L.l < C, X, ldx, a, b, out, ldo};
-  run_rows<S, float>(G, E);
-  dot_flush(G.L, C, E.owner, E.dacc, sm, dot);
+  constexpr int W = WordT<K>::W;
+  const RowPipe G(nrows, C, W, WordT<K>::bytes, xcd, rowmask, indptr, rowlen, indices, w, X, row_bytes, xbytes);
+  PcgEpilogue<K> E{G.L, C, G.L.e == 0 && W * G.L.l < C, (const char*)X, row_bytes, a, b, out, ldo};
+  run_rows<S, K, float>(G, E);
+  __syncthreads();
+  if (E.owner) {
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+      const int c = W * G.L.l + q;
+      if (c < C) atomicAdd(&sm[c], E.dacc[q]);
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&dot[c], sm[c]);
 }
 
 Tuning& tuning() {
@@ -319,33 +358,45 @@ int row_operator_dot(int n, int C, const int32_t* indptr, const int32_t* rowlen,
 }
 
 // The global solve's operator (k_pcg_operator): rowmask = the solved set, rowlen required,
-// 16-byte rows, C <= 256.  Falls back to row_operator_dot when tuning disables the pipelined
-// form (key 6 = 0).
+// 16-byte rows, C <= 256.  X is FP32 (ldx floats per row) or, with X16, BF16 (ldh values per
+// row, a multiple of 8).  The FP32 form falls back to row_operator_dot when tuning disables the
+// pipelined kernel (key 6 = 0) or the block exceeds its 32-bit offsets.
 int pcg_operator(int n, int C, const int32_t* indptr, const int32_t* rowlen, const int32_t* indices, const float* w,
-                 const uint8_t* rowmask, const float* X, int64_t ldx, const float* a, float b, float* out,
-                 int64_t ldo, double* dot, hipStream_t st, const int32_t* skip) {
+                 const uint8_t* rowmask, const float* X, int64_t ldx, const uint16_t* X16, int64_t ldh,
+                 const float* a, float b, float* out, int64_t ldo, double* dot, hipStream_t st, const int32_t* skip) {
   const Tuning& T = tuning();
-  const long long xbytes = (long long)n * ldx * 4;
-  if (!T.pipe || !rowlen || !rowmask || !a || n >= (1 << 24) || xbytes >= (1LL << 31))
+  const bool bf = X16 != nullptr;
+  const long long xbytes = bf ? (long long)n * ldh * 2 : (long long)n * ldx * 4;
+  const bool fits = rowlen && rowmask && a && n < (1 << 24) && xbytes < (1LL << 31);
+  if (!bf && (!T.pipe || !fits))
     return row_operator_dot(n, C, indptr, rowlen, indices, w, rowmask, X, ldx, a, b, out, ldo, dot, st, skip);
-  DLP_CHECK_ARG((ldx % 4) == 0 && (ldo % 4) == 0 && (((uintptr_t)X) & 15) == 0 && (((uintptr_t)out) & 15) == 0 &&
-                    C <= 256,
+  DLP_CHECK_ARG(fits && (ldo % 4) == 0 && (((uintptr_t)out) & 15) == 0 && C <= 256 &&
+                    (bf ? ((ldh % 8) == 0 && (((uintptr_t)X16) & 15) == 0)
+                        : ((ldx % 4) == 0 && (((uintptr_t)X) & 15) == 0)),
                 "pcg_operator: layout");
   if (n <= 0) return DLP_OK;
-  const void* kfn = T.pipe == 8   ? reinterpret_cast<const void*>(&k_pcg_operator<8>)
-                    : T.pipe == 6 ? reinterpret_cast<const void*>(&k_pcg_operator<6>)
-                                  : reinterpret_cast<const void*>(&k_pcg_operator<4>);
+  const int S = T.pipe == 8 ? 8 : T.pipe == 6 ? 6 : 4;
+  const bool w8 = T.bf16_word == 8;
+  const void* kfn = bf ? (w8 ? reinterpret_cast<const void*>(&k_pcg_operator<4, Word::BF16x8>)
+                             : reinterpret_cast<const void*>(&k_pcg_operator<4, Word::BF16x4>))
+                       : S == 8 ? reinterpret_cast<const void*>(&k_pcg_operator<8, Word::F32x4>)
+                       : S == 6 ? reinterpret_cast<const void*>(&k_pcg_operator<6, Word::F32x4>)
+                                : reinterpret_cast<const void*>(&k_pcg_operator<4, Word::F32x4>);
   long long grid = T.dot_grid > 0 ? T.dot_grid : resident_grid(kfn);
   const long long need = ((long long)n + 3) / 4;
   if (grid > need) grid = need;
   const int xcd = T.xcd != 0;
   if (xcd) grid = grid < 8 ? 8 : grid & ~7LL;
-#define DLP_PIPE(SV)                                                                                           \
-  hipLaunchKernelGGL((k_pcg_operator<SV>), dim3((unsigned)grid), dim3(kBlock), 0, st, n, C, indptr, rowlen, indices, \
-                     w, rowmask, X, ldx, (int)xbytes, a, b, out, ldo, dot, xcd, skip)
-  if (T.pipe == 8) DLP_PIPE(8);
-  else if (T.pipe == 6) DLP_PIPE(6);
-  else DLP_PIPE(4);
+  const void* Xp = bf ? (const void*)X16 : (const void*)X;
+  const unsigned rb = bf ? (unsigned)(ldh * 2) : (unsigned)(ldx * 4);
+#define DLP_PIPE(SV, KV)                                                                                      \
+  hipLaunchKernelGGL((k_pcg_operator<SV, KV>), dim3((unsigned)grid), dim3(kBlock), 0, st, n, C, indptr, rowlen, \
+                     indices, w, rowmask, Xp, rb, (int)xbytes, a, b, out, ldo, dot, xcd, skip)
+  if (bf && w8) DLP_PIPE(4, Word::BF16x8);
+  else if (bf) DLP_PIPE(4, Word::BF16x4);
+  else if (S == 8) DLP_PIPE(8, Word::F32x4);
+  else if (S == 6) DLP_PIPE(6, Word::F32x4);
+  else DLP_PIPE(4, Word::F32x4);
 #undef DLP_PIPE
   DLP_CHECK_LAUNCH();
   return DLP_OK;
@@ -609,7 +660,7 @@ extern "C" int dlp_tune_set(int key, int value) {
   // key 2: packed lane groups; key 3: XCD-aware row order; key 4: blocks of the fused-dot
   // operator (0 = resident blocks); key 5: fuse p^T A p into the global solve's operator;
   // key 6: gathers in flight per lane in the pipelined global-solve operator (4, 6 or 8; 0 =
-  // the plain fused operator)
+  // the plain fused operator); key 7: BF16 direction values per lane (4: 8-byte words, 8: 16)
   if (key == 0) {
     DLP_CHECK_ARG(value == 0 || value == 4 || value == 8, "tune: unroll must be 0, 4 or 8");
     tuning().unroll = value;
@@ -628,6 +679,9 @@ extern "C" int dlp_tune_set(int key, int value) {
     DLP_CHECK_ARG(value == 0 || value == 4 || value == 6 || value == 8,
                   "tune: pipelined operator depth must be 0, 4, 6 or 8");
     tuning().pipe = value;
+  } else if (key == 7) {
+    DLP_CHECK_ARG(value == 4 || value == 8, "tune: BF16 values per lane must be 4 or 8");
+    tuning().bf16_word = value;
   } else {
     DLP_CHECK_ARG(false, "tune: unknown key %d", key);
   }

@@ -738,9 +738,10 @@ struct VerifyEpilogue {
 template <int S>
 __global__ __launch_bounds__(256) void k_verify(RefreshArgs A, int xcd, int xbytes, const float* __restrict__ Z,
                                                 float* __restrict__ R) {
-  const RowPipe G(A.nrows, A.C1, xcd, A.solved, A.indptr, A.rowlen, A.col, A.w, Z, A.ld, xbytes);
+  const RowPipe G(A.nrows, A.C1, 4, 16, xcd, A.solved, A.indptr, A.rowlen, A.col, A.w, Z, (unsigned)(A.ld * 4),
+                  xbytes);
   VerifyEpilogue E{G.L, A, Z, R};
-  run_rows<S, double>(G, E);
+  run_rows<S, Word::F32x4, double>(G, E);
   if (!A.rho) return;
   __shared__ double sm[256];
   for (int c = threadIdx.x; c < A.C1; c += blockDim.x) sm[c] = 0.0;
@@ -1072,7 +1073,10 @@ struct LP {
   // global solve (column-wise PCG on the FP32 state)
   float* P = nullptr;
   float* AP = nullptr;
-  double* pcg_scal = nullptr;   // rz, rz_new, pAp, alpha, beta (C1 each)
+  uint16_t* P16 = nullptr;      // BF16 directions of the column PCG (when pcg_bf16)
+  int64_t ldh = 0;              // its row length: C1 rounded up to 8
+  int pcg_bf16 = 0;
+  double* pcg_scal = nullptr;   // rz, rz_new, pAp, alpha, beta, rp (C1 each)
   float* pcg_viol = nullptr;
   hipGraphExec_t pcg_exec = nullptr;
   int pcg_graph_iters = 0;
@@ -1140,6 +1144,7 @@ int lp_free(LP* h) {
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->P) (void)hipFree(h->P);
   if (h->AP) (void)hipFree(h->AP);
+  if (h->P16) (void)hipFree(h->P16);
   if (h->pcg_scal) (void)hipFree(h->pcg_scal);
   if (h->pcg_viol) (void)hipFree(h->pcg_viol);
   if (h->pcg_exec) (void)hipGraphExecDestroy(h->pcg_exec);
@@ -1770,14 +1775,17 @@ static PcgBufs pcg_bufs(LP* h) {
   b.beta = h->pcg_scal + 4 * C1;
   b.viol = h->pcg_viol;
   b.ctl = h->pcg_viol ? reinterpret_cast<int32_t*>(h->pcg_viol) + 2 : nullptr;
+  b.P16 = h->pcg_bf16 ? h->P16 : nullptr;
+  b.ldh = h->ldh;
+  b.rp = h->pcg_scal + 5 * C1;
   return b;
 }
 
 static int pcg_iteration(LP* h, const PcgBufs& b, hipStream_t st) {
   // AP = diag*P - W_UU P, with p^T A p accumulated in the same pass when the layout allows
-  if ((b.ld % 4) == 0 && tuning().vec4 && tuning().fuse_dot) {
-    int rc = pcg_operator(b.n, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.solved, b.P, b.ld, b.diag,
-                          -1.f, b.AP, b.ld, b.pAp, st, b.ctl);
+  if (b.P16 || ((b.ld % 4) == 0 && tuning().vec4 && tuning().fuse_dot)) {
+    int rc = pcg_operator(b.n, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.solved, b.P, b.ld, b.P16,
+                          b.ldh, b.diag, -1.f, b.AP, b.ld, b.pAp, st, b.ctl);
     if (rc) return rc;
     return pcg_finish_iteration(b, st, true);
   }
@@ -1832,10 +1840,16 @@ extern "C" int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int 
   if (!h->P) {
     DLP_CHECK_HIP(hipMalloc((void**)&h->P, nz * sizeof(float)));
     DLP_CHECK_HIP(hipMalloc((void**)&h->AP, nz * sizeof(float)));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_scal, 5 * C1 * sizeof(double)));
+    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_scal, 6 * C1 * sizeof(double)));
     DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_viol, 4 * sizeof(float)));   // viol, pad, done, iterations
     DLP_CHECK_HIP(hipMemsetAsync(h->P, 0, nz * sizeof(float), st));
     DLP_CHECK_HIP(hipMemsetAsync(h->AP, 0, nz * sizeof(float), st));
+  }
+  if (h->pcg_bf16 && !h->P16) {
+    h->ldh = (C1 + 7) / 8 * 8;
+    const size_t nh = (size_t)h->d.n * h->ldh;
+    DLP_CHECK_HIP(hipMalloc((void**)&h->P16, nh * sizeof(uint16_t)));
+    DLP_CHECK_HIP(hipMemsetAsync(h->P16, 0, nh * sizeof(uint16_t), st));
   }
   const PcgBufs b = pcg_bufs(h);
   int rc = pcg_start(b, st);
@@ -1912,6 +1926,18 @@ extern "C" int dlp_lp_scores(void* handle, double* out, int64_t ldo, void* strea
   return DLP_OK;
 }
 
+extern "C" int dlp_lp_set_pcg_bf16(void* handle, int on) {
+  DLP_CHECK_ARG(handle, "lp_set_pcg_bf16: null handle");
+  LP* h = (LP*)handle;
+  DLP_CHECK_ARG(!on || ((h->d.ld % 4) == 0 && h->d.C1 <= 256), "lp_set_pcg_bf16: needs 16-byte FP32 rows");
+  if (h->pcg_bf16 != (on ? 1 : 0) && h->pcg_exec) {   // captured iterations use the other layout
+    (void)hipGraphExecDestroy(h->pcg_exec);
+    h->pcg_exec = nullptr;
+  }
+  h->pcg_bf16 = on ? 1 : 0;
+  return DLP_OK;
+}
+
 // Cumulative timing of the global solve: [0] iterations, [1] microseconds (HIP events).
 extern "C" int dlp_lp_pcg_stats(void* handle, int64_t* out) {
   DLP_CHECK_ARG(handle && out, "lp_pcg_stats: null argument");
@@ -1933,9 +1959,9 @@ extern "C" int dlp_lp_time_operator(void* handle, int reps, double* ms_out, void
   // epilogue when the layout allows it, else the plain row operator
   const bool fused = (b.ld % 4) == 0 && tuning().vec4 && tuning().fuse_dot;
   auto launch = [&]() -> int {
-    if (fused)
-      return pcg_operator(b.n, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.solved, b.P, b.ld, b.diag,
-                          -1.f, b.AP, b.ld, b.pAp, st, nullptr);
+    if (fused || b.P16)
+      return pcg_operator(b.n, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.solved, b.P, b.ld, b.P16,
+                          b.ldh, b.diag, -1.f, b.AP, b.ld, b.pAp, st, nullptr);
     return dlp_row_operator(b.n, nullptr, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, DLP_F32, h->d.solved,
                             b.P, b.ld, DLP_F32, b.diag, -1.0, nullptr, 0, b.AP, b.ld, DLP_F32, 0, 20.0, st);
   };
@@ -2028,13 +2054,14 @@ extern "C" int dlp_lp_bpcg(void* handle, int max_iter, int iters_per_launch, int
   if (!h->P) {
     DLP_CHECK_HIP(hipMalloc((void**)&h->P, nz * sizeof(float)));
     DLP_CHECK_HIP(hipMalloc((void**)&h->AP, nz * sizeof(float)));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_scal, 5 * C1 * sizeof(double)));
+    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_scal, 6 * C1 * sizeof(double)));
     DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_viol, 4 * sizeof(float)));   // viol, pad, done, iterations
     DLP_CHECK_HIP(hipMemsetAsync(h->P, 0, nz * sizeof(float), st));
     DLP_CHECK_HIP(hipMemsetAsync(h->AP, 0, nz * sizeof(float), st));
   }
   if (!h->bk) DLP_CHECK_HIP(hipMalloc((void**)&h->bk, 5 * C1 * C1 * sizeof(double)));
-  const PcgBufs b = pcg_bufs(h);
+  PcgBufs b = pcg_bufs(h);
+  b.P16 = nullptr;                                 // block PCG keeps FP32 directions
   int rc = pcg_start(b, st);                       // P = R/diag
   if (!rc) rc = dlp_gram(b.n, C1, b.R, b.ld, b.R, b.ld, b.diag, h->bk, DLP_F32, st);   // G = R^T D^-1 R
   if (rc) return rc;

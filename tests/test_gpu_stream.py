@@ -41,13 +41,13 @@ def check_against_oracle(lp, g, es, eps, label):
     return err, rc[: sysm.C].max()
 
 
-@pytest.mark.parametrize("fp64", [False, True])
+@pytest.mark.parametrize("fp64,dirs", [(False, "fp32"), (True, "fp32"), (False, "bf16")])
 @pytest.mark.parametrize("K", [2, 8])
-def test_stream_matches_rebuilt_system(fp64, K):
+def test_stream_matches_rebuilt_system(fp64, dirs, K):
     from dynlp_amd.stream import EdgeStreamLP
     g = graphs.sbm(3000, 8, 10, 0.8, K, 0.02, seed=1)
     eps = 1e-6
-    lp = EdgeStreamLP(g, eps=eps, fp64=fp64)
+    lp = EdgeStreamLP(g, eps=eps, fp64=fp64, pcg_directions=dirs)
     lp.initial_solve()
     es = edge_dict(g)
     check_against_oracle(lp, g, es, eps, "initial")
@@ -90,6 +90,31 @@ def test_stream_lane_layouts_and_row_orders(pack, xcd, vec4, pipe):
         _lib.call("dlp_tune_set", 1, 1)
         _lib.call("dlp_tune_set", 2, 1)
         _lib.call("dlp_tune_set", 3, 1)
+
+
+@pytest.mark.parametrize("dirs,word", [("fp32", 4), ("bf16", 4), ("bf16", 8)])
+def test_global_solve_from_zero_certifies(dirs, word):
+    """A global solve from a zero state (every row violating) under FP32 and BF16 directions
+    (8- and 16-byte lane words): certified against the rebuilt oracle, on a fragmenting graph
+    with inert rows."""
+    from dynlp_amd import _lib
+    from dynlp_amd.stream import EdgeStreamLP
+    _lib.call("dlp_tune_set", 7, word)
+    g = graphs.sbm(4000, 16, 3.0, 0.9, 16, 0.01, seed=5, unit=True)
+    lp = EdgeStreamLP(g, eps=1e-6, pcg_directions=dirs)
+    lp.initial_solve()
+    es = edge_dict(g)
+    batches, _ = graphs.update_stream(g, 2, 400, ins_frac=0.3, seed=5, inter_first=True)
+    for b in batches:
+        lp.apply_and_solve(b)
+        O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
+                           zip(b.del_u.tolist(), b.del_v.tolist()))
+    try:
+        st = lp.load_state(np.zeros((g.n, lp.C1), np.float32))
+    finally:
+        _lib.call("dlp_tune_set", 7, 4)
+    assert "pcg" in st.path
+    check_against_oracle(lp, g, es, 1e-6, f"{dirs}/{word}")
 
 
 def test_pcg_group_length_does_not_change_the_solve():
