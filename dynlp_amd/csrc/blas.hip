@@ -337,7 +337,8 @@ static FlatLaunch flat_launch(const PcgBufs& b) {
   const long long used = f.Q4 * (kBlock / f.Q4);
   const long long nq = (long long)b.n * f.Q4;
   long long g = (nq + used - 1) / used;
-  if (g > 2048) g = 2048;
+  const long long cap = tuning().flat_grid > 0 ? tuning().flat_grid : 2048;
+  if (g > cap) g = cap;
   f.grid = (unsigned)(g < 1 ? 1 : g);
   f.ok = true;
   return f;

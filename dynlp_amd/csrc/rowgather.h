@@ -245,6 +245,7 @@ struct Tuning {
   int fuse_dot = 1;   // fuse p^T A p into the operator launch of the global solve
   int pipe = 4;       // gathers per lane in flight in the pipelined global-solve operator (0: off)
   int bf16_word = 4;  // BF16 direction values per lane in the global-solve operator (4 or 8)
+  int flat_grid = 0;  // workgroups of the flat PCG vector passes (0: 2048)
 };
 Tuning& tuning();
 

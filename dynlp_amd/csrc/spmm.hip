@@ -228,7 +228,7 @@ struct PcgEpilogue {
   float b;
   float* __restrict__ out;
   int64_t ldo;
-  double dacc[W] = {};
+  double* sm;  // workgroup column sums of P .* AP (LDS)
   float xr[W];
   float ar;
   __device__ __forceinline__ void pre(int r) {
@@ -240,8 +240,9 @@ struct PcgEpilogue {
     }
     ar = a[r];
   }
-  // out[r, W*l ..] = b*acc + ar*P (columns < C), dacc += P .* out; rows outside the mask get
-  // AP = diag*P (no neighbour sum), as the plain operator writes them
+  // out[r, W*l ..] = b*acc + ar*P (columns < C), sm += P .* out (FP64 LDS atomics: no
+  // per-lane FP64 accumulators, which would cost a wave of occupancy); rows outside the mask
+  // get AP = diag*P (no neighbour sum), as the plain operator writes them
   __device__ __forceinline__ void post(int r, bool, const float (&acc)[W]) {
     if (!owner) return;
     const int c0 = W * L.l;
@@ -254,13 +255,13 @@ struct PcgEpilogue {
       for (int k = 0; k < W / 4; ++k)
         reinterpret_cast<float4*>(op)[k] = make_float4(y[4 * k], y[4 * k + 1], y[4 * k + 2], y[4 * k + 3]);
 #pragma unroll
-      for (int q = 0; q < W; ++q) dacc[q] += (double)xr[q] * (double)y[q];
+      for (int q = 0; q < W; ++q) atomicAdd(&sm[c0 + q], (double)xr[q] * (double)y[q]);
     } else {
 #pragma unroll
       for (int q = 0; q < W; ++q) {
         if (c0 + q < C) {
           op[q] = y[q];
-          dacc[q] += (double)xr[q] * (double)y[q];
+          atomicAdd(&sm[c0 + q], (double)xr[q] * (double)y[q]);
         }
       }
     }
@@ -274,20 +275,13 @@ __global__ __launch_bounds__(256) void k_pcg_operator(
     const void* __restrict__ X, unsigned row_bytes, int xbytes, const float* __restrict__ a, float b,
     float* __restrict__ out, int64_t ldo, double* __restrict__ dot, int xcd, const int32_t* __restrict__ skip) {
   if (skip && *skip) return;
-  __shared__ double sm[256];
-  for (int c = threadIdx.x; c < C; c += blockDim.x) sm[c] = 0.0;
+  __shared__ double sm[264];
+  for (int c = threadIdx.x; c < 264; c += blockDim.x) sm[c] = 0.0;
+  __syncthreads();
   constexpr int W = WordT<K>::W;
   const RowPipe G(nrows, C, W, WordT<K>::bytes, xcd, rowmask, indptr, rowlen, indices, w, X, row_bytes, xbytes);
-  PcgEpilogue<K> E{G.L, C, G.L.e == 0 && W * G.L.l < C, (const char*)X, row_bytes, a, b, out, ldo};
+  PcgEpilogue<K> E{G.L, C, G.L.e == 0 && W * G.L.l < C, (const char*)X, row_bytes, a, b, out, ldo, sm};
   run_rows<S, K, float>(G, E);
-  __syncthreads();
-  if (E.owner) {
-#pragma unroll
-    for (int q = 0; q < W; ++q) {
-      const int c = W * G.L.l + q;
-      if (c < C) atomicAdd(&sm[c], E.dacc[q]);
-    }
-  }
   __syncthreads();
   for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&dot[c], sm[c]);
 }
@@ -377,11 +371,13 @@ int pcg_operator(int n, int C, const int32_t* indptr, const int32_t* rowlen, con
   if (n <= 0) return DLP_OK;
   const int S = T.pipe == 8 ? 8 : T.pipe == 6 ? 6 : 4;
   const bool w8 = T.bf16_word == 8;
-  const void* kfn = bf ? (w8 ? reinterpret_cast<const void*>(&k_pcg_operator<4, Word::BF16x8>)
-                             : reinterpret_cast<const void*>(&k_pcg_operator<4, Word::BF16x4>))
-                       : S == 8 ? reinterpret_cast<const void*>(&k_pcg_operator<8, Word::F32x4>)
-                       : S == 6 ? reinterpret_cast<const void*>(&k_pcg_operator<6, Word::F32x4>)
-                                : reinterpret_cast<const void*>(&k_pcg_operator<4, Word::F32x4>);
+#define DLP_PIPE_FN(SV, KV) reinterpret_cast<const void*>(&k_pcg_operator<SV, KV>)
+  const void* kfn = bf ? (w8 ? DLP_PIPE_FN(4, Word::BF16x8)
+                             : S == 8 ? DLP_PIPE_FN(8, Word::BF16x4)
+                             : S == 6 ? DLP_PIPE_FN(6, Word::BF16x4) : DLP_PIPE_FN(4, Word::BF16x4))
+                       : S == 8 ? DLP_PIPE_FN(8, Word::F32x4)
+                       : S == 6 ? DLP_PIPE_FN(6, Word::F32x4) : DLP_PIPE_FN(4, Word::F32x4);
+#undef DLP_PIPE_FN
   long long grid = T.dot_grid > 0 ? T.dot_grid : resident_grid(kfn);
   const long long need = ((long long)n + 3) / 4;
   if (grid > need) grid = need;
@@ -393,6 +389,8 @@ int pcg_operator(int n, int C, const int32_t* indptr, const int32_t* rowlen, con
   hipLaunchKernelGGL((k_pcg_operator<SV, KV>), dim3((unsigned)grid), dim3(kBlock), 0, st, n, C, indptr, rowlen, \
                      indices, w, rowmask, Xp, rb, (int)xbytes, a, b, out, ldo, dot, xcd, skip)
   if (bf && w8) DLP_PIPE(4, Word::BF16x8);
+  else if (bf && S == 8) DLP_PIPE(8, Word::BF16x4);
+  else if (bf && S == 6) DLP_PIPE(6, Word::BF16x4);
   else if (bf) DLP_PIPE(4, Word::BF16x4);
   else if (S == 8) DLP_PIPE(8, Word::F32x4);
   else if (S == 6) DLP_PIPE(6, Word::F32x4);
@@ -660,7 +658,8 @@ extern "C" int dlp_tune_set(int key, int value) {
   // key 2: packed lane groups; key 3: XCD-aware row order; key 4: blocks of the fused-dot
   // operator (0 = resident blocks); key 5: fuse p^T A p into the global solve's operator;
   // key 6: gathers in flight per lane in the pipelined global-solve operator (4, 6 or 8; 0 =
-  // the plain fused operator); key 7: BF16 direction values per lane (4: 8-byte words, 8: 16)
+  // the plain fused operator); key 7: BF16 direction values per lane (4: 8-byte words, 8: 16);
+  // key 8: workgroups of the flat PCG vector passes (0 = 2048)
   if (key == 0) {
     DLP_CHECK_ARG(value == 0 || value == 4 || value == 8, "tune: unroll must be 0, 4 or 8");
     tuning().unroll = value;
@@ -682,6 +681,9 @@ extern "C" int dlp_tune_set(int key, int value) {
   } else if (key == 7) {
     DLP_CHECK_ARG(value == 4 || value == 8, "tune: BF16 values per lane must be 4 or 8");
     tuning().bf16_word = value;
+  } else if (key == 8) {
+    DLP_CHECK_ARG(value >= 0 && value <= (1 << 20), "tune: flat pass grid out of range");
+    tuning().flat_grid = value;
   } else {
     DLP_CHECK_ARG(false, "tune: unknown key %d", key);
   }
