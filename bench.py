@@ -26,6 +26,7 @@ if ROOT not in sys.path:
 
 METRIC = "edge-updates/s + p50/p99 update latency at rel. residual 1e-6, SBM-1M k=64"
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK = 78.6     # TFLOP/s, dense FP64 matrix
 
 
 def log(*a):
@@ -42,6 +43,79 @@ def operator_bytes(n, C1, stored, p_bytes=4):
     lengths, column/weight of every stored entry, the solved mask, P read (p_bytes per value:
     2 for BF16 directions), diag, AP written (FP32)."""
     return 4 * (n + 1) + 4 * n + 8 * stored + n + p_bytes * n * C1 + 4 * n + 4 * n * C1
+
+
+def timed(fn, reps):
+    """Average ms of fn() over reps launches, HIP events on torch's current stream (the stream
+    the library launches on)."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def kernel_table(g, C, reps=10):
+    """Roofline lines of the other hot kernels at this graph's shape (C columns): the reference
+    API's SpMM, frontier Jacobi and push scatter on a frontier of every 10th row, connected
+    components, and the FP64 Gram of block PCG on the matrix cores.  Bytes count each distinct
+    element once (reads + writes; an atomic read-modify-write twice)."""
+    import numpy as np
+    import torch
+    from dynlp_amd import _lib
+    from dynlp_amd import backend as B
+    from dynlp_amd.cc import connected_components_csr
+    from dynlp_amd.kernels import FrontierOps, spmm_axpy
+    ip, ix, d = g.csr()
+    n, nnz = g.n, int(ix.shape[0])
+    W = B.csr_from_host(ip, ix, d, n, torch.float32)
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    X = torch.rand((n, C), dtype=torch.float32, device="cuda", generator=gen)
+    a = torch.rand(n, dtype=torch.float32, device="cuda", generator=gen) + 20
+    rows = []
+
+    def line(name, ms, nbytes=None, flops=None):
+        if flops is not None:
+            ach = flops / (ms * 1e-3) / 1e12
+            rows.append({"kernel": name, "bound": "mfma", "achieved": round(ach, 2), "peak": FP64_MFMA_PEAK,
+                         "unit": "TFLOP/s", "frac": round(ach / FP64_MFMA_PEAK, 4), "ms": round(ms, 4),
+                         "flops": int(flops)})
+        else:
+            ach = nbytes / (ms * 1e-3) / 1e9
+            rows.append({"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "ms": round(ms, 4),
+                         "algorithmic_bytes": int(nbytes)})
+
+    ms = timed(lambda: spmm_axpy(W, X, a=a, b=-1.0), reps)
+    line(f"spmm_axpy (A*X, n x {C} FP32, reference API)", ms, 4 * (n + 1) + 8 * nnz + 2 * 4 * n * C + 4 * n)
+    f_h = np.arange(0, n, 10, dtype=np.int32)
+    f = torch.as_tensor(f_h, device="cuda")
+    nf = f_h.size
+    deg = np.diff(ip)
+    e_f = int(deg[f_h].sum())
+    nbrs = np.unique(np.concatenate([ix[ip[r]:ip[r + 1]] for r in f_h[: min(nf, 200_000)]]))
+    ops = FrontierOps(W, C)
+    rhs = torch.rand((n, C), dtype=torch.float32, device="cuda", generator=gen)
+    ms = timed(lambda: ops.jacobi(f, X, rhs, a, 1e-4), reps)
+    line(f"frontier_jacobi ({nf} rows)", ms, nf * (4 + 8 + 4 + 3 * C * 4 + 1) + 8 * e_f + 4 * C * nbrs.size)
+    D = torch.rand((nf, C), dtype=torch.float32, device="cuda", generator=gen)
+    R = torch.zeros((n, C), dtype=torch.float32, device="cuda")
+    mark = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    ms = timed(lambda: ops.push(f, D, R, mark), reps)
+    line(f"push_scatter ({nf} rows, FP32 atomics)", ms, nf * (12 + 4 * C) + 8 * e_f + nbrs.size * (2 * 4 * C + 1))
+    ms = timed(lambda: connected_components_csr(W), reps)
+    line("connected components (hook + root + scan + label)", ms, 4 * (n + 1) + 4 * nnz + 8 * n + 4 * 4 * n)
+    Xg = X if C % 4 == 0 else torch.rand((n, (C + 3) // 4 * 4), dtype=torch.float32, device="cuda", generator=gen)
+    G = torch.empty((C, C), dtype=torch.float64, device="cuda")
+    ms = timed(lambda: _lib.call("dlp_gram", n, C, B.ptr(Xg), Xg.shape[1], B.ptr(Xg), Xg.shape[1], None, B.ptr(G),
+                                 _lib.F32, B.stream()), reps)
+    line(f"gram X^T Y (n x {C}, FP64 MFMA)", ms, flops=2.0 * n * C * C)
+    return rows
 
 
 def pmc_probe(g):
@@ -76,6 +150,7 @@ def main(argv=None):
     ap.add_argument("--cpu-batches", type=int, default=1, help="requests timed on the CPU port")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel roofline table")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter probe: initial solve, 5 A*P launches, then a device copy of 512 MiB "
                          "(reads 512 MiB, writes 512 MiB) that calibrates FETCH_SIZE / WRITE_SIZE")
@@ -145,6 +220,9 @@ def main(argv=None):
     log(f"A*P launch: {op_ms:.3f} ms, {nbytes / 1e6:.0f} MB -> {achieved:.0f} GB/s")
     pcg_dirs = lp.pcg_directions
     pcg_iters = pcg1["pcg_iters"] - pcg0["pcg_iters"]
+    err = lp.error_report()
+    log(f"final state: rel l2 residual {err['rel_l2_residual']:.2e}, certified |F - F*| <= {err['cert_bound']:.2e}, "
+        f"argmax certified on {100 * err['argmax_certified_frac']:.3f}% of solved rows")
     pcg_ms = (pcg1["pcg_us"] - pcg0["pcg_us"]) / 1e3
     del lp
     torch.cuda.empty_cache()
@@ -170,6 +248,14 @@ def main(argv=None):
                 "path": "FP64 Z/R, host-driven push rounds (reference IncrementalLP policy), no graphs"}
         log(f"plain FP64 path: {v64:,.0f} edge-updates/s over {len(lat64)} requests")
         del lp64
+        torch.cuda.empty_cache()
+
+    # ---- the other hot kernels at this graph's shape ------------------------------------------
+    ktab = None
+    if not a.no_kernels:
+        ktab = kernel_table(g, 65)
+        for r in ktab:
+            log(f"{r['kernel']}: {r['ms']:.3f} ms -> {r['achieved']} {r['unit']} ({100 * r['frac']:.1f}% of peak)")
         torch.cuda.empty_cache()
 
     # ---- CPU baseline: the FP64 NumPy/SciPy port on the host --------------------------------
@@ -202,9 +288,10 @@ def main(argv=None):
         "p50_ms": round(percentile(lat, 50), 3), "p99_ms": round(percentile(lat, 99), 3),
         "rho_max": max(rhos), "paths": paths,
         "pcg_iters_per_request": round(pcg_iters / a.steps, 2), "pcg_ms_per_request": round(pcg_ms / a.steps, 3),
+        "error": err,
         "fp64_path": fp64,
         "speedup_vs_fp64_path": round(value / fp64["value"], 2) if fp64 else None,
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "kernels": ktab,
         "wall_s": round(time.perf_counter() - t_all, 1),
     }
     print(json.dumps(out), flush=True)

@@ -355,6 +355,46 @@ class EdgeStreamLP:
                   ptr(self.diag64), ptr(self._rho_dev), stream())
         return float(self._rho_dev[: self.C].max().item())
 
+    def error_report(self):
+        """Errors of the held state, from device quantities (no exact solution needed):
+        rel_l2_residual = ||R||_2 / ||rhs||_2 over the label columns of the solved rows (R the
+        kept FP64-refreshed residual, rhs = R + A Z); cert_bound = max_c rho_c * hmax with hmax =
+        max h / (1 - rho_h), which bounds max |F - F*| (solvers.py:295-301); argmax_certified_frac
+        = solved rows whose prediction (solvers.py:166-167) cannot differ from that of F*: top two
+        scores more than 2 * cert_bound apart (K > 2), |F - 0.5| > cert_bound (K = 2)."""
+        C, dev = self.C, device()
+        Zf = self.Z64 if self.fp64 else self.Z32
+        R = (self.R64 if self.fp64 else self.R32)[:, :C].double()
+        AZ = torch.empty((self.n, C), dtype=torch.float64, device=dev)
+        _lib.call("dlp_row_operator", self.n, None, C, ptr(self.indptr), ptr(self.rowlen), ptr(self.col), ptr(self.w),
+                  _lib.F32, ptr(self.solved), ptr(Zf), self.ld, _lib.F64 if self.fp64 else _lib.F32, ptr(self.diag64),
+                  -1.0, None, 0, ptr(AZ), C, _lib.F64, 0, self.nnz / max(self.n, 1), stream())
+        sm = self.solved.bool()
+        Rs, rhs = R[sm], (R + AZ)[sm]
+        rel = float(Rs.norm() / rhs.norm().clamp_min(1e-300))
+        if not hasattr(self, "_rho_dev"):
+            self._rho_dev = torch.zeros(self.C1, dtype=torch.float64, device=dev)
+        Rfull = self.R64 if self.fp64 else self.R32
+        _lib.call("dlp_rho_cols", self.n, None, self.C1, ptr(Rfull), self.ld, _lib.F64 if self.fp64 else _lib.F32,
+                  ptr(self.diag64), ptr(self._rho_dev), stream())
+        rc = self._rho_dev.cpu().numpy()
+        F = Zf[:, : self.C1][sm].double()
+        h_est = float(F[:, C].max()) if F.shape[0] else 1.0
+        if rc[C] > self.eps_h:
+            hmax, cert = h_est, float("inf")
+        else:
+            hmax = h_est / (1 - float(rc[C]))
+            cert = float(rc[:C].max()) * hmax
+        if F.shape[0] == 0:
+            frac = 1.0
+        elif C == 1:
+            frac = float(((F[:, 0] - 0.5).abs() > cert).double().mean())
+        else:
+            top = torch.topk(F[:, :C], 2, dim=1).values
+            frac = float(((top[:, 0] - top[:, 1]) > 2 * cert).double().mean())
+        return dict(rho=float(rc[:C].max()), rho_h=float(rc[C]), hmax=hmax, cert_bound=cert,
+                    rel_l2_residual=rel, argmax_certified_frac=frac, solved_rows=int(F.shape[0]))
+
     def time_operator(self, reps=20):
         """Average ms of the global solve's A*P launch (HIP events on the current stream)."""
         ms = ct.c_double(0.0)

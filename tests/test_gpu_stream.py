@@ -138,6 +138,33 @@ def test_pcg_group_length_does_not_change_the_solve():
     np.testing.assert_allclose(Z1[:, :C], Z7[:, :C], rtol=0, atol=2.1e-6 * hmax)
 
 
+@pytest.mark.parametrize("K", [2, 8])
+def test_error_report_against_oracle(K):
+    """error_report: relative residual equal to the oracle's on the same state, certified bound
+    above the true max |F - F*|, and certified rows predict like F*."""
+    from dynlp_amd.stream import EdgeStreamLP
+    g = graphs.sbm(3000, 8, 10, 0.8, K, 0.02, seed=9)
+    lp = EdgeStreamLP(g, eps=1e-6)
+    lp.initial_solve()
+    es = edge_dict(g)
+    b = graphs.update_stream(g, 1, 300, seed=9)[0][0]
+    lp.apply_and_solve(b)
+    O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
+                       zip(b.del_u.tolist(), b.del_v.tolist()))
+    e = lp.error_report()
+    sysm, _ = oracle_system(g, es)
+    Z = lp.scores_with_h()
+    U, C = sysm.solved, sysm.C
+    R = O.residual(sysm, Z)
+    rel = np.linalg.norm(R[U, :C]) / np.linalg.norm(sysm.rhs[U])
+    assert e["rel_l2_residual"] == pytest.approx(rel, rel=1e-3, abs=1e-12)
+    Fs = O.solve_system(sysm)
+    assert np.abs(Z[U, :C] - Fs[U, :C]).max() <= e["cert_bound"]
+    assert e["cert_bound"] <= 1e-6 * Fs[:, C].max() * 1.1
+    pz, ps = O.predict(Z[U, :C], K), O.predict(Fs[U, :C], K)
+    assert (pz == ps).mean() >= e["argmax_certified_frac"] and e["argmax_certified_frac"] > 0.9
+
+
 def test_fragmenting_stream_regrounds():
     """Sparse SBM with deletions of inter-block edges first: components split off and lose
     (and regain) their labeled contact, so rows toggle between solved and inert."""
