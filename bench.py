@@ -38,6 +38,22 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if xs else float("nan")
 
 
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary (FETCH_SIZE and
+    WRITE_SIZE runs scaled by a copy of known size), or None when it covers another kernel."""
+    import re
+    path = os.path.join(ROOT, "profiles", "pcg_operator_sbm1m_k64_pmc.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    k = re.search(r"^traffic_kernel: (.+)$", text, re.M)
+    b = re.search(r"^traffic_bytes_per_launch: (\d+)$", text, re.M)
+    if not (k and b) or k.group(1).strip() != kernel:
+        return None
+    return int(b.group(1))
+
+
 def operator_bytes(n, C1, stored, p_bytes=4):
     """Algorithmic bytes of one A*P launch (each distinct element once): row starts and
     lengths, column/weight of every stored entry, the solved mask, P read (p_bytes per value:
@@ -213,9 +229,12 @@ def main(argv=None):
     achieved = nbytes / (op_ms * 1e-3) / 1e9
     kname = ("k_pcg_operator<4, BF16x4> (A*P on BF16 directions with fused p^T A p, the global solve)" if bf
              else "k_pcg_operator<4, F32x4> (A*P with fused p^T A p, the global solve)")
+    traffic = pmc_traffic("dlp::k_pcg_operator<4, (dlp::Word)2>") if bf else None
     roofline = {"kernel": kname,
                 "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": "profiles/pcg_operator_sbm1m_k64_pmc.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)"
+                if traffic else None,
                 "avg_launch_ms": round(op_ms, 4), "algorithmic_bytes": int(nbytes)}
     log(f"A*P launch: {op_ms:.3f} ms, {nbytes / 1e6:.0f} MB -> {achieved:.0f} GB/s")
     pcg_dirs = lp.pcg_directions
