@@ -1,98 +1,115 @@
-// Connected components by lock-free union-find, canonical labels by a device scan.
+// Connected components by lock-free union-find (uf.h), canonical labels by a device scan.
 //
-// Hooking always links the larger root under the smaller one (compare-and-swap on the
-// larger root's parent), so every tree's root is its minimum vertex id and parent ids
-// strictly decrease along any path.  After all unions a read-only pass gives each vertex its
-// root, the component minimum; flags (root or not) are scanned so that a component's label
-// is the rank of its minimum among all component minima, which is the labelling the
-// reference's hook/jump + unique produces (cc.py:12-23).
-#include "common.h"
+// A read-only pass gives each vertex its root, the component minimum; flags (root or not) are
+// scanned so that a component's label is the rank of its minimum among all component minima,
+// which is the labelling the reference's hook/jump + unique produces (cc.py:12-23).
 #include "scan.h"
+#include "uf.h"
 
 namespace dlp {
 
-__device__ __forceinline__ int ld_relaxed(const int32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_relaxed(int32_t* p, int v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Root of x with path halving (every stored value is an ancestor, so races are benign).
-__device__ __forceinline__ int uf_find(int32_t* P, int x) {
-  while (true) {
-    const int p = ld_relaxed(P + x);
-    if (p == x) return x;
-    const int gp = ld_relaxed(P + p);
-    if (gp == p) return p;
-    st_relaxed(P + x, gp);
-    x = gp;
-  }
-}
-
-// Each failed CAS means another thread hooked a root, so the number of roots strictly
-// decreases over failures: the loop ends.
-__device__ __forceinline__ void uf_union(int32_t* P, int a, int b) {
-  while (true) {
-    a = uf_find(P, a);
-    b = uf_find(P, b);
-    if (a == b) return;
-    if (a < b) {
-      const int t = a;
-      a = b;
-      b = t;
-    }
-    if (atomicCAS(P + a, a, b) == a) return;
-  }
-}
-
-__global__ void k_cc_init(int n, int32_t* __restrict__ P) {
+__global__ void k_uf_init(int n, int32_t* __restrict__ P, const int32_t* __restrict__ need) {
+  if (need && *need == 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) P[i] = i;
 }
 
-__global__ void k_cc_hook_edges(int64_t m, const int32_t* __restrict__ rows,
-                                const int32_t* __restrict__ cols, int n, int32_t* P) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int a = rows[e], b = cols[e];
-    if (a != b && (unsigned)a < (unsigned)n && (unsigned)b < (unsigned)n) uf_union(P, a, b);
+// Round r of the neighbour links: every filtered vertex joins its r-th filtered entry.
+__global__ void k_uf_link_round(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ rowlen,
+                                const int32_t* __restrict__ indices, const uint8_t* __restrict__ filt, int keep,
+                                int32_t* P, int r, const int32_t* __restrict__ need) {
+  if (need && *need == 0) return;
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n || !uf_in(filt, keep, v)) return;
+  const int len = rowlen ? rowlen[v] : indptr[v + 1] - indptr[v];
+  if (r >= len) return;
+  const int u = indices[indptr[v] + r];
+  if (u != v && (unsigned)u < (unsigned)n && uf_in(filt, keep, u)) uf_union(P, v, u);
+}
+
+// P[v] = root of v for every vertex (each thread stores only its own entry).
+__global__ void k_uf_flatten(int n, int32_t* P, const int32_t* __restrict__ need) {
+  if (need && *need == 0) return;
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) uf_store(P + v, uf_root(P, v));
+}
+
+// giant[0] = the most frequent root among kRootSamples hashed vertices of a flattened forest
+// (ties -> smaller root).  One workgroup of 256.
+__global__ __launch_bounds__(256) void k_uf_sample(int n, const int32_t* __restrict__ P, int32_t* __restrict__ giant,
+                                                   const int32_t* __restrict__ need) {
+  if (need && *need == 0) return;
+  __shared__ int32_t s[kRootSamples];
+  __shared__ unsigned long long best;
+  if (threadIdx.x == 0) best = ~0ull;
+  for (int i = threadIdx.x; i < kRootSamples; i += blockDim.x) {
+    const unsigned v = (unsigned)(((unsigned long long)i * 2654435761ull + 12345ull) % (unsigned long long)n);
+    s[i] = P[v];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kRootSamples; i += blockDim.x) {
+    int cnt = 0;
+    for (int j = 0; j < kRootSamples; ++j) cnt += s[j] == s[i];
+    // smallest key = largest count, then smallest root
+    const unsigned long long key = ((unsigned long long)(kRootSamples - cnt) << 32) | (unsigned)s[i];
+    atomicMin(&best, key);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) giant[0] = (int32_t)(best & 0xffffffffu);
+}
+
+// Remaining links: vertices outside the sampled giant component join every filtered entry past
+// the first kNeighbourRounds (8 lanes per row).
+__global__ __launch_bounds__(256) void k_uf_link_rest(int n, const int32_t* __restrict__ indptr,
+                                                      const int32_t* __restrict__ rowlen,
+                                                      const int32_t* __restrict__ indices,
+                                                      const uint8_t* __restrict__ filt, int keep, int32_t* P,
+                                                      const int32_t* __restrict__ giant,
+                                                      const int32_t* __restrict__ need) {
+  if (need && *need == 0) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * 8) return;
+  const int v = (int)(t >> 3);
+  const int lane = (int)(t & 7);
+  if (!uf_in(filt, keep, v) || uf_load(P + v) == *giant) return;
+  const int beg = indptr[v];
+  const int end = rowlen ? beg + rowlen[v] : indptr[v + 1];
+  for (int j = beg + kNeighbourRounds + lane; j < end; j += 8) {
+    const int u = indices[j];
+    if (u != v && (unsigned)u < (unsigned)n && uf_in(filt, keep, u)) uf_union(P, v, u);
   }
 }
 
-// One group of 8 lanes per row; entry (r, c) with c > r joins r and c (each undirected
-// pair once when both directions are stored; a one-direction entry with c < r is handled
-// by the lane that sees it, so unsymmetric inputs are also covered).
-__global__ __launch_bounds__(256) void k_cc_hook_csr(int n, const int32_t* __restrict__ indptr,
-                                                     const int32_t* __restrict__ rowlen,
-                                                     const int32_t* __restrict__ indices,
-                                                     const uint8_t* __restrict__ mask,
-                                                     int32_t* P) {
-  const int64_t r64 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 3;
-  const int lane = threadIdx.x & 7;
-  if (r64 >= n) return;
-  const int r = (int)r64;
-  if (mask && !mask[r]) return;
-  const int beg = indptr[r];
-  const int end = rowlen ? beg + rowlen[r] : indptr[r + 1];
-  for (int j = beg + lane; j < end; j += 8) {
-    const int c = indices[j];
-    if (c == r || (unsigned)c >= (unsigned)n) continue;
-    if (mask && !mask[c]) continue;
-    uf_union(P, r, c);
+int uf_components_csr(int n, const int32_t* indptr, const int32_t* rowlen, const int32_t* indices,
+                      const uint8_t* filt, int keep, int32_t* P, int32_t* giant, const int32_t* need,
+                      hipStream_t st) {
+  if (n <= 0) return DLP_OK;
+  const dim3 g1(grid_for(n)), b(kBlock);
+  hipLaunchKernelGGL(k_uf_init, g1, b, 0, st, n, P, need);
+  // unions compress no paths, so every round starts from flattened trees
+  for (int r = 0; r < kNeighbourRounds; ++r) {
+    hipLaunchKernelGGL(k_uf_link_round, g1, b, 0, st, n, indptr, rowlen, indices, filt, keep, P, r, need);
+    hipLaunchKernelGGL(k_uf_flatten, g1, b, 0, st, n, P, need);
   }
+  hipLaunchKernelGGL(k_uf_sample, dim3(1), b, 0, st, n, P, giant, need);
+  hipLaunchKernelGGL(k_uf_link_rest, dim3(grid_for((long long)n * 8)), b, 0, st, n, indptr, rowlen, indices, filt,
+                     keep, P, giant, need);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
 }
 
-// Root of x without path compression.  The final pass must not store into other vertices'
-// entries: a halving store by one thread could overwrite the root another thread has just
-// written for its own vertex, leaving a non-root there.
-__device__ __forceinline__ int uf_root(const int32_t* P, int x) {
-  int p = ld_relaxed(P + x);
-  while (p != x) {
-    x = p;
-    p = ld_relaxed(P + x);
+// Edge-list links over [e0, e1): with giant non-null an edge whose two endpoints already sit in
+// the sampled giant component is skipped.
+__global__ void k_uf_link_edges(int64_t e0, int64_t e1, const int32_t* __restrict__ rows,
+                                const int32_t* __restrict__ cols, int n, int32_t* P,
+                                const int32_t* __restrict__ giant) {
+  const int gc = giant ? *giant : -1;
+  for (int64_t e = e0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < e1; e += (int64_t)gridDim.x * blockDim.x) {
+    const int a = rows[e], c = cols[e];
+    if (a == c || (unsigned)a >= (unsigned)n || (unsigned)c >= (unsigned)n) continue;
+    if (giant && uf_load(P + a) == gc && uf_load(P + c) == gc) continue;
+    uf_union(P, a, c);
   }
-  return x;
 }
 
 // roots[i] = root of i (its component's minimum id), flag[i] = (i is a root).
@@ -114,13 +131,14 @@ __global__ void k_cc_label(int n, const int32_t* roots, const int32_t* __restric
 struct CCWork {
   int32_t* parent;
   int32_t* flag;   // flags, scanned in place into ranks
+  int32_t* giant;
   void* scan_ws;
   size_t scan_bytes;
 };
 
 static size_t cc_bytes(int n) {
   const size_t a = ((size_t)n * 4 + 255) & ~(size_t)255;
-  return 2 * a + scan_workspace_bytes(n);
+  return 2 * a + 256 + scan_workspace_bytes(n);
 }
 
 static int cc_carve(int n, void* ws, size_t ws_bytes, CCWork& w) {
@@ -130,8 +148,9 @@ static int cc_carve(int n, void* ws, size_t ws_bytes, CCWork& w) {
   char* p = (char*)ws;
   w.parent = (int32_t*)p;
   w.flag = (int32_t*)(p + a);
-  w.scan_ws = p + 2 * a;
-  w.scan_bytes = ws_bytes - 2 * a;
+  w.giant = (int32_t*)(p + 2 * a);
+  w.scan_ws = p + 2 * a + 256;
+  w.scan_bytes = ws_bytes - 2 * a - 256;
   return DLP_OK;
 }
 
@@ -164,12 +183,28 @@ extern "C" int dlp_cc_edges(int n, int64_t m, const int32_t* rows, const int32_t
   CCWork w;
   int rc = cc_carve(n, ws, ws_bytes, w);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_cc_init, dim3(grid_for(n)), dim3(kBlock), 0, st, n, w.parent);
+  hipLaunchKernelGGL(k_uf_init, dim3(grid_for(n)), dim3(kBlock), 0, st, n, w.parent, nullptr);
   DLP_CHECK_LAUNCH();
   if (m > 0) {
-    unsigned g = grid_for(m);
-    if (g > 65536) g = 65536;
-    hipLaunchKernelGGL(k_cc_hook_edges, dim3(g), dim3(kBlock), 0, st, m, rows, cols, n, w.parent);
+    // a prefix of the edges first (about kNeighbourRounds per vertex), then the rest with the
+    // edges inside the sampled giant component skipped
+    const int64_t m1 = m < (int64_t)kNeighbourRounds * n ? m : (int64_t)kNeighbourRounds * n;
+    auto grid_of = [](int64_t cnt) {
+      unsigned g = grid_for(cnt);
+      return g > 65536 ? 65536u : (g < 1 ? 1u : g);
+    };
+    // the prefix in chunks of n edges, each followed by a flattening pass
+    for (int64_t e0 = 0; e0 < m1; e0 += n) {
+      const int64_t e1 = e0 + n < m1 ? e0 + n : m1;
+      hipLaunchKernelGGL(k_uf_link_edges, dim3(grid_of(e1 - e0)), dim3(kBlock), 0, st, e0, e1, rows, cols, n,
+                         w.parent, nullptr);
+      hipLaunchKernelGGL(k_uf_flatten, dim3(grid_for(n)), dim3(kBlock), 0, st, n, w.parent, nullptr);
+    }
+    if (m > m1) {
+      hipLaunchKernelGGL(k_uf_sample, dim3(1), dim3(kBlock), 0, st, n, w.parent, w.giant, nullptr);
+      hipLaunchKernelGGL(k_uf_link_edges, dim3(grid_of(m - m1)), dim3(kBlock), 0, st, m1, m, rows, cols, n,
+                         w.parent, w.giant);
+    }
     DLP_CHECK_LAUNCH();
   }
   return cc_finish(n, w, labels, n_comp, st);
@@ -188,10 +223,7 @@ extern "C" int dlp_cc_csr(int n, const int32_t* indptr, const int32_t* rowlen,
   CCWork w;
   int rc = cc_carve(n, ws, ws_bytes, w);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_cc_init, dim3(grid_for(n)), dim3(kBlock), 0, st, n, w.parent);
-  DLP_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_cc_hook_csr, dim3(grid_for((long long)n * 8)), dim3(kBlock), 0, st, n,
-                     indptr, rowlen, indices, mask, w.parent);
-  DLP_CHECK_LAUNCH();
+  rc = uf_components_csr(n, indptr, rowlen, indices, mask, 1, w.parent, w.giant, nullptr, st);
+  if (rc) return rc;
   return cc_finish(n, w, labels, n_comp, st);
 }

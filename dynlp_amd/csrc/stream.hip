@@ -11,6 +11,7 @@
 #include "rowpipe.h"
 #include "scan.h"
 #include "blas.h"
+#include "uf.h"
 
 namespace dlp {
 
@@ -142,76 +143,21 @@ __global__ void k_regrow_caps(int n, const int32_t* __restrict__ rowlen, int ext
 // Grounding: U = unlabeled vertices whose component in the unlabeled subgraph holds a vertex
 // with a labeled neighbour (stream.py:149-158).
 
-__device__ __forceinline__ int ld_rlx(const int32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_rlx(int32_t* p, int v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ int uf_find2(int32_t* P, int x) {
-  while (true) {
-    const int p = ld_rlx(P + x);
-    if (p == x) return x;
-    const int gp = ld_rlx(P + p);
-    if (gp == p) return p;
-    st_rlx(P + x, gp);
-    x = gp;
-  }
-}
-__device__ __forceinline__ void uf_union2(int32_t* P, int a, int b) {
-  while (true) {
-    a = uf_find2(P, a);
-    b = uf_find2(P, b);
-    if (a == b) return;
-    if (a < b) {
-      const int t = a;
-      a = b;
-      b = t;
-    }
-    if (atomicCAS(P + a, a, b) == a) return;
-  }
-}
-
-__global__ void k_ground_init(int n, int32_t* __restrict__ P, uint8_t* __restrict__ grounded,
-                              const int32_t* __restrict__ need) {
+__global__ void k_ground_init(int n, uint8_t* __restrict__ grounded, const int32_t* __restrict__ need) {
   if (need && *need == 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    P[i] = i;
-    grounded[i] = 0;
-  }
+  if (i < n) grounded[i] = 0;
 }
 
-__global__ __launch_bounds__(256) void k_ground_hook(int n, const int32_t* __restrict__ indptr,
-                                                     const int32_t* __restrict__ rowlen,
-                                                     const int32_t* __restrict__ col,
-                                                     const uint8_t* __restrict__ labeled, int32_t* P,
-                                                     const int32_t* __restrict__ need) {
-  if (need && *need == 0) return;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = (int)(t >> 3);
-  const int lane = (int)(t & 7);
-  if (r >= n || labeled[r]) return;
-  const int beg = indptr[r], end = beg + rowlen[r];
-  for (int j = beg + lane; j < end; j += 8) {
-    const int c = col[j];
-    if (c > r && !labeled[c]) uf_union2(P, r, c);
-  }
-}
-
-// Roots without path compression (see cc.hip: a halving store in this pass could overwrite
-// another vertex's final root); roots[i] is written for unlabeled i only.
+// Roots by a read-only walk (uf_root: a store in this pass could overwrite another vertex's
+// final root); roots[i] is written for unlabeled i only.
 __global__ void k_ground_roots(int n, const int32_t* P, const uint8_t* __restrict__ labeled,
                                const int32_t* __restrict__ labcnt, uint8_t* __restrict__ grounded,
                                int32_t* __restrict__ roots, const int32_t* __restrict__ need) {
   if (need && *need == 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || labeled[i]) return;
-  int x = i, p = ld_rlx(P + i);
-  while (p != x) {
-    x = p;
-    p = ld_rlx(P + x);
-  }
+  const int x = uf_root(P, i);
   roots[i] = x;
   if (labcnt[i] > 0) grounded[x] = 1;
 }
@@ -1083,7 +1029,8 @@ struct LP {
   double pcg_ms = 0.0;
   int64_t pcg_iters = 0;
   double* bk = nullptr;         // block PCG k x k matrices: G, PAP, Gn, alpha, beta
-  int32_t* gflags = nullptr;    // [0] need full CC, [1] inert unlabeled vertices, [2] checks run
+  int32_t* gflags = nullptr;    // [0] need full CC, [1] inert unlabeled vertices, [2] checks run,
+                                // [4] giant component root (CC scratch)
   hipGraphExec_t bpcg_exec = nullptr;
   int bpcg_graph_iters = 0;
   double push_ms = 0.0;         // summed duration of push-round launches (events)
@@ -1220,7 +1167,7 @@ extern "C" int dlp_lp_create(const dlp_lp_desc* d, void** handle) {
   A_(h->S.bcount, 1); A_(h->S.edges, 1); A_(h->S.stats, 4);
   int32_t* bst = nullptr;
   float* eps32 = nullptr;
-  A_(bst, 1); A_(eps32, d->C1); A_(h->gflags, 4); A_(h->mass, 1);
+  A_(bst, 1); A_(eps32, d->C1); A_(h->gflags, 8); A_(h->mass, 1);
 #undef A_
   h->S.bstamp = bst;
   h->S.eps = eps32;
@@ -1247,7 +1194,7 @@ extern "C" int dlp_lp_create(const dlp_lp_desc* d, void** handle) {
   (void)hipMemset(h->S.edges, 0, sizeof(unsigned long long));
   (void)hipMemset(h->S.stats, 0, 4 * sizeof(unsigned long long));
   (void)hipMemset(h->mark, 0, n);
-  (void)hipMemset(h->gflags, 0, 4 * sizeof(int32_t));
+  (void)hipMemset(h->gflags, 0, 8 * sizeof(int32_t));
   h->S.n = n;
   h->S.C1 = d->C1;
   h->S.ld = d->ld;
@@ -1479,11 +1426,12 @@ static int ground_impl(LP* h, int force, int nd, const int32_t* du, const int32_
   }
   DLP_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_ground_inert_reset, dim3(1), dim3(64), 0, st, h->gflags);
-  hipLaunchKernelGGL(k_ground_init, dim3(grid_for(n)), dim3(kBlock), 0, st, n, h->parent, h->grounded, need);
+  hipLaunchKernelGGL(k_ground_init, dim3(grid_for(n)), dim3(kBlock), 0, st, n, h->grounded, need);
   DLP_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_ground_hook, dim3(grid_for((long long)n * 8)), dim3(kBlock), 0, st, n, h->d.indptr,
-                     h->d.rowlen, h->d.col, h->d.labeled, h->parent, need);
-  DLP_CHECK_LAUNCH();
+  // components of the unlabeled subgraph (labeled vertices excluded)
+  int rc = uf_components_csr(n, h->d.indptr, h->d.rowlen, h->d.col, h->d.labeled, 0, h->parent, h->gflags + 4,
+                             need, st);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_ground_roots, dim3(grid_for(n)), dim3(kBlock), 0, st, n, h->parent, h->d.labeled,
                      h->d.labcnt, h->grounded, h->roots, need);
   DLP_CHECK_LAUNCH();
