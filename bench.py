@@ -7,7 +7,7 @@ with h).  After the initial solve, a stream of requests, each a batch of `--batc
 updates (50% insertions of new uniform pairs, 50% deletions of existing edges; seed S+2).  A step
 is one request: apply on the device-resident slack CSR, re-ground (connected components),
 refresh the changed rows' FP64 residual, repair to max |R_uc|/diag_u <= 1e-6 (push or global PCG
-on FP32 vectors with BF16 search directions, replayed from hipGraphs) and verify in FP64.  Latency
+on FP32 vectors with scaled FP16 search directions, replayed from hipGraphs) and verify in FP64.  Latency
 is submit -> certified.
 
 Printed on stdout: one JSON line.  Progress goes to stderr.
@@ -57,7 +57,7 @@ def pmc_traffic(kernel):
 def operator_bytes(n, C1, stored, p_bytes=4):
     """Algorithmic bytes of one A*P launch (each distinct element once): row starts and
     lengths, column/weight of every stored entry, the solved mask, P read (p_bytes per value:
-    2 for BF16 directions), diag, AP written (FP32)."""
+    2 for 16-bit directions), diag, AP written (FP32)."""
     return 4 * (n + 1) + 4 * n + 8 * stored + n + p_bytes * n * C1 + 4 * n + 4 * n * C1
 
 
@@ -143,7 +143,7 @@ def pmc_probe(g):
     lp.initial_solve()
     torch.cuda.synchronize()
     op_ms = lp.time_operator(reps=4)           # 1 + 4 launches
-    nbytes = operator_bytes(g.n, lp.C1, lp.stored_entries(), 2 if lp.pcg_directions == "bf16" else 4)
+    nbytes = operator_bytes(g.n, lp.C1, lp.stored_entries(), 4 if lp.pcg_directions == "fp32" else 2)
     src = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
@@ -224,12 +224,12 @@ def main(argv=None):
     # ---- roofline of the dominant kernel (the global solve's A*P launch) --------------------
     op_ms = lp.time_operator(reps=20)
     stored = lp.stored_entries()
-    bf = lp.pcg_directions == "bf16"
-    nbytes = operator_bytes(g.n, lp.C1, stored, 2 if bf else 4)
+    dirs = lp.pcg_directions
+    nbytes = operator_bytes(g.n, lp.C1, stored, 4 if dirs == "fp32" else 2)
     achieved = nbytes / (op_ms * 1e-3) / 1e9
-    kname = ("k_pcg_operator<4, BF16x4> (A*P on BF16 directions with fused p^T A p, the global solve)" if bf
-             else "k_pcg_operator<4, F32x4> (A*P with fused p^T A p, the global solve)")
-    traffic = pmc_traffic("dlp::k_pcg_operator<4, (dlp::Word)2>") if bf else None
+    word, wid = {"fp32": ("F32x4", 0), "bf16": ("BF16x4", 2), "fp16": ("F16x4", 3)}[dirs]
+    kname = f"k_pcg_operator<4, {word}> (A*P on {dirs.upper()} directions with fused p^T A p, the global solve)"
+    traffic = pmc_traffic(f"dlp::k_pcg_operator<4, (dlp::Word){wid}>")
     roofline = {"kernel": kname,
                 "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

@@ -22,10 +22,13 @@ struct PcgBufs {
   // [0] done (set once the violation is <= 1: every later pass of a captured group returns at
   // once), [1] iterations completed; null disables the device-side exit
   int32_t* ctl;
-  // BF16 directions (null: P in FP32): n x ldh values, ldh % 8 == 0; rp = r.p per column
+  // 16-bit directions (null: P in FP32): n x ldh values, ldh % 8 == 0; rp = r.p per column;
+  // p16 = 1 BF16, 2 FP16 held scaled by sig[c] (a power of two per column, C floats)
   uint16_t* P16;
   int64_t ldh;
   double* rp;
+  int p16;
+  float* sig;
 };
 
 // P = R/diag, rz = sum R*R/diag, pAp = 0.
@@ -40,9 +43,10 @@ int row_operator_dot(int n, int C, const int32_t* indptr, const int32_t* rowlen,
                      float* out, int64_t ldo, double* dot, hipStream_t st, const int32_t* skip = nullptr);
 
 // The global solve's operator: pipelined row gathers, rows outside rowmask skipped, neighbour
-// rows unmasked (P = 0 outside the solved set), fused p^T A p; skip = device exit flag (spmm.hip).
+// rows unmasked (P = 0 outside the solved set), fused p^T A p; skip = device exit flag;
+// x16_kind = 1 (BF16) or 2 (FP16) for X16 (spmm.hip).
 int pcg_operator(int n, int C, const int32_t* indptr, const int32_t* rowlen, const int32_t* indices, const float* w,
-                 const uint8_t* rowmask, const float* X, int64_t ldx, const uint16_t* X16, int64_t ldh,
+                 const uint8_t* rowmask, const float* X, int64_t ldx, const uint16_t* X16, int64_t ldh, int x16_kind,
                  const float* a, float b, float* out, int64_t ldo, double* dot, hipStream_t st, const int32_t* skip);
 
 // Blocks of a 256-thread kernel that are resident on the current device at once (spmm.hip).

@@ -148,28 +148,39 @@ __global__ __launch_bounds__(256) void k_pcg_dir(int n, int C, int lc_shift, flo
 // no lane idles on a power-of-two column layout.  Quad components at columns >= C are left
 // unchanged and excluded from the reductions.
 
-// The direction block P of the flat passes: FP32 (the n x Q4 float4 layout of Z and R) or BF16
-// (rows of H4 groups of 4 values, 8 bytes each; stores round to nearest even).
-template <bool BF>
+// The direction block P of the flat passes (PK): 0 = FP32 (the n x Q4 float4 layout of Z and
+// R); 1 = BF16 or 2 = FP16 (rows of H4 groups of 4 values, 8 bytes each; stores round to
+// nearest even, FP16 saturating).  FP16 directions are held scaled by a power of two per column
+// (the caller passes the scaled values), which keeps them inside FP16's range.
+template <int PK>
 struct PBlock {
   void* base;
   int64_t H4;
   __device__ __forceinline__ float4 load(int64_t i, int64_t r, int q4) const {
-    if constexpr (BF) {
+    if constexpr (PK == 1) {
       const uint2 u = reinterpret_cast<const uint2*>(base)[r * H4 + q4];
       return make_float4(bf16_lo(u.x), bf16_hi(u.x), bf16_lo(u.y), bf16_hi(u.y));
+    } else if constexpr (PK == 2) {
+      const uint2 u = reinterpret_cast<const uint2*>(base)[r * H4 + q4];
+      return make_float4(f16_lo(u.x), f16_hi(u.x), f16_lo(u.y), f16_hi(u.y));
     } else {
       return reinterpret_cast<const float4*>(base)[i];
     }
   }
-  // stores v and returns the value held (rounded to BF16 in the BF16 block)
+  // stores v and returns the value held (rounded in a 16-bit block)
   __device__ __forceinline__ float4 store(int64_t i, int64_t r, int q4, const float4& v) const {
-    if constexpr (BF) {
+    if constexpr (PK == 1) {
       uint2 u;
       u.x = bf16_pack(v.x, v.y);
       u.y = bf16_pack(v.z, v.w);
       reinterpret_cast<uint2*>(base)[r * H4 + q4] = u;
       return make_float4(bf16_lo(u.x), bf16_hi(u.x), bf16_lo(u.y), bf16_hi(u.y));
+    } else if constexpr (PK == 2) {
+      uint2 u;
+      u.x = f16_pack(v.x, v.y);
+      u.y = f16_pack(v.z, v.w);
+      reinterpret_cast<uint2*>(base)[r * H4 + q4] = u;
+      return make_float4(f16_lo(u.x), f16_hi(u.x), f16_lo(u.y), f16_hi(u.y));
     } else {
       reinterpret_cast<float4*>(base)[i] = v;
       return v;
@@ -187,16 +198,22 @@ __device__ __forceinline__ void flush_quads(const Quad& Qd, int Q4, int C, const
   for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&out[c], sm[c]);
 }
 
-// pcg start: P = R/diag, rz[c] += sum R*R/diag; with a BF16 block also rp[c] += sum R*P as held.
-template <bool BF>
-__global__ __launch_bounds__(256) void k_pcg_start4(int n, int Q4, int C, PBlock<BF> P, const float4* __restrict__ R,
+// pcg start: P = R/diag, rz[c] += sum R*R/diag; with a 16-bit block also rp[c] += sum R*P as
+// held.  FP16 blocks hold sig[c] * R/diag (sig from the rz of a prior k_pcg_rz4 pass).
+template <int PK>
+__global__ __launch_bounds__(256) void k_pcg_start4(int n, int Q4, int C, PBlock<PK> P, const float4* __restrict__ R,
                                                     const float* __restrict__ diag, double* __restrict__ rz,
-                                                    double* __restrict__ rp) {
+                                                    double* __restrict__ rp, const float* __restrict__ sig) {
   __shared__ double sm[kMaxCols + 4];
   __shared__ double sm2[kMaxCols + 4];
   for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) sm[c] = sm2[c] = 0.0;
   __syncthreads();
   const Quad Qd = quad_of(Q4, C);
+  float sg[4] = {1.f, 1.f, 1.f, 1.f};
+  if constexpr (PK == 2) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sg[j] = j < Qd.cnt ? sig[4 * Qd.q4 + j] : 1.f;
+  }
   const int64_t nq = (int64_t)n * Q4;
   double acc[4] = {0.0, 0.0, 0.0, 0.0}, acc2[4] = {0.0, 0.0, 0.0, 0.0};
   if (Qd.active && Qd.r0 < n) {
@@ -209,24 +226,46 @@ __global__ __launch_bounds__(256) void k_pcg_start4(int n, int Q4, int C, PBlock
       y.y = Qd.cnt > 1 ? rv.y * inv : y.y;
       y.z = Qd.cnt > 2 ? rv.z * inv : y.z;
       y.w = Qd.cnt > 3 ? rv.w * inv : y.w;
-      const float4 p = P.store(i, r, Qd.q4, y);
+      const float4 p = P.store(i, r, Qd.q4, make_float4(y.x * sg[0], y.y * sg[1], y.z * sg[2], y.w * sg[3]));
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         if (j < Qd.cnt) {
           acc[j] += (double)quad_comp(rv, j) * (double)quad_comp(y, j);
-          if constexpr (BF) acc2[j] += (double)quad_comp(rv, j) * (double)quad_comp(p, j);
+          if constexpr (PK != 0) acc2[j] += (double)quad_comp(rv, j) * (double)quad_comp(p, j);
         }
       }
     }
   }
   flush_quads(Qd, Q4, C, acc, sm, rz);
-  if constexpr (BF) flush_quads(Qd, Q4, C, acc2, sm2, rp);
+  if constexpr (PK != 0) flush_quads(Qd, Q4, C, acc2, sm2, rp);
+}
+
+// rz[c] += sum R*R/diag (the scale pass before an FP16 start).
+__global__ __launch_bounds__(256) void k_pcg_rz4(int n, int Q4, int C, const float4* __restrict__ R,
+                                                 const float* __restrict__ diag, double* __restrict__ rz) {
+  __shared__ double sm[kMaxCols + 4];
+  for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) sm[c] = 0.0;
+  __syncthreads();
+  const Quad Qd = quad_of(Q4, C);
+  const int64_t nq = (int64_t)n * Q4;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (Qd.active && Qd.r0 < n) {
+    int64_t r = Qd.r0;
+    for (int64_t i = Qd.i0; i < nq; i += Qd.step, r += Qd.rstep) {
+      const double inv = 1.0 / (double)diag[r];
+      const float4 rv = R[i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (j < Qd.cnt) acc[j] += (double)quad_comp(rv, j) * (double)quad_comp(rv, j) * inv;
+    }
+  }
+  flush_quads(Qd, Q4, C, acc, sm, rz);
 }
 
 // Z += a P, R -= a AP; rz_new[c] += sum R*R/diag; viol = max |R|/(eps_c*diag).
-template <bool BF>
+template <int PK>
 __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float4* __restrict__ Z,
-                                                     float4* __restrict__ R, PBlock<BF> P,
+                                                     float4* __restrict__ R, PBlock<PK> P,
                                                      const float4* __restrict__ AP,
                                                      const double* __restrict__ alpha,
                                                      const float* __restrict__ diag, const float* __restrict__ eps,
@@ -281,22 +320,28 @@ __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float
   if (threadIdx.x == 0) atomic_max_nonneg(viol, vmax);
 }
 
-// P = R/diag + beta P; with a BF16 block also rp[c] += sum R*P as held (the next step length's
-// numerator, exact for the rounded direction).
-template <bool BF>
-__global__ __launch_bounds__(256) void k_pcg_dir4(int n, int Q4, int C, PBlock<BF> P, const float4* __restrict__ R,
+// P = R/diag + beta P; with a 16-bit block also rp[c] += sum R*P as held (the next step
+// length's numerator, exact for the rounded direction).  FP16 blocks: P = sig*R/diag + beta*P
+// with sig the new column scale and beta already rescaled to the held direction.
+template <int PK>
+__global__ __launch_bounds__(256) void k_pcg_dir4(int n, int Q4, int C, PBlock<PK> P, const float4* __restrict__ R,
                                                   const float* __restrict__ diag, const double* __restrict__ beta,
-                                                  double* __restrict__ rp, const int32_t* __restrict__ skip) {
+                                                  double* __restrict__ rp, const int32_t* __restrict__ skip,
+                                                  const float* __restrict__ sig) {
   if (skip && *skip) return;
   __shared__ double sm[kMaxCols + 4];
-  if constexpr (BF) {
+  if constexpr (PK != 0) {
     for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) sm[c] = 0.0;
     __syncthreads();
   }
   const Quad Qd = quad_of(Q4, C);
-  float b[4];
+  float b[4], sg[4] = {1.f, 1.f, 1.f, 1.f};
 #pragma unroll
   for (int j = 0; j < 4; ++j) b[j] = j < Qd.cnt ? (float)beta[4 * Qd.q4 + j] : 0.f;
+  if constexpr (PK == 2) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sg[j] = j < Qd.cnt ? sig[4 * Qd.q4 + j] : 1.f;
+  }
   const int64_t nq = (int64_t)n * Q4;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   if (Qd.active && Qd.r0 < n) {
@@ -305,19 +350,26 @@ __global__ __launch_bounds__(256) void k_pcg_dir4(int n, int Q4, int C, PBlock<B
       const float inv = 1.f / diag[r];
       const float4 rv = R[i];
       float4 p = P.load(i, r, Qd.q4);
-      if (Qd.cnt > 0) p.x = rv.x * inv + b[0] * p.x;
-      if (Qd.cnt > 1) p.y = rv.y * inv + b[1] * p.y;
-      if (Qd.cnt > 2) p.z = rv.z * inv + b[2] * p.z;
-      if (Qd.cnt > 3) p.w = rv.w * inv + b[3] * p.w;
+      if constexpr (PK == 2) {
+        if (Qd.cnt > 0) p.x = rv.x * inv * sg[0] + b[0] * p.x;
+        if (Qd.cnt > 1) p.y = rv.y * inv * sg[1] + b[1] * p.y;
+        if (Qd.cnt > 2) p.z = rv.z * inv * sg[2] + b[2] * p.z;
+        if (Qd.cnt > 3) p.w = rv.w * inv * sg[3] + b[3] * p.w;
+      } else {
+        if (Qd.cnt > 0) p.x = rv.x * inv + b[0] * p.x;
+        if (Qd.cnt > 1) p.y = rv.y * inv + b[1] * p.y;
+        if (Qd.cnt > 2) p.z = rv.z * inv + b[2] * p.z;
+        if (Qd.cnt > 3) p.w = rv.w * inv + b[3] * p.w;
+      }
       const float4 held = P.store(i, r, Qd.q4, p);
-      if constexpr (BF) {
+      if constexpr (PK != 0) {
 #pragma unroll
         for (int j = 0; j < 4; ++j)
           if (j < Qd.cnt) acc[j] += (double)quad_comp(rv, j) * (double)quad_comp(held, j);
       }
     }
   }
-  if constexpr (BF) flush_quads(Qd, Q4, C, acc, sm, rp);
+  if constexpr (PK != 0) flush_quads(Qd, Q4, C, acc, sm, rp);
 }
 
 struct FlatLaunch {
@@ -344,13 +396,24 @@ static FlatLaunch flat_launch(const PcgBufs& b) {
   return f;
 }
 
+// Power-of-two scale of an FP16 direction column from its rz = sum R*R/diag: the held values
+// sig*R/diag stay within about sqrt(1/diag) of 1 (far inside FP16's range); 1 when rz = 0.
+__device__ __forceinline__ float pcg_scale(double rz) {
+  if (!(rz > 0.0)) return 1.f;
+  double e = rint(-0.5 * log2(rz));
+  e = fmin(fmax(e, -100.0), 100.0);
+  return (float)ldexp(1.0, (int)e);
+}
+
 // Scalars of one iteration (one workgroup): mode 0: alpha = rz/pAp (0 where pAp <= 0), then
 // clear rz_new and viol for the update pass; mode 1: beta = rz_new/rz (0 where rz <= 0),
-// rz = rz_new, then clear pAp for the next iteration.
+// rz = rz_new, then clear pAp for the next iteration; with sig (FP16 directions) also the new
+// column scale, and beta rescaled to the held direction (beta * sig_new / sig_old).  Mode 2
+// (before an FP16 start): sig from rz, then rz = 0.
 __global__ void k_pcg_scalars(int C, int mode, double* __restrict__ rz, double* __restrict__ rz_new,
                               double* __restrict__ pAp, double* __restrict__ alpha,
                               double* __restrict__ beta, float* __restrict__ viol, int32_t* ctl,
-                              double* __restrict__ rp) {
+                              double* __restrict__ rp, float* __restrict__ sig) {
   // every thread reads the flag before thread 0 may set it below (after the barrier)
   if (ctl && ctl[0]) return;
   for (int c = threadIdx.x; c < C; c += blockDim.x) {
@@ -359,12 +422,21 @@ __global__ void k_pcg_scalars(int C, int mode, double* __restrict__ rz, double* 
       const double p = pAp[c];
       alpha[c] = p > 0.0 ? (rp ? rp[c] : rz[c]) / p : 0.0;
       rz_new[c] = 0.0;
-    } else {
+    } else if (mode == 1) {
       const double r = rz[c];
-      beta[c] = r > 0.0 ? rz_new[c] / r : 0.0;
+      double bt = r > 0.0 ? rz_new[c] / r : 0.0;
+      if (sig) {
+        const float s = pcg_scale(rz_new[c]);
+        bt *= (double)s / (double)sig[c];
+        sig[c] = s;
+      }
+      beta[c] = bt;
       rz[c] = rz_new[c];
       pAp[c] = 0.0;
       if (rp) rp[c] = 0.0;
+    } else {
+      sig[c] = pcg_scale(rz[c]);
+      rz[c] = 0.0;
     }
   }
   if (mode == 0 && threadIdx.x == 0) *viol = 0.f;
@@ -401,17 +473,27 @@ int pcg_start(const PcgBufs& b, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, b.C, b.pAp);
   if (b.ctl) hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, 1, (double*)b.ctl);
   const FlatLaunch f = flat_launch(b);
-  DLP_CHECK_ARG(f.ok || !b.P16, "pcg: BF16 directions need 16-byte FP32 rows");
+  DLP_CHECK_ARG(f.ok || !b.P16, "pcg: 16-bit directions need 16-byte FP32 rows");
+  DLP_CHECK_ARG(!b.P16 || b.p16 == 1 || (b.p16 == 2 && b.sig), "pcg: 16-bit direction kind %d", b.p16);
   if (b.P16) {
     hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, b.C, b.rp);
-    hipLaunchKernelGGL(k_pcg_start4<true>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
-                       PBlock<true>{b.P16, b.ldh / 4}, (const float4*)b.R, b.diag, b.rz, b.rp);
+    if (b.p16 == 2) {
+      hipLaunchKernelGGL(k_pcg_rz4, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (const float4*)b.R, b.diag,
+                         b.rz);
+      hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 2, b.rz, b.rz_new, b.pAp, b.alpha,
+                         b.beta, b.viol, nullptr, nullptr, b.sig);
+      hipLaunchKernelGGL(k_pcg_start4<2>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
+                         PBlock<2>{b.P16, b.ldh / 4}, (const float4*)b.R, b.diag, b.rz, b.rp, b.sig);
+    } else {
+      hipLaunchKernelGGL(k_pcg_start4<1>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
+                         PBlock<1>{b.P16, b.ldh / 4}, (const float4*)b.R, b.diag, b.rz, b.rp, nullptr);
+    }
     DLP_CHECK_LAUNCH();
     return DLP_OK;
   }
   if (f.ok) {
-    hipLaunchKernelGGL(k_pcg_start4<false>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
-                       PBlock<false>{b.P, 0}, (const float4*)b.R, b.diag, b.rz, nullptr);
+    hipLaunchKernelGGL(k_pcg_start4<0>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
+                       PBlock<0>{b.P, 0}, (const float4*)b.R, b.diag, b.rz, nullptr, nullptr);
     DLP_CHECK_LAUNCH();
     return DLP_OK;
   }
@@ -439,38 +521,40 @@ int pcg_finish_iteration(const PcgBufs& b, hipStream_t st, bool have_pap) {
 #undef K
     }
     double* rp = b.P16 ? b.rp : nullptr;
+    float* sig = (b.P16 && b.p16 == 2) ? b.sig : nullptr;
     hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 0, b.rz, b.rz_new, b.pAp, b.alpha,
-                       b.beta, b.viol, b.ctl, rp);
-    if (b.P16)
-      hipLaunchKernelGGL(k_pcg_update4<true>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (float4*)b.Z,
-                         (float4*)b.R, PBlock<true>{b.P16, b.ldh / 4}, (const float4*)b.AP, b.alpha, b.diag, b.eps,
-                         b.rz_new, b.viol, b.ctl);
-    else
-      hipLaunchKernelGGL(k_pcg_update4<false>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (float4*)b.Z,
-                         (float4*)b.R, PBlock<false>{b.P, 0}, (const float4*)b.AP, b.alpha, b.diag, b.eps,
-                         b.rz_new, b.viol, b.ctl);
+                       b.beta, b.viol, b.ctl, rp, sig);
+#define DLP_UPDATE(PKV, PB)                                                                                  \
+  hipLaunchKernelGGL(k_pcg_update4<PKV>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (float4*)b.Z, \
+                     (float4*)b.R, PBlock<PKV>{PB, b.P16 ? b.ldh / 4 : 0}, (const float4*)b.AP, b.alpha, b.diag, \
+                     b.eps, b.rz_new, b.viol, b.ctl)
+#define DLP_DIR(PKV, PB)                                                                                     \
+  hipLaunchKernelGGL(k_pcg_dir4<PKV>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,                  \
+                     PBlock<PKV>{PB, b.P16 ? b.ldh / 4 : 0}, (const float4*)b.R, b.diag, b.beta, rp, b.ctl, sig)
+    if (!b.P16) DLP_UPDATE(0, b.P);
+    else if (b.p16 == 2) DLP_UPDATE(2, b.P16);
+    else DLP_UPDATE(1, b.P16);
     hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 1, b.rz, b.rz_new, b.pAp, b.alpha,
-                       b.beta, b.viol, b.ctl, rp);
-    if (b.P16)
-      hipLaunchKernelGGL(k_pcg_dir4<true>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
-                         PBlock<true>{b.P16, b.ldh / 4}, (const float4*)b.R, b.diag, b.beta, b.rp, b.ctl);
-    else
-      hipLaunchKernelGGL(k_pcg_dir4<false>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
-                         PBlock<false>{b.P, 0}, (const float4*)b.R, b.diag, b.beta, nullptr, b.ctl);
+                       b.beta, b.viol, b.ctl, rp, sig);
+    if (!b.P16) DLP_DIR(0, b.P);
+    else if (b.p16 == 2) DLP_DIR(2, b.P16);
+    else DLP_DIR(1, b.P16);
+#undef DLP_UPDATE
+#undef DLP_DIR
     DLP_CHECK_LAUNCH();
     return DLP_OK;
   }
-  DLP_CHECK_ARG(!b.P16, "pcg: BF16 directions need 16-byte FP32 rows");
+  DLP_CHECK_ARG(!b.P16, "pcg: 16-bit directions need 16-byte FP32 rows");
 #define K(CPLV)                                                                                       \
   if (!have_pap)                                                                                      \
     hipLaunchKernelGGL((k_coldot<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.P, b.AP, \
                        b.ld, b.pAp);                                                                 \
   hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 0, b.rz, b.rz_new, b.pAp,        \
-                     b.alpha, b.beta, b.viol, b.ctl, nullptr);                                       \
+                     b.alpha, b.beta, b.viol, b.ctl, nullptr, nullptr);                              \
   hipLaunchKernelGGL((k_pcg_update<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.Z, b.R, \
                      b.P, b.AP, b.ld, b.alpha, b.diag, b.eps, b.rz_new, b.viol, b.ctl);              \
   hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 1, b.rz, b.rz_new, b.pAp,        \
-                     b.alpha, b.beta, b.viol, b.ctl, nullptr);                                       \
+                     b.alpha, b.beta, b.viol, b.ctl, nullptr, nullptr);                              \
   hipLaunchKernelGGL((k_pcg_dir<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.P, b.R,   \
                      b.ld, b.diag, b.beta, 0, b.ctl);
   DLP_DISPATCH_CPL(m.cpl, K)

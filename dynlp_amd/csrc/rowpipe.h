@@ -34,9 +34,21 @@ __device__ __forceinline__ unsigned bf16_bits(float x) {
 __device__ __forceinline__ unsigned bf16_pack(float lo, float hi) { return bf16_bits(lo) | (bf16_bits(hi) << 16); }
 __device__ __forceinline__ float bf16_round(float x) { return __uint_as_float(bf16_bits(x) << 16); }
 
-// Lane words of the gathered rows: FP32 x 4 (16 bytes), BF16 x 8 (16 bytes) or BF16 x 4
-// (8 bytes).  W values per lane, column order.
-enum class Word { F32x4, BF16x8, BF16x4 };
+// FP16 <-> FP32.  The conversion to FP32 happens inside the multiply-add (v_fma_mix_f32 reads
+// either half of a 32-bit register), so FP16 words cost no unpacking instructions.
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float f16_lo(unsigned u) { return (float)__builtin_bit_cast(h2v, u).x; }
+__device__ __forceinline__ float f16_hi(unsigned u) { return (float)__builtin_bit_cast(h2v, u).y; }
+// round to nearest even, saturating at +-65504 (a direction value never becomes infinite)
+__device__ __forceinline__ unsigned f16_pack(float lo, float hi) {
+  const float m = 65504.f;
+  const h2v h = {(_Float16)fminf(fmaxf(lo, -m), m), (_Float16)fminf(fmaxf(hi, -m), m)};
+  return __builtin_bit_cast(unsigned, h);
+}
+
+// Lane words of the gathered rows: FP32 x 4 (16 bytes), BF16 x 8 (16 bytes), BF16 x 4 or
+// FP16 x 4 (8 bytes).  W values per lane, column order.
+enum class Word { F32x4, BF16x8, BF16x4, F16x4 };
 
 template <Word K> struct WordT;
 template <> struct WordT<Word::F32x4> {
@@ -48,6 +60,10 @@ template <> struct WordT<Word::BF16x8> {
   using raw = u4v;
 };
 template <> struct WordT<Word::BF16x4> {
+  static constexpr int W = 4, bytes = 8;
+  using raw = unsigned int __attribute__((ext_vector_type(2)));
+};
+template <> struct WordT<Word::F16x4> {
   static constexpr int W = 4, bytes = 8;
   using raw = unsigned int __attribute__((ext_vector_type(2)));
 };
@@ -78,11 +94,16 @@ __device__ __forceinline__ void unpack(const typename WordT<K>::raw& u, float (&
     x[5] = bf16_hi(u.z);
     x[6] = bf16_lo(u.w);
     x[7] = bf16_hi(u.w);
-  } else {
+  } else if constexpr (K == Word::BF16x4) {
     x[0] = bf16_lo(u.x);
     x[1] = bf16_hi(u.x);
     x[2] = bf16_lo(u.y);
     x[3] = bf16_hi(u.y);
+  } else {
+    x[0] = f16_lo(u.x);
+    x[1] = f16_hi(u.x);
+    x[2] = f16_lo(u.y);
+    x[3] = f16_hi(u.y);
   }
 }
 
@@ -135,7 +156,10 @@ __device__ __forceinline__ void fmaw(TA (&acc)[WordT<K>::W], float wv, const typ
   float x[W];
   unpack<K>(u, x);
 #pragma unroll
-  for (int q = 0; q < W; ++q) acc[q] += (TA)wv * (TA)x[q];
+  for (int q = 0; q < W; ++q) {
+    if constexpr (std::is_same<TA, float>::value) acc[q] = __builtin_fmaf(wv, x[q], acc[q]);
+    else acc[q] += (TA)wv * (TA)x[q];
+  }
 }
 
 // Runs every row of the pass: epi.pre(r) before row r's gathers are issued (own-row loads),

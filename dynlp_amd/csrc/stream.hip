@@ -1019,10 +1019,10 @@ struct LP {
   // global solve (column-wise PCG on the FP32 state)
   float* P = nullptr;
   float* AP = nullptr;
-  uint16_t* P16 = nullptr;      // BF16 directions of the column PCG (when pcg_bf16)
+  uint16_t* P16 = nullptr;      // 16-bit directions of the column PCG (pcg_p16 != 0)
   int64_t ldh = 0;              // its row length: C1 rounded up to 8
-  int pcg_bf16 = 0;
-  double* pcg_scal = nullptr;   // rz, rz_new, pAp, alpha, beta, rp (C1 each)
+  int pcg_p16 = 0;              // 0 FP32 directions, 1 BF16, 2 FP16 with a per-column scale
+  double* pcg_scal = nullptr;   // rz, rz_new, pAp, alpha, beta, rp (C1 doubles each), sig (C1 floats)
   float* pcg_viol = nullptr;
   hipGraphExec_t pcg_exec = nullptr;
   int pcg_graph_iters = 0;
@@ -1723,9 +1723,11 @@ static PcgBufs pcg_bufs(LP* h) {
   b.beta = h->pcg_scal + 4 * C1;
   b.viol = h->pcg_viol;
   b.ctl = h->pcg_viol ? reinterpret_cast<int32_t*>(h->pcg_viol) + 2 : nullptr;
-  b.P16 = h->pcg_bf16 ? h->P16 : nullptr;
+  b.P16 = h->pcg_p16 ? h->P16 : nullptr;
   b.ldh = h->ldh;
   b.rp = h->pcg_scal + 5 * C1;
+  b.p16 = h->pcg_p16;
+  b.sig = reinterpret_cast<float*>(h->pcg_scal + 6 * C1);
   return b;
 }
 
@@ -1733,7 +1735,7 @@ static int pcg_iteration(LP* h, const PcgBufs& b, hipStream_t st) {
   // AP = diag*P - W_UU P, with p^T A p accumulated in the same pass when the layout allows
   if (b.P16 || ((b.ld % 4) == 0 && tuning().vec4 && tuning().fuse_dot)) {
     int rc = pcg_operator(b.n, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.solved, b.P, b.ld, b.P16,
-                          b.ldh, b.diag, -1.f, b.AP, b.ld, b.pAp, st, b.ctl);
+                          b.ldh, b.p16, b.diag, -1.f, b.AP, b.ld, b.pAp, st, b.ctl);
     if (rc) return rc;
     return pcg_finish_iteration(b, st, true);
   }
@@ -1788,12 +1790,12 @@ extern "C" int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int 
   if (!h->P) {
     DLP_CHECK_HIP(hipMalloc((void**)&h->P, nz * sizeof(float)));
     DLP_CHECK_HIP(hipMalloc((void**)&h->AP, nz * sizeof(float)));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_scal, 6 * C1 * sizeof(double)));
+    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_scal, 7 * C1 * sizeof(double)));
     DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_viol, 4 * sizeof(float)));   // viol, pad, done, iterations
     DLP_CHECK_HIP(hipMemsetAsync(h->P, 0, nz * sizeof(float), st));
     DLP_CHECK_HIP(hipMemsetAsync(h->AP, 0, nz * sizeof(float), st));
   }
-  if (h->pcg_bf16 && !h->P16) {
+  if (h->pcg_p16 && !h->P16) {
     h->ldh = (C1 + 7) / 8 * 8;
     const size_t nh = (size_t)h->d.n * h->ldh;
     DLP_CHECK_HIP(hipMalloc((void**)&h->P16, nh * sizeof(uint16_t)));
@@ -1874,15 +1876,16 @@ extern "C" int dlp_lp_scores(void* handle, double* out, int64_t ldo, void* strea
   return DLP_OK;
 }
 
-extern "C" int dlp_lp_set_pcg_bf16(void* handle, int on) {
-  DLP_CHECK_ARG(handle, "lp_set_pcg_bf16: null handle");
+extern "C" int dlp_lp_set_pcg_directions(void* handle, int kind) {
+  DLP_CHECK_ARG(handle, "lp_set_pcg_directions: null handle");
+  DLP_CHECK_ARG(kind >= 0 && kind <= 2, "lp_set_pcg_directions: kind %d (0 FP32, 1 BF16, 2 FP16)", kind);
   LP* h = (LP*)handle;
-  DLP_CHECK_ARG(!on || ((h->d.ld % 4) == 0 && h->d.C1 <= 256), "lp_set_pcg_bf16: needs 16-byte FP32 rows");
-  if (h->pcg_bf16 != (on ? 1 : 0) && h->pcg_exec) {   // captured iterations use the other layout
+  DLP_CHECK_ARG(!kind || ((h->d.ld % 4) == 0 && h->d.C1 <= 256), "lp_set_pcg_directions: needs 16-byte FP32 rows");
+  if (h->pcg_p16 != kind && h->pcg_exec) {   // captured iterations use the other layout
     (void)hipGraphExecDestroy(h->pcg_exec);
     h->pcg_exec = nullptr;
   }
-  h->pcg_bf16 = on ? 1 : 0;
+  h->pcg_p16 = kind;
   return DLP_OK;
 }
 
@@ -1909,7 +1912,7 @@ extern "C" int dlp_lp_time_operator(void* handle, int reps, double* ms_out, void
   auto launch = [&]() -> int {
     if (fused || b.P16)
       return pcg_operator(b.n, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.solved, b.P, b.ld, b.P16,
-                          b.ldh, b.diag, -1.f, b.AP, b.ld, b.pAp, st, nullptr);
+                          b.ldh, b.p16, b.diag, -1.f, b.AP, b.ld, b.pAp, st, nullptr);
     return dlp_row_operator(b.n, nullptr, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, DLP_F32, h->d.solved,
                             b.P, b.ld, DLP_F32, b.diag, -1.0, nullptr, 0, b.AP, b.ld, DLP_F32, 0, 20.0, st);
   };
@@ -2002,7 +2005,7 @@ extern "C" int dlp_lp_bpcg(void* handle, int max_iter, int iters_per_launch, int
   if (!h->P) {
     DLP_CHECK_HIP(hipMalloc((void**)&h->P, nz * sizeof(float)));
     DLP_CHECK_HIP(hipMalloc((void**)&h->AP, nz * sizeof(float)));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_scal, 6 * C1 * sizeof(double)));
+    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_scal, 7 * C1 * sizeof(double)));
     DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_viol, 4 * sizeof(float)));   // viol, pad, done, iterations
     DLP_CHECK_HIP(hipMemsetAsync(h->P, 0, nz * sizeof(float), st));
     DLP_CHECK_HIP(hipMemsetAsync(h->AP, 0, nz * sizeof(float), st));
@@ -2010,6 +2013,7 @@ extern "C" int dlp_lp_bpcg(void* handle, int max_iter, int iters_per_launch, int
   if (!h->bk) DLP_CHECK_HIP(hipMalloc((void**)&h->bk, 5 * C1 * C1 * sizeof(double)));
   PcgBufs b = pcg_bufs(h);
   b.P16 = nullptr;                                 // block PCG keeps FP32 directions
+  b.p16 = 0;
   int rc = pcg_start(b, st);                       // P = R/diag
   if (!rc) rc = dlp_gram(b.n, C1, b.R, b.ld, b.R, b.ld, b.diag, h->bk, DLP_F32, st);   // G = R^T D^-1 R
   if (rc) return rc;

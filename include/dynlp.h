@@ -310,12 +310,15 @@ int dlp_lp_pcg_stats(void* handle, int64_t* out);
 /* Average duration in ms (HIP events on `stream`, synchronises) of the global solve's operator
  * launch AP = diag*P - W_UU P on the live state, over reps launches (after dlp_lp_pcg). */
 int dlp_lp_time_operator(void* handle, int reps, double* ms_out, void* stream);
-/* Column PCG directions stored in BF16 (on = 1) or FP32 (0, default).  Z, R and A*P stay FP32
- * with FP64 column dots; the step length is r.p / p.Ap for the direction as held, so the
- * residual recurrence stays that of the FP32 path and the FP64 verification certifies as
- * before.  Halves the bytes of every neighbour-row gather of the global solve.  No reference
- * counterpart (solvers.py:66-97 keeps P in the working dtype). */
-int dlp_lp_set_pcg_bf16(void* handle, int on);
+/* Storage of the column PCG directions: kind 0 = FP32 (default), 1 = BF16, 2 = FP16 held
+ * scaled by a power of two per column (chosen from r.z so the values sit near 1; saturating
+ * stores).  Z, R and A*P stay FP32 with FP64 column dots; the step length is r.p / p.Ap for
+ * the direction as held, so the residual recurrence stays that of the FP32 path and the FP64
+ * verification certifies as before.  16-bit directions halve the bytes of every neighbour-row
+ * gather of the global solve; FP16 keeps 11 significant bits (BF16: 8) and its values enter the
+ * multiply-adds without unpacking.  No reference counterpart (solvers.py:66-97 keeps P in the
+ * working dtype). */
+int dlp_lp_set_pcg_directions(void* handle, int kind);
 /* Plain FP64 path, one host-driven round of solvers.push (solvers.py:108-116): candidates
  * take D = R/diag, Z += D, R = 0, scatter W D with FP64 atomics and mark N(cand) and cand. */
 int dlp_lp_push64_round(void* handle, int nc, const int32_t* cand, void* stream);

@@ -41,7 +41,7 @@ def check_against_oracle(lp, g, es, eps, label):
     return err, rc[: sysm.C].max()
 
 
-@pytest.mark.parametrize("fp64,dirs", [(False, "fp32"), (True, "fp32"), (False, "bf16")])
+@pytest.mark.parametrize("fp64,dirs", [(False, "fp32"), (True, "fp32"), (False, "bf16"), (False, "fp16")])
 @pytest.mark.parametrize("K", [2, 8])
 def test_stream_matches_rebuilt_system(fp64, dirs, K):
     from dynlp_amd.stream import EdgeStreamLP
@@ -92,11 +92,11 @@ def test_stream_lane_layouts_and_row_orders(pack, xcd, vec4, pipe):
         _lib.call("dlp_tune_set", 3, 1)
 
 
-@pytest.mark.parametrize("dirs,word", [("fp32", 4), ("bf16", 4), ("bf16", 8)])
+@pytest.mark.parametrize("dirs,word", [("fp32", 4), ("bf16", 4), ("bf16", 8), ("fp16", 4)])
 def test_global_solve_from_zero_certifies(dirs, word):
-    """A global solve from a zero state (every row violating) under FP32 and BF16 directions
-    (8- and 16-byte lane words): certified against the rebuilt oracle, on a fragmenting graph
-    with inert rows."""
+    """A global solve from a zero state (every row violating) under FP32, BF16 (8- and 16-byte
+    lane words) and scaled FP16 directions: certified against the rebuilt oracle, on a
+    fragmenting graph with inert rows."""
     from dynlp_amd import _lib
     from dynlp_amd.stream import EdgeStreamLP
     _lib.call("dlp_tune_set", 7, word)
