@@ -10,8 +10,7 @@ __global__ __launch_bounds__(256) void k_frontier_jacobi(
     int nf, const int32_t* __restrict__ frontier, int C, int g_shift, int lc_shift,
     const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
     const T* __restrict__ w, const T* __restrict__ X, const T* __restrict__ rhs,
-    const T* __restrict__ diag, T delta, T* __restrict__ Y, uint8_t* __restrict__ changed,
-    unsigned long long* __restrict__ edges) {
+    const T* __restrict__ diag, T delta, T* __restrict__ Y, uint8_t* __restrict__ changed) {
   const Lanes L(g_shift, lc_shift);
   const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> g_shift;
   const bool valid = g < nf;
@@ -39,10 +38,6 @@ __global__ __launch_bounds__(256) void k_frontier_jacobi(
   const int wl = threadIdx.x & 63;
   const unsigned long long gm = (L.G == 64 ? ~0ull : ((1ull << L.G) - 1)) << (wl & ~(L.G - 1));
   if (valid && L.lane == 0) changed[g] = (bal & gm) != 0;
-  if (edges) {
-    const unsigned long long s = wave_sum_u64((valid && L.lane == 0) ? (unsigned long long)(end - beg) : 0ull);
-    if (wl == 0 && s) atomicAdd(edges, s);
-  }
 }
 
 // R[col] += w * D[g] and mark[col] = 1 over the rows of the frontier.
@@ -51,7 +46,7 @@ __global__ __launch_bounds__(256) void k_push_scatter(
     int nf, const int32_t* __restrict__ frontier, int C, int g_shift, int lc_shift,
     const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
     const T* __restrict__ w, const T* __restrict__ D, T* __restrict__ R,
-    uint8_t* __restrict__ mark, unsigned long long* __restrict__ edges) {
+    uint8_t* __restrict__ mark) {
   const Lanes L(g_shift, lc_shift);
   const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> g_shift;
   const bool valid = g < nf;
@@ -73,27 +68,47 @@ __global__ __launch_bounds__(256) void k_push_scatter(
     }
     if (L.l == 0) mark[v] = 1;
   }
-  if (edges) {
-    const unsigned long long s = wave_sum_u64((valid && L.lane == 0) ? (unsigned long long)(end - beg) : 0ull);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(edges, s);
-  }
 }
 
 __global__ __launch_bounds__(256) void k_mark_neighbors(int nr, const int32_t* __restrict__ rows,
                                                         const int32_t* __restrict__ indptr,
                                                         const int32_t* __restrict__ indices,
-                                                        uint8_t* __restrict__ mark,
-                                                        unsigned long long* __restrict__ edges) {
+                                                        uint8_t* __restrict__ mark) {
   const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 3;
   const int lane = threadIdx.x & 7;
   const bool valid = g < nr;
   const int r = valid ? rows[g] : 0;
   const int beg = valid ? indptr[r] : 0, end = valid ? indptr[r + 1] : 0;
   for (int j = beg + lane; j < end; j += 8) mark[indices[j]] = 1;
-  if (edges) {
-    const unsigned long long s = wave_sum_u64((valid && lane == 0) ? (unsigned long long)(end - beg) : 0ull);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(edges, s);
+}
+
+// edges += sum of the row lengths of rows[0..nr): a grid-stride pass with one atomic per
+// workgroup (a counter updated by every wavefront of a large frontier serialises at one L2
+// address and costs more than the gathers themselves).
+__global__ __launch_bounds__(256) void k_count_edges(int nr, const int32_t* __restrict__ rows,
+                                                     const int32_t* __restrict__ indptr,
+                                                     unsigned long long* __restrict__ edges) {
+  __shared__ unsigned long long part[4];
+  unsigned long long s = 0;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < nr; g += (int64_t)gridDim.x * blockDim.x) {
+    const int r = rows[g];
+    s += (unsigned long long)(indptr[r + 1] - indptr[r]);
   }
+  s = wave_sum_u64(s);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+    if (t) atomicAdd(edges, t);
+  }
+}
+
+static void count_edges(int nr, const int32_t* rows, const int32_t* indptr, unsigned long long* edges,
+                        hipStream_t st) {
+  if (!edges || nr <= 0) return;
+  unsigned g = grid_for(nr);
+  if (g > 512) g = 512;
+  hipLaunchKernelGGL(k_count_edges, dim3(g), dim3(kBlock), 0, st, nr, rows, indptr, edges);
 }
 
 // One asynchronous push round (chaotic relaxation): the residual of every frontier vertex is
@@ -204,9 +219,10 @@ static int frontier_jacobi_t(int nf, const int32_t* frontier, int C, const int32
 #define K(CPLV)                                                                               \
   hipLaunchKernelGGL((k_frontier_jacobi<T, CPLV>), grid, block, 0, st, nf, frontier, C,       \
                      m.g_shift, m.lc_shift, indptr, indices, (const T*)w, (const T*)X,        \
-                     (const T*)rhs, (const T*)diag, (T)delta, (T*)Y, changed, edges)
+                     (const T*)rhs, (const T*)diag, (T)delta, (T*)Y, changed)
   DLP_DISPATCH_CPL(m.cpl, K)
 #undef K
+  count_edges(nf, frontier, indptr, edges, st);
   DLP_CHECK_LAUNCH();
   return DLP_OK;
 }
@@ -241,9 +257,10 @@ static int push_scatter_t(int nf, const int32_t* frontier, int C, const int32_t*
 #define K(CPLV)                                                                              \
   hipLaunchKernelGGL((k_push_scatter<T, CPLV>), grid, block, 0, st, nf, frontier, C,         \
                      m.g_shift, m.lc_shift, indptr, indices, (const T*)w, (const T*)D, (T*)R, \
-                     mark, edges)
+                     mark)
   DLP_DISPATCH_CPL(m.cpl, K)
 #undef K
+  count_edges(nf, frontier, indptr, edges, st);
   DLP_CHECK_LAUNCH();
   return DLP_OK;
 }
@@ -271,7 +288,8 @@ extern "C" int dlp_mark_neighbors(int nr, const int32_t* rows, const int32_t* in
   if (nr == 0) return DLP_OK;
   DLP_CHECK_ARG(rows && indptr && indices && mark, "mark_neighbors: null pointer");
   hipLaunchKernelGGL(k_mark_neighbors, dim3(grid_for((long long)nr * 8)), dim3(kBlock), 0,
-                     (hipStream_t)stream, nr, rows, indptr, indices, mark, edges);
+                     (hipStream_t)stream, nr, rows, indptr, indices, mark);
+  count_edges(nr, rows, indptr, edges, (hipStream_t)stream);
   DLP_CHECK_LAUNCH();
   return DLP_OK;
 }
