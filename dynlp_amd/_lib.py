@@ -30,7 +30,7 @@ _SIGS = {
                                     P, I64, I32, I32, DBL, P]),
     "dlp_rho_cols": (ct.c_int, [I32, P, I32, P, I64, I32, P, P, P]),
     "dlp_violating_rows": (ct.c_int, [I32, P, I32, P, I64, I32, P, P, P, P, P]),
-    "dlp_frontier_jacobi": (ct.c_int, [I32, P, I32, P, P, P, P, P, P, DBL, P, P, P, I32, DBL, P]),
+    "dlp_frontier_jacobi": (ct.c_int, [I32, P, I32, I32, P, P, P, P, P, P, DBL, P, P, P, I32, DBL, P]),
     "dlp_push_scatter": (ct.c_int, [I32, P, I32, P, P, P, P, P, P, P, I32, DBL, P]),
     "dlp_mark_neighbors": (ct.c_int, [I32, P, P, P, P, P, P]),
     "dlp_flatnonzero_u8": (ct.c_int, [I32, P, P, P, P, SZ, P]),

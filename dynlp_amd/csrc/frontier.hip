@@ -1,6 +1,7 @@
 // Frontier primitives: Jacobi on a vertex subset, residual scatter, neighbour marking, and the
 // asynchronous fused push (take residual, update, scatter, compact).
-#include "rowgather.h"
+#include "blas.h"
+#include "rowpipe.h"
 
 namespace dlp {
 
@@ -38,6 +39,61 @@ __global__ __launch_bounds__(256) void k_frontier_jacobi(
   const int wl = threadIdx.x & 63;
   const unsigned long long gm = (L.G == 64 ? ~0ull : ((1ull << L.G) - 1)) << (wl & ~(L.G - 1));
   if (valid && L.lane == 0) changed[g] = (bal & gm) != 0;
+}
+
+// FP32 Jacobi on the pipelined packed-lane gathers (rowpipe.h): row r = frontier[g],
+// Y[g] = (rhs[r] + sum w X[col]) / diag[r], changed[g] = any |Y[g] - X[r]| > delta.  X and rhs
+// are n x C, Y is nf x C, all contiguous (4-byte aligned rows).
+template <Word K>
+struct JacobiEpilogue {
+  const LanesP& L;
+  int C;
+  bool owner;
+  const float* __restrict__ X;
+  const float* __restrict__ rhs;
+  const float* __restrict__ diag;
+  float delta;
+  float* __restrict__ Y;
+  uint8_t* __restrict__ changed;
+  bool vec_out;
+  float xr[4], br[4];
+  float inv;
+  __device__ __forceinline__ void pre(int, int r) {
+    if (owner) {
+      load4(X + (int64_t)r * C, 4 * L.l, C, xr);
+      load4(rhs + (int64_t)r * C, 4 * L.l, C, br);
+    }
+    inv = 1.f / diag[r];
+  }
+  __device__ __forceinline__ void post(int g, int, bool, const float (&acc)[4]) {
+    bool ch = false;
+    if (owner) {
+      float y[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        y[q] = (br[q] + acc[q]) * inv;
+        ch |= (4 * L.l + q < C) && fabsf(y[q] - xr[q]) > delta;
+      }
+      store4(Y + (int64_t)g * C, 4 * L.l, C, vec_out, y);
+    }
+    const unsigned long long bal = __ballot(ch);
+    if ((threadIdx.x & 63) == 0) changed[g] = bal != 0;
+  }
+};
+
+template <Word K>
+__global__ __launch_bounds__(256) void k_frontier_jacobi_rows(int nf, const int32_t* __restrict__ frontier, int C,
+                                                              const int32_t* __restrict__ indptr,
+                                                              const int32_t* __restrict__ indices,
+                                                              const float* __restrict__ w, const float* __restrict__ X,
+                                                              int xbytes, const float* __restrict__ rhs,
+                                                              const float* __restrict__ diag, float delta,
+                                                              float* __restrict__ Y, uint8_t* __restrict__ changed,
+                                                              int vec_out) {
+  RowPipe G(nf, C, 4, 16, 0, nullptr, indptr, nullptr, indices, w, X, (unsigned)(C * 4), xbytes);
+  G.rows = frontier;
+  JacobiEpilogue<K> E{G.L, C, G.L.e == 0 && 4 * G.L.l < C, X, rhs, diag, delta, Y, changed, vec_out != 0};
+  run_rows<4, K, float>(G, E);
 }
 
 // R[col] += w * D[g] and mark[col] = 1 over the rows of the frontier.
@@ -208,6 +264,30 @@ __global__ __launch_bounds__(256) void k_compact_frontier(
 using namespace dlp;
 
 
+// The packed form for FP32 (returns false when its limits do not hold).
+static bool frontier_jacobi_rows(int nf, const int32_t* frontier, int C, const int32_t* indptr,
+                                 const int32_t* indices, const float* w, const float* X, const float* rhs,
+                                 const float* diag, float delta, float* Y, uint8_t* changed, int n_hint,
+                                 hipStream_t st) {
+  const long long xbytes = (long long)n_hint * C * 4;
+  if (!tuning().pipe || n_hint <= 0 || n_hint >= (1 << 24) || xbytes >= (1LL << 31) || (((uintptr_t)X) & 3) != 0)
+    return false;
+  const bool v16 = (C % 4) == 0 && (((uintptr_t)X) & 15) == 0;
+  const int vec_out = (C % 4) == 0 && (((uintptr_t)Y) & 15) == 0;
+  const void* kfn = v16 ? reinterpret_cast<const void*>(&k_frontier_jacobi_rows<Word::F32x4>)
+                        : reinterpret_cast<const void*>(&k_frontier_jacobi_rows<Word::F32x4u>);
+  long long grid = resident_grid(kfn);
+  const long long need = ((long long)nf + 3) / 4;
+  if (grid > need) grid = need;
+  if (v16)
+    hipLaunchKernelGGL((k_frontier_jacobi_rows<Word::F32x4>), dim3((unsigned)grid), dim3(kBlock), 0, st, nf, frontier,
+                       C, indptr, indices, w, X, (int)xbytes, rhs, diag, delta, Y, changed, vec_out);
+  else
+    hipLaunchKernelGGL((k_frontier_jacobi_rows<Word::F32x4u>), dim3((unsigned)grid), dim3(kBlock), 0, st, nf,
+                       frontier, C, indptr, indices, w, X, (int)xbytes, rhs, diag, delta, Y, changed, vec_out);
+  return true;
+}
+
 template <typename T>
 static int frontier_jacobi_t(int nf, const int32_t* frontier, int C, const int32_t* indptr,
                              const int32_t* indices, const void* w, const void* X,
@@ -227,16 +307,23 @@ static int frontier_jacobi_t(int nf, const int32_t* frontier, int C, const int32
   return DLP_OK;
 }
 
-extern "C" int dlp_frontier_jacobi(int nf, const int32_t* frontier, int C, const int32_t* indptr,
+extern "C" int dlp_frontier_jacobi(int nf, const int32_t* frontier, int n, int C, const int32_t* indptr,
                                    const int32_t* indices, const void* w, const void* X,
                                    const void* rhs, const void* diag, double delta, void* Y,
                                    uint8_t* changed, unsigned long long* edges, int dtype,
                                    double avg_deg, void* stream) {
-  DLP_CHECK_ARG(nf >= 0 && C >= 1 && C <= 256, "frontier_jacobi: bad nf=%d C=%d", nf, C);
+  DLP_CHECK_ARG(nf >= 0 && n >= 0 && C >= 1 && C <= 256, "frontier_jacobi: bad nf=%d n=%d C=%d", nf, n, C);
   if (nf == 0) return DLP_OK;
   DLP_CHECK_ARG(frontier && indptr && indices && w && X && rhs && diag && Y && changed,
                 "frontier_jacobi: null pointer");
   hipStream_t st = (hipStream_t)stream;
+  if (dtype == DLP_F32 && frontier_jacobi_rows(nf, frontier, C, indptr, indices, (const float*)w, (const float*)X,
+                                               (const float*)rhs, (const float*)diag, (float)delta, (float*)Y,
+                                               changed, n, st)) {
+    count_edges(nf, frontier, indptr, edges, st);
+    DLP_CHECK_LAUNCH();
+    return DLP_OK;
+  }
   if (dtype == DLP_F32)
     return frontier_jacobi_t<float>(nf, frontier, C, indptr, indices, w, X, rhs, diag, delta, Y,
                                     changed, edges, avg_deg, st);

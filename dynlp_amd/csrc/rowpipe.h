@@ -47,8 +47,9 @@ __device__ __forceinline__ unsigned f16_pack(float lo, float hi) {
 }
 
 // Lane words of the gathered rows: FP32 x 4 (16 bytes), BF16 x 8 (16 bytes), BF16 x 4 or
-// FP16 x 4 (8 bytes).  W values per lane, column order.
-enum class Word { F32x4, BF16x8, BF16x4, F16x4 };
+// FP16 x 4 (8 bytes), and FP32 x 4 at 4-byte alignment (F32x4u: four dword loads, for
+// contiguous rows whose length is not a multiple of 4 columns).  W values per lane, column order.
+enum class Word { F32x4, BF16x8, BF16x4, F16x4, F32x4u };
 
 template <Word K> struct WordT;
 template <> struct WordT<Word::F32x4> {
@@ -67,11 +68,26 @@ template <> struct WordT<Word::F16x4> {
   static constexpr int W = 4, bytes = 8;
   using raw = unsigned int __attribute__((ext_vector_type(2)));
 };
+template <> struct WordT<Word::F32x4u> {
+  static constexpr int W = 4, bytes = 16;
+  using raw = u4v;
+};
 
 template <Word K>
 __device__ __forceinline__ typename WordT<K>::raw load_word(__amdgpu_buffer_rsrc_t rs, unsigned off) {
-  if constexpr (WordT<K>::bytes == 16) return __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0);
-  else return __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, 0);
+  if constexpr (K == Word::F32x4u) {
+    // an out-of-range offset (>= 2^31) stays out of range for all four dwords
+    u4v r;
+    r.x = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, 0);
+    r.y = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)(off + 4u), 0, 0);
+    r.z = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)(off + 8u), 0, 0);
+    r.w = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)(off + 12u), 0, 0);
+    return r;
+  } else if constexpr (WordT<K>::bytes == 16) {
+    return __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0);
+  } else {
+    return __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, 0);
+  }
 }
 template <Word K>
 __device__ __forceinline__ typename WordT<K>::raw load_word(const void* p) {
@@ -80,7 +96,7 @@ __device__ __forceinline__ typename WordT<K>::raw load_word(const void* p) {
 
 template <Word K>
 __device__ __forceinline__ void unpack(const typename WordT<K>::raw& u, float (&x)[WordT<K>::W]) {
-  if constexpr (K == Word::F32x4) {
+  if constexpr (K == Word::F32x4 || K == Word::F32x4u) {
     x[0] = __uint_as_float(u.x);
     x[1] = __uint_as_float(u.y);
     x[2] = __uint_as_float(u.z);
@@ -126,9 +142,12 @@ __device__ __forceinline__ void xcd_rows(int nrows, int xcd, int64_t& lo, int64_
   }
 }
 
+// The rows of a pass are ordinals g in [0, nrows): row id rows[g] (rows null: g).  rowmask
+// null: every row live; rowlen null: plain CSR (row length indptr[r+1] - indptr[r]).
 struct RowPipe {
   LanesP L;
   int64_t lo, hi, wx, nwx;
+  const int32_t* __restrict__ rows = nullptr;
   const uint8_t* __restrict__ rowmask;
   const int32_t* __restrict__ indptr;
   const int32_t* __restrict__ rowlen;
@@ -137,8 +156,8 @@ struct RowPipe {
   __amdgpu_buffer_rsrc_t rx;
   unsigned rowb, laneq;
 
-  // C columns of W per lane word of word_bytes; X rows of row_bytes bytes (a multiple of the
-  // word), the block xbytes long
+  // C columns of W per lane word of word_bytes; X rows of row_bytes bytes (4-byte multiples;
+  // 16-byte and 8-byte words need rows that are multiples of the word), the block xbytes long
   __device__ __forceinline__ RowPipe(int nrows, int C, int W, int word_bytes, int xcd, const uint8_t* rm,
                                      const int32_t* ip, const int32_t* rl, const int32_t* ix, const float* wv,
                                      const void* X, unsigned row_bytes, int xbytes)
@@ -149,6 +168,34 @@ struct RowPipe {
     laneq = (unsigned)word_bytes * (unsigned)L.l;
   }
 };
+
+// x[q] = p[c0 + q] for c0 + q < C (0 beyond): an own-row read of 4 columns at 4-byte alignment.
+__device__ __forceinline__ void load4(const float* __restrict__ p, int c0, int C, float (&x)[4]) {
+  if (c0 + 3 < C) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) x[q] = p[c0 + q];
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) x[q] = c0 + q < C ? p[c0 + q] : 0.f;
+  }
+}
+
+// p[c0 + q] = y[q] for c0 + q < C; one 16-byte store when vec (p + c0 16-byte aligned) and the
+// four columns exist.
+__device__ __forceinline__ void store4(float* __restrict__ p, int c0, int C, bool vec, const float (&y)[4]) {
+  if (c0 + 3 < C) {
+    if (vec) {
+      *reinterpret_cast<float4*>(p + c0) = make_float4(y[0], y[1], y[2], y[3]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) p[c0 + q] = y[q];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (c0 + q < C) p[c0 + q] = y[q];
+  }
+}
 
 template <Word K, typename TA>
 __device__ __forceinline__ void fmaw(TA (&acc)[WordT<K>::W], float wv, const typename WordT<K>::raw& u) {
@@ -162,25 +209,29 @@ __device__ __forceinline__ void fmaw(TA (&acc)[WordT<K>::W], float wv, const typ
   }
 }
 
-// Runs every row of the pass: epi.pre(r) before row r's gathers are issued (own-row loads),
-// epi.post(r, live, acc) with live = rowmask[r] and acc = sum over the row's entries of
-// w_j * X[col_j, W*l .. W*l+W-1] (zero when !live; valid in edge lane 0), called by every lane
-// of the wavefront.
+// Runs every row of the pass: epi.pre(g, r) before row r = rows[g]'s gathers are issued
+// (own-row loads), epi.post(g, r, live, acc) with live = rowmask[r] and acc = sum over the row's
+// entries of w_j * X[col_j, W*l .. W*l+W-1] (zero when !live; valid in edge lane 0), called by
+// every lane of the wavefront.
 template <int S, Word K, typename TA, typename E>
 __device__ __forceinline__ void run_rows(const RowPipe& G, E& epi) {
   constexpr int W = WordT<K>::W;
   using raw = typename WordT<K>::raw;
   const LanesP& L = G.L;
   constexpr unsigned kOOB = 0x80000000u;
-  auto range = [&](int64_t ii, int& beg, int& end, bool& live) {
+  auto range = [&](int64_t ii, int& row, int& beg, int& end, bool& live) {
     beg = end = 0;
     live = false;
+    row = 0;
     if (ii < G.hi) {
-      const int ri = __builtin_amdgcn_readfirstlane((int)ii);
-      live = G.rowmask[ri] != 0;
+      const int gi = __builtin_amdgcn_readfirstlane((int)ii);
+      const int ri = G.rows ? __builtin_amdgcn_readfirstlane(G.rows[gi]) : gi;
+      row = ri;
+      live = G.rowmask ? G.rowmask[ri] != 0 : true;
       if (live) {
         beg = __builtin_amdgcn_readfirstlane(G.indptr[ri]);
-        end = beg + __builtin_amdgcn_readfirstlane(G.rowlen[ri]);
+        end = G.rowlen ? beg + __builtin_amdgcn_readfirstlane(G.rowlen[ri])
+                       : __builtin_amdgcn_readfirstlane(G.indptr[ri + 1]);
       }
     }
   };
@@ -203,9 +254,9 @@ __device__ __forceinline__ void run_rows(const RowPipe& G, E& epi) {
     }
   };
   int64_t i = G.lo + G.wx;
-  int beg0, end0;
+  int row0, beg0, end0;
   bool live0;
-  range(i, beg0, end0, live0);
+  range(i, row0, beg0, end0, live0);
   int v0 = 0;
   float w0 = 0.f;
   if (beg0 + L.lane < end0) {
@@ -213,11 +264,12 @@ __device__ __forceinline__ void run_rows(const RowPipe& G, E& epi) {
     w0 = G.w[beg0 + L.lane];
   }
   for (; i < G.hi; i += G.nwx) {
-    int beg1, end1;
+    int row1, beg1, end1;
     bool live1;
-    range(i + G.nwx, beg1, end1, live1);
-    const int r = __builtin_amdgcn_readfirstlane((int)i);
-    epi.pre(r);
+    range(i + G.nwx, row1, beg1, end1, live1);
+    const int g = __builtin_amdgcn_readfirstlane((int)i);
+    const int r = row0;
+    epi.pre(g, r);
     const int cnt = min(64, end0 - beg0);
     const int cnt_l = L.on ? cnt : 0;
     raw xv[S];
@@ -256,9 +308,10 @@ __device__ __forceinline__ void run_rows(const RowPipe& G, E& epi) {
 #pragma unroll
       for (int q = 0; q < W; ++q) tot[q] += __shfl(acc[q], src, 64);
     }
-    epi.post(r, live0, tot);
+    epi.post(g, r, live0, tot);
     v0 = v1;
     w0 = w1;
+    row0 = row1;
     beg0 = beg1;
     end0 = end1;
     live0 = live1;

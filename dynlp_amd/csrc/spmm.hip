@@ -231,7 +231,7 @@ struct PcgEpilogue {
   double* sm;  // workgroup column sums of P .* AP (LDS)
   float xr[W];
   float ar;
-  __device__ __forceinline__ void pre(int r) {
+  __device__ __forceinline__ void pre(int, int r) {
     if (owner) {
       unpack<K>(load_word<K>(X + (int64_t)r * row_bytes + WordT<K>::bytes * L.l), xr);
     } else {
@@ -243,7 +243,7 @@ struct PcgEpilogue {
   // out[r, W*l ..] = b*acc + ar*P (columns < C), sm += P .* out (FP64 LDS atomics: no
   // per-lane FP64 accumulators, which would cost a wave of occupancy); rows outside the mask
   // get AP = diag*P (no neighbour sum), as the plain operator writes them
-  __device__ __forceinline__ void post(int r, bool, const float (&acc)[W]) {
+  __device__ __forceinline__ void post(int, int r, bool, const float (&acc)[W]) {
     if (!owner) return;
     const int c0 = W * L.l;
     float y[W];
@@ -572,10 +572,92 @@ extern "C" int dlp_row_operator_dot(int n, int C, const int32_t* indptr, const i
                           (hipStream_t)stream, nullptr);
 }
 
+// Y[r] = a[r] * X[r] + b * sum_j w_j X[col_j] (a null: no first term) over every row of a plain
+// CSR, FP32, on the pipelined packed-lane gathers; X rows of ldx floats at 4-byte alignment
+// (16-byte words when ldx % 4 == 0 and X is 16-byte aligned, else four dword loads per lane).
+template <Word K>
+struct AxpyEpilogue {
+  const LanesP& L;
+  int C;
+  bool owner;
+  const float* __restrict__ X;
+  int64_t ldx;
+  const float* __restrict__ a;
+  float b;
+  float* __restrict__ out;
+  int64_t ldo;
+  bool vec_out;
+  float xr[4];
+  float ar;
+  __device__ __forceinline__ void pre(int, int r) {
+    ar = 0.f;
+    if (a && owner) {
+      load4(X + (int64_t)r * ldx, 4 * L.l, C, xr);
+      ar = a[r];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) xr[q] = 0.f;
+    }
+  }
+  __device__ __forceinline__ void post(int, int r, bool, const float (&acc)[4]) {
+    if (!owner) return;
+    float y[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) y[q] = b * acc[q] + ar * xr[q];
+    store4(out + (int64_t)r * ldo, 4 * L.l, C, vec_out, y);
+  }
+};
+
+template <int S, Word K>
+__global__ __launch_bounds__(256) void k_spmm_rows(int nrows, int C, const int32_t* __restrict__ indptr,
+                                                   const int32_t* __restrict__ indices, const float* __restrict__ w,
+                                                   const float* __restrict__ X, int64_t ldx, int xbytes,
+                                                   const float* __restrict__ a, float b, float* __restrict__ out,
+                                                   int64_t ldo, int vec_out, int xcd) {
+  const RowPipe G(nrows, C, 4, 16, xcd, nullptr, indptr, nullptr, indices, w, X, (unsigned)(ldx * 4), xbytes);
+  AxpyEpilogue<K> E{G.L, C, G.L.e == 0 && 4 * G.L.l < C, X, ldx, a, b, out, ldo, vec_out != 0};
+  run_rows<S, K, float>(G, E);
+}
+
+// 0 when the packed form does not apply (the caller falls back to the generic row kernel).
+static int spmm_rows_f32(int n, int C, const int32_t* indptr, const int32_t* indices, const float* w,
+                         const float* X, int64_t ldx, const float* a, float b, float* Y, int64_t ldy,
+                         hipStream_t st, bool& done) {
+  done = false;
+  const long long xbytes = (long long)n * ldx * 4;
+  if (!tuning().pipe || C > 256 || ldx < C || n >= (1 << 24) || xbytes >= (1LL << 31) || ldx * 4 >= (1 << 24) ||
+      (((uintptr_t)X) & 3) != 0 || (((uintptr_t)Y) & 3) != 0)
+    return DLP_OK;
+  const bool v16 = (ldx % 4) == 0 && (((uintptr_t)X) & 15) == 0;
+  const int vec_out = (ldy % 4) == 0 && (((uintptr_t)Y) & 15) == 0;
+  const void* kfn = v16 ? reinterpret_cast<const void*>(&k_spmm_rows<4, Word::F32x4>)
+                        : reinterpret_cast<const void*>(&k_spmm_rows<4, Word::F32x4u>);
+  long long grid = resident_grid(kfn);
+  const long long need = ((long long)n + 3) / 4;
+  if (grid > need) grid = need;
+  const int xcd = tuning().xcd != 0;
+  if (xcd) grid = grid < 8 ? 8 : grid & ~7LL;
+  if (v16)
+    hipLaunchKernelGGL((k_spmm_rows<4, Word::F32x4>), dim3((unsigned)grid), dim3(kBlock), 0, st, n, C, indptr,
+                       indices, w, X, ldx, (int)xbytes, a, b, Y, ldy, vec_out, xcd);
+  else
+    hipLaunchKernelGGL((k_spmm_rows<4, Word::F32x4u>), dim3((unsigned)grid), dim3(kBlock), 0, st, n, C, indptr,
+                       indices, w, X, ldx, (int)xbytes, a, b, Y, ldy, vec_out, xcd);
+  DLP_CHECK_LAUNCH();
+  done = true;
+  return DLP_OK;
+}
+
 extern "C" int dlp_spmm_axpy(int n, int C, const int32_t* indptr, const int32_t* indices,
                              const void* w, const void* X, int64_t ldx, const void* a, double b,
                              void* Y, int64_t ldy, int dtype, double avg_deg, void* stream) {
   DLP_CHECK_ARG(dtype == DLP_F32 || dtype == DLP_F64, "spmm_axpy: bad dtype %d", dtype);
+  if (dtype == DLP_F32 && n > 0 && C >= 1 && indptr && indices && w && X && Y && ldx >= C && ldy >= C) {
+    bool done = false;
+    const int rc = spmm_rows_f32(n, C, indptr, indices, (const float*)w, (const float*)X, ldx, (const float*)a,
+                                 (float)b, (float*)Y, ldy, (hipStream_t)stream, done);
+    if (rc || done) return rc;
+  }
   return dlp_row_operator(n, nullptr, C, indptr, nullptr, indices, w, dtype, nullptr, X, ldx,
                           dtype, a, b, nullptr, 0, Y, ldy, dtype, 0, avg_deg, stream);
 }

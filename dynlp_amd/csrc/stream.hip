@@ -652,11 +652,11 @@ struct VerifyEpilogue {
   double rmax[4] = {0.0, 0.0, 0.0, 0.0};
   float4 zr;
   __device__ __forceinline__ bool owner() const { return L.e == 0 && 4 * L.l < A.C1; }
-  __device__ __forceinline__ void pre(int r) {
+  __device__ __forceinline__ void pre(int, int r) {
     zr = owner() ? *(reinterpret_cast<const float4*>(Z + (int64_t)r * A.ld) + L.l) : make_float4(0.f, 0.f, 0.f, 0.f);
   }
   // rows outside the solved set keep their parked state (R = 0)
-  __device__ __forceinline__ void post(int r, bool live, const double (&acc)[4]) {
+  __device__ __forceinline__ void post(int, int r, bool live, const double (&acc)[4]) {
     if (!live) return;  // wave-uniform
     bool viol = false;
     if (owner()) {

@@ -120,7 +120,7 @@ class FrontierOps:
         Y = torch.empty((nf, self.C), dtype=X.dtype, device=X.device)
         ch = torch.empty(nf, dtype=torch.uint8, device=X.device)
         W = self.W
-        _lib.call("dlp_frontier_jacobi", nf, ptr(f), self.C, ptr(W.indptr), ptr(W.indices), ptr(W.data),
+        _lib.call("dlp_frontier_jacobi", nf, ptr(f), int(X.shape[0]), self.C, ptr(W.indptr), ptr(W.indices), ptr(W.data),
                   ptr(X.contiguous()), ptr(rhs.contiguous()), ptr(diag), float(delta), ptr(Y), ptr(ch),
                   ptr(self.edges), dtype_code(X.dtype), self.deg, stream())
         return Y, ch.bool()

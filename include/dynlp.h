@@ -135,10 +135,10 @@ int dlp_amg_prolong_add(int n, int C, const int32_t* agg, const void* Xc, int64_
 
 /* ---- Frontier primitives (FrontierOps, kernels.py:436-492) ----------------------------- */
 /* Y[g] = (rhs[f_g] + sum_j w_j X[col_j]) / diag[f_g], changed[g] = any_c |Y[g,c]-X[f_g,c]| > delta
- * for g < nf (f = frontier); adds the frontier's edge count to *edges (device uint64, may be
- * NULL).  dtype for w, X, rhs, diag, Y.  Replaces FrontierOps.jacobi (kernels.py:464-474;
- * CUDA kernels.py:31-102, 186-220). */
-int dlp_frontier_jacobi(int nf, const int32_t* frontier, int C, const int32_t* indptr,
+ * for g < nf (f = frontier); X and rhs are n x C, Y is nf x C (contiguous); adds the frontier's
+ * edge count to *edges (device uint64, may be NULL).  dtype for w, X, rhs, diag, Y.  Replaces
+ * FrontierOps.jacobi (kernels.py:464-474; CUDA kernels.py:31-102, 186-220). */
+int dlp_frontier_jacobi(int nf, const int32_t* frontier, int n, int C, const int32_t* indptr,
                         const int32_t* indices, const void* w, const void* X, const void* rhs,
                         const void* diag, double delta, void* Y, uint8_t* changed,
                         unsigned long long* edges, int dtype, double avg_deg, void* stream);
