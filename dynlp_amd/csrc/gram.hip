@@ -77,6 +77,129 @@ __global__ __launch_bounds__(256) void k_gram_mfma(int n, int C, const T* __rest
   }
 }
 
+// Register-tiled Gram for C = 16*NT (+1 with EXTRA): every wavefront keeps all NT x NT output
+// tiles in accumulator registers and streams its own 16-row chunks of X and Y straight from
+// memory (4 k-steps of 4 rows per iteration, all loads issued before the multiplies); an
+// operand fragment is read once and feeds NT MFMAs.  The extra column (h of the label block)
+// goes to VALU FP64 FMAs instead of a fifth, mostly padded tile row.  SYM: Y is X (the B
+// fragments are the A fragments scaled by 1/rowdiv).  Partials are summed in LDS per workgroup,
+// then added to G with one FP64 atomic per element per workgroup.
+// Operand fragments of D k-steps (4 rows each) starting at k-step k0: lane (li, lk) holds
+// column 16t + li of row 4*k + lk (zero beyond n).
+template <typename T, int NT, bool EXTRA, bool SYM, int D>
+struct GramFrag {
+  T xa[D][NT], yb[SYM ? 1 : D][NT], xe[D], ye[D], dv[D];
+  __device__ __forceinline__ void load(int64_t k0, int n, int li, int lk, const T* __restrict__ X, int64_t ldx,
+                                       const T* __restrict__ Y, int64_t ldy, const T* __restrict__ rowdiv) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int64_t r = ((k0 + d) << 2) + lk;
+      const bool ok = r < n;
+      const T* xr = X + (ok ? r : 0) * ldx;
+      const T* yr = Y + (ok ? r : 0) * ldy;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        xa[d][t] = ok ? xr[16 * t + li] : (T)0;
+        if constexpr (!SYM) yb[d][t] = ok ? yr[16 * t + li] : (T)0;
+      }
+      if constexpr (EXTRA) {
+        xe[d] = ok ? xr[16 * NT] : (T)0;
+        ye[d] = ok ? yr[16 * NT] : (T)0;
+      }
+      dv[d] = (rowdiv && ok) ? rowdiv[r] : (T)1;
+    }
+  }
+};
+
+// Register-tiled Gram for C = 16*NT (+1 with EXTRA): every wavefront keeps all NT x NT output
+// tiles in accumulator registers and streams its own 16-row chunks of X and Y straight from
+// memory; an operand fragment is read once and feeds NT MFMAs, and the fragments of the next
+// chunk are loaded while the current one is multiplied.  The extra column (h of the label
+// block) goes to VALU FP64 FMAs instead of a fifth, mostly padded tile row.  SYM: Y is X (the B
+// fragments are the A fragments scaled by 1/rowdiv).  Partials are summed in LDS per workgroup,
+// then added to G with one FP64 atomic per element per workgroup.
+template <typename T, int NT, bool EXTRA, bool SYM>
+__global__ __launch_bounds__(sizeof(T) == 4 ? 512 : 256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1))) void k_gram_reg(int n, int C, const T* __restrict__ X, int64_t ldx,
+                                                  const T* __restrict__ Y, int64_t ldy, const T* __restrict__ rowdiv,
+                                                  double* __restrict__ G) {
+  constexpr int CT = 16 * NT;
+  constexpr int C1 = CT + (EXTRA ? 1 : 0);
+  // k-steps per chunk: 4, or 2 when the two operands' double-buffered fragments would cost a
+  // wave of occupancy
+  constexpr int D = (SYM || NT < 4) ? 4 : 2;
+  __shared__ double red[C1 * C1];
+  for (int i = threadIdx.x; i < C1 * C1; i += blockDim.x) red[i] = 0.0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  d4_t acc[NT][NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int u = 0; u < NT; ++u) acc[t][u] = (d4_t){0.0, 0.0, 0.0, 0.0};
+  double er[NT], ec[NT], ed = 0.0;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) er[t] = ec[t] = 0.0;
+  const int64_t nks = ((int64_t)n + 3) >> 2;
+  const int64_t nwb = blockDim.x >> 6;
+  const int64_t wid = (int64_t)blockIdx.x * nwb + wave, nw = (int64_t)gridDim.x * nwb;
+  GramFrag<T, NT, EXTRA, SYM, D> cur, nxt;
+  int64_t k0 = wid * D;
+  if (k0 < nks) cur.load(k0, n, li, lk, X, ldx, Y, ldy, rowdiv);
+  for (; k0 < nks; k0 += nw * D) {
+    if (k0 + nw * D < nks) nxt.load(k0 + nw * D, n, li, lk, X, ldx, Y, ldy, rowdiv);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double inv = 1.0 / (double)cur.dv[d];
+      double a[NT], b[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        a[t] = (double)cur.xa[d][t];
+        if constexpr (SYM) b[t] = a[t] * inv;
+        else b[t] = (double)cur.yb[d][t] * inv;
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int u = 0; u < NT; ++u) acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[u], acc[t][u], 0, 0, 0);
+      if constexpr (EXTRA) {
+        const double xd = (double)cur.xe[d], yd = (double)cur.ye[d] * inv;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          er[t] += xd * b[t];   // G[CT][16t + li]
+          ec[t] += a[t] * yd;   // G[16t + li][CT]
+        }
+        if (li == 0) ed += xd * yd;
+      }
+    }
+    cur = nxt;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int u = 0; u < NT; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) atomicAdd(&red[(16 * t + lk + 4 * v) * C1 + 16 * u + li], acc[t][u][v]);
+  if constexpr (EXTRA) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      double x = er[t], y = ec[t];
+      x += __shfl_xor(x, 16, 64);
+      x += __shfl_xor(x, 32, 64);
+      y += __shfl_xor(y, 16, 64);
+      y += __shfl_xor(y, 32, 64);
+      if (lk == 0) {
+        atomicAdd(&red[CT * C1 + 16 * t + li], x);
+        atomicAdd(&red[(16 * t + li) * C1 + CT], y);
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) ed += __shfl_xor(ed, off, 64);
+    if (lane == 0) atomicAdd(&red[CT * C1 + CT], ed);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < C1 * C1; i += blockDim.x) atomicAdd(&G[i], red[i]);
+}
+
 // S = A^{-1} B for SPD A (k x k, FP64, row-major), B k x k; one workgroup.  The system is
 // symmetrically scaled to unit diagonal; a pivot below tol (relative) marks a direction that
 // has collapsed (its column of the block has converged or become dependent): that row and
@@ -267,6 +390,48 @@ extern "C" int dlp_gram(int n, int C, const void* X, int64_t ldx, const void* Y,
   DLP_CHECK_ARG(n == 0 || (X && Y), "gram: null pointer");
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_zero_f64b, dim3(8), dim3(kBlock), 0, st, C * C, G);
+  const int nt = C / 16, rem = C % 16;
+  if (n > 0 && nt >= 1 && nt <= 4 && rem <= 1) {
+    // register-tiled form: C = 16, 17, 32, 33, 48, 49, 64, 65
+    const bool sym = X == Y && ldx == ldy;
+    // FP32 operands: workgroups of 8 wavefronts (the two per SIMD the kernel keeps resident),
+    // one per CU, so the per-workgroup partials cost one FP64 atomic per element of G per CU;
+    // FP64 operands (more registers per fragment): 4 wavefronts, two workgroups per CU
+    int cus = 256, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+      cus = 256;
+    const int wg = dtype == DLP_F32 ? 512 : 256;
+    const long long need = ((long long)n + 15) / 16 / (wg / 64);
+    long long grid = 2LL * cus * 256 / wg;
+    if (grid > need) grid = need < 1 ? 1 : need;
+#define DLP_GRAM(TV, NTV, EV, SV)                                                                        \
+  hipLaunchKernelGGL((k_gram_reg<TV, NTV, EV, SV>), dim3((unsigned)grid), dim3(wg), 0, st, n, C,         \
+                     (const TV*)X, ldx, (const TV*)Y, ldy, (const TV*)rowdiv, G)
+#define DLP_GRAM_NT(TV, EV, SV)            \
+  switch (nt) {                            \
+    case 1: DLP_GRAM(TV, 1, EV, SV); break; \
+    case 2: DLP_GRAM(TV, 2, EV, SV); break; \
+    case 3: DLP_GRAM(TV, 3, EV, SV); break; \
+    default: DLP_GRAM(TV, 4, EV, SV); break; \
+  }
+#define DLP_GRAM_T(TV)                                   \
+  if (rem) {                                             \
+    if (sym) DLP_GRAM_NT(TV, true, true) else DLP_GRAM_NT(TV, true, false) \
+  } else {                                               \
+    if (sym) DLP_GRAM_NT(TV, false, true) else DLP_GRAM_NT(TV, false, false) \
+  }
+    if (dtype == DLP_F32) {
+      DLP_GRAM_T(float)
+    } else {
+      DLP_GRAM_T(double)
+    }
+#undef DLP_GRAM_T
+#undef DLP_GRAM_NT
+#undef DLP_GRAM
+    DLP_CHECK_LAUNCH();
+    return DLP_OK;
+  }
   if (n > 0) {
     unsigned grid = (unsigned)((n + kGramRows - 1) / kGramRows);
     if (grid > 1024) grid = 1024;

@@ -231,10 +231,12 @@ def test_cc_csr_masked_matches_oracle():
     assert nc == ncr and np.array_equal(lab.cpu().numpy(), ref)
 
 
-@pytest.mark.parametrize("C", [1, 16, 17, 65, 80])
+@pytest.mark.parametrize("C", [1, 16, 17, 32, 33, 49, 64, 65, 80])
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
 def test_gram_on_matrix_cores_matches_numpy(C, dtype):
-    """X^T diag(1/d) Y on v_mfma_f64_16x16x4f64 (also pins the operand/result lane layout)."""
+    """X^T diag(1/d) Y on v_mfma_f64_16x16x4f64 (also pins the operand/result lane layout): the
+    register-tiled form (C = 16t or 16t + 1, t <= 4; general and X = Y operands) and the staged
+    form (other C), on a row count that is not a multiple of the 16-row chunks."""
     import torch
     from dynlp_amd import _lib
     from dynlp_amd.backend import ptr, stream
@@ -251,6 +253,9 @@ def test_gram_on_matrix_cores_matches_numpy(C, dtype):
     np.testing.assert_allclose(G.cpu().numpy(), ref, rtol=1e-11, atol=1e-11)
     _lib.call("dlp_gram", n, C, ptr(Xd), C, ptr(Yd), C, None, ptr(G), dt, stream())
     np.testing.assert_allclose(G.cpu().numpy(), X.astype(np.float64).T @ Y.astype(np.float64), rtol=1e-11, atol=1e-11)
+    _lib.call("dlp_gram", n, C, ptr(Xd), C, ptr(Xd), C, ptr(dd), ptr(G), dt, stream())
+    ref = X.astype(np.float64).T @ (X.astype(np.float64) / d.astype(np.float64)[:, None])
+    np.testing.assert_allclose(G.cpu().numpy(), ref, rtol=1e-11, atol=1e-11)
 
 
 def test_cholesky_solve_and_deflation():
