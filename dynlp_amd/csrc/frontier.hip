@@ -96,6 +96,57 @@ __global__ __launch_bounds__(256) void k_frontier_jacobi_rows(int nf, const int3
   run_rows<4, K, float>(G, E);
 }
 
+// FP32 push scatter, one frontier row per wavefront step: 2^lc_shift lanes cover consecutive
+// columns of one edge (CPL chunks of them), 64 >> lc_shift edges per instruction, so every
+// atomic instruction adds to consecutive dwords of a row of R.  A row's (column, weight) pairs
+// are read 64 at a time, one per lane, and broadcast from registers, so the atomics of
+// consecutive edges do not wait on index loads.  Each lane marks the column of its own pair.
+template <int CPL>
+__global__ __launch_bounds__(256) void k_push_scatter_rows(int nf, const int32_t* __restrict__ frontier, int C,
+                                                           int lc_shift, const int32_t* __restrict__ indptr,
+                                                           const int32_t* __restrict__ indices,
+                                                           const float* __restrict__ w, const float* __restrict__ D,
+                                                           float* __restrict__ R, uint8_t* __restrict__ mark) {
+  const int LC = 1 << lc_shift, LE = 64 >> lc_shift;
+  const int lane = threadIdx.x & 63, l = lane & (LC - 1), e = lane >> lc_shift;
+  const int64_t nwx = (int64_t)gridDim.x * 4;
+  for (int64_t gg = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); gg < nf; gg += nwx) {
+    const int g = __builtin_amdgcn_readfirstlane((int)gg);
+    const int r = __builtin_amdgcn_readfirstlane(frontier[g]);
+    const int beg = __builtin_amdgcn_readfirstlane(indptr[r]);
+    const int end = __builtin_amdgcn_readfirstlane(indptr[r + 1]);
+    float d[CPL];
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) {
+      const int c = l + k * LC;
+      d[k] = c < C ? D[(int64_t)g * C + c] : 0.f;
+    }
+    for (int base = beg; base < end; base += 64) {
+      const int cnt = min(64, end - base);
+      int v = 0;
+      float wv = 0.f;
+      if (lane < cnt) {
+        v = indices[base + lane];
+        wv = w[base + lane];
+        mark[v] = 1;
+      }
+      for (int t0 = 0; t0 < cnt; t0 += LE) {
+        const int t = t0 + e;
+        const int vv = __shfl(v, t & 63, 64);
+        const float ww = __shfl(wv, t & 63, 64);
+        if (t < cnt) {
+          float* rp = R + (int64_t)vv * C;
+#pragma unroll
+          for (int k = 0; k < CPL; ++k) {
+            const int c = l + k * LC;
+            if (c < C) atomicAdd(rp + c, ww * d[k]);
+          }
+        }
+      }
+    }
+  }
+}
+
 // R[col] += w * D[g] and mark[col] = 1 over the rows of the frontier.
 template <typename T, int CPL>
 __global__ __launch_bounds__(256) void k_push_scatter(
@@ -339,6 +390,22 @@ static int push_scatter_t(int nf, const int32_t* frontier, int C, const int32_t*
                           const int32_t* indices, const void* w, const void* D, void* R,
                           uint8_t* mark, unsigned long long* edges, double avg_deg,
                           hipStream_t st) {
+  if (std::is_same<T, float>::value && tuning().pipe) {
+    int lc = C <= 1 ? 0 : ceil_log2(C);
+    if (lc > 6) lc = 6;
+    const int cpl = (C + (1 << lc) - 1) >> lc;   // <= 4 for C <= 256
+    long long grid = resident_grid(reinterpret_cast<const void*>(&k_push_scatter_rows<1>));
+    const long long need = ((long long)nf + 3) / 4;
+    if (grid > need) grid = need;
+#define K(CPLV)                                                                                             \
+  hipLaunchKernelGGL((k_push_scatter_rows<CPLV>), dim3((unsigned)grid), dim3(kBlock), 0, st, nf, frontier, C, lc, \
+                     indptr, indices, (const float*)w, (const float*)D, (float*)R, mark)
+    DLP_DISPATCH_CPL(cpl, K)
+#undef K
+    count_edges(nf, frontier, indptr, edges, st);
+    DLP_CHECK_LAUNCH();
+    return DLP_OK;
+  }
   RowMap m = choose_map(C, avg_deg);
   dim3 grid(grid_for((long long)nf << m.g_shift)), block(kBlock);
 #define K(CPLV)                                                                              \
