@@ -29,7 +29,27 @@ struct PcgBufs {
   double* rp;
   int p16;
   float* sig;
+  // two-level preconditioner M = D^-1 + P Ac^-1 P^T (na = 0: Jacobi only): aggregates of S
+  // consecutive rows, P[r, r/S] = 1 on solved rows; rc = P^T r (na x ldc doubles), ec = the
+  // coarse correction Ac^-1 rc (na x ldc floats), Ainv = Ac^-1 (na x na), solved = row mask
+  int na, S;
+  int64_t ldc;
+  double* rc;
+  float* ec;
+  const double* Ainv;
+  const uint8_t* solved;
 };
+
+// Rows per workgroup of the flat PCG passes for this layout (0 when the flat form does not
+// apply): the coarse aggregates must be at least this long.
+int pcg_flat_chunk(const PcgBufs& b);
+
+// Ac = P^T A P of the solved rows (aggregates of S consecutive rows) into Ac (na x na), then
+// Ainv = Ac^-1 (FP64 Cholesky in one workgroup, na <= kCoarseMax).
+constexpr int kCoarseMax = 128;
+int coarse_build(int n, int na, int S, const int32_t* indptr, const int32_t* rowlen, const int32_t* col,
+                 const float* w, const uint8_t* solved, const double* diag, double* Ac, double* Ainv,
+                 hipStream_t st);
 
 // P = R/diag, rz = sum R*R/diag, pAp = 0.
 int pcg_start(const PcgBufs& b, hipStream_t st);

@@ -198,12 +198,86 @@ __device__ __forceinline__ void flush_quads(const Quad& Qd, int Q4, int C, const
   for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&out[c], sm[c]);
 }
 
-// pcg start: P = R/diag, rz[c] += sum R*R/diag; with a 16-bit block also rp[c] += sum R*P as
-// held.  FP16 blocks hold sig[c] * R/diag (sig from the rz of a prior k_pcg_rz4 pass).
+// Two-level preconditioner arguments of the flat passes (na = 0: Jacobi only).
+struct CoarseArg {
+  int na, S;
+  int64_t ldc;
+  double* rc;               // P^T r, na x ldc (restricting passes)
+  const float* ec;          // Ac^-1 P^T r, na x ldc (prolonging passes)
+  const uint8_t* solved;
+};
+
+// Rows of this workgroup in a contiguous split of n rows over the grid, [r_lo, r_hi); a thread
+// keeps column quad q4 and walks rows r_lo + roff, stepping rstep (consecutive threads still
+// read consecutive 16-byte words).
+struct QuadRange {
+  bool active;
+  int q4, cnt, roff, rstep;
+  int64_t r_lo, r_hi;
+};
+
+__device__ __forceinline__ QuadRange quad_range(int Q4, int C, int64_t n) {
+  QuadRange q;
+  q.rstep = (int)blockDim.x / Q4;
+  q.active = (int)threadIdx.x < Q4 * q.rstep;
+  q.q4 = threadIdx.x % Q4;
+  q.roff = threadIdx.x / Q4;
+  q.cnt = q.active ? max(0, min(4, C - 4 * q.q4)) : 0;
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  q.r_lo = min(n, (int64_t)blockIdx.x * chunk);
+  q.r_hi = min(n, q.r_lo + chunk);
+  return q;
+}
+
+// Column sums of a workgroup's rows per aggregate (r / S): the caller keeps the workgroup's
+// rows within two aggregates (rows per workgroup <= S), so a thread holds two quads of sums.
+struct Restrict {
+  int64_t A0, bnd;
+  double a0[4], a1[4];
+  __device__ __forceinline__ void init(int64_t r_lo, int S) {
+    A0 = r_lo / S;
+    bnd = (A0 + 1) * (int64_t)S;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) a0[j] = a1[j] = 0.0;
+  }
+  __device__ __forceinline__ void add(int64_t r, int j, double v) {
+    if (r < bnd) a0[j] += v;
+    else a1[j] += v;
+  }
+  // LDS part (before the workgroup barrier): sm2 holds 2 x 4*Q4 doubles, zeroed beforehand
+  __device__ __forceinline__ void to_lds(const QuadRange& q, int Q4, double* sm2) const {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (j < q.cnt) {
+        lds_add(&sm2[4 * q.q4 + j], a0[j]);
+        lds_add(&sm2[4 * Q4 + 4 * q.q4 + j], a1[j]);
+      }
+    }
+  }
+  // global part (after the barrier)
+  __device__ __forceinline__ void to_global(int Q4, int C, const CoarseArg& K, const double* sm2) const {
+    for (int c = threadIdx.x; c < 8 * Q4; c += blockDim.x) {
+      const int h = c / (4 * Q4), cc = c - h * 4 * Q4;
+      const int64_t A = A0 + h;
+      if (cc < C && A < K.na && sm2[c] != 0.0) atomicAdd(&K.rc[A * K.ldc + cc], sm2[c]);
+    }
+  }
+};
+
+// The coarse correction of row r (column quad q4), zero outside the solved set.
+__device__ __forceinline__ float4 coarse_quad(const CoarseArg& K, int64_t r, int q4) {
+  if (K.na == 0 || !K.solved[r]) return make_float4(0.f, 0.f, 0.f, 0.f);
+  return reinterpret_cast<const float4*>(K.ec + (r / K.S) * K.ldc)[q4];
+}
+
+// pcg start: P = M R (M R = R/diag, plus the coarse correction on solved rows), rz[c] += sum
+// R*(M R); with a 16-bit block also rp[c] += sum R*P as held.  FP16 blocks hold sig[c] * M R
+// (sig from the rz of a prior k_pcg_rz4 pass).
 template <int PK>
 __global__ __launch_bounds__(256) void k_pcg_start4(int n, int Q4, int C, PBlock<PK> P, const float4* __restrict__ R,
                                                     const float* __restrict__ diag, double* __restrict__ rz,
-                                                    double* __restrict__ rp, const float* __restrict__ sig) {
+                                                    double* __restrict__ rp, const float* __restrict__ sig,
+                                                    CoarseArg K) {
   __shared__ double sm[kMaxCols + 4];
   __shared__ double sm2[kMaxCols + 4];
   for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) sm[c] = sm2[c] = 0.0;
@@ -221,11 +295,12 @@ __global__ __launch_bounds__(256) void k_pcg_start4(int n, int Q4, int C, PBlock
     for (int64_t i = Qd.i0; i < nq; i += Qd.step, r += Qd.rstep) {
       const float inv = 1.f / diag[r];
       const float4 rv = R[i];
+      const float4 ev = coarse_quad(K, r, Qd.q4);
       float4 y = P.load(i, r, Qd.q4);
-      y.x = Qd.cnt > 0 ? rv.x * inv : y.x;
-      y.y = Qd.cnt > 1 ? rv.y * inv : y.y;
-      y.z = Qd.cnt > 2 ? rv.z * inv : y.z;
-      y.w = Qd.cnt > 3 ? rv.w * inv : y.w;
+      y.x = Qd.cnt > 0 ? rv.x * inv + ev.x : y.x;
+      y.y = Qd.cnt > 1 ? rv.y * inv + ev.y : y.y;
+      y.z = Qd.cnt > 2 ? rv.z * inv + ev.z : y.z;
+      y.w = Qd.cnt > 3 ? rv.w * inv + ev.w : y.w;
       const float4 p = P.store(i, r, Qd.q4, make_float4(y.x * sg[0], y.y * sg[1], y.z * sg[2], y.w * sg[3]));
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
@@ -240,29 +315,43 @@ __global__ __launch_bounds__(256) void k_pcg_start4(int n, int Q4, int C, PBlock
   if constexpr (PK != 0) flush_quads(Qd, Q4, C, acc2, sm2, rp);
 }
 
-// rz[c] += sum R*R/diag (the scale pass before an FP16 start).
+// rz[c] += sum R*R/diag, and with K the restriction rc += P^T R (the pass before a start that
+// needs the column scales or the coarse correction).
 __global__ __launch_bounds__(256) void k_pcg_rz4(int n, int Q4, int C, const float4* __restrict__ R,
-                                                 const float* __restrict__ diag, double* __restrict__ rz) {
+                                                 const float* __restrict__ diag, double* __restrict__ rz, CoarseArg K) {
   __shared__ double sm[kMaxCols + 4];
-  for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) sm[c] = 0.0;
+  __shared__ double sm2[2 * (kMaxCols + 4)];
+  for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) sm[c] = sm2[c] = sm2[4 * Q4 + c] = 0.0;
   __syncthreads();
-  const Quad Qd = quad_of(Q4, C);
-  const int64_t nq = (int64_t)n * Q4;
+  const QuadRange q = quad_range(Q4, C, n);
+  Restrict rs;
+  rs.init(q.r_lo, K.na ? K.S : 1);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  if (Qd.active && Qd.r0 < n) {
-    int64_t r = Qd.r0;
-    for (int64_t i = Qd.i0; i < nq; i += Qd.step, r += Qd.rstep) {
+  if (q.active) {
+    for (int64_t r = q.r_lo + q.roff; r < q.r_hi; r += q.rstep) {
       const double inv = 1.0 / (double)diag[r];
-      const float4 rv = R[i];
+      const float4 rv = R[r * Q4 + q.q4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (j < Qd.cnt) acc[j] += (double)quad_comp(rv, j) * (double)quad_comp(rv, j) * inv;
+      for (int j = 0; j < 4; ++j) {
+        if (j < q.cnt) {
+          const double x = (double)quad_comp(rv, j);
+          acc[j] += x * x * inv;
+          if (K.na) rs.add(r, j, x);
+        }
+      }
     }
   }
-  flush_quads(Qd, Q4, C, acc, sm, rz);
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if (j < q.cnt) lds_add(&sm[4 * q.q4 + j], acc[j]);
+  if (K.na) rs.to_lds(q, Q4, sm2);
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&rz[c], sm[c]);
+  if (K.na) rs.to_global(Q4, C, K, sm2);
 }
 
-// Z += a P, R -= a AP; rz_new[c] += sum R*R/diag; viol = max |R|/(eps_c*diag).
+// Z += a P, R -= a AP; rz_new[c] += sum R*R/diag; viol = max |R|/(eps_c*diag); with K also
+// rc += P^T R (the new residual restricted to the aggregates).  Rows in a contiguous split.
 template <int PK>
 __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float4* __restrict__ Z,
                                                      float4* __restrict__ R, PBlock<PK> P,
@@ -270,64 +359,70 @@ __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float
                                                      const double* __restrict__ alpha,
                                                      const float* __restrict__ diag, const float* __restrict__ eps,
                                                      double* __restrict__ rz_new, float* __restrict__ viol,
-                                                     const int32_t* __restrict__ skip) {
+                                                     const int32_t* __restrict__ skip, CoarseArg K) {
   if (skip && *skip) return;
   __shared__ double sm[kMaxCols + 4];
+  __shared__ double sm2[2 * (kMaxCols + 4)];
   __shared__ float vmax;
-  for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) sm[c] = 0.0;
+  for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) sm[c] = sm2[c] = sm2[4 * Q4 + c] = 0.0;
   if (threadIdx.x == 0) vmax = 0.f;
   __syncthreads();
-  const Quad Qd = quad_of(Q4, C);
+  const QuadRange q = quad_range(Q4, C, n);
+  Restrict rs;
+  rs.init(q.r_lo, K.na ? K.S : 1);
   float a[4], ie[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const int c = 4 * Qd.q4 + j;
-    a[j] = j < Qd.cnt ? (float)alpha[c] : 0.f;
-    ie[j] = j < Qd.cnt ? 1.f / eps[c] : 0.f;
+    const int c = 4 * q.q4 + j;
+    a[j] = j < q.cnt ? (float)alpha[c] : 0.f;
+    ie[j] = j < q.cnt ? 1.f / eps[c] : 0.f;
   }
-  const int64_t nq = (int64_t)n * Q4;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   float vm = 0.f;
-  if (Qd.active && Qd.r0 < n) {
-    int64_t r = Qd.r0;
-    for (int64_t i = Qd.i0; i < nq; i += Qd.step, r += Qd.rstep) {
+  if (q.active) {
+    for (int64_t r = q.r_lo + q.roff; r < q.r_hi; r += q.rstep) {
+      const int64_t i = r * Q4 + q.q4;
       const float inv = 1.f / diag[r];
       float4 z = Z[i], rv = R[i];
-      const float4 p = P.load(i, r, Qd.q4), ap = AP[i];
-      if (Qd.cnt > 0) { z.x += a[0] * p.x; rv.x -= a[0] * ap.x; }
-      if (Qd.cnt > 1) { z.y += a[1] * p.y; rv.y -= a[1] * ap.y; }
-      if (Qd.cnt > 2) { z.z += a[2] * p.z; rv.z -= a[2] * ap.z; }
-      if (Qd.cnt > 3) { z.w += a[3] * p.w; rv.w -= a[3] * ap.w; }
+      const float4 p = P.load(i, r, q.q4), ap = AP[i];
+      if (q.cnt > 0) { z.x += a[0] * p.x; rv.x -= a[0] * ap.x; }
+      if (q.cnt > 1) { z.y += a[1] * p.y; rv.y -= a[1] * ap.y; }
+      if (q.cnt > 2) { z.z += a[2] * p.z; rv.z -= a[2] * ap.z; }
+      if (q.cnt > 3) { z.w += a[3] * p.w; rv.w -= a[3] * ap.w; }
       Z[i] = z;
       R[i] = rv;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        if (j < Qd.cnt) {
+        if (j < q.cnt) {
           const float x = quad_comp(rv, j);
           acc[j] += (double)x * (double)x * (double)inv;
           vm = fmaxf(vm, fabsf(x) * inv * ie[j]);
+          if (K.na) rs.add(r, j, (double)x);
         }
       }
     }
   }
 #pragma unroll
   for (int j = 0; j < 4; ++j)
-    if (j < Qd.cnt) lds_add(&sm[4 * Qd.q4 + j], acc[j]);
+    if (j < q.cnt) lds_add(&sm[4 * q.q4 + j], acc[j]);
+  if (K.na) rs.to_lds(q, Q4, sm2);
   for (int off = 32; off > 0; off >>= 1) vm = fmaxf(vm, __shfl_xor(vm, off, 64));
   if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&vmax, vm);
   __syncthreads();
   for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&rz_new[c], sm[c]);
+  if (K.na) rs.to_global(Q4, C, K, sm2);
   if (threadIdx.x == 0) atomic_max_nonneg(viol, vmax);
 }
 
-// P = R/diag + beta P; with a 16-bit block also rp[c] += sum R*P as held (the next step
-// length's numerator, exact for the rounded direction).  FP16 blocks: P = sig*R/diag + beta*P
-// with sig the new column scale and beta already rescaled to the held direction.
+// P = M R + beta P (M R = R/diag plus the coarse correction on solved rows); with a 16-bit block
+// also rp[c] += sum R*P as held (the next step length's numerator, exact for the rounded
+// direction).  FP16 blocks: P = sig*M R + beta*P with sig the new column scale and beta already
+// rescaled to the held direction.
 template <int PK>
 __global__ __launch_bounds__(256) void k_pcg_dir4(int n, int Q4, int C, PBlock<PK> P, const float4* __restrict__ R,
                                                   const float* __restrict__ diag, const double* __restrict__ beta,
                                                   double* __restrict__ rp, const int32_t* __restrict__ skip,
-                                                  const float* __restrict__ sig) {
+                                                  const float* __restrict__ sig, CoarseArg K) {
   if (skip && *skip) return;
   __shared__ double sm[kMaxCols + 4];
   if constexpr (PK != 0) {
@@ -349,17 +444,18 @@ __global__ __launch_bounds__(256) void k_pcg_dir4(int n, int Q4, int C, PBlock<P
     for (int64_t i = Qd.i0; i < nq; i += Qd.step, r += Qd.rstep) {
       const float inv = 1.f / diag[r];
       const float4 rv = R[i];
+      const float4 ev = coarse_quad(K, r, Qd.q4);
       float4 p = P.load(i, r, Qd.q4);
       if constexpr (PK == 2) {
-        if (Qd.cnt > 0) p.x = rv.x * inv * sg[0] + b[0] * p.x;
-        if (Qd.cnt > 1) p.y = rv.y * inv * sg[1] + b[1] * p.y;
-        if (Qd.cnt > 2) p.z = rv.z * inv * sg[2] + b[2] * p.z;
-        if (Qd.cnt > 3) p.w = rv.w * inv * sg[3] + b[3] * p.w;
+        if (Qd.cnt > 0) p.x = (rv.x * inv + ev.x) * sg[0] + b[0] * p.x;
+        if (Qd.cnt > 1) p.y = (rv.y * inv + ev.y) * sg[1] + b[1] * p.y;
+        if (Qd.cnt > 2) p.z = (rv.z * inv + ev.z) * sg[2] + b[2] * p.z;
+        if (Qd.cnt > 3) p.w = (rv.w * inv + ev.w) * sg[3] + b[3] * p.w;
       } else {
-        if (Qd.cnt > 0) p.x = rv.x * inv + b[0] * p.x;
-        if (Qd.cnt > 1) p.y = rv.y * inv + b[1] * p.y;
-        if (Qd.cnt > 2) p.z = rv.z * inv + b[2] * p.z;
-        if (Qd.cnt > 3) p.w = rv.w * inv + b[3] * p.w;
+        if (Qd.cnt > 0) p.x = rv.x * inv + ev.x + b[0] * p.x;
+        if (Qd.cnt > 1) p.y = rv.y * inv + ev.y + b[1] * p.y;
+        if (Qd.cnt > 2) p.z = rv.z * inv + ev.z + b[2] * p.z;
+        if (Qd.cnt > 3) p.w = rv.w * inv + ev.w + b[3] * p.w;
       }
       const float4 held = P.store(i, r, Qd.q4, p);
       if constexpr (PK != 0) {
@@ -408,14 +504,17 @@ __device__ __forceinline__ float pcg_scale(double rz) {
 // Scalars of one iteration (one workgroup): mode 0: alpha = rz/pAp (0 where pAp <= 0), then
 // clear rz_new and viol for the update pass; mode 1: beta = rz_new/rz (0 where rz <= 0),
 // rz = rz_new, then clear pAp for the next iteration; with sig (FP16 directions) also the new
-// column scale, and beta rescaled to the held direction (beta * sig_new / sig_old).  Mode 2
-// (before an FP16 start): sig from rz, then rz = 0.
+// column scale, and beta rescaled to the held direction (beta * sig_new / sig_old).  Mode 0
+// also clears rc (nrc doubles) for the update pass's restriction.  Mode 2 (before a start that
+// had a scale pass): sig from rz (when sig), then rz = 0.
 __global__ void k_pcg_scalars(int C, int mode, double* __restrict__ rz, double* __restrict__ rz_new,
                               double* __restrict__ pAp, double* __restrict__ alpha,
                               double* __restrict__ beta, float* __restrict__ viol, int32_t* ctl,
-                              double* __restrict__ rp, float* __restrict__ sig) {
+                              double* __restrict__ rp, float* __restrict__ sig, double* __restrict__ rc, int nrc) {
   // every thread reads the flag before thread 0 may set it below (after the barrier)
   if (ctl && ctl[0]) return;
+  if (mode == 0 && rc)
+    for (int i = threadIdx.x; i < nrc; i += blockDim.x) rc[i] = 0.0;
   for (int c = threadIdx.x; c < C; c += blockDim.x) {
     if (mode == 0) {
       // step length along P as held: r.p / p.Ap (r.p = rz for an exact direction)
@@ -435,7 +534,7 @@ __global__ void k_pcg_scalars(int C, int mode, double* __restrict__ rz, double* 
       pAp[c] = 0.0;
       if (rp) rp[c] = 0.0;
     } else {
-      sig[c] = pcg_scale(rz[c]);
+      if (sig) sig[c] = pcg_scale(rz[c]);
       rz[c] = 0.0;
     }
   }
@@ -447,6 +546,147 @@ __global__ void k_pcg_scalars(int C, int mode, double* __restrict__ rz, double* 
       if (*viol <= 1.f) ctl[0] = 1;
     }
   }
+}
+
+// ---- two-level preconditioner: coarse operator, its inverse, and the coarse correction ----
+
+// ec[:, c] = Ainv rc[:, c] (one workgroup of kCoarseMax threads per column c < C, thread I =
+// aggregate, rc[:, c] staged in LDS), and rz[c] += rc[:, c] . ec[:, c] (the coarse part of
+// r^T M r).
+__global__ __launch_bounds__(kCoarseMax) void k_coarse_apply(int na, int C, int64_t ldc,
+                                                             const double* __restrict__ Ainv,
+                                                             const double* __restrict__ rc, float* __restrict__ ec,
+                                                             double* __restrict__ rz, const int32_t* __restrict__ ctl) {
+  if (ctl && ctl[0]) return;
+  __shared__ double rcs[kCoarseMax];
+  __shared__ double part[kCoarseMax / 64];
+  const int c = blockIdx.x, I = threadIdx.x;
+  rcs[I] = I < na ? rc[(int64_t)I * ldc + c] : 0.0;
+  __syncthreads();
+  double v = 0.0;
+  if (I < na) {
+    double e = 0.0;
+#pragma unroll 8
+    for (int J = 0; J < na; ++J) e += Ainv[(int64_t)J * na + I] * rcs[J];   // Ainv symmetric
+    ec[(int64_t)I * ldc + c] = (float)e;
+    v = rcs[I] * e;
+  }
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < kCoarseMax / 64; ++k) t += part[k];
+    atomicAdd(&rz[c], t);
+  }
+}
+
+// Ac += P^T A P over a share of aggregate I's rows (workgroup I * parts + part): for every
+// solved row r of aggregate I, diag_r at column I and -w_rs at column s/S for every solved
+// neighbour s.  Entries of aggregate I itself (most of them: neighbours are mostly near in
+// row order) are summed in registers; the others go through an LDS row; the row is added to Ac
+// with one atomic per nonzero entry.  Ac must be zero beforehand.
+__global__ __launch_bounds__(256) void k_coarse_build(int n, int na, int S, int parts,
+                                                      const int32_t* __restrict__ indptr,
+                                                      const int32_t* __restrict__ rowlen,
+                                                      const int32_t* __restrict__ col, const float* __restrict__ w,
+                                                      const uint8_t* __restrict__ solved,
+                                                      const double* __restrict__ diag, double* __restrict__ Ac) {
+  __shared__ double row[kCoarseMax];
+  for (int t = threadIdx.x; t < na; t += blockDim.x) row[t] = 0.0;
+  __syncthreads();
+  const int I = blockIdx.x / parts, part = blockIdx.x - I * parts;
+  const int64_t lo0 = (int64_t)I * S, hi0 = min((int64_t)n, lo0 + S);
+  const int64_t span = (hi0 - lo0 + parts - 1) / parts;
+  const int64_t lo = min(hi0, lo0 + part * span), hi = min(hi0, lo + span);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  double own = 0.0;
+  for (int64_t r = lo + wave; r < hi; r += nw) {
+    if (!solved[r]) continue;
+    if (lane == 0) own += diag[r];
+    const int beg = indptr[r], end = beg + rowlen[r];
+    for (int j = beg + lane; j < end; j += 64) {
+      const int sc = col[j];
+      if (sc >= 0 && sc < n && solved[sc]) {
+        const int J = sc / S;
+        if (J == I) own -= (double)w[j];
+        else lds_add(&row[J], -(double)w[j]);
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) own += __shfl_xor(own, off, 64);
+  if (lane == 0 && own != 0.0) lds_add(&row[I], own);
+  __syncthreads();
+  for (int t = threadIdx.x; t < na; t += blockDim.x)
+    if (row[t] != 0.0) atomicAdd(&Ac[(int64_t)I * na + t], row[t]);
+}
+
+// Aggregates without solved rows get a unit diagonal (decoupled from the rest).
+__global__ void k_coarse_fixdiag(int na, double* __restrict__ Ac) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I < na && Ac[(int64_t)I * na + I] == 0.0) Ac[(int64_t)I * na + I] = 1.0;
+}
+
+// Cholesky factor L (lower, row-major na x na) of Ac, one workgroup with the matrix in LDS.  A
+// pivot that is not positive (a coarse row without coupling) is replaced by 1.
+__global__ __launch_bounds__(256) void k_coarse_factor(int na, const double* Ac, double* L) {   // L may be Ac
+  extern __shared__ double Ls[];
+  for (int i = threadIdx.x; i < na * na; i += blockDim.x) Ls[i] = Ac[i];
+  __syncthreads();
+  for (int k = 0; k < na; ++k) {
+    if (threadIdx.x == 0) {
+      const double d = Ls[k * na + k];
+      Ls[k * na + k] = d > 0.0 ? sqrt(d) : 1.0;
+    }
+    __syncthreads();
+    const double pk = Ls[k * na + k];
+    for (int i = k + 1 + threadIdx.x; i < na; i += blockDim.x) Ls[i * na + k] /= pk;
+    __syncthreads();
+    const int m = na - k - 1;
+    for (int idx = threadIdx.x; idx < m * m; idx += blockDim.x) {
+      const int i = k + 1 + idx / m, j = k + 1 + idx % m;
+      if (j <= i) Ls[i * na + j] -= Ls[i * na + k] * Ls[j * na + k];
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < na * na; i += blockDim.x) {
+    const int r = i / na, c = i - r * na;
+    L[i] = c <= r ? Ls[i] : 0.0;
+  }
+}
+
+// Column j of Ainv = L^-T L^-1 e_j (one workgroup per column): forward, then back substitution,
+// each step's dot product reduced over the workgroup.
+__global__ __launch_bounds__(256) void k_coarse_inverse(int na, const double* __restrict__ L, double* __restrict__ Ainv) {
+  __shared__ double y[kCoarseMax];
+  __shared__ double part[4];
+  const int j = blockIdx.x, t = threadIdx.x;
+  for (int i = t; i < na; i += blockDim.x) y[i] = 0.0;
+  __syncthreads();
+  auto reduce = [&](double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((t & 63) == 0) part[t >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int k = 0; k < ((int)blockDim.x >> 6); ++k) s += part[k];
+    __syncthreads();
+    return s;
+  };
+  for (int i = j; i < na; ++i) {   // L y = e_j (y[i] = 0 for i < j)
+    double v = 0.0;
+    for (int k = j + t; k < i; k += blockDim.x) v += L[(int64_t)i * na + k] * y[k];
+    const double s = reduce(v);
+    if (t == 0) y[i] = ((i == j ? 1.0 : 0.0) - s) / L[(int64_t)i * na + i];
+    __syncthreads();
+  }
+  for (int i = na - 1; i >= 0; --i) {   // L^T x = y, in place
+    double v = 0.0;
+    for (int k = i + 1 + t; k < na; k += blockDim.x) v += L[(int64_t)k * na + i] * y[k];
+    const double s = reduce(v);
+    if (t == 0) y[i] = (y[i] - s) / L[(int64_t)i * na + i];
+    __syncthreads();
+  }
+  for (int i = t; i < na; i += blockDim.x) Ainv[(int64_t)i * na + j] = y[i];
 }
 
 // Clears n doubles (a fill kernel, so it can sit in a captured graph).
@@ -465,6 +705,26 @@ static unsigned pass_grid(int n, int lc_shift) {
   return g > 2048 ? 2048 : g;
 }
 
+int pcg_flat_chunk(const PcgBufs& b) {
+  const FlatLaunch f = flat_launch(b);
+  if (!f.ok) return 0;
+  return (int)(((int64_t)b.n + f.grid - 1) / f.grid);
+}
+
+static CoarseArg coarse_arg(const PcgBufs& b) {
+  if (!b.na) return CoarseArg{0, 1, 0, nullptr, nullptr, nullptr};
+  return CoarseArg{b.na, b.S, b.ldc, b.rc, b.ec, b.solved};
+}
+
+// The restriction of the flat passes keeps a workgroup's rows within two aggregates.
+static int check_coarse(const PcgBufs& b, const FlatLaunch& f) {
+  if (!b.na) return DLP_OK;
+  DLP_CHECK_ARG(f.ok && b.na <= kCoarseMax && b.rc && b.ec && b.Ainv && b.solved && b.ldc >= 4 * f.Q4 &&
+                    (int64_t)b.na * b.S >= b.n && ((int64_t)b.n + f.grid - 1) / f.grid <= b.S,
+                "pcg: coarse layout (na=%d S=%d n=%d grid=%u)", b.na, b.S, b.n, f.grid);
+  return DLP_OK;
+}
+
 int pcg_start(const PcgBufs& b, hipStream_t st) {
   DLP_CHECK_ARG(b.C >= 1 && b.C <= kMaxCols, "pcg: C=%d outside [1, %d]", b.C, kMaxCols);
   const ColMap m = col_map(b.C);
@@ -475,25 +735,37 @@ int pcg_start(const PcgBufs& b, hipStream_t st) {
   const FlatLaunch f = flat_launch(b);
   DLP_CHECK_ARG(f.ok || !b.P16, "pcg: 16-bit directions need 16-byte FP32 rows");
   DLP_CHECK_ARG(!b.P16 || b.p16 == 1 || (b.p16 == 2 && b.sig), "pcg: 16-bit direction kind %d", b.p16);
-  if (b.P16) {
-    hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, b.C, b.rp);
-    if (b.p16 == 2) {
-      hipLaunchKernelGGL(k_pcg_rz4, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (const float4*)b.R, b.diag,
-                         b.rz);
-      hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 2, b.rz, b.rz_new, b.pAp, b.alpha,
-                         b.beta, b.viol, nullptr, nullptr, b.sig);
-      hipLaunchKernelGGL(k_pcg_start4<2>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
-                         PBlock<2>{b.P16, b.ldh / 4}, (const float4*)b.R, b.diag, b.rz, b.rp, b.sig);
-    } else {
-      hipLaunchKernelGGL(k_pcg_start4<1>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
-                         PBlock<1>{b.P16, b.ldh / 4}, (const float4*)b.R, b.diag, b.rz, b.rp, nullptr);
-    }
-    DLP_CHECK_LAUNCH();
-    return DLP_OK;
-  }
+  DLP_CHECK_ARG(f.ok || !b.na, "pcg: the coarse correction needs 16-byte FP32 rows");
+  int rc = check_coarse(b, f);
+  if (rc) return rc;
   if (f.ok) {
-    hipLaunchKernelGGL(k_pcg_start4<0>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
-                       PBlock<0>{b.P, 0}, (const float4*)b.R, b.diag, b.rz, nullptr, nullptr);
+    const CoarseArg K = coarse_arg(b);
+    float* sig = (b.P16 && b.p16 == 2) ? b.sig : nullptr;
+    if (b.P16) hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, b.C, b.rp);
+    if (sig || b.na) {
+      // scale pass: rz (for the FP16 column scales) and the restricted residual, then the
+      // coarse correction of the start
+      if (b.na) {
+        const int nrc = b.na * (int)b.ldc;
+        hipLaunchKernelGGL(k_zero_f64, dim3(grid_for(nrc)), dim3(kBlock), 0, st, nrc, b.rc);
+      }
+      hipLaunchKernelGGL(k_pcg_rz4, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (const float4*)b.R, b.diag,
+                         b.rz, K);
+      if (b.na)
+        hipLaunchKernelGGL(k_coarse_apply, dim3(b.C), dim3(kCoarseMax), 0, st, b.na, b.C, b.ldc, b.Ainv, b.rc, b.ec,
+                           b.rz, nullptr);
+      hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 2, b.rz, b.rz_new, b.pAp, b.alpha,
+                         b.beta, b.viol, nullptr, nullptr, sig, nullptr, 0);
+    }
+    if (!b.P16)
+      hipLaunchKernelGGL(k_pcg_start4<0>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
+                         PBlock<0>{b.P, 0}, (const float4*)b.R, b.diag, b.rz, nullptr, nullptr, K);
+    else if (b.p16 == 2)
+      hipLaunchKernelGGL(k_pcg_start4<2>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
+                         PBlock<2>{b.P16, b.ldh / 4}, (const float4*)b.R, b.diag, b.rz, b.rp, b.sig, K);
+    else
+      hipLaunchKernelGGL(k_pcg_start4<1>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
+                         PBlock<1>{b.P16, b.ldh / 4}, (const float4*)b.R, b.diag, b.rz, b.rp, nullptr, K);
     DLP_CHECK_LAUNCH();
     return DLP_OK;
   }
@@ -504,6 +776,35 @@ int pcg_start(const PcgBufs& b, hipStream_t st) {
                      b.ld, b.rz);
   DLP_DISPATCH_CPL(m.cpl, K)
 #undef K
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+// Makes A exactly symmetric: A[i][j] = A[j][i] = (A[i][j] + A[j][i]) / 2.
+__global__ void k_symmetrize(int na, double* __restrict__ A) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= na * na) return;
+  const int i = idx / na, j = idx - i * na;
+  if (j < i) {
+    const double v = 0.5 * (A[i * na + j] + A[j * na + i]);
+    A[i * na + j] = v;
+    A[j * na + i] = v;
+  }
+}
+
+int coarse_build(int n, int na, int S, const int32_t* indptr, const int32_t* rowlen, const int32_t* col,
+                 const float* w, const uint8_t* solved, const double* diag, double* Ac, double* Ainv,
+                 hipStream_t st) {
+  DLP_CHECK_ARG(n >= 1 && na >= 1 && na <= kCoarseMax && S >= 1 && (int64_t)na * S >= n,
+                "coarse_build: na=%d S=%d n=%d", na, S, n);
+  const int parts = 8;
+  hipLaunchKernelGGL(k_zero_f64, dim3(grid_for((long long)na * na)), dim3(kBlock), 0, st, na * na, Ac);
+  hipLaunchKernelGGL(k_coarse_build, dim3(na * parts), dim3(kBlock), 0, st, n, na, S, parts, indptr, rowlen, col, w,
+                     solved, diag, Ac);
+  hipLaunchKernelGGL(k_coarse_fixdiag, dim3(1), dim3(kCoarseMax), 0, st, na, Ac);
+  hipLaunchKernelGGL(k_coarse_factor, dim3(1), dim3(kBlock), sizeof(double) * na * na, st, na, Ac, Ac);
+  hipLaunchKernelGGL(k_coarse_inverse, dim3(na), dim3(kBlock), 0, st, na, Ac, Ainv);
+  hipLaunchKernelGGL(k_symmetrize, dim3(grid_for((long long)na * na)), dim3(kBlock), 0, st, na, Ainv);
   DLP_CHECK_LAUNCH();
   return DLP_OK;
 }
@@ -520,22 +821,29 @@ int pcg_finish_iteration(const PcgBufs& b, hipStream_t st, bool have_pap) {
       DLP_DISPATCH_CPL(m.cpl, K)
 #undef K
     }
+    int rc = check_coarse(b, f);
+    if (rc) return rc;
+    const CoarseArg K = coarse_arg(b);
     double* rp = b.P16 ? b.rp : nullptr;
     float* sig = (b.P16 && b.p16 == 2) ? b.sig : nullptr;
     hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 0, b.rz, b.rz_new, b.pAp, b.alpha,
-                       b.beta, b.viol, b.ctl, rp, sig);
+                       b.beta, b.viol, b.ctl, rp, sig, b.na ? b.rc : nullptr, b.na * (int)b.ldc);
 #define DLP_UPDATE(PKV, PB)                                                                                  \
   hipLaunchKernelGGL(k_pcg_update4<PKV>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (float4*)b.Z, \
                      (float4*)b.R, PBlock<PKV>{PB, b.P16 ? b.ldh / 4 : 0}, (const float4*)b.AP, b.alpha, b.diag, \
-                     b.eps, b.rz_new, b.viol, b.ctl)
+                     b.eps, b.rz_new, b.viol, b.ctl, K)
 #define DLP_DIR(PKV, PB)                                                                                     \
   hipLaunchKernelGGL(k_pcg_dir4<PKV>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,                  \
-                     PBlock<PKV>{PB, b.P16 ? b.ldh / 4 : 0}, (const float4*)b.R, b.diag, b.beta, rp, b.ctl, sig)
+                     PBlock<PKV>{PB, b.P16 ? b.ldh / 4 : 0}, (const float4*)b.R, b.diag, b.beta, rp, b.ctl, sig, K)
     if (!b.P16) DLP_UPDATE(0, b.P);
     else if (b.p16 == 2) DLP_UPDATE(2, b.P16);
     else DLP_UPDATE(1, b.P16);
+    // rz_new += rc . Ac^-1 rc (the coarse part of r^T M r) and the correction the direction uses
+    if (b.na)
+      hipLaunchKernelGGL(k_coarse_apply, dim3(b.C), dim3(kCoarseMax), 0, st, b.na, b.C, b.ldc, b.Ainv, b.rc, b.ec,
+                         b.rz_new, b.ctl);
     hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 1, b.rz, b.rz_new, b.pAp, b.alpha,
-                       b.beta, b.viol, b.ctl, rp, sig);
+                       b.beta, b.viol, b.ctl, rp, sig, nullptr, 0);
     if (!b.P16) DLP_DIR(0, b.P);
     else if (b.p16 == 2) DLP_DIR(2, b.P16);
     else DLP_DIR(1, b.P16);
@@ -544,17 +852,17 @@ int pcg_finish_iteration(const PcgBufs& b, hipStream_t st, bool have_pap) {
     DLP_CHECK_LAUNCH();
     return DLP_OK;
   }
-  DLP_CHECK_ARG(!b.P16, "pcg: 16-bit directions need 16-byte FP32 rows");
+  DLP_CHECK_ARG(!b.P16 && !b.na, "pcg: 16-bit directions and the coarse correction need 16-byte FP32 rows");
 #define K(CPLV)                                                                                       \
   if (!have_pap)                                                                                      \
     hipLaunchKernelGGL((k_coldot<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.P, b.AP, \
                        b.ld, b.pAp);                                                                 \
   hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 0, b.rz, b.rz_new, b.pAp,        \
-                     b.alpha, b.beta, b.viol, b.ctl, nullptr, nullptr);                              \
+                     b.alpha, b.beta, b.viol, b.ctl, nullptr, nullptr, nullptr, 0);                  \
   hipLaunchKernelGGL((k_pcg_update<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.Z, b.R, \
                      b.P, b.AP, b.ld, b.alpha, b.diag, b.eps, b.rz_new, b.viol, b.ctl);              \
   hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 1, b.rz, b.rz_new, b.pAp,        \
-                     b.alpha, b.beta, b.viol, b.ctl, nullptr, nullptr);                              \
+                     b.alpha, b.beta, b.viol, b.ctl, nullptr, nullptr, nullptr, 0);                  \
   hipLaunchKernelGGL((k_pcg_dir<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.P, b.R,   \
                      b.ld, b.diag, b.beta, 0, b.ctl);
   DLP_DISPATCH_CPL(m.cpl, K)

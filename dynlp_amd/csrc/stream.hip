@@ -1023,6 +1023,14 @@ struct LP {
   int64_t ldh = 0;              // its row length: C1 rounded up to 8
   int pcg_p16 = 0;              // 0 FP32 directions, 1 BF16, 2 FP16 with a per-column scale
   double* pcg_scal = nullptr;   // rz, rz_new, pAp, alpha, beta, rp (C1 doubles each), sig (C1 floats)
+  // two-level preconditioner of the column PCG: coarse_na aggregates requested (0: Jacobi
+  // only), rebuilt every coarse_refresh global solves; cg_* = the current layout and buffers
+  int coarse_na = 0, coarse_refresh = 64;
+  int cg_na = 0, cg_S = 0, cg_age = -1;
+  double* cg_Ac = nullptr;
+  double* cg_Ainv = nullptr;
+  double* cg_rc = nullptr;
+  float* cg_ec = nullptr;
   float* pcg_viol = nullptr;
   hipGraphExec_t pcg_exec = nullptr;
   int pcg_graph_iters = 0;
@@ -1092,6 +1100,10 @@ int lp_free(LP* h) {
   if (h->P) (void)hipFree(h->P);
   if (h->AP) (void)hipFree(h->AP);
   if (h->P16) (void)hipFree(h->P16);
+  if (h->cg_Ac) (void)hipFree(h->cg_Ac);
+  if (h->cg_Ainv) (void)hipFree(h->cg_Ainv);
+  if (h->cg_rc) (void)hipFree(h->cg_rc);
+  if (h->cg_ec) (void)hipFree(h->cg_ec);
   if (h->pcg_scal) (void)hipFree(h->pcg_scal);
   if (h->pcg_viol) (void)hipFree(h->pcg_viol);
   if (h->pcg_exec) (void)hipGraphExecDestroy(h->pcg_exec);
@@ -1705,7 +1717,7 @@ extern "C" int dlp_lp_status(void* handle, int64_t* out, double* rho, void* stre
 // kept residual R32 (solvers.py:66-97, the DynLP+ global path solvers.py:268-288).
 
 static PcgBufs pcg_bufs(LP* h) {
-  PcgBufs b;
+  PcgBufs b{};
   b.n = h->d.n;
   b.C = h->d.C1;
   b.ld = h->d.ld;
@@ -1728,6 +1740,15 @@ static PcgBufs pcg_bufs(LP* h) {
   b.rp = h->pcg_scal + 5 * C1;
   b.p16 = h->pcg_p16;
   b.sig = reinterpret_cast<float*>(h->pcg_scal + 6 * C1);
+  if (h->cg_na && h->cg_age >= 0) {
+    b.na = h->cg_na;
+    b.S = h->cg_S;
+    b.ldc = h->d.ld;
+    b.rc = h->cg_rc;
+    b.ec = h->cg_ec;
+    b.Ainv = h->cg_Ainv;
+    b.solved = h->d.solved;
+  }
   return b;
 }
 
@@ -1744,6 +1765,66 @@ static int pcg_iteration(LP* h, const PcgBufs& b, hipStream_t st) {
                             0, 20.0, st);
   if (rc) return rc;
   return pcg_finish_iteration(b, st);
+}
+
+// The coarse operator of the two-level preconditioner: layout (aggregates of S consecutive rows,
+// at least the row span of one workgroup of the flat passes), buffers, and Ac^-1 rebuilt from the
+// current equations every coarse_refresh global solves (a stale Ac still gives an SPD
+// preconditioner, only a weaker one: the equations of a few percent of the rows change between
+// rebuilds).  Changing the layout drops the captured iterations, which hold its pointers.
+static int coarse_prepare(LP* h, hipStream_t st) {
+  const int n = h->d.n;
+  int na = 0, S = 0;
+  if (h->coarse_na > 0 && n >= 64) {
+    PcgBufs probe = pcg_bufs(h);
+    probe.na = 0;
+    const int chunk = pcg_flat_chunk(probe);
+    if (chunk > 0) {
+      na = h->coarse_na < kCoarseMax ? h->coarse_na : kCoarseMax;
+      S = (n + na - 1) / na;
+      if (S < chunk) S = chunk;
+      if (S < 32) S = 32;
+      na = (n + S - 1) / S;
+    }
+  }
+  if (na != h->cg_na || S != h->cg_S) {
+    if (h->pcg_exec) {
+      (void)hipGraphExecDestroy(h->pcg_exec);
+      h->pcg_exec = nullptr;
+    }
+    if (h->cg_Ac) (void)hipFree(h->cg_Ac);
+    if (h->cg_Ainv) (void)hipFree(h->cg_Ainv);
+    if (h->cg_rc) (void)hipFree(h->cg_rc);
+    if (h->cg_ec) (void)hipFree(h->cg_ec);
+    h->cg_Ac = h->cg_Ainv = h->cg_rc = nullptr;
+    h->cg_ec = nullptr;
+    h->cg_na = na;
+    h->cg_S = S;
+    h->cg_age = -1;
+    if (na) {
+      const size_t ldc = (size_t)h->d.ld;
+      DLP_CHECK_HIP(hipMalloc((void**)&h->cg_Ac, sizeof(double) * na * na));
+      DLP_CHECK_HIP(hipMalloc((void**)&h->cg_Ainv, sizeof(double) * na * na));
+      DLP_CHECK_HIP(hipMalloc((void**)&h->cg_rc, sizeof(double) * na * ldc));
+      DLP_CHECK_HIP(hipMalloc((void**)&h->cg_ec, sizeof(float) * na * ldc));
+      DLP_CHECK_HIP(hipMemsetAsync(h->cg_rc, 0, sizeof(double) * na * ldc, st));
+      DLP_CHECK_HIP(hipMemsetAsync(h->cg_ec, 0, sizeof(float) * na * ldc, st));
+    }
+  }
+  if (!na) return DLP_OK;
+  if (h->cg_age < 0 || h->cg_age >= h->coarse_refresh) {
+    const bool first = h->cg_age < 0;
+    const int rc = coarse_build(n, na, S, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.solved, h->d.diag64,
+                                h->cg_Ac, h->cg_Ainv, st);
+    if (rc) return rc;
+    h->cg_age = 0;
+    if (first && h->pcg_exec) {   // captured before the coarse buffers existed
+      (void)hipGraphExecDestroy(h->pcg_exec);
+      h->pcg_exec = nullptr;
+    }
+  }
+  h->cg_age += 1;
+  return DLP_OK;
 }
 
 static int capture_pcg(LP* h, int iters) {
@@ -1801,8 +1882,10 @@ extern "C" int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int 
     DLP_CHECK_HIP(hipMalloc((void**)&h->P16, nh * sizeof(uint16_t)));
     DLP_CHECK_HIP(hipMemsetAsync(h->P16, 0, nh * sizeof(uint16_t), st));
   }
+  int rc = coarse_prepare(h, st);
+  if (rc) return rc;
   const PcgBufs b = pcg_bufs(h);
-  int rc = pcg_start(b, st);
+  rc = pcg_start(b, st);
   if (rc) return rc;
   if (use_graph) {
     rc = capture_pcg(h, iters_per_launch);
@@ -1873,6 +1956,14 @@ extern "C" int dlp_lp_scores(void* handle, double* out, int64_t ldo, void* strea
     hipLaunchKernelGGL(k_scores<float>, dim3(grid_for(total)), dim3(kBlock), 0, st, h->d.n, h->d.C1, h->d.Z32,
                        h->d.ld, h->d.solved, out, ldo);
   DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+extern "C" int dlp_lp_set_coarse(void* handle, int na, int refresh) {
+  DLP_CHECK_ARG(handle && na >= 0 && refresh >= 1, "lp_set_coarse: bad argument (na=%d refresh=%d)", na, refresh);
+  LP* h = (LP*)handle;
+  h->coarse_na = na < kCoarseMax ? na : kCoarseMax;
+  h->coarse_refresh = refresh;
   return DLP_OK;
 }
 
@@ -2014,6 +2105,7 @@ extern "C" int dlp_lp_bpcg(void* handle, int max_iter, int iters_per_launch, int
   PcgBufs b = pcg_bufs(h);
   b.P16 = nullptr;                                 // block PCG keeps FP32 directions
   b.p16 = 0;
+  b.na = 0;                                        // and the Jacobi preconditioner
   int rc = pcg_start(b, st);                       // P = R/diag
   if (!rc) rc = dlp_gram(b.n, C1, b.R, b.ld, b.R, b.ld, b.diag, h->bk, DLP_F32, st);   // G = R^T D^-1 R
   if (rc) return rc;

@@ -71,12 +71,15 @@ class EdgeStreamLP:
     Z/R in FP64 and repairs with the host-driven push rounds of the reference.  pcg_directions:
     storage of the global solve's search directions, "fp16" (default: half the gather bytes, a
     power-of-two scale per column, 11 significant bits, step lengths exact for the rounded
-    direction), "bf16" (half the bytes, 8 significant bits) or "fp32"."""
+    direction), "bf16" (half the bytes, 8 significant bits) or "fp32".  pcg_coarse: aggregates of
+    the two-level preconditioner of the global solve (runs of consecutive rows; 0 = Jacobi only),
+    its coarse operator rebuilt from the current equations every coarse_refresh global solves."""
 
     def __init__(self, g: Graph, eps=1e-6, eps_h=0.05, fp64=False, eta=0.0, slack_min=4,
                  slack_frac=0.25, push_scale=0.9, rounds_per_launch=4, use_graph=True, max_rounds=200_000,
-                 push_frac=0.05, push_max_rows=4096, push_budget=1.0, push_mass=0.15, pcg_iters_per_launch=6,
-                 pcg_max_iter=20_000, pcg_mode="column", incremental_ground=True, pcg_directions="fp16"):
+                 push_frac=0.05, push_max_rows=4096, push_budget=1.0, push_mass=0.15, pcg_iters_per_launch=4,
+                 pcg_max_iter=20_000, pcg_mode="column", incremental_ground=True, pcg_directions="fp16", pcg_coarse=128,
+                 coarse_refresh=64):
         self.g, self.n, self.K = g, g.n, g.K
         self.C = 1 if g.K == 2 else g.K
         self.C1 = self.C + 1
@@ -133,6 +136,8 @@ class EdgeStreamLP:
             raise ValueError(f"pcg_directions {pcg_directions!r}: 'fp32', 'bf16' or 'fp16'")
         self.pcg_directions = pcg_directions
         _lib.call("dlp_lp_set_pcg_directions", self.h, 0 if self.fp64 else kinds[pcg_directions])
+        self.pcg_coarse = int(pcg_coarse)
+        _lib.call("dlp_lp_set_coarse", self.h, self.pcg_coarse, int(coarse_refresh))
         self._status = np.zeros(16, dtype=np.int64)
         self._rho = np.zeros(self.C1, dtype=np.float64)
         self.batches = 0

@@ -117,6 +117,47 @@ def test_global_solve_from_zero_certifies(dirs, word):
     check_against_oracle(lp, g, es, 1e-6, f"{dirs}/{word}")
 
 
+@pytest.mark.parametrize("dirs", ["fp32", "bf16", "fp16"])
+def test_two_level_preconditioner_certifies_and_cuts_iterations(dirs):
+    """The coarse correction (runs of consecutive rows as aggregates, Ac = P^T A P factored and
+    inverted on the device) certifies against the rebuilt oracle like the Jacobi-preconditioned
+    solve, whether Ac is rebuilt for every solve or kept from the first, and needs fewer PCG
+    iterations.  Push is switched off so every batch takes the global solve."""
+    from dynlp_amd.stream import EdgeStreamLP
+    g = graphs.sbm(3000, 8, 10, 0.8, 8, 0.01, seed=11)
+    batches, _ = graphs.update_stream(g, 3, 200, seed=12)
+    iters = {}
+    for na, refresh in ((0, 64), (128, 1), (128, 64), (20, 64)):
+        lp = EdgeStreamLP(g, eps=1e-6, pcg_directions=dirs, pcg_coarse=na, coarse_refresh=refresh,
+                          push_max_rows=0)
+        lp.initial_solve()
+        es = edge_dict(g)
+        s0 = lp.pcg_stats()
+        for b in batches:
+            st = lp.apply_and_solve(b)
+            assert "pcg" in st.path, st.path
+            O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
+                               zip(b.del_u.tolist(), b.del_v.tolist()))
+        iters[(na, refresh)] = lp.pcg_stats()["pcg_iters"] - s0["pcg_iters"]
+        check_against_oracle(lp, g, es, 1e-6, f"{dirs} coarse={na} refresh={refresh}")
+    assert iters[(128, 1)] < iters[(0, 64)] and iters[(128, 64)] < iters[(0, 64)], iters
+
+
+def test_tiny_graph_solves_without_coarse_layout():
+    """Below 64 vertices the global solve runs with the Jacobi preconditioner alone."""
+    from dynlp_amd.stream import EdgeStreamLP
+    g = graphs.sbm(48, 2, 6, 0.8, 2, 0.2, seed=3)
+    lp = EdgeStreamLP(g, eps=1e-6, push_max_rows=0)
+    lp.initial_solve()
+    es = edge_dict(g)
+    batches, _ = graphs.update_stream(g, 2, 6, seed=4)
+    for b in batches:
+        lp.apply_and_solve(b)
+        O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
+                           zip(b.del_u.tolist(), b.del_v.tolist()))
+    check_against_oracle(lp, g, es, 1e-6, "tiny")
+
+
 def test_pcg_group_length_does_not_change_the_solve():
     """Iterations past convergence inside a captured group return at once (device flag): group
     lengths 1 and 7 run the same iterations and give the same state."""
