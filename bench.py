@@ -6,9 +6,9 @@ U[0.5, 1.5] rounded to FP32, 0.5% labelled with label = block mod 64 (k = 64, 65
 with h).  After the initial solve, a stream of requests, each a batch of `--batch-edges` edge
 updates (50% insertions of new uniform pairs, 50% deletions of existing edges; seed S+2).  A step
 is one request: apply on the device-resident slack CSR, re-ground (connected components),
-refresh the changed rows' FP64 residual, repair to max |R_uc|/diag_u <= 1e-6 (push or global PCG
-on FP32 vectors with scaled FP16 search directions, replayed from hipGraphs) and verify in FP64.  Latency
-is submit -> certified.
+refresh the changed rows' FP64 residual, repair to max |R_uc|/diag_u <= 1e-6 (push, or the
+global PCG on FP32 vectors with scaled FP16 search directions and a two-level preconditioner,
+replayed from hipGraphs) and verify in FP64.  Latency is submit -> certified.
 
 Printed on stdout: one JSON line.  Progress goes to stderr.
 """
@@ -303,7 +303,7 @@ def main(argv=None):
         "config": {"workload": "sbm-1m-k64 edge-update stream" if a.scale == 1.0 else f"sbm-{g.n}-k64 edge-update stream",
                    "n": g.n, "blocks": 64, "deg": 20, "p_in": 0.8, "k": 64, "label_frac": 0.005,
                    "batch_edges": a.batch_edges, "ins_frac": 0.5, "eps": 1e-6, "seed": a.seed,
-                   "pcg_directions": pcg_dirs},
+                   "pcg_directions": pcg_dirs, "pcg_coarse_aggregates": lp.pcg_coarse},
         "p50_ms": round(percentile(lat, 50), 3), "p99_ms": round(percentile(lat, 99), 3),
         "rho_max": max(rhos), "paths": paths,
         "pcg_iters_per_request": round(pcg_iters / a.steps, 2), "pcg_ms_per_request": round(pcg_ms / a.steps, 3),
