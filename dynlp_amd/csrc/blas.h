@@ -18,7 +18,7 @@ struct PcgBufs {
   double* pAp;
   double* alpha;
   double* beta;
-  float* viol;
+  float* viol;   // [0] the iteration's violation, [1] the last completed one (for the host)
   // [0] done (set once the violation is <= 1: every later pass of a captured group returns at
   // once), [1] iterations completed; null disables the device-side exit
   int32_t* ctl;

@@ -350,13 +350,15 @@ __global__ __launch_bounds__(256) void k_pcg_rz4(int n, int Q4, int C, const flo
   if (K.na) rs.to_global(Q4, C, K, sm2);
 }
 
-// Z += a P, R -= a AP; rz_new[c] += sum R*R/diag; viol = max |R|/(eps_c*diag); with K also
-// rc += P^T R (the new residual restricted to the aggregates).  Rows in a contiguous split.
+// Z += a P, R -= a AP with the step length a = num/pAp of every column (num = r.p for 16-bit
+// directions, r.z otherwise; 0 where pAp <= 0); rz_new[c] += sum R*R/diag; viol = max
+// |R|/(eps_c*diag); with K also rc += P^T R (the new residual restricted to the aggregates).
+// Rows in a contiguous split.
 template <int PK>
 __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float4* __restrict__ Z,
                                                      float4* __restrict__ R, PBlock<PK> P,
                                                      const float4* __restrict__ AP,
-                                                     const double* __restrict__ alpha,
+                                                     const double* __restrict__ pAp, const double* __restrict__ num,
                                                      const float* __restrict__ diag, const float* __restrict__ eps,
                                                      double* __restrict__ rz_new, float* __restrict__ viol,
                                                      const int32_t* __restrict__ skip, CoarseArg K) {
@@ -374,7 +376,8 @@ __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int c = 4 * q.q4 + j;
-    a[j] = j < q.cnt ? (float)alpha[c] : 0.f;
+    const double pa = j < q.cnt ? pAp[c] : 0.0;
+    a[j] = pa > 0.0 ? (float)(num[c] / pa) : 0.f;
     ie[j] = j < q.cnt ? 1.f / eps[c] : 0.f;
   }
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -503,10 +506,10 @@ __device__ __forceinline__ float pcg_scale(double rz) {
 
 // Scalars of one iteration (one workgroup): mode 0: alpha = rz/pAp (0 where pAp <= 0), then
 // clear rz_new and viol for the update pass; mode 1: beta = rz_new/rz (0 where rz <= 0),
-// rz = rz_new, then clear pAp for the next iteration; with sig (FP16 directions) also the new
-// column scale, and beta rescaled to the held direction (beta * sig_new / sig_old).  Mode 0
-// also clears rc (nrc doubles) for the update pass's restriction.  Mode 2 (before a start that
-// had a scale pass): sig from rz (when sig), then rz = 0.
+// rz = rz_new, then clear rz_new, pAp, rp and (after the convergence test, keeping its value in
+// viol[1] for the host) viol for the next iteration; with sig (FP16 directions) also the new column scale, and beta rescaled to the
+// held direction (beta * sig_new / sig_old).  Mode 0 (the generic passes) also clears rc (nrc
+// doubles).  Mode 2 (before a start that had a scale pass): sig from rz (when sig), then rz = 0.
 __global__ void k_pcg_scalars(int C, int mode, double* __restrict__ rz, double* __restrict__ rz_new,
                               double* __restrict__ pAp, double* __restrict__ alpha,
                               double* __restrict__ beta, float* __restrict__ viol, int32_t* ctl,
@@ -531,6 +534,7 @@ __global__ void k_pcg_scalars(int C, int mode, double* __restrict__ rz, double* 
       }
       beta[c] = bt;
       rz[c] = rz_new[c];
+      rz_new[c] = 0.0;
       pAp[c] = 0.0;
       if (rp) rp[c] = 0.0;
     } else {
@@ -539,23 +543,49 @@ __global__ void k_pcg_scalars(int C, int mode, double* __restrict__ rz, double* 
     }
   }
   if (mode == 0 && threadIdx.x == 0) *viol = 0.f;
-  if (mode == 1 && ctl) {
+  if (mode == 1) {
     __syncthreads();
     if (threadIdx.x == 0) {
-      ctl[1] += 1;
-      if (*viol <= 1.f) ctl[0] = 1;
+      if (ctl) {
+        ctl[1] += 1;
+        if (viol[0] <= 1.f) ctl[0] = 1;
+      }
+      viol[1] = viol[0];   // the last violation, for the host
+      viol[0] = 0.f;
     }
   }
 }
 
 // ---- two-level preconditioner: coarse operator, its inverse, and the coarse correction ----
 
+// ec[I][c] = sum_J Ainv[J][I] rcs[J] for thread I < na (Ainv symmetric, rcs = rc[:, c] in LDS);
+// returns sum_I rcs[I] ec[I][c] in thread 0 (the workgroup's threads all call it).
+__device__ __forceinline__ double coarse_column(int na, int c, int64_t ldc, const double* __restrict__ Ainv,
+                                                const double* rcs, float* __restrict__ ec, double* part) {
+  const int I = threadIdx.x;
+  double v = 0.0;
+  if (I < na) {
+    double e = 0.0;
+#pragma unroll 8
+    for (int J = 0; J < na; ++J) e += Ainv[(int64_t)J * na + I] * rcs[J];
+    ec[(int64_t)I * ldc + c] = (float)e;
+    v = rcs[I] * e;
+  }
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < ((int)blockDim.x >> 6); ++k) t += part[k];
+  return t;
+}
+
 // ec[:, c] = Ainv rc[:, c] (one workgroup of kCoarseMax threads per column c < C, thread I =
-// aggregate, rc[:, c] staged in LDS), and rz[c] += rc[:, c] . ec[:, c] (the coarse part of
-// r^T M r).
+// aggregate, rc[:, c] staged in LDS, then cleared), and rz[c] += rc[:, c] . ec[:, c] (the
+// coarse part of r^T M r) -- the start's correction.
 __global__ __launch_bounds__(kCoarseMax) void k_coarse_apply(int na, int C, int64_t ldc,
                                                              const double* __restrict__ Ainv,
-                                                             const double* __restrict__ rc, float* __restrict__ ec,
+                                                             double* __restrict__ rc, float* __restrict__ ec,
                                                              double* __restrict__ rz, const int32_t* __restrict__ ctl) {
   if (ctl && ctl[0]) return;
   __shared__ double rcs[kCoarseMax];
@@ -563,21 +593,54 @@ __global__ __launch_bounds__(kCoarseMax) void k_coarse_apply(int na, int C, int6
   const int c = blockIdx.x, I = threadIdx.x;
   rcs[I] = I < na ? rc[(int64_t)I * ldc + c] : 0.0;
   __syncthreads();
-  double v = 0.0;
-  if (I < na) {
-    double e = 0.0;
-#pragma unroll 8
-    for (int J = 0; J < na; ++J) e += Ainv[(int64_t)J * na + I] * rcs[J];   // Ainv symmetric
-    ec[(int64_t)I * ldc + c] = (float)e;
-    v = rcs[I] * e;
-  }
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  if (I < na) rc[(int64_t)I * ldc + c] = 0.0;   // ready for the next restriction
+  const double v = coarse_column(na, c, ldc, Ainv, rcs, ec, part);
+  if (threadIdx.x == 0) atomicAdd(&rz[c], v);
+}
+
+// One iteration's scalars with the coarse correction (one workgroup of kCoarseMax threads per
+// column c): ec[:, c] = Ainv rc[:, c]; rz_c = rz_new[c] + rc . ec (r^T M r); beta = rz_c/rz[c]
+// (with FP16 directions the new column scale and beta rescaled to the held direction);
+// rz[c] = rz_c; rz_new, pAp, rp and rc of the column cleared for the next iteration.
+// Workgroup 0 counts the iteration and sets the done flag when viol <= 1, then moves viol to
+// viol[1] (read by the host) and clears it.
+__global__ __launch_bounds__(kCoarseMax) void k_coarse_scalars(int na, int C, int64_t ldc,
+                                                               const double* __restrict__ Ainv,
+                                                               double* __restrict__ rc, float* __restrict__ ec,
+                                                               double* __restrict__ rz, double* __restrict__ rz_new,
+                                                               double* __restrict__ pAp, double* __restrict__ rp,
+                                                               double* __restrict__ beta, float* __restrict__ sig,
+                                                               float* __restrict__ viol, int32_t* ctl) {
+  if (ctl && ctl[0]) return;
+  __shared__ double rcs[kCoarseMax];
+  __shared__ double part[kCoarseMax / 64];
+  const int c = blockIdx.x, I = threadIdx.x;
+  rcs[I] = I < na ? rc[(int64_t)I * ldc + c] : 0.0;
   __syncthreads();
+  if (I < na) rc[(int64_t)I * ldc + c] = 0.0;
+  const double v = coarse_column(na, c, ldc, Ainv, rcs, ec, part);
   if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int k = 0; k < kCoarseMax / 64; ++k) t += part[k];
-    atomicAdd(&rz[c], t);
+    const double rzc = rz_new[c] + v;
+    const double r = rz[c];
+    double bt = r > 0.0 ? rzc / r : 0.0;
+    if (sig) {
+      const float sn = pcg_scale(rzc);
+      bt *= (double)sn / (double)sig[c];
+      sig[c] = sn;
+    }
+    beta[c] = bt;
+    rz[c] = rzc;
+    rz_new[c] = 0.0;
+    pAp[c] = 0.0;
+    if (rp) rp[c] = 0.0;
+    if (c == 0) {
+      if (ctl) {
+        ctl[1] += 1;
+        if (viol[0] <= 1.f) ctl[0] = 1;
+      }
+      viol[1] = viol[0];   // the last violation, for the host
+      viol[0] = 0.f;
+    }
   }
 }
 
@@ -730,7 +793,10 @@ int pcg_start(const PcgBufs& b, hipStream_t st) {
   const ColMap m = col_map(b.C);
   const unsigned g = pass_grid(b.n, m.lc_shift);
   hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, b.C, b.rz);
+  hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, b.C, b.rz_new);
   hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, b.C, b.pAp);
+  // viol and the pad word after it (the iterations clear viol after each convergence test)
+  hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, 1, (double*)b.viol);
   if (b.ctl) hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, 1, (double*)b.ctl);
   const FlatLaunch f = flat_launch(b);
   DLP_CHECK_ARG(f.ok || !b.P16, "pcg: 16-bit directions need 16-byte FP32 rows");
@@ -826,24 +892,26 @@ int pcg_finish_iteration(const PcgBufs& b, hipStream_t st, bool have_pap) {
     const CoarseArg K = coarse_arg(b);
     double* rp = b.P16 ? b.rp : nullptr;
     float* sig = (b.P16 && b.p16 == 2) ? b.sig : nullptr;
-    hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 0, b.rz, b.rz_new, b.pAp, b.alpha,
-                       b.beta, b.viol, b.ctl, rp, sig, b.na ? b.rc : nullptr, b.na * (int)b.ldc);
+    // the update pass takes its step lengths from pAp and r.p (16-bit directions) or r.z
+    const double* num = b.P16 ? b.rp : b.rz;
 #define DLP_UPDATE(PKV, PB)                                                                                  \
   hipLaunchKernelGGL(k_pcg_update4<PKV>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (float4*)b.Z, \
-                     (float4*)b.R, PBlock<PKV>{PB, b.P16 ? b.ldh / 4 : 0}, (const float4*)b.AP, b.alpha, b.diag, \
-                     b.eps, b.rz_new, b.viol, b.ctl, K)
+                     (float4*)b.R, PBlock<PKV>{PB, b.P16 ? b.ldh / 4 : 0}, (const float4*)b.AP, b.pAp, num,  \
+                     b.diag, b.eps, b.rz_new, b.viol, b.ctl, K)
 #define DLP_DIR(PKV, PB)                                                                                     \
   hipLaunchKernelGGL(k_pcg_dir4<PKV>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,                  \
                      PBlock<PKV>{PB, b.P16 ? b.ldh / 4 : 0}, (const float4*)b.R, b.diag, b.beta, rp, b.ctl, sig, K)
     if (!b.P16) DLP_UPDATE(0, b.P);
     else if (b.p16 == 2) DLP_UPDATE(2, b.P16);
     else DLP_UPDATE(1, b.P16);
-    // rz_new += rc . Ac^-1 rc (the coarse part of r^T M r) and the correction the direction uses
+    // beta (from r^T M r: the update's r^T D^-1 r plus the coarse part), the convergence test,
+    // and the coarse correction the direction uses
     if (b.na)
-      hipLaunchKernelGGL(k_coarse_apply, dim3(b.C), dim3(kCoarseMax), 0, st, b.na, b.C, b.ldc, b.Ainv, b.rc, b.ec,
-                         b.rz_new, b.ctl);
-    hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 1, b.rz, b.rz_new, b.pAp, b.alpha,
-                       b.beta, b.viol, b.ctl, rp, sig, nullptr, 0);
+      hipLaunchKernelGGL(k_coarse_scalars, dim3(b.C), dim3(kCoarseMax), 0, st, b.na, b.C, b.ldc, b.Ainv, b.rc, b.ec,
+                         b.rz, b.rz_new, b.pAp, rp, b.beta, sig, b.viol, b.ctl);
+    else
+      hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 1, b.rz, b.rz_new, b.pAp, b.alpha,
+                         b.beta, b.viol, b.ctl, rp, sig, nullptr, 0);
     if (!b.P16) DLP_DIR(0, b.P);
     else if (b.p16 == 2) DLP_DIR(2, b.P16);
     else DLP_DIR(1, b.P16);

@@ -682,7 +682,7 @@ struct VerifyEpilogue {
 };
 
 template <int S>
-__global__ __launch_bounds__(256) void k_verify(RefreshArgs A, int xcd, int xbytes, const float* __restrict__ Z,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_verify(RefreshArgs A, int xcd, int xbytes, const float* __restrict__ Z,
                                                 float* __restrict__ R) {
   const RowPipe G(A.nrows, A.C1, 4, 16, xcd, A.solved, A.indptr, A.rowlen, A.col, A.w, Z, (unsigned)(A.ld * 4),
                   xbytes);
@@ -1872,7 +1872,7 @@ extern "C" int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int 
     DLP_CHECK_HIP(hipMalloc((void**)&h->P, nz * sizeof(float)));
     DLP_CHECK_HIP(hipMalloc((void**)&h->AP, nz * sizeof(float)));
     DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_scal, 7 * C1 * sizeof(double)));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_viol, 4 * sizeof(float)));   // viol, pad, done, iterations
+    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_viol, 4 * sizeof(float)));   // viol, last viol, done, iterations
     DLP_CHECK_HIP(hipMemsetAsync(h->P, 0, nz * sizeof(float), st));
     DLP_CHECK_HIP(hipMemsetAsync(h->AP, 0, nz * sizeof(float), st));
   }
@@ -1914,7 +1914,7 @@ extern "C" int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int 
     if (hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess) h->pcg_ms += ms;
     const volatile float* hv = reinterpret_cast<volatile float*>(h->host);
     const volatile int32_t* hc = reinterpret_cast<volatile int32_t*>(h->host) + 2;
-    v = hv[0];
+    v = hv[1];   // the last completed iteration's violation
     const int done = hc[0];
     const int64_t dev_it = hc[1];
     h->pcg_iters += dev_it - it;
@@ -2097,7 +2097,7 @@ extern "C" int dlp_lp_bpcg(void* handle, int max_iter, int iters_per_launch, int
     DLP_CHECK_HIP(hipMalloc((void**)&h->P, nz * sizeof(float)));
     DLP_CHECK_HIP(hipMalloc((void**)&h->AP, nz * sizeof(float)));
     DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_scal, 7 * C1 * sizeof(double)));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_viol, 4 * sizeof(float)));   // viol, pad, done, iterations
+    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_viol, 4 * sizeof(float)));   // viol, last viol, done, iterations
     DLP_CHECK_HIP(hipMemsetAsync(h->P, 0, nz * sizeof(float), st));
     DLP_CHECK_HIP(hipMemsetAsync(h->AP, 0, nz * sizeof(float), st));
   }
