@@ -1414,9 +1414,10 @@ extern "C" int dlp_lp_regrow_commit(void* handle, const int32_t* new_indptr, int
   return DLP_OK;
 }
 
-__global__ void k_ground_prep(int32_t* flags, int force) {
+__global__ void k_ground_prep(int32_t* flags, int force, int32_t* toggled_count) {
   if (threadIdx.x == 0) {
     flags[0] = force ? 1 : 0;
+    *toggled_count = 0;
   }
 }
 __global__ void k_ground_inert_reset(int32_t* flags) {
@@ -1426,8 +1427,7 @@ __global__ void k_ground_inert_reset(int32_t* flags) {
 static int ground_impl(LP* h, int force, int nd, const int32_t* du, const int32_t* dv, int ni, hipStream_t st) {
   const int n = h->d.n;
   int32_t* need = h->gflags;
-  DLP_CHECK_HIP(hipMemsetAsync(h->cnt + CNT_TOGGLED, 0, sizeof(int32_t), st));
-  hipLaunchKernelGGL(k_ground_prep, dim3(1), dim3(64), 0, st, h->gflags, force);
+  hipLaunchKernelGGL(k_ground_prep, dim3(1), dim3(64), 0, st, h->gflags, force, h->cnt + CNT_TOGGLED);
   if (!force && nd > 0) {
     hipLaunchKernelGGL(k_ground_ball, dim3((unsigned)((2LL * nd + 3) / 4)), dim3(kBlock), 0, st, nd, du, dv, ni,
                        h->d.indptr, h->d.rowlen, h->d.col, h->d.labeled, h->d.labcnt, h->gflags);
@@ -1480,14 +1480,23 @@ extern "C" int dlp_lp_ground_flags(void* handle, int32_t* out, void* stream) {
   return DLP_OK;
 }
 
+// Clears the refresh outputs (violator count, column maxima, push mass) in one launch.
+__global__ void k_refresh_reset(int32_t* __restrict__ nviol, double* __restrict__ rho, int C1,
+                                double* __restrict__ mass) {
+  for (int c = threadIdx.x; c < C1; c += blockDim.x) rho[c] = 0.0;
+  if (threadIdx.x == 0) {
+    *nviol = 0;
+    if (mass) *mass = 0.0;
+  }
+}
+
 extern "C" int dlp_lp_refresh(void* handle, int which, int prec, void* stream) {
   DLP_CHECK_ARG(handle && which >= 0 && which <= 3 && (prec == 0 || prec == 1), "lp_refresh: bad argument");
   LP* h = (LP*)handle;
   DLP_CHECK_ARG(prec == 0 || (h->d.Z64 && h->d.R64), "lp_refresh: FP64 state not attached");
   hipStream_t st = (hipStream_t)stream;
-  DLP_CHECK_HIP(hipMemsetAsync(h->cnt + CNT_VIOL, 0, sizeof(int32_t), st));
-  DLP_CHECK_HIP(hipMemsetAsync(h->rho, 0, sizeof(double) * h->d.C1, st));
-  if (which == 0) DLP_CHECK_HIP(hipMemsetAsync(h->mass, 0, sizeof(double), st));
+  hipLaunchKernelGGL(k_refresh_reset, dim3(1), dim3(kBlock), 0, st, h->cnt + CNT_VIOL, h->rho, h->d.C1,
+                     which == 0 ? h->mass : nullptr);
   const long long xbytes = (long long)h->d.n * h->d.ld * 4;
   if (which == 3) {
     // verification of every solved row with unchanged equations: the pipelined kernel when the

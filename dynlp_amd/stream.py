@@ -213,6 +213,7 @@ class EdgeStreamLP:
                       ptr(self._rho_dev), stream())
             self._rho[:] = self._rho_dev.cpu().numpy()
             return s
+        self._rho_all = False
         s = self.status()
         ok = False
         # push only when it can finish cheaply: few violators, and a residual mass (sum of the
@@ -232,9 +233,12 @@ class EdgeStreamLP:
             # refresh, so the verification pass (mode 3) suffices
             _lib.call("dlp_lp_refresh", self.h, 3, 0, stream())
             s = self.status()
-            if s["viol"]:
-                r, ok = self._push()
-                st.iters += r
+            if s["viol"] == 0:
+                # certified on every row: the pass's column maxima are rho over all rows
+                self._rho_all = True
+                return s
+            r, ok = self._push()
+            st.iters += r
         for it in range(8):
             _lib.call("dlp_lp_refresh", self.h, 1, 0, stream())
             s = self.status()
@@ -350,7 +354,8 @@ class EdgeStreamLP:
 
     def _finish(self, st, s):
         st.edges, st.touched, st.changed, st.toggled = s["edges"], s["touched"], s["changed"], s["toggled"]
-        st.rho = self.rho()
+        # after a clean verification of every row its column maxima are rho (no second pass)
+        st.rho = float(self._rho[: self.C].max()) if getattr(self, "_rho_all", False) else self.rho()
 
     def rho(self):
         """max over label columns and all rows of |R_uc|/diag_u of the kept residual (every row
