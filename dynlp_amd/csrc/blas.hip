@@ -198,6 +198,27 @@ __device__ __forceinline__ void flush_quads(const Quad& Qd, int Q4, int C, const
   for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&out[c], sm[c]);
 }
 
+// Loads and stores of the FP32 vector streams of the flat passes; NT marks them non-temporal so
+// the caches keep the direction block the next operator gathers.
+template <bool NT>
+__device__ __forceinline__ float4 ld4(const float4* p) {
+  if constexpr (NT) {
+    const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+  } else {
+    return *p;
+  }
+}
+template <bool NT>
+__device__ __forceinline__ void st4(float4* p, const float4& v) {
+  if constexpr (NT) {
+    const f4v u = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(u, reinterpret_cast<f4v*>(p));
+  } else {
+    *p = v;
+  }
+}
+
 // Two-level preconditioner arguments of the flat passes (na = 0: Jacobi only).
 struct CoarseArg {
   int na, S;
@@ -354,7 +375,7 @@ __global__ __launch_bounds__(256) void k_pcg_rz4(int n, int Q4, int C, const flo
 // directions, r.z otherwise; 0 where pAp <= 0); rz_new[c] += sum R*R/diag; viol = max
 // |R|/(eps_c*diag); with K also rc += P^T R (the new residual restricted to the aggregates).
 // Rows in a contiguous split.
-template <int PK>
+template <int PK, bool NT = false>
 __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float4* __restrict__ Z,
                                                      float4* __restrict__ R, PBlock<PK> P,
                                                      const float4* __restrict__ AP,
@@ -386,14 +407,14 @@ __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float
     for (int64_t r = q.r_lo + q.roff; r < q.r_hi; r += q.rstep) {
       const int64_t i = r * Q4 + q.q4;
       const float inv = 1.f / diag[r];
-      float4 z = Z[i], rv = R[i];
-      const float4 p = P.load(i, r, q.q4), ap = AP[i];
+      float4 z = ld4<NT>(Z + i), rv = ld4<NT>(R + i);
+      const float4 p = P.load(i, r, q.q4), ap = ld4<NT>(AP + i);
       if (q.cnt > 0) { z.x += a[0] * p.x; rv.x -= a[0] * ap.x; }
       if (q.cnt > 1) { z.y += a[1] * p.y; rv.y -= a[1] * ap.y; }
       if (q.cnt > 2) { z.z += a[2] * p.z; rv.z -= a[2] * ap.z; }
       if (q.cnt > 3) { z.w += a[3] * p.w; rv.w -= a[3] * ap.w; }
-      Z[i] = z;
-      R[i] = rv;
+      st4<NT>(Z + i, z);
+      st4<NT>(R + i, rv);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         if (j < q.cnt) {
@@ -421,7 +442,7 @@ __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float
 // also rp[c] += sum R*P as held (the next step length's numerator, exact for the rounded
 // direction).  FP16 blocks: P = sig*M R + beta*P with sig the new column scale and beta already
 // rescaled to the held direction.
-template <int PK>
+template <int PK, bool NT = false>
 __global__ __launch_bounds__(256) void k_pcg_dir4(int n, int Q4, int C, PBlock<PK> P, const float4* __restrict__ R,
                                                   const float* __restrict__ diag, const double* __restrict__ beta,
                                                   double* __restrict__ rp, const int32_t* __restrict__ skip,
@@ -446,7 +467,7 @@ __global__ __launch_bounds__(256) void k_pcg_dir4(int n, int Q4, int C, PBlock<P
     int64_t r = Qd.r0;
     for (int64_t i = Qd.i0; i < nq; i += Qd.step, r += Qd.rstep) {
       const float inv = 1.f / diag[r];
-      const float4 rv = R[i];
+      const float4 rv = ld4<NT>(R + i);
       const float4 ev = coarse_quad(K, r, Qd.q4);
       float4 p = P.load(i, r, Qd.q4);
       if constexpr (PK == 2) {
@@ -894,16 +915,23 @@ int pcg_finish_iteration(const PcgBufs& b, hipStream_t st, bool have_pap) {
     float* sig = (b.P16 && b.p16 == 2) ? b.sig : nullptr;
     // the update pass takes its step lengths from pAp and r.p (16-bit directions) or r.z
     const double* num = b.P16 ? b.rp : b.rz;
-#define DLP_UPDATE(PKV, PB)                                                                                  \
-  hipLaunchKernelGGL(k_pcg_update4<PKV>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (float4*)b.Z, \
-                     (float4*)b.R, PBlock<PKV>{PB, b.P16 ? b.ldh / 4 : 0}, (const float4*)b.AP, b.pAp, num,  \
-                     b.diag, b.eps, b.rz_new, b.viol, b.ctl, K)
-#define DLP_DIR(PKV, PB)                                                                                     \
-  hipLaunchKernelGGL(k_pcg_dir4<PKV>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,                  \
+    const bool nt = tuning().nt_streams != 0;
+#define DLP_UPDATE_NT(PKV, PB, NTV)                                                                          \
+  hipLaunchKernelGGL((k_pcg_update4<PKV, NTV>), dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,         \
+                     (float4*)b.Z, (float4*)b.R, PBlock<PKV>{PB, b.P16 ? b.ldh / 4 : 0}, (const float4*)b.AP,  \
+                     b.pAp, num, b.diag, b.eps, b.rz_new, b.viol, b.ctl, K)
+#define DLP_UPDATE(PKV, PB)              \
+  if (nt) DLP_UPDATE_NT(PKV, PB, true); \
+  else DLP_UPDATE_NT(PKV, PB, false)
+#define DLP_DIR_NT(PKV, PB, NTV)                                                                             \
+  hipLaunchKernelGGL((k_pcg_dir4<PKV, NTV>), dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,           \
                      PBlock<PKV>{PB, b.P16 ? b.ldh / 4 : 0}, (const float4*)b.R, b.diag, b.beta, rp, b.ctl, sig, K)
-    if (!b.P16) DLP_UPDATE(0, b.P);
-    else if (b.p16 == 2) DLP_UPDATE(2, b.P16);
-    else DLP_UPDATE(1, b.P16);
+#define DLP_DIR(PKV, PB)              \
+  if (nt) DLP_DIR_NT(PKV, PB, true); \
+  else DLP_DIR_NT(PKV, PB, false)
+    if (!b.P16) { DLP_UPDATE(0, b.P); }
+    else if (b.p16 == 2) { DLP_UPDATE(2, b.P16); }
+    else { DLP_UPDATE(1, b.P16); }
     // beta (from r^T M r: the update's r^T D^-1 r plus the coarse part), the convergence test,
     // and the coarse correction the direction uses
     if (b.na)
@@ -912,11 +940,13 @@ int pcg_finish_iteration(const PcgBufs& b, hipStream_t st, bool have_pap) {
     else
       hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 1, b.rz, b.rz_new, b.pAp, b.alpha,
                          b.beta, b.viol, b.ctl, rp, sig, nullptr, 0);
-    if (!b.P16) DLP_DIR(0, b.P);
-    else if (b.p16 == 2) DLP_DIR(2, b.P16);
-    else DLP_DIR(1, b.P16);
+    if (!b.P16) { DLP_DIR(0, b.P); }
+    else if (b.p16 == 2) { DLP_DIR(2, b.P16); }
+    else { DLP_DIR(1, b.P16); }
 #undef DLP_UPDATE
 #undef DLP_DIR
+#undef DLP_UPDATE_NT
+#undef DLP_DIR_NT
     DLP_CHECK_LAUNCH();
     return DLP_OK;
   }

@@ -18,6 +18,7 @@
 namespace dlp {
 
 typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float4 buf_load4(__amdgpu_buffer_rsrc_t rs, unsigned off) {
   const u4v r = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0);

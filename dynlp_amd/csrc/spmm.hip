@@ -748,7 +748,8 @@ extern "C" int dlp_tune_set(int key, int value) {
   // operator (0 = resident blocks); key 5: fuse p^T A p into the global solve's operator;
   // key 6: gathers in flight per lane in the pipelined global-solve operator (4, 6 or 8; 0 =
   // the plain fused operator); key 7: BF16 direction values per lane (4: 8-byte words, 8: 16);
-  // key 8: workgroups of the flat PCG vector passes (0 = 2048)
+  // key 8: workgroups of the flat PCG vector passes (0 = 2048); key 9: non-temporal FP32 streams
+  // in the PCG update and direction passes (1 default)
   if (key == 0) {
     DLP_CHECK_ARG(value == 0 || value == 4 || value == 8, "tune: unroll must be 0, 4 or 8");
     tuning().unroll = value;
@@ -773,6 +774,9 @@ extern "C" int dlp_tune_set(int key, int value) {
   } else if (key == 8) {
     DLP_CHECK_ARG(value >= 0 && value <= (1 << 20), "tune: flat pass grid out of range");
     tuning().flat_grid = value;
+  } else if (key == 9) {
+    tuning().nt_streams = value ? 1 : 0;
+
 
   } else {
     DLP_CHECK_ARG(false, "tune: unknown key %d", key);

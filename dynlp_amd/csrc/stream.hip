@@ -81,27 +81,42 @@ __global__ void k_compact_rows(const int32_t* __restrict__ nrows, const int32_t*
 }
 
 // Directed insertion e < 2*ni: upsert (a, b, wt).  A full row records e as overflow.
-__global__ void k_apply_insert(int ni, const int32_t* __restrict__ iu, const int32_t* __restrict__ iv,
-                               const float* __restrict__ iw, const int32_t* __restrict__ indptr,
-                               int32_t* __restrict__ rowlen, int32_t* __restrict__ col,
-                               float* __restrict__ w, const uint8_t* __restrict__ labeled,
-                               int32_t* __restrict__ labcnt, int32_t* __restrict__ stamp, int stampv,
-                               int32_t* __restrict__ rows, int32_t* __restrict__ nrows,
-                               int32_t* __restrict__ overflow, int32_t* __restrict__ noverflow) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= 2 * (int64_t)ni) return;
+// One wavefront per directed insertion (2*ni of them): the lanes read the row's live entries
+// in parallel; an existing pair takes the new weight, else one lane claims a slot (pairs are
+// distinct within a batch, so concurrent insertions into one row never look for each other).
+__global__ __launch_bounds__(256) void k_apply_insert(int ni, const int32_t* __restrict__ iu,
+                                                      const int32_t* __restrict__ iv, const float* __restrict__ iw,
+                                                      const int32_t* __restrict__ indptr,
+                                                      int32_t* __restrict__ rowlen, int32_t* __restrict__ col,
+                                                      float* __restrict__ w, const uint8_t* __restrict__ labeled,
+                                                      int32_t* __restrict__ labcnt, int32_t* __restrict__ stamp,
+                                                      int stampv, int32_t* __restrict__ rows,
+                                                      int32_t* __restrict__ nrows, int32_t* __restrict__ overflow,
+                                                      int32_t* __restrict__ noverflow) {
+  const int64_t e = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (e >= 2 * (int64_t)ni) return;   // wave-uniform
   const int a = e < ni ? iu[e] : iv[e - ni];
   const int b = e < ni ? iv[e] : iu[e - ni];
   const float wt = e < ni ? iw[e] : iw[e - ni];
   if (a == b) return;
   const int beg = indptr[a], cap = indptr[a + 1] - beg;
-  const int len = __hip_atomic_load(rowlen + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (int j = beg; j < beg + len; ++j) {
-    if (col[j] == b) {  // existing pair: new weight
-      w[j] = wt;
-      stamp_append(a, stamp, stampv, rows, nrows);
-      return;
+  // a concurrent insertion that overflows the row raises rowlen past cap for a moment
+  const int len = min(cap, __hip_atomic_load(rowlen + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  int hit = -1;
+  for (int j0 = 0; j0 < len; j0 += 64) {
+    const int j = j0 + lane;
+    const unsigned long long m = __ballot(j < len && col[beg + j] == b);
+    if (m) {
+      hit = j0 + __ffsll((long long)m) - 1;
+      break;
     }
+  }
+  if (lane != 0) return;
+  if (hit >= 0) {   // existing pair: new weight
+    w[beg + hit] = wt;
+    stamp_append(a, stamp, stampv, rows, nrows);
+    return;
   }
   const int slot = atomicAdd(rowlen + a, 1);
   if (slot < cap) {
@@ -1316,7 +1331,7 @@ extern "C" int dlp_lp_apply(void* handle, int nd, const int32_t* du, const int32
     DLP_CHECK_LAUNCH();
   }
   if (ni > 0) {
-    hipLaunchKernelGGL(k_apply_insert, dim3(grid_for(2LL * ni)), dim3(kBlock), 0, st, ni, iu, iv, iw,
+    hipLaunchKernelGGL(k_apply_insert, dim3(grid_for(2LL * ni * 64)), dim3(kBlock), 0, st, ni, iu, iv, iw,
                        h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.labeled, h->d.labcnt,
                        h->stamp, h->stampv, h->rows, h->cnt + CNT_ROWS, h->overflow,
                        h->cnt + CNT_OVERFLOW);

@@ -81,7 +81,9 @@ int dlp_row_operator_dot(int n, int C, const int32_t* indptr, const int32_t* row
  *   key 6 = gathers in flight per lane in the global solve's pipelined operator (4, 6 or 8, 4
  *           default; 0 = the plain fused operator);
  *   key 7 = BF16 direction values per lane of that operator (4 default: 8-byte words; 8);
- *   key 8 = workgroups of the flat PCG vector passes (0 default = 2048). */
+ *   key 8 = workgroups of the flat PCG vector passes (0 default = 2048);
+ *   key 9 = non-temporal FP32 streams (Z, R, AP) in the PCG update and direction passes
+ *           (1 default: the caches keep the direction block the next operator gathers). */
 int dlp_tune_set(int key, int value);
 
 /* ---- Column residual ratios ------------------------------------------------------------
