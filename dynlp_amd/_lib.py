@@ -66,6 +66,7 @@ _SIGS = {
     "dlp_lp_scores": (ct.c_int, [P, P, I64, P]),
     "dlp_lp_set_pcg_directions": (ct.c_int, [P, I32]),
     "dlp_lp_set_coarse": (ct.c_int, [P, I32, I32]),
+    "dlp_lp_coarse_info": (ct.c_int, [P, P, P, P]),
     "dlp_tune_set": (ct.c_int, [I32, I32]),
     "dlp_row_operator_dot": (ct.c_int, [I32, I32, P, P, P, P, P, P, I64, P, FLT, P, I64, P, P]),
     "dlp_coldot": (ct.c_int, [I32, I32, P, I64, P, I64, P, P, I32, P]),

@@ -29,10 +29,13 @@ struct PcgBufs {
   double* rp;
   int p16;
   float* sig;
-  // two-level preconditioner M = D^-1 + P Ac^-1 P^T (na = 0: Jacobi only): aggregates of S
-  // consecutive rows, P[r, r/S] = 1 on solved rows; rc = P^T r (na x ldc doubles), ec = the
-  // coarse correction Ac^-1 rc (na x ldc floats), Ainv = Ac^-1 (na x na), solved = row mask
-  int na, S;
+  // two-level preconditioner M = D^-1 + P Ac^-1 P^T (na = 0: Jacobi only): aggregate I is the
+  // row range [bounds[I], bounds[I+1]) (at least min_agg rows), aggmap[r] = the aggregate of row
+  // r, P[r, aggmap[r]] = 1 on solved rows; rc = P^T r (na x ldc doubles), ec = the coarse
+  // correction Ac^-1 rc (na x ldc floats), Ainv = Ac^-1 (na x na), solved = row mask
+  int na, min_agg;
+  const int32_t* bounds;
+  const uint8_t* aggmap;
   int64_t ldc;
   double* rc;
   float* ec;
@@ -44,12 +47,23 @@ struct PcgBufs {
 // apply): the coarse aggregates must be at least this long.
 int pcg_flat_chunk(const PcgBufs& b);
 
-// Ac = P^T A P of the solved rows (aggregates of S consecutive rows) into Ac (na x na), then
-// Ainv = Ac^-1 (FP64 Cholesky in one workgroup, na <= kCoarseMax).
+// Ac = P^T A P of the solved rows (aggregates = row ranges of bounds, aggmap) into Ac (na x na),
+// then Ainv = Ac^-1 (FP64 Cholesky in one workgroup, na <= kCoarseMax).
 constexpr int kCoarseMax = 128;
-int coarse_build(int n, int na, int S, const int32_t* indptr, const int32_t* rowlen, const int32_t* col,
-                 const float* w, const uint8_t* solved, const double* diag, double* Ac, double* Ainv,
-                 hipStream_t st);
+int coarse_build(int n, int na, const int32_t* bounds, const uint8_t* aggmap, const int32_t* indptr,
+                 const int32_t* rowlen, const int32_t* col, const float* w, const uint8_t* solved,
+                 const double* diag, double* Ac, double* Ainv, hipStream_t st);
+
+// Candidate aggregate boundaries: cross[b] = edges (u < v) with u < b <= v, then for every
+// k in [1, nk] the b within [k*S0 - S0/2, k*S0 + S0/2) (clamped to [min_agg, n - min_agg)) with
+// the fewest crossing edges (ties: nearest to k*S0) into cand[4(k-1)] (-1: empty window),
+// followed by cross[b], cross[b - S0/2] and cross[b + S0/2].  diff and cross hold n ints each;
+// ws/ws_bytes: scan workspace (scan_workspace_bytes(n)); cand holds 4 nk ints.
+int coarse_boundaries(int n, int nk, int S0, int min_agg, const int32_t* indptr, const int32_t* rowlen,
+                      const int32_t* col, int32_t* diff, int32_t* cross, void* ws, size_t ws_bytes,
+                      int32_t* cand, hipStream_t st);
+// aggmap[r] = I with bounds[I] <= r < bounds[I+1].
+int coarse_aggmap(int n, int na, const int32_t* bounds, uint8_t* aggmap, hipStream_t st);
 
 // P = R/diag, rz = sum R*R/diag, pAp = 0.
 int pcg_start(const PcgBufs& b, hipStream_t st);

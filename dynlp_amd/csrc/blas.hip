@@ -5,6 +5,7 @@
 // in FP64: per lane, then per workgroup in LDS, then one FP64 atomic per column per workgroup.
 #include "blas.h"
 #include "rowpipe.h"
+#include "scan.h"
 
 namespace dlp {
 
@@ -221,11 +222,13 @@ __device__ __forceinline__ void st4(float4* p, const float4& v) {
 
 // Two-level preconditioner arguments of the flat passes (na = 0: Jacobi only).
 struct CoarseArg {
-  int na, S;
+  int na;
   int64_t ldc;
   double* rc;               // P^T r, na x ldc (restricting passes)
   const float* ec;          // Ac^-1 P^T r, na x ldc (prolonging passes)
   const uint8_t* solved;
+  const int32_t* bounds;    // aggregate I = rows [bounds[I], bounds[I+1])
+  const uint8_t* aggmap;    // aggregate of every row
 };
 
 // Rows of this workgroup in a contiguous split of n rows over the grid, [r_lo, r_hi); a thread
@@ -250,14 +253,18 @@ __device__ __forceinline__ QuadRange quad_range(int Q4, int C, int64_t n) {
   return q;
 }
 
-// Column sums of a workgroup's rows per aggregate (r / S): the caller keeps the workgroup's
-// rows within two aggregates (rows per workgroup <= S), so a thread holds two quads of sums.
+// Column sums of a workgroup's rows per aggregate: the caller keeps the workgroup's rows within
+// two aggregates (rows per workgroup <= the smallest aggregate), so a thread holds two quads.
 struct Restrict {
   int64_t A0, bnd;
   double a0[4], a1[4];
-  __device__ __forceinline__ void init(int64_t r_lo, int S) {
-    A0 = r_lo / S;
-    bnd = (A0 + 1) * (int64_t)S;
+  __device__ __forceinline__ void init(int64_t r_lo, int64_t n, const CoarseArg& K) {
+    A0 = 0;
+    bnd = INT64_MAX;
+    if (K.na && r_lo < n) {
+      A0 = K.aggmap[r_lo];
+      bnd = K.bounds[A0 + 1];
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) a0[j] = a1[j] = 0.0;
   }
@@ -288,7 +295,7 @@ struct Restrict {
 // The coarse correction of row r (column quad q4), zero outside the solved set.
 __device__ __forceinline__ float4 coarse_quad(const CoarseArg& K, int64_t r, int q4) {
   if (K.na == 0 || !K.solved[r]) return make_float4(0.f, 0.f, 0.f, 0.f);
-  return reinterpret_cast<const float4*>(K.ec + (r / K.S) * K.ldc)[q4];
+  return reinterpret_cast<const float4*>(K.ec + (int64_t)K.aggmap[r] * K.ldc)[q4];
 }
 
 // pcg start: P = M R (M R = R/diag, plus the coarse correction on solved rows), rz[c] += sum
@@ -346,7 +353,7 @@ __global__ __launch_bounds__(256) void k_pcg_rz4(int n, int Q4, int C, const flo
   __syncthreads();
   const QuadRange q = quad_range(Q4, C, n);
   Restrict rs;
-  rs.init(q.r_lo, K.na ? K.S : 1);
+  rs.init(q.r_lo, n, K);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   if (q.active) {
     for (int64_t r = q.r_lo + q.roff; r < q.r_hi; r += q.rstep) {
@@ -392,7 +399,7 @@ __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float
   __syncthreads();
   const QuadRange q = quad_range(Q4, C, n);
   Restrict rs;
-  rs.init(q.r_lo, K.na ? K.S : 1);
+  rs.init(q.r_lo, n, K);
   float a[4], ie[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -666,11 +673,12 @@ __global__ __launch_bounds__(kCoarseMax) void k_coarse_scalars(int na, int C, in
 }
 
 // Ac += P^T A P over a share of aggregate I's rows (workgroup I * parts + part): for every
-// solved row r of aggregate I, diag_r at column I and -w_rs at column s/S for every solved
-// neighbour s.  Entries of aggregate I itself (most of them: neighbours are mostly near in
-// row order) are summed in registers; the others go through an LDS row; the row is added to Ac
-// with one atomic per nonzero entry.  Ac must be zero beforehand.
-__global__ __launch_bounds__(256) void k_coarse_build(int n, int na, int S, int parts,
+// solved row r of aggregate I, diag_r at column I and -w_rs at column aggmap[s] for every solved
+// neighbour s.  Entries of aggregate I itself (most of them: neighbours are mostly near in row
+// order) are summed in registers; the others go through an LDS row; the row is added to Ac with
+// one atomic per nonzero entry.  Ac must be zero beforehand.
+__global__ __launch_bounds__(256) void k_coarse_build(int n, int na, int parts, const int32_t* __restrict__ bounds,
+                                                      const uint8_t* __restrict__ aggmap,
                                                       const int32_t* __restrict__ indptr,
                                                       const int32_t* __restrict__ rowlen,
                                                       const int32_t* __restrict__ col, const float* __restrict__ w,
@@ -680,7 +688,7 @@ __global__ __launch_bounds__(256) void k_coarse_build(int n, int na, int S, int 
   for (int t = threadIdx.x; t < na; t += blockDim.x) row[t] = 0.0;
   __syncthreads();
   const int I = blockIdx.x / parts, part = blockIdx.x - I * parts;
-  const int64_t lo0 = (int64_t)I * S, hi0 = min((int64_t)n, lo0 + S);
+  const int64_t lo0 = bounds[I], hi0 = bounds[I + 1];
   const int64_t span = (hi0 - lo0 + parts - 1) / parts;
   const int64_t lo = min(hi0, lo0 + part * span), hi = min(hi0, lo + span);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
@@ -692,7 +700,7 @@ __global__ __launch_bounds__(256) void k_coarse_build(int n, int na, int S, int 
     for (int j = beg + lane; j < end; j += 64) {
       const int sc = col[j];
       if (sc >= 0 && sc < n && solved[sc]) {
-        const int J = sc / S;
+        const int J = aggmap[sc];
         if (J == I) own -= (double)w[j];
         else lds_add(&row[J], -(double)w[j]);
       }
@@ -703,6 +711,79 @@ __global__ __launch_bounds__(256) void k_coarse_build(int n, int na, int S, int 
   __syncthreads();
   for (int t = threadIdx.x; t < na; t += blockDim.x)
     if (row[t] != 0.0) atomicAdd(&Ac[(int64_t)I * na + t], row[t]);
+}
+
+// diff[u] += (live entries v > u of row u), diff[v] -= 1 for each of them: the exclusive prefix
+// sum of diff is the number of edges crossing each row boundary.
+__global__ void k_cross_diff(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ rowlen,
+                             const int32_t* __restrict__ col, int32_t* __restrict__ diff) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n) return;
+  const int beg = indptr[u], end = beg + rowlen[u];
+  int cnt = 0;
+  for (int j = beg; j < end; ++j) {
+    const int v = col[j];
+    if (v > u && v < n) {
+      ++cnt;
+      atomicSub(diff + v, 1);
+    }
+  }
+  if (cnt) atomicAdd(diff + u, cnt);
+}
+
+// cand[4(k-1) ..] = the boundary b with the fewest crossing edges in target k's window (ties:
+// nearest to the target, then smaller; -1 when the window is empty), its crossing count, and
+// the counts half a spacing to either side of it (clamped to the row range).  One workgroup per
+// target.
+__global__ __launch_bounds__(256) void k_window_argmin(int n, int S0, int min_agg, const int32_t* __restrict__ cross,
+                                                       int32_t* __restrict__ cand) {
+  __shared__ long long bc[256], bd[256], bb[256];
+  const int k = blockIdx.x + 1;
+  const long long t = (long long)k * S0;
+  const long long lo = max((long long)min_agg, t - S0 / 2), hi = min((long long)n - min_agg, t + S0 / 2);
+  long long c0 = LLONG_MAX, d0 = LLONG_MAX, b0 = -1;
+  for (long long b = lo + threadIdx.x; b < hi; b += blockDim.x) {
+    const long long c = cross[b], d = b > t ? b - t : t - b;
+    if (c < c0 || (c == c0 && (d < d0 || (d == d0 && b < b0)))) {
+      c0 = c;
+      d0 = d;
+      b0 = b;
+    }
+  }
+  bc[threadIdx.x] = c0;
+  bd[threadIdx.x] = d0;
+  bb[threadIdx.x] = b0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < (int)blockDim.x; ++i) {
+      if (bb[i] < 0) continue;
+      if (b0 < 0 || bc[i] < c0 || (bc[i] == c0 && (bd[i] < d0 || (bd[i] == d0 && bb[i] < b0)))) {
+        c0 = bc[i];
+        d0 = bd[i];
+        b0 = bb[i];
+      }
+    }
+    int32_t* o = cand + 4 * (k - 1);
+    o[0] = (int32_t)b0;
+    if (b0 >= 0) {
+      const long long bl = max(0LL, b0 - S0 / 2), br = min((long long)n - 1, b0 + S0 / 2);
+      o[1] = cross[b0];
+      o[2] = cross[bl];
+      o[3] = cross[br];
+    }
+  }
+}
+
+__global__ void k_aggmap(int n, int na, const int32_t* __restrict__ bounds, uint8_t* __restrict__ aggmap) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int lo = 0, hi = na;   // bounds[lo] <= r < bounds[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (bounds[mid] <= r) lo = mid;
+    else hi = mid;
+  }
+  aggmap[r] = (uint8_t)lo;
 }
 
 // Aggregates without solved rows get a unit diagonal (decoupled from the rest).
@@ -796,16 +877,16 @@ int pcg_flat_chunk(const PcgBufs& b) {
 }
 
 static CoarseArg coarse_arg(const PcgBufs& b) {
-  if (!b.na) return CoarseArg{0, 1, 0, nullptr, nullptr, nullptr};
-  return CoarseArg{b.na, b.S, b.ldc, b.rc, b.ec, b.solved};
+  if (!b.na) return CoarseArg{0, 0, nullptr, nullptr, nullptr, nullptr, nullptr};
+  return CoarseArg{b.na, b.ldc, b.rc, b.ec, b.solved, b.bounds, b.aggmap};
 }
 
 // The restriction of the flat passes keeps a workgroup's rows within two aggregates.
 static int check_coarse(const PcgBufs& b, const FlatLaunch& f) {
   if (!b.na) return DLP_OK;
-  DLP_CHECK_ARG(f.ok && b.na <= kCoarseMax && b.rc && b.ec && b.Ainv && b.solved && b.ldc >= 4 * f.Q4 &&
-                    (int64_t)b.na * b.S >= b.n && ((int64_t)b.n + f.grid - 1) / f.grid <= b.S,
-                "pcg: coarse layout (na=%d S=%d n=%d grid=%u)", b.na, b.S, b.n, f.grid);
+  DLP_CHECK_ARG(f.ok && b.na <= kCoarseMax && b.rc && b.ec && b.Ainv && b.solved && b.bounds && b.aggmap &&
+                    b.ldc >= 4 * f.Q4 && ((int64_t)b.n + f.grid - 1) / f.grid <= b.min_agg,
+                "pcg: coarse layout (na=%d min_agg=%d n=%d grid=%u)", b.na, b.min_agg, b.n, f.grid);
   return DLP_OK;
 }
 
@@ -879,19 +960,38 @@ __global__ void k_symmetrize(int na, double* __restrict__ A) {
   }
 }
 
-int coarse_build(int n, int na, int S, const int32_t* indptr, const int32_t* rowlen, const int32_t* col,
-                 const float* w, const uint8_t* solved, const double* diag, double* Ac, double* Ainv,
-                 hipStream_t st) {
-  DLP_CHECK_ARG(n >= 1 && na >= 1 && na <= kCoarseMax && S >= 1 && (int64_t)na * S >= n,
-                "coarse_build: na=%d S=%d n=%d", na, S, n);
+int coarse_build(int n, int na, const int32_t* bounds, const uint8_t* aggmap, const int32_t* indptr,
+                 const int32_t* rowlen, const int32_t* col, const float* w, const uint8_t* solved,
+                 const double* diag, double* Ac, double* Ainv, hipStream_t st) {
+  DLP_CHECK_ARG(n >= 1 && na >= 1 && na <= kCoarseMax && bounds && aggmap, "coarse_build: na=%d n=%d", na, n);
   const int parts = 8;
   hipLaunchKernelGGL(k_zero_f64, dim3(grid_for((long long)na * na)), dim3(kBlock), 0, st, na * na, Ac);
-  hipLaunchKernelGGL(k_coarse_build, dim3(na * parts), dim3(kBlock), 0, st, n, na, S, parts, indptr, rowlen, col, w,
-                     solved, diag, Ac);
+  hipLaunchKernelGGL(k_coarse_build, dim3(na * parts), dim3(kBlock), 0, st, n, na, parts, bounds, aggmap, indptr,
+                     rowlen, col, w, solved, diag, Ac);
   hipLaunchKernelGGL(k_coarse_fixdiag, dim3(1), dim3(kCoarseMax), 0, st, na, Ac);
   hipLaunchKernelGGL(k_coarse_factor, dim3(1), dim3(kBlock), sizeof(double) * na * na, st, na, Ac, Ac);
   hipLaunchKernelGGL(k_coarse_inverse, dim3(na), dim3(kBlock), 0, st, na, Ac, Ainv);
   hipLaunchKernelGGL(k_symmetrize, dim3(grid_for((long long)na * na)), dim3(kBlock), 0, st, na, Ainv);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+int coarse_boundaries(int n, int nk, int S0, int min_agg, const int32_t* indptr, const int32_t* rowlen,
+                      const int32_t* col, int32_t* diff, int32_t* cross, void* ws, size_t ws_bytes,
+                      int32_t* cand, hipStream_t st) {
+  DLP_CHECK_ARG(n >= 1 && nk >= 1 && S0 >= 1 && min_agg >= 1, "coarse_boundaries: bad sizes");
+  DLP_CHECK_HIP(hipMemsetAsync(diff, 0, sizeof(int32_t) * n, st));
+  hipLaunchKernelGGL(k_cross_diff, dim3(grid_for(n)), dim3(kBlock), 0, st, n, indptr, rowlen, col, diff);
+  int rc = exclusive_scan_i32(n, diff, cross, nullptr, ws, ws_bytes, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_window_argmin, dim3(nk), dim3(kBlock), 0, st, n, S0, min_agg, cross, cand);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+int coarse_aggmap(int n, int na, const int32_t* bounds, uint8_t* aggmap, hipStream_t st) {
+  DLP_CHECK_ARG(n >= 1 && na >= 1 && na <= 256, "coarse_aggmap: na=%d", na);
+  hipLaunchKernelGGL(k_aggmap, dim3(grid_for(n)), dim3(kBlock), 0, st, n, na, bounds, aggmap);
   DLP_CHECK_LAUNCH();
   return DLP_OK;
 }

@@ -12,6 +12,8 @@
 #include "scan.h"
 #include "blas.h"
 #include "uf.h"
+#include <algorithm>
+#include <vector>
 
 namespace dlp {
 
@@ -1041,11 +1043,17 @@ struct LP {
   // two-level preconditioner of the column PCG: coarse_na aggregates requested (0: Jacobi
   // only), rebuilt every coarse_refresh global solves; cg_* = the current layout and buffers
   int coarse_na = 0, coarse_refresh = 64;
-  int cg_na = 0, cg_S = 0, cg_age = -1;
-  double* cg_Ac = nullptr;
+  int cg_na = 0, cg_min = 0, cg_age = -1;
+  double* cg_Ac = nullptr;      // buffers sized for kCoarseMax aggregates
   double* cg_Ainv = nullptr;
   double* cg_rc = nullptr;
   float* cg_ec = nullptr;
+  int32_t* cg_bounds = nullptr; // kCoarseMax + 1
+  uint8_t* cg_aggmap = nullptr; // n
+  int32_t* cg_diff = nullptr;   // n, then the crossing counts (n)
+  int32_t* cg_cand = nullptr;   // kCoarseMax candidate boundaries, 4 ints each
+  void* cg_ws = nullptr;        // scan workspace
+  size_t cg_ws_bytes = 0;
   float* pcg_viol = nullptr;
   hipGraphExec_t pcg_exec = nullptr;
   int pcg_graph_iters = 0;
@@ -1119,6 +1127,11 @@ int lp_free(LP* h) {
   if (h->cg_Ainv) (void)hipFree(h->cg_Ainv);
   if (h->cg_rc) (void)hipFree(h->cg_rc);
   if (h->cg_ec) (void)hipFree(h->cg_ec);
+  if (h->cg_bounds) (void)hipFree(h->cg_bounds);
+  if (h->cg_aggmap) (void)hipFree(h->cg_aggmap);
+  if (h->cg_diff) (void)hipFree(h->cg_diff);
+  if (h->cg_cand) (void)hipFree(h->cg_cand);
+  if (h->cg_ws) (void)hipFree(h->cg_ws);
   if (h->pcg_scal) (void)hipFree(h->pcg_scal);
   if (h->pcg_viol) (void)hipFree(h->pcg_viol);
   if (h->pcg_exec) (void)hipGraphExecDestroy(h->pcg_exec);
@@ -1766,7 +1779,9 @@ static PcgBufs pcg_bufs(LP* h) {
   b.sig = reinterpret_cast<float*>(h->pcg_scal + 6 * C1);
   if (h->cg_na && h->cg_age >= 0) {
     b.na = h->cg_na;
-    b.S = h->cg_S;
+    b.min_agg = h->cg_min;
+    b.bounds = h->cg_bounds;
+    b.aggmap = h->cg_aggmap;
     b.ldc = h->d.ld;
     b.rc = h->cg_rc;
     b.ec = h->cg_ec;
@@ -1791,63 +1806,103 @@ static int pcg_iteration(LP* h, const PcgBufs& b, hipStream_t st) {
   return pcg_finish_iteration(b, st);
 }
 
-// The coarse operator of the two-level preconditioner: layout (aggregates of S consecutive rows,
-// at least the row span of one workgroup of the flat passes), buffers, and Ac^-1 rebuilt from the
-// current equations every coarse_refresh global solves (a stale Ac still gives an SPD
-// preconditioner, only a weaker one: the equations of a few percent of the rows change between
-// rebuilds).  Changing the layout drops the captured iterations, which hold its pointers.
+// The coarse operator of the two-level preconditioner, rebuilt from the current graph and
+// equations every coarse_refresh global solves (a stale Ac still gives an SPD preconditioner,
+// only a weaker one: the equations of a few percent of the rows change between rebuilds).
+// Aggregates are row ranges: coarse_na targets spaced S0 rows apart, each moved to the row
+// boundary with the fewest crossing edges within half a spacing; a moved boundary is kept when it
+// cuts fewer edges than the rows half a spacing to either side of it (on a graph whose ids follow
+// its communities these are the community edges, where the coarse space works best; a target
+// inside a community is dropped), all of them when under a quarter qualify (no such structure);
+// ranges hold at least the rows one workgroup of the vector passes covers.  A new
+// aggregate count drops the captured iterations, which hold it as a launch argument.
 static int coarse_prepare(LP* h, hipStream_t st) {
   const int n = h->d.n;
-  int na = 0, S = 0;
+  int chunk = 0;
   if (h->coarse_na > 0 && n >= 64) {
     PcgBufs probe = pcg_bufs(h);
     probe.na = 0;
-    const int chunk = pcg_flat_chunk(probe);
-    if (chunk > 0) {
-      na = h->coarse_na < kCoarseMax ? h->coarse_na : kCoarseMax;
-      S = (n + na - 1) / na;
-      if (S < chunk) S = chunk;
-      if (S < 32) S = 32;
-      na = (n + S - 1) / S;
+    chunk = pcg_flat_chunk(probe);
+  }
+  if (chunk <= 0) {
+    if (h->cg_na) {
+      if (h->pcg_exec) {
+        (void)hipGraphExecDestroy(h->pcg_exec);
+        h->pcg_exec = nullptr;
+      }
+      h->cg_na = 0;
+    }
+    h->cg_age = -1;
+    return DLP_OK;
+  }
+  if (!h->cg_Ac) {
+    const size_t ldc = (size_t)h->d.ld;
+    DLP_CHECK_HIP(hipMalloc((void**)&h->cg_Ac, sizeof(double) * kCoarseMax * kCoarseMax));
+    DLP_CHECK_HIP(hipMalloc((void**)&h->cg_Ainv, sizeof(double) * kCoarseMax * kCoarseMax));
+    DLP_CHECK_HIP(hipMalloc((void**)&h->cg_rc, sizeof(double) * kCoarseMax * ldc));
+    DLP_CHECK_HIP(hipMalloc((void**)&h->cg_ec, sizeof(float) * kCoarseMax * ldc));
+    DLP_CHECK_HIP(hipMalloc((void**)&h->cg_bounds, sizeof(int32_t) * (kCoarseMax + 1)));
+    DLP_CHECK_HIP(hipMalloc((void**)&h->cg_aggmap, (size_t)n));
+    DLP_CHECK_HIP(hipMalloc((void**)&h->cg_diff, sizeof(int32_t) * 2 * (size_t)n));
+    DLP_CHECK_HIP(hipMalloc((void**)&h->cg_cand, sizeof(int32_t) * 4 * kCoarseMax));
+    h->cg_ws_bytes = scan_workspace_bytes(n);
+    DLP_CHECK_HIP(hipMalloc(&h->cg_ws, h->cg_ws_bytes));
+    DLP_CHECK_HIP(hipMemsetAsync(h->cg_rc, 0, sizeof(double) * kCoarseMax * ldc, st));
+    DLP_CHECK_HIP(hipMemsetAsync(h->cg_ec, 0, sizeof(float) * kCoarseMax * ldc, st));
+  }
+  if (h->cg_age >= 0 && h->cg_age < h->coarse_refresh) {
+    h->cg_age += 1;
+    return DLP_OK;
+  }
+  // boundaries: targets k*S0, each moved to the lowest-cut row boundary near it
+  const int min_agg = chunk > 32 ? chunk : 32;
+  const int want = h->coarse_na < kCoarseMax ? h->coarse_na : kCoarseMax;
+  int S0 = (n + want - 1) / want;
+  if (S0 < min_agg) S0 = min_agg;
+  const int nk = (n - 1) / S0;   // targets k*S0 < n, k >= 1
+  std::vector<int32_t> bounds(1, 0);
+  if (nk >= 1) {
+    int rc = coarse_boundaries(n, nk, S0, min_agg, h->d.indptr, h->d.rowlen, h->d.col, h->cg_diff,
+                               h->cg_diff + n, h->cg_ws, h->cg_ws_bytes, h->cg_cand, st);
+    if (rc) return rc;
+    std::vector<int32_t> c4(4 * (size_t)nk);
+    DLP_CHECK_HIP(hipMemcpyAsync(c4.data(), h->cg_cand, sizeof(int32_t) * c4.size(), hipMemcpyDeviceToHost, st));
+    DLP_CHECK_HIP(hipStreamSynchronize(st));
+    // keep the boundaries that cut fewer edges than the rows half a spacing to either side (a
+    // community edge, not a slope the window ran down); without such structure keep them all
+    std::vector<int32_t> dip, all;
+    for (int k = 0; k < nk; ++k) {
+      const int32_t* o = &c4[4 * (size_t)k];
+      if (o[0] < 0) continue;
+      all.push_back(o[0]);
+      if (o[1] < o[2] && o[1] < o[3]) dip.push_back(o[0]);
+    }
+    std::vector<int32_t>& cand = (int)dip.size() * 4 >= nk ? dip : all;
+    std::sort(cand.begin(), cand.end());
+    for (const int32_t b : cand) {
+      if (b - bounds.back() >= min_agg && n - b >= min_agg && (int)bounds.size() < kCoarseMax) bounds.push_back(b);
     }
   }
-  if (na != h->cg_na || S != h->cg_S) {
+  bounds.push_back(n);
+  const int na = (int)bounds.size() - 1;
+  DLP_CHECK_HIP(hipMemcpyAsync(h->cg_bounds, bounds.data(), sizeof(int32_t) * bounds.size(), hipMemcpyHostToDevice,
+                               st));
+  int rc = coarse_aggmap(n, na, h->cg_bounds, h->cg_aggmap, st);
+  if (rc) return rc;
+  rc = coarse_build(n, na, h->cg_bounds, h->cg_aggmap, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.solved,
+                    h->d.diag64, h->cg_Ac, h->cg_Ainv, st);
+  if (rc) return rc;
+  // the host vector goes out of scope: finish the copy first
+  DLP_CHECK_HIP(hipStreamSynchronize(st));
+  if (na != h->cg_na || h->cg_age < 0 || min_agg != h->cg_min) {
     if (h->pcg_exec) {
       (void)hipGraphExecDestroy(h->pcg_exec);
       h->pcg_exec = nullptr;
     }
-    if (h->cg_Ac) (void)hipFree(h->cg_Ac);
-    if (h->cg_Ainv) (void)hipFree(h->cg_Ainv);
-    if (h->cg_rc) (void)hipFree(h->cg_rc);
-    if (h->cg_ec) (void)hipFree(h->cg_ec);
-    h->cg_Ac = h->cg_Ainv = h->cg_rc = nullptr;
-    h->cg_ec = nullptr;
-    h->cg_na = na;
-    h->cg_S = S;
-    h->cg_age = -1;
-    if (na) {
-      const size_t ldc = (size_t)h->d.ld;
-      DLP_CHECK_HIP(hipMalloc((void**)&h->cg_Ac, sizeof(double) * na * na));
-      DLP_CHECK_HIP(hipMalloc((void**)&h->cg_Ainv, sizeof(double) * na * na));
-      DLP_CHECK_HIP(hipMalloc((void**)&h->cg_rc, sizeof(double) * na * ldc));
-      DLP_CHECK_HIP(hipMalloc((void**)&h->cg_ec, sizeof(float) * na * ldc));
-      DLP_CHECK_HIP(hipMemsetAsync(h->cg_rc, 0, sizeof(double) * na * ldc, st));
-      DLP_CHECK_HIP(hipMemsetAsync(h->cg_ec, 0, sizeof(float) * na * ldc, st));
-    }
   }
-  if (!na) return DLP_OK;
-  if (h->cg_age < 0 || h->cg_age >= h->coarse_refresh) {
-    const bool first = h->cg_age < 0;
-    const int rc = coarse_build(n, na, S, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.solved, h->d.diag64,
-                                h->cg_Ac, h->cg_Ainv, st);
-    if (rc) return rc;
-    h->cg_age = 0;
-    if (first && h->pcg_exec) {   // captured before the coarse buffers existed
-      (void)hipGraphExecDestroy(h->pcg_exec);
-      h->pcg_exec = nullptr;
-    }
-  }
-  h->cg_age += 1;
+  h->cg_na = na;
+  h->cg_min = min_agg;
+  h->cg_age = 1;
   return DLP_OK;
 }
 
@@ -1980,6 +2035,22 @@ extern "C" int dlp_lp_scores(void* handle, double* out, int64_t ldo, void* strea
     hipLaunchKernelGGL(k_scores<float>, dim3(grid_for(total)), dim3(kBlock), 0, st, h->d.n, h->d.C1, h->d.Z32,
                        h->d.ld, h->d.solved, out, ldo);
   DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+// [0] aggregates of the current coarse operator (0: none), [1] their minimum rows, [2] global
+// solves since its build; bounds_out (kCoarseMax + 1 ints, may be NULL): the row ranges.
+extern "C" int dlp_lp_coarse_info(void* handle, int32_t* out, int32_t* bounds_out, void* stream) {
+  DLP_CHECK_ARG(handle && out, "lp_coarse_info: null argument");
+  LP* h = (LP*)handle;
+  out[0] = h->cg_age >= 0 ? h->cg_na : 0;
+  out[1] = h->cg_min;
+  out[2] = h->cg_age;
+  if (bounds_out && out[0] > 0) {
+    DLP_CHECK_HIP(hipMemcpyAsync(bounds_out, h->cg_bounds, sizeof(int32_t) * (out[0] + 1), hipMemcpyDeviceToHost,
+                                 (hipStream_t)stream));
+    DLP_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream));
+  }
   return DLP_OK;
 }
 

@@ -179,6 +179,14 @@ class EdgeStreamLP:
                   ct.byref(it), ct.byref(v), stream())
         return int(it.value), float(v.value)
 
+    def coarse_info(self):
+        """The global solve's coarse operator: aggregates (0: none), their minimum rows, solves
+        since it was built, and the row ranges (bounds, na + 1 ints)."""
+        out = np.zeros(3, dtype=np.int32)
+        bounds = np.zeros(129, dtype=np.int32)
+        _lib.call("dlp_lp_coarse_info", self.h, out.ctypes.data, bounds.ctypes.data, stream())
+        return dict(aggregates=int(out[0]), min_rows=int(out[1]), age=int(out[2]), bounds=bounds[: out[0] + 1].copy())
+
     def pcg_stats(self):
         out = np.zeros(2, dtype=np.int64)
         _lib.call("dlp_lp_pcg_stats", self.h, out.ctypes.data)

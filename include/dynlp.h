@@ -322,13 +322,19 @@ int dlp_lp_time_operator(void* handle, int reps, double* ms_out, void* stream);
  * working dtype). */
 int dlp_lp_set_pcg_directions(void* handle, int kind);
 /* Two-level preconditioner of the column PCG, M = D^-1 + P Ac^-1 P^T: P aggregates the solved
- * rows into na (<= 128) runs of consecutive rows (at least the rows one workgroup of the vector
- * passes spans), Ac = P^T A P is factored in FP64 and inverted on the device, and rebuilt from
+ * rows into up to na (<= 128) runs of consecutive rows (at least the rows one workgroup of the vector
+ * passes spans; each of the na evenly spaced boundaries moves to the row boundary with the fewest
+ * crossing edges within half a spacing), Ac = P^T A P is factored in FP64 and inverted on the
+ * device, and rebuilt from
  * the current equations every `refresh` global solves (a stale Ac is still an SPD preconditioner,
  * only a weaker one; 0 aggregates: Jacobi only, the default of a new handle).  The coarse correction removes the slow, nearly constant modes of the
  * label-grounded system (12 -> 6-7 iterations at SBM-1M, k = 64).  No reference counterpart:
  * solvers.pcg takes any SPD preconditioner M (solvers.py:66-97; jacobi_M :62-63, AMG amg.py). */
 int dlp_lp_set_coarse(void* handle, int na, int refresh);
+/* The current coarse operator: out[0] = aggregates (0: none built), out[1] = their minimum rows,
+ * out[2] = global solves since it was built; bounds_out (host, na + 1 ints, may be NULL) = the
+ * row ranges [bounds[I], bounds[I+1]) (synchronises).  No reference counterpart. */
+int dlp_lp_coarse_info(void* handle, int32_t* out, int32_t* bounds_out, void* stream);
 /* Plain FP64 path, one host-driven round of solvers.push (solvers.py:108-116): candidates
  * take D = R/diag, Z += D, R = 0, scatter W D with FP64 atomics and mark N(cand) and cand. */
 int dlp_lp_push64_round(void* handle, int nc, const int32_t* cand, void* stream);

@@ -143,12 +143,42 @@ def test_two_level_preconditioner_certifies_and_cuts_iterations(dirs):
     assert iters[(128, 1)] < iters[(0, 64)] and iters[(128, 64)] < iters[(0, 64)], iters
 
 
+@pytest.mark.parametrize("target", [6, 12])
+def test_coarse_boundaries_follow_communities(target):
+    """Aggregate boundaries move to the row boundaries with the fewest crossing edges and stay
+    where they cut fewer edges than half a spacing away: on an SBM whose ids follow its blocks
+    (1000-row blocks) they are the block edges whether 6 or 12 are asked for, and the solve
+    certifies against the oracle."""
+    from dynlp_amd.stream import EdgeStreamLP
+    g = graphs.sbm(6000, 6, 12, 0.9, 4, 0.01, seed=21)
+    lp = EdgeStreamLP(g, eps=1e-6, pcg_coarse=target, push_max_rows=0)
+    lp.initial_solve()
+    info = lp.coarse_info()
+    assert info["aggregates"] == 6, info
+    assert np.array_equal(info["bounds"], np.arange(0, 6001, 1000)), info["bounds"]
+    check_against_oracle(lp, g, edge_dict(g), 1e-6, "communities")
+
+
+def test_coarse_boundaries_without_communities():
+    """On an Erdos-Renyi graph (no structure in the ids) the evenly spaced boundaries are kept
+    and the two-level solve still certifies."""
+    from dynlp_amd.stream import EdgeStreamLP
+    g = graphs.erdos_renyi(4000, 10, 4, 0.02, seed=22)
+    lp = EdgeStreamLP(g, eps=1e-6, pcg_coarse=16, push_max_rows=0)
+    lp.initial_solve()
+    info = lp.coarse_info()
+    assert 4 <= info["aggregates"] <= 16, info
+    assert np.all(np.diff(info["bounds"]) >= info["min_rows"]), info
+    check_against_oracle(lp, g, edge_dict(g), 1e-6, "er")
+
+
 def test_tiny_graph_solves_without_coarse_layout():
     """Below 64 vertices the global solve runs with the Jacobi preconditioner alone."""
     from dynlp_amd.stream import EdgeStreamLP
     g = graphs.sbm(48, 2, 6, 0.8, 2, 0.2, seed=3)
     lp = EdgeStreamLP(g, eps=1e-6, push_max_rows=0)
     lp.initial_solve()
+    assert lp.coarse_info()["aggregates"] == 0
     es = edge_dict(g)
     batches, _ = graphs.update_stream(g, 2, 6, seed=4)
     for b in batches:
