@@ -237,7 +237,7 @@ def main(argv=None):
                 if traffic else None,
                 "avg_launch_ms": round(op_ms, 4), "algorithmic_bytes": int(nbytes)}
     log(f"A*P launch: {op_ms:.3f} ms, {nbytes / 1e6:.0f} MB -> {achieved:.0f} GB/s")
-    pcg_dirs = lp.pcg_directions
+    pcg_dirs, pcg_coarse = lp.pcg_directions, lp.pcg_coarse
     pcg_iters = pcg1["pcg_iters"] - pcg0["pcg_iters"]
     err = lp.error_report()
     log(f"final state: rel l2 residual {err['rel_l2_residual']:.2e}, certified |F - F*| <= {err['cert_bound']:.2e}, "
@@ -303,7 +303,7 @@ def main(argv=None):
         "config": {"workload": "sbm-1m-k64 edge-update stream" if a.scale == 1.0 else f"sbm-{g.n}-k64 edge-update stream",
                    "n": g.n, "blocks": 64, "deg": 20, "p_in": 0.8, "k": 64, "label_frac": 0.005,
                    "batch_edges": a.batch_edges, "ins_frac": 0.5, "eps": 1e-6, "seed": a.seed,
-                   "pcg_directions": pcg_dirs, "pcg_coarse_aggregates": lp.pcg_coarse},
+                   "pcg_directions": pcg_dirs, "pcg_coarse_aggregates": pcg_coarse},
         "p50_ms": round(percentile(lat, 50), 3), "p99_ms": round(percentile(lat, 99), 3),
         "rho_max": max(rhos), "paths": paths,
         "pcg_iters_per_request": round(pcg_iters / a.steps, 2), "pcg_ms_per_request": round(pcg_ms / a.steps, 3),
