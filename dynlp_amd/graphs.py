@@ -15,6 +15,7 @@ All generators use numpy's PCG64 (`numpy.random.default_rng`).
 """
 from __future__ import annotations
 
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -43,8 +44,12 @@ class Graph:
         return csr_from_pairs(self.n, self.u, self.v, self.w)
 
 
-def canonical_pairs(n, a, b, w):
-    """Unique undirected pairs (u < v), self-loops dropped, first occurrence's weight kept."""
+def canonical_pairs(n, a, b, w, merge="first"):
+    """Unique undirected pairs (u < v), self-loops dropped.  A pair given more than once (in
+    either direction) keeps its first occurrence's weight (merge="first", the generators), or
+    the FP32 sum of all its weights in the order given (merge="sum", the reference's `coalesce`,
+    graphs.py:36-44, which external edge lists go through).  Pairs come out in the order of
+    their first occurrence."""
     a = np.asarray(a, dtype=np.int64)
     b = np.asarray(b, dtype=np.int64)
     w = np.asarray(w, dtype=np.float32)
@@ -52,9 +57,18 @@ def canonical_pairs(n, a, b, w):
     a, b, w = a[keep], b[keep], w[keep]
     lo, hi = np.minimum(a, b), np.maximum(a, b)
     key = lo * n + hi
-    _, first = np.unique(key, return_index=True)
-    first.sort()
-    return lo[first], hi[first], w[first]
+    _, first, inv = np.unique(key, return_index=True, return_inverse=True)
+    if merge == "sum":
+        # sequential FP32 sums per pair, in the order the pairs were given
+        order = np.argsort(inv, kind="stable")
+        starts = np.searchsorted(inv[order], np.arange(first.shape[0]))
+        wsum = np.add.reduceat(w[order], starts, dtype=np.float32) if order.size else w[:0]
+    elif merge != "first":
+        raise ValueError(f"merge must be 'first' or 'sum', not {merge!r}")
+    pos = np.argsort(first, kind="stable")
+    first = first[pos]
+    wout = w[first] if merge == "first" else wsum[pos]
+    return lo[first], hi[first], wout
 
 
 def csr_from_pairs(n, u, v, w):
@@ -123,6 +137,76 @@ def sbm(n, blocks, deg, p_in, K, label_frac=0.01, seed=0, wlo=0.5, whi=1.5, unit
     y = (blk % K).astype(np.int32)
     return Graph(name or f"sbm-n{n}-b{blocks}-d{deg:g}-k{K}", n, u, v, w, lab, y, K, block=blk,
                  meta=dict(kind="sbm", blocks=blocks, deg=deg, p_in=p_in, seed=seed))
+
+
+# ---------------------------------------------------------------------------------------------
+# On-disk edge lists (the reference's graphs.py:176-204 format).
+
+def from_npz(path, label_frac=0.01, seed=0, name=None) -> Graph:
+    """Graph of an edge-list .npz as the reference reads it (`from_npz`, graphs.py:176-184):
+    arrays src, dst (int), y (int, one per vertex; n = len(y), K = max(y) + 1) and optionally
+    w (default 1) and order (a vertex permutation, the arrival order; default a permutation
+    drawn by numpy's legacy RandomState(seed + 1), the generator the reference calls on its
+    array module).  Pairs may come in either direction and more than once; duplicates are
+    summed and self-loops dropped as in `coalesce`.  Labelled vertices: a label_frac sample
+    (seed S+1) of the vertices with y >= 0 (y < 0 marks a vertex without ground truth).  The
+    file is read without unpickling."""
+    with np.load(path, allow_pickle=False) as z:
+        keys = set(z.files)
+        missing = {"src", "dst", "y"} - keys
+        if missing:
+            raise ValueError(f"{path}: missing array(s) {sorted(missing)}")
+        src = z["src"].astype(np.int64).ravel()
+        dst = z["dst"].astype(np.int64).ravel()
+        y = z["y"].astype(np.int32).ravel()
+        w = z["w"].astype(np.float32).ravel() if "w" in keys else np.ones(src.shape[0], dtype=np.float32)
+        order = z["order"].astype(np.int64).ravel() if "order" in keys else None
+    n = int(y.shape[0])
+    if src.shape != dst.shape or w.shape != src.shape:
+        raise ValueError(f"{path}: src, dst and w must have one entry per edge")
+    if src.size and (min(src.min(), dst.min()) < 0 or max(src.max(), dst.max()) >= n):
+        raise ValueError(f"{path}: vertex ids must lie in [0, {n})")
+    if not np.all(np.isfinite(w)) or (w.size and w.min() <= 0):
+        raise ValueError(f"{path}: weights must be finite and positive")
+    if order is None:
+        order = np.random.RandomState(seed + 1).permutation(n).astype(np.int64)
+    elif not np.array_equal(np.sort(order), np.arange(n)):
+        raise ValueError(f"{path}: order must be a permutation of 0..{n - 1}")
+    known = np.flatnonzero(y >= 0)
+    if known.size == 0:
+        raise ValueError(f"{path}: no vertex has a label (y >= 0)")
+    K = int(y.max()) + 1
+    u, v, w = canonical_pairs(n, src, dst, w, merge="sum")
+    rng = np.random.default_rng(seed + 1)
+    cnt = min(known.size, max(1, int(round(label_frac * known.size))))
+    lab = np.zeros(n, dtype=bool)
+    lab[rng.choice(known, size=cnt, replace=False)] = True
+    return Graph(name or os.path.splitext(os.path.basename(path))[0], n, u, v, w, lab, y, K,
+                 meta=dict(kind="npz", path=str(path), seed=seed, order=order))
+
+
+def save_npz(g: Graph, path) -> None:
+    """Writes g in the format `from_npz` reads: its unique pairs as the upper triangle
+    (src < dst, the layout of the reference's `build_cached`, graphs.py:187-204), w, y and the
+    arrival order when g carries one."""
+    arrays = dict(src=g.u.astype(np.int64), dst=g.v.astype(np.int64), w=g.w.astype(np.float32),
+                  y=g.y.astype(np.int32))
+    if g.meta.get("order") is not None:
+        arrays["order"] = np.asarray(g.meta["order"], dtype=np.int64)
+    np.savez(path, **arrays)
+
+
+def load(spec: str, *, n=100_000, K=2, deg=10.0, p_in=0.85, label_frac=0.01, seed=0) -> Graph:
+    """The reference's dataset switch (`load`, graphs.py:207-214) over the sources this package
+    has: "sbm" (n/K equal blocks, labels = block), "er", or a path to an edge-list .npz.  The
+    reference's downloaded and cached corpora (imdb, synthK) are not part of this package."""
+    if spec == "sbm":
+        return sbm(n, K, deg, p_in, K, label_frac, seed)
+    if spec == "er":
+        return erdos_renyi(n, deg, K, label_frac, seed)
+    if spec.endswith(".npz"):
+        return from_npz(spec, label_frac, seed)
+    raise ValueError(f"unknown dataset {spec!r}: 'sbm', 'er' or a path to an edge-list .npz")
 
 
 # ---------------------------------------------------------------------------------------------
