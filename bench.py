@@ -134,6 +134,30 @@ def kernel_table(g, C, reps=10):
     return rows
 
 
+def request_row(lp, st, b, batch, g, a):
+    """The request's row in the reference's CSV schema (dynlp_amd.report.FIELDS): solver
+    "dynlp+inc" (the reference's resident incremental solver, whose role this path takes),
+    n_u solved rows, nnz stored entries, iters = push rounds + PCG iterations, the request's own
+    certificate (cert_self, hmax) and the accuracy of its predictions on the solved rows; tol
+    holds the residual-ratio threshold the request is solved to (rho <= eps), delta (the
+    Jacobi solvers' change threshold) does not apply; no exact solution is computed, so
+    err_max, cert_post, hmax_ref, agree and ref_ms are NaN."""
+    from dynlp_amd import report
+    C = lp.C
+    Z = (lp.Z64 if lp.fp64 else lp.Z32)[:, :C]
+    known = lp.solved.bool() & (lp.y >= 0)
+    pred = (Z[:, 0] > 0.5).int() if g.K == 2 else Z.argmax(dim=1).int()
+    nk = int(known.sum())
+    acc = float((pred[known] == lp.y[known]).sum()) / nk if nk else float("nan")
+    nnz = lp.stored_entries()
+    return report.row(tag=a.tag or g.name, dataset=g.name, K=g.K, batch=batch, solver="dynlp+inc", n_active=g.n,
+                      n_u=int(lp.solved.sum()), nnz=nnz, inserted=int(b.ins_u.shape[0]),
+                      deleted=int(b.del_u.shape[0]), iters=st.iters + st.extra.get("pcg_iters", 0), edges=st.edges,
+                      sweeps=st.edges / max(nnz, 1), ms=st.ms, setup_ms=st.setup_ms, path=st.path,
+                      cert_self=st.cert, hmax=st.hmax, acc_true=acc, delta=float("nan"), tol=lp.eps,
+                      dtype="float32+float64", group="auto")
+
+
 def pmc_probe(g):
     """Launch sequence for hardware-counter runs (rocprofv3 --pmc ...): the counters of the 5
     A*P launches are read against those of one plain copy of known size."""
@@ -167,6 +191,10 @@ def main(argv=None):
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel roofline table")
+    ap.add_argument("--out", default=None,
+                    help="also append one row per timed request to this CSV in the reference's schema "
+                         "(bench.py FIELDS), with a .json record of the run next to it")
+    ap.add_argument("--tag", default="", help="tag column of the CSV rows")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="counter probe: initial solve, 5 A*P launches, then a device copy of 512 MiB "
                          "(reads 512 MiB, writes 512 MiB) that calibrates FETCH_SIZE / WRITE_SIZE")
@@ -204,17 +232,22 @@ def main(argv=None):
     for b in batches[: a.warmup]:
         lp.apply_and_solve(b)
     torch.cuda.synchronize()
-    lat, rhos, paths = [], [], {}
+    lat, rhos, paths, csv_rows = [], [], {}, []
     pcg0 = lp.pcg_stats()
+    paused = 0.0
     t0 = time.perf_counter()
-    for b in batches[a.warmup:]:
+    for t, b in enumerate(batches[a.warmup:]):
         ts = time.perf_counter()
         st = lp.apply_and_solve(b)
         lat.append((time.perf_counter() - ts) * 1e3)
         rhos.append(st.rho)
         paths[st.path] = paths.get(st.path, 0) + 1
+        if a.out:
+            tp = time.perf_counter()
+            csv_rows.append(request_row(lp, st, b, a.warmup + t + 1, g, a))
+            paused += time.perf_counter() - tp   # the row's extra device reads are not timed
     torch.cuda.synchronize()
-    T = time.perf_counter() - t0
+    T = time.perf_counter() - t0 - paused
     pcg1 = lp.pcg_stats()
     edges = a.steps * a.batch_edges
     value = edges / T
@@ -313,6 +346,13 @@ def main(argv=None):
         "roofline": roofline, "cpu_baseline": cpu, "kernels": ktab,
         "wall_s": round(time.perf_counter() - t_all, 1),
     }
+    if a.out:
+        from dynlp_amd import report
+        for r in csv_rows:
+            r["run_s"] = out["wall_s"]
+        report.write(csv_rows, a.out, args=vars(a), env={"gpu": torch.cuda.get_device_name(0),
+                                                          "torch": torch.__version__}, run_s=out["wall_s"])
+        log(f"{len(csv_rows)} rows -> {a.out}\n" + report.summarize(csv_rows))
     print(json.dumps(out), flush=True)
     return 0
 

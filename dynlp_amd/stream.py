@@ -361,9 +361,36 @@ class EdgeStreamLP:
         return st
 
     def _finish(self, st, s):
+        """Request statistics, with the certificate of solvers.py:295-301 as the reference's
+        incremental solver reports it per batch (solvers.py:421-422): column maxima rho_c of
+        |R_uc|/diag_u over every row, h_est = max h over every row (parked rows hold h = 1),
+        hmax = h_est / (1 - rho_h) and cert = max_c<C rho_c * hmax (inf while rho_h > eps_h)."""
         st.edges, st.touched, st.changed, st.toggled = s["edges"], s["touched"], s["changed"], s["toggled"]
-        # after a clean verification of every row its column maxima are rho (no second pass)
-        st.rho = float(self._rho[: self.C].max()) if getattr(self, "_rho_all", False) else self.rho()
+        C, C1 = self.C, self.C1
+        dev = device()
+        if getattr(self, "_cert_buf", None) is None:
+            self._cert_buf = torch.zeros(C1 + 1, dtype=torch.float64, device=dev)
+            self._ones64 = torch.ones(self.n, dtype=torch.float64, device=dev)
+        buf = self._cert_buf
+        # after a clean verification of every row (or on the FP64 path, a pass over the exact
+        # kept residual) the column maxima are already on the host; else one pass over R
+        full = self.fp64 or getattr(self, "_rho_all", False)
+        if not full:
+            _lib.call("dlp_rho_cols", self.n, None, C1, ptr(self.R32), self.ld, _lib.F32, ptr(self.diag64),
+                      ptr(buf), stream())
+        Zf = self.Z64 if self.fp64 else self.Z32
+        _lib.call("dlp_rho_cols", self.n, None, 1, ptr(Zf) + C * Zf.element_size(), self.ld,
+                  _lib.F64 if self.fp64 else _lib.F32, ptr(self._ones64), ptr(buf) + 8 * C1, stream())
+        host = buf.cpu().numpy()
+        if not full:
+            self._rho[:] = host[:C1]
+        rc, h_est = self._rho, float(host[C1])
+        st.rho = float(rc[:C].max())
+        if rc[C] > self.eps_h:
+            st.hmax, st.cert = h_est, float("inf")
+        else:
+            st.hmax = h_est / (1.0 - float(rc[C]))
+            st.cert = float(rc[:C].max()) * st.hmax
 
     def rho(self):
         """max over label columns and all rows of |R_uc|/diag_u of the kept residual (every row

@@ -59,6 +59,11 @@ def test_stream_fixture(fp64):
         C = Zs.shape[1] - 1
         assert st.rho <= 1e-6 * 1.0001, t
         assert np.abs(Z[:, :C] - Zs[:, :C]).max() <= 1e-6 * Zs[:, C].max() * 1.05, t
+        # the request's own certificate (solvers.py:295-301): hmax bounds the exact h from
+        # above, and rho * hmax bounds the error of every label score
+        assert np.isfinite(st.cert) and st.hmax >= Zs[:, C].max() * (1 - 1e-9), t
+        assert np.abs(Z[:, :C] - Zs[:, :C]).max() <= st.cert, t
+        assert st.cert == pytest.approx(st.rho * st.hmax, rel=1e-12), t
 
 
 def test_cc_fixture_bit_exact():

@@ -219,10 +219,15 @@ def test_error_report_against_oracle(K):
     lp.initial_solve()
     es = edge_dict(g)
     b = graphs.update_stream(g, 1, 300, seed=9)[0][0]
-    lp.apply_and_solve(b)
+    st = lp.apply_and_solve(b)
     O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
                        zip(b.del_u.tolist(), b.del_v.tolist()))
     e = lp.error_report()
+    # the request's own certificate is the report's (column maxima of the FP64 verification
+    # against those of its FP32-kept copy)
+    assert st.hmax == pytest.approx(e["hmax"], rel=1e-6)
+    assert st.cert == pytest.approx(e["cert_bound"], rel=1e-6)
+    assert st.rho == pytest.approx(e["rho"], rel=1e-6)
     sysm, _ = oracle_system(g, es)
     Z = lp.scores_with_h()
     U, C = sysm.solved, sysm.C
