@@ -29,6 +29,14 @@ struct PcgBufs {
   double* rp;
   int p16;
   float* sig;
+  // deferred solution updates (16-bit directions on the flat passes; slot_stride 0: Z is updated
+  // by every iteration): a group of `group` iterations keeps its directions in slots of
+  // slot_stride values (slot s at P16 + s*slot_stride); iteration `slot` gathers slot `slot`,
+  // writes the next direction to slot `slot`+1 and its step lengths to alpha_hist[slot*C ..]
+  // (group*C floats, rounded up to an even count); pcg_flush then adds sum_s alpha_s P_s to Z
+  int64_t slot_stride;
+  int slot, group;
+  float* alpha_hist;
   // two-level preconditioner M = D^-1 + P Ac^-1 P^T (na = 0: Jacobi only): aggregate I is the
   // row range [bounds[I], bounds[I+1]) (at least min_agg rows), aggmap[r] = the aggregate of row
   // r, P[r, aggmap[r]] = 1 on solved rows; rc = P^T r (na x ldc doubles), ec = the coarse
@@ -42,6 +50,13 @@ struct PcgBufs {
   const double* Ainv;
   const uint8_t* solved;
 };
+
+// Longest group of iterations with deferred solution updates.
+constexpr int kMaxGroup = 8;
+
+// The group's deferred update Z += sum_s alpha_s P_s (no-op without deferred updates), and the
+// next group's first direction moved from slot `group` to slot 0 unless the solve is done.
+int pcg_flush(const PcgBufs& b, hipStream_t st);
 
 // Rows per workgroup of the flat PCG passes for this layout (0 when the flat form does not
 // apply): the coarse aggregates must be at least this long.

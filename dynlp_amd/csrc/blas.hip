@@ -382,15 +382,24 @@ __global__ __launch_bounds__(256) void k_pcg_rz4(int n, int Q4, int C, const flo
 // directions, r.z otherwise; 0 where pAp <= 0); rz_new[c] += sum R*R/diag; viol = max
 // |R|/(eps_c*diag); with K also rc += P^T R (the new residual restricted to the aggregates).
 // Rows in a contiguous split.
-template <int PK, bool NT = false>
+// DEFER: Z and P are not touched; the step lengths go to alpha_out (C floats, written by the
+// first workgroup; zeros when the iteration is skipped) for the group's deferred update.
+template <int PK, bool NT = false, bool DEFER = false>
 __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float4* __restrict__ Z,
                                                      float4* __restrict__ R, PBlock<PK> P,
                                                      const float4* __restrict__ AP,
                                                      const double* __restrict__ pAp, const double* __restrict__ num,
                                                      const float* __restrict__ diag, const float* __restrict__ eps,
                                                      double* __restrict__ rz_new, float* __restrict__ viol,
-                                                     const int32_t* __restrict__ skip, CoarseArg K) {
-  if (skip && *skip) return;
+                                                     const int32_t* __restrict__ skip, CoarseArg K,
+                                                     float* __restrict__ alpha_out) {
+  if (skip && *skip) {
+    if constexpr (DEFER) {
+      if (blockIdx.x == 0)
+        for (int c = threadIdx.x; c < C; c += blockDim.x) alpha_out[c] = 0.f;
+    }
+    return;
+  }
   __shared__ double sm[kMaxCols + 4];
   __shared__ double sm2[2 * (kMaxCols + 4)];
   __shared__ float vmax;
@@ -408,19 +417,34 @@ __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float
     a[j] = pa > 0.0 ? (float)(num[c] / pa) : 0.f;
     ie[j] = j < q.cnt ? 1.f / eps[c] : 0.f;
   }
+  if constexpr (DEFER) {
+    if (blockIdx.x == 0 && q.active && q.roff == 0) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (j < q.cnt) alpha_out[4 * q.q4 + j] = a[j];
+    }
+  }
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   float vm = 0.f;
   if (q.active) {
     for (int64_t r = q.r_lo + q.roff; r < q.r_hi; r += q.rstep) {
       const int64_t i = r * Q4 + q.q4;
       const float inv = 1.f / diag[r];
-      float4 z = ld4<NT>(Z + i), rv = ld4<NT>(R + i);
-      const float4 p = P.load(i, r, q.q4), ap = ld4<NT>(AP + i);
-      if (q.cnt > 0) { z.x += a[0] * p.x; rv.x -= a[0] * ap.x; }
-      if (q.cnt > 1) { z.y += a[1] * p.y; rv.y -= a[1] * ap.y; }
-      if (q.cnt > 2) { z.z += a[2] * p.z; rv.z -= a[2] * ap.z; }
-      if (q.cnt > 3) { z.w += a[3] * p.w; rv.w -= a[3] * ap.w; }
-      st4<NT>(Z + i, z);
+      float4 rv = ld4<NT>(R + i);
+      const float4 ap = ld4<NT>(AP + i);
+      if constexpr (!DEFER) {
+        float4 z = ld4<NT>(Z + i);
+        const float4 p = P.load(i, r, q.q4);
+        if (q.cnt > 0) z.x += a[0] * p.x;
+        if (q.cnt > 1) z.y += a[1] * p.y;
+        if (q.cnt > 2) z.z += a[2] * p.z;
+        if (q.cnt > 3) z.w += a[3] * p.w;
+        st4<NT>(Z + i, z);
+      }
+      if (q.cnt > 0) rv.x -= a[0] * ap.x;
+      if (q.cnt > 1) rv.y -= a[1] * ap.y;
+      if (q.cnt > 2) rv.z -= a[2] * ap.z;
+      if (q.cnt > 3) rv.w -= a[3] * ap.w;
       st4<NT>(R + i, rv);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
@@ -445,12 +469,13 @@ __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float
   if (threadIdx.x == 0) atomic_max_nonneg(viol, vmax);
 }
 
-// P = M R + beta P (M R = R/diag plus the coarse correction on solved rows); with a 16-bit block
-// also rp[c] += sum R*P as held (the next step length's numerator, exact for the rounded
-// direction).  FP16 blocks: P = sig*M R + beta*P with sig the new column scale and beta already
-// rescaled to the held direction.
+// Pout = M R + beta P (M R = R/diag plus the coarse correction on solved rows; Pout may be P);
+// with a 16-bit block also rp[c] += sum R*Pout as held (the next step length's numerator, exact
+// for the rounded direction).  FP16 blocks: Pout = sig*M R + beta*P with sig the new column
+// scale and beta already rescaled to the held direction.
 template <int PK, bool NT = false>
-__global__ __launch_bounds__(256) void k_pcg_dir4(int n, int Q4, int C, PBlock<PK> P, const float4* __restrict__ R,
+__global__ __launch_bounds__(256) void k_pcg_dir4(int n, int Q4, int C, PBlock<PK> P, PBlock<PK> Pout,
+                                                  const float4* __restrict__ R,
                                                   const float* __restrict__ diag, const double* __restrict__ beta,
                                                   double* __restrict__ rp, const int32_t* __restrict__ skip,
                                                   const float* __restrict__ sig, CoarseArg K) {
@@ -488,7 +513,7 @@ __global__ __launch_bounds__(256) void k_pcg_dir4(int n, int Q4, int C, PBlock<P
         if (Qd.cnt > 2) p.z = rv.z * inv + ev.z + b[2] * p.z;
         if (Qd.cnt > 3) p.w = rv.w * inv + ev.w + b[3] * p.w;
       }
-      const float4 held = P.store(i, r, Qd.q4, p);
+      const float4 held = Pout.store(i, r, Qd.q4, p);
       if constexpr (PK != 0) {
 #pragma unroll
         for (int j = 0; j < 4; ++j)
@@ -497,6 +522,66 @@ __global__ __launch_bounds__(256) void k_pcg_dir4(int n, int Q4, int C, PBlock<P
     }
   }
   if constexpr (PK != 0) flush_quads(Qd, Q4, C, acc, sm, rp);
+}
+
+// The deferred solution update of a group of G <= kMaxGroup iterations: Z += sum_s alpha[s] *
+// P_s over the direction slots s < G (slot s = P0 + s*stride), in the order of the iterations,
+// with the same FP32 multiply-adds the per-iteration update makes.  Iterations skipped after
+// convergence hold zero step lengths and come last: the workgroup reads only the slots up to
+// the last one with a nonzero step length, and each thread issues all its slot loads before the
+// first multiply-add.  Unless the solve is done (ctl[0]), the next group's first direction
+// (slot G) moves to slot 0.
+
+template <int PK, bool NT = false>
+__global__ __launch_bounds__(256) void k_pcg_flush4(int n, int Q4, int C, float4* __restrict__ Z, PBlock<PK> P0,
+                                                    int64_t stride, int G, const float* __restrict__ alpha,
+                                                    const int32_t* __restrict__ ctl) {
+  __shared__ int used;
+  if (threadIdx.x == 0) used = 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < G * C; k += blockDim.x)
+    if (alpha[k] != 0.f) atomicMax(&used, k / C + 1);
+  __syncthreads();
+  const int S = used;
+  const Quad Qd = quad_of(Q4, C);
+  if (!Qd.active || Qd.r0 >= n) return;
+  const bool copy = !(ctl && ctl[0]);
+  float a[kMaxGroup][4];
+#pragma unroll
+  for (int g = 0; g < kMaxGroup; ++g)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) a[g][j] = (g < S && j < Qd.cnt) ? alpha[g * C + 4 * Qd.q4 + j] : 0.f;
+  const int64_t nq = (int64_t)n * Q4;
+  int64_t r = Qd.r0;
+  for (int64_t i = Qd.i0; i < nq; i += Qd.step, r += Qd.rstep) {
+    float4 p[kMaxGroup];
+#pragma unroll
+    for (int g = 0; g < kMaxGroup; ++g) {
+      if (g < S) {
+        PBlock<PK> Ps = P0;
+        Ps.base = reinterpret_cast<uint16_t*>(P0.base) + g * stride;
+        p[g] = Ps.load(i, r, Qd.q4);
+      }
+    }
+    if (S > 0) {
+      float4 z = ld4<NT>(Z + i);
+#pragma unroll
+      for (int g = 0; g < kMaxGroup; ++g) {
+        if (g < S) {
+          if (Qd.cnt > 0) z.x += a[g][0] * p[g].x;
+          if (Qd.cnt > 1) z.y += a[g][1] * p[g].y;
+          if (Qd.cnt > 2) z.z += a[g][2] * p[g].z;
+          if (Qd.cnt > 3) z.w += a[g][3] * p[g].w;
+        }
+      }
+      st4<NT>(Z + i, z);
+    }
+    if (copy) {
+      PBlock<PK> last = P0;
+      last.base = reinterpret_cast<uint16_t*>(P0.base) + G * stride;
+      P0.store(i, r, Qd.q4, last.load(i, r, Qd.q4));
+    }
+  }
 }
 
 struct FlatLaunch {
@@ -910,6 +995,10 @@ int pcg_start(const PcgBufs& b, hipStream_t st) {
     const CoarseArg K = coarse_arg(b);
     float* sig = (b.P16 && b.p16 == 2) ? b.sig : nullptr;
     if (b.P16) hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, b.C, b.rp);
+    // step lengths of the deferred updates (held as doubles' worth of float pairs)
+    if (b.P16 && b.slot_stride > 0 && b.alpha_hist)
+      hipLaunchKernelGGL(k_zero_f64, dim3(grid_for((b.group * b.C + 1) / 2)), dim3(kBlock), 0, st,
+                         (b.group * b.C + 1) / 2, reinterpret_cast<double*>(b.alpha_hist));
     if (sig || b.na) {
       // scale pass: rz (for the FP16 column scales) and the restricted residual, then the
       // coarse correction of the start
@@ -1016,22 +1105,33 @@ int pcg_finish_iteration(const PcgBufs& b, hipStream_t st, bool have_pap) {
     // the update pass takes its step lengths from pAp and r.p (16-bit directions) or r.z
     const double* num = b.P16 ? b.rp : b.rz;
     const bool nt = tuning().nt_streams != 0;
-#define DLP_UPDATE_NT(PKV, PB, NTV)                                                                          \
-  hipLaunchKernelGGL((k_pcg_update4<PKV, NTV>), dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,         \
+    // deferred solution updates: this iteration gathers slot b.slot, the next direction goes to
+    // the slot after it, and the step lengths to the group's history
+    const bool defer = b.P16 && b.slot_stride > 0;
+    DLP_CHECK_ARG(!defer || (b.alpha_hist && b.slot >= 0 && b.slot < b.group), "pcg: deferred update slot %d",
+                  b.slot);
+    uint16_t* Pcur = b.P16 ? b.P16 + (defer ? b.slot * b.slot_stride : 0) : nullptr;
+    uint16_t* Pnext = defer ? Pcur + b.slot_stride : Pcur;
+    float* aout = defer ? b.alpha_hist + (int64_t)b.slot * b.C : nullptr;
+#define DLP_UPDATE_NT(PKV, PB, NTV, DV)                                                                      \
+  hipLaunchKernelGGL((k_pcg_update4<PKV, NTV, DV>), dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,     \
                      (float4*)b.Z, (float4*)b.R, PBlock<PKV>{PB, b.P16 ? b.ldh / 4 : 0}, (const float4*)b.AP,  \
-                     b.pAp, num, b.diag, b.eps, b.rz_new, b.viol, b.ctl, K)
-#define DLP_UPDATE(PKV, PB)              \
-  if (nt) DLP_UPDATE_NT(PKV, PB, true); \
-  else DLP_UPDATE_NT(PKV, PB, false)
-#define DLP_DIR_NT(PKV, PB, NTV)                                                                             \
+                     b.pAp, num, b.diag, b.eps, b.rz_new, b.viol, b.ctl, K, aout)
+#define DLP_UPDATE(PKV, PB)                                   \
+  if (defer && nt) DLP_UPDATE_NT(PKV, PB, true, true);       \
+  else if (defer) DLP_UPDATE_NT(PKV, PB, false, true);       \
+  else if (nt) DLP_UPDATE_NT(PKV, PB, true, false);          \
+  else DLP_UPDATE_NT(PKV, PB, false, false)
+#define DLP_DIR_NT(PKV, PB, PO, NTV)                                                                         \
   hipLaunchKernelGGL((k_pcg_dir4<PKV, NTV>), dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,           \
-                     PBlock<PKV>{PB, b.P16 ? b.ldh / 4 : 0}, (const float4*)b.R, b.diag, b.beta, rp, b.ctl, sig, K)
-#define DLP_DIR(PKV, PB)              \
-  if (nt) DLP_DIR_NT(PKV, PB, true); \
-  else DLP_DIR_NT(PKV, PB, false)
+                     PBlock<PKV>{PB, b.P16 ? b.ldh / 4 : 0}, PBlock<PKV>{PO, b.P16 ? b.ldh / 4 : 0},           \
+                     (const float4*)b.R, b.diag, b.beta, rp, b.ctl, sig, K)
+#define DLP_DIR(PKV, PB, PO)              \
+  if (nt) DLP_DIR_NT(PKV, PB, PO, true); \
+  else DLP_DIR_NT(PKV, PB, PO, false)
     if (!b.P16) { DLP_UPDATE(0, b.P); }
-    else if (b.p16 == 2) { DLP_UPDATE(2, b.P16); }
-    else { DLP_UPDATE(1, b.P16); }
+    else if (b.p16 == 2) { DLP_UPDATE(2, Pcur); }
+    else { DLP_UPDATE(1, Pcur); }
     // beta (from r^T M r: the update's r^T D^-1 r plus the coarse part), the convergence test,
     // and the coarse correction the direction uses
     if (b.na)
@@ -1040,9 +1140,9 @@ int pcg_finish_iteration(const PcgBufs& b, hipStream_t st, bool have_pap) {
     else
       hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 1, b.rz, b.rz_new, b.pAp, b.alpha,
                          b.beta, b.viol, b.ctl, rp, sig, nullptr, 0);
-    if (!b.P16) { DLP_DIR(0, b.P); }
-    else if (b.p16 == 2) { DLP_DIR(2, b.P16); }
-    else { DLP_DIR(1, b.P16); }
+    if (!b.P16) { DLP_DIR(0, b.P, b.P); }
+    else if (b.p16 == 2) { DLP_DIR(2, Pcur, Pnext); }
+    else { DLP_DIR(1, Pcur, Pnext); }
 #undef DLP_UPDATE
 #undef DLP_DIR
 #undef DLP_UPDATE_NT
@@ -1065,6 +1165,27 @@ int pcg_finish_iteration(const PcgBufs& b, hipStream_t st, bool have_pap) {
                      b.ld, b.diag, b.beta, 0, b.ctl);
   DLP_DISPATCH_CPL(m.cpl, K)
 #undef K
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+int pcg_flush(const PcgBufs& b, hipStream_t st) {
+  if (!b.P16 || b.slot_stride <= 0) return DLP_OK;
+  const FlatLaunch f = flat_launch(b);
+  DLP_CHECK_ARG(f.ok && b.alpha_hist && b.group >= 1 && b.group <= kMaxGroup,
+                "pcg: deferred updates need the flat passes and at most %d iterations per group", kMaxGroup);
+  const bool nt = tuning().nt_streams != 0;
+#define DLP_FLUSH(PKV, NTV)                                                                                 \
+  hipLaunchKernelGGL((k_pcg_flush4<PKV, NTV>), dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (float4*)b.Z, \
+                     PBlock<PKV>{b.P16, b.ldh / 4}, b.slot_stride, b.group, b.alpha_hist, b.ctl)
+  if (b.p16 == 2) {
+    if (nt) DLP_FLUSH(2, true);
+    else DLP_FLUSH(2, false);
+  } else {
+    if (nt) DLP_FLUSH(1, true);
+    else DLP_FLUSH(1, false);
+  }
+#undef DLP_FLUSH
   DLP_CHECK_LAUNCH();
   return DLP_OK;
 }

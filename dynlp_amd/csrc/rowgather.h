@@ -247,6 +247,7 @@ struct Tuning {
   int bf16_word = 4;  // BF16 direction values per lane in the global-solve operator (4 or 8)
   int flat_grid = 0;  // workgroups of the flat PCG vector passes (0: 2048)
   int nt_streams = 1; // non-temporal FP32 streams (Z, R, AP) in the PCG update and direction passes
+  int defer_z = 1;    // 16-bit directions: Z updated once per captured group of PCG iterations
 };
 Tuning& tuning();
 

@@ -695,8 +695,10 @@ extern "C" int dlp_rho_cols(int nrows, const int32_t* rows, int C, const void* R
   hipStream_t st = (hipStream_t)stream;
   DLP_CHECK_HIP(hipMemsetAsync(rho, 0, sizeof(double) * C, st));
   if (nrows == 0) return DLP_OK;
+  // 16-byte quads when most of each row is read (one column of a wide block takes a thread
+  // per row instead: the quad layout would leave all but one thread of each row group idle)
   if (!rows && r_dtype == DLP_F32 && (ldr % 4) == 0 && ldr <= 256 && (((uintptr_t)R) & 15) == 0 &&
-      tuning().vec4) {
+      2 * C > ldr && tuning().vec4) {
     const int Q4 = (int)(ldr / 4);
     const long long used = Q4 * (kBlock / Q4);
     long long g = ((long long)nrows * Q4 + used - 1) / used;
@@ -776,6 +778,8 @@ extern "C" int dlp_tune_set(int key, int value) {
     tuning().flat_grid = value;
   } else if (key == 9) {
     tuning().nt_streams = value ? 1 : 0;
+  } else if (key == 10) {
+    tuning().defer_z = value ? 1 : 0;
 
 
   } else {

@@ -1038,6 +1038,10 @@ struct LP {
   float* AP = nullptr;
   uint16_t* P16 = nullptr;      // 16-bit directions of the column PCG (pcg_p16 != 0)
   int64_t ldh = 0;              // its row length: C1 rounded up to 8
+  int p16_slots = 0;            // n x ldh blocks allocated in P16
+  int pcg_group = 0;            // iterations per deferred solution update (0: every iteration)
+  float* pcg_ahist = nullptr;   // their step lengths, pcg_group x C1 (rounded up to even)
+  int ahist_len = 0;
   int pcg_p16 = 0;              // 0 FP32 directions, 1 BF16, 2 FP16 with a per-column scale
   double* pcg_scal = nullptr;   // rz, rz_new, pAp, alpha, beta, rp (C1 doubles each), sig (C1 floats)
   // two-level preconditioner of the column PCG: coarse_na aggregates requested (0: Jacobi
@@ -1123,6 +1127,7 @@ int lp_free(LP* h) {
   if (h->P) (void)hipFree(h->P);
   if (h->AP) (void)hipFree(h->AP);
   if (h->P16) (void)hipFree(h->P16);
+  if (h->pcg_ahist) (void)hipFree(h->pcg_ahist);
   if (h->cg_Ac) (void)hipFree(h->cg_Ac);
   if (h->cg_Ainv) (void)hipFree(h->cg_Ainv);
   if (h->cg_rc) (void)hipFree(h->cg_rc);
@@ -1777,6 +1782,11 @@ static PcgBufs pcg_bufs(LP* h) {
   b.rp = h->pcg_scal + 5 * C1;
   b.p16 = h->pcg_p16;
   b.sig = reinterpret_cast<float*>(h->pcg_scal + 6 * C1);
+  if (b.P16 && h->pcg_group > 0) {
+    b.slot_stride = (int64_t)h->d.n * h->ldh;
+    b.group = h->pcg_group;
+    b.alpha_hist = h->pcg_ahist;
+  }
   if (h->cg_na && h->cg_age >= 0) {
     b.na = h->cg_na;
     b.min_agg = h->cg_min;
@@ -1794,7 +1804,8 @@ static PcgBufs pcg_bufs(LP* h) {
 static int pcg_iteration(LP* h, const PcgBufs& b, hipStream_t st) {
   // AP = diag*P - W_UU P, with p^T A p accumulated in the same pass when the layout allows
   if (b.P16 || ((b.ld % 4) == 0 && tuning().vec4 && tuning().fuse_dot)) {
-    int rc = pcg_operator(b.n, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.solved, b.P, b.ld, b.P16,
+    const uint16_t* P16 = b.P16 && b.slot_stride > 0 ? b.P16 + b.slot * b.slot_stride : b.P16;
+    int rc = pcg_operator(b.n, b.C, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.solved, b.P, b.ld, P16,
                           b.ldh, b.p16, b.diag, -1.f, b.AP, b.ld, b.pAp, st, b.ctl);
     if (rc) return rc;
     return pcg_finish_iteration(b, st, true);
@@ -1906,6 +1917,19 @@ static int coarse_prepare(LP* h, hipStream_t st) {
   return DLP_OK;
 }
 
+// One group of iterations: iteration i gathers direction slot i (deferred updates) and the
+// group ends with the deferred solution update.
+static int pcg_group_launch(LP* h, const PcgBufs& b, int iters, hipStream_t st) {
+  int rc = DLP_OK;
+  for (int i = 0; i < iters && rc == DLP_OK; ++i) {
+    PcgBufs bi = b;
+    bi.slot = i;
+    rc = pcg_iteration(h, bi, st);
+  }
+  if (rc == DLP_OK) rc = pcg_flush(b, st);
+  return rc;
+}
+
 static int capture_pcg(LP* h, int iters) {
   if (h->pcg_exec && h->pcg_graph_iters == iters) return DLP_OK;
   if (h->pcg_exec) {
@@ -1919,7 +1943,7 @@ static int capture_pcg(LP* h, int iters) {
   int rc = DLP_OK;
   hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    for (int i = 0; i < iters && rc == DLP_OK; ++i) rc = pcg_iteration(h, b, cs);
+    rc = pcg_group_launch(h, b, iters, cs);
     e = hipStreamEndCapture(cs, &graph);
   }
   if (e == hipSuccess && rc == DLP_OK) {
@@ -1955,12 +1979,35 @@ extern "C" int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int 
     DLP_CHECK_HIP(hipMemsetAsync(h->P, 0, nz * sizeof(float), st));
     DLP_CHECK_HIP(hipMemsetAsync(h->AP, 0, nz * sizeof(float), st));
   }
-  if (h->pcg_p16 && !h->P16) {
+  // deferred solution updates keep a group's directions: iters_per_launch + 1 slots
+  const int group = (h->pcg_p16 && tuning().defer_z && (h->d.ld % 4) == 0 && iters_per_launch <= kMaxGroup)
+                        ? iters_per_launch : 0;
+  const int slots = group > 0 ? group + 1 : 1;
+  if (h->pcg_p16 && (!h->P16 || h->p16_slots < slots)) {
+    if (h->P16) (void)hipFree(h->P16);
+    h->P16 = nullptr;
     h->ldh = (C1 + 7) / 8 * 8;
-    const size_t nh = (size_t)h->d.n * h->ldh;
+    const size_t nh = (size_t)h->d.n * h->ldh * slots;
     DLP_CHECK_HIP(hipMalloc((void**)&h->P16, nh * sizeof(uint16_t)));
     DLP_CHECK_HIP(hipMemsetAsync(h->P16, 0, nh * sizeof(uint16_t), st));
+    h->p16_slots = slots;
+    if (h->pcg_exec) {   // captured iterations address the old block
+      (void)hipGraphExecDestroy(h->pcg_exec);
+      h->pcg_exec = nullptr;
+    }
   }
+  if (group > 0 && h->ahist_len < group * C1 + 1) {
+    if (h->pcg_ahist) (void)hipFree(h->pcg_ahist);
+    h->pcg_ahist = nullptr;
+    const int len = (group * C1 + 2) & ~1;
+    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_ahist, len * sizeof(float)));
+    h->ahist_len = len;
+  }
+  if (group != h->pcg_group && h->pcg_exec) {   // captured with the other update scheme
+    (void)hipGraphExecDestroy(h->pcg_exec);
+    h->pcg_exec = nullptr;
+  }
+  h->pcg_group = group;
   int rc = coarse_prepare(h, st);
   if (rc) return rc;
   const PcgBufs b = pcg_bufs(h);
@@ -1980,10 +2027,8 @@ extern "C" int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int 
     if (use_graph) {
       DLP_CHECK_HIP(hipGraphLaunch(h->pcg_exec, st));
     } else {
-      for (int i = 0; i < iters_per_launch; ++i) {
-        rc = pcg_iteration(h, b, st);
-        if (rc) return rc;
-      }
+      rc = pcg_group_launch(h, b, iters_per_launch, st);
+      if (rc) return rc;
     }
     launched += iters_per_launch;
     DLP_CHECK_HIP(hipEventRecord(h->ev1, st));
@@ -2200,6 +2245,7 @@ extern "C" int dlp_lp_bpcg(void* handle, int max_iter, int iters_per_launch, int
   PcgBufs b = pcg_bufs(h);
   b.P16 = nullptr;                                 // block PCG keeps FP32 directions
   b.p16 = 0;
+  b.slot_stride = 0;
   b.na = 0;                                        // and the Jacobi preconditioner
   int rc = pcg_start(b, st);                       // P = R/diag
   if (!rc) rc = dlp_gram(b.n, C1, b.R, b.ld, b.R, b.ld, b.diag, h->bk, DLP_F32, st);   // G = R^T D^-1 R

@@ -73,11 +73,14 @@ class EdgeStreamLP:
     power-of-two scale per column, 11 significant bits, step lengths exact for the rounded
     direction), "bf16" (half the bytes, 8 significant bits) or "fp32".  pcg_coarse: aggregates of
     the two-level preconditioner of the global solve (runs of consecutive rows; 0 = Jacobi only),
-    its coarse operator rebuilt from the current equations every coarse_refresh global solves."""
+    its coarse operator rebuilt from the current equations every coarse_refresh global solves.
+    pcg_iters_per_launch: global-solve iterations per replayed graph (iterations past
+    convergence return at once; with 16-bit directions each group adds its directions to Z once,
+    at its end), 7 by default: a 10,000-edge repair of SBM-1M takes 6-7."""
 
     def __init__(self, g: Graph, eps=1e-6, eps_h=0.05, fp64=False, eta=0.0, slack_min=4,
                  slack_frac=0.25, push_scale=0.9, rounds_per_launch=4, use_graph=True, max_rounds=200_000,
-                 push_frac=0.05, push_max_rows=4096, push_budget=1.0, push_mass=0.15, pcg_iters_per_launch=4,
+                 push_frac=0.05, push_max_rows=4096, push_budget=1.0, push_mass=0.15, pcg_iters_per_launch=7,
                  pcg_max_iter=20_000, pcg_mode="column", incremental_ground=True, pcg_directions="fp16", pcg_coarse=128,
                  coarse_refresh=64):
         self.g, self.n, self.K = g, g.n, g.K

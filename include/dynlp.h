@@ -83,7 +83,11 @@ int dlp_row_operator_dot(int n, int C, const int32_t* indptr, const int32_t* row
  *   key 7 = BF16 direction values per lane of that operator (4 default: 8-byte words; 8);
  *   key 8 = workgroups of the flat PCG vector passes (0 default = 2048);
  *   key 9 = non-temporal FP32 streams (Z, R, AP) in the PCG update and direction passes
- *           (1 default: the caches keep the direction block the next operator gathers). */
+ *           (1 default: the caches keep the direction block the next operator gathers);
+ *   key 10 = deferred solution updates with 16-bit directions (1 default: each captured group
+ *           of at most 8 global-solve iterations keeps its directions, one per slot, and adds
+ *           sum alpha_s P_s to Z once at its end; 0, or longer groups: Z updated by every
+ *           iteration). */
 int dlp_tune_set(int key, int value);
 
 /* ---- Column residual ratios ------------------------------------------------------------

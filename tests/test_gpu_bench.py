@@ -21,7 +21,7 @@ def test_bench_line_and_csv_rows(tmp_path, capsys):
     assert line["steps"] == 3 and line["higher_is_better"] is True
     roof = line["roofline"]
     assert roof["bound"] == "hbm" and roof["unit"] == "GB/s" and roof["peak"] == 8000.0
-    assert roof["achieved"] > 0 and roof["frac"] == pytest.approx(roof["achieved"] / roof["peak"], rel=1e-3)
+    assert roof["achieved"] > 0 and roof["frac"] == pytest.approx(roof["achieved"] / roof["peak"], abs=1e-4)
     cpu = line["cpu_baseline"]
     assert cpu["kind"] == "port" and cpu["cores"] == 1 and cpu["value"] > 0
     assert line["fp64_path"]["value"] > 0 and line["rho_max"] <= 1e-6 * 1.0001

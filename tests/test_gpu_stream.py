@@ -209,6 +209,41 @@ def test_pcg_group_length_does_not_change_the_solve():
     np.testing.assert_allclose(Z1[:, :C], Z7[:, :C], rtol=0, atol=2.1e-6 * hmax)
 
 
+@pytest.mark.parametrize("dirs", ["fp16", "bf16"])
+@pytest.mark.parametrize("per", [1, 4, 8])
+def test_deferred_solution_updates_match_per_iteration_updates(dirs, per):
+    """Deferred solution updates (Z += sum_s alpha_s P_s once per captured group of iterations,
+    tuning key 10) make the same FP32 multiply-adds, in the same order, as the update inside
+    every iteration, for groups of 1, 4 and 8 iterations.  Two runs differ only through the
+    order of the FP64 column sums (atomics across workgroups), so the check is: the same
+    iteration counts (to one) over an initial solve and three global repairs, both states
+    certified, and each within the certificate of the other."""
+    from dynlp_amd import _lib
+    from dynlp_amd.stream import EdgeStreamLP
+    g = graphs.sbm(4000, 8, 10, 0.8, 8, 0.02, seed=12)
+    batches, _ = graphs.update_stream(g, 3, 400, seed=12)
+    res = []
+    try:
+        for defer in (0, 1):
+            _lib.call("dlp_tune_set", 10, defer)
+            lp = EdgeStreamLP(g, eps=1e-6, pcg_directions=dirs, pcg_iters_per_launch=per, push_max_rows=0)
+            lp.initial_solve()
+            its = []
+            for b in batches:
+                st = lp.apply_and_solve(b)
+                assert "pcg" in st.path and st.rho <= 1e-6 * 1.0001
+                its.append(st.extra.get("pcg_iters", 0))
+            res.append((its, lp.scores_with_h()))
+            del lp
+    finally:
+        _lib.call("dlp_tune_set", 10, 1)
+    (i0, Z0), (i1, Z1) = res
+    assert all(abs(a - b) <= 1 for a, b in zip(i0, i1)) and min(i0) > 0, (i0, i1)
+    C = Z0.shape[1] - 1
+    hmax = max(Z0[:, C].max(), Z1[:, C].max())
+    np.testing.assert_allclose(Z0[:, :C], Z1[:, :C], rtol=0, atol=2.1e-6 * hmax)
+
+
 @pytest.mark.parametrize("K", [2, 8])
 def test_error_report_against_oracle(K):
     """error_report: relative residual equal to the oracle's on the same state, certified bound
