@@ -321,6 +321,7 @@ struct RefreshArgs {
   int32_t* viol;
   int32_t* nviol;
   double* rho;             // C1 column maxima, or NULL
+  double* hmax;            // the verification pass: max over solved rows of h (column C1-1), or NULL
   double* mass;            // sum over violating rows of max_c |R_c|/(eps_c*diag), or NULL
 };
 
@@ -667,6 +668,7 @@ struct VerifyEpilogue {
   const float* __restrict__ Z;
   float* __restrict__ R;
   double rmax[4] = {0.0, 0.0, 0.0, 0.0};
+  float hm = 0.f;          // max h over this lane's solved rows (the lane holding column C1-1)
   float4 zr;
   __device__ __forceinline__ bool owner() const { return L.e == 0 && 4 * L.l < A.C1; }
   __device__ __forceinline__ void pre(int, int r) {
@@ -690,6 +692,7 @@ struct VerifyEpilogue {
           const double ratio = fabs(res) / dg;
           viol |= ratio > A.eps[c];
           rmax[q] = fmax(rmax[q], ratio);
+          if (c == C) hm = fmaxf(hm, zv[q]);
         }
       }
     }
@@ -707,8 +710,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   run_rows<S, Word::F32x4, double>(G, E);
   if (!A.rho) return;
   __shared__ double sm[256];
+  __shared__ double shm;
   for (int c = threadIdx.x; c < A.C1; c += blockDim.x) sm[c] = 0.0;
+  if (threadIdx.x == 0) shm = 0.0;
   __syncthreads();
+  if (A.hmax) {
+    float hm = E.hm;
+    for (int off = 32; off > 0; off >>= 1) hm = fmaxf(hm, __shfl_xor(hm, off, 64));
+    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&shm, (double)fmaxf(hm, 0.f));
+  }
   if (E.owner()) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -718,6 +728,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   }
   __syncthreads();
   for (int c = threadIdx.x; c < A.C1; c += blockDim.x) atomic_max_nonneg(A.rho + c, sm[c]);
+  if (A.hmax && threadIdx.x == 0) atomic_max_nonneg(A.hmax, shm);
 }
 
 // Grid-stride over the listed rows in wave-uniform steps (the row count may live on the device).
@@ -1091,7 +1102,8 @@ __global__ void k_status_gather(const int32_t* __restrict__ cnt, const int32_t* 
     out[CNT_N + 2] = (int64_t)*edges;
     out[CNT_N + 9] = __double_as_longlong(*mass);
   }
-  if (t < 4) out[CNT_N + 3 + t] = (int64_t)stats[t];
+  if (t < 3) out[CNT_N + 3 + t] = (int64_t)stats[t];
+  if (t == 3) out[CNT_N + 6] = rho ? __double_as_longlong(rho[C1]) : 0;   // the verification's h maximum
   if (rho)
     for (int c = t; c < C1; c += blockDim.x) reinterpret_cast<double*>(out + kStatRho)[c] = rho[c];
 }
@@ -1179,6 +1191,7 @@ RefreshArgs refresh_args(LP* h, int which) {
   A.viol = h->viol;
   A.nviol = h->cnt + CNT_VIOL;
   A.rho = h->rho;
+  A.hmax = nullptr;
   A.mass = which == 0 ? h->mass : nullptr;
   return A;
 }
@@ -1206,7 +1219,7 @@ extern "C" int dlp_lp_create(const dlp_lp_desc* d, void** handle) {
 #define A_(p, c) if (!rc) rc = dev_alloc(&(p), (c))
   A_(h->stamp, n); A_(h->rows, n); A_(h->cnt, CNT_N); A_(h->overflow, 1 << 20);
   A_(h->toggled, n); A_(h->parent, n); A_(h->roots, n); A_(h->grounded, n); A_(h->viol, n);
-  A_(h->eps64, d->C1); A_(h->rho, d->C1); A_(h->mark, n); A_(h->cand64, n);
+  A_(h->eps64, d->C1); A_(h->rho, d->C1 + 1); A_(h->mark, n); A_(h->cand64, n);
   A_(h->S.ctag, n); A_(h->S.list[0], n); A_(h->S.list[1], n); A_(h->S.list[2], n);
   A_(h->S.count, 3); A_(h->S.rnd, 2); A_(h->S.done, 1); A_(h->S.btag, n); A_(h->S.blist, n);
   A_(h->S.bcount, 1); A_(h->S.edges, 1); A_(h->S.stats, 4);
@@ -1516,7 +1529,7 @@ extern "C" int dlp_lp_ground_flags(void* handle, int32_t* out, void* stream) {
 // Clears the refresh outputs (violator count, column maxima, push mass) in one launch.
 __global__ void k_refresh_reset(int32_t* __restrict__ nviol, double* __restrict__ rho, int C1,
                                 double* __restrict__ mass) {
-  for (int c = threadIdx.x; c < C1; c += blockDim.x) rho[c] = 0.0;
+  for (int c = threadIdx.x; c <= C1; c += blockDim.x) rho[c] = 0.0;   // C1 maxima and the h maximum
   if (threadIdx.x == 0) {
     *nviol = 0;
     if (mass) *mass = 0.0;
@@ -1537,6 +1550,7 @@ extern "C" int dlp_lp_refresh(void* handle, int which, int prec, void* stream) {
     if (prec == 0 && (h->d.ld % 4) == 0 && h->d.C1 <= 256 && tuning().vec4 && tuning().pipe &&
         h->d.n < (1 << 24) && xbytes < (1LL << 31)) {
       RefreshArgs A = refresh_args(h, 2);
+      A.hmax = h->rho + h->d.C1;
       const int xcd = tuning().xcd != 0;
       const void* kfn = reinterpret_cast<const void*>(&k_verify<4>);
       long long g = resident_grid(kfn);

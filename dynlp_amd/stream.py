@@ -167,7 +167,8 @@ class EdgeStreamLP:
         return dict(changed=int(s[0]), missing=int(s[1]), overflow=int(s[2]), toggled=int(s[3]), viol=int(s[4]),
                     touched=int(s[6]), round=int(s[7]), edges=int(s[8]), cand_total=int(s[9]),
                     push_total=int(s[10]), scatter_total=int(s[11]),
-                    rounds_launched=int(s[13]), push_us=int(s[14]), mass=float(s[15:16].view(np.float64)[0]))
+                    rounds_launched=int(s[13]), push_us=int(s[14]), mass=float(s[15:16].view(np.float64)[0]),
+                    hmax=float(s[12:13].view(np.float64)[0]))
 
     def _push(self, budget=0):
         r, ok = ct.c_int64(0), ct.c_int32(0)
@@ -366,28 +367,33 @@ class EdgeStreamLP:
     def _finish(self, st, s):
         """Request statistics, with the certificate of solvers.py:295-301 as the reference's
         incremental solver reports it per batch (solvers.py:421-422): column maxima rho_c of
-        |R_uc|/diag_u over every row, h_est = max h over every row (parked rows hold h = 1),
-        hmax = h_est / (1 - rho_h) and cert = max_c<C rho_c * hmax (inf while rho_h > eps_h)."""
+        |R_uc|/diag_u over every row, h_est = max h over every row (the reference parks rows
+        outside the solved set at h = 1, solvers.py:365-368; solved rows have h >= 1),
+        hmax = h_est / (1 - rho_h) and cert = max_c<C rho_c * hmax (inf while rho_h > eps_h).
+        After a clean verification of every solved row both come with its status; otherwise one
+        pass over the kept residual and one over the h column, read back together."""
         st.edges, st.touched, st.changed, st.toggled = s["edges"], s["touched"], s["changed"], s["toggled"]
         C, C1 = self.C, self.C1
-        dev = device()
-        if getattr(self, "_cert_buf", None) is None:
-            self._cert_buf = torch.zeros(C1 + 1, dtype=torch.float64, device=dev)
-            self._ones64 = torch.ones(self.n, dtype=torch.float64, device=dev)
-        buf = self._cert_buf
-        # after a clean verification of every row (or on the FP64 path, a pass over the exact
-        # kept residual) the column maxima are already on the host; else one pass over R
-        full = self.fp64 or getattr(self, "_rho_all", False)
-        if not full:
-            _lib.call("dlp_rho_cols", self.n, None, C1, ptr(self.R32), self.ld, _lib.F32, ptr(self.diag64),
-                      ptr(buf), stream())
-        Zf = self.Z64 if self.fp64 else self.Z32
-        _lib.call("dlp_rho_cols", self.n, None, 1, ptr(Zf) + C * Zf.element_size(), self.ld,
-                  _lib.F64 if self.fp64 else _lib.F32, ptr(self._ones64), ptr(buf) + 8 * C1, stream())
-        host = buf.cpu().numpy()
-        if not full:
-            self._rho[:] = host[:C1]
-        rc, h_est = self._rho, float(host[C1])
+        if getattr(self, "_rho_all", False) and not self.fp64 and s.get("hmax", 0.0) > 0.0:
+            rc, h_est = self._rho, s["hmax"]
+        else:
+            dev = device()
+            if getattr(self, "_cert_buf", None) is None:
+                self._cert_buf = torch.zeros(C1 + 1, dtype=torch.float64, device=dev)
+                self._ones64 = torch.ones(self.n, dtype=torch.float64, device=dev)
+            buf = self._cert_buf
+            # the FP64 path's column maxima come from a pass over its exact kept residual
+            if not self.fp64:
+                _lib.call("dlp_rho_cols", self.n, None, C1, ptr(self.R32), self.ld, _lib.F32, ptr(self.diag64),
+                          ptr(buf), stream())
+            Zf = self.Z64 if self.fp64 else self.Z32
+            _lib.call("dlp_rho_cols", self.n, None, 1, ptr(Zf) + C * Zf.element_size(), self.ld,
+                      _lib.F64 if self.fp64 else _lib.F32, ptr(self._ones64), ptr(buf) + 8 * C1, stream())
+            host = buf.cpu().numpy()
+            if not self.fp64:
+                self._rho[:] = host[:C1]
+            rc, h_est = self._rho, float(host[C1])
+        h_est = max(h_est, 1.0)
         st.rho = float(rc[:C].max())
         if rc[C] > self.eps_h:
             st.hmax, st.cert = h_est, float("inf")

@@ -349,7 +349,9 @@ int dlp_lp_violators(void* handle, int32_t** list, int32_t** count);
 /* Counters (host; synchronises): [0] changed rows, [1] missing deletions, [2] overflowed
  * insertions, [3] toggled vertices, [4] violators, [5] FP64-path candidates, [6] rows touched
  * by pushes, [7] round number, [8] edge visits of this batch, cumulative push work [9]
- * candidates, [10] pushes, [11] entries scattered by pushes, [12] reserved,
+ * candidates, [10] pushes, [11] entries scattered by pushes, [12] the bits of an FP64 h
+ * maximum: max over the solved rows of h (column C1-1) from the last refresh when it was a
+ * verification of every solved row (mode 3 on the 16-byte FP32 layout), else 0,
  * [13] push rounds launched, [14] their summed duration in microseconds (HIP events),
  * [15] the bits of an FP64 "push mass": over the rows the last changed-row refresh found
  * violating, the sum of max_c |R_c|/(eps_c*diag) (about the row pushes a repair by push alone
