@@ -244,6 +244,36 @@ def test_deferred_solution_updates_match_per_iteration_updates(dirs, per):
     np.testing.assert_allclose(Z0[:, :C], Z1[:, :C], rtol=0, atol=2.1e-6 * hmax)
 
 
+def test_launches_without_graphs_match_replayed_graphs():
+    """use_graph=False launches the push rounds and the groups of global-solve iterations (with
+    their deferred solution updates) one kernel at a time: the same paths, the same iteration
+    counts (to one) and both states within each other's certificate, over small requests (push
+    rounds first) and large ones (global solve only)."""
+    from dynlp_amd.stream import EdgeStreamLP
+    g = graphs.sbm(30000, 8, 10, 0.8, 8, 0.02, seed=13)
+    batches, _ = graphs.update_stream(g, 3, 3000, seed=13)
+    small, _ = graphs.update_stream(g, 2, 4, seed=14)
+    res = []
+    for use_graph in (True, False):
+        # no mass limit: the small requests repair by push, the large ones exceed the row limit
+        lp = EdgeStreamLP(g, eps=1e-6, use_graph=use_graph, pcg_iters_per_launch=4, push_mass=1e12)
+        lp.initial_solve()
+        its, paths = [], []
+        for b in small + batches:
+            st = lp.apply_and_solve(b)
+            assert st.rho <= 1e-6 * 1.0001 and np.isfinite(st.cert)
+            its.append(st.extra.get("pcg_iters", 0))
+            paths.append(st.path)
+        res.append((its, paths, lp.scores_with_h()))
+        del lp
+    (i0, p0, Z0), (i1, p1, Z1) = res
+    assert p0 == p1 and any(p.startswith("push") for p in p0) and "pcg" in p0, (p0, p1)
+    assert all(abs(a - b) <= 1 for a, b in zip(i0, i1)), (i0, i1)
+    C = Z0.shape[1] - 1
+    hmax = max(Z0[:, C].max(), Z1[:, C].max())
+    np.testing.assert_allclose(Z0[:, :C], Z1[:, :C], rtol=0, atol=2.1e-6 * hmax)
+
+
 @pytest.mark.parametrize("K", [2, 8])
 def test_error_report_against_oracle(K):
     """error_report: relative residual equal to the oracle's on the same state, certified bound
