@@ -226,7 +226,11 @@ class EdgeStreamLP:
             self._rho[:] = self._rho_dev.cpu().numpy()
             return s
         self._rho_all = False
-        s = self.status()
+        # a batch of more edges than push_max_rows changes more rows than a push may start from
+        # (two per edge, nearly all violating at eps = 1e-6): it goes to the global solve
+        # without waiting for the violator count
+        force_global = force_global or getattr(self, "_batch_edges", 0) > self.push_max_rows
+        s = None if force_global else self.status()
         ok = False
         # push only when it can finish cheaply: few violators, and a residual mass (sum of the
         # violators' max |R|/(eps*diag), about the row pushes a push-only repair takes) that
@@ -339,6 +343,7 @@ class EdgeStreamLP:
         st = Stats(path="fp64-push" if self.fp64 else "push")
         t0 = time.perf_counter()
         du, dv, iu, iv, iw = self._upload(b)
+        self._batch_edges = int(du.shape[0]) + int(iu.shape[0])
         _lib.call("dlp_lp_begin_batch", self.h, stream())
         _lib.call("dlp_lp_apply", self.h, int(du.shape[0]), ptr(du), ptr(dv), int(iu.shape[0]), ptr(iu), ptr(iv),
                   ptr(iw), stream())
