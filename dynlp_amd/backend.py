@@ -162,3 +162,27 @@ def csr(data, indices, indptr, shape):
 
 def csr_from_host(indptr, indices, data, n, dtype=torch.float32):
     return CSR(asarray(data, dtype), asarray(indices, torch.int32), asarray(indptr, torch.int32), (n, n))
+
+
+class _SparseModule:
+    """`sp` of the reference's backend (backend.py:6-7, its array module's scipy.sparse): here
+    only the CSR constructor, over device tensors."""
+
+    @staticmethod
+    def csr_matrix(arg, shape=None, dtype=None):
+        """csr_matrix((data, indices, indptr), shape=...) or csr_matrix(<SciPy sparse matrix>):
+        a CSR of device tensors (int32 indices, FP32 or FP64 values)."""
+        if isinstance(arg, tuple):
+            data, indices, indptr = arg
+            if shape is None:
+                raise ValueError("csr_matrix((data, indices, indptr)) needs shape")
+        else:
+            m = arg.tocsr()
+            data, indices, indptr, shape = m.data, m.indices, m.indptr, m.shape
+        src = dtype if dtype is not None else getattr(data, "dtype", "float32")
+        dt = torch.float64 if str(src) in ("float64", "torch.float64", "<class 'numpy.float64'>") else torch.float32
+        return CSR(asarray(data, dt), asarray(indices, torch.int32), asarray(indptr, torch.int32),
+                   (int(shape[0]), int(shape[1])))
+
+
+sp = _SparseModule()

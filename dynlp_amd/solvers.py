@@ -53,10 +53,19 @@ class System:
     eta: float = 0.0
     is_new: torch.Tensor | None = None
     seed: torch.Tensor | None = None
+    t: int = 0                # batch number of a stream
+    comps: dict | None = None  # new vertices' supernode components (not computed here)
+    n_inserted: int = 0
+    n_deleted: int = 0
+    n_active: int = 0
+    n_ungrounded: int = 0
+    solved: torch.Tensor | None = None    # global-index systems: rows really solved (others inert)
+    changed: torch.Tensor | None = None   # global-index systems: rows whose equation changed
     _conv: dict = field(default_factory=dict, repr=False, compare=False)
 
     n_u = property(lambda self: int(self.U.shape[0]))
     nnz = property(lambda self: self.W.nnz)
+    n_solved = property(lambda self: self.n_u if self.solved is None else int(self.solved.sum()))
 
     def label_cols(self, M):
         return M[:, 1:2] if self.K == 2 else M
@@ -72,16 +81,17 @@ class System:
         return self._conv[dt]
 
 
-def build_system(g, dtype="float64", eta=0.0, active=None) -> System:
+def build_system(g, dtype="float64", eta=0.0, active=None, labeled=None) -> System:
     """Stream._build (stream.py:142-177) for the graph's current edge set: U = active, unlabeled
     and grounded (components of the unlabeled subgraph with labeled contact, found with the
-    device CC); W = A restricted to U; s, diag, B, rhs.  Host ingest of the edge list, device
-    grounding and products."""
+    device CC); W = A restricted to U; s, diag, B, rhs.  labeled overrides the graph's labeled
+    set (a stream draws its own).  Host ingest of the edge list, device grounding and products."""
     dt = torch.float64 if dtype in ("float64", torch.float64) else torch.float32
     n, K = g.n, g.K
     act = np.ones(n, dtype=bool) if active is None else np.asarray(active, dtype=bool)
-    lab = g.labeled & act
-    unl = act & ~g.labeled
+    glab = g.labeled if labeled is None else np.asarray(labeled, dtype=bool)
+    lab = glab & act
+    unl = act & ~glab
     u, v, w = g.u, g.v, g.w.astype(np.float64)
     # class weights to labeled neighbours
     Bfull = np.zeros((n, K))

@@ -24,6 +24,7 @@ import torch
 from . import _lib
 from .backend import device, ptr, stream
 from .graphs import EdgeBatch, Graph
+from .solvers import System  # noqa: F401  (the batch system, reference stream.py:17-58)
 
 
 @dataclass
@@ -494,3 +495,198 @@ class EdgeStreamLP:
         W = CSR(self.w, self.col, self.indptr, (self.n, self.n), rowlen=self.rowlen)
         lab, _ = _cc.connected_components_csr(W)
         return lab.cpu().numpy()
+
+
+# ---------------------------------------------------------------------------------------------
+# The reference's vertex-arrival protocol (stream.py:84-346), producing systems for the
+# reference-API solvers of solvers.py.  Host schedule; systems built on the device.
+
+def _arrival_order(g: Graph, seed: int) -> np.ndarray:
+    """The graph's arrival order (an edge-list file's `order`), else the permutation numpy's
+    RandomState(seed + 1) draws (graphs.from_npz's default)."""
+    order = g.meta.get("order")
+    if order is None:
+        order = np.random.RandomState(seed + 1).permutation(g.n)
+    return np.asarray(order, dtype=np.int64)
+
+
+def _stream_labels(y, K, n, label_frac, rs) -> np.ndarray:
+    """The stream's labeled set: the first max(K, label_frac*n) of one random permutation of
+    the vertices with known ground truth (y >= 0)."""
+    known = np.flatnonzero(y >= 0)
+    lab = np.zeros(n, dtype=bool)
+    lab[known[rs.permutation(known.shape[0])[: max(K, int(label_frac * n))]]] = True
+    return lab
+
+
+def _seat_classes(lab, y, K, order, n0, rs) -> None:
+    """Every class gets a labeled vertex inside the initial snapshot order[:n0] (in place): a
+    class without one labels its lowest-id vertex; a class whose labeled vertices all arrive
+    later moves its lowest-id labeled vertex into the snapshot, trading places with an unlabeled
+    snapshot vertex picked by one randint draw."""
+    slot_of = np.empty(order.shape[0], dtype=np.int64)
+    slot_of[order] = np.arange(order.shape[0])
+    for c in range(K):
+        members = np.flatnonzero(lab & (y == c))
+        if members.shape[0] == 0:
+            members = np.flatnonzero(y == c)[:1]
+            lab[members] = True
+        if members.shape[0] == 0 or int(slot_of[members].min()) < n0:
+            continue
+        open_slots = np.flatnonzero(~lab[order[:n0]])
+        j = int(open_slots[int(rs.randint(0, open_slots.shape[0]))])
+        v = int(members[0])
+        u, sv = int(order[j]), int(slot_of[v])
+        order[j], order[sv] = v, u
+        slot_of[v], slot_of[u] = j, sv
+
+
+class Stream:
+    """A graph revealed in batches by the reference's protocol (stream.py:84-140): an initial
+    snapshot of max(init_frac*n, 2K) vertices in arrival order, then n_batches batches of new
+    vertices, each with deletions of del_frac x its size random active unlabeled vertices.  The
+    labeled set is the stream's own (max(K, label_frac*n) vertices with known ground truth, every
+    class present in the snapshot); all draws come from numpy's RandomState(seed + 7), in the
+    reference's order.  batches() yields one System per batch (local ids over U, built on the
+    device by solvers.build_system) with the reference's bookkeeping: t, is_new (rows not in U
+    at the previous batch), seed (is_new, or adjacent to an inserted or deleted vertex),
+    n_inserted, n_deleted, n_active, n_ungrounded.  The new vertices' supernode components are
+    not built (comps is None): the solvers here start new rows from the prior."""
+
+    def __init__(self, g: Graph, *, label_frac=0.01, init_frac=0.1, n_batches=10, del_frac=0.1,
+                 eta_rel=0.0, seed=0, dtype="float32"):
+        self.g, self.dtype, self.del_frac = g, dtype, float(del_frac)
+        n, K, y = g.n, g.K, np.asarray(g.y)
+        self.rs = np.random.RandomState(seed + 7)
+        self.labeled = _stream_labels(y, K, n, label_frac, self.rs)
+        self.order = _arrival_order(g, seed).copy()
+        self.n0 = max(int(init_frac * n), 2 * K)
+        _seat_classes(self.labeled, y, K, self.order, self.n0, self.rs)
+        self.y = y
+        self.chunks = np.array_split(self.order[self.n0:], n_batches) if n_batches > 0 else []
+        self.indptr, self.indices, self.data = g.csr()
+        self.entry_row = np.repeat(np.arange(n, dtype=np.int64), np.diff(self.indptr))
+        # eta_rel times the mean weighted degree (both directions of every edge)
+        self.eta = float(eta_rel * 2.0 * float(np.sum(g.w, dtype=np.float64)) / n)
+        self.schedule = self._plan()
+
+    def _plan(self):
+        """(inserted, deleted) vertices of every batch: batch 0 inserts the snapshot; batch t
+        inserts chunk t-1 and deletes min(del_frac*|chunk|, |pool| - 1) vertices of the pool of
+        active unlabeled ones (one permutation draw of the pool when it deletes any)."""
+        n = self.g.n
+        active = np.zeros(n, dtype=bool)
+        none = np.zeros(0, dtype=np.int64)
+        plan = []
+        for t in range(1 + len(self.chunks)):
+            ins = self.order[: self.n0] if t == 0 else self.chunks[t - 1]
+            dele = none
+            if t:
+                pool = np.flatnonzero(active & ~self.labeled)
+                nd = min(int(self.del_frac * ins.shape[0]), max(pool.shape[0] - 1, 0))
+                if nd > 0:
+                    dele = pool[self.rs.permutation(pool.shape[0])[:nd]]
+            active[ins] = True
+            active[dele] = False
+            plan.append((np.asarray(ins, dtype=np.int64), np.asarray(dele, dtype=np.int64)))
+        return plan
+
+    def __len__(self):
+        return len(self.schedule)
+
+    def _adjacent(self, V) -> np.ndarray:
+        """Vertices with a neighbour in V (in the whole graph, active or not)."""
+        mark = np.zeros(self.g.n, dtype=bool)
+        mark[V] = True
+        adj = np.zeros(self.g.n, dtype=bool)
+        adj[self.entry_row[mark[self.indices]]] = True
+        return adj
+
+    def batches(self):
+        n = self.g.n
+        active = np.zeros(n, dtype=bool)
+        prevU = np.zeros(n, dtype=bool)
+        for t, (ins, dele) in enumerate(self.schedule):
+            active[ins] = True
+            active[dele] = False
+            sysm = self._build(t, active, prevU, ins, dele)
+            prevU = np.zeros(n, dtype=bool)
+            prevU[sysm.U.cpu().numpy()] = True
+            yield sysm
+
+    def _build(self, t, active, prevU, ins, dele):
+        from .solvers import build_system
+        dev = device()
+        sysm = build_system(self.g, self.dtype, self.eta, active=active, labeled=self.labeled)
+        U = sysm.U.cpu().numpy()
+        is_new = ~prevU[U]
+        seed = is_new | self._adjacent(np.concatenate([ins, dele]))[U]
+        sysm.t = t
+        sysm.is_new = torch.as_tensor(is_new, device=dev)
+        sysm.seed = torch.as_tensor(seed, device=dev)
+        sysm.n_inserted, sysm.n_deleted = int(ins.shape[0]), int(dele.shape[0])
+        sysm.n_active = int(active.sum())
+        sysm.n_ungrounded = int((active & ~self.labeled).sum()) - int(U.shape[0])
+        return sysm
+
+
+class IncrementalStream(Stream):
+    """The same stream as global-index systems (stream.py:180-346): every vertex keeps its row;
+    rows outside U (inactive, labeled or ungrounded) are inert (diag 1, rhs 0, their entries of
+    the graph's fixed sparsity weighted 0), so every solver runs unchanged; `solved` marks U and
+    `changed` lists the rows of U whose equation changed in the batch (the vertices that entered
+    U and the neighbours of those that entered or left it or were labeled and arrived or left).
+    The reference keeps these arrays in place and regrounds with local ball checks backed by a
+    full check; both give the exact grounded set, from which each batch is rebuilt here.  The
+    class weights, s, diag and rhs are FP64 masters; a float32 stream's system returns them from
+    astype("float64"), as the reference's cached FP64 twin does."""
+
+    def __init__(self, g: Graph, *, ball_hops=3, **kw):
+        super().__init__(g, **kw)
+        self.ball_hops = ball_hops
+
+    def batches(self):
+        from .backend import CSR
+        from .solvers import System, build_system
+        g, n, K, dev = self.g, self.g.n, self.g.K, device()
+        C = 1 if K == 2 else K
+        rows, cols = self.entry_row, self.indices.astype(np.int64)
+        data64 = self.data.astype(np.float64)
+        prior64 = np.full(C, 0.5 if K == 2 else 1.0 / K)
+        ids = torch.arange(n, dtype=torch.int64, device=dev)
+        indptr = torch.as_tensor(self.indptr, device=dev)
+        indices = torch.as_tensor(self.indices, device=dev)
+        dt = torch.float64 if self.dtype in ("float64", torch.float64) else torch.float32
+        active = np.zeros(n, dtype=bool)
+        prevU = np.zeros(n, dtype=bool)
+        for t, (ins, dele) in enumerate(self.schedule):
+            active[ins] = True
+            active[dele] = False
+            Um = np.zeros(n, dtype=bool)
+            Um[build_system(g, "float64", self.eta, active=active, labeled=self.labeled).U.cpu().numpy()] = True
+            w64 = np.where(Um[rows] & Um[cols], data64, 0.0)
+            labact = active & self.labeled
+            B64 = np.zeros((n, K))
+            sel = labact[cols]
+            np.add.at(B64, (rows[sel], self.y[cols[sel]]), data64[sel])
+            rowsum = np.bincount(rows, weights=w64, minlength=n)
+            s64 = np.where(Um, B64.sum(axis=1) + self.eta, 1.0)
+            diag64 = np.where(Um, s64 + rowsum, 1.0)
+            rhs64 = np.where(Um[:, None], (B64[:, 1:2] if K == 2 else B64) + self.eta * prior64[None, :], 0.0)
+            entering = Um & ~prevU
+            moved = np.flatnonzero(entering | (prevU & ~Um))
+            status = np.concatenate([moved, ins[self.labeled[ins]], dele[self.labeled[dele]]])
+            changed = np.flatnonzero((self._adjacent(status) | entering) & Um)
+            seed = (entering | self._adjacent(np.concatenate([ins, dele]))) & Um
+            t64 = lambda a: torch.as_tensor(a, device=dev)  # noqa: E731
+            sys64 = System(ids, CSR(t64(w64), indices, indptr, (n, n)), t64(s64), t64(diag64), t64(B64),
+                           t64(np.ascontiguousarray(rhs64)), t64(prior64), K, C, self.eta,
+                           is_new=t64(entering), seed=t64(seed), t=t, comps=None, n_inserted=int(ins.shape[0]),
+                           n_deleted=int(dele.shape[0]), n_active=int(active.sum()),
+                           n_ungrounded=int((active & ~self.labeled & ~Um).sum()), solved=t64(Um),
+                           changed=t64(changed))
+            sysm = sys64 if dt == torch.float64 else sys64.astype("float32")
+            if sysm is not sys64:
+                sysm._conv[torch.float64] = sys64
+            prevU = Um
+            yield sysm

@@ -1,0 +1,61 @@
+"""The vertex-arrival schedule of stream.Stream (the reference's stream.py:84-140) on the host:
+labeled set, arrival order and batches.  The exact random draws follow the reference's
+sequence; with no reference run possible here they are parity-unpinned, and these tests check
+what the protocol fixes."""
+import numpy as np
+import pytest
+
+from dynlp_amd import graphs
+from dynlp_amd.stream import IncrementalStream, Stream
+
+
+@pytest.mark.parametrize("K", [2, 5])
+def test_schedule_invariants(K):
+    g = graphs.sbm(3000, K, 6, 0.85, K, 0.02, seed=7)
+    st = Stream(g, init_frac=0.2, n_batches=5, del_frac=0.3, label_frac=0.01, seed=4)
+    n = g.n
+    assert st.labeled.sum() >= max(K, int(0.01 * n))
+    assert np.array_equal(np.sort(st.order), np.arange(n))
+    assert st.n0 == max(int(0.2 * n), 2 * K)
+    # every class has a labeled vertex in the initial snapshot
+    snap = st.order[: st.n0]
+    assert set(g.y[snap[st.labeled[snap]]]) == set(range(K))
+    assert len(st) == 6
+    ins0, del0 = st.schedule[0]
+    assert np.array_equal(ins0, snap) and del0.size == 0
+    assert np.array_equal(np.concatenate([st.schedule[t][0] for t in range(1, 6)]), st.order[st.n0:])
+    active = np.zeros(n, dtype=bool)
+    active[ins0] = True
+    for t in range(1, 6):
+        ins, dele = st.schedule[t]
+        pool = np.flatnonzero(active & ~st.labeled)
+        assert dele.size == min(int(0.3 * ins.size), max(pool.size - 1, 0))
+        assert np.unique(dele).size == dele.size and np.isin(dele, pool).all()
+        assert not active[ins].any()
+        active[ins] = True
+        active[dele] = False
+
+
+def test_schedule_is_seeded():
+    g = graphs.sbm(2000, 3, 6, 0.85, 3, 0.02, seed=8)
+    a = Stream(g, n_batches=4, seed=1)
+    b = Stream(g, n_batches=4, seed=1)
+    c = Stream(g, n_batches=4, seed=2)
+    assert np.array_equal(a.labeled, b.labeled) and np.array_equal(a.order, b.order)
+    assert all(np.array_equal(x[1], y[1]) for x, y in zip(a.schedule, b.schedule))
+    assert not np.array_equal(a.labeled, c.labeled)
+    # an edge-list file's arrival order is used as given (up to the class seating swaps)
+    order = np.random.default_rng(3).permutation(g.n)
+    g.meta["order"] = order
+    d = Stream(g, n_batches=2, seed=1, init_frac=0.5)
+    assert (d.order != order).sum() <= 2 * g.K
+    del g.meta["order"]
+
+
+def test_unlabeled_ground_truth_and_incremental_subclass():
+    g = graphs.sbm(1000, 2, 6, 0.85, 2, 0.05, seed=9)
+    g.y = g.y.copy()
+    g.y[::3] = -1                      # no ground truth: never labeled by the stream
+    st = IncrementalStream(g, n_batches=2, label_frac=0.05)
+    assert not st.labeled[g.y < 0].any()
+    assert st.ball_hops == 3 and len(st) == 3
