@@ -6,8 +6,9 @@ Same surface as dynlp/solvers.py of the reference: `Stats`, `spmm`, `apply_A`, `
 `SOLVERS`.  Every sweep, product and reduction runs in the HIP kernels of the C ABI; Python
 keeps the reference's control flow (and its host synchronisations).  Warm starts by supernodes
 (`supernode_init`/`init_h`, :128-163) are out of scope: new rows start from the prior and h
-from 1.  The resident incremental method (`dynlp+inc`, IncrementalLP :346-434) is
-`stream.EdgeStreamLP` for edge-level streams.
+from 1.  The resident incremental method (`dynlp+inc`, IncrementalLP :346-434) runs on the
+global-index systems of `stream.IncrementalStream`; for edge-level streams the resident path
+is `stream.EdgeStreamLP`.
 
 A `System` holds the batch system on the active unlabeled set U in local ids (stream.py:17-58):
 W (CSR U x U), s, diag, B, rhs, prior.
@@ -24,7 +25,7 @@ import torch
 from . import _lib
 from .backend import CSR, Timer, device, dtype_code, ptr, stream
 from .cc import connected_components
-from .kernels import FrontierOps, spmm_axpy
+from .kernels import FrontierOps, FusedPush, spmm_axpy
 
 
 @dataclass
@@ -469,6 +470,104 @@ class DynLPPlus(Solver):
         return st
 
 
+class IncrementalLP(DynLPPlus):
+    """Certified DynLP+ with resident state (solvers.py:346-434), for the global-index systems
+    of stream.IncrementalStream: Z = [F, h] and its FP64 residual R are kept between batches; a
+    batch parks the rows outside the solved set (F = 0, h = 1, R = 0), refreshes R on its
+    changed rows, and when few rows violate repairs them with the fused asynchronous push (up to
+    4 rounds, the thresholds re-tightened from the bound each time), the certificate read off
+    the kept residual; large changes and the first batch go through DynLP+ and a full residual.
+    R is recomputed in full every full_every batches to bound rounding drift.  New rows keep
+    their parked values (no supernode warm start).  The edge-update service path of this package
+    is stream.EdgeStreamLP."""
+
+    def __init__(self, n_total, K, dtype="float32", tol=1e-3, full_every=16, max_rounds=50_000, **kw):
+        super().__init__(n_total, K, dtype, method="auto", tol=tol, **kw)
+        self.name, self.full_every, self.max_rounds = "dynlp+inc", full_every, max_rounds
+        self.Z = self.R = self._fused = None
+        self.since_full = 0
+
+    @staticmethod
+    def _rhs64(s64):
+        return torch.cat([s64.rhs, s64.diag[:, None]], dim=1).contiguous()
+
+    def _park(self, Z, out):
+        """Exact values of inert rows (diag 1, rhs 0; h's rhs is diag): F = 0, h = 1."""
+        Z[out, : self.C] = 0
+        Z[out, self.C] = 1
+
+    def _full_residual(self, s64, st):
+        self.R = (self._rhs64(s64) - apply_A(s64, self.Z)).contiguous()
+        st.edges += s64.nnz
+        self.since_full = 0
+
+    def _refresh_rows(self, s64, Q, st):
+        """R[Q] = rhs[Q] - diag[Q] Z[Q] + W[Q] Z (solvers.py:404-407), one row operator launch."""
+        C1 = self.C + 1
+        rhs = self._rhs64(s64)
+        neg_diag = (-s64.diag).contiguous()
+        W = s64.W
+        Q = Q.to(torch.int32).contiguous()
+        _lib.call("dlp_row_operator", int(Q.shape[0]), ptr(Q), C1, ptr(W.indptr), None, ptr(W.indices), ptr(W.data),
+                  _lib.F64, None, ptr(self.Z), C1, _lib.F64, ptr(neg_diag), 1.0, ptr(rhs), C1, ptr(self.R), C1,
+                  _lib.F64, 0, W.avg_deg, stream())
+        rows = Q.long()
+        st.edges += int((W.indptr[rows + 1] - W.indptr[rows]).sum())
+
+    def _push(self, s64, eps, st):
+        """Repair the violating rows of R; True when every row satisfies eps."""
+        Z, R = self.Z, self.R
+        eps_d = torch.as_tensor(eps, dtype=torch.float64, device=R.device)
+        if self._fused is None or self._fused.W is not s64.W:
+            self._fused = FusedPush(s64.W, Z.shape[1], self.group)
+        viol = (R.abs() > eps_d[None, :] * s64.diag[:, None]).any(dim=1)
+        cand = torch.nonzero(viol).flatten().to(torch.int32)
+        rounds, ok, edges = self._fused.run(Z, R, s64.diag, eps_d, cand, self.max_rounds)
+        st.iters, st.edges = st.iters + rounds, st.edges + edges
+        return ok
+
+    def solve(self, sys):
+        if sys.solved is None:
+            raise ValueError("dynlp+inc needs an IncrementalStream system (bench --incremental)")
+        s64, C, tol, st = sys.astype("float64"), self.C, self.tol, Stats(path="inc-push")
+        with Timer() as tm:
+            out = ~sys.solved.bool()
+            small = self.Z is not None
+            if small:
+                self._park(self.Z, out)
+                self.R[out] = 0
+                if sys.changed is not None and sys.changed.numel():
+                    self._refresh_rows(s64, sys.changed, st)
+                self.since_full += 1
+                if self.since_full >= self.full_every:  # bound rounding drift in the kept residual
+                    self._full_residual(s64, st)
+                eps = [tol / (1.25 * (self.hmax or 1.0))] * C + [self.eps_h]
+                eps_d = torch.as_tensor(eps, dtype=torch.float64, device=self.R.device)
+                viol = (self.R.abs() > eps_d[None, :] * s64.diag[:, None]).any(dim=1)
+                small = float(viol.sum()) / max(sys.n_solved, 1) <= self.push_frac
+            if small:
+                bound, hmax = float("inf"), self.hmax
+                for _ in range(4):
+                    ok = self._push(s64, eps, st)
+                    hmax, bound = self._bound(rho_cols(s64, self.R), float(self.Z[:, C].max()))
+                    if ok and bound <= tol:
+                        break
+                    eps = [tol / (1.25 * hmax)] * C + [self.eps_h]
+                small, st.cert, st.hmax, self.hmax = bound <= tol, bound, hmax, hmax
+            if not small:  # first or large batch, or the push did not certify: global solve
+                if self.Z is not None:
+                    self.F, self.H = self.Z[:, :C].clone(), self.Z[:, C:].clone()
+                inner = DynLPPlus.solve(self, sys)
+                st.iters, st.edges, st.setup_ms = st.iters + inner.iters, st.edges + inner.edges, inner.setup_ms
+                st.cert, st.hmax, st.path = inner.cert, inner.hmax, "inc-" + inner.path
+                self.Z = torch.cat([self.F, self.H], dim=1).double().contiguous()
+                self._park(self.Z, out)
+                self._full_residual(s64, st)
+            self.F, self.H = self.Z[:, :C], self.Z[:, C:]
+        st.ms = tm.ms
+        return st
+
+
 class Reference:
     """High-accuracy FP64 solution F* and absorption bound (solvers.py:437-445)."""
 
@@ -481,15 +580,18 @@ class Reference:
 
 
 def make(name, n_total, K, *, dtype, delta, tol, group, max_iter=None):
-    """solvers.py:448-458 (dynlp+inc is stream.EdgeStreamLP for edge streams)."""
+    """solvers.py:448-458 (dynlp+inc on IncrementalStream systems; for edge-update streams the
+    resident path is stream.EdgeStreamLP)."""
     kw = {} if max_iter is None else {"max_iter": max_iter}
     if name == "itlp":
         return ItLP(n_total, K, dtype, delta, **kw)
     if name in ("dynlp", "dynlp-fh"):
         return DynLP(n_total, K, dtype, delta, group, comp="fh" if name == "dynlp-fh" else "tau", **kw)
-    if name.startswith("dynlp+") and name != "dynlp+inc":
+    if name == "dynlp+inc":
+        return IncrementalLP(n_total, K, dtype, tol=tol, group=group, **kw)
+    if name.startswith("dynlp+"):
         return DynLPPlus(n_total, K, dtype, method=name.split("+", 1)[1], tol=tol, group=group, **kw)
     raise ValueError(f"unknown solver {name!r}")
 
 
-SOLVERS = ["itlp", "dynlp", "dynlp-fh", "dynlp+pcg", "dynlp+amg", "dynlp+auto"]
+SOLVERS = ["itlp", "dynlp", "dynlp-fh", "dynlp+pcg", "dynlp+amg", "dynlp+auto", "dynlp+inc"]

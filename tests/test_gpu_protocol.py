@@ -158,3 +158,32 @@ def test_backend_sparse_module():
     X = torch.rand((50, 3), dtype=torch.float32, device="cuda")
     np.testing.assert_allclose((W32 @ X).cpu().numpy(), m.toarray().astype(np.float32) @ X.cpu().numpy(),
                                rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("K", [2, 3])
+def test_incremental_solver_is_certified_on_small_batches(K):
+    """dynlp+inc on IncrementalStream systems (tests/test_incremental.py:45-65 of the reference):
+    every batch certified to tol, the scores within the certificate of a direct solve, the kept
+    residual equal to a fresh one, and small batches repaired by push alone."""
+    import torch
+    from dynlp_amd.solvers import IncrementalLP, apply_A, make
+    from dynlp_amd.stream import IncrementalStream
+    tol = 1e-3
+    g = graphs.sbm(3000, K, 6, 0.85, K, 0.02, seed=6)
+    st = IncrementalStream(g, init_frac=0.85, n_batches=8, del_frac=0.1, label_frac=0.02, dtype="float32")
+    solver = IncrementalLP(g.n, K, "float32", tol=tol, full_every=4, push_frac=0.5)
+    assert make("dynlp+inc", g.n, K, dtype="float32", delta=1e-4, tol=tol, group="auto").name == "dynlp+inc"
+    paths = []
+    for sysm in st.batches():
+        stats = solver.solve(sysm)
+        paths.append(stats.path)
+        s64 = sysm.astype("float64")
+        U = np.flatnonzero(sysm.solved.cpu().numpy())
+        A, rhs = dense_system(s64)
+        err = float(np.abs(solver.scores(sysm).double().cpu().numpy()[U] - np.linalg.solve(A[U][:, U], rhs[U])).max())
+        assert stats.cert <= tol * 1.0001, (sysm.t, stats)
+        assert err <= stats.cert * 1.05 + 1e-9, (sysm.t, err, stats.cert)
+        R_true = torch.cat([s64.rhs, s64.diag[:, None]], dim=1) - apply_A(s64, solver.Z)
+        drift = float(((R_true - solver.R).abs() / s64.diag[:, None]).max())
+        assert drift < 1e-9, drift
+    assert "inc-push" in paths, paths
