@@ -48,3 +48,75 @@ def connected_components_csr(W: CSR, mask=None, out=None, count=None, ws=None):
     if count is None:
         return labels, int(nc.item())
     return labels, nc
+
+
+def _lightest(nc, cu, cv, de):
+    """For every component with an outgoing edge (cc.py:26-35): the component, the one across
+    its lightest outgoing edge (ties: the smallest such component) and that edge's weight."""
+    from .backend import scatter_min
+    dev = de.device
+    best = torch.full((nc,), float("inf"), dtype=torch.float32, device=dev)
+    scatter_min(best, cu, de)
+    across = torch.full((nc,), nc, dtype=torch.int64, device=dev)
+    sel = de <= best[cu.long()]
+    scatter_min(across, cu[sel], cv[sel].long())
+    c = torch.nonzero(across < nc).flatten()
+    return c, across[c], best[c]
+
+
+def fh_components(m, rows, cols, d, k, ground=None):
+    """Felzenszwalb-Huttenlocher merging in parallel Boruvka rounds (cc.py:38-74): every
+    component takes its lightest outgoing edge e and merges across it when
+        d(e) <= min(Int(C1) + k/|C1|, Int(C2) + k/|C2|),
+    Int(C) being C's largest internal dissimilarity; the merges of a round are joined by
+    connected components.  When no edge qualifies and ground (per vertex) is given, components
+    with zero ground then join across their lightest outgoing edge until none can.  FP32
+    dissimilarities, as the reference keeps them.  Returns (labels int32[m], count)."""
+    from .backend import scatter_add, scatter_max
+    dev = device()
+    lab = torch.arange(m, dtype=torch.int32, device=dev)
+    Int = torch.zeros(m, dtype=torch.float32, device=dev)
+    size = torch.ones(m, dtype=torch.float32, device=dev)
+    rows = torch.as_tensor(rows, device=dev).long()
+    cols = torch.as_tensor(cols, device=dev).long()
+    d = torch.as_tensor(d, device=dev).to(torch.float32)
+    gvert = None if ground is None else torch.as_tensor(ground, device=dev).to(torch.float32)
+    nc, fh = m, True
+    while rows.shape[0]:
+        cu, cv = lab[rows].long(), lab[cols].long()
+        ext = cu != cv
+        if not fh:   # grounding pass: only components without ground pick an edge
+            g = torch.zeros(nc, dtype=torch.float32, device=dev)
+            scatter_add(g, lab, gvert)
+            ext &= g[cu] == 0
+        if not bool(ext.any()):
+            break
+        c, t, dc = _lightest(nc, cu[ext], cv[ext], d[ext])
+        if fh:
+            ok = dc <= torch.minimum(Int[c] + k / size[c], Int[t] + k / size[t])
+            if not bool(ok.any()):
+                if gvert is None:
+                    break
+                fh = False
+                continue
+            c, t, dc = c[ok], t[ok], dc[ok]
+        root, nc = connected_components(nc, c, t)
+        root = root.long()
+        Int2 = torch.zeros(nc, dtype=torch.float32, device=dev)
+        size2 = torch.zeros(nc, dtype=torch.float32, device=dev)
+        scatter_max(Int2, root, Int)
+        scatter_max(Int2, root[c], dc)
+        scatter_add(size2, root, size)
+        lab, Int, size = root[lab.long()].to(torch.int32), Int2, size2
+    return lab, nc
+
+
+def adaptive_components(m, rows, cols, w, s_r, s_c, wmax, ground=None):
+    """Component-adaptive merging (cc.py:77-83): dissimilarity 1 - w/wmax scaled by the two
+    endpoints' own mean dissimilarities, k = the mean scaled dissimilarity."""
+    if m == 0:
+        return torch.zeros(0, dtype=torch.int32, device=device()), 0
+    w = torch.as_tensor(w, device=device())
+    s_rc = torch.as_tensor(s_r, device=device()) * torch.as_tensor(s_c, device=device())
+    d = (1 - w / wmax) / torch.sqrt(s_rc).to(w.dtype)   # in w's precision; FP32 from fh_components on
+    return fh_components(m, rows, cols, d, float(d.mean()) if d.shape[0] else 0.0, ground)

@@ -4,9 +4,9 @@ Same surface as dynlp/solvers.py of the reference: `Stats`, `spmm`, `apply_A`, `
 `violation`, `jacobi_M`, `pcg` (:66-97), `push` (:100-125), `predict`, `ItLP` (:190-210),
 `DynLP` (:213-241), `DynLPPlus` (:244-343), `Reference` (:437-445), `make` (:448-458) and
 `SOLVERS`.  Every sweep, product and reduction runs in the HIP kernels of the C ABI; Python
-keeps the reference's control flow (and its host synchronisations).  Warm starts by supernodes
-(`supernode_init`/`init_h`, :128-163) are out of scope: new rows start from the prior and h
-from 1.  The resident incremental method (`dynlp+inc`, IncrementalLP :346-434) runs on the
+keeps the reference's control flow (and its host synchronisations).  New rows of a stream's
+systems start from their supernode components (`supernode_init`/`init_h`, :128-163).  The
+resident incremental method (`dynlp+inc`, IncrementalLP :346-434) runs on the
 global-index systems of `stream.IncrementalStream`; for edge-level streams the resident path
 is `stream.EdgeStreamLP`.
 
@@ -55,7 +55,7 @@ class System:
     is_new: torch.Tensor | None = None
     seed: torch.Tensor | None = None
     t: int = 0                # batch number of a stream
-    comps: dict | None = None  # new vertices' supernode components (not computed here)
+    comps: dict | None = None  # new vertices' supernode components {"tau"|"fh": (labels, count)}
     n_inserted: int = 0
     n_deleted: int = 0
     n_active: int = 0
@@ -276,6 +276,66 @@ def push(sys: System, Z, R, eps, ops: FrontierOps, st: Stats, budget_edges=None,
     return Z, R, nc == 0
 
 
+def _comp_avg(lab, nc, S_rows, T_rows, fallback):
+    """Per component of the new vertices: sum S / sum T, or fallback where sum T = 0
+    (solvers.py:128-135)."""
+    from .backend import scatter_add
+    lab = lab.long()
+    S = torch.zeros((nc,) + tuple(S_rows.shape[1:]), dtype=S_rows.dtype, device=S_rows.device)
+    T = torch.zeros(nc, dtype=S_rows.dtype, device=S_rows.device)
+    scatter_add(S, lab, S_rows)
+    scatter_add(T, lab, T_rows.to(S_rows.dtype))
+    ok = T > 0
+    Tn = torch.where(ok, T, torch.ones_like(T))
+    fb = torch.as_tensor(fallback, dtype=S.dtype, device=S.device)
+    if S.dim() == 2:
+        return torch.where(ok[:, None], S / Tn[:, None], fb.expand_as(S))
+    return torch.where(ok, S / Tn, fb.expand_as(S))
+
+
+def _rows_product(W: CSR, rows, Y):
+    """(W @ Y)[rows] (the product of the row subset W[rows])."""
+    return spmm(W, Y.contiguous())[rows]
+
+
+def supernode_init(sys: System, X, known=None, cols=None, comp="tau"):
+    """New vertices take their supernode's value (solvers.py:138-153): with known None, the
+    share of the component's weight to each class (DynLP's rule); with a known-row mask, the
+    weighted average over all known neighbours (labels and the last solution).  Components from
+    sys.comps[comp]; a system without them (or without new rows) leaves X as it is."""
+    if sys.is_new is None or sys.comps is None or comp not in sys.comps:
+        return X
+    new = torch.nonzero(sys.is_new).flatten()
+    if not new.shape[0]:
+        return X
+    cols = slice(0, sys.C) if cols is None else cols
+    B = sys.B.to(X.dtype)
+    s_, t = sys.label_cols(B)[new], B[new].sum(dim=1)
+    if known is not None:
+        kf = known.to(X.dtype)
+        s_ = s_ + _rows_product(sys.W, new, X[:, cols] * kf[:, None])
+        t = t + _rows_product(sys.W, new, kf[:, None])[:, 0]
+    lab, nc = sys.comps[comp]
+    X[new, cols] = _comp_avg(lab, nc, s_, t, sys.prior.to(X.dtype)[None, :])[lab.long()]
+    return X
+
+
+def init_h(sys: System, H, known, fallback, comp="fh"):
+    """New vertices: h ~ 1 + the component's weighted average of h over known neighbours
+    (labeled ones count h = 0) (solvers.py:156-163)."""
+    if sys.is_new is None or sys.comps is None or comp not in sys.comps:
+        return H
+    new = torch.nonzero(sys.is_new).flatten()
+    if new.shape[0]:
+        kf = known.to(H.dtype)
+        lab, nc = sys.comps[comp]
+        num = _rows_product(sys.W, new, (H[:, 0] * kf)[:, None])[:, 0]
+        den = sys.B.to(H.dtype)[new].sum(dim=1) + _rows_product(sys.W, new, kf[:, None])[:, 0]
+        hc = _comp_avg(lab, nc, num, den, fallback - 1) + 1
+        H[new, 0] = hc[lab.long()]
+    return H
+
+
 def predict(F, K):
     """solvers.py:166-167."""
     return (F[:, 0] > 0.5).to(torch.int32) if K == 2 else F.argmax(dim=1).to(torch.int32)
@@ -341,7 +401,7 @@ class DynLP(Solver):
     def solve(self, sys):
         st, sys = Stats(path="frontier-jacobi"), sys.astype(self.dtype)
         with Timer() as tm:
-            X = self._warm(sys)
+            X = supernode_init(sys, self._warm(sys), comp=self.comp)
             ops = FrontierOps(sys.W, self.C, self.group)
             n = sys.n_u
             seed = sys.seed if sys.seed is not None else torch.ones(n, dtype=torch.bool, device=X.device)
@@ -431,8 +491,13 @@ class DynLPPlus(Solver):
         with Timer() as tm:
             if self.H is None:
                 self.H = torch.ones((self.n_total, 1), dtype=self.store_dtype, device=device())
-            X = self._warm(sw)
+            # new rows start from their supernodes (systems of a stream carry the components;
+            # other systems keep the stored values: the prior and h = 1 on first sight)
+            known = ~sw.is_new if sw.is_new is not None else None
+            X = supernode_init(sw, self._warm(sw), known, comp="fh")
             Hc = self.H[sw.U].to(self.dtype)
+            if known is not None and sw.comps is not None:
+                Hc = init_h(sw, Hc, known, self.hmax or 1.0)
             Z = torch.cat([X, Hc], dim=1).contiguous()
             rhs = torch.cat([sw.rhs, sw.diag[:, None]], dim=1).contiguous()
             ops, amg_box, hmax = FrontierOps(sw.W, C + 1, self.group), [None], self.hmax
@@ -477,9 +542,9 @@ class IncrementalLP(DynLPPlus):
     changed rows, and when few rows violate repairs them with the fused asynchronous push (up to
     4 rounds, the thresholds re-tightened from the bound each time), the certificate read off
     the kept residual; large changes and the first batch go through DynLP+ and a full residual.
-    R is recomputed in full every full_every batches to bound rounding drift.  New rows keep
-    their parked values (no supernode warm start).  The edge-update service path of this package
-    is stream.EdgeStreamLP."""
+    R is recomputed in full every full_every batches to bound rounding drift.  New rows start
+    from their supernode's averages (supernode_init, init_h).  The edge-update service path of
+    this package is stream.EdgeStreamLP."""
 
     def __init__(self, n_total, K, dtype="float32", tol=1e-3, full_every=16, max_rounds=50_000, **kw):
         super().__init__(n_total, K, dtype, method="auto", tol=tol, **kw)
@@ -534,8 +599,13 @@ class IncrementalLP(DynLPPlus):
             out = ~sys.solved.bool()
             small = self.Z is not None
             if small:
-                self._park(self.Z, out)
+                Z = self.Z
+                self._park(Z, out)
                 self.R[out] = 0
+                if s64.is_new is not None:
+                    known = s64.solved.bool() & ~s64.is_new
+                    Z[:, :C] = supernode_init(s64, Z[:, :C].clone(), known, comp="fh")
+                    Z[:, C:] = init_h(s64, Z[:, C:].clone(), known, self.hmax or 1.0)
                 if sys.changed is not None and sys.changed.numel():
                     self._refresh_rows(s64, sys.changed, st)
                 self.since_full += 1

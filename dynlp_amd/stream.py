@@ -501,6 +501,32 @@ class EdgeStreamLP:
 # The reference's vertex-arrival protocol (stream.py:84-346), producing systems for the
 # reference-API solvers of solvers.py.  Host schedule; systems built on the device.
 
+def new_components(m, lr, lc, w, tau, wmax, ground):
+    """DynLP step 1 for the m new vertices (stream.py:67-81), from every stored entry (lr -> lc,
+    w) of their rows (lc = -1 when the neighbour is not new): "tau" = connected components over
+    the edges heavier than the graph's mean weight tau (the paper's supernodes); "fh" = adaptive
+    components (cc.adaptive_components), whose scale is each vertex's mean dissimilarity
+    1 - w/wmax over all its live entries; ground = each vertex's weight to labeled vertices."""
+    from .backend import scatter_add
+    from .cc import adaptive_components, connected_components
+    dev = device()
+    lr = torch.as_tensor(lr, device=dev).long()
+    lc = torch.as_tensor(lc, device=dev).long()
+    w = torch.as_tensor(w, device=dev)          # the system's precision, rounded to FP32 after use
+    on = w > 0
+    s = torch.zeros(m, dtype=torch.float32, device=dev)
+    cnt = torch.zeros(m, dtype=torch.float32, device=dev)
+    scatter_add(s, lr, torch.where(on, 1 - w / wmax, torch.zeros_like(w)).to(torch.float32))
+    scatter_add(cnt, lr, on.to(torch.float32))
+    s = s / torch.clamp(cnt, min=1) + 1e-6
+    e = on & (lc >= 0)
+    r, c, we = lr[e], lc[e], w[e]
+    heavy = we > tau
+    return {"tau": connected_components(m, r[heavy], c[heavy]),
+            "fh": adaptive_components(m, r, c, we, s[r], s[c], wmax,
+                                      torch.as_tensor(ground, device=dev).to(torch.float32))}
+
+
 def _arrival_order(g: Graph, seed: int) -> np.ndarray:
     """The graph's arrival order (an edge-list file's `order`), else the permutation numpy's
     RandomState(seed + 1) draws (graphs.from_npz's default)."""
@@ -550,8 +576,8 @@ class Stream:
     reference's order.  batches() yields one System per batch (local ids over U, built on the
     device by solvers.build_system) with the reference's bookkeeping: t, is_new (rows not in U
     at the previous batch), seed (is_new, or adjacent to an inserted or deleted vertex),
-    n_inserted, n_deleted, n_active, n_ungrounded.  The new vertices' supernode components are
-    not built (comps is None): the solvers here start new rows from the prior."""
+    n_inserted, n_deleted, n_active, n_ungrounded, and comps, the new vertices' supernode
+    components ("tau" and "fh", new_components) that the solvers' warm starts average over."""
 
     def __init__(self, g: Graph, *, label_frac=0.01, init_frac=0.1, n_batches=10, del_frac=0.1,
                  eta_rel=0.0, seed=0, dtype="float32"):
@@ -566,6 +592,8 @@ class Stream:
         self.chunks = np.array_split(self.order[self.n0:], n_batches) if n_batches > 0 else []
         self.indptr, self.indices, self.data = g.csr()
         self.entry_row = np.repeat(np.arange(n, dtype=np.int64), np.diff(self.indptr))
+        self.tau = float(self.data.mean()) if self.data.size else 0.0
+        self.wmax = float(self.data.max()) if self.data.size else 1.0
         # eta_rel times the mean weighted degree (both directions of every edge)
         self.eta = float(eta_rel * 2.0 * float(np.sum(g.w, dtype=np.float64)) / n)
         self.schedule = self._plan()
@@ -602,6 +630,20 @@ class Stream:
         adj[self.entry_row[mark[self.indices]]] = True
         return adj
 
+    def _components(self, W, is_new, B):
+        """The new rows' supernode components over the system's W (rows and columns in the
+        system's ids; entries to rows that are not new carry lc = -1)."""
+        new = np.flatnonzero(is_new)
+        loc = np.full(is_new.shape[0], -1, dtype=np.int64)
+        loc[new] = np.arange(new.shape[0])
+        ip = W.indptr.cpu().numpy().astype(np.int64)
+        rows = np.repeat(np.arange(is_new.shape[0]), np.diff(ip))
+        on = is_new[rows]
+        cols = W.indices.cpu().numpy().astype(np.int64)[on]
+        newt = torch.as_tensor(new, device=device())
+        return new_components(int(new.shape[0]), loc[rows[on]], loc[cols], W.data[torch.as_tensor(np.flatnonzero(on),
+                              device=device())], self.tau, self.wmax, B[newt].sum(dim=1))
+
     def batches(self):
         n = self.g.n
         active = np.zeros(n, dtype=bool)
@@ -624,6 +666,7 @@ class Stream:
         sysm.t = t
         sysm.is_new = torch.as_tensor(is_new, device=dev)
         sysm.seed = torch.as_tensor(seed, device=dev)
+        sysm.comps = self._components(sysm.W, is_new, sysm.B)
         sysm.n_inserted, sysm.n_deleted = int(ins.shape[0]), int(dele.shape[0])
         sysm.n_active = int(active.sum())
         sysm.n_ungrounded = int((active & ~self.labeled).sum()) - int(U.shape[0])
@@ -679,9 +722,12 @@ class IncrementalStream(Stream):
             changed = np.flatnonzero((self._adjacent(status) | entering) & Um)
             seed = (entering | self._adjacent(np.concatenate([ins, dele]))) & Um
             t64 = lambda a: torch.as_tensor(a, device=dev)  # noqa: E731
-            sys64 = System(ids, CSR(t64(w64), indices, indptr, (n, n)), t64(s64), t64(diag64), t64(B64),
+            W64 = CSR(t64(w64), indices, indptr, (n, n))
+            # components over the system's working-precision weights, as the reference's Wd
+            comps = self._components(W64 if dt == torch.float64 else W64.astype(dt), entering, t64(B64))
+            sys64 = System(ids, W64, t64(s64), t64(diag64), t64(B64),
                            t64(np.ascontiguousarray(rhs64)), t64(prior64), K, C, self.eta,
-                           is_new=t64(entering), seed=t64(seed), t=t, comps=None, n_inserted=int(ins.shape[0]),
+                           is_new=t64(entering), seed=t64(seed), t=t, comps=comps, n_inserted=int(ins.shape[0]),
                            n_deleted=int(dele.shape[0]), n_active=int(active.sum()),
                            n_ungrounded=int((active & ~self.labeled & ~Um).sum()), solved=t64(Um),
                            changed=t64(changed))

@@ -11,6 +11,12 @@ from dynlp_amd import graphs
 pytestmark = pytest.mark.gpu
 
 
+def same_partition(a, b):
+    """Two labelings describe the same partition."""
+    a, b = list(a), list(b)
+    return len(a) == len(b) and len(set(zip(a, b))) == len(set(a)) == len(set(b))
+
+
 def dense_A(g):
     ip, ix, d = g.csr()
     A = np.zeros((g.n, g.n))
@@ -75,6 +81,9 @@ def test_incremental_stream_matches_rebuilt_system():
                                        rtol=1e-10, atol=1e-12, err_msg=f)
         assert np.array_equal(s.is_new.cpu().numpy()[U], c.is_new.cpu().numpy())
         assert np.array_equal(s.seed.cpu().numpy()[U], c.seed.cpu().numpy())
+        for k in ("tau", "fh"):
+            assert s.comps[k][1] == c.comps[k][1], k
+            assert same_partition(s.comps[k][0].cpu().numpy(), c.comps[k][0].cpu().numpy()), k
         out = ~solved
         assert abs(Wg[out]).sum() == 0 and abs(Wg[:, out]).sum() == 0
         assert (s.diag.cpu().numpy()[out] == 1.0).all() and (s.rhs.cpu().numpy()[out] == 0.0).all()
@@ -134,7 +143,8 @@ def test_baselines_run_and_are_close():
     from dynlp_amd.stream import Stream
     g = graphs.sbm(2000, 2, 8, 0.85, 2, 0.02, seed=4)
     st = Stream(g, init_frac=0.3, n_batches=2, label_frac=0.02, dtype="float32")
-    solvers = [make(nm, g.n, 2, dtype="float32", delta=1e-5, tol=1e-3, group="auto") for nm in ["itlp", "dynlp"]]
+    solvers = [make(nm, g.n, 2, dtype="float32", delta=1e-5, tol=1e-3, group="auto")
+               for nm in ["itlp", "dynlp", "dynlp-fh"]]
     for sysm in st.batches():
         A, rhs = dense_system(sysm.astype("float64"))
         Fd = np.linalg.solve(A, rhs)
@@ -187,3 +197,67 @@ def test_incremental_solver_is_certified_on_small_batches(K):
         drift = float(((R_true - solver.R).abs() / s64.diag[:, None]).max())
         assert drift < 1e-9, drift
     assert "inc-push" in paths, paths
+
+
+def _components(edges, m, tau, ground=None):
+    """new_components on an explicit undirected edge list (all vertices new)."""
+    from dynlp_amd.stream import new_components
+    a, b, w = zip(*edges)
+    gr = np.ones(m) if ground is None else np.asarray(ground, dtype=np.float64)
+    return new_components(m, np.asarray(a + b), np.asarray(b + a), np.asarray(w + w, dtype=np.float32), tau, 1.0, gr)
+
+
+def test_adaptive_components_cut_a_bridge_that_tau_chains():
+    """Two tight cliques joined by an edge above the global mean but weak relative to both
+    (the reference's tests/test_dynlp.py:56-62)."""
+    edges = [(a + o, b + o, 0.95 + 0.01 * ((a + b) % 3)) for o in (0, 6) for a in range(6) for b in range(a + 1, 6)]
+    edges.append((5, 6, 0.8))
+    comps = _components(edges, 12, tau=0.5)
+    assert comps["tau"][1] == 1
+    assert comps["fh"][1] == 2 and same_partition(comps["fh"][0].cpu().numpy(), [0] * 6 + [1] * 6)
+
+
+def test_adaptive_components_merge_a_sparse_cluster():
+    """A weak chain (every edge below tau) stays singletons under tau but merges under fh, and a
+    component without labeled contact joins its strongest neighbour (tests/test_dynlp.py:65-71)."""
+    edges = [(i, i + 1, 0.3 + 0.02 * (i % 3)) for i in range(7)]
+    comps = _components(edges, 8, tau=0.5, ground=[1, 0, 0, 0, 0, 0, 0, 0])
+    assert comps["tau"][1] == 8
+    assert comps["fh"][1] == 1
+
+
+def test_supernode_init_by_hand():
+    """Two supernodes of new rows: DynLP's rule gives each its share of class weight (prior when
+    it has none); with known rows, the weighted average over labeled and known neighbours."""
+    import torch
+    from dynlp_amd.backend import CSR
+    from dynlp_amd.solvers import System, init_h, supernode_init
+    dev = "cuda"
+    # rows 0,1 new (component 0), row 2 new (component 1), row 3 known; edges 0-1 (1), 1-3 (2), 2-3 (1)
+    indptr = torch.tensor([0, 1, 3, 4, 6], dtype=torch.int32, device=dev)
+    indices = torch.tensor([1, 0, 3, 3, 1, 2], dtype=torch.int32, device=dev)
+    data = torch.tensor([1, 1, 2, 1, 2, 1], dtype=torch.float64, device=dev)
+    W = CSR(data, indices, indptr, (4, 4))
+    B = torch.tensor([[3, 1, 0], [0, 0, 0], [0, 0, 0], [1, 1, 2]], dtype=torch.float64, device=dev)
+    prior = torch.full((3,), 1 / 3, dtype=torch.float64, device=dev)
+    is_new = torch.tensor([True, True, True, False], device=dev)
+    comps = {"tau": (torch.tensor([0, 0, 1], dtype=torch.int32, device=dev), 2)}
+    comps["fh"] = comps["tau"]
+    sysm = System(torch.arange(4, device=dev), W, B.sum(1), B.sum(1) + 2, B, B.clone(), prior, 3, 3,
+                  is_new=is_new, seed=is_new, comps=comps)
+    X = torch.zeros((4, 3), dtype=torch.float64, device=dev)
+    X[3] = torch.tensor([0.2, 0.3, 0.5], dtype=torch.float64, device=dev)
+    Y = supernode_init(sysm, X.clone())
+    np.testing.assert_allclose(Y[0].cpu().numpy(), [0.75, 0.25, 0.0])
+    np.testing.assert_allclose(Y[1].cpu().numpy(), [0.75, 0.25, 0.0])
+    np.testing.assert_allclose(Y[2].cpu().numpy(), [1 / 3] * 3)     # no class weight: the prior
+    known = ~is_new
+    Y = supernode_init(sysm, X.clone(), known)
+    # component 0: class weight (3, 1, 0) + 2 * X[3] = (3.4, 1.6, 1.0) over 4 + 2 = 6
+    np.testing.assert_allclose(Y[0].cpu().numpy(), [3.4 / 6, 1.6 / 6, 1.0 / 6])
+    np.testing.assert_allclose(Y[2].cpu().numpy(), [0.2, 0.3, 0.5])
+    H = torch.zeros((4, 1), dtype=torch.float64, device=dev)
+    H[3] = 10.0
+    Hn = init_h(sysm, H.clone(), known, 7.0)
+    # component 0: (2 * 10) / (4 + 2) + 1; component 1: (1 * 10) / (0 + 1) + 1
+    np.testing.assert_allclose(Hn[:3, 0].cpu().numpy(), [20 / 6 + 1, 20 / 6 + 1, 11.0])
