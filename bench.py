@@ -131,7 +131,41 @@ def kernel_table(g, C, reps=10):
     ms = timed(lambda: _lib.call("dlp_gram", n, C, B.ptr(Xg), Xg.shape[1], B.ptr(Xg), Xg.shape[1], None, B.ptr(G),
                                  _lib.F32, B.stream()), reps)
     line(f"gram X^T Y (n x {C}, FP64 MFMA)", ms, flops=2.0 * n * C * C)
+    # aggregation AMG on this graph's system W_UU: setup, then one V-cycle M(R) on n_u x C FP32
+    from dynlp_amd.amg import AMG
+    from dynlp_amd.solvers import build_system
+    log("kernel table: AMG on the graph's system")
+    sysm = build_system(g, "float32")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    amg = AMG(sysm.W, sysm.s)
+    torch.cuda.synchronize()
+    setup_ms = (time.perf_counter() - t0) * 1e3
+    log(f"AMG setup {setup_ms:.0f} ms, levels {amg.sizes}")
+    Ru = torch.rand((sysm.n_u, C), dtype=torch.float32, device="cuda", generator=gen)
+    ms = timed(lambda: amg(Ru), reps)
+    line(f"amg V-cycle M(R) ({len(amg.sizes)} levels {amg.sizes}, n_u x {C} FP32; setup {setup_ms:.0f} ms)", ms,
+         amg_cycle_bytes(amg, C))
     return rows
+
+
+def amg_cycle_bytes(amg, C, s=4):
+    """Algorithmic bytes of one V-cycle (nu = 1): its passes depend on each other, so each pass
+    counts its distinct elements once.  Per level (n rows, nnz entries, nc aggregates): first
+    sweep 3nCs + ns, two residuals 2(3nCs + 4(n+1) + 8nnz + ns), restriction nCs + 2ncCs + 4n,
+    prolongation 2nCs + ncCs + 4n, correction 3nCs + ns; coarsest level: its inverse (FP32 n^2)
+    applied to an n x C block."""
+    total = 0
+    for L in amg.levels:
+        n, nnz, nc = L.W.shape[0], L.W.nnz, L.nc
+        total += 3 * n * C * s + n * s                                   # first sweep
+        total += 2 * (3 * n * C * s + 4 * (n + 1) + 8 * nnz + n * s)    # residuals
+        total += n * C * s + 2 * nc * C * s + 4 * n                      # restriction
+        total += 2 * n * C * s + nc * C * s + 4 * n                      # prolongation
+        total += 3 * n * C * s + n * s                                   # correction
+    nc = amg.sizes[-1]
+    total += (nc * nc + 2 * nc * C) * s
+    return total
 
 
 def request_row(lp, st, b, batch, g, a):

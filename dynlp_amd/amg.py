@@ -7,7 +7,8 @@ P^T W P with duplicates summed, s_c = P^T s; stop at max_coarse rows or when a l
 by less than min_ratio; exact FP64 inverse at the coarsest level (n <= 4*max_coarse) applied
 as a library GEMM; weighted-Jacobi smoothing (omega 0.7, nu 1) in a symmetric V-cycle.
 Matching, coarsening, restriction and prolongation are HIP kernels (csrc/amg.hip); level
-operators use the row-major SpMM.
+residuals R - A X are one row-operator launch each and the smoother's correction one column
+pass (csrc/spmm.hip, csrc/blas.hip).
 """
 from __future__ import annotations
 
@@ -27,9 +28,31 @@ class Level:
     diag: torch.Tensor
     agg: torch.Tensor | None = None   # fine -> coarse map (int32)
     nc: int = 0
+    neg_diag: torch.Tensor | None = None
 
     def apply(self, X):
         return spmm_axpy(self.W, X.contiguous(), a=self.diag, b=-1.0)
+
+    def residual(self, R, X):
+        """R - (diag X - W X) in one row-operator launch."""
+        if self.neg_diag is None:
+            self.neg_diag = (-self.diag).contiguous()
+        W, C = self.W, X.shape[1]
+        out = torch.empty_like(R)
+        dt = dtype_code(X.dtype)
+        _lib.call("dlp_row_operator", int(W.shape[0]), None, C, ptr(W.indptr), None, ptr(W.indices), ptr(W.data),
+                  dtype_code(W.data.dtype), None, ptr(X), C, dt, ptr(self.neg_diag), 1.0, ptr(R), C, ptr(out), C, dt,
+                  0, W.avg_deg, stream())
+        return out
+
+
+def _jacobi_add(omega, T, X, d):
+    """X += omega * T / d (rows), in place: the weighted-Jacobi correction (dlp_colop mode 3)."""
+    C = T.shape[1]
+    a = torch.full((C,), float(omega), dtype=torch.float64, device=T.device)
+    _lib.call("dlp_colop", int(T.shape[0]), C, 3, 1.0, ptr(a), ptr(T), C, ptr(d), ptr(X), C, dtype_code(T.dtype),
+              stream())
+    return X
 
 
 def _scan_ws(n):
@@ -135,22 +158,25 @@ class AMG:
         self.sizes = [L.W.shape[0] for L in self.levels] + [n]
 
     def _smooth(self, L: Level, X, R, sweeps):
+        """sweeps of X += omega (R - A X) / diag (amg.py:114-117), in place on X."""
         for _ in range(sweeps):
-            X = X + self.omega * (R - L.apply(X)) / L.diag[:, None]
+            _jacobi_add(self.omega, L.residual(R, X), X, L.diag)
         return X
+
+    def _first(self, L: Level, R):
+        """The first sweep from X = 0: X = omega R / diag."""
+        return _jacobi_add(self.omega, R, torch.zeros_like(R), L.diag)
 
     def _cycle(self, lvl, R):
         if lvl == len(self.levels):
             L = self.coarse
             if self.inv is not None:
-                return self.inv @ R
-            X = self.omega * R / L.diag[:, None]
-            return self._smooth(L, X, R, self.coarse_sweeps - 1)
+                return self.inv @ R          # library GEMM with the exact coarse inverse
+            return self._smooth(L, self._first(L, R), R, self.coarse_sweeps - 1)
         L = self.levels[lvl]
-        X = self.omega * R / L.diag[:, None]
-        X = self._smooth(L, X, R, self.nu - 1)
-        Rc = restrict(L.agg, (R - L.apply(X)).contiguous(), L.nc)
-        X = prolong_add(L.agg, self._cycle(lvl + 1, Rc), X.contiguous())
+        X = self._smooth(L, self._first(L, R), R, self.nu - 1)
+        Rc = restrict(L.agg, L.residual(R, X), L.nc)
+        X = prolong_add(L.agg, self._cycle(lvl + 1, Rc), X)
         return self._smooth(L, X, R, self.nu)
 
     def __call__(self, R):

@@ -5,7 +5,10 @@
 // Matching choices are made row-locally over CSR rows (a row is scanned by one thread), which
 // gives the reference's scatter_max/scatter_min result (heaviest weight, ties -> smallest
 // neighbour id, amg.py:38-44) without atomics.  Coarse rows are built by a counting sort on
-// the aggregate id followed by a per-row insertion sort and duplicate merge.
+// the aggregate id, a segmented radix sort of every row's columns (rocPRIM: coarse rows of an
+// expander graph hold thousands of entries after a few levels) and a duplicate merge per row.
+#include <rocprim/device/device_segmented_radix_sort.hpp>
+
 #include "common.h"
 #include "scan.h"
 
@@ -117,33 +120,36 @@ __global__ void k_coarse_fill(int n, const int32_t* __restrict__ indptr, const i
 }
 
 // Sort a coarse row's entries by column and merge duplicates (summing weights); ucnt = unique.
+// Duplicate merge of row-sorted entries (sk, sv): every run of equal columns becomes one entry
+// (weights summed in the sorted order) at the front of the row's segment of ci/cw; unique count
+// into ucnt.
 template <typename T>
-__global__ void k_coarse_merge(int nc, const int32_t* __restrict__ off, int32_t* __restrict__ ci,
-                               T* __restrict__ cw, int32_t* __restrict__ ucnt) {
+__global__ void k_coarse_merge_sorted(int nc, const int32_t* __restrict__ off, const uint32_t* __restrict__ sk,
+                                      const T* __restrict__ sv, int32_t* __restrict__ ci, T* __restrict__ cw,
+                                      int32_t* __restrict__ ucnt) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nc) return;
   const int beg = off[r], end = off[r + 1];
-  for (int i = beg + 1; i < end; ++i) {
-    const int key = ci[i];
-    const T val = cw[i];
-    int j = i - 1;
-    while (j >= beg && ci[j] > key) {
-      ci[j + 1] = ci[j];
-      cw[j + 1] = cw[j];
-      --j;
-    }
-    ci[j + 1] = key;
-    cw[j + 1] = val;
-  }
-  int o = beg;
+  int o = beg, last = -1;
+  T acc = T(0);
   for (int i = beg; i < end; ++i) {
-    if (o > beg && ci[o - 1] == ci[i]) {
-      cw[o - 1] += cw[i];
+    const int key = (int)sk[i];
+    if (key == last) {
+      acc += sv[i];
     } else {
-      ci[o] = ci[i];
-      cw[o] = cw[i];
-      ++o;
+      if (last >= 0) {
+        ci[o] = last;
+        cw[o] = acc;
+        ++o;
+      }
+      last = key;
+      acc = sv[i];
     }
+  }
+  if (last >= 0) {
+    ci[o] = last;
+    cw[o] = acc;
+    ++o;
   }
   ucnt[r] = o - beg;
 }
@@ -251,20 +257,54 @@ extern "C" int dlp_amg_coarsen_fill(int n, int nc, const int32_t* indptr, const 
   return DLP_OK;
 }
 
+template <typename T>
+static int coarsen_merge_sorted(int nc, const int32_t* off, int32_t* ci, T* cw, int32_t* ucnt, hipStream_t st) {
+  int total = 0;
+  DLP_CHECK_HIP(hipMemcpyAsync(&total, off + nc, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  DLP_CHECK_HIP(hipStreamSynchronize(st));
+  if (total <= 0) {
+    hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(nc)), dim3(kBlock), 0, st, nc, ucnt);
+    DLP_CHECK_LAUNCH();
+    return DLP_OK;
+  }
+  unsigned bits = 1;
+  while (bits < 31 && (1u << bits) < (unsigned)nc) ++bits;   // columns are coarse ids < nc
+  // sorted copies (keys, values) and the sort's temporary storage, stream-ordered
+  size_t tmp_bytes = 0;
+  const uint32_t* keys = reinterpret_cast<const uint32_t*>(ci);
+  DLP_CHECK_HIP(rocprim::segmented_radix_sort_pairs(nullptr, tmp_bytes, keys, (uint32_t*)nullptr, cw, (T*)nullptr,
+                                                    (unsigned)total, (unsigned)nc, off, off + 1, 0, bits, st));
+  const size_t kb = ((size_t)total * sizeof(uint32_t) + 255) & ~(size_t)255;
+  const size_t vb = ((size_t)total * sizeof(T) + 255) & ~(size_t)255;
+  void* buf = nullptr;
+  DLP_CHECK_HIP(hipMallocAsync(&buf, kb + vb + tmp_bytes, st));
+  uint32_t* sk = reinterpret_cast<uint32_t*>(buf);
+  T* sv = reinterpret_cast<T*>(reinterpret_cast<char*>(buf) + kb);
+  void* tmp = reinterpret_cast<char*>(buf) + kb + vb;
+  hipError_t e = rocprim::segmented_radix_sort_pairs(tmp, tmp_bytes, keys, sk, cw, sv, (unsigned)total, (unsigned)nc,
+                                                     off, off + 1, 0, bits, st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_coarse_merge_sorted<T>, dim3(grid_for(nc)), dim3(kBlock), 0, st, nc, off, sk, sv, ci, cw,
+                       ucnt);
+    e = hipGetLastError();
+  }
+  const hipError_t ef = hipFreeAsync(buf, st);
+  DLP_CHECK_HIP(e);
+  DLP_CHECK_HIP(ef);
+  return DLP_OK;
+}
+
 extern "C" int dlp_amg_coarsen_merge(int nc, const int32_t* off, int32_t* ci, void* cw, int dtype, int32_t* ucnt,
                                      void* scan_ws, size_t scan_bytes, int32_t* total_dev, void* stream) {
   DLP_CHECK_ARG(nc >= 1, "amg_coarsen_merge: bad nc");
   hipStream_t st = (hipStream_t)stream;
-  if (dtype == DLP_F32)
-    hipLaunchKernelGGL(k_coarse_merge<float>, dim3(grid_for(nc)), dim3(kBlock), 0, st, nc, off, ci, (float*)cw, ucnt);
-  else
-    hipLaunchKernelGGL(k_coarse_merge<double>, dim3(grid_for(nc)), dim3(kBlock), 0, st, nc, off, ci, (double*)cw,
-                       ucnt);
-  DLP_CHECK_LAUNCH();
+  int rc = dtype == DLP_F32 ? coarsen_merge_sorted<float>(nc, off, ci, (float*)cw, ucnt, st)
+                            : coarsen_merge_sorted<double>(nc, off, ci, (double*)cw, ucnt, st);
+  if (rc) return rc;
   // ucnt has nc+1 slots: scan to row pointers, total in total_dev
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, 1, ucnt + nc);
-  int rc = exclusive_scan_i32(nc + 1, ucnt, ucnt, total_dev, scan_ws, scan_bytes, st);
-  return rc;
+  DLP_CHECK_LAUNCH();
+  return exclusive_scan_i32(nc + 1, ucnt, ucnt, total_dev, scan_ws, scan_bytes, st);
 }
 
 extern "C" int dlp_amg_coarsen_compact(int nc, const int32_t* off, const int32_t* ci, const void* cw, int dtype,

@@ -1244,7 +1244,7 @@ __global__ __launch_bounds__(256) void k_colop_g(int n, int C, int lc_shift, int
   }
   const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> lc_shift;
   for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> lc_shift; r < n; r += groups) {
-    const T inv = (mode == 2) ? (T)1 / d[r] : (T)0;
+    const T inv = (mode >= 2) ? (T)1 / d[r] : (T)0;
 #pragma unroll
     for (int k = 0; k < CPL; ++k) {
       const int c = l + k * LC;
@@ -1253,7 +1253,8 @@ __global__ __launch_bounds__(256) void k_colop_g(int n, int C, int lc_shift, int
         T* y = Y + r * ldy + c;
         if (mode == 0) *y += av[k] * x;
         else if (mode == 1) *y = x + av[k] * *y;
-        else *y = x * inv;
+        else if (mode == 2) *y = x * inv;
+        else *y += av[k] * (x * inv);
       }
     }
   }
@@ -1287,9 +1288,10 @@ extern "C" int dlp_coldot(int n, int C, const void* X, int64_t ldx, const void* 
 
 extern "C" int dlp_colop(int n, int C, int mode, double sign, const double* a, const void* X, int64_t ldx,
                          const void* d, void* Y, int64_t ldy, int dtype, void* stream) {
-  DLP_CHECK_ARG(n >= 0 && C >= 1 && C <= kMaxCols && mode >= 0 && mode <= 2, "colop: bad arguments");
+  DLP_CHECK_ARG(n >= 0 && C >= 1 && C <= kMaxCols && mode >= 0 && mode <= 3, "colop: bad arguments");
   if (n == 0) return DLP_OK;
-  DLP_CHECK_ARG(X && Y && (mode == 2 ? d != nullptr : a != nullptr), "colop: null pointer");
+  const bool needs_a = mode != 2, needs_d = mode >= 2;
+  DLP_CHECK_ARG(X && Y && (!needs_a || a != nullptr) && (!needs_d || d != nullptr), "colop: null pointer");
   hipStream_t st = (hipStream_t)stream;
   const ColMap m = col_map(C);
   const unsigned g = pass_grid(n, m.lc_shift);

@@ -109,7 +109,8 @@ int dlp_violating_rows(int nrows, const int32_t* rows, int C, const void* R, int
 int dlp_coldot(int n, int C, const void* X, int64_t ldx, const void* Y, int64_t ldy, const void* d,
                double* out, int dtype, void* stream);
 /* mode 0: Y += sign*a[c]*X (solvers.py:81-82); mode 1: Y = X + a[c]*Y (:95); mode 2:
- * Y = X / d[r] (jacobi_M, :62-63).  a: device FP64[C]. */
+ * Y = X / d[r] (jacobi_M, :62-63); mode 3: Y += sign*a[c]*X / d[r] (the weighted-Jacobi
+ * correction of the AMG smoother, amg.py:114-117).  a: device FP64[C]. */
 int dlp_colop(int n, int C, int mode, double sign, const double* a, const void* X, int64_t ldx,
               const void* d, void* Y, int64_t ldy, int dtype, void* stream);
 
