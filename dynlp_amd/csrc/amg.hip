@@ -120,38 +120,74 @@ __global__ void k_coarse_fill(int n, const int32_t* __restrict__ indptr, const i
 }
 
 // Sort a coarse row's entries by column and merge duplicates (summing weights); ucnt = unique.
-// Duplicate merge of row-sorted entries (sk, sv): every run of equal columns becomes one entry
-// (weights summed in the sorted order) at the front of the row's segment of ci/cw; unique count
-// into ucnt.
+// Duplicate merge of row-sorted entries (sk, sv), one wavefront per row: every run of equal
+// columns becomes one entry (weights summed) at the front of the row's segment of ci/cw, unique
+// count into ucnt.  The row is read in chunks of 64 entries; inside a chunk a segmented
+// inclusive scan over the lanes sums each run up to every lane, the run's last lane writes it
+// at the position the earlier run ends of the chunk give (ballot + count), and a run still open
+// at the chunk's end is carried into the next chunk (added to its first run when that run has
+// the same column, else written before it).
 template <typename T>
-__global__ void k_coarse_merge_sorted(int nc, const int32_t* __restrict__ off, const uint32_t* __restrict__ sk,
-                                      const T* __restrict__ sv, int32_t* __restrict__ ci, T* __restrict__ cw,
-                                      int32_t* __restrict__ ucnt) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nc) return;
+__global__ __launch_bounds__(256) void k_coarse_merge_sorted(int nc, const int32_t* __restrict__ off,
+                                                             const uint32_t* __restrict__ sk,
+                                                             const T* __restrict__ sv, int32_t* __restrict__ ci,
+                                                             T* __restrict__ cw, int32_t* __restrict__ ucnt) {
+  const int r = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= nc) return;   // wave-uniform
   const int beg = off[r], end = off[r + 1];
-  int o = beg, last = -1;
-  T acc = T(0);
-  for (int i = beg; i < end; ++i) {
-    const int key = (int)sk[i];
-    if (key == last) {
-      acc += sv[i];
-    } else {
-      if (last >= 0) {
-        ci[o] = last;
-        cw[o] = acc;
-        ++o;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int o = beg;
+  int carry_key = -1;   // the run left open by the previous chunk (-1: none)
+  T carry_val = T(0);
+  for (int base = beg; base < end; base += 64) {
+    const int cnt = min(64, end - base);
+    const bool valid = lane < cnt;
+    const int key = valid ? (int)sk[base + lane] : -2;
+    T val = valid ? sv[base + lane] : T(0);
+    // a carried run that this chunk does not continue is complete: write it first
+    if (carry_key >= 0 && __shfl(key, 0, 64) != carry_key) {
+      if (lane == 0) {
+        ci[o] = carry_key;
+        cw[o] = carry_val;
       }
-      last = key;
-      acc = sv[i];
+      ++o;
+      carry_key = -1;
+    }
+    const int prev = __shfl_up(key, 1, 64);
+    const bool head = valid && (lane == 0 ? key != carry_key : key != prev);
+    // segmented inclusive scan: f marks a run start at or below this lane within the window
+    bool f = head;
+    for (int d = 1; d < 64; d <<= 1) {
+      const T up_val = __shfl_up(val, d, 64);
+      const int up_f = __shfl_up((int)f, d, 64);
+      if (lane >= d && !f) val += up_val;
+      f = f || (lane >= d && up_f);
+    }
+    // lanes of the chunk's first run continue the carried run when it has the same key
+    if (valid && !f && carry_key >= 0) val += carry_val;
+    const int next = __shfl_down(key, 1, 64);
+    const bool last_lane = lane == cnt - 1;
+    const bool open_end = last_lane && base + cnt < end;   // may continue in the next chunk
+    const bool tail = valid && (last_lane ? !open_end : next != key);
+    const unsigned long long tails = __ballot(tail);
+    if (tail) {
+      const int pos = o + __popcll(tails & below);
+      ci[pos] = key;
+      cw[pos] = val;
+    }
+    o += __popcll(tails);
+    // the open run (if any) moves on as the carry
+    const int last_key = __shfl(key, cnt - 1, 64);
+    const T last_val = __shfl(val, cnt - 1, 64);
+    if (base + cnt < end) {
+      carry_key = last_key;
+      carry_val = last_val;
+    } else {
+      carry_key = -1;
     }
   }
-  if (last >= 0) {
-    ci[o] = last;
-    cw[o] = acc;
-    ++o;
-  }
-  ucnt[r] = o - beg;
+  if (lane == 0) ucnt[r] = o - beg;
 }
 
 template <typename T>
@@ -284,8 +320,8 @@ static int coarsen_merge_sorted(int nc, const int32_t* off, int32_t* ci, T* cw, 
   hipError_t e = rocprim::segmented_radix_sort_pairs(tmp, tmp_bytes, keys, sk, cw, sv, (unsigned)total, (unsigned)nc,
                                                      off, off + 1, 0, bits, st);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_coarse_merge_sorted<T>, dim3(grid_for(nc)), dim3(kBlock), 0, st, nc, off, sk, sv, ci, cw,
-                       ucnt);
+    hipLaunchKernelGGL(k_coarse_merge_sorted<T>, dim3(grid_for((long long)nc * 64)), dim3(kBlock), 0, st, nc, off,
+                       sk, sv, ci, cw, ucnt);
     e = hipGetLastError();
   }
   const hipError_t ef = hipFreeAsync(buf, st);
