@@ -179,9 +179,21 @@ class AMG:
         X = prolong_add(L.agg, self._cycle(lvl + 1, Rc), X)
         return self._smooth(L, X, R, self.nu)
 
+    def _apply_block(self, R):
+        """One V-cycle on a block of at most 256 columns.  FP32 blocks run padded to a multiple
+        of 4 columns (zero columns stay zero: the cycle is linear), so the level operators read
+        16-byte words."""
+        C = R.shape[1]
+        Cp = (C + 3) // 4 * 4
+        if R.dtype != torch.float32 or Cp == C or Cp > 256:
+            return self._cycle(0, R)
+        Rp = torch.zeros((R.shape[0], Cp), dtype=R.dtype, device=R.device)
+        Rp[:, :C] = R
+        return self._cycle(0, Rp)[:, :C].contiguous()
+
     def __call__(self, R):
         R = R.contiguous()
         if R.shape[1] <= 256:
-            return self._cycle(0, R)
+            return self._apply_block(R)
         # the row kernels take at most 256 columns: apply the (linear) cycle per column block
-        return torch.cat([self._cycle(0, R[:, j:j + 256].contiguous()) for j in range(0, R.shape[1], 256)], dim=1)
+        return torch.cat([self._apply_block(R[:, j:j + 256].contiguous()) for j in range(0, R.shape[1], 256)], dim=1)
