@@ -234,7 +234,7 @@ def main(argv=None):
                          "(reads 512 MiB, writes 512 MiB) that calibrates FETCH_SIZE / WRITE_SIZE")
     a = ap.parse_args(argv)
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {a.gpus}); "
+        print(f"bench.py runs on one GPU: the package has no multi-GPU path (asked for {a.gpus}); "
               "the per-GPU workload is the whole graph", file=sys.stderr)
         return 2
 
