@@ -109,17 +109,23 @@ __global__ void k_coarse_fill(int n, const int32_t* __restrict__ indptr, const i
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const int ar = agg[r];
-  for (int j = indptr[r]; j < indptr[r + 1]; ++j) {
+  const int beg = indptr[r], end = indptr[r + 1];
+  // every entry of fine row r lands in coarse row agg[r]: one reservation for all of them (the
+  // few coarse rows of a deep level would otherwise take one atomic per fine entry)
+  int k = 0;
+  for (int j = beg; j < end; ++j) k += agg[indices[j]] != ar;
+  if (k == 0) return;
+  int pos = atomicAdd(cursor + ar, k);
+  for (int j = beg; j < end; ++j) {
     const int ac = agg[indices[j]];
     if (ac != ar) {
-      const int pos = atomicAdd(cursor + ar, 1);
       ci[pos] = ac;
       cw[pos] = w[j];
+      ++pos;
     }
   }
 }
 
-// Sort a coarse row's entries by column and merge duplicates (summing weights); ucnt = unique.
 // Duplicate merge of row-sorted entries (sk, sv), one wavefront per row: every run of equal
 // columns becomes one entry (weights summed) at the front of the row's segment of ci/cw, unique
 // count into ucnt.  The row is read in chunks of 64 entries; inside a chunk a segmented
