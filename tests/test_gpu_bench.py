@@ -25,7 +25,9 @@ def test_bench_line_and_csv_rows(tmp_path, capsys):
     cpu = line["cpu_baseline"]
     assert cpu["kind"] == "port" and cpu["cores"] == 1 and cpu["value"] > 0
     assert line["fp64_path"]["value"] > 0 and line["rho_max"] <= 1e-6 * 1.0001
-    assert {k["kernel"].split()[0] for k in line["kernels"]} >= {"spmm_axpy", "frontier_jacobi", "push_scatter"}
+    assert {k["kernel"].split()[0] for k in line["kernels"]} >= {"spmm_axpy", "frontier_jacobi", "push_scatter",
+                                                                   "amg"}
+    assert all(k["achieved"] > 0 and 0 < k["frac"] < 1 for k in line["kernels"])
     with open(out) as fh:
         rows = list(csv.DictReader(fh))
     assert len(rows) == 3 and list(rows[0]) == report.FIELDS
