@@ -230,7 +230,13 @@ def config_graph(cfg: int, scale: float = 1.0, seed: int = 0) -> Graph:
 
 @dataclass
 class EdgeBatch:
-    """One request: edge insertions (u, v, w) and deletions (u, v); pairs distinct in a batch."""
+    """One request: edge insertions (u, v, w) and deletions (u, v) of undirected pairs.
+
+    Vertex ids lie in [0, n) (checked when the batch is applied).  The unordered pairs are
+    distinct within the deletions and within the insertions (not checked: a repeated pair is
+    applied once per occurrence).  An insertion of a present edge replaces its weight, a
+    deletion of an absent edge is counted as missing, and self-loops are ignored, as in the
+    system built from the edge set (its off-diagonal part only)."""
     ins_u: np.ndarray
     ins_v: np.ndarray
     ins_w: np.ndarray

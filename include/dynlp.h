@@ -236,8 +236,10 @@ int dlp_lp_set_store(void* handle, const int32_t* indptr, int32_t* rowlen, int32
 int dlp_lp_set_eps(void* handle, const double* eps, double push_scale, void* stream);
 /* New batch: clears the changed-row list, counters and the batch-touched list. */
 int dlp_lp_begin_batch(void* handle, void* stream);
-/* Deletions (du, dv) then insertions (iu, iv, iw) of undirected pairs (distinct in a call):
- * both stored directions change; endpoints join the changed-row list.  Missing deletions and
+/* Deletions (du, dv) then insertions (iu, iv, iw) of undirected pairs (distinct in a call;
+ * device arrays; ids in [0, n), which the caller checks: the kernels index rows by them).
+ * Self-loops are skipped; an insertion of a stored pair replaces its weight.
+ * Both stored directions change; endpoints join the changed-row list.  Missing deletions and
  * insertions that found a full row are counted (dlp_lp_status); overflowed insertions are
  * re-inserted by dlp_lp_regrow_commit.  Replaces IncrementalStream._toggle/_set_w
  * (stream.py:202-228) and the weight/row-sum/class-weight updates (stream.py:221-235). */

@@ -360,6 +360,57 @@ def test_verify_pass_matches_full_refresh():
         assert np.all(Zs[~solved, : lp.C] == 0) and np.all(Zs[~solved, lp.C] == 1)
 
 
+def test_batch_edge_cases_match_oracle():
+    """Requests at the edges of the batch contract, each against the oracle's edge set: an empty
+    batch, deletions only, deletions of absent edges (counted missing, no change), insertions of
+    present edges (weight replaced), self-loops (ignored), a labeled vertex losing and regaining
+    every edge; out-of-range ids and ragged arrays are refused on the host before any device
+    work, and the state stays usable."""
+    from dynlp_amd.graphs import EdgeBatch
+    from dynlp_amd.stream import EdgeStreamLP
+    g = graphs.sbm(2000, 4, 8, 0.8, 4, 0.05, seed=4)
+    eps = 1e-6
+    lp = EdgeStreamLP(g, eps=eps)
+    lp.initial_solve()
+    es = edge_dict(g)
+    i32, f32 = (lambda x: np.asarray(x, dtype=np.int32)), (lambda x: np.asarray(x, dtype=np.float32))
+    keys = sorted(es)
+    absent = [(a, b) for a, b in ((0, 1999), (3, 1500), (7, 1234)) if (a, b) not in es]
+    lab = int(np.flatnonzero(g.labeled)[0])
+    nbrs = [(min(a, b), max(a, b)) for a, b in keys if lab in (a, b)]
+    cases = [
+        ("empty", [], [], [], [], []),
+        ("deletions only", [], [], [], [k[0] for k in keys[:40]], [k[1] for k in keys[:40]]),
+        ("absent deletions", [], [], [], [a for a, _ in absent], [b for _, b in absent]),
+        ("reweight present", [k[0] for k in keys[100:140]], [k[1] for k in keys[100:140]],
+         np.linspace(0.5, 3.0, 40), [], []),
+        ("self-loops", [5, 6, 10], [5, 6, 11], [2.0, 2.0, 1.5], [9], [9]),
+        ("labeled vertex isolated", [], [], [], [a for a, _ in nbrs], [b for _, b in nbrs]),
+        ("labeled vertex back", [a for a, _ in nbrs], [b for _, b in nbrs], np.full(len(nbrs), 0.7), [], []),
+    ]
+    for name, iu, iv, iw, du, dv in cases:
+        b = EdgeBatch(i32(iu), i32(iv), f32(iw), i32(du), i32(dv))
+        absent_now = sum((min(a, c), max(a, c)) not in es for a, c in zip(du, dv) if a != c)
+        st = lp.apply_and_solve(b)
+        O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
+                           zip(b.del_u.tolist(), b.del_v.tolist()))
+        # both directions of every deletion that finds no stored edge are counted (self-loops
+        # are skipped, never stored)
+        assert st.extra.get("missing_deletions", 0) == 2 * absent_now, name
+        assert st.rho <= eps * 1.0001, name
+        check_against_oracle(lp, g, es, eps, name)
+    for bad in (EdgeBatch(i32([0]), i32([g.n]), f32([1.0]), i32([]), i32([])),
+                EdgeBatch(i32([]), i32([]), f32([]), i32([-1]), i32([3])),
+                EdgeBatch(i32([1, 2]), i32([3]), f32([1.0]), i32([]), i32([]))):
+        with pytest.raises((IndexError, ValueError)):
+            lp.apply_and_solve(bad)
+    b = graphs.update_stream(g, 1, 200, seed=4)[0][0]
+    lp.apply_and_solve(b)
+    O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
+                       zip(b.del_u.tolist(), b.del_v.tolist()))
+    check_against_oracle(lp, g, es, eps, "after refused batches")
+
+
 def test_overflow_regrow_keeps_results():
     from dynlp_amd.stream import EdgeStreamLP
     g = graphs.sbm(2000, 4, 6, 0.8, 4, 0.03, seed=3)
