@@ -14,6 +14,10 @@ the serving layer the workload asks for:
   requests that arrive while the graph is busy are served by its next repair.
 - Overlap.  `slots` workers run concurrently, each on its own HIP stream (default priority);
   the C ABI releases the GIL, so repairs of different graphs overlap on the device.
+- Failures.  A request whose batch is malformed (ragged arrays, ids outside the graph) is
+  refused on its own, before fusing; a repair that raises fails the requests fused into it and
+  drops the graph's device state, which is rebuilt (initial solve plus the updates already
+  served) when the graph's next request is taken.  Either way the request carries `error`.
 - Latency is arrival -> certified result; throughput counts edge updates.
 """
 from __future__ import annotations
@@ -26,8 +30,11 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from .graphs import EdgeBatch, Graph
+from .graphs import EdgeBatch, Graph, check_batch
 from .stream import EdgeStreamLP
+
+# insertions per apply call of a replayed history (the device apply takes at most 2^19)
+_REPLAY_CHUNK = 1 << 18
 
 
 @dataclass
@@ -38,6 +45,7 @@ class Request:
     done: float = float("nan")
     rho: float = float("nan")
     fused_with: int = 1
+    error: str | None = None       # why the request was refused or its repair failed
 
     @property
     def latency_ms(self):
@@ -71,6 +79,16 @@ def fuse(batches):
     return EdgeBatch(k[ins] // n, k[ins] % n, w[ins], k[dele] // n, k[dele] % n)
 
 
+def replay_chunks(b: EdgeBatch, chunk=_REPLAY_CHUNK):
+    """A fused batch (every pair once, as a deletion or an insertion) split into batches of at
+    most `chunk` operations; since no pair repeats, applying them in any order gives b."""
+    ops = [(b.del_u[i:i + chunk], b.del_v[i:i + chunk]) for i in range(0, len(b.del_u), chunk)]
+    out = [EdgeBatch(b.ins_u[:0], b.ins_v[:0], b.ins_w[:0], du, dv) for du, dv in ops]
+    out += [EdgeBatch(b.ins_u[i:i + chunk], b.ins_v[i:i + chunk], b.ins_w[i:i + chunk], b.del_u[:0], b.del_v[:0])
+            for i in range(0, len(b.ins_u), chunk)]
+    return out
+
+
 def footprint_bytes(g: Graph, fp64=False):
     """Device bytes of one resident graph (store with slack, state, PCG buffers, scratch)."""
     n, C1 = g.n, (1 if g.K == 2 else g.K) + 1
@@ -96,6 +114,7 @@ class Slot:
     bytes: int = 0
     served: int = 0
     solves: int = 0
+    history: list = field(default_factory=list)   # batches served, replayed after an eviction
 
 
 class Service:
@@ -116,7 +135,8 @@ class Service:
         self.lock = threading.Lock()
         self.cv = threading.Condition(self.lock)
         self.done = []
-        self.stats = {"admissions": 0, "evictions": 0, "fused_requests": 0, "repairs": 0}
+        self.stats = {"admissions": 0, "evictions": 0, "fused_requests": 0, "repairs": 0, "refused": 0,
+                      "failed_repairs": 0}
         self._t0 = None
         self._stop = False
 
@@ -160,27 +180,54 @@ class Service:
                     s, build = job
                     s.busy = True
                     reqs = [s.queue.popleft() for _ in range(min(len(s.queue), self.max_fuse))]
-                if build:
-                    s.lp = EdgeStreamLP(s.graph, eps=self.eps, **self.lp_kwargs)
-                    s.lp.initial_solve()
-                    s.solves += 1
-                    # an evicted graph re-enters with the updates it had already served
-                    if getattr(s, "history", None):
-                        s.lp.apply_and_solve(fuse(s.history))
-                b = fuse([r.batch for r in reqs])
-                st = s.lp.apply_and_solve(b)
-                stream.synchronize()
+                bad = []
+                for r in reqs:
+                    try:
+                        check_batch(r.batch, s.graph.n)
+                    except (IndexError, ValueError) as e:
+                        r.error = f"refused: {e}"
+                        bad.append(r)
+                good = [r for r in reqs if r.error is None]
+                st, b, err = None, None, None
+                try:
+                    if build:
+                        s.lp = EdgeStreamLP(s.graph, eps=self.eps, **self.lp_kwargs)
+                        s.lp.initial_solve()
+                        s.solves += 1
+                        # an evicted graph re-enters with the updates it had already served
+                        for piece in (replay_chunks(fuse(s.history)) if s.history else []):
+                            s.lp.apply_and_solve(piece)
+                    if good:
+                        b = fuse([r.batch for r in good])
+                        st = s.lp.apply_and_solve(b)
+                    stream.synchronize()
+                except Exception as e:  # noqa: BLE001  (reported on the requests, service goes on)
+                    err = f"repair failed: {type(e).__name__}: {e}"
                 now = time.perf_counter() - self._t0
                 with self.cv:
-                    for r in reqs:
-                        r.done, r.rho, r.fused_with = now, st.rho, len(reqs)
-                    s.history = getattr(s, "history", []) + [b]
+                    for r in bad:
+                        r.done = now
+                    if err is not None:
+                        for r in good:
+                            r.done, r.error = now, err
+                        # resident (or reserved by the admission): state unknown, rebuilt on next use
+                        s.lp = None
+                        self.resident -= s.bytes
+                        self.stats["failed_repairs"] += 1
+                    else:
+                        for r in good:
+                            r.done, r.rho, r.fused_with = now, st.rho, len(good)
+                        if b is not None:
+                            s.history.append(b)
+                            if len(s.history) > 32:     # keep the replay log one fused batch
+                                s.history = [fuse(s.history)]
+                        s.served += len(good)
+                        self.stats["fused_requests"] += len(good)
+                        self.stats["repairs"] += int(b is not None)
                     self.done.extend(reqs)
-                    s.served += len(reqs)
+                    self.stats["refused"] += len(bad)
                     s.busy = False
                     s.last_used = now
-                    self.stats["fused_requests"] += len(reqs)
-                    self.stats["repairs"] += 1
                     self.cv.notify_all()
 
     # -- driver ----------------------------------------------------------------------------
@@ -224,14 +271,15 @@ class Service:
         return self.done
 
     def report(self):
-        lat = np.array([r.latency_ms for r in self.done])
-        edges = sum(r.batch.size for r in self.done)
-        return {"requests": len(self.done), "edge_updates": edges,
+        ok = [r for r in self.done if r.error is None]
+        lat = np.array([r.latency_ms for r in ok])
+        edges = sum(r.batch.size for r in ok)
+        return {"requests": len(ok), "errors": len(self.done) - len(ok), "edge_updates": edges,
                 "edge_updates_per_s": edges / self.wall if self.wall > 0 else float("nan"),
                 "p50_ms": float(np.percentile(lat, 50)) if lat.size else float("nan"),
                 "p99_ms": float(np.percentile(lat, 99)) if lat.size else float("nan"),
-                "max_rho": max((r.rho for r in self.done), default=float("nan")),
-                "mean_fused": float(np.mean([r.fused_with for r in self.done])) if self.done else 0.0,
+                "max_rho": max((r.rho for r in ok), default=float("nan")),
+                "mean_fused": float(np.mean([r.fused_with for r in ok])) if ok else 0.0,
                 **self.stats}
 
 

@@ -23,7 +23,7 @@ import torch
 
 from . import _lib
 from .backend import device, ptr, stream
-from .graphs import EdgeBatch, Graph
+from .graphs import EdgeBatch, Graph, check_batch
 from .solvers import System  # noqa: F401  (the batch system, reference stream.py:17-58)
 
 
@@ -269,15 +269,8 @@ class EdgeStreamLP:
     def _upload(self, b: EdgeBatch):
         """The batch's five arrays packed into one pinned staging buffer and sent with one copy
         (the previous request has finished with both buffers: every request ends in a sync)."""
+        check_batch(b, self.n)
         nd, ni = len(b.del_u), len(b.ins_u)
-        if len(b.del_v) != nd or len(b.ins_v) != ni or len(b.ins_w) != ni:
-            raise ValueError(f"batch arrays differ in length: del {nd}/{len(b.del_v)}, "
-                             f"ins {ni}/{len(b.ins_v)}/{len(b.ins_w)}")
-        for a in (b.del_u, b.del_v, b.ins_u, b.ins_v):
-            # the apply kernels index rows by these ids; out-of-range ids are caller errors
-            if len(a) and (int(a.min()) < 0 or int(a.max()) >= self.n):
-                raise IndexError(f"batch vertex ids must lie in [0, {self.n}): found "
-                                 f"{int(a.min())}..{int(a.max())}")
         total = 2 * nd + 3 * ni
         if self._stage is None or self._stage.numel() < total:
             cap = max(total, 1 << 16)

@@ -186,8 +186,10 @@ def bound(rc, h_est, C, eps_h):
 # ---------------------------------------------------------------------------------------------
 # Exact solution (the reference's tests compare against dense solves, tests/test_dynlp.py:17-19).
 
-def exact_solve(W, diag, rhs, tol=1e-14, dense_max=3000):
-    """F* with (diag - W) F* = rhs in FP64: dense solve for small n, sparse LU otherwise."""
+def exact_solve(W, diag, rhs, tol=1e-14, dense_max=16000):
+    """F* with (diag - W) F* = rhs in FP64: dense solve up to dense_max rows (LAPACK, seconds at
+    16,000; sparse LU fills in badly on these expander-like graphs: 168 s at 12,000 rows of an
+    SBM), sparse LU beyond."""
     n = W.shape[0]
     rhs = np.asarray(rhs, dtype=np.float64)
     if n == 0:

@@ -80,3 +80,44 @@ def test_fuse_last_operation_wins():
     ins = {(min(x, y), max(x, y)): w for x, y, w in zip(f.ins_u, f.ins_v, f.ins_w)}
     dels = {(min(x, y), max(x, y)) for x, y in zip(f.del_u, f.del_v)}
     assert ins == {(2, 6): 2.0, (3, 4): 7.0} and dels == {(1, 5)}
+
+
+def test_malformed_requests_are_refused_and_the_rest_served():
+    """Requests with ids outside their graph or ragged arrays fail on their own (error set,
+    nothing applied); every other request is served and certified, and each graph ends at the
+    exact solution of its well-formed updates."""
+    from dynlp_amd.graphs import EdgeBatch
+    from dynlp_amd.serve import Request, Service, footprint_bytes
+    gs = _graphs()
+    reqs = _requests(gs, n=24)
+    i32 = lambda x: np.asarray(x, dtype=np.int32)  # noqa: E731
+    t_mid = reqs[len(reqs) // 2].arrival
+    bad = [Request(0, EdgeBatch(i32([1]), i32([gs[0].n]), np.ones(1, np.float32), i32([]), i32([])), t_mid),
+           Request(2, EdgeBatch(i32([1, 2]), i32([3]), np.ones(1, np.float32), i32([]), i32([])), t_mid)]
+    svc = Service(gs, sum(footprint_bytes(g) for g in gs) + (1 << 20), slots=3, eps=EPS)
+    svc.run(reqs + bad)
+    rep = svc.report()
+    assert rep["errors"] == 2 and rep["refused"] == 2 and rep["requests"] == len(reqs)
+    assert all(r.error and r.error.startswith("refused") for r in bad)
+    assert all(r.error is None and r.rho <= EPS * 1.0001 for r in reqs)
+    for gid, g in enumerate(gs):
+        sysm, Fs = _oracle_hmax(g, [r for r in reqs if r.gid == gid])
+        Z = svc.graphs[gid].lp.scores_with_h()
+        assert np.abs(Z[:, : sysm.C] - Fs[:, : sysm.C]).max() <= EPS * Fs[:, sysm.C].max() * 1.05
+
+
+def test_failed_builds_fail_their_requests_and_the_service_drains():
+    """A graph whose device state cannot be built (here: an invalid solver option, refused before
+    any allocation) fails each of its requests with the reason; the service still drains every
+    queue and returns."""
+    from dynlp_amd.serve import Service, footprint_bytes
+    gs = _graphs()[:2]
+    reqs = _requests(gs, n=10)
+    svc = Service(gs, sum(footprint_bytes(g) for g in gs) + (1 << 20), slots=2, eps=EPS,
+                  lp_kwargs={"pcg_mode": "no-such-mode"})
+    done = svc.run(reqs, preload=False)
+    assert len(done) == len(reqs)
+    assert all(r.error and "repair failed: ValueError" in r.error for r in done)
+    rep = svc.report()
+    assert rep["requests"] == 0 and rep["errors"] == len(reqs) and rep["failed_repairs"] >= 1
+    assert svc.resident == 0
