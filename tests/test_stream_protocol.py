@@ -1,7 +1,7 @@
 """The vertex-arrival schedule of stream.Stream (the reference's stream.py:84-140) on the host:
-labeled set, arrival order and batches.  The exact random draws follow the reference's
-sequence; with no reference run possible here they are parity-unpinned, and these tests check
-what the protocol fixes."""
+labeled set, arrival order and batches.  The batch sizes are pinned against the reference's
+published runs; the exact random draws follow the reference's sequence but, with no reference
+run possible here, are parity-unpinned, and these tests check what the protocol fixes."""
 import numpy as np
 import pytest
 
@@ -59,3 +59,29 @@ def test_unlabeled_ground_truth_and_incremental_subclass():
     st = IncrementalStream(g, n_batches=2, label_frac=0.05)
     assert not st.labeled[g.y < 0].any()
     assert st.ball_hops == 3 and len(st) == 3
+
+
+def _reference_schedules():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "reference_schedules.json")) as fh:
+        return json.load(fh)
+
+
+@pytest.mark.parametrize("run", _reference_schedules(), ids=lambda r: r["source"].split("/")[-1])
+def test_schedule_sizes_match_the_reference_runs(run):
+    """Per batch, the active, inserted and deleted vertex counts of the reference's published
+    stream runs (results/*.csv, arguments from results/*.json; fixture written by
+    tests/golden/make_reference_schedules.py).  The runs drew on CuPy's generator, so which
+    vertices arrive is not comparable; the counts are fixed by the protocol's arithmetic
+    (snapshot, chunking of the arrival order, deletions per batch) and must be equal."""
+    a = run["args"]
+    n, K = int(a["n"]), int(a["classes"])
+    g = graphs.erdos_renyi(n, 2.0, K, 0.01, seed=1)      # any graph of n vertices: counts only
+    st = Stream(g, label_frac=a["label_frac"], init_frac=a["init_frac"], n_batches=int(a["batches"]),
+                del_frac=a["del_frac"], seed=int(a["seed"]))
+    assert len(st) == len(run["rows"])
+    active = 0
+    for (t, n_active, inserted, deleted), (ins, dele) in zip(run["rows"], st.schedule):
+        active += ins.size - dele.size
+        assert (ins.size, dele.size, active) == (inserted, deleted, n_active), t
