@@ -51,3 +51,21 @@ def test_replay_chunks_equal_the_fused_batch():
         for p in pieces:
             _apply(chunked, p)
         assert one == chunked, trial
+
+
+def test_check_batch_refuses_ragged_and_out_of_range():
+    import pytest
+    from dynlp_amd.graphs import EdgeBatch, check_batch
+    i = lambda *x: np.asarray(x, dtype=np.int32)  # noqa: E731
+    f = lambda *x: np.asarray(x, dtype=np.float32)  # noqa: E731
+    e = np.zeros(0, dtype=np.int32)
+    check_batch(EdgeBatch(i(0, 9), i(9, 0), f(1, 2), i(3), i(4)), 10)       # valid, ids at both ends
+    check_batch(EdgeBatch(e, e, e.astype(np.float32), e, e), 10)           # empty
+    check_batch(EdgeBatch(i(5), i(5), f(1), i(6), i(6)), 10)               # self-loops: ignored later
+    for bad, exc in ((EdgeBatch(i(0), i(10), f(1), e, e), IndexError),
+                     (EdgeBatch(e, e, e.astype(np.float32), i(-1), i(2)), IndexError),
+                     (EdgeBatch(i(1, 2), i(3), f(1, 1), e, e), ValueError),
+                     (EdgeBatch(i(1), i(3), f(1, 1), e, e), ValueError),
+                     (EdgeBatch(e, e, e.astype(np.float32), i(1, 2), i(3)), ValueError)):
+        with pytest.raises(exc):
+            check_batch(bad, 10)
