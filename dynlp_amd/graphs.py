@@ -248,13 +248,16 @@ class EdgeBatch:
         return int(self.ins_u.shape[0] + self.del_u.shape[0])
 
 
-def check_batch(b: EdgeBatch, n: int) -> None:
-    """Refuses a batch whose arrays differ in length (ValueError) or whose vertex ids leave
-    [0, n) (IndexError): the device apply indexes rows by these ids."""
+def check_batch(b: EdgeBatch, n: int, max_insertions: int | None = None) -> None:
+    """Refuses a batch whose arrays differ in length or that holds more than max_insertions
+    insertions (ValueError), or whose vertex ids leave [0, n) (IndexError): the device apply
+    indexes rows by these ids."""
     nd, ni = len(b.del_u), len(b.ins_u)
     if len(b.del_v) != nd or len(b.ins_v) != ni or len(b.ins_w) != ni:
         raise ValueError(f"batch arrays differ in length: del {nd}/{len(b.del_v)}, "
                          f"ins {ni}/{len(b.ins_v)}/{len(b.ins_w)}")
+    if max_insertions is not None and ni > max_insertions:
+        raise ValueError(f"{ni} insertions in one batch: at most {max_insertions} (split the batch)")
     for a in (b.del_u, b.del_v, b.ins_u, b.ins_v):
         if len(a) and (int(a.min()) < 0 or int(a.max()) >= n):
             raise IndexError(f"batch vertex ids must lie in [0, {n}): found {int(a.min())}..{int(a.max())}")

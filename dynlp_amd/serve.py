@@ -9,9 +9,10 @@ the serving layer the workload asks for:
   make room (an evicted graph keeps its host-side edge set and is re-admitted with a fresh
   solve).
 - Continuous batching.  Requests (edge-update batches for one graph) queue per graph.  A worker
-  takes *all* requests queued for a graph at that moment and fuses them into one batch (per
-  pair, the last operation wins, which is what applying them in arrival order gives), so
-  requests that arrive while the graph is busy are served by its next repair.
+  takes the requests queued for a graph at that moment (up to max_fuse, and while their
+  insertions fit one device apply call, 2^19) and fuses them into one batch (per pair, the
+  last operation wins, which is what applying them in arrival order gives), so requests that
+  arrive while the graph is busy are served by its next repair.
 - Overlap.  `slots` workers run concurrently, each on its own HIP stream (default priority);
   the C ABI releases the GIL, so repairs of different graphs overlap on the device.
 - Failures.  A request whose batch is malformed (ragged arrays, ids outside the graph) is
@@ -33,7 +34,9 @@ import torch
 from .graphs import EdgeBatch, Graph, check_batch
 from .stream import EdgeStreamLP
 
-# insertions per apply call of a replayed history (the device apply takes at most 2^19)
+# the device apply takes at most 2^19 insertions per call: requests are fused up to that many,
+# and a replayed history is applied in chunks of half of it
+_MAX_INSERTIONS = 1 << 19
 _REPLAY_CHUNK = 1 << 18
 
 
@@ -179,11 +182,18 @@ class Service:
                             self.cv.wait(timeout=0.05)
                     s, build = job
                     s.busy = True
-                    reqs = [s.queue.popleft() for _ in range(min(len(s.queue), self.max_fuse))]
+                    # fuse queued requests while their insertions fit one apply call
+                    reqs, ins = [], 0
+                    while s.queue and len(reqs) < self.max_fuse:
+                        k = len(s.queue[0].batch.ins_u)
+                        if reqs and ins + k > _MAX_INSERTIONS:
+                            break
+                        reqs.append(s.queue.popleft())
+                        ins += k
                 bad = []
                 for r in reqs:
                     try:
-                        check_batch(r.batch, s.graph.n)
+                        check_batch(r.batch, s.graph.n, max_insertions=_MAX_INSERTIONS)
                     except (IndexError, ValueError) as e:
                         r.error = f"refused: {e}"
                         bad.append(r)

@@ -269,7 +269,7 @@ class EdgeStreamLP:
     def _upload(self, b: EdgeBatch):
         """The batch's five arrays packed into one pinned staging buffer and sent with one copy
         (the previous request has finished with both buffers: every request ends in a sync)."""
-        check_batch(b, self.n)
+        check_batch(b, self.n, max_insertions=1 << 19)   # the device apply's per-call limit
         nd, ni = len(b.del_u), len(b.ins_u)
         total = 2 * nd + 3 * ni
         if self._stage is None or self._stage.numel() < total:

@@ -121,3 +121,24 @@ def test_failed_builds_fail_their_requests_and_the_service_drains():
     rep = svc.report()
     assert rep["requests"] == 0 and rep["errors"] == len(reqs) and rep["failed_repairs"] >= 1
     assert svc.resident == 0
+
+
+def test_fusion_stops_at_the_insertion_limit(monkeypatch):
+    """Requests queued together are fused only while their insertions fit one device apply call
+    (the limit lowered here so that two of these requests fit); every request is served and the
+    graph ends at the exact solution of all of them."""
+    from dynlp_amd import serve
+    gs = _graphs()[:1]
+    reqs = serve.poisson_requests(gs, rate=1e9, duration=1.0, size_lo=600, size_hi=601, seed=7, max_requests=6)
+    ins = [len(r.batch.ins_u) for r in reqs]
+    limit = max(a + b for a, b in zip(ins, ins[1:]))
+    assert limit < min(ins) * 3
+    monkeypatch.setattr(serve, "_MAX_INSERTIONS", limit)
+    svc = serve.Service(gs, serve.footprint_bytes(gs[0]) + (1 << 20), slots=1, eps=EPS)
+    svc.run(reqs)
+    rep = svc.report()
+    assert rep["errors"] == 0 and rep["requests"] == len(reqs)
+    assert max(r.fused_with for r in reqs) <= 2 and rep["repairs"] >= 3
+    sysm, Fs = _oracle_hmax(gs[0], reqs)
+    Z = svc.graphs[0].lp.scores_with_h()
+    assert np.abs(Z[:, : sysm.C] - Fs[:, : sysm.C]).max() <= EPS * Fs[:, sysm.C].max() * 1.05

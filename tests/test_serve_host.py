@@ -69,3 +69,7 @@ def test_check_batch_refuses_ragged_and_out_of_range():
                      (EdgeBatch(e, e, e.astype(np.float32), i(1, 2), i(3)), ValueError)):
         with pytest.raises(exc):
             check_batch(bad, 10)
+    two = EdgeBatch(i(1, 2), i(3, 4), f(1, 1), e, e)
+    check_batch(two, 10, max_insertions=2)
+    with pytest.raises(ValueError, match="at most 1"):
+        check_batch(two, 10, max_insertions=1)
