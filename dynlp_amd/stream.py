@@ -341,7 +341,9 @@ class EdgeStreamLP:
         return st
 
     def apply_and_solve(self, b: EdgeBatch) -> Stats:
-        """Apply one batch of edge updates and repair the solution to eps (certified in FP64)."""
+        """Apply one batch of edge updates and repair the solution to eps (certified in FP64).
+        Ragged arrays, more than 2^19 insertions (ValueError) and ids outside [0, n)
+        (IndexError) are refused before any device work."""
         st = Stats(path="fp64-push" if self.fp64 else "push")
         t0 = time.perf_counter()
         du, dv, iu, iv, iw = self._upload(b)
