@@ -61,6 +61,14 @@ def operator_bytes(n, C1, stored, p_bytes=4):
     return 4 * (n + 1) + 4 * n + 8 * stored + n + p_bytes * n * C1 + 4 * n + 4 * n * C1
 
 
+def verify_bytes(n, C1, stored):
+    """Algorithmic bytes of the FP64 verification pass over n solved rows (each element once):
+    row offsets and lengths, stored entries (column + FP32 weight), the solved mask, the FP32
+    solution rows, the FP64 diagonal, the labeled mask and labels of the right-hand side, and
+    the FP32 residual rows written back."""
+    return 4 * (n + 1) + 4 * n + 8 * stored + n + 4 * n * C1 + 8 * n + n + 4 * n + 4 * n * C1
+
+
 def timed(fn, reps):
     """Average ms of fn() over reps launches, HIP events on torch's current stream (the stream
     the library launches on)."""
@@ -304,6 +312,17 @@ def main(argv=None):
                 if traffic else None,
                 "avg_launch_ms": round(op_ms, 4), "algorithmic_bytes": int(nbytes)}
     log(f"A*P launch: {op_ms:.3f} ms, {nbytes / 1e6:.0f} MB -> {achieved:.0f} GB/s")
+    # the FP64 verification pass every global repair ends with (refresh mode 3 over every
+    # solved row, on the certified state it leaves unchanged)
+    from dynlp_amd import _lib
+    from dynlp_amd.backend import stream as cur_stream
+    ver_ms = timed(lambda: _lib.call("dlp_lp_refresh", lp.h, 3, 0, cur_stream()), 10)
+    vbytes = verify_bytes(g.n, lp.C1, stored)
+    verify = {"kernel": "k_verify<4> (FP64 residual of every solved row, column maxima, h maximum)",
+              "bound": "hbm", "ms": round(ver_ms, 4), "algorithmic_bytes": int(vbytes),
+              "achieved": round(vbytes / (ver_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+              "frac": round(vbytes / (ver_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"verification pass: {ver_ms:.3f} ms, {vbytes / 1e6:.0f} MB -> {verify['achieved']:.0f} GB/s")
     pcg_dirs, pcg_coarse = lp.pcg_directions, lp.pcg_coarse
     pcg_iters = pcg1["pcg_iters"] - pcg0["pcg_iters"]
     err = lp.error_report()
@@ -377,7 +396,7 @@ def main(argv=None):
         "error": err,
         "fp64_path": fp64,
         "speedup_vs_fp64_path": round(value / fp64["value"], 2) if fp64 else None,
-        "roofline": roofline, "cpu_baseline": cpu, "kernels": ktab,
+        "roofline": roofline, "verify_pass": verify, "cpu_baseline": cpu, "kernels": ktab,
         "wall_s": round(time.perf_counter() - t_all, 1),
     }
     if a.out:

@@ -22,6 +22,9 @@ def test_bench_line_and_csv_rows(tmp_path, capsys):
     roof = line["roofline"]
     assert roof["bound"] == "hbm" and roof["unit"] == "GB/s" and roof["peak"] == 8000.0
     assert roof["achieved"] > 0 and roof["frac"] == pytest.approx(roof["achieved"] / roof["peak"], abs=1e-4)
+    ver = line["verify_pass"]
+    assert ver["ms"] > 0 and 0 < ver["frac"] < 1 and ver["achieved"] == pytest.approx(
+        ver["algorithmic_bytes"] / (ver["ms"] * 1e-3) / 1e9, rel=5e-3)   # ms is rounded to 4 decimals
     cpu = line["cpu_baseline"]
     assert cpu["kind"] == "port" and cpu["cores"] == 1 and cpu["value"] > 0
     assert line["fp64_path"]["value"] > 0 and line["rho_max"] <= 1e-6 * 1.0001
