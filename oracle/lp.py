@@ -508,7 +508,12 @@ PINS = {
     "sync_push": "same invariants; equals push when one round is run",
     "pcg": "exact solve; sum_c F_c = 1 identity (tests/test_dynlp.py:148-156)",
     "block_pcg": "exact solve; single column == scalar CG iterates",
-    "AMG": "M(I) symmetric positive definite (tests/test_dynlp.py:159-166); two-level hand example",
+    "AMG": "M(I) equals the dense two-level V-cycle X1 = w D^-1 R, X2 = X1 + P (P^T A P)^-1 P^T (R - A X1), "
+           "M R = X2 + w D^-1 (R - A X2) on hand and random 7-8 vertex graphs; M(I) SPD "
+           "(tests/test_dynlp.py:159-166); no-coarsening case M = A^-1",
+    "_handshake": "hand cases for the smallest-id tie, the join of unmatched vertices to the heaviest "
+                  "matched neighbour (and its tie), singletons, the round limit; a loop-level brute force "
+                  "of the matching rules on random graphs, composed over two passes",
     "exact_solve": "numpy.linalg.solve dense; residual ~ 1e-14",
     "rho_cols/bound": "certificate |F - F*| <= rho h for perturbed iterates (tests/test_dynlp.py:136-145)",
 }

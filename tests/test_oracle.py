@@ -227,6 +227,176 @@ def test_amg_two_level_hand_example():
     np.testing.assert_allclose(sc, [1, 2])
 
 
+def _sym(n, edges):
+    W = np.zeros((n, n))
+    for a, b, w in edges:
+        W[a, b] = W[b, a] = w
+    return W
+
+
+def _hs(W, rounds=3):
+    c = sp.coo_matrix(W)
+    return O._handshake(W.shape[0], c.row.astype(np.int64), c.col.astype(np.int64), c.data, rounds)
+
+
+def test_handshake_tie_takes_smallest_id():
+    """amg.py:38-44: among equally heavy edges the smallest neighbour id is chosen.  Vertex 0
+    sees 1 and 2 at weight 2: it must pick 1, so {0,1} and then {2,3} match.  A largest-id
+    rule would match {0,2}, leave 1 and 3 without an unmatched neighbour and put all four in
+    one aggregate through the join step."""
+    agg, nc = _hs(_sym(4, [(0, 1, 2.0), (0, 2, 2.0), (2, 3, 1.0)]))
+    assert nc == 2 and list(agg) == [0, 0, 1, 1]
+
+
+def test_handshake_unmatched_join_heaviest_matched_neighbour():
+    """amg.py:56-60: after the rounds, a vertex with no unmatched neighbour left joins the
+    aggregate of its heaviest matched neighbour (ties: smallest id); one with no matched
+    neighbour either stays a singleton (amg.py:61-62)."""
+    # 4 proposes to 3 (weight 2 > 1) but 3 matched 2: 4 joins {2,3}
+    agg, nc = _hs(_sym(5, [(0, 1, 5.0), (2, 3, 5.0), (4, 1, 1.0), (4, 3, 2.0)]))
+    assert nc == 2 and list(agg) == [0, 0, 1, 1, 1]
+    # equal weights to 1 and 3: the smaller id (1) wins, so 4 joins {0,1}
+    agg, nc = _hs(_sym(5, [(0, 1, 5.0), (2, 3, 5.0), (4, 1, 2.0), (4, 3, 2.0)]))
+    assert nc == 2 and list(agg) == [0, 0, 1, 1, 0]
+    # an isolated vertex keeps its own aggregate; numbering follows the smallest member id
+    agg, nc = _hs(_sym(5, [(1, 2, 1.0), (3, 4, 1.0)]))
+    assert nc == 3 and list(agg) == [0, 1, 1, 2, 2]
+
+
+def test_handshake_round_limit():
+    """Matching stops after `rounds` rounds (amg.py:45-55): on the path 0-1-2-3 with weights
+    1, 2, 3 round one matches {2,3} only; a second round matches {0,1}.  With one round the
+    join step must put 1 with {2,3} (its only matched neighbour) and 0 stays alone."""
+    W = _sym(4, [(0, 1, 1.0), (1, 2, 2.0), (2, 3, 3.0)])
+    agg, nc = _hs(W, rounds=2)
+    assert nc == 2 and list(agg) == [0, 0, 1, 1]
+    agg, nc = _hs(W, rounds=1)
+    assert nc == 2 and list(agg) == [0, 1, 1, 1]
+
+
+def _two_level_dense(W, s, agg, omega):
+    """The V-cycle with nu = 1 on one fine level and an exact coarse solve, written out on
+    dense matrices (amg.py:114-131):  X1 = w D^-1 R;  X2 = X1 + P Ac^-1 P^T (R - A X1) with
+    Ac = P^T A P;  M R = X2 + w D^-1 (R - A X2)."""
+    n = W.shape[0]
+    D = s + W.sum(axis=1)
+    A = np.diag(D) - W
+    P = np.zeros((n, int(agg.max()) + 1))
+    P[np.arange(n), agg] = 1.0
+    Ac = P.T @ A @ P
+    R = np.eye(n)
+    X1 = omega * R / D[:, None]
+    X2 = X1 + P @ np.linalg.solve(Ac, P.T @ (R - A @ X1))
+    return X2 + omega * (R - A @ X2) / D[:, None]
+
+
+def test_amg_two_level_vcycle_matches_dense_formula():
+    """Path of 8 with weights 3,1,3,1,3,1,3: the first matching pass pairs {0,1},{2,3},{4,5},
+    {6,7}; on the coarse path (all weights 1) the second pass pairs {0,1},{2,3} (ties to the
+    smallest id), so the composed aggregation (amg.py:90-93) is [0,0,0,0,1,1,1,1] and two
+    coarse vertices remain (<= max_coarse: exact inverse, amg.py:104-108).  M(I) must equal
+    the dense two-level operator entrywise; that fixes omega, the smoother's sweep count, the
+    restriction/prolongation and the Galerkin coarse operator."""
+    W = _sym(8, [(i, i + 1, 3.0 if i % 2 == 0 else 1.0) for i in range(7)])
+    s = np.array([1.0, 0, 0, 0.5, 0, 0, 0, 2.0])
+    amg = O.AMG(sp.csr_matrix(W), s, max_coarse=3)
+    assert amg.sizes == [8, 2]
+    hand = np.array([0, 0, 0, 0, 1, 1, 1, 1])
+    assert list(amg.levels[0]["agg"]) == list(hand)
+    M = amg(np.eye(8))
+    np.testing.assert_allclose(M, _two_level_dense(W, s, hand, 0.7), rtol=1e-12, atol=1e-14)
+    # the operator really depends on omega and on the aggregation: neither check is vacuous
+    assert np.abs(M - _two_level_dense(W, s, hand, 0.5)).max() > 1e-3
+    assert np.abs(M - _two_level_dense(W, s, np.array([0, 0, 1, 1, 1, 1, 0, 0]), 0.7)).max() > 1e-3
+
+
+def test_amg_weighted_two_level_random_small():
+    """Same dense formula on random weighted graphs of 7-8 vertices whose aggregation is
+    worked out independently by a brute-force restatement of the matching rules."""
+    rs = np.random.RandomState(3)
+    for trial in range(20):
+        n = 7 + trial % 2
+        W = np.triu(rs.uniform(0.5, 1.5, (n, n)) * (rs.rand(n, n) < 0.5), 1)
+        W = W + W.T
+        s = rs.uniform(0.0, 0.3, n)
+        s[0] += 1.0
+        agg = _brute_aggregate(W, passes=2, rounds=3)
+        nc = int(agg.max()) + 1
+        amg = O.AMG(sp.csr_matrix(W), s, max_coarse=3)
+        if nc > 0.85 * n or nc > 3:
+            continue
+        assert list(amg.levels[0]["agg"]) == list(agg)
+        np.testing.assert_allclose(amg(np.eye(n)), _two_level_dense(W, s, agg, 0.7),
+                                   rtol=1e-10, atol=1e-13)
+
+
+def _brute_handshake(W, rounds):
+    """Matching rules of amg.py:32-64 in plain loops: each unmatched vertex proposes to its
+    heaviest unmatched neighbour (smallest id on ties); mutual proposals match (aggregate id =
+    the smaller vertex); leftovers join their heaviest matched neighbour; renumber by sorted
+    aggregate id."""
+    n = W.shape[0]
+    agg = [-1] * n
+    matched = [False] * n
+    for _ in range(rounds):
+        prop = [None] * n
+        for u in range(n):
+            if matched[u]:
+                continue
+            best = None
+            for v in range(n):
+                if v != u and W[u, v] != 0 and not matched[v]:
+                    if best is None or W[u, v] > W[u, best]:
+                        best = v
+            prop[u] = best
+        if all(p is None for p in prop):
+            break
+        for u in range(n):
+            v = prop[u]
+            if v is not None and prop[v] == u:
+                matched[u] = True
+                agg[u] = min(u, v)
+    final = list(agg)
+    for u in range(n):
+        if matched[u]:
+            continue
+        best = None
+        for v in range(n):
+            if v != u and W[u, v] != 0 and matched[v]:
+                if best is None or W[u, v] > W[u, best]:
+                    best = v
+        if best is not None:
+            final[u] = agg[best]
+    final = [u if a < 0 else a for u, a in enumerate(final)]
+    ids = sorted(set(final))
+    return np.array([ids.index(a) for a in final])
+
+
+def _brute_aggregate(W, passes, rounds):
+    n = W.shape[0]
+    agg = np.arange(n)
+    Wp = W
+    for _ in range(passes):
+        a2 = _brute_handshake(Wp, rounds)
+        agg = a2[agg]
+        nc = int(a2.max()) + 1
+        P = np.zeros((Wp.shape[0], nc))
+        P[np.arange(Wp.shape[0]), a2] = 1.0
+        Wp = P.T @ Wp @ P
+        np.fill_diagonal(Wp, 0.0)
+    return agg
+
+
+def test_amg_no_coarsening_is_exact_inverse():
+    """With no edges every vertex stays its own aggregate, nc = n > 0.85 n, so no level is
+    built (amg.py:95-96) and the coarse level is the whole system, inverted exactly: M = A^-1
+    = diag(1/s)."""
+    s = np.array([1.0, 2.0, 4.0, 0.5, 3.0])
+    amg = O.AMG(sp.csr_matrix((5, 5)), s, max_coarse=2)
+    assert amg.levels == [] and amg.sizes == [5]
+    np.testing.assert_allclose(amg(np.eye(5)), np.diag(1.0 / s), rtol=1e-14)
+
+
 def test_edge_batch_apply_semantics():
     es = {(0, 1): 1.0, (1, 2): 2.0}
     O.apply_edge_batch(es, [(3, 0, 0.5), (2, 1, 7.0), (4, 4, 1.0)], [(1, 0), (5, 6)])
