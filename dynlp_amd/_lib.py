@@ -59,6 +59,9 @@ _SIGS = {
     "dlp_lp_status": (ct.c_int, [P, P, P, P]),
     "dlp_lp_pcg": (ct.c_int, [P, I32, I32, I32, I32, P, P, P]),
     "dlp_lp_pcg_stats": (ct.c_int, [P, P]),
+    "dlp_lp_reserve": (ct.c_int, [P, I32, I32, P]),
+    "dlp_lp_device_bytes": (ct.c_int, [P, P]),
+    "dlp_lp_handle_bytes": (ct.c_int64, [I32, I32, I64, I32, I32, I32, I32, I32]),
     "dlp_lp_bpcg": (ct.c_int, [P, I32, I32, I32, I32, P, P, P]),
     "dlp_gram": (ct.c_int, [I32, I32, P, I64, P, I64, P, P, I32, P]),
     "dlp_chol_solve": (ct.c_int, [I32, P, P, P, DBL, P]),

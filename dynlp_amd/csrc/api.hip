@@ -17,4 +17,4 @@ void set_error(const char* fmt, ...) {
 
 extern "C" const char* dlp_last_error(void) { return dlp::g_err; }
 
-extern "C" int dlp_abi_version(void) { return 1; }
+extern "C" int dlp_abi_version(void) { return 2; }

@@ -44,6 +44,12 @@ void set_error(const char* fmt, ...);
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
 
+// A non-negative ratio with NaN mapped to +inf, so that max reductions (fmax drops NaN) and
+// threshold tests (NaN compares false) treat a row that is not a number as violating.
+__device__ __forceinline__ double nan_to_inf(double x) {
+  return x <= 1.79769313486231570e308 ? x : __builtin_huge_val();
+}
+
 inline int ceil_log2(int x) {
   int s = 0;
   while ((1 << s) < x) ++s;

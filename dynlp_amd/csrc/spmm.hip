@@ -495,7 +495,7 @@ __global__ __launch_bounds__(256) void k_rho_cols(int nrows, const int32_t* __re
       const int r = rows ? rows[i] : (int)i;
       const double inv = 1.0 / diag[r];
       for (int c = c0; c < C; c += cstep) {
-        const double v = fabs((double)R[(int64_t)r * ldr + c]) * inv;
+        const double v = nan_to_inf(fabs((double)R[(int64_t)r * ldr + c]) * inv);
         if (c == c0) local = fmax(local, v);
         else atomic_max_nonneg(&sm[c], v);
       }
@@ -523,7 +523,7 @@ __global__ __launch_bounds__(256) void k_violating(int nrows, const int32_t* __r
   bool viol = false;
   if (valid) {
     const double d = diag[r];
-    for (int c = l; c < C; c += LC) viol |= fabs((double)R[(int64_t)r * ldr + c]) > eps[c] * d;
+    for (int c = l; c < C; c += LC) viol |= !(fabs((double)R[(int64_t)r * ldr + c]) <= eps[c] * d);
   }
   const unsigned long long bal = __ballot(viol);
   const int lane = threadIdx.x & 63;
@@ -678,7 +678,7 @@ __global__ __launch_bounds__(256) void k_rho_cols4(int n, int Q4, int C, const f
       const float4 v = R[i];
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        if (j < Qd.cnt) m[j] = fmax(m[j], fabs((double)quad_comp(v, j)) * inv);
+        if (j < Qd.cnt) m[j] = fmax(m[j], nan_to_inf(fabs((double)quad_comp(v, j)) * inv));
     }
   }
 #pragma unroll

@@ -25,26 +25,42 @@ __device__ __forceinline__ void stamp_append(int v, int32_t* __restrict__ stamp,
   if (atomicExch(stamp + v, stampv) != stampv) list[atomicAdd(count, 1)] = v;
 }
 
+__device__ __forceinline__ bool id_ok(int v, int n) { return (unsigned)v < (unsigned)n; }
+// An insertion weight must be finite and positive (NaN fails the first test).
+__device__ __forceinline__ bool weight_ok(float x) { return x > 0.f && x <= 3.402823466e38f; }
+
+// |res| / diag with NaN mapped to +inf: a row whose residual or diagonal is not a number
+// fails every threshold test and reaches the column maxima, so it is never certified.
+__device__ __forceinline__ double ratio_or_inf(double res, double dg) { return nan_to_inf(fabs(res) / dg); }
+
 // Directed deletion e < 2*nd: row a loses entry b.  Rows are stamped into the changed list;
-// the labeled-neighbour counter of a drops when b is labeled.
-__global__ void k_apply_delete(int nd, const int32_t* __restrict__ du, const int32_t* __restrict__ dv,
+// the labeled-neighbour counter of a drops when b is labeled.  A pair with an id outside
+// [0, n) is skipped and counted once (invalid); the tombstone is a compare-and-swap, so a pair
+// deleted twice in one call is removed once (the second counts as missing).
+__global__ void k_apply_delete(int n, int nd, const int32_t* __restrict__ du, const int32_t* __restrict__ dv,
                                const int32_t* __restrict__ indptr, const int32_t* __restrict__ rowlen,
                                int32_t* __restrict__ col, float* __restrict__ w,
                                const uint8_t* __restrict__ labeled, int32_t* __restrict__ labcnt,
                                int32_t* __restrict__ stamp, int stampv, int32_t* __restrict__ rows,
-                               int32_t* __restrict__ nrows, int32_t* __restrict__ missing) {
+                               int32_t* __restrict__ nrows, int32_t* __restrict__ missing,
+                               int32_t* __restrict__ invalid) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= 2 * (int64_t)nd) return;
   const int a = e < nd ? du[e] : dv[e - nd];
   const int b = e < nd ? dv[e] : du[e - nd];
+  if (!id_ok(a, n) || !id_ok(b, n)) {
+    if (e < nd) atomicAdd(invalid, 1);
+    return;
+  }
   if (a == b) return;
   const int beg = indptr[a], end = beg + rowlen[a];
   bool found = false;
   for (int j = beg; j < end; ++j) {
     if (col[j] == b) {
-      col[j] = -1;
-      w[j] = 0.f;
-      found = true;
+      if (atomicCAS(col + j, b, -1) == b) {
+        w[j] = 0.f;
+        found = true;
+      }
       break;
     }
   }
@@ -82,26 +98,69 @@ __global__ void k_compact_rows(const int32_t* __restrict__ nrows, const int32_t*
   rowlen[r] = o - beg;
 }
 
-// Directed insertion e < 2*ni: upsert (a, b, wt).  A full row records e as overflow.
-// One wavefront per directed insertion (2*ni of them): the lanes read the row's live entries
-// in parallel; an existing pair takes the new weight, else one lane claims a slot (pairs are
-// distinct within a batch, so concurrent insertions into one row never look for each other).
-__global__ __launch_bounds__(256) void k_apply_insert(int ni, const int32_t* __restrict__ iu,
-                                                      const int32_t* __restrict__ iv, const float* __restrict__ iw,
-                                                      const int32_t* __restrict__ indptr,
-                                                      int32_t* __restrict__ rowlen, int32_t* __restrict__ col,
-                                                      float* __restrict__ w, const uint8_t* __restrict__ labeled,
-                                                      int32_t* __restrict__ labcnt, int32_t* __restrict__ stamp,
-                                                      int stampv, int32_t* __restrict__ rows,
-                                                      int32_t* __restrict__ nrows, int32_t* __restrict__ overflow,
-                                                      int32_t* __restrict__ noverflow) {
-  const int64_t e = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (e >= 2 * (int64_t)ni) return;   // wave-uniform
-  const int a = e < ni ? iu[e] : iv[e - ni];
-  const int b = e < ni ? iv[e] : iu[e - ni];
-  const float wt = e < ni ? iw[e] : iw[e - ni];
-  if (a == b) return;
+// Insertions of one call are reduced to the last occurrence of each unordered pair before
+// any row changes, so both directions of a pair come from the same occurrence (the stored
+// matrix stays symmetric) and no two wavefronts insert the same entry: an open-addressing
+// table keyed by min*n+max holds the largest occurrence index of every pair (its capacity is
+// a power of two at least twice the insertions, so every probe sequence meets an empty slot).
+constexpr unsigned long long kHashEmpty = ~0ull;
+constexpr int kPairTableCap = 1 << 20;   // 2 x the 2^19 insertions one call may hold
+
+__device__ __forceinline__ unsigned pair_slot(unsigned long long key, int bits) {
+  return (unsigned)((key * 0x9E3779B97F4A7C15ull) >> (64 - bits));
+}
+
+__device__ __forceinline__ bool insertion_ok(int a, int b, float wt, int n) {
+  return id_ok(a, n) && id_ok(b, n) && weight_ok(wt);
+}
+
+__global__ void k_pair_table_clear(int cap, unsigned long long* __restrict__ keys, int32_t* __restrict__ vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cap) {
+    keys[i] = kHashEmpty;
+    vals[i] = -1;
+  }
+}
+
+__global__ void k_pair_table_fill(int n, int ni, const int32_t* __restrict__ iu, const int32_t* __restrict__ iv,
+                                  const float* __restrict__ iw, int bits, unsigned long long* __restrict__ keys,
+                                  int32_t* __restrict__ vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ni) return;
+  const int a = iu[i], b = iv[i];
+  if (!insertion_ok(a, b, iw[i], n) || a == b) return;
+  const unsigned long long key = (unsigned long long)min(a, b) * (unsigned)n + (unsigned)max(a, b);
+  const unsigned mask = (1u << bits) - 1u;
+  for (unsigned h = pair_slot(key, bits);; h = (h + 1) & mask) {
+    const unsigned long long old = atomicCAS(keys + h, kHashEmpty, key);
+    if (old == kHashEmpty || old == key) {
+      atomicMax(vals + h, i);
+      return;
+    }
+  }
+}
+
+// Largest occurrence index of the pair (present: filled by k_pair_table_fill).
+__device__ __forceinline__ int pair_last(unsigned long long key, int bits, const unsigned long long* __restrict__ keys,
+                                         const int32_t* __restrict__ vals) {
+  const unsigned mask = (1u << bits) - 1u;
+  for (unsigned h = pair_slot(key, bits);; h = (h + 1) & mask) {
+    const unsigned long long k = keys[h];
+    if (k == key) return vals[h];
+    if (k == kHashEmpty) return -1;
+  }
+}
+
+// Upsert of the directed entry (a -> b, weight wt) by one whole wavefront: the lanes read the
+// row's live entries in parallel; an existing entry takes the new weight, else lane 0 claims a
+// slot (a full row records `tag` as overflow).  Callers never upsert one entry twice at once.
+__device__ __forceinline__ void upsert_entry(int a, int b, float wt, int lane, const int32_t* __restrict__ indptr,
+                                             int32_t* __restrict__ rowlen, int32_t* __restrict__ col,
+                                             float* __restrict__ w, const uint8_t* __restrict__ labeled,
+                                             int32_t* __restrict__ labcnt, int32_t* __restrict__ stamp,
+                                             int stampv, int32_t* __restrict__ rows, int32_t* __restrict__ nrows,
+                                             int32_t* __restrict__ overflow, int32_t* __restrict__ noverflow,
+                                             int32_t tag) {
   const int beg = indptr[a], cap = indptr[a + 1] - beg;
   // a concurrent insertion that overflows the row raises rowlen past cap for a moment
   const int len = min(cap, __hip_atomic_load(rowlen + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
@@ -128,8 +187,45 @@ __global__ __launch_bounds__(256) void k_apply_insert(int ni, const int32_t* __r
     stamp_append(a, stamp, stampv, rows, nrows);
   } else {
     atomicSub(rowlen + a, 1);
-    overflow[atomicAdd(noverflow, 1)] = (int32_t)e;
+    overflow[atomicAdd(noverflow, 1)] = tag;
   }
+}
+
+// Directed insertion e < 2*ni: upsert (a, b, wt), one wavefront each (2*ni of them), when this
+// occurrence is the pair's last in the call (earlier ones are counted in *ndup).  A pair with
+// an id outside [0, n) or a weight that is not finite and positive is skipped and counted once
+// (invalid).
+__global__ __launch_bounds__(256) void k_apply_insert(int n, int ni, const int32_t* __restrict__ iu,
+                                                      const int32_t* __restrict__ iv, const float* __restrict__ iw,
+                                                      int bits, const unsigned long long* __restrict__ keys,
+                                                      const int32_t* __restrict__ vals,
+                                                      const int32_t* __restrict__ indptr,
+                                                      int32_t* __restrict__ rowlen, int32_t* __restrict__ col,
+                                                      float* __restrict__ w, const uint8_t* __restrict__ labeled,
+                                                      int32_t* __restrict__ labcnt, int32_t* __restrict__ stamp,
+                                                      int stampv, int32_t* __restrict__ rows,
+                                                      int32_t* __restrict__ nrows, int32_t* __restrict__ overflow,
+                                                      int32_t* __restrict__ noverflow, int32_t* __restrict__ invalid,
+                                                      int32_t* __restrict__ ndup) {
+  const int64_t e = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (e >= 2 * (int64_t)ni) return;   // wave-uniform
+  const int i = (int)(e < ni ? e : e - ni);
+  const int a = e < ni ? iu[i] : iv[i];
+  const int b = e < ni ? iv[i] : iu[i];
+  const float wt = iw[i];
+  if (!insertion_ok(a, b, wt, n)) {
+    if (e < ni && lane == 0) atomicAdd(invalid, 1);
+    return;
+  }
+  if (a == b) return;
+  const unsigned long long key = (unsigned long long)min(a, b) * (unsigned)n + (unsigned)max(a, b);
+  if (pair_last(key, bits, keys, vals) != i) {
+    if (e < ni && lane == 0) atomicAdd(ndup, 1);
+    return;
+  }
+  upsert_entry(a, b, wt, lane, indptr, rowlen, col, w, labeled, labcnt, stamp, stampv, rows, nrows, overflow,
+               noverflow, (int32_t)e);
 }
 
 // Regrow: copy every row into a store with new offsets (rows keep their live prefix).
@@ -207,7 +303,7 @@ __global__ void k_ground_toggle(int n, const int32_t* __restrict__ roots, const 
 constexpr int kBallCap = 256;      // vertices a check may visit
 constexpr int kBallHash = 1024;    // open-addressing slots (power of two)
 
-__global__ __launch_bounds__(256) void k_ground_ball(int nd, const int32_t* __restrict__ du, const int32_t* __restrict__ dv,
+__global__ __launch_bounds__(256) void k_ground_ball(int n, int nd, const int32_t* __restrict__ du, const int32_t* __restrict__ dv,
                                                      int ni, const int32_t* __restrict__ indptr,
                                                      const int32_t* __restrict__ rowlen, const int32_t* __restrict__ col,
                                                      const uint8_t* __restrict__ labeled,
@@ -223,6 +319,7 @@ __global__ __launch_bounds__(256) void k_ground_ball(int nd, const int32_t* __re
   const int64_t e = (int64_t)blockIdx.x * 4 + wave;
   if (e >= 2 * (int64_t)nd) return;
   const int x = e < nd ? du[e] : dv[e - nd];
+  if (!id_ok(x, n) || !id_ok(e < nd ? dv[e] : du[e - nd], n)) return;   // refused by the apply
   if (labeled[x]) return;
   if (labcnt[x] > 0) return;              // grounded directly
   for (int t = lane; t < kBallHash; t += 64) hset[wave][t] = -1;
@@ -458,7 +555,7 @@ __device__ __forceinline__ void refresh_row(const RefreshArgs& A, const Lanes& L
           const double rh = c == C ? dg : rhs[k] + A.eta * A.prior;
           const double res = rh - dg * (double)Z[zo + c] + acc[k];
           R[zo + c] = (TR)res;
-          const double ratio = fabs(res) / dg;
+          const double ratio = ratio_or_inf(res, dg);
           viol |= ratio > A.eps[c];
           rmax[k] = fmax(rmax[k], ratio);
           m = fmax(m, ratio / A.eps[c]);
@@ -603,7 +700,7 @@ __device__ __forceinline__ void refresh_row_v4(const RefreshArgs& A, const Lanes
           const double rh = c == C ? dg : rhs[q] + A.eta * A.prior;
           const double res = rh - dg * (double)zv[q] + acc[q];
           R[zo + c] = (float)res;
-          const double ratio = fabs(res) / dg;
+          const double ratio = ratio_or_inf(res, dg);
           viol |= ratio > A.eps[c];
           rmax[q] = fmax(rmax[q], ratio);
           m = fmax(m, ratio / A.eps[c]);
@@ -689,7 +786,7 @@ struct VerifyEpilogue {
           const double rh = c == C ? dg : A.eta * A.prior;
           const double res = rh - dg * (double)zv[q] + acc[q];
           R[(int64_t)r * A.ld + c] = (float)res;
-          const double ratio = fabs(res) / dg;
+          const double ratio = ratio_or_inf(res, dg);
           viol |= ratio > A.eps[c];
           rmax[q] = fmax(rmax[q], ratio);
           if (c == C) hm = fmaxf(hm, zv[q]);
@@ -1079,11 +1176,18 @@ struct LP {
                                 // [4] giant component root (CC scratch)
   hipGraphExec_t bpcg_exec = nullptr;
   int bpcg_graph_iters = 0;
+  unsigned long long* pair_keys = nullptr;  // insertion pair table (kPairTableCap slots)
+  int32_t* pair_vals = nullptr;
+  int64_t dev_bytes = 0;        // device memory this handle holds (its own buffers)
   double push_ms = 0.0;         // summed duration of push-round launches (events)
   int64_t push_launched = 0;    // rounds launched
 };
 
 enum { CNT_ROWS = 0, CNT_MISSING, CNT_OVERFLOW, CNT_TOGGLED, CNT_VIOL, CNT_CAND64, CNT_N };
+// counters past the status block's first words: pairs refused by the apply (ids outside
+// [0, n), insertion weights not finite and positive), insertion occurrences superseded by a
+// later one of the same pair; CNT_ALL counters are cleared per batch
+enum { CNT_INVALID = CNT_N, CNT_DUP, CNT_ALL };
 // status words: [0, 32) counters, [32, 32 + 256) rho as FP64 bits
 constexpr int kStatRho = 32;
 constexpr int kStatWords = kStatRho + 256;
@@ -1101,6 +1205,8 @@ __global__ void k_status_gather(const int32_t* __restrict__ cnt, const int32_t* 
     out[CNT_N + 1] = rnd[0];
     out[CNT_N + 2] = (int64_t)*edges;
     out[CNT_N + 9] = __double_as_longlong(*mass);
+    out[CNT_N + 10] = cnt[CNT_INVALID];
+    out[CNT_N + 11] = cnt[CNT_DUP];
   }
   if (t < 3) out[CNT_N + 3 + t] = (int64_t)stats[t];
   if (t == 3) out[CNT_N + 6] = rho ? __double_as_longlong(rho[C1]) : 0;   // the verification's h maximum
@@ -1118,10 +1224,42 @@ __global__ void k_push_probe(const int32_t* __restrict__ done, const unsigned lo
 }
 
 template <typename T>
-int dev_alloc(T** p, size_t count) {
-  DLP_CHECK_HIP(hipMalloc((void**)p, sizeof(T) * (count ? count : 1)));
+int dev_alloc(T** p, size_t count, int64_t* acc) {
+  const size_t bytes = sizeof(T) * (count ? count : 1);
+  DLP_CHECK_HIP(hipMalloc((void**)p, bytes));
+  *acc += (int64_t)bytes;
   return DLP_OK;
 }
+
+// A set of device buffers that is allocated whole or not at all: commit() allocates every
+// request, and on a failure frees what it took and leaves every destination pointer untouched,
+// so a handle never holds part of a group (a later call simply tries again).
+struct AllocGroup {
+  void** dst[16];
+  size_t bytes[16];
+  int k = 0;
+  void add(void** d, size_t b) {
+    dst[k] = d;
+    bytes[k] = b ? b : 1;
+    ++k;
+  }
+  int commit(int64_t* acc, const char* what) {
+    void* got[16];
+    for (int i = 0; i < k; ++i) {
+      if (hipMalloc(&got[i], bytes[i]) != hipSuccess) {
+        (void)hipGetLastError();
+        for (int j = 0; j < i; ++j) (void)hipFree(got[j]);
+        set_error("%s: device allocation of %zu bytes failed", what, bytes[i]);
+        return DLP_ERR_HIP;
+      }
+    }
+    for (int i = 0; i < k; ++i) {
+      *dst[i] = got[i];
+      *acc += (int64_t)bytes[i];
+    }
+    return DLP_OK;
+  }
+};
 
 int lp_free(LP* h) {
   void* ptrs[] = {h->stamp, h->rows, h->cnt, h->overflow, h->toggled, h->parent, h->roots, h->grounded,
@@ -1154,6 +1292,8 @@ int lp_free(LP* h) {
   if (h->pcg_exec) (void)hipGraphExecDestroy(h->pcg_exec);
   if (h->bk) (void)hipFree(h->bk);
   if (h->gflags) (void)hipFree(h->gflags);
+  if (h->pair_keys) (void)hipFree(h->pair_keys);
+  if (h->pair_vals) (void)hipFree(h->pair_vals);
   if (h->mass) (void)hipFree(h->mass);
   if (h->bpcg_exec) (void)hipGraphExecDestroy(h->bpcg_exec);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -1216,8 +1356,8 @@ extern "C" int dlp_lp_create(const dlp_lp_desc* d, void** handle) {
   const int n = d->n;
   const size_t nz = (size_t)n * d->ld;
   int rc = DLP_OK;
-#define A_(p, c) if (!rc) rc = dev_alloc(&(p), (c))
-  A_(h->stamp, n); A_(h->rows, n); A_(h->cnt, CNT_N); A_(h->overflow, 1 << 20);
+#define A_(p, c) if (!rc) rc = dev_alloc(&(p), (c), &h->dev_bytes)
+  A_(h->stamp, n); A_(h->rows, n); A_(h->cnt, CNT_ALL); A_(h->overflow, 1 << 20);
   A_(h->toggled, n); A_(h->parent, n); A_(h->roots, n); A_(h->grounded, n); A_(h->viol, n);
   A_(h->eps64, d->C1); A_(h->rho, d->C1 + 1); A_(h->mark, n); A_(h->cand64, n);
   A_(h->S.ctag, n); A_(h->S.list[0], n); A_(h->S.list[1], n); A_(h->S.list[2], n);
@@ -1226,14 +1366,15 @@ extern "C" int dlp_lp_create(const dlp_lp_desc* d, void** handle) {
   int32_t* bst = nullptr;
   float* eps32 = nullptr;
   A_(bst, 1); A_(eps32, d->C1); A_(h->gflags, 8); A_(h->mass, 1);
+  A_(h->pair_keys, kPairTableCap); A_(h->pair_vals, kPairTableCap);
 #undef A_
   h->S.bstamp = bst;
   h->S.eps = eps32;
-  if (!rc && d->Z64) rc = dev_alloc(&h->dsnap, nz);
+  if (!rc && d->Z64) rc = dev_alloc(&h->dsnap, nz, &h->dev_bytes);
   h->scan_bytes = dlp_scan_workspace_bytes(n);
-  if (!rc && hipMalloc(&h->scan_ws, h->scan_bytes) != hipSuccess) rc = DLP_ERR_HIP;
+  if (!rc) rc = dev_alloc((uint8_t**)&h->scan_ws, h->scan_bytes, &h->dev_bytes);
   if (!rc && hipHostMalloc((void**)&h->host, kStatWords * sizeof(int64_t)) != hipSuccess) rc = DLP_ERR_HIP;
-  if (!rc) rc = dev_alloc(&h->stat_dev, kStatWords);
+  if (!rc) rc = dev_alloc(&h->stat_dev, kStatWords, &h->dev_bytes);
   if (rc) {
     set_error("lp_create: device allocation failed");
     lp_free(h);
@@ -1243,7 +1384,7 @@ extern "C" int dlp_lp_create(const dlp_lp_desc* d, void** handle) {
   (void)hipMemset(h->stamp, 0xff, sizeof(int32_t) * n);
   (void)hipMemset(h->S.ctag, 0xff, sizeof(int32_t) * n);
   (void)hipMemset(h->S.btag, 0xff, sizeof(int32_t) * n);
-  (void)hipMemset(h->cnt, 0, sizeof(int32_t) * CNT_N);
+  (void)hipMemset(h->cnt, 0, sizeof(int32_t) * CNT_ALL);
   (void)hipMemset(h->S.count, 0, sizeof(int32_t) * 3);
   (void)hipMemset(h->S.rnd, 0, sizeof(int32_t) * 2);
   (void)hipMemset(h->S.done, 0, sizeof(int32_t));
@@ -1327,7 +1468,7 @@ extern "C" int dlp_lp_set_eps(void* handle, const double* eps, double push_scale
 }
 
 __global__ void k_begin_batch(int32_t* cnt, int32_t* bstamp, int32_t* bcount, unsigned long long* edges) {
-  if (threadIdx.x < CNT_N) cnt[threadIdx.x] = 0;
+  if (threadIdx.x < CNT_ALL) cnt[threadIdx.x] = 0;
   if (threadIdx.x == 0) {
     *bstamp = *bstamp + 1;
     *bcount = 0;
@@ -1352,20 +1493,30 @@ extern "C" int dlp_lp_apply(void* handle, int nd, const int32_t* du, const int32
   DLP_CHECK_ARG(ni <= (1 << 19), "lp_apply: at most 524288 insertions per call");
   LP* h = (LP*)handle;
   hipStream_t st = (hipStream_t)stream;
+  const int n = h->d.n;
   if (nd > 0) {
-    hipLaunchKernelGGL(k_apply_delete, dim3(grid_for(2LL * nd)), dim3(kBlock), 0, st, nd, du, dv,
+    hipLaunchKernelGGL(k_apply_delete, dim3(grid_for(2LL * nd)), dim3(kBlock), 0, st, n, nd, du, dv,
                        h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.labeled, h->d.labcnt,
-                       h->stamp, h->stampv, h->rows, h->cnt + CNT_ROWS, h->cnt + CNT_MISSING);
+                       h->stamp, h->stampv, h->rows, h->cnt + CNT_ROWS, h->cnt + CNT_MISSING,
+                       h->cnt + CNT_INVALID);
     DLP_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_compact_rows, dim3(grid_for(2LL * nd)), dim3(kBlock), 0, st,
                        h->cnt + CNT_ROWS, h->rows, h->d.indptr, h->d.rowlen, h->d.col, h->d.w);
     DLP_CHECK_LAUNCH();
   }
   if (ni > 0) {
-    hipLaunchKernelGGL(k_apply_insert, dim3(grid_for(2LL * ni * 64)), dim3(kBlock), 0, st, ni, iu, iv, iw,
-                       h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.labeled, h->d.labcnt,
-                       h->stamp, h->stampv, h->rows, h->cnt + CNT_ROWS, h->overflow,
-                       h->cnt + CNT_OVERFLOW);
+    int bits = 1;
+    while ((1LL << bits) < 2LL * ni) ++bits;
+    const int cap = 1 << bits;   // <= kPairTableCap (ni <= 2^19)
+    hipLaunchKernelGGL(k_pair_table_clear, dim3(grid_for(cap)), dim3(kBlock), 0, st, cap, h->pair_keys,
+                       h->pair_vals);
+    hipLaunchKernelGGL(k_pair_table_fill, dim3(grid_for(ni)), dim3(kBlock), 0, st, n, ni, iu, iv, iw, bits,
+                       h->pair_keys, h->pair_vals);
+    hipLaunchKernelGGL(k_apply_insert, dim3(grid_for(2LL * ni * 64)), dim3(kBlock), 0, st, n, ni, iu, iv, iw, bits,
+                       (const unsigned long long*)h->pair_keys, (const int32_t*)h->pair_vals, h->d.indptr,
+                       h->d.rowlen, h->d.col, h->d.w, h->d.labeled, h->d.labcnt, h->stamp, h->stampv, h->rows,
+                       h->cnt + CNT_ROWS, h->overflow, h->cnt + CNT_OVERFLOW, h->cnt + CNT_INVALID,
+                       h->cnt + CNT_DUP);
     DLP_CHECK_LAUNCH();
   }
   return DLP_OK;
@@ -1384,29 +1535,19 @@ __global__ void k_retry_gather(const int32_t* __restrict__ ovf, const int32_t* _
   rw[i] = e < ni ? iw[e] : iw[e - ni];
 }
 
-// Directed re-insertion (each overflowed entry is one direction only).
-__global__ void k_apply_insert_directed(const int32_t* __restrict__ nent, const int32_t* __restrict__ ru,
-                                        const int32_t* __restrict__ rv, const float* __restrict__ rw,
-                                        const int32_t* __restrict__ indptr, int32_t* __restrict__ rowlen,
-                                        int32_t* __restrict__ col, float* __restrict__ w,
-                                        const uint8_t* __restrict__ labeled, int32_t* __restrict__ labcnt,
-                                        int32_t* __restrict__ stamp, int stampv, int32_t* __restrict__ rows,
-                                        int32_t* __restrict__ nrows, int32_t* __restrict__ overflow,
-                                        int32_t* __restrict__ noverflow) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= *nent) return;
-  const int a = ru[i], b = rv[i];
-  const int beg = indptr[a], cap = indptr[a + 1] - beg;
-  const int slot = atomicAdd(rowlen + a, 1);
-  if (slot < cap) {
-    w[beg + slot] = rw[i];
-    col[beg + slot] = b;
-    if (labeled[b]) atomicAdd(labcnt + a, 1);
-    stamp_append(a, stamp, stampv, rows, nrows);
-  } else {
-    atomicSub(rowlen + a, 1);
-    overflow[atomicAdd(noverflow, 1)] = i;
-  }
+// Directed re-insertion (each overflowed entry is one direction only), one wavefront each,
+// by the same upsert as the batch insertion (the entries are distinct pairs).
+__global__ __launch_bounds__(256) void k_apply_insert_directed(
+    const int32_t* __restrict__ nent, const int32_t* __restrict__ ru, const int32_t* __restrict__ rv,
+    const float* __restrict__ rw, const int32_t* __restrict__ indptr, int32_t* __restrict__ rowlen,
+    int32_t* __restrict__ col, float* __restrict__ w, const uint8_t* __restrict__ labeled,
+    int32_t* __restrict__ labcnt, int32_t* __restrict__ stamp, int stampv, int32_t* __restrict__ rows,
+    int32_t* __restrict__ nrows, int32_t* __restrict__ overflow, int32_t* __restrict__ noverflow) {
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (i >= *nent) return;   // wave-uniform
+  upsert_entry(ru[i], rv[i], rw[i], lane, indptr, rowlen, col, w, labeled, labcnt, stamp, stampv, rows, nrows,
+               overflow, noverflow, (int32_t)i);
 }
 
 extern "C" int dlp_lp_regrow_plan(void* handle, int extra, double growth, int32_t* new_indptr,
@@ -1452,8 +1593,8 @@ extern "C" int dlp_lp_regrow_commit(void* handle, const int32_t* new_indptr, int
                        iu, iv, iw, ru, rv, rw);
     DLP_CHECK_LAUNCH();
     DLP_CHECK_HIP(hipMemsetAsync(h->cnt + CNT_OVERFLOW, 0, sizeof(int32_t), st));
-    hipLaunchKernelGGL(k_apply_insert_directed, dim3(grid_for(1 << 20)), dim3(kBlock), 0, st, cnt2, ru, rv,
-                       rw, new_indptr, h->d.rowlen, new_col, new_w, h->d.labeled, h->d.labcnt, h->stamp,
+    hipLaunchKernelGGL(k_apply_insert_directed, dim3(grid_for((1LL << 20) * 64)), dim3(kBlock), 0, st, cnt2, ru,
+                       rv, rw, new_indptr, h->d.rowlen, new_col, new_w, h->d.labeled, h->d.labcnt, h->stamp,
                        h->stampv, h->rows, h->cnt + CNT_ROWS, h->overflow, h->cnt + CNT_OVERFLOW);
     DLP_CHECK_LAUNCH();
   }
@@ -1475,11 +1616,11 @@ static int ground_impl(LP* h, int force, int nd, const int32_t* du, const int32_
   int32_t* need = h->gflags;
   hipLaunchKernelGGL(k_ground_prep, dim3(1), dim3(64), 0, st, h->gflags, force, h->cnt + CNT_TOGGLED);
   if (!force && nd > 0) {
-    hipLaunchKernelGGL(k_ground_ball, dim3((unsigned)((2LL * nd + 3) / 4)), dim3(kBlock), 0, st, nd, du, dv, ni,
+    hipLaunchKernelGGL(k_ground_ball, dim3((unsigned)((2LL * nd + 3) / 4)), dim3(kBlock), 0, st, n, nd, du, dv, ni,
                        h->d.indptr, h->d.rowlen, h->d.col, h->d.labeled, h->d.labcnt, h->gflags);
   } else if (!force) {
     // insertions only: they matter when inert vertices exist
-    hipLaunchKernelGGL(k_ground_ball, dim3(1), dim3(kBlock), 0, st, 0, du, dv, ni, h->d.indptr, h->d.rowlen,
+    hipLaunchKernelGGL(k_ground_ball, dim3(1), dim3(kBlock), 0, st, n, 0, du, dv, ni, h->d.indptr, h->d.rowlen,
                        h->d.col, h->d.labeled, h->d.labcnt, h->gflags);
   }
   DLP_CHECK_LAUNCH();
@@ -1722,7 +1863,7 @@ __global__ void k_filter64(const int32_t* __restrict__ idx, const int32_t* __res
   mark[v] = 0;
   const double d = diag[v];
   bool viol = false;
-  for (int c = 0; c < C1; ++c) viol |= fabs(R[(int64_t)v * ld + c]) > eps[c] * d;
+  for (int c = 0; c < C1; ++c) viol |= !(fabs(R[(int64_t)v * ld + c]) <= eps[c] * d);
   if (viol) out[atomicAdd(nout, 1)] = v;
 }
 
@@ -1756,12 +1897,14 @@ extern "C" int dlp_lp_status(void* handle, int64_t* out, double* rho, void* stre
   hipLaunchKernelGGL(k_status_gather, dim3(1), dim3(kBlock), 0, st, h->cnt, h->S.bcount, h->S.rnd, h->S.edges,
                      h->S.stats, h->mass, rho ? h->rho : nullptr, C1, h->stat_dev);
   DLP_CHECK_LAUNCH();
-  const size_t words = rho ? (size_t)kStatRho + C1 : (size_t)CNT_N + 10;
+  const size_t words = rho ? (size_t)kStatRho + C1 : (size_t)CNT_N + 12;
   DLP_CHECK_HIP(hipMemcpyAsync(h->host, h->stat_dev, words * sizeof(int64_t), hipMemcpyDeviceToHost, st));
   DLP_CHECK_HIP(hipStreamSynchronize(st));
   const volatile int64_t* hs = h->host;
   for (int i = 0; i < CNT_N + 7; ++i) out[i] = hs[i];
   out[CNT_N + 9] = hs[CNT_N + 9];
+  out[CNT_N + 10] = hs[CNT_N + 10];
+  out[CNT_N + 11] = hs[CNT_N + 11];
   if (rho) std::memcpy(rho, (const void*)(h->host + kStatRho), sizeof(double) * C1);
   out[CNT_N + 7] = h->push_launched;
   out[CNT_N + 8] = (int64_t)(h->push_ms * 1000.0);  // microseconds
@@ -1841,6 +1984,8 @@ static int pcg_iteration(LP* h, const PcgBufs& b, hipStream_t st) {
 // inside a community is dropped), all of them when under a quarter qualify (no such structure);
 // ranges hold at least the rows one workgroup of the vector passes covers.  A new
 // aggregate count drops the captured iterations, which hold it as a launch argument.
+static int coarse_alloc(LP* h, hipStream_t st);
+
 static int coarse_prepare(LP* h, hipStream_t st) {
   const int n = h->d.n;
   int chunk = 0;
@@ -1860,20 +2005,9 @@ static int coarse_prepare(LP* h, hipStream_t st) {
     h->cg_age = -1;
     return DLP_OK;
   }
-  if (!h->cg_Ac) {
-    const size_t ldc = (size_t)h->d.ld;
-    DLP_CHECK_HIP(hipMalloc((void**)&h->cg_Ac, sizeof(double) * kCoarseMax * kCoarseMax));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->cg_Ainv, sizeof(double) * kCoarseMax * kCoarseMax));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->cg_rc, sizeof(double) * kCoarseMax * ldc));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->cg_ec, sizeof(float) * kCoarseMax * ldc));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->cg_bounds, sizeof(int32_t) * (kCoarseMax + 1)));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->cg_aggmap, (size_t)n));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->cg_diff, sizeof(int32_t) * 2 * (size_t)n));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->cg_cand, sizeof(int32_t) * 4 * kCoarseMax));
-    h->cg_ws_bytes = scan_workspace_bytes(n);
-    DLP_CHECK_HIP(hipMalloc(&h->cg_ws, h->cg_ws_bytes));
-    DLP_CHECK_HIP(hipMemsetAsync(h->cg_rc, 0, sizeof(double) * kCoarseMax * ldc, st));
-    DLP_CHECK_HIP(hipMemsetAsync(h->cg_ec, 0, sizeof(float) * kCoarseMax * ldc, st));
+  {
+    const int rc = coarse_alloc(h, st);
+    if (rc) return rc;
   }
   if (h->cg_age >= 0 && h->cg_age < h->coarse_refresh) {
     h->cg_age += 1;
@@ -1974,6 +2108,109 @@ static int capture_pcg(LP* h, int iters) {
   return DLP_OK;
 }
 
+// Iterations per deferred solution update for captured groups of iters_per_launch (0: the
+// solution is updated by every iteration).
+static int pcg_group_for(const LP* h, int iters_per_launch) {
+  return (h->pcg_p16 && tuning().defer_z && (h->d.ld % 4) == 0 && iters_per_launch <= kMaxGroup)
+             ? iters_per_launch : 0;
+}
+
+// Buffers of the coarse correction (sized for kCoarseMax aggregates), whole or not at all.
+static int coarse_alloc(LP* h, hipStream_t st) {
+  if (h->cg_Ac) return DLP_OK;
+  const int n = h->d.n;
+  const size_t ldc = (size_t)h->d.ld;
+  const size_t ws = scan_workspace_bytes(n);
+  AllocGroup g;
+  g.add((void**)&h->cg_Ac, sizeof(double) * kCoarseMax * kCoarseMax);
+  g.add((void**)&h->cg_Ainv, sizeof(double) * kCoarseMax * kCoarseMax);
+  g.add((void**)&h->cg_rc, sizeof(double) * kCoarseMax * ldc);
+  g.add((void**)&h->cg_ec, sizeof(float) * kCoarseMax * ldc);
+  g.add((void**)&h->cg_bounds, sizeof(int32_t) * (kCoarseMax + 1));
+  g.add((void**)&h->cg_aggmap, (size_t)n);
+  g.add((void**)&h->cg_diff, sizeof(int32_t) * 2 * (size_t)n);
+  g.add((void**)&h->cg_cand, sizeof(int32_t) * 4 * kCoarseMax);
+  g.add(&h->cg_ws, ws);
+  int rc = g.commit(&h->dev_bytes, "coarse operator");
+  if (rc) return rc;
+  h->cg_ws_bytes = ws;
+  DLP_CHECK_HIP(hipMemsetAsync(h->cg_rc, 0, sizeof(double) * kCoarseMax * ldc, st));
+  DLP_CHECK_HIP(hipMemsetAsync(h->cg_ec, 0, sizeof(float) * kCoarseMax * ldc, st));
+  return DLP_OK;
+}
+
+// Every buffer a global solve with captured groups of iters_per_launch iterations uses (the
+// block PCG's k x k matrices when block), so that a caller can take them at admission; the
+// solves call it too and then allocate nothing new.  Each set of buffers is allocated whole or
+// not at all (a failed call leaves the handle as it was).
+static int pcg_reserve(LP* h, int iters_per_launch, bool block, hipStream_t st) {
+  const int C1 = h->d.C1;
+  const size_t nz = (size_t)h->d.n * h->d.ld;
+  if (!h->P) {
+    AllocGroup g;
+    g.add((void**)&h->P, nz * sizeof(float));
+    g.add((void**)&h->AP, nz * sizeof(float));
+    g.add((void**)&h->pcg_scal, 7 * C1 * sizeof(double));
+    g.add((void**)&h->pcg_viol, 4 * sizeof(float));   // viol, last viol, done, iterations
+    int rc = g.commit(&h->dev_bytes, "global solve");
+    if (rc) return rc;
+    DLP_CHECK_HIP(hipMemsetAsync(h->P, 0, nz * sizeof(float), st));
+    DLP_CHECK_HIP(hipMemsetAsync(h->AP, 0, nz * sizeof(float), st));
+  }
+  if (block) {   // FP32 directions, Jacobi preconditioner, and the k x k matrices
+    if (!h->bk) {
+      AllocGroup g;
+      g.add((void**)&h->bk, 5 * C1 * C1 * sizeof(double));
+      return g.commit(&h->dev_bytes, "block PCG");
+    }
+    return DLP_OK;
+  }
+  // deferred solution updates keep a group's directions: iters_per_launch + 1 slots
+  const int group = pcg_group_for(h, iters_per_launch);
+  const int slots = group > 0 ? group + 1 : 1;
+  if (h->pcg_p16 && (!h->P16 || h->p16_slots < slots)) {
+    const int64_t ldh = (C1 + 7) / 8 * 8;
+    const size_t nh = (size_t)h->d.n * ldh * slots;
+    uint16_t* fresh = nullptr;
+    AllocGroup g;
+    g.add((void**)&fresh, nh * sizeof(uint16_t));
+    int rc = g.commit(&h->dev_bytes, "global solve directions");
+    if (rc) return rc;
+    if (h->P16) {
+      (void)hipFree(h->P16);
+      h->dev_bytes -= (int64_t)((size_t)h->d.n * h->ldh * h->p16_slots * sizeof(uint16_t));
+    }
+    h->P16 = fresh;
+    h->ldh = ldh;
+    h->p16_slots = slots;
+    DLP_CHECK_HIP(hipMemsetAsync(h->P16, 0, nh * sizeof(uint16_t), st));
+    if (h->pcg_exec) {   // captured iterations address the old block
+      (void)hipGraphExecDestroy(h->pcg_exec);
+      h->pcg_exec = nullptr;
+    }
+  }
+  if (group > 0 && h->ahist_len < group * C1 + 1) {
+    const int len = (group * C1 + 2) & ~1;
+    float* fresh = nullptr;
+    AllocGroup g;
+    g.add((void**)&fresh, len * sizeof(float));
+    int rc = g.commit(&h->dev_bytes, "global solve step lengths");
+    if (rc) return rc;
+    if (h->pcg_ahist) {
+      (void)hipFree(h->pcg_ahist);
+      h->dev_bytes -= (int64_t)h->ahist_len * (int64_t)sizeof(float);
+      if (h->pcg_exec) {   // captured iterations address the old history
+        (void)hipGraphExecDestroy(h->pcg_exec);
+        h->pcg_exec = nullptr;
+      }
+    }
+    h->pcg_ahist = fresh;
+    h->ahist_len = len;
+  }
+  if (h->coarse_na > 0 && h->d.n >= 64) return coarse_alloc(h, st);
+  return DLP_OK;
+}
+
 // Runs PCG iterations (in captured groups of iters_per_launch when use_graph) until the FP32
 // recurrence residual meets the push thresholds (max |R|/(eps*diag) <= 1), the violation
 // stalls for `patience` launches, or max_iter.  *iters_out: iterations run; *viol_out: last
@@ -1983,46 +2220,15 @@ extern "C" int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int 
   DLP_CHECK_ARG(handle && max_iter >= 0 && iters_per_launch >= 1, "lp_pcg: bad argument");
   LP* h = (LP*)handle;
   hipStream_t st = (hipStream_t)stream;
-  const int C1 = h->d.C1;
-  const size_t nz = (size_t)h->d.n * h->d.ld;
-  if (!h->P) {
-    DLP_CHECK_HIP(hipMalloc((void**)&h->P, nz * sizeof(float)));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->AP, nz * sizeof(float)));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_scal, 7 * C1 * sizeof(double)));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_viol, 4 * sizeof(float)));   // viol, last viol, done, iterations
-    DLP_CHECK_HIP(hipMemsetAsync(h->P, 0, nz * sizeof(float), st));
-    DLP_CHECK_HIP(hipMemsetAsync(h->AP, 0, nz * sizeof(float), st));
-  }
-  // deferred solution updates keep a group's directions: iters_per_launch + 1 slots
-  const int group = (h->pcg_p16 && tuning().defer_z && (h->d.ld % 4) == 0 && iters_per_launch <= kMaxGroup)
-                        ? iters_per_launch : 0;
-  const int slots = group > 0 ? group + 1 : 1;
-  if (h->pcg_p16 && (!h->P16 || h->p16_slots < slots)) {
-    if (h->P16) (void)hipFree(h->P16);
-    h->P16 = nullptr;
-    h->ldh = (C1 + 7) / 8 * 8;
-    const size_t nh = (size_t)h->d.n * h->ldh * slots;
-    DLP_CHECK_HIP(hipMalloc((void**)&h->P16, nh * sizeof(uint16_t)));
-    DLP_CHECK_HIP(hipMemsetAsync(h->P16, 0, nh * sizeof(uint16_t), st));
-    h->p16_slots = slots;
-    if (h->pcg_exec) {   // captured iterations address the old block
-      (void)hipGraphExecDestroy(h->pcg_exec);
-      h->pcg_exec = nullptr;
-    }
-  }
-  if (group > 0 && h->ahist_len < group * C1 + 1) {
-    if (h->pcg_ahist) (void)hipFree(h->pcg_ahist);
-    h->pcg_ahist = nullptr;
-    const int len = (group * C1 + 2) & ~1;
-    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_ahist, len * sizeof(float)));
-    h->ahist_len = len;
-  }
+  int rc = pcg_reserve(h, iters_per_launch, false, st);
+  if (rc) return rc;
+  const int group = pcg_group_for(h, iters_per_launch);
   if (group != h->pcg_group && h->pcg_exec) {   // captured with the other update scheme
     (void)hipGraphExecDestroy(h->pcg_exec);
     h->pcg_exec = nullptr;
   }
   h->pcg_group = group;
-  int rc = coarse_prepare(h, st);
+  rc = coarse_prepare(h, st);
   if (rc) return rc;
   const PcgBufs b = pcg_bufs(h);
   rc = pcg_start(b, st);
@@ -2132,6 +2338,50 @@ extern "C" int dlp_lp_set_pcg_directions(void* handle, int kind) {
   }
   h->pcg_p16 = kind;
   return DLP_OK;
+}
+
+extern "C" int dlp_lp_reserve(void* handle, int iters_per_launch, int block, void* stream) {
+  DLP_CHECK_ARG(handle && iters_per_launch >= 1, "lp_reserve: bad argument");
+  LP* h = (LP*)handle;
+  if (block) DLP_CHECK_ARG(h->d.C1 <= 80, "lp_reserve: block PCG takes at most 80 columns (C1=%d)", h->d.C1);
+  hipStream_t st = (hipStream_t)stream;
+  const int rc = pcg_reserve(h, iters_per_launch, block != 0, st);
+  if (rc) return rc;
+  DLP_CHECK_HIP(hipStreamSynchronize(st));
+  return DLP_OK;
+}
+
+extern "C" int dlp_lp_device_bytes(void* handle, int64_t* out) {
+  DLP_CHECK_ARG(handle && out, "lp_device_bytes: null argument");
+  *out = ((LP*)handle)->dev_bytes;
+  return DLP_OK;
+}
+
+extern "C" int64_t dlp_lp_handle_bytes(int n, int C1, int64_t ld, int fp64, int dirs, int iters_per_launch,
+                                       int coarse, int block) {
+  if (n < 1 || C1 < 1 || ld < C1 || iters_per_launch < 1) return -1;
+  const int64_t N = n, nz = N * ld;
+  int64_t b = 0;
+  b += 13 * 4 * N + 2 * N;                                   // int32 lists and stamps, byte masks
+  b += 4 * (int64_t)(1 << 20);                               // overflow list
+  b += 4 * CNT_ALL + 8 * C1 + 8 * (C1 + 1);                  // counters, eps64, rho
+  b += 4 * 3 + 4 * 2 + 4 + 4 + 8 + 8 * 4 + 4 + 4 * C1 + 4 * 8 + 8;   // push state words
+  b += (8 + 4) * (int64_t)kPairTableCap;                     // insertion pair table
+  if (fp64) b += 8 * nz;                                     // FP64 increments
+  b += (int64_t)dlp_scan_workspace_bytes(n) + 8 * kStatWords;
+  if (fp64) return b;   // the plain FP64 path repairs by push only: no global-solve buffers
+  // global solve
+  b += 2 * 4 * nz + 7 * 8 * C1 + 16;
+  if (block) return b + 5 * 8 * (int64_t)C1 * C1;
+  const bool p16 = dirs != 0;
+  const int group =
+      (p16 && tuning().defer_z && (ld % 4) == 0 && iters_per_launch <= kMaxGroup) ? iters_per_launch : 0;
+  if (p16) b += 2 * N * ((C1 + 7) / 8 * 8) * (group > 0 ? group + 1 : 1);
+  if (group > 0) b += 4 * ((group * C1 + 2) & ~1);
+  if (coarse > 0 && n >= 64)
+    b += 2 * 8 * kCoarseMax * kCoarseMax + (8 + 4) * kCoarseMax * ld + 4 * (kCoarseMax + 1) + N + 8 * N +
+         16 * kCoarseMax + (int64_t)scan_workspace_bytes(n);
+  return b;
 }
 
 // Cumulative timing of the global solve: [0] iterations, [1] microseconds (HIP events).
@@ -2246,22 +2496,14 @@ extern "C" int dlp_lp_bpcg(void* handle, int max_iter, int iters_per_launch, int
   DLP_CHECK_ARG(h->d.C1 <= 80, "lp_bpcg: block PCG takes at most 80 columns (C1=%d)", h->d.C1);
   hipStream_t st = (hipStream_t)stream;
   const int C1 = h->d.C1;
-  const size_t nz = (size_t)h->d.n * h->d.ld;
-  if (!h->P) {
-    DLP_CHECK_HIP(hipMalloc((void**)&h->P, nz * sizeof(float)));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->AP, nz * sizeof(float)));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_scal, 7 * C1 * sizeof(double)));
-    DLP_CHECK_HIP(hipMalloc((void**)&h->pcg_viol, 4 * sizeof(float)));   // viol, last viol, done, iterations
-    DLP_CHECK_HIP(hipMemsetAsync(h->P, 0, nz * sizeof(float), st));
-    DLP_CHECK_HIP(hipMemsetAsync(h->AP, 0, nz * sizeof(float), st));
-  }
-  if (!h->bk) DLP_CHECK_HIP(hipMalloc((void**)&h->bk, 5 * C1 * C1 * sizeof(double)));
+  int rc = pcg_reserve(h, iters_per_launch, true, st);
+  if (rc) return rc;
   PcgBufs b = pcg_bufs(h);
   b.P16 = nullptr;                                 // block PCG keeps FP32 directions
   b.p16 = 0;
   b.slot_stride = 0;
   b.na = 0;                                        // and the Jacobi preconditioner
-  int rc = pcg_start(b, st);                       // P = R/diag
+  rc = pcg_start(b, st);                           // P = R/diag
   if (!rc) rc = dlp_gram(b.n, C1, b.R, b.ld, b.R, b.ld, b.diag, h->bk, DLP_F32, st);   // G = R^T D^-1 R
   if (rc) return rc;
   if (use_graph) {

@@ -232,11 +232,13 @@ def config_graph(cfg: int, scale: float = 1.0, seed: int = 0) -> Graph:
 class EdgeBatch:
     """One request: edge insertions (u, v, w) and deletions (u, v) of undirected pairs.
 
-    Vertex ids lie in [0, n) (checked when the batch is applied).  The unordered pairs are
-    distinct within the deletions and within the insertions (not checked: a repeated pair is
-    applied once per occurrence).  An insertion of a present edge replaces its weight, a
-    deletion of an absent edge is counted as missing, and self-loops are ignored, as in the
-    system built from the edge set (its off-diagonal part only)."""
+    Vertex ids lie in [0, n) and insertion weights are finite and positive (both checked when
+    the batch is applied).  Deletions apply before insertions; a deletion of an absent edge is
+    counted as missing, an insertion of a present edge replaces its weight, and self-loops are
+    ignored, as in the system built from the edge set (its off-diagonal part only).  An
+    unordered pair repeated within the deletions is deleted once; one repeated within the
+    insertions, in either orientation, is inserted once with the weight of its last occurrence
+    (`normalize_batch`)."""
     ins_u: np.ndarray
     ins_v: np.ndarray
     ins_w: np.ndarray
@@ -249,9 +251,9 @@ class EdgeBatch:
 
 
 def check_batch(b: EdgeBatch, n: int, max_insertions: int | None = None) -> None:
-    """Refuses a batch whose arrays differ in length or that holds more than max_insertions
-    insertions (ValueError), or whose vertex ids leave [0, n) (IndexError): the device apply
-    indexes rows by these ids."""
+    """Refuses a batch whose arrays differ in length, that holds more than max_insertions
+    insertions or an insertion weight that is not finite and positive (ValueError), or whose
+    vertex ids leave [0, n) (IndexError): the device apply indexes rows by these ids."""
     nd, ni = len(b.del_u), len(b.ins_u)
     if len(b.del_v) != nd or len(b.ins_v) != ni or len(b.ins_w) != ni:
         raise ValueError(f"batch arrays differ in length: del {nd}/{len(b.del_v)}, "
@@ -261,6 +263,46 @@ def check_batch(b: EdgeBatch, n: int, max_insertions: int | None = None) -> None
     for a in (b.del_u, b.del_v, b.ins_u, b.ins_v):
         if len(a) and (int(a.min()) < 0 or int(a.max()) >= n):
             raise IndexError(f"batch vertex ids must lie in [0, {n}): found {int(a.min())}..{int(a.max())}")
+    if ni:
+        w = np.asarray(b.ins_w, dtype=np.float64)
+        with np.errstate(over="ignore"):
+            w32 = w.astype(np.float32)
+        bad = ~(np.isfinite(w32) & (w32 > 0))
+        if bad.any():
+            i = int(np.flatnonzero(bad)[0])
+            raise ValueError(f"insertion weights must be finite and positive (as FP32): ins_w[{i}] = {w[i]!r}")
+
+
+def _last_unique(key):
+    """Position of the last occurrence of each distinct key (ordered by key)."""
+    rev = key[::-1]
+    _, first_in_rev = np.unique(rev, return_index=True)
+    return key.shape[0] - 1 - first_in_rev
+
+
+def normalize_batch(b: EdgeBatch, n: int, max_insertions: int | None = None) -> EdgeBatch:
+    """check_batch, then one entry per unordered pair: repeated deletions collapse to one,
+    repeated insertions (either orientation) to their last occurrence.  Returns b itself when
+    nothing repeats.  Self-loops pass through (the apply ignores them)."""
+    check_batch(b, n, max_insertions)
+    out = {}
+    for kind, (u, v) in (("del", (b.del_u, b.del_v)), ("ins", (b.ins_u, b.ins_v))):
+        if len(u) < 2:
+            continue
+        u64, v64 = np.asarray(u, dtype=np.int64), np.asarray(v, dtype=np.int64)
+        key = np.minimum(u64, v64) * n + np.maximum(u64, v64)
+        keep = _last_unique(key)
+        if keep.shape[0] != key.shape[0]:
+            out[kind] = np.sort(keep)
+    if not out:
+        return b
+    dk = out.get("del")
+    ik = out.get("ins")
+    return EdgeBatch(b.ins_u if ik is None else np.asarray(b.ins_u)[ik],
+                     b.ins_v if ik is None else np.asarray(b.ins_v)[ik],
+                     b.ins_w if ik is None else np.asarray(b.ins_w)[ik],
+                     b.del_u if dk is None else np.asarray(b.del_u)[dk],
+                     b.del_v if dk is None else np.asarray(b.del_v)[dk])
 
 
 class EdgeSet:

@@ -57,9 +57,9 @@ class Request:
 
 def fuse(batches):
     """One batch equivalent to applying `batches` in order: per undirected pair the last
-    operation wins (deletion, or insertion with its weight)."""
-    if len(batches) == 1:
-        return batches[0]
+    operation wins (deletion, or insertion with its weight; within a batch deletions come
+    first).  A single batch goes through the same pass, so a request changes its graph in the
+    same way whether it is served alone or fused with others."""
     keys, ops, ws = [], [], []
     n = max(int(max(b.ins_u.max(initial=0), b.ins_v.max(initial=0), b.del_u.max(initial=0),
                     b.del_v.max(initial=0))) for b in batches) + 1
@@ -92,18 +92,10 @@ def replay_chunks(b: EdgeBatch, chunk=_REPLAY_CHUNK):
     return out
 
 
-def footprint_bytes(g: Graph, fp64=False):
-    """Device bytes of one resident graph (store with slack, state, PCG buffers, scratch)."""
-    n, C1 = g.n, (1 if g.K == 2 else g.K) + 1
-    ld = (C1 + 3) // 4 * 4
-    entries = 2 * g.m
-    cap = int(entries * 1.25) + 4 * n
-    store = cap * 8 + n * (4 + 4 + 1 + 1 + 4 + 4)
-    state = n * ld * 4 * 4 + n * 12          # Z, R, P, AP (FP32) + diag
-    if fp64:
-        state += n * ld * 8 * 3
-    scratch = n * 4 * 14 + (1 << 22)
-    return store + state + scratch
+def footprint_bytes(g: Graph, **lp_kwargs):
+    """Device bytes of one resident graph with its global solve reserved (EdgeStreamLP.footprint:
+    store with slack, state, direction slots, coarse operator, handle scratch)."""
+    return EdgeStreamLP.footprint(g, **lp_kwargs)
 
 
 @dataclass
@@ -131,7 +123,7 @@ class Service:
         self.lp_kwargs = dict(lp_kwargs or {})
         self.graphs = {i: Slot(i, g) for i, g in enumerate(graphs)}
         for s in self.graphs.values():
-            s.bytes = footprint_bytes(s.graph)
+            s.bytes = footprint_bytes(s.graph, **self.lp_kwargs)
             if s.bytes > self.budget:
                 raise ValueError(f"graph {s.gid} needs {s.bytes / 1e9:.2f} GB > budget {self.budget / 1e9:.2f} GB")
         self.resident = 0

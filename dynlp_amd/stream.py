@@ -23,7 +23,7 @@ import torch
 
 from . import _lib
 from .backend import device, ptr, stream
-from .graphs import EdgeBatch, Graph, check_batch
+from .graphs import EdgeBatch, Graph, normalize_batch
 from .solvers import System  # noqa: F401  (the batch system, reference stream.py:17-58)
 
 
@@ -142,12 +142,48 @@ class EdgeStreamLP:
         _lib.call("dlp_lp_set_pcg_directions", self.h, 0 if self.fp64 else kinds[pcg_directions])
         self.pcg_coarse = int(pcg_coarse)
         _lib.call("dlp_lp_set_coarse", self.h, self.pcg_coarse, int(coarse_refresh))
-        self._status = np.zeros(16, dtype=np.int64)
+        if not self.fp64:
+            # every buffer of the global solve now, so that no request allocates device memory
+            _lib.call("dlp_lp_reserve", self.h, int(pcg_iters_per_launch), int(pcg_mode == "block"), stream())
+        self._status = np.zeros(18, dtype=np.int64)
         self._rho = np.zeros(self.C1, dtype=np.float64)
         self.batches = 0
         self.scratch3 = None
         self._stage = self._stage_dev = None
         self.nnz = int(ix.shape[0])
+
+    @staticmethod
+    def footprint(g: Graph, fp64=False, slack_min=4, slack_frac=0.25, pcg_directions="fp16",
+                  pcg_coarse=128, pcg_iters_per_launch=7, pcg_mode="column", **_):
+        """Device bytes of a resident graph built with these settings, before its first
+        request: the arrays this object allocates (slack store, masks, labels, FP32 state, FP64
+        diagonal; the FP64 state when fp64) plus the library handle with its global solve
+        reserved (dlp_lp_handle_bytes, equal to what the handle then reports).  A batch of up
+        to 2^16 words stages without growing; a regrow after an overflow allocates a new store
+        beside the old one for the duration of the request."""
+        n = int(g.n)
+        C1 = (1 if g.K == 2 else g.K) + 1
+        ld = (C1 + 3) // 4 * 4
+        deg = np.bincount(g.u, minlength=n) + np.bincount(g.v, minlength=n)
+        cap = int((deg + np.maximum(slack_min, np.ceil(slack_frac * deg))).sum())
+        b = 4 * (n + 1) + 4 * n + 8 * cap                  # indptr, rowlen, col + w
+        b += n + 4 * n + n + 4 * n                         # labeled, y, solved, labcnt
+        b += 2 * 4 * n * ld + 8 * n + 4 * n                # Z32, R32, diag64, diag32
+        b += 4 * (1 << 16)                                 # batch staging
+        if fp64:
+            b += 2 * 8 * n * ld + 4 * n + 4 + 8 * C1       # Z64, R64, candidates, rho
+        kinds = {"fp32": 0, "bf16": 1, "fp16": 2}
+        hb = _lib.load().dlp_lp_handle_bytes(n, C1, ld, int(fp64), kinds[pcg_directions], int(pcg_iters_per_launch),
+                                              int(pcg_coarse), int(pcg_mode == "block"))
+        if hb < 0:
+            raise ValueError("dlp_lp_handle_bytes: bad sizes")
+        return int(b + hb)
+
+    def device_bytes(self):
+        """Device bytes the library handle holds (its own buffers)."""
+        out = ct.c_int64(0)
+        _lib.call("dlp_lp_device_bytes", self.h, ct.byref(out))
+        return int(out.value)
 
     def __del__(self):
         try:
@@ -169,7 +205,7 @@ class EdgeStreamLP:
                     touched=int(s[6]), round=int(s[7]), edges=int(s[8]), cand_total=int(s[9]),
                     push_total=int(s[10]), scatter_total=int(s[11]),
                     rounds_launched=int(s[13]), push_us=int(s[14]), mass=float(s[15:16].view(np.float64)[0]),
-                    hmax=float(s[12:13].view(np.float64)[0]))
+                    hmax=float(s[12:13].view(np.float64)[0]), invalid=int(s[16]), duplicates=int(s[17]))
 
     def _push(self, budget=0):
         r, ok = ct.c_int64(0), ct.c_int32(0)
@@ -268,8 +304,9 @@ class EdgeStreamLP:
 
     def _upload(self, b: EdgeBatch):
         """The batch's five arrays packed into one pinned staging buffer and sent with one copy
-        (the previous request has finished with both buffers: every request ends in a sync)."""
-        check_batch(b, self.n, max_insertions=1 << 19)   # the device apply's per-call limit
+        (the previous request has finished with both buffers: every request ends in a sync).
+        Returns the device views and the batch as applied (repeated pairs collapsed)."""
+        b = normalize_batch(b, self.n, max_insertions=1 << 19)   # the device apply's per-call limit
         nd, ni = len(b.del_u), len(b.ins_u)
         total = 2 * nd + 3 * ni
         if self._stage is None or self._stage.numel() < total:
@@ -285,7 +322,7 @@ class EdgeStreamLP:
         h[o + 2 * ni:total] = np.asarray(b.ins_w, dtype=np.float32).view(np.int32)
         d = self._stage_dev
         d[:total].copy_(self._stage[:total], non_blocking=True)
-        return (d[:nd], d[nd:2 * nd], d[o:o + ni], d[o + ni:o + 2 * ni], d[o + 2 * ni:total].view(torch.float32))
+        return (d[:nd], d[nd:2 * nd], d[o:o + ni], d[o + ni:o + 2 * ni], d[o + 2 * ni:total].view(torch.float32)), b
 
     def _regrow(self, ins, extra):
         iu, iv, iw = ins
@@ -342,11 +379,13 @@ class EdgeStreamLP:
 
     def apply_and_solve(self, b: EdgeBatch) -> Stats:
         """Apply one batch of edge updates and repair the solution to eps (certified in FP64).
-        Ragged arrays, more than 2^19 insertions (ValueError) and ids outside [0, n)
-        (IndexError) are refused before any device work."""
+        Ragged arrays, more than 2^19 insertions, insertion weights that are not finite and
+        positive (ValueError) and ids outside [0, n) (IndexError) are refused before any device
+        work; a pair repeated within the deletions or within the insertions counts once (the
+        last insertion's weight)."""
         st = Stats(path="fp64-push" if self.fp64 else "push")
         t0 = time.perf_counter()
-        du, dv, iu, iv, iw = self._upload(b)
+        (du, dv, iu, iv, iw), b = self._upload(b)
         self._batch_edges = int(du.shape[0]) + int(iu.shape[0])
         _lib.call("dlp_lp_begin_batch", self.h, stream())
         _lib.call("dlp_lp_apply", self.h, int(du.shape[0]), ptr(du), ptr(dv), int(iu.shape[0]), ptr(iu), ptr(iv),
@@ -359,6 +398,9 @@ class EdgeStreamLP:
         prec = 1 if self.fp64 else 0
         _lib.call("dlp_lp_refresh", self.h, 0, prec, stream())
         s = self._repair(st)
+        if s["invalid"] or s["duplicates"]:
+            raise RuntimeError(f"device apply refused {s['invalid']} pairs and merged {s['duplicates']} "
+                               "repeated entries that the host checks had passed")
         if s["overflow"]:
             # room for every insertion of this batch in any one row, so one regrow suffices
             per_row = np.bincount(np.concatenate([b.ins_u, b.ins_v]), minlength=1).max()

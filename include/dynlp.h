@@ -236,10 +236,14 @@ int dlp_lp_set_store(void* handle, const int32_t* indptr, int32_t* rowlen, int32
 int dlp_lp_set_eps(void* handle, const double* eps, double push_scale, void* stream);
 /* New batch: clears the changed-row list, counters and the batch-touched list. */
 int dlp_lp_begin_batch(void* handle, void* stream);
-/* Deletions (du, dv) then insertions (iu, iv, iw) of undirected pairs (distinct in a call;
- * device arrays; ids in [0, n), which the caller checks: the kernels index rows by them).
- * Self-loops are skipped; an insertion of a stored pair replaces its weight.
- * Both stored directions change; endpoints join the changed-row list.  Missing deletions and
+/* Deletions (du, dv) then insertions (iu, iv, iw) of undirected pairs (device arrays).
+ * Self-loops are skipped; an insertion of a stored pair replaces its weight.  Both stored
+ * directions change; endpoints join the changed-row list.  The kernels check every pair before
+ * indexing a row by it: a pair with an id outside [0, n), or an insertion whose weight is not
+ * finite and positive, is skipped and counted as invalid (dlp_lp_status out[16]).  A pair
+ * repeated within the deletions is deleted once (the repeats count as missing); of a pair
+ * repeated within the insertions, in either orientation, only the last occurrence is applied,
+ * to both directions (the earlier ones are counted, out[17]).  Missing deletions and
  * insertions that found a full row are counted (dlp_lp_status); overflowed insertions are
  * re-inserted by dlp_lp_regrow_commit.  Replaces IncrementalStream._toggle/_set_w
  * (stream.py:202-228) and the weight/row-sum/class-weight updates (stream.py:221-235). */
@@ -314,6 +318,24 @@ int dlp_gram(int n, int C, const void* X, int64_t ldx, const void* Y, int64_t ld
 /* S = A^{-1} B for SPD A (k x k FP64, k <= 128) by Cholesky in one workgroup, scaled to unit
  * diagonal; pivots below tol deflate their direction (row of S = 0). */
 int dlp_chol_solve(int k, const double* A, const double* B, double* S, double tol, void* stream);
+/* Takes now every device buffer the global solve needs for captured groups of
+ * iters_per_launch iterations (block != 0: the block PCG of dlp_lp_bpcg), with the current
+ * direction storage and coarse setting, so that a service can allocate at admission and a
+ * request allocates nothing; dlp_lp_pcg / dlp_lp_bpcg reserve what is missing themselves.
+ * Each set of buffers is allocated whole or not at all: after a failure (DLP_ERR_HIP) the
+ * handle holds what it held before and stays usable.  Synchronises the stream.  No
+ * reference counterpart (CuPy allocates inside solvers.pcg, solvers.py:66-97). */
+int dlp_lp_reserve(void* handle, int iters_per_launch, int block, void* stream);
+/* Device bytes the handle holds in its own buffers (caller-owned arrays not included). */
+int dlp_lp_device_bytes(void* handle, int64_t* out);
+/* Device bytes a handle for n vertices and C1 columns (row length ld) holds once its global
+ * solve is reserved (fp64 != 0: the plain FP64 path, which repairs by push and reserves no
+ * global-solve buffers): dirs = direction storage kind of
+ * dlp_lp_set_pcg_directions, coarse = aggregates of dlp_lp_set_coarse, block = block PCG.
+ * Equals dlp_lp_device_bytes after dlp_lp_create + the same settings + dlp_lp_reserve; -1 on
+ * bad sizes.  For admission by memory budget (no reference counterpart). */
+int64_t dlp_lp_handle_bytes(int n, int C1, int64_t ld, int fp64, int dirs, int iters_per_launch, int coarse,
+                            int block);
 /* Cumulative global-solve timing: out[0] iterations, out[1] microseconds (HIP events). */
 int dlp_lp_pcg_stats(void* handle, int64_t* out);
 /* Average duration in ms (HIP events on `stream`, synchronises) of the global solve's operator
@@ -358,8 +380,10 @@ int dlp_lp_violators(void* handle, int32_t** list, int32_t** count);
  * [13] push rounds launched, [14] their summed duration in microseconds (HIP events),
  * [15] the bits of an FP64 "push mass": over the rows the last changed-row refresh found
  * violating, the sum of max_c |R_c|/(eps_c*diag) (about the row pushes a repair by push alone
- * would take); rho (C1, may be NULL): column maxima of the last refresh.  `out` holds 16
- * entries. */
+ * would take), [16] pairs of this batch refused by dlp_lp_apply (ids outside [0, n) or
+ * insertion weights not finite and positive), [17] repeated insertion entries it merged;
+ * rho (C1, may be NULL): column maxima of the last refresh, in which a NaN residual counts as
+ * +inf (so it never certifies).  `out` holds 18 entries. */
 int dlp_lp_status(void* handle, int64_t* out, double* rho, void* stream);
 
 #ifdef __cplusplus

@@ -467,3 +467,135 @@ def test_block_pcg_repair_certifies(K):
         O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
                            zip(b.del_u.tolist(), b.del_v.tolist()))
         check_against_oracle(lp, g, es, 1e-6, f"batch {t}")
+
+
+def _stored_entries(lp):
+    ip = lp.indptr.cpu().numpy()
+    rl = lp.rowlen.cpu().numpy()
+    col = lp.col.cpu().numpy()
+    w = lp.w.cpu().numpy()
+    out = {}
+    for r in range(lp.n):
+        cs = col[ip[r]: ip[r] + rl[r]]
+        assert np.unique(cs).shape == cs.shape, f"row {r} stores a column twice"
+        assert np.all(cs >= 0), f"row {r} keeps a tombstone"
+        for c, x in zip(cs, w[ip[r]: ip[r] + rl[r]]):
+            out[(r, int(c))] = float(x)
+    return out
+
+
+def _check_store(lp, g, es):
+    """The device store equals the oracle edge set: both directions, one entry each, same
+    weight; labeled-neighbour counts match."""
+    st = _stored_entries(lp)
+    want = {}
+    for (a, b), x in es.items():
+        want[(a, b)] = want[(b, a)] = float(np.float32(x))
+    assert st == want
+    lab = np.zeros(g.n, dtype=np.int64)
+    for (a, b) in want:
+        lab[a] += int(g.labeled[b])
+    assert np.array_equal(lp.labcnt.cpu().numpy(), lab)
+
+
+def test_repeated_pairs_in_a_batch_match_oracle():
+    """Pairs repeated in one request, in both orientations, within the insertions (new and
+    stored pairs) and within the deletions: the result equals the oracle's sequential
+    application (deletions first, last insertion wins)."""
+    from dynlp_amd.graphs import EdgeBatch
+    from dynlp_amd.stream import EdgeStreamLP
+    g = graphs.sbm(1500, 4, 8, 0.8, 4, 0.05, seed=8)
+    eps = 1e-6
+    lp = EdgeStreamLP(g, eps=eps)
+    lp.initial_solve()
+    es = edge_dict(g)
+    keys = sorted(es)
+    new = [(a, b) for a, b in ((0, 1400), (3, 1100), (7, 900), (11, 1300)) if (a, b) not in es]
+    iu = [new[0][0], new[0][1], new[1][0], keys[5][1], keys[5][0], new[1][1], new[0][0]]
+    iv = [new[0][1], new[0][0], new[1][1], keys[5][0], keys[5][1], new[1][0], new[0][1]]
+    iw = [float(np.float32(x)) for x in (0.6, 1.2, 0.9, 2.0, 0.7, 1.4, 0.8)]   # the stored FP32 weights
+    du = [keys[10][0], keys[10][1], keys[10][0], keys[20][1], keys[20][0]]
+    dv = [keys[10][1], keys[10][0], keys[10][1], keys[20][0], keys[20][1]]
+    i32 = lambda x: np.asarray(x, dtype=np.int32)  # noqa: E731
+    b = EdgeBatch(i32(iu), i32(iv), np.asarray(iw, dtype=np.float32), i32(du), i32(dv))
+    st = lp.apply_and_solve(b)
+    O.apply_edge_batch(es, zip(iu, iv, iw), zip(du, dv))
+    assert st.extra.get("missing_deletions", 0) == 0
+    _check_store(lp, g, es)
+    check_against_oracle(lp, g, es, eps, "repeated pairs")
+
+
+def test_device_apply_refuses_bad_pairs_and_keeps_last_occurrence():
+    """Straight through the C ABI, with no host check in front: ids outside [0, n) and
+    insertion weights that are not finite and positive are skipped and counted (the kernels
+    test every id before indexing a row by it), a repeated deletion removes the pair once,
+    and of a pair inserted several times, in either orientation, only the last occurrence is
+    stored, in both directions.  The state then repairs to the oracle's solution."""
+    import torch
+    from dynlp_amd import _lib
+    from dynlp_amd.backend import device, ptr, stream
+    from dynlp_amd.stream import EdgeStreamLP, Stats
+    g = graphs.sbm(1500, 4, 8, 0.8, 4, 0.05, seed=9)
+    eps = 1e-6
+    lp = EdgeStreamLP(g, eps=eps)
+    lp.initial_solve()
+    es = edge_dict(g)
+    keys = sorted(es)
+    n = g.n
+    new = [(a, b) for a, b in ((2, 1450), (5, 1200), (9, 1000)) if (a, b) not in es]
+    nan, inf = float("nan"), float("inf")
+    ins = [(new[0][0], new[0][1], 0.5), (new[0][1], new[0][0], 1.5), (new[1][0], new[1][1], 0.75),
+           (keys[3][1], keys[3][0], 2.5), (keys[3][0], keys[3][1], 0.25),
+           (n, 4, 1.0), (4, -1, 1.0), (1 << 30, 7, 1.0),                 # ids out of range
+           (new[2][0], new[2][1], nan), (6, 8, inf), (6, 9, 0.0), (6, 10, -2.0)]   # bad weights
+    dels = [(keys[30][0], keys[30][1]), (keys[30][1], keys[30][0]), (n + 5, 3), (-7, 2)]
+    dev = device()
+    t32 = lambda x: torch.as_tensor(np.asarray(x, dtype=np.int32)).to(dev)  # noqa: E731
+    iu, iv = t32([a for a, _, _ in ins]), t32([b for _, b, _ in ins])
+    iw = torch.as_tensor(np.asarray([x for _, _, x in ins], dtype=np.float32)).to(dev)
+    du, dv = t32([a for a, _ in dels]), t32([b for _, b in dels])
+    _lib.call("dlp_lp_begin_batch", lp.h, stream())
+    _lib.call("dlp_lp_apply", lp.h, len(dels), ptr(du), ptr(dv), len(ins), ptr(iu), ptr(iv), ptr(iw), stream())
+    s = lp.status()
+    assert s["invalid"] == 2 + 3 + 4, s      # two deletions, three insertion ids, four weights
+    assert s["duplicates"] == 2, s           # the earlier occurrences of new[0] and keys[3]
+    assert s["missing"] == 2, s              # the repeat of keys[30], both directions
+    O.apply_edge_batch(es, [(a, b, x) for a, b, x in ins[:5]], dels[:2])
+    _check_store(lp, g, es)
+    _lib.call("dlp_lp_ground", lp.h, stream())
+    _lib.call("dlp_lp_refresh", lp.h, 0, 0, stream())
+    st = Stats()
+    s = lp._repair(st)
+    lp._finish(st, s)
+    assert st.rho <= eps * 1.0001
+    check_against_oracle(lp, g, es, eps, "after the raw apply")
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(pcg_directions="fp32", pcg_coarse=0), dict(fp64=True),
+                                dict(pcg_mode="block"), dict(pcg_iters_per_launch=3)])
+def test_footprint_is_what_the_graph_holds(kw):
+    """The admission estimate equals what the handle holds once built (its own count) and,
+    within allocation granularity, the device memory a resident graph takes; a request
+    afterwards allocates nothing in the library (the handle's bytes do not change)."""
+    import torch
+    from dynlp_amd.stream import EdgeStreamLP
+    g = graphs.sbm(20000, 8, 10, 0.8, 8, 0.02, seed=2)
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    lp = EdgeStreamLP(g, eps=1e-6, **kw)
+    torch.cuda.synchronize()
+    free1 = torch.cuda.mem_get_info()[0]
+    est = EdgeStreamLP.footprint(g, **kw)
+    from dynlp_amd import _lib
+    n, C1, ld = lp.n, lp.C1, lp.ld
+    kinds = {"fp32": 0, "bf16": 1, "fp16": 2}
+    hb = _lib.load().dlp_lp_handle_bytes(n, C1, ld, int(lp.fp64), kinds[lp.pcg_directions],
+                                         lp.pcg_iters_per_launch, lp.pcg_coarse, int(lp.pcg_mode == "block"))
+    assert lp.device_bytes() == hb
+    used = free0 - free1
+    # torch's caching allocator rounds its blocks up (2 MiB segments) and hipMalloc to pages
+    assert abs(used - est) <= 0.05 * est + (64 << 20), (used, est)
+    lp.initial_solve()
+    b = graphs.update_stream(g, 1, 2000, seed=2)[0][0]
+    lp.apply_and_solve(b)
+    assert lp.device_bytes() == hb

@@ -73,3 +73,69 @@ def test_check_batch_refuses_ragged_and_out_of_range():
     check_batch(two, 10, max_insertions=2)
     with pytest.raises(ValueError, match="at most 1"):
         check_batch(two, 10, max_insertions=1)
+
+
+def test_check_batch_refuses_bad_weights():
+    """Insertion weights must be finite and positive as FP32 (the device operator's weights):
+    NaN, +-inf, 0, negatives and values that overflow FP32 are refused."""
+    import pytest
+    from dynlp_amd.graphs import EdgeBatch, check_batch
+    i = lambda *x: np.asarray(x, dtype=np.int32)  # noqa: E731
+    e = np.zeros(0, dtype=np.int32)
+    check_batch(EdgeBatch(i(1), i(2), np.array([1e-30]), e, e), 10)
+    for w in (np.nan, np.inf, -np.inf, 0.0, -1.0, 1e39):
+        with pytest.raises(ValueError, match="finite and positive"):
+            check_batch(EdgeBatch(i(1, 3), i(2, 4), np.array([1.0, w]), e, e), 10)
+
+
+def test_normalize_batch_equals_sequential_semantics():
+    """A batch with pairs repeated in both orientations, within the deletions and within the
+    insertions, normalised to one entry per pair, changes an edge set exactly as the oracle's
+    sequential application of the raw batch (deletions first, then insertions in order: the
+    last weight wins)."""
+    from dynlp_amd.graphs import EdgeBatch, normalize_batch
+    from oracle import lp as O
+    rs = np.random.RandomState(5)
+    n = 12
+    for trial in range(200):
+        base = {}
+        for _ in range(20):
+            a, b = rs.randint(0, n, 2)
+            if a != b:
+                base[(min(a, b), max(a, b))] = float(rs.randint(1, 5))
+        k = rs.randint(0, 15)
+        iu, iv = rs.randint(0, n, k), rs.randint(0, n, k)
+        iw = rs.randint(1, 9, k).astype(np.float32)
+        m = rs.randint(0, 10)
+        du, dv = rs.randint(0, n, m), rs.randint(0, n, m)
+        raw = EdgeBatch(iu.astype(np.int32), iv.astype(np.int32), iw, du.astype(np.int32), dv.astype(np.int32))
+        nb = normalize_batch(raw, n)
+        ki = np.minimum(nb.ins_u, nb.ins_v).astype(np.int64) * n + np.maximum(nb.ins_u, nb.ins_v)
+        kd = np.minimum(nb.del_u, nb.del_v).astype(np.int64) * n + np.maximum(nb.del_u, nb.del_v)
+        assert np.unique(ki).shape == ki.shape and np.unique(kd).shape == kd.shape
+        seq = O.apply_edge_batch(dict(base), zip(iu.tolist(), iv.tolist(), iw.tolist()), zip(du.tolist(), dv.tolist()))
+        one = O.apply_edge_batch(dict(base), zip(nb.ins_u.tolist(), nb.ins_v.tolist(), nb.ins_w.tolist()),
+                                 zip(nb.del_u.tolist(), nb.del_v.tolist()))
+        assert seq == one, trial
+    b = EdgeBatch(np.array([1, 2], np.int32), np.array([2, 3], np.int32), np.array([1.0, 2.0], np.float32),
+                  np.array([4], np.int32), np.array([5], np.int32))
+    assert normalize_batch(b, n) is b
+
+
+def test_fuse_of_one_batch_collapses_repeats():
+    """A single request goes through the same de-duplication as a fused group, so it changes its
+    graph identically whether served alone or fused."""
+    from dynlp_amd.graphs import EdgeBatch
+    from dynlp_amd.serve import fuse
+    from oracle import lp as O
+    b = EdgeBatch(np.array([1, 2, 1], np.int32), np.array([2, 1, 3], np.int32), np.array([1.0, 5.0, 2.0], np.float32),
+                  np.array([3, 1], np.int32), np.array([1, 3], np.int32))
+    f = fuse([b])
+    keys = np.minimum(f.ins_u, f.ins_v) * 10 + np.maximum(f.ins_u, f.ins_v)
+    assert np.unique(keys).shape == keys.shape
+    base = {(1, 3): 4.0, (0, 1): 1.0}
+    seq = O.apply_edge_batch(dict(base), zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
+                             zip(b.del_u.tolist(), b.del_v.tolist()))
+    one = O.apply_edge_batch(dict(base), zip(f.ins_u.tolist(), f.ins_v.tolist(), f.ins_w.tolist()),
+                             zip(f.del_u.tolist(), f.del_v.tolist()))
+    assert seq == one == {(0, 1): 1.0, (1, 2): 5.0, (1, 3): 2.0}
