@@ -60,6 +60,8 @@ _SIGS = {
     "dlp_lp_pcg": (ct.c_int, [P, I32, I32, I32, I32, P, P, P]),
     "dlp_lp_pcg_stats": (ct.c_int, [P, P]),
     "dlp_lp_reserve": (ct.c_int, [P, I32, I32, P]),
+    "dlp_cc_csr_static": (ct.c_int, [I32, P, P, P, P, P, P, P]),
+    "dlp_cc_static_max_n": (ct.c_int, []),
     "dlp_lp_device_bytes": (ct.c_int, [P, P]),
     "dlp_lp_handle_bytes": (ct.c_int64, [I32, I32, I64, I32, I32, I32, I32, I32]),
     "dlp_lp_bpcg": (ct.c_int, [P, I32, I32, I32, I32, P, P, P]),

@@ -3,6 +3,7 @@
 // A read-only pass gives each vertex its root, the component minimum; flags (root or not) are
 // scanned so that a component's label is the rank of its minimum among all component minima,
 // which is the labelling the reference's hook/jump + unique produces (cc.py:12-23).
+#include <mutex>
 #include "scan.h"
 #include "uf.h"
 
@@ -136,7 +137,7 @@ struct CCWork {
   size_t scan_bytes;
 };
 
-static size_t cc_bytes(int n) {
+constexpr size_t cc_bytes(int n) {
   const size_t a = ((size_t)n * 4 + 255) & ~(size_t)255;
   return 2 * a + 256 + scan_workspace_bytes(n);
 }
@@ -152,6 +153,12 @@ static int cc_carve(int n, void* ws, size_t ws_bytes, CCWork& w) {
   w.scan_ws = p + 2 * a + 256;
   w.scan_bytes = ws_bytes - 2 * a - 256;
   return DLP_OK;
+}
+
+// *out = 0 (a kernel rather than a memset, so that the no-workspace entry point can be
+// captured into a graph).
+__global__ void k_cc_zero(int32_t* out) {
+  if (threadIdx.x == 0) *out = 0;
 }
 
 static int cc_finish(int n, CCWork& w, int32_t* labels, int32_t* n_comp, hipStream_t st) {
@@ -176,7 +183,9 @@ extern "C" int dlp_cc_edges(int n, int64_t m, const int32_t* rows, const int32_t
   DLP_CHECK_ARG(n >= 0 && m >= 0, "cc_edges: bad sizes n=%d m=%lld", n, (long long)m);
   hipStream_t st = (hipStream_t)stream;
   if (n == 0) {
-    DLP_CHECK_HIP(hipMemsetAsync(n_comp, 0, sizeof(int32_t), st));
+    DLP_CHECK_ARG(n_comp, "cc: null pointer");
+    hipLaunchKernelGGL(k_cc_zero, dim3(1), dim3(64), 0, st, n_comp);
+    DLP_CHECK_LAUNCH();
     return DLP_OK;
   }
   DLP_CHECK_ARG(labels && n_comp && (m == 0 || (rows && cols)), "cc_edges: null pointer");
@@ -216,7 +225,9 @@ extern "C" int dlp_cc_csr(int n, const int32_t* indptr, const int32_t* rowlen,
   DLP_CHECK_ARG(n >= 0, "cc_csr: bad n=%d", n);
   hipStream_t st = (hipStream_t)stream;
   if (n == 0) {
-    DLP_CHECK_HIP(hipMemsetAsync(n_comp, 0, sizeof(int32_t), st));
+    DLP_CHECK_ARG(n_comp, "cc: null pointer");
+    hipLaunchKernelGGL(k_cc_zero, dim3(1), dim3(64), 0, st, n_comp);
+    DLP_CHECK_LAUNCH();
     return DLP_OK;
   }
   DLP_CHECK_ARG(indptr && indices && labels && n_comp, "cc_csr: null pointer");
@@ -227,3 +238,46 @@ extern "C" int dlp_cc_csr(int n, const int32_t* indptr, const int32_t* rowlen,
   if (rc) return rc;
   return cc_finish(n, w, labels, n_comp, st);
 }
+
+// ---- Connected components with no workspace and no allocation ------------------------------
+// Scratch of dlp_cc_csr_static: a file-scope device array, one copy per GPU (each at its own
+// address), sized for kCCStaticMaxN vertices and held until the process ends.  Its address is
+// looked up once per device, with that device current, on the entry point's first call there;
+// the array's name is only a host-side stand-in and is never passed to a kernel.
+namespace {
+constexpr int kCCStaticMaxN = 1 << 21;
+constexpr size_t kCCStaticBytes = dlp::cc_bytes(kCCStaticMaxN);
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kCCStaticBytes];
+
+static int static_scratch(void** out) {
+  int dev = 0;
+  DLP_CHECK_HIP(hipGetDevice(&dev));
+  DLP_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "cc_csr_static: device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    DLP_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    DLP_CHECK_ARG(p != nullptr, "cc_csr_static: no device address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return DLP_OK;
+}
+
+extern "C" int dlp_cc_csr_static(int n, const int32_t* indptr, const int32_t* rowlen, const int32_t* indices,
+                                 const uint8_t* mask, int32_t* labels, int32_t* n_comp, void* stream) {
+  DLP_CHECK_ARG(n >= 0 && n <= kCCStaticMaxN,
+                "cc_csr_static: n=%d outside [0, %d], the vertices its fixed scratch holds (use dlp_cc_csr)", n,
+                kCCStaticMaxN);
+  void* ws = nullptr;
+  const int rc = static_scratch(&ws);
+  if (rc) return rc;
+  return dlp_cc_csr(n, indptr, rowlen, indices, mask, labels, n_comp, ws, kCCStaticBytes, stream);
+}
+
+extern "C" int dlp_cc_static_max_n(void) { return kCCStaticMaxN; }

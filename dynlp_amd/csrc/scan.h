@@ -8,7 +8,7 @@ namespace dlp {
 constexpr int kScanItems = 4;                       // items per thread
 constexpr int kScanTile = kBlock * kScanItems;      // 1024 items per tile
 
-inline size_t scan_workspace_bytes(int n) {
+constexpr inline size_t scan_workspace_bytes(int n) {
   const size_t tiles = (size_t)(n + kScanTile - 1) / kScanTile + 1;
   return ((tiles * 4 + 255) & ~(size_t)255);
 }

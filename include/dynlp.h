@@ -196,6 +196,18 @@ int dlp_cc_csr(int n, const int32_t* indptr, const int32_t* rowlen, const int32_
                const uint8_t* mask, int32_t* labels, int32_t* n_comp, void* ws, size_t ws_bytes,
                void* stream);
 size_t dlp_cc_workspace_bytes(int n);
+/* dlp_cc_csr with no workspace and no device allocation: its scratch is a fixed device array
+ * of the library (one copy per GPU, about 17 MB, held until the process ends), so n is at most
+ * dlp_cc_static_max_n() (2^21; larger n: DLP_ERR_ARG before any device work).  Calls no
+ * hipMalloc/hipFree and no memset, so it can be captured into a HIP graph once it has run
+ * outside capture on that device (the first call there looks up the array's address).  Every
+ * call on one GPU shares the array: calls that overlap on one GPU (different streams or
+ * threads) race.  No counterpart in the reference; for callers that cannot allocate (graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). */
+int dlp_cc_csr_static(int n, const int32_t* indptr, const int32_t* rowlen, const int32_t* indices,
+                      const uint8_t* mask, int32_t* labels, int32_t* n_comp, void* stream);
+/* The largest n dlp_cc_csr_static takes. */
+int dlp_cc_static_max_n(void);
 
 
 /* ---- Resident incremental state of one graph (the service's per-graph handle) ----------
