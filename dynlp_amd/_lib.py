@@ -62,6 +62,12 @@ _SIGS = {
     "dlp_lp_reserve": (ct.c_int, [P, I32, I32, P]),
     "dlp_cc_csr_static": (ct.c_int, [I32, P, P, P, P, P, P, P]),
     "dlp_cc_static_max_n": (ct.c_int, []),
+    "dlp_scatter": (ct.c_int, [I32, I32, I64, I64, P, I64, P, I32, P, I64, P, P]),
+    "dlp_fh_components": (ct.c_int, [I32, I64, P, P, P, DBL, P, P, P, P, SZ, P]),
+    "dlp_fh_workspace_bytes": (SZ, [I32]),
+    "dlp_fh_scale": (ct.c_int, [I32, I64, P, P, I32, DBL, P, P, P]),
+    "dlp_fh_dissimilarity": (ct.c_int, [I64, P, I32, P, P, P, P, DBL, P, P, P, SZ, P]),
+    "dlp_fh_dissimilarity_workspace_bytes": (SZ, []),
     "dlp_lp_device_bytes": (ct.c_int, [P, P]),
     "dlp_lp_handle_bytes": (ct.c_int64, [I32, I32, I64, I32, I32, I32, I32, I32]),
     "dlp_lp_bpcg": (ct.c_int, [P, I32, I32, I32, I32, P, P, P]),

@@ -64,22 +64,45 @@ def empty(*shape, dtype=torch.float32):
     return torch.empty(*shape, dtype=dtype, device=device())
 
 
+_SCATTER_DTYPES = {torch.float32: 0, torch.float64: 1, torch.int32: 2, torch.int64: 3}
+
+
+def _scatter(op, a, idx, v):
+    """a[idx[i]] op= v[i] on the device (dlp_scatter), unbuffered: repeated indices all apply.
+    v: one row per index (shape (m, *a.shape[1:])), or one value for every element; indices
+    outside a's first dimension are skipped."""
+    if a.dtype not in _SCATTER_DTYPES or not a.is_contiguous():
+        raise TypeError(f"scatter: contiguous float32/float64/int32/int64 target expected, got {a.dtype}")
+    idx = torch.as_tensor(idx, device=a.device)
+    if idx.dtype not in (torch.int32, torch.int64):
+        idx = idx.long()
+    idx = idx.contiguous()
+    m = int(idx.shape[0])
+    cols = 1
+    for d in a.shape[1:]:
+        cols *= int(d)
+    v = torch.as_tensor(v, dtype=a.dtype, device=a.device)
+    if v.numel() == 1:
+        v, stride = v.reshape(1).contiguous(), 0
+    else:
+        v, stride = v.expand(m, *a.shape[1:]).contiguous(), cols
+    _lib.call("dlp_scatter", op, _SCATTER_DTYPES[a.dtype], int(a.shape[0]), cols, a.data_ptr(), m, idx.data_ptr(),
+              int(idx.dtype == torch.int64), v.data_ptr(), stride, None, stream())
+
+
 def scatter_add(a, idx, v):
     """a[idx] += v, unbuffered (repeated indices accumulate) (backend.py:19-20)."""
-    a.index_put_((idx.long(),), torch.as_tensor(v, dtype=a.dtype, device=a.device).expand(idx.shape[0], *a.shape[1:]),
-                 accumulate=True)
+    _scatter(0, a, idx, v)
 
 
 def scatter_max(a, idx, v):
     """a[idx] = max(a[idx], v), unbuffered (backend.py:23-24)."""
-    a.scatter_reduce_(0, idx.long(), torch.as_tensor(v, dtype=a.dtype, device=a.device).expand(idx.shape[0]),
-                      reduce="amax", include_self=True)
+    _scatter(1, a, idx, v)
 
 
 def scatter_min(a, idx, v):
     """a[idx] = min(a[idx], v), unbuffered (backend.py:27-28)."""
-    a.scatter_reduce_(0, idx.long(), torch.as_tensor(v, dtype=a.dtype, device=a.device).expand(idx.shape[0]),
-                      reduce="amin", include_self=True)
+    _scatter(2, a, idx, v)
 
 
 def sync():

@@ -5,6 +5,8 @@ among all component minima, bit-identical to the reference's hook/jump + unique.
 """
 from __future__ import annotations
 
+import ctypes as ct
+
 import torch
 
 from . import _lib
@@ -50,73 +52,74 @@ def connected_components_csr(W: CSR, mask=None, out=None, count=None, ws=None):
     return labels, nc
 
 
-def _lightest(nc, cu, cv, de):
-    """For every component with an outgoing edge (cc.py:26-35): the component, the one across
-    its lightest outgoing edge (ties: the smallest such component) and that edge's weight."""
-    from .backend import scatter_min
-    dev = de.device
-    best = torch.full((nc,), float("inf"), dtype=torch.float32, device=dev)
-    scatter_min(best, cu, de)
-    across = torch.full((nc,), nc, dtype=torch.int64, device=dev)
-    sel = de <= best[cu.long()]
-    scatter_min(across, cu[sel], cv[sel].long())
-    c = torch.nonzero(across < nc).flatten()
-    return c, across[c], best[c]
-
-
 def fh_components(m, rows, cols, d, k, ground=None):
-    """Felzenszwalb-Huttenlocher merging in parallel Boruvka rounds (cc.py:38-74): every
-    component takes its lightest outgoing edge e and merges across it when
+    """Felzenszwalb-Huttenlocher merging of m vertices over the edge list (rows, cols) with
+    dissimilarities d (FP32), in parallel Boruvka rounds on the device (dlp_fh_components,
+    cc.py:38-74): a component merges across its lightest outgoing edge e when
         d(e) <= min(Int(C1) + k/|C1|, Int(C2) + k/|C2|),
-    Int(C) being C's largest internal dissimilarity; the merges of a round are joined by
-    connected components.  When no edge qualifies and ground (per vertex) is given, components
-    with zero ground then join across their lightest outgoing edge until none can.  FP32
-    dissimilarities, as the reference keeps them.  Returns (labels int32[m], count)."""
-    from .backend import scatter_add, scatter_max
+    Int(C) being C's largest internal dissimilarity; when no edge qualifies and ground (per
+    vertex, >= 0) is given, components with no ground then join across their lightest outgoing
+    edge until none can.  Returns (labels int32[m], count)."""
     dev = device()
-    lab = torch.arange(m, dtype=torch.int32, device=dev)
-    Int = torch.zeros(m, dtype=torch.float32, device=dev)
-    size = torch.ones(m, dtype=torch.float32, device=dev)
-    rows = torch.as_tensor(rows, device=dev).long()
-    cols = torch.as_tensor(cols, device=dev).long()
-    d = torch.as_tensor(d, device=dev).to(torch.float32)
-    gvert = None if ground is None else torch.as_tensor(ground, device=dev).to(torch.float32)
-    nc, fh = m, True
-    while rows.shape[0]:
-        cu, cv = lab[rows].long(), lab[cols].long()
-        ext = cu != cv
-        if not fh:   # grounding pass: only components without ground pick an edge
-            g = torch.zeros(nc, dtype=torch.float32, device=dev)
-            scatter_add(g, lab, gvert)
-            ext &= g[cu] == 0
-        if not bool(ext.any()):
-            break
-        c, t, dc = _lightest(nc, cu[ext], cv[ext], d[ext])
-        if fh:
-            ok = dc <= torch.minimum(Int[c] + k / size[c], Int[t] + k / size[t])
-            if not bool(ok.any()):
-                if gvert is None:
-                    break
-                fh = False
-                continue
-            c, t, dc = c[ok], t[ok], dc[ok]
-        root, nc = connected_components(nc, c, t)
-        root = root.long()
-        Int2 = torch.zeros(nc, dtype=torch.float32, device=dev)
-        size2 = torch.zeros(nc, dtype=torch.float32, device=dev)
-        scatter_max(Int2, root, Int)
-        scatter_max(Int2, root[c], dc)
-        scatter_add(size2, root, size)
-        lab, Int, size = root[lab.long()].to(torch.int32), Int2, size2
-    return lab, nc
+    m = int(m)
+    labels = torch.empty(max(m, 1), dtype=torch.int32, device=dev)[:m]
+    rows = torch.as_tensor(rows, device=dev).to(torch.int32).contiguous()
+    cols = torch.as_tensor(cols, device=dev).to(torch.int32).contiguous()
+    d = torch.as_tensor(d, device=dev).to(torch.float32).contiguous()
+    if not (rows.shape == cols.shape == d.shape):
+        raise ValueError("rows, cols and d differ in length")
+    g = None if ground is None else torch.as_tensor(ground, device=dev).to(torch.float32).contiguous()
+    nb = int(_lib.load().dlp_fh_workspace_bytes(max(m, 1)))
+    ws = torch.empty(nb, dtype=torch.uint8, device=dev)
+    nc = ct.c_int32(0)
+    _lib.call("dlp_fh_components", m, int(rows.shape[0]), ptr(rows), ptr(cols), ptr(d), float(k), ptr(g),
+              ptr(labels), ct.byref(nc), ptr(ws), nb, stream())
+    return labels, int(nc.value)
 
 
-def adaptive_components(m, rows, cols, w, s_r, s_c, wmax, ground=None):
-    """Component-adaptive merging (cc.py:77-83): dissimilarity 1 - w/wmax scaled by the two
-    endpoints' own mean dissimilarities, k = the mean scaled dissimilarity."""
-    if m == 0:
-        return torch.zeros(0, dtype=torch.int32, device=device()), 0
-    w = torch.as_tensor(w, device=device())
-    s_rc = torch.as_tensor(s_r, device=device()) * torch.as_tensor(s_c, device=device())
-    d = (1 - w / wmax) / torch.sqrt(s_rc).to(w.dtype)   # in w's precision; FP32 from fh_components on
-    return fh_components(m, rows, cols, d, float(d.mean()) if d.shape[0] else 0.0, ground)
+def _dtype_of(w):
+    return _lib.F64 if w.dtype == torch.float64 else _lib.F32
+
+
+def adaptive_components(m, rows, cols, w, s_r, s_c, wmax, ground=None, scale_index=None):
+    """Component-adaptive merging (cc.py:77-83): the dissimilarity 1 - w/wmax scaled by the two
+    endpoints' own mean dissimilarities (s_r, s_c per edge), k = the mean scaled
+    dissimilarity; both on the device (dlp_fh_dissimilarity), then fh_components.  With
+    scale_index=(ir, ic), s_r and s_c are per-vertex scales read at ir[e], ic[e] instead."""
+    dev = device()
+    if int(m) == 0:
+        return torch.zeros(0, dtype=torch.int32, device=dev), 0
+    w = torch.as_tensor(w, device=dev)
+    if w.dtype != torch.float64:
+        w = w.to(torch.float32)
+    w = w.contiguous()
+    rows = torch.as_tensor(rows, device=dev).to(torch.int32).contiguous()
+    cols = torch.as_tensor(cols, device=dev).to(torch.int32).contiguous()
+    s_r = torch.as_tensor(s_r, device=dev).to(torch.float32).contiguous()
+    s_c = torch.as_tensor(s_c, device=dev).to(torch.float32).contiguous()
+    ir, ic = (None, None) if scale_index is None else (
+        torch.as_tensor(x, device=dev).to(torch.int32).contiguous() for x in scale_index)
+    ne = int(w.shape[0])
+    d = torch.empty(max(ne, 1), dtype=torch.float32, device=dev)[:ne]
+    nb = int(_lib.load().dlp_fh_dissimilarity_workspace_bytes())
+    ws = torch.empty(nb, dtype=torch.uint8, device=dev)
+    total = ct.c_double(0.0)
+    _lib.call("dlp_fh_dissimilarity", ne, ptr(w), _dtype_of(w), ptr(s_r), ptr(s_c), ptr(ir), ptr(ic), float(wmax),
+              ptr(d), ct.byref(total), ptr(ws), nb, stream())
+    return fh_components(m, rows, cols, d, total.value / ne if ne else 0.0, ground)
+
+
+def vertex_scales(m, lr, w, wmax):
+    """Local scale of each of m vertices over its stored entries (lr[e] = row, w[e] > 0 live):
+    the mean of 1 - w/wmax, plus 1e-6 (FP32, stream.py:72-74), on the device (dlp_fh_scale)."""
+    dev = device()
+    w = torch.as_tensor(w, device=dev)
+    if w.dtype != torch.float64:
+        w = w.to(torch.float32)
+    w = w.contiguous()
+    lr = torch.as_tensor(lr, device=dev).to(torch.int32).contiguous()
+    s = torch.empty(max(int(m), 1), dtype=torch.float32, device=dev)
+    cnt = torch.empty_like(s)
+    _lib.call("dlp_fh_scale", int(m), int(w.shape[0]), ptr(lr), ptr(w), _dtype_of(w), float(wmax), ptr(s), ptr(cnt),
+              stream())
+    return s[: int(m)]

@@ -552,24 +552,18 @@ def new_components(m, lr, lc, w, tau, wmax, ground):
     the edges heavier than the graph's mean weight tau (the paper's supernodes); "fh" = adaptive
     components (cc.adaptive_components), whose scale is each vertex's mean dissimilarity
     1 - w/wmax over all its live entries; ground = each vertex's weight to labeled vertices."""
-    from .backend import scatter_add
-    from .cc import adaptive_components, connected_components
+    from .cc import adaptive_components, connected_components, vertex_scales
     dev = device()
-    lr = torch.as_tensor(lr, device=dev).long()
-    lc = torch.as_tensor(lc, device=dev).long()
-    w = torch.as_tensor(w, device=dev)          # the system's precision, rounded to FP32 after use
-    on = w > 0
-    s = torch.zeros(m, dtype=torch.float32, device=dev)
-    cnt = torch.zeros(m, dtype=torch.float32, device=dev)
-    scatter_add(s, lr, torch.where(on, 1 - w / wmax, torch.zeros_like(w)).to(torch.float32))
-    scatter_add(cnt, lr, on.to(torch.float32))
-    s = s / torch.clamp(cnt, min=1) + 1e-6
-    e = on & (lc >= 0)
+    lr = torch.as_tensor(lr, device=dev).to(torch.int32)
+    lc = torch.as_tensor(lc, device=dev).to(torch.int32)
+    w = torch.as_tensor(w, device=dev)          # the system's precision
+    s = vertex_scales(m, lr, w, wmax)
+    e = torch.nonzero((w > 0) & (lc >= 0)).flatten()
     r, c, we = lr[e], lc[e], w[e]
-    heavy = we > tau
+    heavy = torch.nonzero(we > tau).flatten()
     return {"tau": connected_components(m, r[heavy], c[heavy]),
-            "fh": adaptive_components(m, r, c, we, s[r], s[c], wmax,
-                                      torch.as_tensor(ground, device=dev).to(torch.float32))}
+            "fh": adaptive_components(m, r, c, we, s, s, wmax, torch.as_tensor(ground, device=dev).to(torch.float32),
+                                      scale_index=(r, c))}
 
 
 def _arrival_order(g: Graph, seed: int) -> np.ndarray:

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-enum { DLP_F32 = 0, DLP_F64 = 1 };
+enum { DLP_F32 = 0, DLP_F64 = 1, DLP_I32 = 2, DLP_I64 = 3 };
 enum { DLP_OK = 0, DLP_ERR_ARG = -1, DLP_ERR_HIP = -2, DLP_ERR_CAPACITY = -3, DLP_ERR_STATE = -4 };
 
 /* Last error message of the calling thread (empty string if none). */
@@ -209,6 +209,42 @@ int dlp_cc_csr_static(int n, const int32_t* indptr, const int32_t* rowlen, const
 /* The largest n dlp_cc_csr_static takes. */
 int dlp_cc_static_max_n(void);
 
+
+/* ---- Unbuffered scatters (backend.scatter_add/max/min, backend.py:11-28) ---------------
+ * a[idx[i], j] op= v[i * v_stride + j] for i < m, j < cols (a: n_out x cols row-major; v_stride
+ * = cols, or 0 for one value v[0] everywhere); op 0 add, 1 max, 2 min; repeated indices all
+ * apply (atomics).  dtype DLP_F32/F64/I32/I64 for a and v; idx int32 (idx64 = 0) or int64.
+ * Floating max/min let a NaN win (as numpy's maximum.at).  Indices outside [0, n_out) are
+ * skipped and counted in *bad (device int32, may be NULL). */
+int dlp_scatter(int op, int dtype, int64_t n_out, int64_t cols, void* a, int64_t m, const void* idx, int idx64,
+                const void* v, int64_t v_stride, int32_t* bad, void* stream);
+
+/* ---- Supernodes of new vertices (DynLP step 1, stream.py:67-81, cc.py:26-83) -------------
+ * Adaptive (Felzenszwalb-Huttenlocher) components of m vertices over the directed edge list
+ * (rows[e], cols[e]) with FP32 dissimilarities d[e], in Boruvka rounds: every component takes
+ * its lightest outgoing edge (ties: the smallest component across) and merges across it when
+ * d <= min(Int(C1) + k/|C1|, Int(C2) + k/|C2|) (Int = largest internal dissimilarity, FP32 as
+ * the reference keeps it); when no edge qualifies and ground (FP32 per vertex, >= 0, may be
+ * NULL) is given, components whose ground sums to 0 then join across their lightest outgoing
+ * edge until none can.  labels (int32[m]) = rank of the component's first numbering as the
+ * reference's connected_components gives it; *n_comp (host).  Synchronises once per round.
+ * Replaces cc.fh_components and _lightest (cc.py:26-74). */
+int dlp_fh_components(int m, int64_t ne, const int32_t* rows, const int32_t* cols, const float* d, double k,
+                      const float* ground, int32_t* labels, int32_t* n_comp, void* ws, size_t ws_bytes,
+                      void* stream);
+size_t dlp_fh_workspace_bytes(int m);
+/* s[v] = mean over the entries (lr[e] = v) with w > 0 of (1 - w/wmax), plus 1e-6 (FP32; w FP32 or
+ * FP64 by dtype): the local scale of new_components (stream.py:72-74). cnt: FP32[m] scratch. */
+int dlp_fh_scale(int m, int64_t ne, const int32_t* lr, const void* w, int dtype, double wmax, float* s, float* cnt,
+                 void* stream);
+/* d[e] = (1 - w[e]/wmax) / sqrt(sr_e * sc_e) (in w's precision, stored FP32), with sr_e =
+ * s_r[ir[e]] when ir is non-NULL, else s_r[e] (likewise sc_e with ic, s_c), and *sum_out = the
+ * sum of d (host, FP64, deterministic order; synchronises): the dissimilarities and k = mean of
+ * cc.adaptive_components (cc.py:77-83).  Workspace: dlp_fh_dissimilarity_workspace_bytes(). */
+int dlp_fh_dissimilarity(int64_t ne, const void* w, int dtype, const float* s_r, const float* s_c, const int32_t* ir,
+                         const int32_t* ic, double wmax, float* d, double* sum_out, void* ws, size_t ws_bytes,
+                         void* stream);
+size_t dlp_fh_dissimilarity_workspace_bytes(void);
 
 /* ---- Resident incremental state of one graph (the service's per-graph handle) ----------
  * Replaces IncrementalStream's in-place batch apply (stream.py:180-346) and IncrementalLP's

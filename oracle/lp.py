@@ -483,6 +483,81 @@ def block_pcg(W, diag, X, rhs, eps, M=None, max_iter=10_000):
 
 
 # ---------------------------------------------------------------------------------------------
+# Supernodes of new vertices (cc.py:26-83, stream.py:67-81), NumPy float32 as the reference
+# keeps Int, size, d and the scales.
+
+def fh_components(m, rows, cols, d, k, ground=None):
+    """Boruvka rounds of Felzenszwalb-Huttenlocher merging (cc.py:38-74): each component's
+    lightest outgoing edge (ties: smallest component across, cc.py:26-35) merges when
+    d <= min(Int_c + k/size_c, Int_t + k/size_t); then, with ground, components whose ground
+    sums to zero join across their lightest outgoing edge until none can."""
+    rows = np.asarray(rows, dtype=np.int64)
+    cols = np.asarray(cols, dtype=np.int64)
+    d = np.asarray(d, dtype=np.float32)
+    gv = None if ground is None else np.asarray(ground, dtype=np.float32)
+    kf = np.float32(k)
+    lab = np.arange(m, dtype=np.int64)
+    Int = np.zeros(m, dtype=np.float32)
+    size = np.ones(m, dtype=np.float32)
+    nc, fh = m, True
+    while rows.shape[0]:
+        cu, cv = lab[rows], lab[cols]
+        ext = cu != cv
+        if not fh:
+            g = np.zeros(nc, dtype=np.float32)
+            np.add.at(g, lab, gv)
+            ext &= g[cu] == 0
+        if not ext.any():
+            break
+        eu, ev, de = cu[ext], cv[ext], d[ext]
+        best = np.full(nc, np.inf, dtype=np.float32)
+        np.minimum.at(best, eu, de)
+        across = np.full(nc, nc, dtype=np.int64)
+        sel = de <= best[eu]
+        np.minimum.at(across, eu[sel], ev[sel])
+        c = np.flatnonzero(across < nc)
+        t, dc = across[c], best[c]
+        if fh:
+            ok = dc <= np.minimum(Int[c] + kf / size[c], Int[t] + kf / size[t])
+            if not ok.any():
+                if gv is None:
+                    break
+                fh = False
+                continue
+            c, t, dc = c[ok], t[ok], dc[ok]
+        root, nc = connected_components(nc, c, t)
+        root = root.astype(np.int64)
+        Int2 = np.zeros(nc, dtype=np.float32)
+        size2 = np.zeros(nc, dtype=np.float32)
+        np.maximum.at(Int2, root, Int)
+        np.maximum.at(Int2, root[c], dc)
+        np.add.at(size2, root, size)
+        lab, Int, size = root[lab], Int2, size2
+    return lab.astype(np.int32), int(nc)
+
+
+def vertex_scales(m, lr, w, wmax):
+    """Mean of 1 - w/wmax over each vertex's live entries, plus 1e-6, in float32
+    (stream.py:72-74)."""
+    w = np.asarray(w)
+    on = w > 0
+    s = np.zeros(m, dtype=np.float32)
+    cnt = np.zeros(m, dtype=np.float32)
+    np.add.at(s, np.asarray(lr)[on], (1 - w[on] / w.dtype.type(wmax)).astype(np.float32))
+    np.add.at(cnt, np.asarray(lr)[on], np.float32(1))
+    return s / np.maximum(cnt, np.float32(1)) + np.float32(1e-6)
+
+
+def adaptive_dissimilarity(w, s_r, s_c, wmax):
+    """(d float32, k) of cc.py:77-83: d = (1 - w/wmax) / sqrt(s_r s_c) in w's precision (the
+    scale product and its root in float32), k = the mean of d before rounding."""
+    w = np.asarray(w)
+    ss = np.sqrt(np.asarray(s_r, dtype=np.float32) * np.asarray(s_c, dtype=np.float32))
+    x = (1 - w / w.dtype.type(wmax)) / ss.astype(w.dtype)
+    return x.astype(np.float32), (float(x.astype(np.float64).sum()) / x.shape[0] if x.shape[0] else 0.0)
+
+
+# ---------------------------------------------------------------------------------------------
 # Edge-batch apply: the incremental result must equal the system rebuilt from the updated
 # edge set (tests/test_incremental.py:18-41 pattern).
 
@@ -511,6 +586,10 @@ PINS = {
     "AMG": "M(I) equals the dense two-level V-cycle X1 = w D^-1 R, X2 = X1 + P (P^T A P)^-1 P^T (R - A X1), "
            "M R = X2 + w D^-1 (R - A X2) on hand and random 7-8 vertex graphs; M(I) SPD "
            "(tests/test_dynlp.py:159-166); no-coarsening case M = A^-1",
+    "fh_components": "hand cases (a weak bridge kept cut, a chain merged, the smallest-component tie, the "
+                     "grounding pass) and the reference's two closed-form component cases "
+                     "(tests/test_dynlp.py:56-71)",
+    "vertex_scales/adaptive_dissimilarity": "hand values on a 3-vertex example",
     "_handshake": "hand cases for the smallest-id tie, the join of unmatched vertices to the heaviest "
                   "matched neighbour (and its tie), singletons, the round limit; a loop-level brute force "
                   "of the matching rules on random graphs, composed over two passes",

@@ -431,3 +431,91 @@ def test_cpu_incremental_port_certifies_each_batch():
     sysm = O.build_system(g.n, u, v, w, g.labeled, g.y, g.K)
     R = O.residual(sysm, cpu.Z)
     assert O.rho_cols(sysm.diag, R)[: sysm.C].max() <= 1e-6 * 1.0001
+
+
+def _fh(m, edges, k, ground=None):
+    a = [x for x, _, _ in edges] + [y for _, y, _ in edges]
+    b = [y for _, y, _ in edges] + [x for x, _, _ in edges]
+    d = [z for _, _, z in edges] * 2
+    return O.fh_components(m, np.array(a), np.array(b), np.array(d, dtype=np.float32), k, ground)
+
+
+def test_fh_keeps_a_weak_bridge_cut():
+    """Two triangles (d = 0.1 inside) joined by d = 0.9, k = 0.3: round one merges each triangle
+    (0.1 <= 0 + 0.3/1); the bridge then fails 0.9 <= 0.1 + 0.3/3."""
+    tri = [(0, 1, 0.1), (0, 2, 0.1), (1, 2, 0.1), (3, 4, 0.1), (3, 5, 0.1), (4, 5, 0.1)]
+    lab, nc = _fh(6, tri + [(2, 3, 0.9)], 0.3)
+    assert nc == 2 and list(lab) == [0, 0, 0, 1, 1, 1]
+    lab, nc = _fh(6, tri + [(2, 3, 0.9)], 3.0)           # a large k lets the bridge pass
+    assert nc == 1
+
+
+def test_fh_tie_and_grounding_pass():
+    """Vertex 2 (no ground) sees 0 and 1 at equal d; k = 0 accepts nothing in the merge pass, so
+    the grounding pass runs: only the ungrounded component picks, and the tie goes to the
+    smallest component across (0).  Without ground nothing merges."""
+    edges = [(2, 0, 0.5), (2, 1, 0.5)]
+    lab, nc = _fh(3, edges, 0.0, ground=np.array([1.0, 1.0, 0.0]))
+    assert nc == 2 and list(lab) == [0, 1, 0]
+    lab, nc = _fh(3, edges, 0.0)
+    assert nc == 3
+
+
+def test_fh_grounding_pass_repeats_until_grounded():
+    """Chain 0-1-2, ground only on 0, d(0,1) = 0.8, d(1,2) = 0.2, k = 0.1: no edge passes the
+    merge test, then {1,2} join (both ungrounded) and, still ungrounded, join 0."""
+    edges = [(0, 1, 0.8), (1, 2, 0.2)]
+    lab, nc = _fh(3, edges, 0.1, ground=np.array([1.0, 0.0, 0.0]))
+    assert nc == 1
+    lab, nc = _fh(3, edges, 0.1)
+    assert nc == 3
+    lab, nc = _fh(4, [(0, 1, 0.1), (1, 2, 0.1), (2, 3, 0.1)], 1.0)   # a chain within k merges whole
+    assert nc == 1
+
+
+def test_vertex_scales_and_dissimilarity_by_hand():
+    s = O.vertex_scales(3, np.array([0, 0, 1, 2]), np.array([0.5, 1.0, 0.25, 0.0], dtype=np.float32), 1.0)
+    np.testing.assert_allclose(s, [0.25 + 1e-6, 0.75 + 1e-6, 1e-6], rtol=1e-6)
+    d, k = O.adaptive_dissimilarity(np.array([0.5, 0.25], dtype=np.float32), np.array([0.25, 0.25], np.float32),
+                                    np.array([0.25, 1.0], np.float32), 1.0)
+    np.testing.assert_allclose(d, [2.0, 1.5], rtol=1e-6)
+    assert k == pytest.approx(1.75, rel=1e-6)
+
+
+def _new_components_oracle(edges, m, tau, ground=None):
+    """The reference's tests/test_dynlp.py:50-53 helper on the oracle: every vertex new, both
+    directions of every edge stored."""
+    a, b, w = zip(*edges)
+    lr, lc = np.array(a + b), np.array(b + a)
+    w = np.array(w + w, dtype=np.float32)
+    g = np.ones(m) if ground is None else np.asarray(ground, dtype=np.float64)
+    s = O.vertex_scales(m, lr, w, 1.0)
+    heavy = w > tau
+    tau_c = O.connected_components(m, lr[heavy], lc[heavy])
+    d, k = O.adaptive_dissimilarity(w, s[lr], s[lc], 1.0)
+    return {"tau": tau_c, "fh": O.fh_components(m, lr, lc, d, k, g.astype(np.float32))}
+
+
+def test_fh_reference_closed_form_cases():
+    """tests/test_dynlp.py:56-71 of the reference."""
+    edges = [(a + o, b + o, 0.95 + 0.01 * ((a + b) % 3)) for o in (0, 6) for a in range(6) for b in range(a + 1, 6)]
+    edges.append((5, 6, 0.8))
+    comps = _new_components_oracle(edges, 12, tau=0.5)
+    assert comps["tau"][1] == 1
+    assert comps["fh"][1] == 2 and list(comps["fh"][0]) == [0] * 6 + [1] * 6
+    edges = [(i, i + 1, 0.3 + 0.02 * (i % 3)) for i in range(7)]
+    comps = _new_components_oracle(edges, 8, tau=0.5, ground=[1, 0, 0, 0, 0, 0, 0, 0])
+    assert comps["tau"][1] == 8
+    assert comps["fh"][1] == 1
+
+
+def test_fh_threshold_uses_each_sides_size():
+    """A triangle (Int 0.1, size 3) and a pair (Int 0.1, size 2) joined at d = 0.22, k = 0.3:
+    the thresholds are 0.1 + 0.3/3 = 0.2 and 0.1 + 0.3/2 = 0.25, and the smaller one decides,
+    so the bridge stays cut (without the size terms, or with the larger threshold, it would
+    merge)."""
+    edges = [(0, 1, 0.1), (0, 2, 0.1), (1, 2, 0.1), (3, 4, 0.1), (2, 3, 0.22)]
+    lab, nc = _fh(5, edges, 0.3)
+    assert nc == 2 and list(lab) == [0, 0, 0, 1, 1]
+    lab, nc = _fh(5, edges[:-1] + [(2, 3, 0.19)], 0.3)
+    assert nc == 1
