@@ -35,8 +35,6 @@ _SIGS = {
     "dlp_mark_neighbors": (ct.c_int, [I32, P, P, P, P, P, P]),
     "dlp_flatnonzero_u8": (ct.c_int, [I32, P, P, P, P, SZ, P]),
     "dlp_scan_workspace_bytes": (SZ, [I32]),
-    "dlp_fused_push_round": (ct.c_int, [P, P, P, P, P, P, P, I32, P, P, P, P, P, P, I32, I32, P]),
-    "dlp_compact_frontier": (ct.c_int, [P, P, P, P, I32, P, P, P, P, P, I32, I32, P]),
     "dlp_cc_edges": (ct.c_int, [I32, I64, P, P, P, P, P, SZ, P]),
     "dlp_cc_csr": (ct.c_int, [I32, P, P, P, P, P, P, P, SZ, P]),
     "dlp_cc_workspace_bytes": (SZ, [I32]),
@@ -62,6 +60,16 @@ _SIGS = {
     "dlp_lp_reserve": (ct.c_int, [P, I32, I32, P]),
     "dlp_cc_csr_static": (ct.c_int, [I32, P, P, P, P, P, P, P]),
     "dlp_cc_static_max_n": (ct.c_int, []),
+    "dlp_push_rounds": (ct.c_int, [I32, I32, I64, I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, I32, P]),
+    "dlp_cg_residual": (ct.c_int, [I32, I32, I64, I32, P, P, P, DBL, P, P, I64, P, P, P, P, P]),
+    "dlp_cg_direction": (ct.c_int, [I32, I32, I64, I32, P, P, P, P, I32, P]),
+    "dlp_cg_iteration": (ct.c_int, [I32, I32, I64, I32, P, P, P, DBL, P, P, P, P, P, P, P, P]),
+    "dlp_park_rows": (ct.c_int, [I32, P, I32, I64, I32, P, P, P]),
+    "dlp_rows_copy": (ct.c_int, [I32, P, I32, P, I64, I32, P, I64, I32, I32, P]),
+    "dlp_supernode_average": (ct.c_int, [I32, P, P, I32, P, P, P, P, I32, P, I64, P, I32, I32, P, DBL, P, I64, I32,
+                                         P, SZ, P]),
+    "dlp_supernode_workspace_bytes": (SZ, [I32, I32, I32]),
+    "dlp_frontier_commit": (ct.c_int, [I32, P, I32, I64, P, P, P, P, P, P, P, P, I32, P]),
     "dlp_scatter": (ct.c_int, [I32, I32, I64, I64, P, I64, P, I32, P, I64, P, P]),
     "dlp_fh_components": (ct.c_int, [I32, I64, P, P, P, DBL, P, P, P, P, SZ, P]),
     "dlp_fh_workspace_bytes": (SZ, [I32]),

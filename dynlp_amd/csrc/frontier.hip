@@ -1,5 +1,4 @@
-// Frontier primitives: Jacobi on a vertex subset, residual scatter, neighbour marking, and the
-// asynchronous fused push (take residual, update, scatter, compact).
+// Frontier primitives: Jacobi on a vertex subset, residual scatter and neighbour marking.
 #include "blas.h"
 #include "rowpipe.h"
 
@@ -218,98 +217,6 @@ static void count_edges(int nr, const int32_t* rows, const int32_t* indptr, unsi
   hipLaunchKernelGGL(k_count_edges, dim3(g), dim3(kBlock), 0, st, nr, rows, indptr, edges);
 }
 
-// One asynchronous push round (chaotic relaxation): the residual of every frontier vertex is
-// read-and-zeroed atomically so pushes landing during the round are kept for the next one.
-template <typename T, int CPL>
-__global__ __launch_bounds__(256) void k_fused_push_round(
-    const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
-    const T* __restrict__ w, const T* __restrict__ diag, const int32_t* __restrict__ front,
-    const int32_t* __restrict__ fcount, int32_t* __restrict__ fcount_next, int C, int g_shift,
-    int lc_shift, T* __restrict__ Z, T* __restrict__ R, int32_t* __restrict__ mark,
-    int32_t* __restrict__ touched, int32_t* __restrict__ tcount,
-    unsigned long long* __restrict__ edges) {
-  const Lanes L(g_shift, lc_shift);
-  const int nf = *fcount;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *fcount_next = 0;
-  const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> g_shift;
-  const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> g_shift;
-  const int64_t wave_first = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) >> g_shift;
-  unsigned long long my_edges = 0;
-  // iterate in wave-uniform steps so the cross-lane reductions see whole waves
-  for (int64_t base = wave_first; base < nf; base += groups) {
-    const int64_t g = base + (g0 - wave_first);
-    const bool valid = g < nf;
-    const int u = valid ? front[g] : 0;
-    T d[CPL];
-#pragma unroll
-    for (int k = 0; k < CPL; ++k) d[k] = 0;
-    if (valid && L.e == 0) {
-      const T inv = (T)1 / diag[u];
-#pragma unroll
-      for (int k = 0; k < CPL; ++k) {
-        const int c = L.l + k * L.LC;
-        if (c < C) {
-          d[k] = atomic_take(&R[(int64_t)u * C + c]) * inv;
-          Z[(int64_t)u * C + c] += d[k];
-        }
-      }
-    }
-    // broadcast the column values from edge lane 0 to the other edge lanes of the group
-#pragma unroll
-    for (int k = 0; k < CPL; ++k) d[k] = __shfl(d[k], L.l, L.G);
-    if (valid) {
-      const int beg = indptr[u], end = indptr[u + 1];
-      if (L.lane == 0) my_edges += (unsigned long long)(end - beg);
-      for (int j = beg + L.e; j < end; j += L.LE) {
-        const T wv = w[j];
-        if (wv == (T)0) continue;  // inert entries (outside the solved set)
-        const int v = indices[j];
-#pragma unroll
-        for (int k = 0; k < CPL; ++k) {
-          const int c = L.l + k * L.LC;
-          if (c < C) atomic_add(&R[(int64_t)v * C + c], wv * d[k]);
-        }
-        if (L.l == 0 && atomicExch(&mark[v], 1) == 0) touched[atomicAdd(tcount, 1)] = v;
-      }
-    }
-  }
-  if (edges) {
-    const unsigned long long s = wave_sum_u64(my_edges);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(edges, s);
-  }
-}
-
-// Touched vertices are unmarked; violators form the next frontier.
-template <typename T>
-__global__ __launch_bounds__(256) void k_compact_frontier(
-    const int32_t* __restrict__ touched, const int32_t* __restrict__ tcount,
-    int32_t* __restrict__ tcount_next, int32_t* __restrict__ mark, int C, int lc_shift,
-    const T* __restrict__ R, const T* __restrict__ diag, const T* __restrict__ eps,
-    int32_t* __restrict__ out, int32_t* __restrict__ fcount_out) {
-  const int LC = 1 << lc_shift;
-  const int nt = *tcount;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *tcount_next = 0;
-  const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> lc_shift;
-  const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> lc_shift;
-  const int64_t wave_first = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) >> lc_shift;
-  const int l = threadIdx.x & (LC - 1);
-  const int wl = threadIdx.x & 63;
-  const unsigned long long gm = (LC == 64 ? ~0ull : ((1ull << LC) - 1)) << (wl & ~(LC - 1));
-  for (int64_t base = wave_first; base < nt; base += groups) {
-    const int64_t i = base + (g0 - wave_first);
-    const bool valid = i < nt;
-    const int v = valid ? touched[i] : 0;
-    bool viol = false;
-    if (valid) {
-      if (l == 0) mark[v] = 0;
-      const T dv = diag[v];
-      for (int c = l; c < C; c += LC) viol |= fabs(R[(int64_t)v * C + c]) > eps[c] * dv;
-    }
-    const unsigned long long bal = __ballot(viol);
-    if (valid && l == 0 && (bal & gm)) out[atomicAdd(fcount_out, 1)] = v;
-  }
-}
-
 }  // namespace dlp
 
 using namespace dlp;
@@ -444,65 +351,6 @@ extern "C" int dlp_mark_neighbors(int nr, const int32_t* rows, const int32_t* in
   hipLaunchKernelGGL(k_mark_neighbors, dim3(grid_for((long long)nr * 8)), dim3(kBlock), 0,
                      (hipStream_t)stream, nr, rows, indptr, indices, mark);
   count_edges(nr, rows, indptr, edges, (hipStream_t)stream);
-  DLP_CHECK_LAUNCH();
-  return DLP_OK;
-}
-
-extern "C" int dlp_fused_push_round(const int32_t* indptr, const int32_t* indices, const void* w,
-                                    const void* diag, const int32_t* front,
-                                    const int32_t* fcount, int32_t* fcount_next, int C, void* Z,
-                                    void* R, int32_t* mark, int32_t* touched, int32_t* tcount,
-                                    unsigned long long* edges, int dtype, int grid,
-                                    void* stream) {
-  DLP_CHECK_ARG(C >= 1 && C <= 256 && grid >= 1, "fused_push_round: bad C=%d grid=%d", C, grid);
-  DLP_CHECK_ARG(indptr && indices && w && diag && front && fcount && fcount_next && Z && R &&
-                    mark && touched && tcount,
-                "fused_push_round: null pointer");
-  hipStream_t st = (hipStream_t)stream;
-  // lanes over columns only (the scatter is atomic per column): G = LC
-  RowMap m = choose_map(C, 1.0);
-  m.g_shift = m.lc_shift < 2 ? 2 : m.lc_shift;
-  if (dtype == DLP_F32) {
-#define K(CPLV)                                                                                \
-  hipLaunchKernelGGL((k_fused_push_round<float, CPLV>), dim3(grid), dim3(kBlock), 0, st, indptr, \
-                     indices, (const float*)w, (const float*)diag, front, fcount, fcount_next, C, \
-                     m.g_shift, m.lc_shift, (float*)Z, (float*)R, mark, touched, tcount, edges)
-    DLP_DISPATCH_CPL(m.cpl, K)
-#undef K
-  } else if (dtype == DLP_F64) {
-#define K(CPLV)                                                                                   \
-  hipLaunchKernelGGL((k_fused_push_round<double, CPLV>), dim3(grid), dim3(kBlock), 0, st, indptr,   \
-                     indices, (const double*)w, (const double*)diag, front, fcount, fcount_next, C, \
-                     m.g_shift, m.lc_shift, (double*)Z, (double*)R, mark, touched, tcount, edges)
-    DLP_DISPATCH_CPL(m.cpl, K)
-#undef K
-  } else {
-    DLP_CHECK_ARG(false, "fused_push_round: bad dtype %d", dtype);
-  }
-  DLP_CHECK_LAUNCH();
-  return DLP_OK;
-}
-
-extern "C" int dlp_compact_frontier(const int32_t* touched, const int32_t* tcount,
-                                    int32_t* tcount_next, int32_t* mark, int C, const void* R,
-                                    const void* diag, const void* eps, int32_t* out,
-                                    int32_t* fcount_out, int dtype, int grid, void* stream) {
-  DLP_CHECK_ARG(C >= 1 && grid >= 1, "compact_frontier: bad C=%d grid=%d", C, grid);
-  DLP_CHECK_ARG(touched && tcount && tcount_next && mark && R && diag && eps && out && fcount_out,
-                "compact_frontier: null pointer");
-  hipStream_t st = (hipStream_t)stream;
-  int ls = C <= 1 ? 0 : ceil_log2(C);
-  if (ls > 6) ls = 6;
-  if (dtype == DLP_F32)
-    hipLaunchKernelGGL(k_compact_frontier<float>, dim3(grid), dim3(kBlock), 0, st, touched, tcount,
-                       tcount_next, mark, C, ls, (const float*)R, (const float*)diag,
-                       (const float*)eps, out, fcount_out);
-  else if (dtype == DLP_F64)
-    hipLaunchKernelGGL(k_compact_frontier<double>, dim3(grid), dim3(kBlock), 0, st, touched,
-                       tcount, tcount_next, mark, C, ls, (const double*)R, (const double*)diag,
-                       (const double*)eps, out, fcount_out);
-  else
-    DLP_CHECK_ARG(false, "compact_frontier: bad dtype %d", dtype);
   DLP_CHECK_LAUNCH();
   return DLP_OK;
 }

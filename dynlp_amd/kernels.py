@@ -52,53 +52,79 @@ def spmm_axpy(W: CSR, X, a=None, b=1.0, group="auto"):
 
 
 class FusedPush:
-    """Asynchronous residual push with resident buffers (kernels.py:396-433).
-
-    Takes the residual of every violating vertex (|r_u| > eps_c * diag_u), adds r_u / diag_u
-    to z_u and scatters it to the neighbours with hardware float atomics, keeping r exact up
-    to rounding (chaotic relaxation; converges for this M-matrix).  Any column count.
-    """
+    """Residual push with device-resident candidate lists (kernels.py:396-433 is the interface):
+    rounds of take (D = R/diag, Z += D, R = 0), scatter (R[N(u)] += w D, atomics) and select
+    (touched rows that still violate |R_uc| > eps_c diag_u) run back to back through
+    dlp_push_rounds; the host reads the candidate count every `check_every` rounds.  R stays
+    exact up to rounding; any column count up to 256; FP32 or FP64 (W, Z, R and diag share it).
+    threads and blocks_per_sm are accepted for the reference's signature; the launch shapes are
+    the library's."""
 
     def __init__(self, W: CSR, C, group="auto", threads=256, blocks_per_sm=8):
         _check_cols(C)
         n = W.shape[0]
-        self.W, self.C = W, C
-        z = lambda *s: torch.zeros(s, dtype=torch.int32, device=device())  # noqa: E731
-        self.front, self.touched, self.mark, self.fcnt, self.tcnt = z(2, n), z(n), z(n), z(2), z(2)
-        self.edges = torch.zeros(1, dtype=torch.int64, device=device())
-        sms = torch.cuda.get_device_properties(device()).multi_processor_count
-        self.grid = int(sms * blocks_per_sm)
-        if threads != 256:
-            raise ValueError("FusedPush runs 256-thread workgroups")
+        dev = device()
+        self.W, self.C, self.n = W, C, n
+        self.lists = [torch.empty(max(n, 1), dtype=torch.int32, device=dev) for _ in range(2)]
+        self.counts = torch.zeros(3, dtype=torch.int32, device=dev)
+        self.stamp = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
+        self.touched = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+        self.edges = torch.zeros(1, dtype=torch.int64, device=dev)
+        self._D = {}
+        self.parity = 0
+        self.eps = None
 
-    def run(self, Z, R, diag, eps, cand, max_rounds, check_every=8):
-        """Push until no vertex violates; returns (rounds, converged, edge visits)."""
+    def _scratch(self, dtype):
+        D = self._D.get(dtype)
+        if D is None:
+            D = self._D[dtype] = torch.empty((max(self.n, 1), self.C), dtype=dtype, device=device())
+        return D
+
+    def seed(self, R, diag, eps, cand=None):
+        """Current candidates: the given rows, or (cand None) every row that violates."""
+        self.counts.zero_()
+        self.parity = 0
+        self.eps = torch.as_tensor(eps, dtype=torch.float64, device=R.device).reshape(-1).contiguous()
+        if cand is None:
+            d64 = diag.double().contiguous()
+            _lib.call("dlp_violating_rows", self.n, None, self.C, ptr(R), R.stride(0), dtype_code(R.dtype), ptr(d64),
+                      ptr(self.eps), ptr(self.lists[0]), ptr(self.counts), stream())
+        else:
+            k = int(cand.shape[0])
+            self.lists[0][:k] = cand.to(torch.int32)
+            self.counts[0] = k
+
+    def launch(self, Z, R, diag, rounds, edges=None):
+        """Enqueue `rounds` rounds (no host synchronisation); edge visits go to `edges` (a device
+        int64 counter) or this object's own."""
         W = self.W
-        n0 = int(cand.shape[0])
-        self.edges.zero_()
-        if not n0:
-            return 0, True, 0
-        dt = dtype_code(Z.dtype)
         if W.data.dtype != Z.dtype or R.dtype != Z.dtype or diag.dtype != Z.dtype:
             raise TypeError("W, Z, R and diag must share a dtype")
-        self.front[0, :n0] = cand.to(torch.int32)
-        self.fcnt.zero_()
-        self.tcnt.zero_()
-        self.fcnt[0] = n0
-        eps = torch.as_tensor(eps, dtype=Z.dtype, device=Z.device).contiguous()
-        rounds, s = 0, stream()
-        fc, tc = self.fcnt.data_ptr(), self.tcnt.data_ptr()
-        while rounds < max_rounds:
-            p, q = rounds & 1, 1 - (rounds & 1)
-            _lib.call("dlp_fused_push_round", ptr(W.indptr), ptr(W.indices), ptr(W.data), ptr(diag),
-                      ptr(self.front[p]), fc + 4 * p, fc + 4 * q, self.C, ptr(Z), ptr(R), ptr(self.mark),
-                      ptr(self.touched), tc + 4 * p, ptr(self.edges), dt, self.grid, s)
-            _lib.call("dlp_compact_frontier", ptr(self.touched), tc + 4 * p, tc + 4 * q, ptr(self.mark),
-                      self.C, ptr(R), ptr(diag), ptr(eps), ptr(self.front[q]), fc + 4 * q, dt, self.grid, s)
-            rounds += 1
-            if rounds % check_every == 0 and int(self.fcnt[q]) == 0:
-                break
-        return rounds, int(self.fcnt[rounds & 1]) == 0, int(self.edges[0])
+        cur, nxt = self.lists[self.parity], self.lists[1 - self.parity]
+        _lib.call("dlp_push_rounds", self.n, self.C, Z.stride(0), dtype_code(Z.dtype), ptr(W.indptr), ptr(W.indices),
+                  ptr(W.data), ptr(diag), ptr(self.eps), ptr(Z), ptr(R), ptr(self._scratch(Z.dtype)), ptr(cur),
+                  ptr(nxt), ptr(self.counts), ptr(self.stamp), ptr(self.touched),
+                  ptr(self.edges if edges is None else edges), int(rounds), stream())
+        self.parity ^= int(rounds) & 1
+
+    def pending(self):
+        """Candidates of the next round (reads one count)."""
+        return int(self.counts[0].item())
+
+    def run(self, Z, R, diag, eps, cand, max_rounds, check_every=8):
+        """Push until no vertex violates or max_rounds; returns (rounds, converged, edge visits).
+        cand None: start from every violating row."""
+        self.edges.zero_()
+        if not self.n:
+            return 0, True, 0
+        self.seed(R, diag, eps, cand)
+        rounds, left = 0, self.pending()
+        while left and rounds < max_rounds:
+            step = int(min(check_every, max_rounds - rounds))
+            self.launch(Z, R, diag, step)
+            rounds += step
+            left = self.pending()
+        return rounds, left == 0, int(self.edges.item())
 
 
 class FrontierOps:

@@ -168,21 +168,42 @@ int dlp_flatnonzero_u8(int n, const uint8_t* mask, int32_t* out, int32_t* count,
                        size_t ws_bytes, void* stream);
 size_t dlp_scan_workspace_bytes(int n);
 
-/* ---- Fused asynchronous push (FusedPush, kernels.py:396-433) -------------------------
- * One round: every frontier vertex u (front[0..*fcount)) takes R[u] atomically, adds
- * R[u]/diag[u] to Z[u] and scatters w*d to its neighbours (entries with w == 0 skipped);
- * first touches are appended to touched (count in tcount); fcount_next is reset.  CUDA
- * kernels.py:287-331. */
-int dlp_fused_push_round(const int32_t* indptr, const int32_t* indices, const void* w,
-                         const void* diag, const int32_t* front, const int32_t* fcount,
-                         int32_t* fcount_next, int C, void* Z, void* R, int32_t* mark,
-                         int32_t* touched, int32_t* tcount, unsigned long long* edges,
-                         int dtype, int grid, void* stream);
-/* Compaction: touched vertices are unmarked and those violating |R| > eps*diag appended to
- * out (count in fcount_out); tcount_next is reset.  CUDA kernels.py:333-350. */
-int dlp_compact_frontier(const int32_t* touched, const int32_t* tcount, int32_t* tcount_next,
-                         int32_t* mark, int C, const void* R, const void* diag, const void* eps,
-                         int32_t* out, int32_t* fcount_out, int dtype, int grid, void* stream);
+/* ---- Residual push rounds with device-resident lists (solvers.push, FusedPush) ----------
+ * `rounds` push rounds on the batch system (n rows, plain CSR, state Z, R: n x C with leading
+ * dimension ld, dtype FP32/FP64 for w, diag, Z, R, D): a round takes the residual of every
+ * current candidate (D = R/diag, Z += D, R = 0), scatters w*D to the neighbours (atomics) and
+ * lists as next candidates the vertices it touched (candidates included) that violate
+ * |R_uc| > eps_c * diag_u (eps: device FP64[C]).  counts (device int32[3]): [0] current
+ * candidates, [1] and [2] zero between rounds; candidates alternate between cand and next
+ * (after an odd number of rounds the current list is in next).  stamp: int32[n] zeros between
+ * rounds; touched: int32[n]; D: n x C scratch; edges (device uint64, may be NULL) += entries
+ * scattered.  No host synchronisation: the caller polls counts[0].  Replaces the host-driven
+ * rounds of solvers.push (solvers.py:100-125) and the FusedPush kernels (kernels.py:287-350). */
+int dlp_push_rounds(int n, int C, int64_t ld, int dtype, const int32_t* indptr, const int32_t* indices,
+                    const void* w, const void* diag, const double* eps, void* Z, void* R, void* D, int32_t* cand,
+                    int32_t* next, int32_t* counts, int32_t* stamp, int32_t* touched, unsigned long long* edges,
+                    int rounds, void* stream);
+
+/* ---- Column-wise PCG on a batch system (solvers.pcg, solvers.py:66-97) -------------------
+ * Z, R, P, AP (and Y) are n x C with leading dimension ld, dtype FP32/FP64 (with w, diag, rhs);
+ * scal: device FP64[3C + 1] = [rz | pAp | r.z of the new direction | violation]; eps: device
+ * FP64[C].  dlp_cg_residual: R = rhs - (diag Z - W Z) and scal-style violation
+ * *viol = max |R_uc| / (eps_c diag_u) (NaN counts as +inf).  dlp_cg_direction: r.z_new = R.Y per
+ * column, then P = Y (first != 0) or Y + beta P with beta = r.z_new / rz (0 when rz <= 0), and rz
+ * = r.z_new.  dlp_cg_iteration: AP = diag P - W P, pAp = P.AP, alpha = rz/pAp (0 when pAp <= 0),
+ * Z += alpha P, R -= alpha AP, violation of the new R in scal[3C].  No host synchronisation. */
+int dlp_cg_residual(int n, int C, int64_t ld, int dtype, const int32_t* indptr, const int32_t* indices,
+                    const void* w, double avg_deg, const void* diag, const void* rhs, int64_t ldr, const void* Z,
+                    void* R, const double* eps, double* viol, void* stream);
+int dlp_cg_direction(int n, int C, int64_t ld, int dtype, const void* R, const void* Y, void* P, double* scal,
+                     int first, void* stream);
+int dlp_cg_iteration(int n, int C, int64_t ld, int dtype, const int32_t* indptr, const int32_t* indices,
+                     const void* w, double avg_deg, const void* diag, void* Z, void* R, void* P, void* AP,
+                     double* scal, const double* eps, void* stream);
+
+/* Rows with solved[r] == 0 take the values of an inert row: Z = (0, .., 0, 1) over C1 columns
+ * (h last), R = 0 (R may be NULL).  IncrementalLP._park (solvers.py:365-368). */
+int dlp_park_rows(int n, const uint8_t* solved, int C1, int64_t ld, int dtype, void* Z, void* R, void* stream);
 
 /* ---- Connected components (cc.connected_components, cc.py:8-23) ----------------------
  * labels[v] = rank of the minimum vertex id of v's component (int32, bit-exact with the
@@ -209,6 +230,34 @@ int dlp_cc_csr_static(int n, const int32_t* indptr, const int32_t* rowlen, const
 /* The largest n dlp_cc_csr_static takes. */
 int dlp_cc_static_max_n(void);
 
+
+/* Row copies with a precision change (FP32/FP64 either side): gather != 0: dst[i] = src[rows[i]],
+ * else dst[rows[i]] = src[i], C values per row, rows int64 (NULL: row i), distinct when scattering.
+ * The global-store reads and writes of Solver._warm/_store (solvers.py:176-184). */
+int dlp_rows_copy(int nrows, const int64_t* rows, int C, const void* src, int64_t lds, int src_dtype, void* dst,
+                  int64_t ldd, int dst_dtype, int gather, void* stream);
+
+/* Supernode averages of the m new vertices newrows[i] (components lab[i] < nc): per vertex
+ * S_i[c] = base + sum over its entries j with known[col_j] of w_j X[col_j, c] and T_i = base' +
+ * sum of those w_j (known NULL: no neighbour terms); base_mode 1: S base = the label columns of B
+ * (n x K; column 1 when K = 2) and T base = B's row sum, 2: S base 0 and T base = B's row sum, 0:
+ * none.  Per component S and T add up; out[newrows[i], c] = (T > 0 ? S/T : fallback[c]) + offset
+ * (fallback: device, C values).  dtype for w, X, B, fallback, out.  Workspace:
+ * dlp_supernode_workspace_bytes(nc, C, dtype).  solvers.supernode_init / init_h / _comp_avg
+ * (solvers.py:128-163). */
+int dlp_supernode_average(int m, const int32_t* newrows, const int32_t* lab, int nc, const int32_t* indptr,
+                          const int32_t* indices, const void* w, const uint8_t* known, int C, const void* X, int64_t ldx,
+                          const void* B, int K, int base_mode, const void* fallback, double offset, void* out,
+                          int64_t ldo, int dtype, void* ws, size_t ws_bytes, void* stream);
+size_t dlp_supernode_workspace_bytes(int nc, int C, int dtype);
+
+/* Frontier rows whose Jacobi value changed (changed[g] != 0) take it, X[frontier[g]] = Y[g]
+ * (Y: nf x C contiguous), and mark their neighbours; *nchanged (device) += their count and
+ * *edges (device, may be NULL) += their entries.  The update and next-frontier marking of
+ * DynLP (solvers.py:228-238). */
+int dlp_frontier_commit(int nf, const int32_t* frontier, int C, int64_t ldx, const void* Y, const uint8_t* changed,
+                        void* X, const int32_t* indptr, const int32_t* indices, uint8_t* mark, int32_t* nchanged,
+                        unsigned long long* edges, int dtype, void* stream);
 
 /* ---- Unbuffered scatters (backend.scatter_add/max/min, backend.py:11-28) ---------------
  * a[idx[i], j] op= v[i * v_stride + j] for i < m, j < cols (a: n_out x cols row-major; v_stride
