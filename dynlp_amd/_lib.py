@@ -70,6 +70,9 @@ _SIGS = {
                                          P, SZ, P]),
     "dlp_supernode_workspace_bytes": (SZ, [I32, I32, I32]),
     "dlp_frontier_commit": (ct.c_int, [I32, P, I32, I64, P, P, P, P, P, P, P, P, I32, P]),
+    "dlp_dense_operator_f64": (ct.c_int, [I32, P, P, P, P, I32, P, P]),
+    "dlp_spd_inverse_f64": (ct.c_int, [I32, P, P, P]),
+    "dlp_dense_apply_f64": (ct.c_int, [I32, I32, P, P, I64, P, I64, I32, P]),
     "dlp_scatter": (ct.c_int, [I32, I32, I64, I64, P, I64, P, I32, P, I64, P, P]),
     "dlp_fh_components": (ct.c_int, [I32, I64, P, P, P, DBL, P, P, P, P, SZ, P]),
     "dlp_fh_workspace_bytes": (SZ, [I32]),

@@ -4,8 +4,9 @@ Same construction and interface as dynlp/amg.py of the reference (`Level` :20-29
 :77-134): A = diag(s + rowsum(W)) - W; aggregates by heavy-edge handshake matching (two
 passes of up to three rounds per level, about 4x coarsening); W_c = off-diagonal part of
 P^T W P with duplicates summed, s_c = P^T s; stop at max_coarse rows or when a level shrinks
-by less than min_ratio; exact FP64 inverse at the coarsest level (n <= 4*max_coarse) applied
-as a library GEMM; weighted-Jacobi smoothing (omega 0.7, nu 1) in a symmetric V-cycle.
+by less than min_ratio; exact FP64 inverse at the coarsest level (n <= 4*max_coarse), formed
+by Gauss-Jordan steps and applied with FP64 accumulation on the device (the reference casts it
+to the working dtype); weighted-Jacobi smoothing (omega 0.7, nu 1) in a symmetric V-cycle.
 Matching, coarsening, restriction and prolongation are HIP kernels (csrc/amg.hip); level
 residuals R - A X are one row-operator launch each and the smoother's correction one column
 pass (csrc/spmm.hip, csrc/blas.hip).
@@ -36,7 +37,10 @@ class Level:
     def residual(self, R, X):
         """R - (diag X - W X) in one row-operator launch."""
         if self.neg_diag is None:
-            self.neg_diag = (-self.diag).contiguous()
+            d = self.diag.contiguous()
+            self.neg_diag = torch.zeros_like(d)
+            _lib.call("dlp_colop", int(d.shape[0]), 1, 0, -1.0, ptr(torch.ones(1, dtype=torch.float64, device=d.device)),
+                      ptr(d), 1, None, ptr(self.neg_diag), 1, dtype_code(d.dtype), stream())
         W, C = self.W, X.shape[1]
         out = torch.empty_like(R)
         dt = dtype_code(X.dtype)
@@ -44,6 +48,18 @@ class Level:
                   dtype_code(W.data.dtype), None, ptr(X), C, dt, ptr(self.neg_diag), 1.0, ptr(R), C, ptr(out), C, dt,
                   0, W.avg_deg, stream())
         return out
+
+
+def _diag_of(W: CSR, s):
+    """diag = s + W 1 in one row-operator launch (the row sums added to s)."""
+    n = W.shape[0]
+    dt = W.data.dtype
+    ones = torch.ones((n, 1), dtype=dt, device=device())
+    out = torch.empty((n, 1), dtype=dt, device=device())
+    _lib.call("dlp_row_operator", n, None, 1, ptr(W.indptr), None, ptr(W.indices), ptr(W.data), dtype_code(dt), None,
+              ptr(ones), 1, dtype_code(dt), None, 1.0, ptr(s.contiguous()), 1, ptr(out), 1, dtype_code(dt), 0,
+              W.avg_deg, stream())
+    return out[:, 0].contiguous()
 
 
 def _jacobi_add(omega, T, X, d):
@@ -124,9 +140,8 @@ class AMG:
         self.levels: list[Level] = []
         dt = W.data.dtype
         n = W.shape[0]
-        ones = lambda k: torch.ones(k, dtype=dt, device=device())  # noqa: E731
         s = s.to(dt).contiguous()
-        cur = Level(W, s, s + spmm_axpy(W, ones(n)[:, None])[:, 0])
+        cur = Level(W, s, _diag_of(W, s))
         while n > max_coarse and len(self.levels) < max_levels:
             agg = torch.arange(n, dtype=torch.int32, device=device())
             nc, Wp = n, cur.W
@@ -143,16 +158,17 @@ class AMG:
             sc = restrict(agg, cur.s, nc)
             cur.agg, cur.nc = agg, nc
             self.levels.append(cur)
-            cur = Level(Wc, sc, sc + spmm_axpy(Wc, ones(nc)[:, None])[:, 0])
+            cur = Level(Wc, sc, _diag_of(Wc, sc))
             n = nc
         self.coarse = cur
         self.inv = None
         if n <= 4 * max_coarse:
-            Ad = torch.zeros((n, n), dtype=torch.float64, device=device())
-            rows = cur.W.row_ids().long()
-            Ad[rows, cur.W.indices.long()] = -cur.W.data.double()
-            Ad[torch.arange(n, device=device()), torch.arange(n, device=device())] += cur.diag.double()
-            self.inv = torch.linalg.inv(Ad).to(dt)
+            # the exact coarse inverse, formed and kept in FP64 on the device
+            self.inv = torch.empty((n, n), dtype=torch.float64, device=device())
+            scratch = torch.empty_like(self.inv)
+            _lib.call("dlp_dense_operator_f64", n, ptr(cur.W.indptr), ptr(cur.W.indices), ptr(cur.W.data),
+                      ptr(cur.diag), dtype_code(dt), ptr(self.inv), stream())
+            _lib.call("dlp_spd_inverse_f64", n, ptr(self.inv), ptr(scratch), stream())
         self.work_per_apply = sum(2 * nu * L.W.nnz + 2 * L.W.shape[0] for L in self.levels)
         self.work_per_apply += n * n if self.inv is not None else coarse_sweeps * cur.W.nnz
         self.sizes = [L.W.shape[0] for L in self.levels] + [n]
@@ -170,8 +186,11 @@ class AMG:
     def _cycle(self, lvl, R):
         if lvl == len(self.levels):
             L = self.coarse
-            if self.inv is not None:
-                return self.inv @ R          # library GEMM with the exact coarse inverse
+            if self.inv is not None:        # FP64 product with the exact coarse inverse
+                X = torch.empty_like(R)
+                _lib.call("dlp_dense_apply_f64", int(R.shape[0]), int(R.shape[1]), ptr(self.inv), ptr(R), R.stride(0),
+                          ptr(X), X.stride(0), dtype_code(R.dtype), stream())
+                return X
             return self._smooth(L, self._first(L, R), R, self.coarse_sweeps - 1)
         L = self.levels[lvl]
         X = self._smooth(L, self._first(L, R), R, self.nu - 1)

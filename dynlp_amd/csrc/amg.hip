@@ -392,3 +392,121 @@ extern "C" int dlp_amg_prolong_add(int n, int C, const int32_t* agg, const void*
   DLP_CHECK_LAUNCH();
   return DLP_OK;
 }
+
+// ---------------------------------------------------------------------------------------------
+// Coarsest level in FP64: the dense operator, its inverse, and the FP64 application.
+
+namespace dlp {
+
+// A = diag(diag) - W as a dense FP64 n x n matrix (the caller zeroes A); one thread per row.
+template <typename T>
+__global__ void k_dense_operator(int n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+                                 const T* __restrict__ w, const T* __restrict__ diag, double* __restrict__ A) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double* row = A + (int64_t)r * n;
+  for (int j = indptr[r]; j < indptr[r + 1]; ++j) row[indices[j]] -= (double)w[j];
+  row[r] += (double)diag[r];
+}
+
+__global__ void k_zero_f64n(int64_t count, double* __restrict__ p) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (int64_t)gridDim.x * blockDim.x)
+    p[t] = 0.0;
+}
+
+// One Gauss-Jordan step on pivot k, out of place (every new entry from the old matrix):
+//   i = j = k: 1/p;  i = k: a_kj / p;  j = k: -a_ik / p;  else a_ij - a_ik a_kj / p,
+// with p = a_kk.  After the n steps the matrix is its inverse (no pivoting: the operator is
+// symmetric positive definite, so every pivot is positive).
+__global__ void k_gauss_jordan_step(int n, int k, const double* __restrict__ A, double* __restrict__ B) {
+  const int64_t total = (int64_t)n * n;
+  const double p = A[(int64_t)k * n + k];
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(t / n), j = (int)(t - (int64_t)i * n);
+    const double aik = A[(int64_t)i * n + k], akj = A[(int64_t)k * n + j];
+    double v;
+    if (i == k) v = (j == k) ? 1.0 / p : akj / p;
+    else if (j == k) v = -aik / p;
+    else v = A[t] - aik * akj / p;
+    B[t] = v;
+  }
+}
+
+// X[r, c] = sum_j A[r, j] R[j, c], FP64 accumulation; one wavefront per output row, its lanes
+// over the columns (rows of R read once per wavefront through the lanes).
+template <typename T>
+__global__ __launch_bounds__(256) void k_dense_apply(int n, int C, const double* __restrict__ A,
+                                                     const T* __restrict__ R, int64_t ldr, T* __restrict__ X,
+                                                     int64_t ldx) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < n; r += waves) {
+    const double* arow = A + r * n;
+    for (int c0 = 0; c0 < C; c0 += 64) {
+      const int c = c0 + lane;
+      double acc = 0.0;
+      if (c < C)
+        for (int j = 0; j < n; ++j) acc += arow[j] * (double)R[(int64_t)j * ldr + c];
+      if (c < C) X[r * ldx + c] = (T)acc;
+    }
+  }
+}
+
+}  // namespace dlp
+
+extern "C" int dlp_dense_operator_f64(int n, const int32_t* indptr, const int32_t* indices, const void* w,
+                                      const void* diag, int dtype, double* A, void* stream) {
+  DLP_CHECK_ARG(n >= 0 && n <= 16384, "dense_operator: n=%d outside [0, 16384]", n);
+  if (n == 0) return DLP_OK;
+  DLP_CHECK_ARG(indptr && indices && w && diag && A, "dense_operator: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t nn = (int64_t)n * n;
+  long long g = (long long)grid_for(nn);
+  hipLaunchKernelGGL(k_zero_f64n, dim3((unsigned)(g > 8192 ? 8192 : g)), dim3(kBlock), 0, st, nn, A);
+  if (dtype == DLP_F64)
+    hipLaunchKernelGGL(k_dense_operator<double>, dim3(grid_for(n)), dim3(kBlock), 0, st, n, indptr, indices,
+                       (const double*)w, (const double*)diag, A);
+  else
+    hipLaunchKernelGGL(k_dense_operator<float>, dim3(grid_for(n)), dim3(kBlock), 0, st, n, indptr, indices,
+                       (const float*)w, (const float*)diag, A);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}
+
+extern "C" int dlp_spd_inverse_f64(int n, double* A, double* scratch, void* stream) {
+  DLP_CHECK_ARG(n >= 0 && n <= 16384, "spd_inverse: n=%d outside [0, 16384]", n);
+  if (n == 0) return DLP_OK;
+  DLP_CHECK_ARG(A && scratch, "spd_inverse: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t nn = (int64_t)n * n;
+  long long g = (long long)grid_for(nn);
+  const dim3 grid((unsigned)(g > 8192 ? 8192 : g));
+  double* src = A;
+  double* dst = scratch;
+  for (int k = 0; k < n; ++k) {
+    hipLaunchKernelGGL(k_gauss_jordan_step, grid, dim3(kBlock), 0, st, n, k, src, dst);
+    double* t = src;
+    src = dst;
+    dst = t;
+  }
+  DLP_CHECK_LAUNCH();
+  if (src != A) DLP_CHECK_HIP(hipMemcpyAsync(A, src, sizeof(double) * nn, hipMemcpyDeviceToDevice, st));
+  return DLP_OK;
+}
+
+extern "C" int dlp_dense_apply_f64(int n, int C, const double* A, const void* R, int64_t ldr, void* X, int64_t ldx,
+                                   int dtype, void* stream) {
+  DLP_CHECK_ARG(n >= 0 && C >= 1 && ldr >= C && ldx >= C, "dense_apply: bad arguments");
+  if (n == 0) return DLP_OK;
+  DLP_CHECK_ARG(A && R && X, "dense_apply: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  long long g = (long long)grid_for((int64_t)n * 64);
+  const dim3 grid((unsigned)(g > 4096 ? 4096 : g));
+  if (dtype == DLP_F64)
+    hipLaunchKernelGGL(k_dense_apply<double>, grid, dim3(kBlock), 0, st, n, C, A, (const double*)R, ldr, (double*)X,
+                       ldx);
+  else
+    hipLaunchKernelGGL(k_dense_apply<float>, grid, dim3(kBlock), 0, st, n, C, A, (const float*)R, ldr, (float*)X, ldx);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
+}

@@ -471,21 +471,22 @@ class EdgeStreamLP:
         scores more than 2 * cert_bound apart (K > 2), |F - 0.5| > cert_bound (K = 2)."""
         C, dev = self.C, device()
         Zf = self.Z64 if self.fp64 else self.Z32
-        R = (self.R64 if self.fp64 else self.R32)[:, :C].double()
         AZ = torch.empty((self.n, C), dtype=torch.float64, device=dev)
         _lib.call("dlp_row_operator", self.n, None, C, ptr(self.indptr), ptr(self.rowlen), ptr(self.col), ptr(self.w),
                   _lib.F32, ptr(self.solved), ptr(Zf), self.ld, _lib.F64 if self.fp64 else _lib.F32, ptr(self.diag64),
                   -1.0, None, 0, ptr(AZ), C, _lib.F64, 0, self.nnz / max(self.n, 1), stream())
-        sm = self.solved.bool()
-        Rs, rhs = R[sm], (R + AZ)[sm]
-        rel = float(Rs.norm() / rhs.norm().clamp_min(1e-300))
         if not hasattr(self, "_rho_dev"):
             self._rho_dev = torch.zeros(self.C1, dtype=torch.float64, device=dev)
         Rfull = self.R64 if self.fp64 else self.R32
         _lib.call("dlp_rho_cols", self.n, None, self.C1, ptr(Rfull), self.ld, _lib.F64 if self.fp64 else _lib.F32,
                   ptr(self.diag64), ptr(self._rho_dev), stream())
         rc = self._rho_dev.cpu().numpy()
-        F = Zf[:, : self.C1][sm].double()
+        # the rest on the host, in FP64, over the solved rows
+        sm = self.solved.cpu().numpy().astype(bool)
+        R = Rfull[:, :C].cpu().numpy().astype(np.float64)[sm]
+        rhs = R + AZ.cpu().numpy()[sm]
+        rel = float(np.linalg.norm(R) / max(np.linalg.norm(rhs), 1e-300))
+        F = Zf[:, : self.C1].cpu().numpy().astype(np.float64)[sm]
         h_est = float(F[:, C].max()) if F.shape[0] else 1.0
         if rc[C] > self.eps_h:
             hmax, cert = h_est, float("inf")
@@ -495,10 +496,10 @@ class EdgeStreamLP:
         if F.shape[0] == 0:
             frac = 1.0
         elif C == 1:
-            frac = float(((F[:, 0] - 0.5).abs() > cert).double().mean())
+            frac = float((np.abs(F[:, 0] - 0.5) > cert).mean())
         else:
-            top = torch.topk(F[:, :C], 2, dim=1).values
-            frac = float(((top[:, 0] - top[:, 1]) > 2 * cert).double().mean())
+            top2 = -np.partition(-F[:, :C], 1, axis=1)[:, :2]
+            frac = float(((top2[:, 0] - top2[:, 1]) > 2 * cert).mean())
         return dict(rho=float(rc[:C].max()), rho_h=float(rc[C]), hmax=hmax, cert_bound=cert,
                     rel_l2_residual=rel, argmax_certified_frac=frac, solved_rows=int(F.shape[0]))
 
@@ -554,15 +555,13 @@ def new_components(m, lr, lc, w, tau, wmax, ground):
     1 - w/wmax over all its live entries; ground = each vertex's weight to labeled vertices."""
     from .cc import adaptive_components, connected_components, vertex_scales
     dev = device()
-    lr = torch.as_tensor(lr, device=dev).to(torch.int32)
-    lc = torch.as_tensor(lc, device=dev).to(torch.int32)
-    w = torch.as_tensor(w, device=dev)          # the system's precision
-    s = vertex_scales(m, lr, w, wmax)
-    e = torch.nonzero((w > 0) & (lc >= 0)).flatten()
+    lr, lc, w = np.asarray(lr), np.asarray(lc), np.asarray(w)    # host arrays of the stored entries
+    s = vertex_scales(m, torch.as_tensor(lr, device=dev), torch.as_tensor(w, device=dev), wmax)
+    e = (w > 0) & (lc >= 0)
     r, c, we = lr[e], lc[e], w[e]
-    heavy = torch.nonzero(we > tau).flatten()
+    heavy = we > tau
     return {"tau": connected_components(m, r[heavy], c[heavy]),
-            "fh": adaptive_components(m, r, c, we, s, s, wmax, torch.as_tensor(ground, device=dev).to(torch.float32),
+            "fh": adaptive_components(m, r, c, we, s, s, wmax, np.asarray(ground, dtype=np.float32),
                                       scale_index=(r, c))}
 
 
@@ -679,9 +678,9 @@ class Stream:
         rows = np.repeat(np.arange(is_new.shape[0]), np.diff(ip))
         on = is_new[rows]
         cols = W.indices.cpu().numpy().astype(np.int64)[on]
-        newt = torch.as_tensor(new, device=device())
-        return new_components(int(new.shape[0]), loc[rows[on]], loc[cols], W.data[torch.as_tensor(np.flatnonzero(on),
-                              device=device())], self.tau, self.wmax, B[newt].sum(dim=1))
+        ground = B.cpu().numpy()[new].sum(axis=1)    # each new vertex's weight to labeled vertices
+        return new_components(int(new.shape[0]), loc[rows[on]], loc[cols], W.data.cpu().numpy()[on], self.tau,
+                              self.wmax, ground)
 
     def batches(self):
         n = self.g.n

@@ -140,6 +140,17 @@ int dlp_amg_restrict(int n, int C, const int32_t* agg, const void* X, int64_t ld
 int dlp_amg_prolong_add(int n, int C, const int32_t* agg, const void* Xc, int64_t ldc, void* X,
                         int64_t ldx, int dtype, void* stream);
 
+/* Coarsest AMG level in FP64 on the device (amg.py:104-108, where the reference inverts with
+ * cuSOLVER and casts the inverse to the working dtype): A = diag(diag) - W as a dense FP64
+ * n x n matrix (n <= 16384); its inverse in place by Gauss-Jordan steps without pivoting (the
+ * operator is symmetric positive definite; scratch: another n x n FP64 buffer); and X = A R with
+ * FP64 accumulation for an n x C block R (FP32 or FP64 by dtype, X likewise). */
+int dlp_dense_operator_f64(int n, const int32_t* indptr, const int32_t* indices, const void* w, const void* diag,
+                           int dtype, double* A, void* stream);
+int dlp_spd_inverse_f64(int n, double* A, double* scratch, void* stream);
+int dlp_dense_apply_f64(int n, int C, const double* A, const void* R, int64_t ldr, void* X, int64_t ldx, int dtype,
+                        void* stream);
+
 /* ---- Frontier primitives (FrontierOps, kernels.py:436-492) ----------------------------- */
 /* Y[g] = (rhs[f_g] + sum_j w_j X[col_j]) / diag[f_g], changed[g] = any_c |Y[g,c]-X[f_g,c]| > delta
  * for g < nf (f = frontier); X and rhs are n x C, Y is nf x C (contiguous); adds the frontier's
