@@ -61,12 +61,29 @@ def operator_bytes(n, C1, stored, p_bytes=4):
     return 4 * (n + 1) + 4 * n + 8 * stored + n + p_bytes * n * C1 + 4 * n + 4 * n * C1
 
 
-def verify_bytes(n, C1, stored):
-    """Algorithmic bytes of the FP64 verification pass over n solved rows (each element once):
-    row offsets and lengths, stored entries (column + FP32 weight), the solved mask, the FP32
-    solution rows, the FP64 diagonal, the labeled mask and labels of the right-hand side, and
-    the FP32 residual rows written back."""
-    return 4 * (n + 1) + 4 * n + 8 * stored + n + 4 * n * C1 + 8 * n + n + 4 * n + 4 * n * C1
+def verify_bytes(n, n_solved, C1, stored_solved):
+    """Algorithmic bytes of the FP64 verification pass (each element once): row offsets and
+    lengths and the solved mask of every row, the stored entries (column + FP32 weight) of the
+    solved rows, the FP32 solution rows of every row (a solved row's own and its neighbours',
+    which cover the graph), the labeled mask and labels, and for the solved rows the FP64
+    diagonal and the FP32 residual rows written back."""
+    return 4 * (n + 1) + 4 * n + n + 8 * stored_solved + 4 * n * C1 + n + 4 * n + 8 * n_solved + 4 * n_solved * C1
+
+
+def canonical_components(n, u, v):
+    """Connected-component labels of the undirected edge list (u, v) in the reference's
+    numbering (a component's label = the rank of its minimum vertex among the component
+    minima, cc.py:8-23), by SciPy's csgraph: the independent check of the device labels."""
+    import numpy as np
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+    G = sp.coo_matrix((np.ones(u.shape[0], dtype=np.int8), (u, v)), shape=(n, n)).tocsr()
+    nc, lab = connected_components(G, directed=False)
+    first = np.full(nc, n, dtype=np.int64)
+    np.minimum.at(first, lab, np.arange(n))
+    rank = np.empty(nc, dtype=np.int64)
+    rank[np.argsort(first)] = np.arange(nc)
+    return rank[lab].astype(np.int32)
 
 
 def timed(fn, reps):
@@ -200,6 +217,85 @@ def request_row(lp, st, b, batch, g, a):
                       dtype="float32+float64", group="auto")
 
 
+# The ladder from the plain FP64 per-batch path to the service: each step changes one thing.
+LADDER = [
+    ("plain FP64 per-batch path: FP64 state, host-driven push rounds (the reference's IncrementalLP policy), "
+     "no graphs", dict(fp64=True)),
+    ("FP64 state, global column PCG (Jacobi preconditioner), no graphs", dict(fp64=True, fp64_repair="pcg")),
+    ("FP32 state and directions, column PCG (Jacobi), no graphs",
+     dict(pcg_directions="fp32", pcg_coarse=0, use_graph=False)),
+    ("FP32, column PCG (Jacobi), iterations replayed from hipGraphs", dict(pcg_directions="fp32", pcg_coarse=0)),
+    ("FP32 state, FP16 directions, column PCG (Jacobi), hipGraphs", dict(pcg_directions="fp16", pcg_coarse=0)),
+]
+
+
+def run_path(g, batches, Z0, name, kw, steps, batch_edges):
+    """Edge-update throughput of one configuration of the resident path on the same requests:
+    load the initial solution, one warm-up request, then `steps` timed requests."""
+    import torch
+    from dynlp_amd.stream import EdgeStreamLP
+    lp = EdgeStreamLP(g, eps=1e-6, **kw)
+    lp.load_state(Z0)
+    lp.apply_and_solve(batches[0])
+    lat = []
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for b in batches[1: 1 + steps]:
+        ts = time.perf_counter()
+        st = lp.apply_and_solve(b)
+        lat.append((time.perf_counter() - ts) * 1e3)
+    torch.cuda.synchronize()
+    T = time.perf_counter() - t1
+    v = len(lat) * batch_edges / T
+    log(f"{name}: {v:,.0f} edge-updates/s over {len(lat)} requests (p50 {percentile(lat, 50):.1f} ms, "
+        f"last path {st.path}, rho {st.rho:.2e})")
+    return {"path": name, "value": round(v, 1), "unit": "edge-updates/s", "requests": len(lat),
+            "p50_ms": round(percentile(lat, 50), 2), "p99_ms": round(percentile(lat, 99), 2),
+            "rho_last": st.rho}
+
+
+def compare_with_cpu(Z_gpu, Z_cpu, sysm, rho_cpu):
+    """The GPU's certified scores after a request against the CPU restatement's FP64 scores
+    after the same request (both from the same initial state): max |F_gpu - F_cpu| over the
+    solved rows and label columns, with the bound (rho_gpu + rho_cpu) * max h that two
+    solutions within rho of F* must satisfy, and the fraction of solved rows whose predicted
+    class is the same."""
+    import numpy as np
+    C, U = sysm.C, sysm.solved
+    Fg, Fc = Z_gpu[U, :C], Z_cpu[U, :C]
+    diff = float(np.abs(Fg - Fc).max()) if Fg.size else 0.0
+    hmax = float(Z_cpu[U, C].max()) if Fg.size else 1.0
+    if C == 1:
+        agree = float(np.mean((Fg[:, 0] > 0.5) == (Fc[:, 0] > 0.5))) if Fg.size else 1.0
+    else:
+        agree = float(np.mean(Fg.argmax(axis=1) == Fc.argmax(axis=1))) if Fg.size else 1.0
+    return {"max_abs_diff": diff, "bound": (1e-6 + rho_cpu) * hmax, "argmax_agree": agree,
+            "rows": int(U.sum())}
+
+
+def gpu_case_run(g, b):
+    """The reference's CPU-runnable case on the GPU: the reference API's Jacobi (ItLP, FP64) from
+    the prior to max |dX| <= 1e-6 (so rho <= 1e-6), then the batch b repaired by the resident
+    path with the push policy forced."""
+    import torch
+    from dynlp_amd.solvers import ItLP, build_system
+    from dynlp_amd.stream import EdgeStreamLP
+    sysd = build_system(g, "float64")
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    st = ItLP(g.n, g.K, "float64", delta=1e-6).solve(sysd)
+    gpu_jacobi_ms = (time.perf_counter() - t) * 1e3
+    lp = EdgeStreamLP(g, eps=1e-6, push_frac=1.0, push_max_rows=g.n, push_mass=float("inf"), push_budget=1e9)
+    lp.initial_solve()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    sb = lp.apply_and_solve(b)
+    gpu_batch_ms = (time.perf_counter() - t) * 1e3
+    log(f"GPU case: Jacobi {st.iters} sweeps {gpu_jacobi_ms:.1f} ms; batch by {sb.path} {gpu_batch_ms:.1f} ms")
+    return {"jacobi_ms": round(gpu_jacobi_ms, 2), "jacobi_sweeps": st.iters, "batch_ms": round(gpu_batch_ms, 2),
+            "batch_path": sb.path, "edge_updates_per_s": round(1000 / (gpu_batch_ms / 1e3), 1), "rho": sb.rho}
+
+
 def pmc_probe(g):
     """Launch sequence for hardware-counter runs (rocprofv3 --pmc ...): the counters of the 5
     A*P launches are read against those of one plain copy of known size."""
@@ -228,10 +324,13 @@ def main(argv=None):
     ap.add_argument("--batch-edges", type=int, default=10_000)
     ap.add_argument("--scale", type=float, default=1.0, help="multiplies n (1.0 = SBM-1M)")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--fp64-steps", type=int, default=4, help="requests timed on the plain FP64 path")
+    ap.add_argument("--fp64-steps", type=int, default=4,
+                    help="requests timed on each path of the ladder from the plain FP64 path to the service")
     ap.add_argument("--cpu-batches", type=int, default=1, help="requests timed on the CPU port")
-    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the path ladder (plain FP64 path included)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu-case", action="store_true",
+                    help="skip the ER-100k (k = 10) CPU case: Jacobi to rho <= 1e-6 and one push batch, CPU and GPU")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel roofline table")
     ap.add_argument("--out", default=None,
                     help="also append one row per timed request to this CSV in the reference's schema "
@@ -262,7 +361,7 @@ def main(argv=None):
         return pmc_probe(g)
     t = time.perf_counter()
     nreq = a.warmup + a.steps
-    batches, _ = graphs.update_stream(g, nreq, a.batch_edges, seed=a.seed)
+    batches, es_final = graphs.update_stream(g, nreq, a.batch_edges, seed=a.seed)
     log(f"update stream: {nreq} requests x {a.batch_edges} edges ({time.perf_counter() - t:.1f}s)")
 
     # ---- mixed-precision service path -----------------------------------------------------
@@ -271,8 +370,11 @@ def main(argv=None):
     st0 = lp.initial_solve()
     Z0 = lp.scores_with_h().astype(np.float32)
     log(f"initial solve: {st0.ms:.0f} ms rho={st0.rho:.2e} (setup {time.perf_counter() - t:.1f}s)")
-    for b in batches[: a.warmup]:
+    Z1 = None
+    for i, b in enumerate(batches[: a.warmup]):
         lp.apply_and_solve(b)
+        if i == 0 and not a.no_cpu:
+            Z1 = lp.scores_with_h()       # the state the CPU leg's first request is compared with
     torch.cuda.synchronize()
     lat, rhos, paths, csv_rows = [], [], {}, []
     pcg0 = lp.pcg_stats()
@@ -317,12 +419,24 @@ def main(argv=None):
     from dynlp_amd import _lib
     from dynlp_amd.backend import stream as cur_stream
     ver_ms = timed(lambda: _lib.call("dlp_lp_refresh", lp.h, 3, 0, cur_stream()), 10)
-    vbytes = verify_bytes(g.n, lp.C1, stored)
-    verify = {"kernel": "k_verify<4> (FP64 residual of every solved row, column maxima, h maximum)",
-              "bound": "hbm", "ms": round(ver_ms, 4), "algorithmic_bytes": int(vbytes),
+    ver_kernel = lp.status()["refresh_kernel"]
+    solved = lp.solved.bool()
+    n_solved = int(solved.sum().item())
+    stored_solved = int(lp.rowlen[solved].sum().item())
+    vbytes = verify_bytes(g.n, n_solved, lp.C1, stored_solved)
+    vname = {2: "k_verify<4> (FP64 residual of every solved row, column maxima, h maximum)",
+             1: "k_refresh_v4 (FP64 refresh of every row: the pipelined verification did not apply)",
+             0: "k_refresh (FP64 refresh of every row: the pipelined verification did not apply)"}[ver_kernel]
+    verify = {"kernel": vname, "bound": "hbm", "ms": round(ver_ms, 4), "algorithmic_bytes": int(vbytes),
+              "solved_rows": n_solved,
               "achieved": round(vbytes / (ver_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
               "frac": round(vbytes / (ver_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     log(f"verification pass: {ver_ms:.3f} ms, {vbytes / 1e6:.0f} MB -> {verify['achieved']:.0f} GB/s")
+    # CC labels of the final graph, bit for bit against SciPy's components in the reference's numbering
+    t = time.perf_counter()
+    u_f, v_f, _ = es_final.pairs()
+    cc_exact = bool(np.array_equal(lp.components(), canonical_components(g.n, np.r_[u_f, v_f], np.r_[v_f, u_f])))
+    log(f"CC labels of the final graph bit-exact against SciPy: {cc_exact} ({time.perf_counter() - t:.1f}s)")
     pcg_dirs, pcg_coarse = lp.pcg_directions, lp.pcg_coarse
     pcg_iters = pcg1["pcg_iters"] - pcg0["pcg_iters"]
     err = lp.error_report()
@@ -332,28 +446,18 @@ def main(argv=None):
     del lp
     torch.cuda.empty_cache()
 
-    # ---- plain FP64 per-batch path ----------------------------------------------------------
-    fp64 = None
+    # ---- the same requests on simpler paths: the plain FP64 per-batch path (the denominator of
+    # the 5x target) and the ladder from it to the service, one change at a time ------------------
+    ladder = []
     if not a.no_fp64 and a.fp64_steps > 0:
-        lp64 = EdgeStreamLP(g, eps=1e-6, fp64=True)
-        lp64.load_state(Z0)
-        lp64.apply_and_solve(batches[0])      # warm-up request
-        lat64 = []
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        for b in batches[1: 1 + a.fp64_steps]:
-            ts = time.perf_counter()
-            lp64.apply_and_solve(b)
-            lat64.append((time.perf_counter() - ts) * 1e3)
-        torch.cuda.synchronize()
-        T64 = time.perf_counter() - t1
-        v64 = len(lat64) * a.batch_edges / T64
-        fp64 = {"value": round(v64, 1), "unit": "edge-updates/s", "requests": len(lat64),
-                "p50_ms": round(percentile(lat64, 50), 2), "p99_ms": round(percentile(lat64, 99), 2),
-                "path": "FP64 Z/R, host-driven push rounds (reference IncrementalLP policy), no graphs"}
-        log(f"plain FP64 path: {v64:,.0f} edge-updates/s over {len(lat64)} requests")
-        del lp64
-        torch.cuda.empty_cache()
+        for name, kw in LADDER:
+            ladder.append(run_path(g, batches, Z0, name, kw, a.fp64_steps, a.batch_edges))
+            torch.cuda.empty_cache()
+        ladder.append({"path": "the service (above): FP32 state, FP16 directions, two-level preconditioner, "
+                               "hipGraph replay, push for small repairs", "value": round(value, 1),
+                       "unit": "edge-updates/s", "requests": a.steps, "p50_ms": round(percentile(lat, 50), 2),
+                       "p99_ms": round(percentile(lat, 99), 2)})
+    fp64 = ladder[0] if ladder else None
 
     # ---- the other hot kernels at this graph's shape ------------------------------------------
     ktab = None
@@ -363,23 +467,43 @@ def main(argv=None):
             log(f"{r['kernel']}: {r['ms']:.3f} ms -> {r['achieved']} {r['unit']} ({100 * r['frac']:.1f}% of peak)")
         torch.cuda.empty_cache()
 
-    # ---- CPU baseline: the FP64 NumPy/SciPy port on the host --------------------------------
-    cpu = None
+    # ---- CPU baseline: the FP64 NumPy/SciPy restatement on the host --------------------------
+    cpu, agreement = None, None
     if not a.no_cpu and a.cpu_batches > 0:
         from oracle.incremental import CPUIncremental
         t = time.perf_counter()
         port = CPUIncremental(g.n, g.u, g.v, g.w, g.labeled, g.y, g.K, eps=1e-6, Z=Z0)
         log(f"CPU port setup {time.perf_counter() - t:.1f}s")
         cms = []
-        for b in batches[: a.cpu_batches]:
+        for i, b in enumerate(batches[: a.cpu_batches]):
             out = port.solve_batch(b.ins_u, b.ins_v, b.ins_w, b.del_u, b.del_v)
             cms.append(out["ms"])
             log(f"CPU request: {out['ms']:.0f} ms rho={out['rho']:.2e} path={out['path']}")
+            if i == 0 and Z1 is not None:
+                agreement = compare_with_cpu(Z1, port.Z, port.sys, out["rho"])
+                log(f"GPU vs CPU after request 1: max |F_gpu - F_cpu| = {agreement['max_abs_diff']:.2e} "
+                    f"(bound {agreement['bound']:.2e}), argmax agreement {agreement['argmax_agree']:.6f}")
         cpu = {"value": round(len(cms) * a.batch_edges / (sum(cms) / 1e3), 1), "unit": "edge-updates/s",
                "cores": 1, "kind": "port",
-               "sample": f"first {len(cms)} request(s) of {a.batch_edges} edge updates on the same graph, "
-                         "FP64 NumPy/SciPy port of the same incremental algorithm (single-threaded sparse kernels)",
+               "sample": f"first {len(cms)} request(s) of {a.batch_edges} edge updates on the same graph from the same "
+                         "initial state: FP64 NumPy/SciPy restatement that updates the edge set, rebuilds the "
+                         "grounding and W_UU from it, refreshes the changed rows' residual and repairs by host push "
+                         "(few violators) or column PCG (single-threaded sparse kernels)",
                "host_cores_available": len(os.sched_getaffinity(0))}
+    case = None
+    if not a.no_cpu_case:
+        # the reference's CPU-runnable case, ER n = 100,000, degree 10, k = 10, 1 % labelled:
+        # Jacobi to rho <= 1e-6, then one batch of 1,000 edge updates by residual push
+        from oracle.incremental import cpu_case
+        gc = graphs.config_graph(1, seed=a.seed)
+        bc = graphs.update_stream(gc, 1, 1000, seed=a.seed)[0][0]
+        hc = cpu_case(gc, bc)
+        log(f"CPU case (ER-100k, k=10): Jacobi {hc['jacobi_sweeps']} sweeps {hc['jacobi_ms']:.0f} ms; batch of "
+            f"1,000 by {hc['batch_path']} {hc['batch_ms']:.0f} ms (rho {hc['rho']:.2e})")
+        case = {"workload": "ER n=100,000, degree 10, k=10, 1% labelled: Jacobi to rho <= 1e-6, then one batch "
+                            "of 1,000 edge updates (50/50) by residual push",
+                "cpu": dict(hc, cores=1, kind="port (FP64 NumPy/SciPy, single-threaded sparse kernels)"),
+                "gpu": gpu_case_run(gc, bc)}
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "edge-updates/s", "n_gpus": 1,
@@ -393,10 +517,12 @@ def main(argv=None):
         "p50_ms": round(percentile(lat, 50), 3), "p99_ms": round(percentile(lat, 99), 3),
         "rho_max": max(rhos), "paths": paths,
         "pcg_iters_per_request": round(pcg_iters / a.steps, 2), "pcg_ms_per_request": round(pcg_ms / a.steps, 3),
-        "error": err,
+        "error": err, "cc_bit_exact": cc_exact,
         "fp64_path": fp64,
         "speedup_vs_fp64_path": round(value / fp64["value"], 2) if fp64 else None,
-        "roofline": roofline, "verify_pass": verify, "cpu_baseline": cpu, "kernels": ktab,
+        "path_ladder": ladder or None,
+        "roofline": roofline, "verify_pass": verify, "cpu_baseline": cpu, "cpu_agreement": agreement,
+        "cpu_case": case, "kernels": ktab,
         "wall_s": round(time.perf_counter() - t_all, 1),
     }
     if a.out:

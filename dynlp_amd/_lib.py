@@ -61,9 +61,9 @@ _SIGS = {
     "dlp_cc_csr_static": (ct.c_int, [I32, P, P, P, P, P, P, P]),
     "dlp_cc_static_max_n": (ct.c_int, []),
     "dlp_push_rounds": (ct.c_int, [I32, I32, I64, I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, I32, P]),
-    "dlp_cg_residual": (ct.c_int, [I32, I32, I64, I32, P, P, P, DBL, P, P, I64, P, P, P, P, P]),
+    "dlp_cg_residual": (ct.c_int, [I32, I32, I64, I32, P, P, P, P, I32, P, DBL, P, P, I64, P, P, P, P, P]),
     "dlp_cg_direction": (ct.c_int, [I32, I32, I64, I32, P, P, P, P, I32, P]),
-    "dlp_cg_iteration": (ct.c_int, [I32, I32, I64, I32, P, P, P, DBL, P, P, P, P, P, P, P, P]),
+    "dlp_cg_iteration": (ct.c_int, [I32, I32, I64, I32, P, P, P, P, I32, P, DBL, P, P, P, P, P, P, P, P]),
     "dlp_park_rows": (ct.c_int, [I32, P, I32, I64, I32, P, P, P]),
     "dlp_rows_copy": (ct.c_int, [I32, P, I32, P, I64, I32, P, I64, I32, I32, P]),
     "dlp_supernode_average": (ct.c_int, [I32, P, P, I32, P, P, P, P, I32, P, I64, P, I32, I32, P, DBL, P, I64, I32,

@@ -223,9 +223,10 @@ using namespace dlp;
     DLP_CHECK_ARG(false, "bad dtype %d", dtype); \
   }
 
-extern "C" int dlp_cg_iteration(int n, int C, int64_t ld, int dtype, const int32_t* indptr, const int32_t* indices,
-                                const void* w, double avg_deg, const void* diag, void* Z, void* R, void* P, void* AP,
-                                double* scal, const double* eps, void* stream) {
+extern "C" int dlp_cg_iteration(int n, int C, int64_t ld, int dtype, const int32_t* indptr, const int32_t* rowlen,
+                                const int32_t* indices, const void* w, int w_dtype, const uint8_t* mask,
+                                double avg_deg, const void* diag, void* Z, void* R, void* P, void* AP, double* scal,
+                                const double* eps, void* stream) {
   DLP_CHECK_ARG(n >= 0 && C >= 1 && C <= 256 && ld >= C && scal && eps, "cg_iteration: bad arguments");
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(kBlock), 0, st, 1, scal + 3 * C, 0.0);
@@ -236,7 +237,7 @@ extern "C" int dlp_cg_iteration(int n, int C, int64_t ld, int dtype, const int32
   }
   DLP_CHECK_ARG(indptr && indices && w && diag && Z && R && P && AP, "cg_iteration: null pointer");
   // AP = diag * P - W P, then pAp per column
-  int rc = dlp_row_operator(n, nullptr, C, indptr, nullptr, indices, w, dtype, nullptr, P, ld, dtype, diag, -1.0,
+  int rc = dlp_row_operator(n, nullptr, C, indptr, rowlen, indices, w, w_dtype, mask, P, ld, dtype, diag, -1.0,
                             nullptr, 0, AP, ld, dtype, 0, avg_deg, stream);
   if (rc) return rc;
   rc = dlp_coldot(n, C, P, ld, AP, ld, nullptr, scal + C, dtype, stream);
@@ -265,16 +266,17 @@ extern "C" int dlp_cg_direction(int n, int C, int64_t ld, int dtype, const void*
   return DLP_OK;
 }
 
-extern "C" int dlp_cg_residual(int n, int C, int64_t ld, int dtype, const int32_t* indptr, const int32_t* indices,
-                               const void* w, double avg_deg, const void* diag, const void* rhs, int64_t ldr,
-                               const void* Z, void* R, const double* eps, double* viol, void* stream) {
+extern "C" int dlp_cg_residual(int n, int C, int64_t ld, int dtype, const int32_t* indptr, const int32_t* rowlen,
+                               const int32_t* indices, const void* w, int w_dtype, const uint8_t* mask,
+                               double avg_deg, const void* diag, const void* rhs, int64_t ldr, const void* Z, void* R,
+                               const double* eps, double* viol, void* stream) {
   DLP_CHECK_ARG(n >= 0 && C >= 1 && C <= 256 && ld >= C && ldr >= C && eps && viol, "cg_residual: bad arguments");
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(kBlock), 0, st, 1, viol, 0.0);
   DLP_CHECK_LAUNCH();
   if (n == 0) return DLP_OK;
   DLP_CHECK_ARG(indptr && indices && w && diag && rhs && Z && R, "cg_residual: null pointer");
-  int rc = dlp_row_operator(n, nullptr, C, indptr, nullptr, indices, w, dtype, nullptr, Z, ld, dtype, diag, -1.0,
+  int rc = dlp_row_operator(n, nullptr, C, indptr, rowlen, indices, w, w_dtype, mask, Z, ld, dtype, diag, -1.0,
                             nullptr, 0, R, ld, dtype, 0, avg_deg, stream);
   if (rc) return rc;
   const unsigned g = flat_grid((int64_t)n * C);

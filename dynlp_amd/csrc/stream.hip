@@ -1179,6 +1179,7 @@ struct LP {
   unsigned long long* pair_keys = nullptr;  // insertion pair table (kPairTableCap slots)
   int32_t* pair_vals = nullptr;
   int64_t dev_bytes = 0;        // device memory this handle holds (its own buffers)
+  int last_refresh_kernel = -1; // kernel of the last refresh: 0 k_refresh, 1 k_refresh_v4, 2 k_verify
   double push_ms = 0.0;         // summed duration of push-round launches (events)
   int64_t push_launched = 0;    // rounds launched
 };
@@ -1701,6 +1702,7 @@ extern "C" int dlp_lp_refresh(void* handle, int which, int prec, void* stream) {
       hipLaunchKernelGGL((k_verify<4>), dim3((unsigned)g), dim3(kBlock), 0, st, A, xcd, (int)xbytes, h->d.Z32,
                          h->d.R32);
       DLP_CHECK_LAUNCH();
+      h->last_refresh_kernel = 2;
       return DLP_OK;
     }
     which = 2;
@@ -1722,6 +1724,7 @@ extern "C" int dlp_lp_refresh(void* handle, int which, int prec, void* stream) {
     hipLaunchKernelGGL(k_refresh_v4, dim3(g4), dim3(kBlock), 0, st, A, tuning().pack ? 0 : 1, xcd, h->d.Z32,
                        h->d.R32);
     DLP_CHECK_LAUNCH();
+    h->last_refresh_kernel = 1;
     return DLP_OK;
   }
 #define K(CPLV)                                                                                    \
@@ -1734,6 +1737,7 @@ extern "C" int dlp_lp_refresh(void* handle, int which, int prec, void* stream) {
   DLP_DISPATCH_CPL(m.cpl, K)
 #undef K
   DLP_CHECK_LAUNCH();
+  h->last_refresh_kernel = 0;
   return DLP_OK;
 }
 
@@ -1908,6 +1912,7 @@ extern "C" int dlp_lp_status(void* handle, int64_t* out, double* rho, void* stre
   if (rho) std::memcpy(rho, (const void*)(h->host + kStatRho), sizeof(double) * C1);
   out[CNT_N + 7] = h->push_launched;
   out[CNT_N + 8] = (int64_t)(h->push_ms * 1000.0);  // microseconds
+  out[CNT_N + 12] = h->last_refresh_kernel;
   return DLP_OK;
 }
 

@@ -256,38 +256,43 @@ def predict(F, K):
 # Column PCG on the device.
 
 class _ColumnCG:
-    """Device workspace of one column-wise PCG solve of A Z = rhs on a batch system: R, P, AP,
-    the FP64 scalars [rz | pAp | r.z_new | violation] and the thresholds.  `residual`,
-    `direction` and `iterate` are single C-ABI calls; `violation` reads one FP64 value."""
+    """Device workspace of one column-wise PCG solve of A Z = rhs: R, P, AP, the FP64 scalars
+    [rz | pAp | r.z_new | violation] and the thresholds.  `residual`, `direction` and `iterate`
+    are single C-ABI calls; `violation` reads one FP64 value.  The operator is diag - W of a
+    batch system, or (`operator=`) any (indptr, rowlen, indices, w, mask, avg_deg, diag) tuple,
+    e.g. a resident graph's slack store restricted to its solved set."""
 
-    def __init__(self, sys: System, Z, rhs, eps):
-        self.sys, self.Z, self.rhs = sys, Z, rhs
+    def __init__(self, sys: System, Z, rhs, eps, operator=None):
+        self.Z, self.rhs = Z, rhs
         self.n, self.C = int(Z.shape[0]), int(Z.shape[1])
+        self.ld = int(Z.stride(0))
         self.dt = dtype_code(Z.dtype)
-        self.R = torch.empty_like(Z)
-        self.P = torch.empty_like(Z)
-        self.AP = torch.empty_like(Z)
+        block = lambda: torch.empty((self.n, self.ld), dtype=Z.dtype, device=Z.device)[:, : self.C]  # noqa: E731
+        self.R, self.P, self.AP = block(), block(), block()
         self.scal = torch.zeros(3 * self.C + 1, dtype=torch.float64, device=Z.device)
         self.eps = _f64(np.broadcast_to(np.asarray(eps, dtype=np.float64), (self.C,)))
-        W = sys.W
-        self._op = (ptr(W.indptr), ptr(W.indices), ptr(W.data), float(W.avg_deg), ptr(sys.diag))
+        if operator is None:
+            W = sys.W
+            operator = (W.indptr, None, W.indices, W.data, None, float(W.avg_deg), sys.diag)
+        ip, rl, ix, w, mask, deg, d = operator
+        self._op = (ptr(ip), ptr(rl), ptr(ix), ptr(w), dtype_code(w.dtype), ptr(mask), float(deg), ptr(d))
 
     def residual(self):
         """R = rhs - A Z; returns its violation max |R|/(eps diag)."""
-        ip, ix, w, deg, d = self._op
-        _lib.call("dlp_cg_residual", self.n, self.C, self.C, self.dt, ip, ix, w, deg, d, ptr(self.rhs),
-                  self.rhs.stride(0), ptr(self.Z), ptr(self.R), ptr(self.eps), ptr(self.scal) + 24 * self.C,
-                  stream())
+        ip, rl, ix, w, wdt, mask, deg, d = self._op
+        _lib.call("dlp_cg_residual", self.n, self.C, self.ld, self.dt, ip, rl, ix, w, wdt, mask, deg, d,
+                  ptr(self.rhs), self.rhs.stride(0), ptr(self.Z), ptr(self.R), ptr(self.eps),
+                  ptr(self.scal) + 24 * self.C, stream())
         return self.violation()
 
     def direction(self, Y, first):
-        _lib.call("dlp_cg_direction", self.n, self.C, self.C, self.dt, ptr(self.R), ptr(Y), ptr(self.P),
+        _lib.call("dlp_cg_direction", self.n, self.C, self.ld, self.dt, ptr(self.R), ptr(Y), ptr(self.P),
                   ptr(self.scal), int(first), stream())
 
     def iterate(self):
-        ip, ix, w, deg, d = self._op
-        _lib.call("dlp_cg_iteration", self.n, self.C, self.C, self.dt, ip, ix, w, deg, d, ptr(self.Z), ptr(self.R),
-                  ptr(self.P), ptr(self.AP), ptr(self.scal), ptr(self.eps), stream())
+        ip, rl, ix, w, wdt, mask, deg, d = self._op
+        _lib.call("dlp_cg_iteration", self.n, self.C, self.ld, self.dt, ip, rl, ix, w, wdt, mask, deg, d, ptr(self.Z),
+                  ptr(self.R), ptr(self.P), ptr(self.AP), ptr(self.scal), ptr(self.eps), stream())
 
     def violation(self):
         return float(self.scal[3 * self.C].item())
@@ -308,35 +313,37 @@ class _Progress:
         return self.flat >= self.patience
 
 
-def pcg(sys: System, Z, rhs, eps, M, max_iter, st: Stats, work_M=0, patience=400):
-    """Column-wise preconditioned CG (solvers.py:66-97) to max |R_uc|/(eps_c diag_u) <= 1, with
-    the true residual confirming convergence and a stall exit after `patience` iterations
-    without a 10 % gain.  Z (n_u x C, row-major) is updated in place and returned."""
-    Z = Z.contiguous()
-    cg = _ColumnCG(sys, Z, rhs.contiguous(), eps)
-    st.edges += sys.nnz
+def cg_solve(cg: _ColumnCG, M, max_iter, st: Stats, nnz, work_M=0, patience=400):
+    """The column PCG loop on a prepared workspace: to max |R_uc|/(eps_c diag_u) <= 1, the true
+    residual confirming convergence, a stall exit after `patience` iterations without a 10 %
+    gain.  M(R) returns the preconditioned residual.  Updates cg.Z in place."""
+    st.edges += nnz
     v = cg.residual()
     if v <= 1:
-        return Z
+        return
     cg.direction(M(cg.R), first=True)
     track = _Progress(v, 0.9, patience)
     for _ in range(max_iter):
         cg.iterate()
-        st.iters, st.edges = st.iters + 1, st.edges + sys.nnz + work_M
+        st.iters, st.edges = st.iters + 1, st.edges + nnz + work_M
         v = cg.violation()
         if v <= 1:
-            st.edges += sys.nnz
+            st.edges += nnz
             v = cg.residual()              # the recurrence converged: check the true residual
             if v <= 1:
                 break
         if track.stalled(v):
             break
         cg.direction(M(cg.R), first=False)
+
+
+def pcg(sys: System, Z, rhs, eps, M, max_iter, st: Stats, work_M=0, patience=400):
+    """Column-wise preconditioned CG (solvers.py:66-97) of A Z = rhs on a batch system, on the
+    device (`cg_solve`).  Z (n_u x C, row-major) is updated in place and returned."""
+    Z = Z.contiguous()
+    cg_solve(_ColumnCG(sys, Z, rhs.contiguous(), eps), M, max_iter, st, sys.nnz, work_M, patience)
     return Z
 
-
-# ---------------------------------------------------------------------------------------------
-# Residual push on the device.
 
 def push(sys: System, Z, R, eps, ops: FrontierOps, st: Stats, budget_edges=None, max_iter=10**7):
     """Residual push on the violating set (solvers.py:100-125) in device rounds (FusedPush):

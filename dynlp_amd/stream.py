@@ -83,7 +83,7 @@ class EdgeStreamLP:
                  slack_frac=0.25, push_scale=0.9, rounds_per_launch=4, use_graph=True, max_rounds=200_000,
                  push_frac=0.05, push_max_rows=4096, push_budget=1.0, push_mass=0.15, pcg_iters_per_launch=7,
                  pcg_max_iter=20_000, pcg_mode="column", incremental_ground=True, pcg_directions="fp16", pcg_coarse=128,
-                 coarse_refresh=64):
+                 coarse_refresh=64, fp64_repair="push"):
         self.g, self.n, self.K = g, g.n, g.K
         self.C = 1 if g.K == 2 else g.K
         self.C1 = self.C + 1
@@ -97,6 +97,9 @@ class EdgeStreamLP:
             raise ValueError(f"pcg_mode {pcg_mode!r}: 'column' or 'block'")
         self.pcg_mode = pcg_mode
         self.incremental_ground = bool(incremental_ground)
+        if fp64_repair not in ("push", "pcg"):
+            raise ValueError(f"fp64_repair {fp64_repair!r}: 'push' or 'pcg'")
+        self.fp64_repair = fp64_repair
         dev = device()
         ip, ix, d = g.csr()
         sp_, rl, col, w = slack_layout(ip, ix, d, slack_min, slack_frac)
@@ -145,7 +148,7 @@ class EdgeStreamLP:
         if not self.fp64:
             # every buffer of the global solve now, so that no request allocates device memory
             _lib.call("dlp_lp_reserve", self.h, int(pcg_iters_per_launch), int(pcg_mode == "block"), stream())
-        self._status = np.zeros(18, dtype=np.int64)
+        self._status = np.zeros(19, dtype=np.int64)
         self._rho = np.zeros(self.C1, dtype=np.float64)
         self.batches = 0
         self.scratch3 = None
@@ -205,7 +208,8 @@ class EdgeStreamLP:
                     touched=int(s[6]), round=int(s[7]), edges=int(s[8]), cand_total=int(s[9]),
                     push_total=int(s[10]), scatter_total=int(s[11]),
                     rounds_launched=int(s[13]), push_us=int(s[14]), mass=float(s[15:16].view(np.float64)[0]),
-                    hmax=float(s[12:13].view(np.float64)[0]), invalid=int(s[16]), duplicates=int(s[17]))
+                    hmax=float(s[12:13].view(np.float64)[0]), invalid=int(s[16]), duplicates=int(s[17]),
+                    refresh_kernel=int(s[18]))
 
     def _push(self, budget=0):
         r, ok = ct.c_int64(0), ct.c_int32(0)
@@ -249,12 +253,38 @@ class EdgeStreamLP:
             rounds += 1
         return rounds
 
+    def _pcg64(self):
+        """The plain FP64 path's global repair: column PCG (Jacobi preconditioner, no graphs) on
+        the correction A D = R64 over the solved rows, then Z64 += D; one FP64 violation read
+        per iteration.  Returns the iterations."""
+        from .solvers import Stats as _S, _ColumnCG, cg_solve
+        n, C1, ld, dev = self.n, self.C1, self.ld, device()
+        D = torch.zeros((n, ld), dtype=torch.float64, device=dev)[:, :C1]
+        eps = np.array([self.eps] * self.C + [self.eps_h])
+        op = (self.indptr, self.rowlen, self.col, self.w, self.solved, self.nnz / max(n, 1), self.diag64)
+        cg = _ColumnCG(None, D, self.R64[:, :C1], eps, operator=op)
+
+        def jacobi(R):
+            Y = torch.empty((n, ld), dtype=torch.float64, device=dev)[:, :C1]
+            _lib.call("dlp_colop", n, C1, 2, 1.0, None, ptr(R), ld, ptr(self.diag64), ptr(Y), ld, _lib.F64, stream())
+            return Y
+
+        st = _S()
+        cg_solve(cg, jacobi, self.pcg_max_iter, st, self.nnz)
+        ones = torch.ones(C1, dtype=torch.float64, device=dev)
+        _lib.call("dlp_colop", n, C1, 0, 1.0, ptr(ones), ptr(D), ld, None, ptr(self.Z64), ld, _lib.F64, stream())
+        return st.iters
+
     def _repair(self, st: Stats, force_global=False):
         """Mixed path (DynLP+ policy, solvers.py:272-288 with refinement :322-337): push while
         the violating set is small and within the edge budget, else the global PCG; then FP64
         verification of every row the repair touched, pushing again until it certifies.
         FP64 path: the reference's incremental push to completion (solvers.py:375-389)."""
         if self.fp64:
+            if self.fp64_repair == "pcg":
+                st.path = "fp64-pcg"
+                st.extra["pcg_iters"] = st.extra.get("pcg_iters", 0) + self._pcg64()
+                _lib.call("dlp_lp_refresh", self.h, 2, 1, stream())   # FP64 residual of every row
             st.iters += self._push64()
             s = self.status()
             # the FP64 kept residual is exact up to rounding: read its column maxima

@@ -196,21 +196,26 @@ int dlp_push_rounds(int n, int C, int64_t ld, int dtype, const int32_t* indptr, 
                     int rounds, void* stream);
 
 /* ---- Column-wise PCG on a batch system (solvers.pcg, solvers.py:66-97) -------------------
- * Z, R, P, AP (and Y) are n x C with leading dimension ld, dtype FP32/FP64 (with w, diag, rhs);
- * scal: device FP64[3C + 1] = [rz | pAp | r.z of the new direction | violation]; eps: device
- * FP64[C].  dlp_cg_residual: R = rhs - (diag Z - W Z) and scal-style violation
- * *viol = max |R_uc| / (eps_c diag_u) (NaN counts as +inf).  dlp_cg_direction: r.z_new = R.Y per
- * column, then P = Y (first != 0) or Y + beta P with beta = r.z_new / rz (0 when rz <= 0), and rz
- * = r.z_new.  dlp_cg_iteration: AP = diag P - W P, pAp = P.AP, alpha = rz/pAp (0 when pAp <= 0),
- * Z += alpha P, R -= alpha AP, violation of the new R in scal[3C].  No host synchronisation. */
-int dlp_cg_residual(int n, int C, int64_t ld, int dtype, const int32_t* indptr, const int32_t* indices,
-                    const void* w, double avg_deg, const void* diag, const void* rhs, int64_t ldr, const void* Z,
-                    void* R, const double* eps, double* viol, void* stream);
+ * Z, R, P, AP (and Y) are n x C with leading dimension ld, dtype FP32/FP64 (with diag, rhs);
+ * the operator is diag - W over a (slack, rowlen non-NULL) CSR with weights of w_dtype
+ * (FP32 weights with FP64 state allowed), restricted to mask != 0 rows and columns when mask
+ * is non-NULL (the W_UU of the handle's solved set); scal: device FP64[3C + 1] = [rz | pAp |
+ * r.z of the new direction | violation]; eps: device FP64[C].  dlp_cg_residual: R = rhs -
+ * (diag Z - W Z) and *viol = max |R_uc| / (eps_c diag_u) (NaN counts as +inf).
+ * dlp_cg_direction: r.z_new = R.Y per column, then P = Y (first != 0) or Y + beta P with beta =
+ * r.z_new / rz (0 when rz <= 0), and rz = r.z_new.  dlp_cg_iteration: AP = diag P - W P, pAp =
+ * P.AP, alpha = rz/pAp (0 when pAp <= 0), Z += alpha P, R -= alpha AP, violation of the new R in
+ * scal[3C].  No host synchronisation. */
+int dlp_cg_residual(int n, int C, int64_t ld, int dtype, const int32_t* indptr, const int32_t* rowlen,
+                    const int32_t* indices, const void* w, int w_dtype, const uint8_t* mask, double avg_deg,
+                    const void* diag, const void* rhs, int64_t ldr, const void* Z, void* R, const double* eps,
+                    double* viol, void* stream);
 int dlp_cg_direction(int n, int C, int64_t ld, int dtype, const void* R, const void* Y, void* P, double* scal,
                      int first, void* stream);
-int dlp_cg_iteration(int n, int C, int64_t ld, int dtype, const int32_t* indptr, const int32_t* indices,
-                     const void* w, double avg_deg, const void* diag, void* Z, void* R, void* P, void* AP,
-                     double* scal, const double* eps, void* stream);
+int dlp_cg_iteration(int n, int C, int64_t ld, int dtype, const int32_t* indptr, const int32_t* rowlen,
+                     const int32_t* indices, const void* w, int w_dtype, const uint8_t* mask, double avg_deg,
+                     const void* diag, void* Z, void* R, void* P, void* AP, double* scal, const double* eps,
+                     void* stream);
 
 /* Rows with solved[r] == 0 take the values of an inert row: Z = (0, .., 0, 1) over C1 columns
  * (h last), R = 0 (R may be NULL).  IncrementalLP._park (solvers.py:365-368). */
@@ -489,9 +494,11 @@ int dlp_lp_violators(void* handle, int32_t** list, int32_t** count);
  * [15] the bits of an FP64 "push mass": over the rows the last changed-row refresh found
  * violating, the sum of max_c |R_c|/(eps_c*diag) (about the row pushes a repair by push alone
  * would take), [16] pairs of this batch refused by dlp_lp_apply (ids outside [0, n) or
- * insertion weights not finite and positive), [17] repeated insertion entries it merged;
+ * insertion weights not finite and positive), [17] earlier occurrences of pairs repeated in its
+ * insertions (not applied), [18] the kernel of the last dlp_lp_refresh (0 the generic refresh,
+ * 1 the 16-byte-row refresh, 2 the pipelined verification of mode 3; -1 none yet);
  * rho (C1, may be NULL): column maxima of the last refresh, in which a NaN residual counts as
- * +inf (so it never certifies).  `out` holds 18 entries. */
+ * +inf (so it never certifies).  `out` holds 19 entries. */
 int dlp_lp_status(void* handle, int64_t* out, double* rho, void* stream);
 
 #ifdef __cplusplus

@@ -85,3 +85,33 @@ class CPUIncremental:
             self.Z, self.R, ok, rounds, _ = O.push(sysm.W, sysm.diag, self.Z, self.R, self.epsv)
         rho = O.rho_cols(sysm.diag, self.R)[:C].max()
         return dict(ms=(time.perf_counter() - t0) * 1e3, rho=float(rho), path=path)
+
+
+def cpu_case(g, b, eps=1e-6):
+    """The reference's CPU-runnable case on the host: Jacobi sweeps X = (rhs + W X) / diag from
+    the prior until max |dX| <= eps (so rho <= eps), h by CG to its own threshold 0.05, then the
+    batch b repaired by host push (solvers.py:100-125) on the rebuilt system.  Timed pieces in
+    ms, single-threaded SciPy."""
+    sysm = O.build_system(g.n, g.u, g.v, g.w, g.labeled, g.y, g.K)
+    Wl, dl, rl = sysm.local()
+    t = time.perf_counter()
+    X = np.full_like(rl, 1.0 / g.K)
+    sweeps = 0
+    while True:
+        Y = (rl + Wl @ X) / dl[:, None]
+        moved = float(np.abs(Y - X).max())
+        X, sweeps = Y, sweeps + 1
+        if moved <= eps:
+            break
+    jacobi_ms = (time.perf_counter() - t) * 1e3
+    hc = dl[:, None].copy()
+    H, _ = O.pcg(Wl, dl, np.zeros_like(hc), hc, [0.05])
+    Z = np.zeros((g.n, sysm.C + 1))
+    Z[:, sysm.C] = 1.0
+    Z[sysm.U, : sysm.C] = X
+    Z[sysm.U, sysm.C] = H[:, 0]
+    port = CPUIncremental(g.n, g.u, g.v, g.w, g.labeled, g.y, g.K, eps=eps, Z=Z, push_max_rows=g.n)
+    out = port.solve_batch(b.ins_u, b.ins_v, b.ins_w, b.del_u, b.del_v)
+    return {"jacobi_ms": round(jacobi_ms, 1), "jacobi_sweeps": sweeps, "batch_ms": round(out["ms"], 1),
+            "batch_path": out["path"], "edge_updates_per_s": round(1000 / (out["ms"] / 1e3), 1),
+            "rho": out["rho"]}

@@ -599,3 +599,23 @@ def test_footprint_is_what_the_graph_holds(kw):
     b = graphs.update_stream(g, 1, 2000, seed=2)[0][0]
     lp.apply_and_solve(b)
     assert lp.device_bytes() == hb
+
+
+@pytest.mark.parametrize("K", [2, 8])
+def test_fp64_pcg_repair_matches_rebuilt_system(K):
+    """The plain FP64 path with its global repair by FP64 column PCG (no graphs, Jacobi
+    preconditioner) instead of push rounds: every batch against the oracle."""
+    from dynlp_amd.stream import EdgeStreamLP
+    g = graphs.sbm(3000, 8, 10, 0.8, K, 0.02, seed=5)
+    eps = 1e-6
+    lp = EdgeStreamLP(g, eps=eps, fp64=True, fp64_repair="pcg")
+    lp.initial_solve()
+    es = edge_dict(g)
+    check_against_oracle(lp, g, es, eps, "initial")
+    batches, _ = graphs.update_stream(g, 3, 400, seed=5)
+    for t, b in enumerate(batches):
+        st = lp.apply_and_solve(b)
+        assert st.path == "fp64-pcg" and st.extra["pcg_iters"] > 0
+        O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
+                           zip(b.del_u.tolist(), b.del_v.tolist()))
+        check_against_oracle(lp, g, es, eps, f"batch {t}")
