@@ -58,6 +58,9 @@ _SIGS = {
     "dlp_lp_pcg": (ct.c_int, [P, I32, I32, I32, I32, P, P, P]),
     "dlp_lp_pcg_stats": (ct.c_int, [P, P]),
     "dlp_lp_reserve": (ct.c_int, [P, I32, I32, P]),
+    "dlp_push_plan_create": (ct.c_int, [P, I32, P]),
+    "dlp_push_plan_destroy": (ct.c_int, [P]),
+    "dlp_lp_push_multi": (ct.c_int, [P, P, I32, I32, I32, P, P, P, P]),
     "dlp_cc_csr_static": (ct.c_int, [I32, P, P, P, P, P, P, P]),
     "dlp_cc_static_max_n": (ct.c_int, []),
     "dlp_push_rounds": (ct.c_int, [I32, I32, I64, I32, P, P, P, P, P, P, P, P, P, P, P, P, P, P, I32, P]),

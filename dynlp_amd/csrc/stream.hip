@@ -917,7 +917,7 @@ __device__ __forceinline__ void wave_append(bool pred, int val, int32_t* __restr
 }
 
 template <int CPL>
-__global__ __launch_bounds__(256) void k_push_round(PushState S, int parity, int g_shift, int lc_shift) {
+__device__ __forceinline__ void push_round_body(const PushState& S, int parity, int g_shift, int lc_shift) {
   const Lanes L(g_shift, lc_shift);
   const int t = S.rnd[parity];
   const int nc = S.count[t % 3];
@@ -1018,6 +1018,34 @@ __global__ __launch_bounds__(256) void k_push_round(PushState S, int parity, int
       atomicAdd(S.edges, ss);
     }
   }
+}
+
+template <int CPL>
+__global__ __launch_bounds__(256) void k_push_round(PushState S, int parity, int g_shift, int lc_shift) {
+  push_round_body<CPL>(S, parity, g_shift, lc_shift);
+}
+
+// The same round for several graphs in one launch: graph blockIdx.y takes gridDim.x blocks
+// (its lane layout from gshift/lcshift); graphs with active[y] == 0 are left alone.
+template <int CPL>
+__global__ __launch_bounds__(256) void k_push_round_multi(const PushState* __restrict__ states,
+                                                          const int32_t* __restrict__ gshift,
+                                                          const int32_t* __restrict__ lcshift,
+                                                          const int32_t* __restrict__ active, int parity) {
+  const int y = blockIdx.y;
+  if (!active[y]) return;
+  const PushState S = states[y];
+  push_round_body<CPL>(S, parity, gshift[y], lcshift[y]);
+}
+
+// done flag and edge count of every graph of a multi push: out[2y] = done, out[2y+1] = edges
+// (a graph outside the call has a zeroed descriptor and reads as done).
+__global__ void k_push_probe_multi(const PushState* __restrict__ states, int ns, int64_t* __restrict__ out) {
+  const int y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y >= ns) return;
+  const PushState S = states[y];
+  out[2 * y] = S.done ? (int64_t)*S.done : 1;
+  out[2 * y + 1] = S.edges ? (int64_t)*S.edges : 0;
 }
 
 // New push phase: the current violators become the candidate list of round t0 = rnd[0].
@@ -1829,6 +1857,237 @@ extern "C" int dlp_lp_push(void* handle, int max_rounds, int rounds_per_launch, 
   }
   if (rounds_out) *rounds_out = rounds;
   if (converged_out) *converged_out = converged;
+  return DLP_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// One push launch across graphs.  A plan holds the graphs' push descriptors (re-uploaded at
+// every call, so a regrown store needs no new plan) grouped by column-chunk class, and a
+// captured hipGraph of rounds in which each round is one launch per class covering every
+// graph of the class (grid x = blocks per graph, y = graph).
+
+namespace {
+struct PushPlan {
+  std::vector<LP*> hs;
+  struct Class {
+    int cpl = 0;
+    std::vector<int> members;   // plan indices
+    PushState* states = nullptr;
+    int32_t* gshift = nullptr;
+    int32_t* lcshift = nullptr;
+    int32_t* active = nullptr;
+  };
+  std::vector<Class> classes;
+  PushState* all = nullptr;     // every graph in plan order (probe)
+  int64_t* probe = nullptr;
+  int64_t* probe_host = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_rounds = 0;
+  unsigned gx = 512;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+int plan_free(PushPlan* p) {
+  for (auto& c : p->classes) {
+    if (c.states) (void)hipFree(c.states);
+    if (c.gshift) (void)hipFree(c.gshift);
+    if (c.lcshift) (void)hipFree(c.lcshift);
+    if (c.active) (void)hipFree(c.active);
+  }
+  if (p->all) (void)hipFree(p->all);
+  if (p->probe) (void)hipFree(p->probe);
+  if (p->probe_host) (void)hipHostFree(p->probe_host);
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->ev0) (void)hipEventDestroy(p->ev0);
+  if (p->ev1) (void)hipEventDestroy(p->ev1);
+  delete p;
+  return DLP_OK;
+}
+
+void launch_multi_round(PushPlan* p, int parity, hipStream_t st) {
+  for (auto& c : p->classes) {
+    const dim3 grid(p->gx, (unsigned)c.members.size());
+#define K(CPLV)                                                                                          \
+  hipLaunchKernelGGL((k_push_round_multi<CPLV>), grid, dim3(kBlock), 0, st, c.states, c.gshift, c.lcshift, \
+                     c.active, parity);
+    DLP_DISPATCH_CPL(c.cpl, K)
+#undef K
+  }
+}
+}  // namespace
+
+extern "C" int dlp_push_plan_create(void* const* handles, int nh, void** plan) {
+  DLP_CHECK_ARG(handles && plan && nh >= 1 && nh <= 4096, "push_plan_create: bad argument");
+  PushPlan* p = new PushPlan();
+  for (int i = 0; i < nh; ++i) {
+    if (!handles[i]) {
+      delete p;
+      DLP_CHECK_ARG(false, "push_plan_create: null handle %d", i);
+    }
+    p->hs.push_back((LP*)handles[i]);
+  }
+  for (int i = 0; i < nh; ++i) {
+    const int cpl = p->hs[i]->map.cpl;
+    PushPlan::Class* c = nullptr;
+    for (auto& k : p->classes)
+      if (k.cpl == cpl) c = &k;
+    if (!c) {
+      p->classes.emplace_back();
+      c = &p->classes.back();
+      c->cpl = cpl;
+    }
+    c->members.push_back(i);
+  }
+  bool ok = true;
+  for (auto& c : p->classes) {
+    const size_t k = c.members.size();
+    ok = ok && hipMalloc((void**)&c.states, sizeof(PushState) * k) == hipSuccess &&
+         hipMalloc((void**)&c.gshift, sizeof(int32_t) * k) == hipSuccess &&
+         hipMalloc((void**)&c.lcshift, sizeof(int32_t) * k) == hipSuccess &&
+         hipMalloc((void**)&c.active, sizeof(int32_t) * k) == hipSuccess;
+    if (!ok) break;
+    std::vector<int32_t> gs(k), ls(k);
+    for (size_t j = 0; j < k; ++j) {
+      gs[j] = p->hs[c.members[j]]->map.g_shift;
+      ls[j] = p->hs[c.members[j]]->map.lc_shift;
+    }
+    ok = hipMemcpy(c.gshift, gs.data(), sizeof(int32_t) * k, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(c.lcshift, ls.data(), sizeof(int32_t) * k, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemset(c.active, 0, sizeof(int32_t) * k) == hipSuccess;
+    if (!ok) break;
+  }
+  ok = ok && hipMalloc((void**)&p->all, sizeof(PushState) * nh) == hipSuccess &&
+       hipMalloc((void**)&p->probe, sizeof(int64_t) * 2 * nh) == hipSuccess &&
+       hipHostMalloc((void**)&p->probe_host, sizeof(int64_t) * 2 * nh) == hipSuccess &&
+       hipEventCreate(&p->ev0) == hipSuccess && hipEventCreate(&p->ev1) == hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    plan_free(p);
+    set_error("push_plan_create: device allocation failed");
+    return DLP_ERR_HIP;
+  }
+  *plan = p;
+  return DLP_OK;
+}
+
+extern "C" int dlp_push_plan_destroy(void* plan) {
+  if (!plan) return DLP_OK;
+  (void)hipDeviceSynchronize();
+  return plan_free((PushPlan*)plan);
+}
+
+// Push rounds for the graphs with active[i] != 0 (each from its last refresh's violators), one
+// launch per round and column class for all of them, until each converges, exceeds its edge
+// budget or max_rounds pass.  Synchronises once per launch of rounds_per_launch rounds.
+extern "C" int dlp_lp_push_multi(void* plan, const int32_t* active, int max_rounds, int rounds_per_launch,
+                                 int use_graph, const int64_t* budget_edges, int64_t* rounds_out,
+                                 int32_t* converged_out, void* stream) {
+  DLP_CHECK_ARG(plan && active && max_rounds >= 0 && rounds_per_launch >= 2 && (rounds_per_launch % 2) == 0,
+                "lp_push_multi: bad argument (rounds_per_launch must be even)");
+  PushPlan* p = (PushPlan*)plan;
+  hipStream_t st = (hipStream_t)stream;
+  const int nh = (int)p->hs.size();
+  std::vector<int32_t> on(active, active + nh);
+  // graphs outside this call are not touched at all (another thread may be using them)
+  std::vector<PushState> all(nh);
+  for (int i = 0; i < nh; ++i) {
+    if (on[i]) all[i] = p->hs[i]->S;
+    else std::memset(&all[i], 0, sizeof(PushState));
+    if (rounds_out) rounds_out[i] = 0;
+    if (converged_out) converged_out[i] = on[i] ? 0 : 1;
+  }
+  DLP_CHECK_HIP(hipMemcpyAsync(p->all, all.data(), sizeof(PushState) * nh, hipMemcpyHostToDevice, st));
+  auto upload_active = [&]() -> int {
+    for (auto& c : p->classes) {
+      std::vector<PushState> cs(c.members.size());
+      std::vector<int32_t> ca(c.members.size());
+      for (size_t j = 0; j < c.members.size(); ++j) {
+        cs[j] = all[c.members[j]];
+        ca[j] = on[c.members[j]] ? 1 : 0;
+      }
+      DLP_CHECK_HIP(hipMemcpyAsync(c.states, cs.data(), sizeof(PushState) * cs.size(), hipMemcpyHostToDevice, st));
+      DLP_CHECK_HIP(hipMemcpyAsync(c.active, ca.data(), sizeof(int32_t) * ca.size(), hipMemcpyHostToDevice, st));
+      // the host vectors go out of scope: wait for the copies
+      DLP_CHECK_HIP(hipStreamSynchronize(st));
+    }
+    return DLP_OK;
+  };
+  int left = 0;
+  for (int i = 0; i < nh; ++i) {
+    if (!on[i]) continue;
+    LP* h = p->hs[i];
+    hipLaunchKernelGGL(k_push_begin, dim3(256), dim3(kBlock), 0, st, h->S, h->viol, h->cnt + CNT_VIOL);
+    ++left;
+  }
+  DLP_CHECK_LAUNCH();
+  int rc = upload_active();
+  if (rc) return rc;
+  if (use_graph && (!p->exec || p->graph_rounds != rounds_per_launch)) {
+    if (p->exec) {
+      (void)hipGraphExecDestroy(p->exec);
+      p->exec = nullptr;
+    }
+    hipStream_t cs;
+    DLP_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t graph;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      for (int r = 0; r < rounds_per_launch; ++r) launch_multi_round(p, r & 1, cs);
+      e = hipStreamEndCapture(cs, &graph);
+    }
+    if (e == hipSuccess) {
+      e = hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+    }
+    (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) {
+      p->exec = nullptr;
+      set_error("multi push graph capture failed: %s", hipGetErrorString(e));
+      return DLP_ERR_HIP;
+    }
+    p->graph_rounds = rounds_per_launch;
+  }
+  int64_t rounds = 0;
+  while (left > 0 && rounds < max_rounds) {
+    DLP_CHECK_HIP(hipEventRecord(p->ev0, st));
+    if (use_graph) {
+      DLP_CHECK_HIP(hipGraphLaunch(p->exec, st));
+    } else {
+      for (int r = 0; r < rounds_per_launch; ++r) launch_multi_round(p, r & 1, st);
+      DLP_CHECK_LAUNCH();
+    }
+    rounds += rounds_per_launch;
+    DLP_CHECK_HIP(hipEventRecord(p->ev1, st));
+    hipLaunchKernelGGL(k_push_probe_multi, dim3(grid_for(nh)), dim3(kBlock), 0, st, p->all, nh, p->probe);
+    DLP_CHECK_LAUNCH();
+    DLP_CHECK_HIP(hipMemcpyAsync(p->probe_host, p->probe, sizeof(int64_t) * 2 * nh, hipMemcpyDeviceToHost, st));
+    DLP_CHECK_HIP(hipStreamSynchronize(st));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, p->ev0, p->ev1);
+    bool changed = false;
+    const volatile int64_t* hp = p->probe_host;
+    for (int i = 0; i < nh; ++i) {
+      if (!on[i]) continue;
+      LP* h = p->hs[i];
+      h->push_ms += ms;
+      h->push_launched += rounds_per_launch;
+      if (rounds_out) rounds_out[i] = rounds;
+      if (hp[2 * i]) {
+        if (converged_out) converged_out[i] = 1;
+        on[i] = 0;
+      } else if (budget_edges && budget_edges[i] > 0 && hp[2 * i + 1] > budget_edges[i]) {
+        on[i] = 0;
+      } else {
+        continue;
+      }
+      changed = true;
+      --left;
+    }
+    if (changed && left > 0) {
+      rc = upload_active();
+      if (rc) return rc;
+    }
+  }
   return DLP_OK;
 }
 

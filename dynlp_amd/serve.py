@@ -13,8 +13,14 @@ the serving layer the workload asks for:
   insertions fit one device apply call, 2^19) and fuses them into one batch (per pair, the
   last operation wins, which is what applying them in arrival order gives), so requests that
   arrive while the graph is busy are served by its next repair.
-- Overlap.  `slots` workers run concurrently, each on its own HIP stream (default priority);
-  the C ABI releases the GIL, so repairs of different graphs overlap on the device.
+- Fused push and overlap.  One scheduler thread takes, per cycle, the queued requests of every
+  idle resident graph and runs the first half of each repair (apply, re-ground, changed-row
+  refresh).  The graphs whose change a push can absorb are then repaired by ONE push across
+  them (dlp_lp_push_multi: per round one launch per column class covering every such graph,
+  replayed from a captured hipGraph), and verified; the others (large changes, or a push that
+  spent its edge budget) go to `slots` worker threads for a global solve, each on its own HIP
+  stream (default priority).  The C ABI releases the GIL, so global solves of different graphs
+  overlap on the device with each other and with the next cycle's push.
 - Failures.  A request whose batch is malformed (ragged arrays, ids outside the graph) is
   refused on its own, before fusing; a repair that raises fails the requests fused into it and
   drops the graph's device state, which is rebuilt (initial solve plus the updates already
@@ -23,6 +29,8 @@ the serving layer the workload asks for:
 """
 from __future__ import annotations
 
+import ctypes as ct
+import itertools
 import threading
 import time
 from collections import deque
@@ -103,6 +111,7 @@ class Slot:
     gid: int
     graph: Graph
     lp: EdgeStreamLP | None = None
+    reserved: bool = False        # its footprint counts against the budget
     queue: deque = field(default_factory=deque)
     busy: bool = False
     last_used: float = 0.0
@@ -113,14 +122,26 @@ class Slot:
 
 
 class Service:
-    """Request loop over resident graphs (see the module docstring)."""
+    """Request loop over resident graphs (see the module docstring).
 
-    def __init__(self, graphs, hbm_budget_bytes, slots=4, eps=1e-6, max_fuse=64, lp_kwargs=None):
-        self.slots_n = int(slots)
+    One scheduler thread takes, in every cycle, the queued requests of each idle resident graph
+    (fused per graph), runs the first half of each repair (apply, re-ground, changed-row
+    refresh), then repairs every graph whose change a push can absorb with ONE push across
+    those graphs (dlp_lp_push_multi: one launch per round and column class, replayed from a
+    captured hipGraph), and hands the graphs that need a global solve to `slots` worker
+    threads, each on its own HIP stream, so global solves of different graphs overlap on the
+    device and with the next cycle.  Admission (and the rebuild of an evicted graph) happens in
+    the workers too."""
+
+    def __init__(self, graphs, hbm_budget_bytes, slots=4, eps=1e-6, max_fuse=64, lp_kwargs=None,
+                 fused_push=True, push_rounds_per_launch=4, push_max_rounds=200_000):
+        self.slots_n = max(1, int(slots))
         self.budget = int(hbm_budget_bytes)
         self.eps = eps
         self.max_fuse = max_fuse
         self.lp_kwargs = dict(lp_kwargs or {})
+        self.fused_push = bool(fused_push)
+        self.push_rpl, self.push_max_rounds = int(push_rounds_per_launch), int(push_max_rounds)
         self.graphs = {i: Slot(i, g) for i, g in enumerate(graphs)}
         for s in self.graphs.values():
             s.bytes = footprint_bytes(s.graph, **self.lp_kwargs)
@@ -131,106 +152,246 @@ class Service:
         self.cv = threading.Condition(self.lock)
         self.done = []
         self.stats = {"admissions": 0, "evictions": 0, "fused_requests": 0, "repairs": 0, "refused": 0,
-                      "failed_repairs": 0}
+                      "failed_repairs": 0, "cycles": 0, "push_launches": 0, "push_graphs": 0,
+                      "global_repairs": 0}
+        self._jobs = deque()          # global repairs and builds for the workers
+        self._running = 0             # jobs the workers are on
+        self._plan_obj, self._plan_key = None, None
+        self._serials = itertools.count()
         self._t0 = None
         self._stop = False
 
-    # -- admission ------------------------------------------------------------------------
+    # -- residency --------------------------------------------------------------------------
+    def _new_state(self, s: Slot):
+        lp = EdgeStreamLP(s.graph, eps=self.eps, **self.lp_kwargs)
+        lp.initial_solve()
+        s.solves += 1
+        lp.serial = next(self._serials)
+        return lp
+
     def _admit(self, s: Slot):
-        """Called with the lock held; returns True when s is resident (may evict idle graphs)."""
+        """Called with the lock held: True when s is resident, "build" when it is admitted now
+        (memory reserved; the caller builds its state), False when it does not fit yet (idle
+        graphs are evicted least-recently-used to make room)."""
         if s.lp is not None:
             return True
+        if s.reserved:
+            return False            # being built
         while self.resident + s.bytes > self.budget:
             idle = [x for x in self.graphs.values() if x.lp is not None and not x.busy and not x.queue]
             if not idle:
                 return False
             victim = min(idle, key=lambda x: x.last_used)
+            self._drop_plans()      # a plan keeps its graphs' states alive
             victim.lp = None        # frees its device buffers (handle destroyed with the object)
+            victim.reserved = False
             self.resident -= victim.bytes
             self.stats["evictions"] += 1
+        s.reserved = True
         self.resident += s.bytes
-        s.busy = True               # reserved; the worker builds the state outside the lock
         self.stats["admissions"] += 1
         return "build"
 
-    # -- worker ----------------------------------------------------------------------------
+    def _take(self, s: Slot):
+        """Queued requests of s fused while their insertions fit one device apply call."""
+        reqs, ins = [], 0
+        while s.queue and len(reqs) < self.max_fuse:
+            k = len(s.queue[0].batch.ins_u)
+            if reqs and ins + k > _MAX_INSERTIONS:
+                break
+            reqs.append(s.queue.popleft())
+            ins += k
+        return reqs
+
+    def _plan(self):
+        """The push plan of the current resident set, rebuilt when the set changes.  The plan
+        holds a reference to each member's state, so a state dropped elsewhere stays valid
+        until its plan goes (the multi push reads nothing of the graphs outside its call)."""
+        lps = {s.gid: s.lp for s in self.graphs.values() if s.lp is not None}
+        key = tuple(sorted((g, lp.serial) for g, lp in lps.items()))
+        if self._plan_key != key:
+            from . import _lib
+            self._drop_plans()
+            order = [g for g, _ in key]
+            hs = (ct.c_void_p * len(order))(*[lps[g].h.value for g in order])
+            h = ct.c_void_p()
+            _lib.call("dlp_push_plan_create", hs, len(order), ct.byref(h))
+            self._plan_obj, self._plan_key = (h, order, [lps[g] for g in order]), key
+        return self._plan_obj[:2]
+
+    def _drop_plans(self):
+        if self._plan_obj is not None:
+            from . import _lib
+            _lib.call("dlp_push_plan_destroy", self._plan_obj[0])
+        self._plan_obj, self._plan_key = None, None
+
+    # -- completion ---------------------------------------------------------------------------
+    def _complete(self, s: Slot, good, bad, b, st, err):
+        now = time.perf_counter() - self._t0
+        with self.cv:
+            for r in bad:
+                r.done = now
+            if err is not None:
+                for r in good:
+                    r.done, r.error = now, err
+                if s.reserved:
+                    # its state is unknown: dropped, rebuilt on next use
+                    s.lp = None
+                    s.reserved = False
+                    self.resident -= s.bytes
+                self.stats["failed_repairs"] += 1
+            else:
+                for r in good:
+                    r.done, r.rho, r.fused_with = now, st.rho if st is not None else float("nan"), len(good)
+                if b is not None:
+                    s.history.append(b)
+                    if len(s.history) > 32:     # keep the replay log one fused batch
+                        s.history = [fuse(s.history)]
+                s.served += len(good)
+                self.stats["fused_requests"] += len(good)
+                self.stats["repairs"] += int(b is not None)
+            self.done.extend(good + bad)
+            self.stats["refused"] += len(bad)
+            s.busy = False
+            s.last_used = now
+            self.cv.notify_all()
+
+    @staticmethod
+    def _split(s: Slot, reqs):
+        bad = []
+        for r in reqs:
+            try:
+                check_batch(r.batch, s.graph.n, max_insertions=_MAX_INSERTIONS)
+            except (IndexError, ValueError) as e:
+                r.error = f"refused: {e}"
+                bad.append(r)
+        return [r for r in reqs if r.error is None], bad
+
+    # -- workers: builds and global repairs -----------------------------------------------------
     def _worker(self, wid):
         stream = torch.cuda.Stream()
         with torch.cuda.stream(stream):
             while True:
                 with self.cv:
-                    job = None
-                    while job is None:
-                        if self._stop and not any(x.queue for x in self.graphs.values()):
+                    while not self._jobs:
+                        if self._stop and self._idle_locked():
                             return
-                        cands = [x for x in self.graphs.values() if x.queue and not x.busy]
-                        cands.sort(key=lambda x: x.queue[0].arrival)
-                        for s in cands:
-                            a = self._admit(s)
-                            if a:
-                                job = (s, a == "build")
-                                break
-                        if job is None:
-                            self.cv.wait(timeout=0.05)
-                    s, build = job
-                    s.busy = True
-                    # fuse queued requests while their insertions fit one apply call
-                    reqs, ins = [], 0
-                    while s.queue and len(reqs) < self.max_fuse:
-                        k = len(s.queue[0].batch.ins_u)
-                        if reqs and ins + k > _MAX_INSERTIONS:
-                            break
-                        reqs.append(s.queue.popleft())
-                        ins += k
-                bad = []
-                for r in reqs:
-                    try:
-                        check_batch(r.batch, s.graph.n, max_insertions=_MAX_INSERTIONS)
-                    except (IndexError, ValueError) as e:
-                        r.error = f"refused: {e}"
-                        bad.append(r)
-                good = [r for r in reqs if r.error is None]
-                st, b, err = None, None, None
+                        self.cv.wait(timeout=0.05)
+                    job = self._jobs.popleft()
+                    self._running += 1
+                kind, s, good, bad, b, req = job
+                st, err = None, None
                 try:
-                    if build:
-                        s.lp = EdgeStreamLP(s.graph, eps=self.eps, **self.lp_kwargs)
-                        s.lp.initial_solve()
-                        s.solves += 1
+                    if kind == "build":
+                        lp = self._new_state(s)
                         # an evicted graph re-enters with the updates it had already served
                         for piece in (replay_chunks(fuse(s.history)) if s.history else []):
-                            s.lp.apply_and_solve(piece)
-                    if good:
-                        b = fuse([r.batch for r in good])
-                        st = s.lp.apply_and_solve(b)
+                            lp.apply_and_solve(piece)
+                        s.lp = lp
+                        if good:
+                            b = fuse([r.batch for r in good])
+                            st = s.lp.apply_and_solve(b)
+                    else:
+                        st = s.lp.end_request(req, pushed=req.get("pushed"))
                     stream.synchronize()
                 except Exception as e:  # noqa: BLE001  (reported on the requests, service goes on)
                     err = f"repair failed: {type(e).__name__}: {e}"
-                now = time.perf_counter() - self._t0
+                self._complete(s, good, bad, b, st, err)
                 with self.cv:
-                    for r in bad:
-                        r.done = now
-                    if err is not None:
-                        for r in good:
-                            r.done, r.error = now, err
-                        # resident (or reserved by the admission): state unknown, rebuilt on next use
-                        s.lp = None
-                        self.resident -= s.bytes
-                        self.stats["failed_repairs"] += 1
-                    else:
-                        for r in good:
-                            r.done, r.rho, r.fused_with = now, st.rho, len(good)
-                        if b is not None:
-                            s.history.append(b)
-                            if len(s.history) > 32:     # keep the replay log one fused batch
-                                s.history = [fuse(s.history)]
-                        s.served += len(good)
-                        self.stats["fused_requests"] += len(good)
-                        self.stats["repairs"] += int(b is not None)
-                    self.done.extend(reqs)
-                    self.stats["refused"] += len(bad)
-                    s.busy = False
-                    s.last_used = now
+                    self._running -= 1
                     self.cv.notify_all()
+
+    def _idle_locked(self):
+        return (not any(x.queue for x in self.graphs.values()) and not any(x.busy for x in self.graphs.values())
+                and not self._jobs and self._running == 0)
+
+    # -- scheduler: one cycle fuses, begins, pushes across graphs, hands off global solves ------
+    def _scheduler(self):
+        from . import _lib
+        stream = torch.cuda.Stream()
+        with torch.cuda.stream(stream):
+            while True:
+                picks = []
+                with self.cv:
+                    while True:
+                        if self._stop and self._idle_locked():
+                            self.cv.notify_all()
+                            return
+                        cands = sorted((x for x in self.graphs.values() if x.queue and not x.busy),
+                                       key=lambda x: x.queue[0].arrival)
+                        for s in cands:
+                            a = self._admit(s)
+                            if not a:
+                                continue
+                            s.busy = True
+                            reqs = self._take(s)
+                            if a == "build":
+                                good, bad = self._split(s, reqs)
+                                self._jobs.append(("build", s, good, bad, None, None))
+                            else:
+                                picks.append((s, reqs))
+                        if picks:
+                            break
+                        self.cv.notify_all()
+                        self.cv.wait(timeout=0.05)
+                self.stats["cycles"] += 1
+                begun = []
+                for s, reqs in picks:
+                    good, bad = self._split(s, reqs)
+                    if not good:
+                        self._complete(s, good, bad, None, None, None)
+                        continue
+                    b = fuse([r.batch for r in good])
+                    try:
+                        begun.append((s, good, bad, b, s.lp.begin_request(b)))
+                    except Exception as e:  # noqa: BLE001
+                        self._complete(s, good, bad, b, None, f"repair failed: {type(e).__name__}: {e}")
+                stream.synchronize()
+                pushers = [x for x in begun if self.fused_push and x[0].lp.push_planned(x[4]["status"])]
+                with self.cv:
+                    for x in begun:
+                        if x not in pushers:
+                            self._jobs.append(("repair",) + x)
+                    self.cv.notify_all()
+                if not pushers:
+                    continue
+                try:
+                    plan, order = self._plan()
+                    pos = {g: i for i, g in enumerate(order)}
+                    nh = len(order)
+                    active = np.zeros(nh, dtype=np.int32)
+                    budget = np.zeros(nh, dtype=np.int64)
+                    for s, _, _, _, _ in pushers:
+                        active[pos[s.gid]] = 1
+                        budget[pos[s.gid]] = s.lp.push_budget_edges()
+                    rounds = np.zeros(nh, dtype=np.int64)
+                    conv = np.zeros(nh, dtype=np.int32)
+                    _lib.call("dlp_lp_push_multi", plan, active.ctypes.data, self.push_max_rounds, self.push_rpl, 1,
+                              budget.ctypes.data, rounds.ctypes.data, conv.ctypes.data, stream.cuda_stream)
+                    self.stats["push_launches"] += 1
+                    self.stats["push_graphs"] += len(pushers)
+                except Exception as e:  # noqa: BLE001
+                    for s, good, bad, b, _ in pushers:
+                        self._complete(s, good, bad, b, None, f"repair failed: {type(e).__name__}: {e}")
+                    continue
+                # converged graphs are verified here; the rest (edge budget spent) go on to a
+                # global solve in a worker
+                for x in pushers:
+                    s, good, bad, b, req = x
+                    i = pos[s.gid]
+                    req["pushed"] = (int(rounds[i]), bool(conv[i]))
+                    if not conv[i]:
+                        with self.cv:
+                            self._jobs.append(("repair",) + x)
+                            self.cv.notify_all()
+                        continue
+                    st, err = None, None
+                    try:
+                        st = s.lp.end_request(req, pushed=req["pushed"])
+                        stream.synchronize()
+                    except Exception as e:  # noqa: BLE001
+                        err = f"repair failed: {type(e).__name__}: {e}"
+                    self._complete(s, good, bad, b, st, err)
 
     # -- driver ----------------------------------------------------------------------------
     def run(self, requests, preload=True):
@@ -240,17 +401,17 @@ class Service:
         if preload:
             for s in sorted(self.graphs.values(), key=lambda x: x.bytes):
                 if self.resident + s.bytes <= self.budget:
-                    s.lp = EdgeStreamLP(s.graph, eps=self.eps, **self.lp_kwargs)
-                    s.lp.initial_solve()
-                    s.solves += 1
+                    s.lp = self._new_state(s)
+                    s.reserved = True
                     self.resident += s.bytes
                     self.stats["admissions"] += 1
         torch.cuda.synchronize()
         self._t0 = time.perf_counter()
         self._stop = False
-        workers = [threading.Thread(target=self._worker, args=(i,), daemon=True) for i in range(self.slots_n)]
-        for w in workers:
-            w.start()
+        threads = [threading.Thread(target=self._scheduler, daemon=True)]
+        threads += [threading.Thread(target=self._worker, args=(i,), daemon=True) for i in range(self.slots_n)]
+        for t in threads:
+            t.start()
         pending = sorted(requests, key=lambda r: r.arrival)
         i = 0
         while i < len(pending):
@@ -267,21 +428,25 @@ class Service:
         with self.cv:
             self._stop = True
             self.cv.notify_all()
-        for w in workers:
-            w.join()
+        for t in threads:
+            t.join()
         self.wall = time.perf_counter() - self._t0
+        self._drop_plans()
         return self.done
 
     def report(self):
         ok = [r for r in self.done if r.error is None]
         lat = np.array([r.latency_ms for r in ok])
         edges = sum(r.batch.size for r in ok)
+        pl = self.stats["push_launches"]
         return {"requests": len(ok), "errors": len(self.done) - len(ok), "edge_updates": edges,
                 "edge_updates_per_s": edges / self.wall if self.wall > 0 else float("nan"),
+                "requests_per_s": len(ok) / self.wall if self.wall > 0 else float("nan"),
                 "p50_ms": float(np.percentile(lat, 50)) if lat.size else float("nan"),
                 "p99_ms": float(np.percentile(lat, 99)) if lat.size else float("nan"),
                 "max_rho": max((r.rho for r in ok), default=float("nan")),
                 "mean_fused": float(np.mean([r.fused_with for r in ok])) if ok else 0.0,
+                "graphs_per_push_launch": self.stats["push_graphs"] / pl if pl else 0.0,
                 **self.stats}
 
 

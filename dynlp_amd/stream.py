@@ -275,11 +275,28 @@ class EdgeStreamLP:
         _lib.call("dlp_colop", n, C1, 0, 1.0, ptr(ones), ptr(D), ld, None, ptr(self.Z64), ld, _lib.F64, stream())
         return st.iters
 
-    def _repair(self, st: Stats, force_global=False):
+    def push_planned(self, s):
+        """Whether the mixed path's repair of the current batch starts with a push: few
+        violators (at most push_frac*n and push_max_rows) and a residual mass (the violators'
+        summed max |R|/(eps*diag), about the row pushes a push-only repair takes) small next to
+        the graph (a global solve costs a few dozen passes over all rows).  A batch of more
+        edges than push_max_rows changes more rows than a push may start from: it goes to the
+        global solve without reading the violator count (s may then be None)."""
+        if self.fp64 or getattr(self, "_batch_edges", 0) > self.push_max_rows or s is None:
+            return False
+        return s["viol"] <= min(self.push_frac * self.n, self.push_max_rows) and s["mass"] <= self.push_mass * self.n
+
+    def push_budget_edges(self):
+        return int(self.push_budget * max(self.nnz, 1))
+
+    def _repair(self, st: Stats, force_global=False, pushed=None, s0=None):
         """Mixed path (DynLP+ policy, solvers.py:272-288 with refinement :322-337): push while
         the violating set is small and within the edge budget, else the global PCG; then FP64
         verification of every row the repair touched, pushing again until it certifies.
-        FP64 path: the reference's incremental push to completion (solvers.py:375-389)."""
+        pushed = (rounds, converged) of a push of this batch already run (the service's push
+        across graphs); s0 = the status after the changed-row refresh, when already read.  FP64
+        path: the reference's incremental push to completion
+        (solvers.py:375-389), or (fp64_repair="pcg") an FP64 global column PCG first."""
         if self.fp64:
             if self.fp64_repair == "pcg":
                 st.path = "fp64-pcg"
@@ -293,21 +310,18 @@ class EdgeStreamLP:
             self._rho[:] = self._rho_dev.cpu().numpy()
             return s
         self._rho_all = False
-        # a batch of more edges than push_max_rows changes more rows than a push may start from
-        # (two per edge, nearly all violating at eps = 1e-6): it goes to the global solve
-        # without waiting for the violator count
-        force_global = force_global or getattr(self, "_batch_edges", 0) > self.push_max_rows
-        s = None if force_global else self.status()
-        ok = False
-        # push only when it can finish cheaply: few violators, and a residual mass (sum of the
-        # violators' max |R|/(eps*diag), about the row pushes a push-only repair takes) that
-        # is small next to the graph (a global solve costs a few dozen passes over all rows)
-        tried = False
-        if (not force_global and s["viol"] <= min(self.push_frac * self.n, self.push_max_rows)
-                and s["mass"] <= self.push_mass * self.n):
-            tried = True
-            r, ok = self._push(budget=int(self.push_budget * max(self.nnz, 1)))
-            st.iters += r
+        ok, tried = False, pushed is not None
+        if pushed is not None:
+            st.iters += pushed[0]
+            ok = pushed[1]
+        elif not force_global:
+            s = s0
+            if s is None and getattr(self, "_batch_edges", 0) <= self.push_max_rows:
+                s = self.status()
+            if self.push_planned(s):
+                tried = True
+                r, ok = self._push(budget=self.push_budget_edges())
+                st.iters += r
         if not ok:
             st.path = "push+pcg" if tried else "pcg"
             it, _ = self._pcg()
@@ -413,6 +427,13 @@ class EdgeStreamLP:
         positive (ValueError) and ids outside [0, n) (IndexError) are refused before any device
         work; a pair repeated within the deletions or within the insertions counts once (the
         last insertion's weight)."""
+        req = self.begin_request(b)
+        return self.end_request(req)
+
+    def begin_request(self, b: EdgeBatch):
+        """First half of apply_and_solve: upload, apply on the slack store, re-ground, refresh
+        the changed rows' FP64 residual.  Returns the request's state for end_request; its
+        "status" holds the refresh's counters when a push could repair it (push_planned)."""
         st = Stats(path="fp64-push" if self.fp64 else "push")
         t0 = time.perf_counter()
         (du, dv, iu, iv, iw), b = self._upload(b)
@@ -425,20 +446,27 @@ class EdgeStreamLP:
                       stream())
         else:
             _lib.call("dlp_lp_ground", self.h, stream())
+        _lib.call("dlp_lp_refresh", self.h, 0, 1 if self.fp64 else 0, stream())
+        s = None if (self.fp64 or self._batch_edges > self.push_max_rows) else self.status()
+        return {"st": st, "t0": t0, "b": b, "ins": (iu, iv, iw), "status": s}
+
+    def end_request(self, req, pushed=None):
+        """Second half: the repair (with pushed = (rounds, converged) of a push already run on
+        this batch), a regrow after overflowed insertions, and the request's statistics."""
+        st, b = req["st"], req["b"]
         prec = 1 if self.fp64 else 0
-        _lib.call("dlp_lp_refresh", self.h, 0, prec, stream())
-        s = self._repair(st)
+        s = self._repair(st, pushed=pushed, s0=req["status"])
         if s["invalid"] or s["duplicates"]:
             raise RuntimeError(f"device apply refused {s['invalid']} pairs and merged {s['duplicates']} "
                                "repeated entries that the host checks had passed")
         if s["overflow"]:
             # room for every insertion of this batch in any one row, so one regrow suffices
             per_row = np.bincount(np.concatenate([b.ins_u, b.ins_v]), minlength=1).max()
-            self._regrow((iu, iv, iw), max(8, int(per_row)))
+            self._regrow(req["ins"], max(8, int(per_row)))
             _lib.call("dlp_lp_ground", self.h, stream())   # re-inserted edges can re-ground rows
             _lib.call("dlp_lp_refresh", self.h, 0, prec, stream())
             s = self._repair(st)
-        st.ms = (time.perf_counter() - t0) * 1e3
+        st.ms = (time.perf_counter() - req["t0"]) * 1e3
         if s["missing"]:
             st.extra["missing_deletions"] = s["missing"]
         self._finish(st, s)

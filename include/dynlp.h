@@ -477,6 +477,22 @@ int dlp_lp_set_coarse(void* handle, int na, int refresh);
  * out[2] = global solves since it was built; bounds_out (host, na + 1 ints, may be NULL) = the
  * row ranges [bounds[I], bounds[I+1]) (synchronises).  No reference counterpart. */
 int dlp_lp_coarse_info(void* handle, int32_t* out, int32_t* bounds_out, void* stream);
+/* One push across several resident graphs (the service's fused repair step; no reference
+ * counterpart: the reference repairs one system at a time, kernels.py:396-433).  A plan
+ * (dlp_push_plan_create, which allocates its own small device arrays; destroy with
+ * dlp_push_plan_destroy) groups the handles' push descriptors by column class; then
+ * dlp_lp_push_multi runs the asynchronous push of dlp_lp_push for every handle i with
+ * active[i] != 0, starting from that handle's last refresh's violators, as one launch per
+ * round and column class covering all of them (grid y = graph), replayed from one captured
+ * hipGraph of rounds_per_launch (even) rounds when use_graph != 0.  A graph leaves the launch
+ * when it converges (converged_out[i] = 1), when its scattered entries in this batch exceed
+ * budget_edges[i] (> 0; NULL: no budget) or after max_rounds; rounds_out[i] = rounds launched
+ * while it was active.  Host arrays of the plan's size; synchronises once per launch.  Calls
+ * on one plan, or on handles of an active plan, must not overlap. */
+int dlp_push_plan_create(void* const* handles, int nh, void** plan);
+int dlp_push_plan_destroy(void* plan);
+int dlp_lp_push_multi(void* plan, const int32_t* active, int max_rounds, int rounds_per_launch, int use_graph,
+                      const int64_t* budget_edges, int64_t* rounds_out, int32_t* converged_out, void* stream);
 /* Plain FP64 path, one host-driven round of solvers.push (solvers.py:108-116): candidates
  * take D = R/diag, Z += D, R = 0, scatter W D with FP64 atomics and mark N(cand) and cand. */
 int dlp_lp_push64_round(void* handle, int nc, const int32_t* cand, void* stream);

@@ -142,3 +142,67 @@ def test_fusion_stops_at_the_insertion_limit(monkeypatch):
     sysm, Fs = _oracle_hmax(gs[0], reqs)
     Z = svc.graphs[0].lp.scores_with_h()
     assert np.abs(Z[:, : sysm.C] - Fs[:, : sysm.C]).max() <= EPS * Fs[:, sysm.C].max() * 1.05
+
+
+def test_push_across_graphs_matches_oracle():
+    """dlp_lp_push_multi through the C ABI: three graphs of different column counts (so more
+    than one column class) take small batches; each call pushes the graphs whose batch a push
+    can repair together (graph 1 is left out of every second call and repairs itself), and every
+    graph ends certified at the exact solution of its final edge set."""
+    import ctypes as ct
+    from dynlp_amd import _lib
+    from dynlp_amd.stream import EdgeStreamLP
+    gs = _graphs()
+    lps = [EdgeStreamLP(g, eps=EPS) for g in gs]
+    for lp in lps:
+        lp.initial_solve()
+    hs = (ct.c_void_p * 3)(*[lp.h.value for lp in lps])
+    plan = ct.c_void_p()
+    _lib.call("dlp_push_plan_create", hs, 3, ct.byref(plan))
+    streams = [graphs.update_stream(g, 4, 60, seed=30 + i)[0] for i, g in enumerate(gs)]
+    sets = [graphs.EdgeSet(g) for g in gs]
+    multi = 0
+    try:
+        for t in range(4):
+            reqs = [lp.begin_request(streams[i][t]) for i, lp in enumerate(lps)]
+            active = np.array([int(lp.push_planned(r["status"]) and not (i == 1 and t % 2))
+                               for i, (lp, r) in enumerate(zip(lps, reqs))], dtype=np.int32)
+            multi = max(multi, int(active.sum()))
+            budget = np.array([lp.push_budget_edges() for lp in lps], dtype=np.int64)
+            rounds = np.zeros(3, dtype=np.int64)
+            conv = np.zeros(3, dtype=np.int32)
+            _lib.call("dlp_lp_push_multi", plan, active.ctypes.data, 200_000, 4, 1, budget.ctypes.data,
+                      rounds.ctypes.data, conv.ctypes.data, None)
+            for i, (lp, r) in enumerate(zip(lps, reqs)):
+                st = lp.end_request(r, pushed=(int(rounds[i]), bool(conv[i])) if active[i] else None)
+                assert st.rho <= EPS * 1.0001, (t, i, st.rho)
+                assert active[i] or rounds[i] == 0
+                sets[i].apply(streams[i][t])
+    finally:
+        _lib.call("dlp_push_plan_destroy", plan)
+    assert multi >= 2
+    for i, g in enumerate(gs):
+        u, v, w = sets[i].pairs()
+        sysm = O.build_system(g.n, u, v, w, g.labeled, g.y, g.K)
+        Fs = O.solve_system(sysm)
+        Z = lps[i].scores_with_h()
+        assert np.abs(Z[:, : sysm.C] - Fs[:, : sysm.C]).max() <= EPS * Fs[:, sysm.C].max() * 1.05, i
+
+
+def test_service_pushes_graphs_together():
+    """Eight small graphs whose requests arrive at once: the scheduler repairs them with pushes
+    that cover several graphs per call, and every graph ends certified at the exact solution of
+    its updates."""
+    from dynlp_amd.serve import Service, footprint_bytes, poisson_requests
+    gs = [graphs.sbm(1500 + 300 * i, 4, 8, 0.8, 4, 0.03, seed=40 + i) for i in range(8)]
+    reqs = poisson_requests(gs, rate=1e9, duration=1.0, size_lo=10, size_hi=40, seed=9, max_requests=32)
+    svc = Service(gs, sum(footprint_bytes(g) for g in gs) + (1 << 20), slots=2, eps=EPS)
+    svc.run(reqs)
+    rep = svc.report()
+    assert rep["errors"] == 0 and rep["requests"] == len(reqs)
+    assert rep["push_launches"] >= 1 and rep["graphs_per_push_launch"] > 1, rep
+    for gid, g in enumerate(gs):
+        mine = [r for r in reqs if r.gid == gid]
+        sysm, Fs = _oracle_hmax(g, mine)
+        Z = svc.graphs[gid].lp.scores_with_h()
+        assert np.abs(Z[:, : sysm.C] - Fs[:, : sysm.C]).max() <= EPS * Fs[:, sysm.C].max() * 1.05, gid
