@@ -9,6 +9,7 @@ from oracle import lp as O
 pytestmark = pytest.mark.gpu
 
 EPS = 1e-6
+_ALWAYS_PUSH = dict(push_frac=1.0, push_max_rows=1 << 20, push_mass=float("inf"), push_budget=1e9)
 
 
 def _graphs():
@@ -148,12 +149,13 @@ def test_push_across_graphs_matches_oracle():
     """dlp_lp_push_multi through the C ABI: three graphs of different column counts (so more
     than one column class) take small batches; each call pushes the graphs whose batch a push
     can repair together (graph 1 is left out of every second call and repairs itself), and every
-    graph ends certified at the exact solution of its final edge set."""
+    graph ends certified at the exact solution of its final edge set.  The repair policy is set
+    to push whatever the batch's residual mass, so that every batch is a push candidate."""
     import ctypes as ct
     from dynlp_amd import _lib
     from dynlp_amd.stream import EdgeStreamLP
     gs = _graphs()
-    lps = [EdgeStreamLP(g, eps=EPS) for g in gs]
+    lps = [EdgeStreamLP(g, eps=EPS, **_ALWAYS_PUSH) for g in gs]
     for lp in lps:
         lp.initial_solve()
     hs = (ct.c_void_p * 3)(*[lp.h.value for lp in lps])
@@ -191,12 +193,13 @@ def test_push_across_graphs_matches_oracle():
 
 def test_service_pushes_graphs_together():
     """Eight small graphs whose requests arrive at once: the scheduler repairs them with pushes
-    that cover several graphs per call, and every graph ends certified at the exact solution of
-    its updates."""
+    that cover several graphs per call (the repair policy set to push whatever the residual
+    mass), and every graph ends certified at the exact solution of its updates."""
     from dynlp_amd.serve import Service, footprint_bytes, poisson_requests
     gs = [graphs.sbm(1500 + 300 * i, 4, 8, 0.8, 4, 0.03, seed=40 + i) for i in range(8)]
     reqs = poisson_requests(gs, rate=1e9, duration=1.0, size_lo=10, size_hi=40, seed=9, max_requests=32)
-    svc = Service(gs, sum(footprint_bytes(g) for g in gs) + (1 << 20), slots=2, eps=EPS)
+    svc = Service(gs, sum(footprint_bytes(g, **_ALWAYS_PUSH) for g in gs) + (1 << 20), slots=2, eps=EPS,
+                  lp_kwargs=_ALWAYS_PUSH)
     svc.run(reqs)
     rep = svc.report()
     assert rep["errors"] == 0 and rep["requests"] == len(reqs)
