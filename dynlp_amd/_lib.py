@@ -44,6 +44,7 @@ _SIGS = {
     "dlp_lp_set_eps": (ct.c_int, [P, P, DBL, P]),
     "dlp_lp_begin_batch": (ct.c_int, [P, P]),
     "dlp_lp_apply": (ct.c_int, [P, I32, P, P, I32, P, P, P, P]),
+    "dlp_lp_apply_ops": (ct.c_int, [P, I32, P, P, P, P]),
     "dlp_lp_regrow_plan": (ct.c_int, [P, I32, DBL, P, P, P]),
     "dlp_lp_regrow_commit": (ct.c_int, [P, P, P, P, I32, P, P, P, P, P]),
     "dlp_lp_ground": (ct.c_int, [P, P]),

@@ -33,6 +33,32 @@ __device__ __forceinline__ bool weight_ok(float x) { return x > 0.f && x <= 3.40
 // fails every threshold test and reaches the column maxima, so it is never certified.
 __device__ __forceinline__ double ratio_or_inf(double res, double dg) { return nan_to_inf(fabs(res) / dg); }
 
+// Row a loses its entry b (tombstone by compare-and-swap; not found counts as missing).
+__device__ __forceinline__ void delete_entry(int a, int b, const int32_t* __restrict__ indptr,
+                                             const int32_t* __restrict__ rowlen, int32_t* __restrict__ col,
+                                             float* __restrict__ w, const uint8_t* __restrict__ labeled,
+                                             int32_t* __restrict__ labcnt, int32_t* __restrict__ stamp, int stampv,
+                                             int32_t* __restrict__ rows, int32_t* __restrict__ nrows,
+                                             int32_t* __restrict__ missing) {
+  const int beg = indptr[a], end = beg + rowlen[a];
+  bool found = false;
+  for (int j = beg; j < end; ++j) {
+    if (col[j] == b) {
+      if (atomicCAS(col + j, b, -1) == b) {
+        w[j] = 0.f;
+        found = true;
+      }
+      break;
+    }
+  }
+  if (!found) {
+    atomicAdd(missing, 1);
+    return;
+  }
+  if (labeled[b]) atomicSub(labcnt + a, 1);
+  stamp_append(a, stamp, stampv, rows, nrows);
+}
+
 // Directed deletion e < 2*nd: row a loses entry b.  Rows are stamped into the changed list;
 // the labeled-neighbour counter of a drops when b is labeled.  A pair with an id outside
 // [0, n) is skipped and counted once (invalid); the tombstone is a compare-and-swap, so a pair
@@ -53,23 +79,7 @@ __global__ void k_apply_delete(int n, int nd, const int32_t* __restrict__ du, co
     return;
   }
   if (a == b) return;
-  const int beg = indptr[a], end = beg + rowlen[a];
-  bool found = false;
-  for (int j = beg; j < end; ++j) {
-    if (col[j] == b) {
-      if (atomicCAS(col + j, b, -1) == b) {
-        w[j] = 0.f;
-        found = true;
-      }
-      break;
-    }
-  }
-  if (!found) {
-    atomicAdd(missing, 1);
-    return;
-  }
-  if (labeled[b]) atomicSub(labcnt + a, 1);
-  stamp_append(a, stamp, stampv, rows, nrows);
+  delete_entry(a, b, indptr, rowlen, col, w, labeled, labcnt, stamp, stampv, rows, nrows, missing);
 }
 
 // Close the tombstone gaps of the listed rows (stable order of the live entries).
@@ -225,6 +235,90 @@ __global__ __launch_bounds__(256) void k_apply_insert(int n, int ni, const int32
     return;
   }
   upsert_entry(a, b, wt, lane, indptr, rowlen, col, w, labeled, labcnt, stamp, stampv, rows, nrows, overflow,
+               noverflow, (int32_t)e);
+}
+
+// ---------------------------------------------------------------------------------------
+// Ordered operations: one sequence of pair operations (u[s], v[s], w[s]), s = 0..nops-1, with
+// w[s] == 0 a deletion and w[s] > 0 an insertion (upsert) of weight w[s].  Applying the
+// sequence in order leaves every unordered pair as its LAST operation made it, so each pair
+// takes that one operation only: the pair table holds the largest s per pair, deletions whose
+// s is the largest tombstone the pair, insertions whose s is the largest upsert it, the rest
+// change nothing.  A group of requests is thereby applied as one call, with their order kept.
+
+__device__ __forceinline__ bool op_ok(int a, int b, float wt, int n) {
+  return id_ok(a, n) && id_ok(b, n) && (wt == 0.f || weight_ok(wt));
+}
+
+__global__ void k_ops_table_fill(int n, int nops, const int32_t* __restrict__ u, const int32_t* __restrict__ v,
+                                 const float* __restrict__ wt, int bits, unsigned long long* __restrict__ keys,
+                                 int32_t* __restrict__ vals, int32_t* __restrict__ invalid) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nops) return;
+  const int a = u[s], b = v[s];
+  if (!op_ok(a, b, wt[s], n)) {
+    atomicAdd(invalid, 1);
+    return;
+  }
+  if (a == b) return;
+  const unsigned long long key = (unsigned long long)min(a, b) * (unsigned)n + (unsigned)max(a, b);
+  const unsigned mask = (1u << bits) - 1u;
+  for (unsigned h = pair_slot(key, bits);; h = (h + 1) & mask) {
+    const unsigned long long old = atomicCAS(keys + h, kHashEmpty, key);
+    if (old == kHashEmpty || old == key) {
+      atomicMax(vals + h, s);
+      return;
+    }
+  }
+}
+
+// Directed entry e < 2*nops of operation s = e mod nops (e < nops: row u[s] gets v[s]).
+__device__ __forceinline__ bool op_entry(int64_t e, int nops, int n, const int32_t* __restrict__ u,
+                                         const int32_t* __restrict__ v, const float* __restrict__ wt, int bits,
+                                         const unsigned long long* __restrict__ keys,
+                                         const int32_t* __restrict__ vals, int& a, int& b, float& x) {
+  const int s = (int)(e < nops ? e : e - nops);
+  a = e < nops ? u[s] : v[s];
+  b = e < nops ? v[s] : u[s];
+  x = wt[s];
+  if (!op_ok(a, b, x, n) || a == b) return false;
+  const unsigned long long key = (unsigned long long)min(a, b) * (unsigned)n + (unsigned)max(a, b);
+  return pair_last(key, bits, keys, vals) == s;
+}
+
+__global__ void k_ops_delete(int n, int nops, const int32_t* __restrict__ u, const int32_t* __restrict__ v,
+                             const float* __restrict__ wt, int bits, const unsigned long long* __restrict__ keys,
+                             const int32_t* __restrict__ vals, const int32_t* __restrict__ indptr,
+                             const int32_t* __restrict__ rowlen, int32_t* __restrict__ col, float* __restrict__ w,
+                             const uint8_t* __restrict__ labeled, int32_t* __restrict__ labcnt,
+                             int32_t* __restrict__ stamp, int stampv, int32_t* __restrict__ rows,
+                             int32_t* __restrict__ nrows, int32_t* __restrict__ missing) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 2 * (int64_t)nops) return;
+  int a, b;
+  float x;
+  if (!op_entry(e, nops, n, u, v, wt, bits, keys, vals, a, b, x) || x != 0.f) return;
+  delete_entry(a, b, indptr, rowlen, col, w, labeled, labcnt, stamp, stampv, rows, nrows, missing);
+}
+
+__global__ __launch_bounds__(256) void k_ops_insert(int n, int nops, const int32_t* __restrict__ u,
+                                                    const int32_t* __restrict__ v, const float* __restrict__ wt,
+                                                    int bits, const unsigned long long* __restrict__ keys,
+                                                    const int32_t* __restrict__ vals,
+                                                    const int32_t* __restrict__ indptr, int32_t* __restrict__ rowlen,
+                                                    int32_t* __restrict__ col, float* __restrict__ w,
+                                                    const uint8_t* __restrict__ labeled,
+                                                    int32_t* __restrict__ labcnt, int32_t* __restrict__ stamp,
+                                                    int stampv, int32_t* __restrict__ rows,
+                                                    int32_t* __restrict__ nrows, int32_t* __restrict__ overflow,
+                                                    int32_t* __restrict__ noverflow) {
+  const int64_t e = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (e >= 2 * (int64_t)nops) return;   // wave-uniform
+  int a, b;
+  float x;
+  if (!op_entry(e, nops, n, u, v, wt, bits, keys, vals, a, b, x) || x == 0.f) return;   // wave-uniform
+  upsert_entry(a, b, x, lane, indptr, rowlen, col, w, labeled, labcnt, stamp, stampv, rows, nrows, overflow,
                noverflow, (int32_t)e);
 }
 
@@ -1208,6 +1302,7 @@ struct LP {
   int32_t* pair_vals = nullptr;
   int64_t dev_bytes = 0;        // device memory this handle holds (its own buffers)
   int last_refresh_kernel = -1; // kernel of the last refresh: 0 k_refresh, 1 k_refresh_v4, 2 k_verify
+  int applies = 0;              // apply calls since the batch began (the changed-row list grows)
   double push_ms = 0.0;         // summed duration of push-round launches (events)
   int64_t push_launched = 0;    // rounds launched
 };
@@ -1509,6 +1604,7 @@ extern "C" int dlp_lp_begin_batch(void* handle, void* stream) {
   DLP_CHECK_ARG(handle, "lp_begin_batch: null handle");
   LP* h = (LP*)handle;
   h->stampv += 1;
+  h->applies = 0;
   hipLaunchKernelGGL(k_begin_batch, dim3(1), dim3(64), 0, (hipStream_t)stream, h->cnt,
                      (int32_t*)h->S.bstamp, h->S.bcount, h->S.edges);
   DLP_CHECK_LAUNCH();
@@ -1529,7 +1625,8 @@ extern "C" int dlp_lp_apply(void* handle, int nd, const int32_t* du, const int32
                        h->stamp, h->stampv, h->rows, h->cnt + CNT_ROWS, h->cnt + CNT_MISSING,
                        h->cnt + CNT_INVALID);
     DLP_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_compact_rows, dim3(grid_for(2LL * nd)), dim3(kBlock), 0, st,
+    // the changed-row list holds this call's rows after those of earlier calls of the batch
+    hipLaunchKernelGGL(k_compact_rows, dim3(grid_for(h->applies ? (long long)n : 2LL * nd)), dim3(kBlock), 0, st,
                        h->cnt + CNT_ROWS, h->rows, h->d.indptr, h->d.rowlen, h->d.col, h->d.w);
     DLP_CHECK_LAUNCH();
   }
@@ -1548,6 +1645,43 @@ extern "C" int dlp_lp_apply(void* handle, int nd, const int32_t* du, const int32
                        h->cnt + CNT_DUP);
     DLP_CHECK_LAUNCH();
   }
+  h->applies += 1;
+  return DLP_OK;
+}
+
+// The ordered operation sequence (see k_ops_table_fill): deletions (w == 0) and insertions
+// (w > 0) of a group of requests in one call, each pair taking its last operation.
+extern "C" int dlp_lp_apply_ops(void* handle, int nops, const int32_t* u, const int32_t* v, const float* w,
+                                void* stream) {
+  DLP_CHECK_ARG(handle && nops >= 0, "lp_apply_ops: bad argument");
+  DLP_CHECK_ARG(nops == 0 || (u && v && w), "lp_apply_ops: null arrays");
+  DLP_CHECK_ARG(nops <= (1 << 19), "lp_apply_ops: at most 524288 operations per call");
+  if (nops == 0) return DLP_OK;
+  LP* h = (LP*)handle;
+  hipStream_t st = (hipStream_t)stream;
+  const int n = h->d.n;
+  int bits = 1;
+  while ((1LL << bits) < 2LL * nops) ++bits;
+  const int cap = 1 << bits;   // <= kPairTableCap
+  hipLaunchKernelGGL(k_pair_table_clear, dim3(grid_for(cap)), dim3(kBlock), 0, st, cap, h->pair_keys, h->pair_vals);
+  hipLaunchKernelGGL(k_ops_table_fill, dim3(grid_for(nops)), dim3(kBlock), 0, st, n, nops, u, v, w, bits,
+                     h->pair_keys, h->pair_vals, h->cnt + CNT_INVALID);
+  DLP_CHECK_LAUNCH();
+  const unsigned long long* keys = h->pair_keys;
+  const int32_t* vals = h->pair_vals;
+  hipLaunchKernelGGL(k_ops_delete, dim3(grid_for(2LL * nops)), dim3(kBlock), 0, st, n, nops, u, v, w, bits, keys,
+                     vals, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.labeled, h->d.labcnt, h->stamp,
+                     h->stampv, h->rows, h->cnt + CNT_ROWS, h->cnt + CNT_MISSING);
+  DLP_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_compact_rows, dim3(grid_for(h->applies ? (long long)n : 2LL * nops)), dim3(kBlock), 0, st,
+                     h->cnt + CNT_ROWS, h->rows,
+                     h->d.indptr, h->d.rowlen, h->d.col, h->d.w);
+  DLP_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_ops_insert, dim3(grid_for(2LL * nops * 64)), dim3(kBlock), 0, st, n, nops, u, v, w, bits,
+                     keys, vals, h->d.indptr, h->d.rowlen, h->d.col, h->d.w, h->d.labeled, h->d.labcnt, h->stamp,
+                     h->stampv, h->rows, h->cnt + CNT_ROWS, h->overflow, h->cnt + CNT_OVERFLOW);
+  DLP_CHECK_LAUNCH();
+  h->applies += 1;
   return DLP_OK;
 }
 

@@ -76,7 +76,7 @@ def csr_from_pairs(n, u, v, w):
     r = np.concatenate([u, v]).astype(np.int64)
     c = np.concatenate([v, u]).astype(np.int64)
     d = np.concatenate([w, w]).astype(np.float32)
-    order = np.lexsort((c, r))
+    order = np.argsort(r * max(int(n), 1) + c)      # one int64 key: row-major, columns ascending
     r, c, d = r[order], c[order], d[order]
     indptr = np.zeros(n + 1, dtype=np.int64)
     np.cumsum(np.bincount(r, minlength=n), out=indptr[1:])
@@ -225,6 +225,24 @@ def config_graph(cfg: int, scale: float = 1.0, seed: int = 0) -> Graph:
     raise ValueError(f"no graph for configuration {cfg}")
 
 
+def serving_graphs(count=32, seed=0, scale=1.0):
+    """The serving workload's resident graphs: graph i is ER or SBM by a coin of seed+i, n
+    log-uniform in [50k, 2M] (times `scale`), average degree uniform in [8, 24], k drawn from
+    {8, 16, 64}; SBM graphs have k blocks, 80 % intra-block edges and weights U[0.5, 1.5], ER
+    graphs unit weights; 1 % labelled."""
+    rng = np.random.default_rng(seed + 7)
+    out = []
+    for i in range(count):
+        n = max(1000, int(np.exp(rng.uniform(np.log(50_000), np.log(2_000_000))) * scale))
+        deg = float(rng.uniform(8, 24))
+        K = int(rng.choice([8, 16, 64]))
+        if rng.random() < 0.5:
+            out.append(erdos_renyi(n, deg, K, 0.01, seed + 100 + i, unit=True))
+        else:
+            out.append(sbm(n, K, deg, 0.8, K, 0.01, seed + 100 + i))
+    return out
+
+
 # ---------------------------------------------------------------------------------------------
 # Edge-update streams.
 
@@ -271,6 +289,21 @@ def check_batch(b: EdgeBatch, n: int, max_insertions: int | None = None) -> None
         if bad.any():
             i = int(np.flatnonzero(bad)[0])
             raise ValueError(f"insertion weights must be finite and positive (as FP32): ins_w[{i}] = {w[i]!r}")
+
+
+def pack_ops(batches):
+    """The ordered operation sequence of `batches` applied one after the other: per batch its
+    deletions (weight 0) then its insertions (their weights), as int32 u, v and float32 w (the
+    layout of dlp_lp_apply_ops, whose result is each pair's last operation)."""
+    us, vs, ws = [], [], []
+    for b in batches:
+        us += [np.asarray(b.del_u), np.asarray(b.ins_u)]
+        vs += [np.asarray(b.del_v), np.asarray(b.ins_v)]
+        ws += [np.zeros(len(b.del_u), dtype=np.float32), np.asarray(b.ins_w, dtype=np.float32)]
+    if not us:
+        return np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32)
+    return (np.concatenate(us).astype(np.int32, copy=False), np.concatenate(vs).astype(np.int32, copy=False),
+            np.concatenate(ws))
 
 
 def _last_unique(key):

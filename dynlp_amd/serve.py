@@ -6,13 +6,15 @@ the serving layer the workload asks for:
 - Residency and admission.  Every graph lives on the device as an `EdgeStreamLP` (slack CSR,
   FP32 scores and kept residual, FP64 diag).  A graph is admitted when its estimated footprint
   fits the HBM budget; otherwise it waits, and idle graphs are evicted least-recently-used to
-  make room (an evicted graph keeps its host-side edge set and is re-admitted with a fresh
-  solve).
-- Continuous batching.  Requests (edge-update batches for one graph) queue per graph.  A worker
-  takes the requests queued for a graph at that moment (up to max_fuse, and while their
-  insertions fit one device apply call, 2^19) and fuses them into one batch (per pair, the
-  last operation wins, which is what applying them in arrival order gives), so requests that
-  arrive while the graph is busy are served by its next repair.
+  make room.  On the host each graph keeps a checkpoint of its edge set (taken from the device
+  store once the batches served since the last one hold about m operations) and those batches;
+  an evicted graph is re-admitted by a solve from the checkpoint with the batches applied.
+- Continuous batching.  Requests (edge-update batches for one graph) queue per graph.  The
+  scheduler takes the requests queued for a graph at that moment (up to max_fuse, and while
+  their operations fit 2^21) and repairs them as one group: their deletions and insertions go
+  to the device as one ordered sequence (dlp_lp_apply_ops), where every pair ends as its last
+  operation leaves it, which is what applying the requests in arrival order gives, with no
+  host-side merge.  Requests that arrive while the graph is busy are served by its next repair.
 - Fused push and overlap.  One scheduler thread takes, per cycle, the queued requests of every
   idle resident graph and runs the first half of each repair (apply, re-ground, changed-row
   refresh).  The graphs whose change a push can absorb are then repaired by ONE push across
@@ -23,8 +25,8 @@ the serving layer the workload asks for:
   overlap on the device with each other and with the next cycle's push.
 - Failures.  A request whose batch is malformed (ragged arrays, ids outside the graph) is
   refused on its own, before fusing; a repair that raises fails the requests fused into it and
-  drops the graph's device state, which is rebuilt (initial solve plus the updates already
-  served) when the graph's next request is taken.  Either way the request carries `error`.
+  drops the graph's device state, which is rebuilt from the checkpoint and the batches served
+  since (the failed ones not among them) when the graph's next request is taken.  Either way the request carries `error`.
 - Latency is arrival -> certified result; throughput counts edge updates.
 """
 from __future__ import annotations
@@ -42,10 +44,9 @@ import torch
 from .graphs import EdgeBatch, Graph, check_batch
 from .stream import EdgeStreamLP
 
-# the device apply takes at most 2^19 insertions per call: requests are fused up to that many,
-# and a replayed history is applied in chunks of half of it
-_MAX_INSERTIONS = 1 << 19
-_REPLAY_CHUNK = 1 << 18
+# operations (deletions + insertions) of the requests fused into one repair; a single larger
+# request is served alone (the device apply takes it in calls of 2^19 operations)
+_MAX_GROUP_OPS = 1 << 21
 
 
 @dataclass
@@ -90,16 +91,6 @@ def fuse(batches):
     return EdgeBatch(k[ins] // n, k[ins] % n, w[ins], k[dele] // n, k[dele] % n)
 
 
-def replay_chunks(b: EdgeBatch, chunk=_REPLAY_CHUNK):
-    """A fused batch (every pair once, as a deletion or an insertion) split into batches of at
-    most `chunk` operations; since no pair repeats, applying them in any order gives b."""
-    ops = [(b.del_u[i:i + chunk], b.del_v[i:i + chunk]) for i in range(0, len(b.del_u), chunk)]
-    out = [EdgeBatch(b.ins_u[:0], b.ins_v[:0], b.ins_w[:0], du, dv) for du, dv in ops]
-    out += [EdgeBatch(b.ins_u[i:i + chunk], b.ins_v[i:i + chunk], b.ins_w[i:i + chunk], b.del_u[:0], b.del_v[:0])
-            for i in range(0, len(b.ins_u), chunk)]
-    return out
-
-
 def footprint_bytes(g: Graph, **lp_kwargs):
     """Device bytes of one resident graph with its global solve reserved (EdgeStreamLP.footprint:
     store with slack, state, direction slots, coarse operator, handle scratch)."""
@@ -118,7 +109,9 @@ class Slot:
     bytes: int = 0
     served: int = 0
     solves: int = 0
-    history: list = field(default_factory=list)   # batches served, replayed after an eviction
+    base: Graph | None = None     # checkpoint of the edge set (None: the graph as given)
+    history: list = field(default_factory=list)   # batches served since the checkpoint
+    hist_ops: int = 0
 
 
 class Service:
@@ -134,13 +127,14 @@ class Service:
     the workers too."""
 
     def __init__(self, graphs, hbm_budget_bytes, slots=4, eps=1e-6, max_fuse=64, lp_kwargs=None,
-                 fused_push=True, push_rounds_per_launch=4, push_max_rounds=200_000):
+                 fused_push=True, push_rounds_per_launch=4, push_max_rounds=200_000, progress=None):
         self.slots_n = max(1, int(slots))
         self.budget = int(hbm_budget_bytes)
         self.eps = eps
         self.max_fuse = max_fuse
         self.lp_kwargs = dict(lp_kwargs or {})
         self.fused_push = bool(fused_push)
+        self.progress = progress      # progress(kind, detail): preload steps and every 5 % of requests served
         self.push_rpl, self.push_max_rounds = int(push_rounds_per_launch), int(push_max_rounds)
         self.graphs = {i: Slot(i, g) for i, g in enumerate(graphs)}
         for s in self.graphs.values():
@@ -154,20 +148,39 @@ class Service:
         self.stats = {"admissions": 0, "evictions": 0, "fused_requests": 0, "repairs": 0, "refused": 0,
                       "failed_repairs": 0, "cycles": 0, "push_launches": 0, "push_graphs": 0,
                       "global_repairs": 0}
+        # seconds spent, summed over threads: host checks of the requests, first halves (apply,
+        # ground, refresh), the pushes across graphs, verification after a push, global repairs
+        self.times = {"checks": 0.0, "begin": 0.0, "push": 0.0, "verify": 0.0, "global": 0.0, "build": 0.0}
         self._jobs = deque()          # global repairs and builds for the workers
         self._running = 0             # jobs the workers are on
         self._plan_obj, self._plan_key = None, None
+        self._n_req = 0
         self._serials = itertools.count()
         self._t0 = None
         self._stop = False
 
     # -- residency --------------------------------------------------------------------------
     def _new_state(self, s: Slot):
-        lp = EdgeStreamLP(s.graph, eps=self.eps, **self.lp_kwargs)
+        """A state built from the checkpoint, solved, with the batches served since then applied
+        (in order, as one repair): an evicted or failed graph re-enters with every update it
+        had served."""
+        lp = EdgeStreamLP(s.base if s.base is not None else s.graph, eps=self.eps, **self.lp_kwargs)
         lp.initial_solve()
+        if s.history:
+            lp.apply_group_and_solve(s.history)
         s.solves += 1
         lp.serial = next(self._serials)
         return lp
+
+    def _served(self, s: Slot, batches):
+        """Record batches served on s; past about m operations since the checkpoint, the device
+        edge set becomes the new checkpoint (host memory stays O(m) per graph).  Called by the
+        thread that repaired s, while s is busy."""
+        s.history.extend(batches)
+        s.hist_ops += sum(b.size for b in batches)
+        if s.hist_ops > max(1 << 20, s.graph.m) and s.lp is not None:
+            s.base = s.lp.edge_graph()
+            s.history, s.hist_ops = [], 0
 
     def _admit(self, s: Slot):
         """Called with the lock held: True when s is resident, "build" when it is admitted now
@@ -193,14 +206,14 @@ class Service:
         return "build"
 
     def _take(self, s: Slot):
-        """Queued requests of s fused while their insertions fit one device apply call."""
-        reqs, ins = [], 0
+        """Queued requests of s, in arrival order, while their operations fit one group."""
+        reqs, ops = [], 0
         while s.queue and len(reqs) < self.max_fuse:
-            k = len(s.queue[0].batch.ins_u)
-            if reqs and ins + k > _MAX_INSERTIONS:
+            k = s.queue[0].batch.size
+            if reqs and ops + k > _MAX_GROUP_OPS:
                 break
             reqs.append(s.queue.popleft())
-            ins += k
+            ops += k
         return reqs
 
     def _plan(self):
@@ -226,7 +239,12 @@ class Service:
         self._plan_obj, self._plan_key = None, None
 
     # -- completion ---------------------------------------------------------------------------
-    def _complete(self, s: Slot, good, bad, b, st, err):
+    def _complete(self, s: Slot, good, bad, st, err):
+        if err is None and good:
+            try:
+                self._served(s, [r.batch for r in good])
+            except Exception as e:  # noqa: BLE001  (checkpoint failed: the state is suspect)
+                err = f"checkpoint failed: {type(e).__name__}: {e}"
         now = time.perf_counter() - self._t0
         with self.cv:
             for r in bad:
@@ -243,14 +261,14 @@ class Service:
             else:
                 for r in good:
                     r.done, r.rho, r.fused_with = now, st.rho if st is not None else float("nan"), len(good)
-                if b is not None:
-                    s.history.append(b)
-                    if len(s.history) > 32:     # keep the replay log one fused batch
-                        s.history = [fuse(s.history)]
                 s.served += len(good)
                 self.stats["fused_requests"] += len(good)
-                self.stats["repairs"] += int(b is not None)
+                self.stats["repairs"] += int(bool(good))
+            before = len(self.done)
             self.done.extend(good + bad)
+            if self.progress and self._n_req and (len(self.done) * 20) // self._n_req > (before * 20) // self._n_req:
+                self.progress("served", f"{len(self.done)}/{self._n_req} requests at {now:.2f}s, "
+                                        f"{self.stats['repairs']} repairs, {self.stats['cycles']} cycles")
             self.stats["refused"] += len(bad)
             s.busy = False
             s.last_used = now
@@ -261,7 +279,7 @@ class Service:
         bad = []
         for r in reqs:
             try:
-                check_batch(r.batch, s.graph.n, max_insertions=_MAX_INSERTIONS)
+                check_batch(r.batch, s.graph.n)
             except (IndexError, ValueError) as e:
                 r.error = f"refused: {e}"
                 bad.append(r)
@@ -279,24 +297,23 @@ class Service:
                         self.cv.wait(timeout=0.05)
                     job = self._jobs.popleft()
                     self._running += 1
-                kind, s, good, bad, b, req = job
+                kind, s, good, bad, req = job
                 st, err = None, None
+                t_job = time.perf_counter()
                 try:
                     if kind == "build":
-                        lp = self._new_state(s)
-                        # an evicted graph re-enters with the updates it had already served
-                        for piece in (replay_chunks(fuse(s.history)) if s.history else []):
-                            lp.apply_and_solve(piece)
-                        s.lp = lp
+                        s.lp = self._new_state(s)
                         if good:
-                            b = fuse([r.batch for r in good])
-                            st = s.lp.apply_and_solve(b)
+                            st = s.lp.apply_group_and_solve([r.batch for r in good])
                     else:
                         st = s.lp.end_request(req, pushed=req.get("pushed"))
                     stream.synchronize()
                 except Exception as e:  # noqa: BLE001  (reported on the requests, service goes on)
                     err = f"repair failed: {type(e).__name__}: {e}"
-                self._complete(s, good, bad, b, st, err)
+                self.times["build" if kind == "build" else "global"] += time.perf_counter() - t_job
+                if kind != "build":
+                    self.stats["global_repairs"] += 1
+                self._complete(s, good, bad, st, err)
                 with self.cv:
                     self._running -= 1
                     self.cv.notify_all()
@@ -327,7 +344,7 @@ class Service:
                             reqs = self._take(s)
                             if a == "build":
                                 good, bad = self._split(s, reqs)
-                                self._jobs.append(("build", s, good, bad, None, None))
+                                self._jobs.append(("build", s, good, bad, None))
                             else:
                                 picks.append((s, reqs))
                         if picks:
@@ -337,17 +354,20 @@ class Service:
                 self.stats["cycles"] += 1
                 begun = []
                 for s, reqs in picks:
+                    t = time.perf_counter()
                     good, bad = self._split(s, reqs)
                     if not good:
-                        self._complete(s, good, bad, None, None, None)
+                        self._complete(s, good, bad, None, None)
                         continue
-                    b = fuse([r.batch for r in good])
+                    t1 = time.perf_counter()
+                    self.times["checks"] += t1 - t
                     try:
-                        begun.append((s, good, bad, b, s.lp.begin_request(b)))
+                        begun.append((s, good, bad, s.lp.begin_group([r.batch for r in good])))
                     except Exception as e:  # noqa: BLE001
-                        self._complete(s, good, bad, b, None, f"repair failed: {type(e).__name__}: {e}")
+                        self._complete(s, good, bad, None, f"repair failed: {type(e).__name__}: {e}")
+                    self.times["begin"] += time.perf_counter() - t1
                 stream.synchronize()
-                pushers = [x for x in begun if self.fused_push and x[0].lp.push_planned(x[4]["status"])]
+                pushers = [x for x in begun if self.fused_push and x[0].lp.push_planned(x[3]["status"])]
                 with self.cv:
                     for x in begun:
                         if x not in pushers:
@@ -355,13 +375,14 @@ class Service:
                     self.cv.notify_all()
                 if not pushers:
                     continue
+                t = time.perf_counter()
                 try:
                     plan, order = self._plan()
                     pos = {g: i for i, g in enumerate(order)}
                     nh = len(order)
                     active = np.zeros(nh, dtype=np.int32)
                     budget = np.zeros(nh, dtype=np.int64)
-                    for s, _, _, _, _ in pushers:
+                    for s, _, _, _ in pushers:
                         active[pos[s.gid]] = 1
                         budget[pos[s.gid]] = s.lp.push_budget_edges()
                     rounds = np.zeros(nh, dtype=np.int64)
@@ -370,14 +391,15 @@ class Service:
                               budget.ctypes.data, rounds.ctypes.data, conv.ctypes.data, stream.cuda_stream)
                     self.stats["push_launches"] += 1
                     self.stats["push_graphs"] += len(pushers)
+                    self.times["push"] += time.perf_counter() - t
                 except Exception as e:  # noqa: BLE001
-                    for s, good, bad, b, _ in pushers:
-                        self._complete(s, good, bad, b, None, f"repair failed: {type(e).__name__}: {e}")
+                    for s, good, bad, _ in pushers:
+                        self._complete(s, good, bad, None, f"repair failed: {type(e).__name__}: {e}")
                     continue
                 # converged graphs are verified here; the rest (edge budget spent) go on to a
                 # global solve in a worker
                 for x in pushers:
-                    s, good, bad, b, req = x
+                    s, good, bad, req = x
                     i = pos[s.gid]
                     req["pushed"] = (int(rounds[i]), bool(conv[i]))
                     if not conv[i]:
@@ -386,25 +408,32 @@ class Service:
                             self.cv.notify_all()
                         continue
                     st, err = None, None
+                    t = time.perf_counter()
                     try:
                         st = s.lp.end_request(req, pushed=req["pushed"])
                         stream.synchronize()
                     except Exception as e:  # noqa: BLE001
                         err = f"repair failed: {type(e).__name__}: {e}"
-                    self._complete(s, good, bad, b, st, err)
+                    self.times["verify"] += time.perf_counter() - t
+                    self._complete(s, good, bad, st, err)
 
     # -- driver ----------------------------------------------------------------------------
     def run(self, requests, preload=True):
         """Serve `requests` (list of Request with arrival times in seconds from start, sorted)
         open-loop; returns the completed requests.  preload: admit and solve every graph that
         fits before the clock starts."""
+        self._n_req = len(requests)
         if preload:
             for s in sorted(self.graphs.values(), key=lambda x: x.bytes):
                 if self.resident + s.bytes <= self.budget:
+                    t = time.perf_counter()
                     s.lp = self._new_state(s)
                     s.reserved = True
                     self.resident += s.bytes
                     self.stats["admissions"] += 1
+                    if self.progress:
+                        self.progress("preload", f"graph {s.gid} n={s.graph.n} k={s.graph.K}: "
+                                                 f"{(time.perf_counter() - t) * 1e3:.0f} ms")
         torch.cuda.synchronize()
         self._t0 = time.perf_counter()
         self._stop = False
@@ -447,7 +476,7 @@ class Service:
                 "max_rho": max((r.rho for r in ok), default=float("nan")),
                 "mean_fused": float(np.mean([r.fused_with for r in ok])) if ok else 0.0,
                 "graphs_per_push_launch": self.stats["push_graphs"] / pl if pl else 0.0,
-                **self.stats}
+                **self.stats, "seconds": {k: round(v, 3) for k, v in self.times.items()}}
 
 
 def poisson_requests(graphs, rate, duration, size_lo=100, size_hi=100_000, seed=0, max_requests=None):
@@ -466,4 +495,55 @@ def poisson_requests(graphs, rate, duration, size_lo=100, size_hi=100_000, seed=
         b = draw_batch(sets[gid], size, 0.5, rng, unit=bool(np.all(graphs[gid].w == 1)))
         sets[gid].apply(b)
         out.append(Request(gid, b, t))
+    return out
+
+
+def stream_requests(graphs, rate, duration, size_lo=100, size_hi=100_000, seed=0, max_requests=None):
+    """Open-loop arrivals at `rate` req/s over `duration` s (seed S+4) for workloads too large
+    for `poisson_requests`' exact sampling: graph uniform, size log-uniform in [size_lo,
+    size_hi], half insertions of uniform pairs (u != v, weights as the graph's: 1 or
+    U[0.5, 1.5]) and half deletions.  Each graph keeps its present edges in a FIFO: a deletion
+    takes the oldest (the initial edges in a random order first, then the inserted ones in
+    insertion order) and an insertion joins the back, so every deletion names an edge present
+    at that point of the stream (a uniform insertion that hits a present pair, probability
+    about deg/n, makes that pair's second deletion a missing one)."""
+    rng = np.random.default_rng(seed + 4)
+    arr, gids, sizes = [], [], []
+    t = 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration or (max_requests and len(arr) >= max_requests):
+            break
+        arr.append(t)
+    nr = len(arr)
+    gids = rng.integers(0, len(graphs), size=nr)
+    sizes = np.exp(rng.uniform(np.log(size_lo), np.log(size_hi), size=nr)).astype(np.int64)
+    n_ins = sizes // 2
+    n_del = sizes - n_ins
+    out = [None] * nr
+    for gid, g in enumerate(graphs):
+        mine = np.flatnonzero(gids == gid)
+        if not mine.size:
+            continue
+        tot_ins = int(n_ins[mine].sum())
+        m = g.u.shape[0]
+        perm = rng.permutation(m)
+        fu = np.empty(m + tot_ins, dtype=np.int32)
+        fv = np.empty(m + tot_ins, dtype=np.int32)
+        fu[:m], fv[:m] = g.u[perm], g.v[perm]
+        a = rng.integers(0, g.n, size=tot_ins, dtype=np.int64)
+        b = rng.integers(0, g.n - 1, size=tot_ins, dtype=np.int64)
+        b = b + (b >= a)                              # b != a
+        fu[m:], fv[m:] = a, b
+        unit = bool(np.all(g.w == 1))
+        ws = np.ones(tot_ins, np.float32) if unit else rng.uniform(0.5, 1.5, size=tot_ins).astype(np.float32)
+        head, tail = 0, m                             # FIFO [head, tail)
+        for r in mine:
+            nd = int(min(n_del[r], tail - head))
+            ni = int(n_ins[r])
+            b_ = EdgeBatch(fu[tail:tail + ni], fv[tail:tail + ni], ws[tail - m:tail - m + ni],
+                           fu[head:head + nd], fv[head:head + nd])
+            head += nd
+            tail += ni
+            out[r] = Request(gid, b_, float(arr[r]))
     return out

@@ -23,8 +23,10 @@ import torch
 
 from . import _lib
 from .backend import device, ptr, stream
-from .graphs import EdgeBatch, Graph, normalize_batch
+from .graphs import EdgeBatch, Graph, check_batch, normalize_batch, pack_ops
 from .solvers import System  # noqa: F401  (the batch system, reference stream.py:17-58)
+
+_OPS_PER_CALL = 1 << 19     # operations one dlp_lp_apply_ops call takes (its pair table's capacity / 2)
 
 
 @dataclass
@@ -351,22 +353,26 @@ class EdgeStreamLP:
         (the previous request has finished with both buffers: every request ends in a sync).
         Returns the device views and the batch as applied (repeated pairs collapsed)."""
         b = normalize_batch(b, self.n, max_insertions=1 << 19)   # the device apply's per-call limit
-        nd, ni = len(b.del_u), len(b.ins_u)
-        total = 2 * nd + 3 * ni
+        du, dv, iu, iv, iw = self._stage_arrays(b.del_u, b.del_v, b.ins_u, b.ins_v,
+                                                np.asarray(b.ins_w, dtype=np.float32).view(np.int32))
+        return (du, dv, iu, iv, iw.view(torch.float32)), b
+
+    def _stage_arrays(self, *arrays):
+        """int32 host arrays packed into the pinned staging buffer and sent with one copy;
+        returns their device views (valid until the next staging: callers synchronise first)."""
+        total = sum(len(a) for a in arrays)
         if self._stage is None or self._stage.numel() < total:
             cap = max(total, 1 << 16)
             self._stage = torch.empty(cap, dtype=torch.int32, pin_memory=True)
             self._stage_dev = torch.empty(cap, dtype=torch.int32, device=device())
         h = self._stage.numpy()
-        h[:nd] = b.del_u
-        h[nd:2 * nd] = b.del_v
-        o = 2 * nd
-        h[o:o + ni] = b.ins_u
-        h[o + ni:o + 2 * ni] = b.ins_v
-        h[o + 2 * ni:total] = np.asarray(b.ins_w, dtype=np.float32).view(np.int32)
+        offs = [0]
+        for a in arrays:
+            h[offs[-1]:offs[-1] + len(a)] = a
+            offs.append(offs[-1] + len(a))
         d = self._stage_dev
         d[:total].copy_(self._stage[:total], non_blocking=True)
-        return (d[:nd], d[nd:2 * nd], d[o:o + ni], d[o + ni:o + 2 * ni], d[o + 2 * ni:total].view(torch.float32)), b
+        return [d[offs[i]:offs[i + 1]] for i in range(len(arrays))]
 
     def _regrow(self, ins, extra):
         iu, iv, iw = ins
@@ -448,21 +454,67 @@ class EdgeStreamLP:
             _lib.call("dlp_lp_ground", self.h, stream())
         _lib.call("dlp_lp_refresh", self.h, 0, 1 if self.fp64 else 0, stream())
         s = None if (self.fp64 or self._batch_edges > self.push_max_rows) else self.status()
-        return {"st": st, "t0": t0, "b": b, "ins": (iu, iv, iw), "status": s}
+        return {"st": st, "t0": t0, "ins": (iu, iv, iw), "ins_rows": (b.ins_u, b.ins_v), "status": s}
+
+    def begin_group(self, batches):
+        """begin_request for several batches applied in order, as one repair: their operations
+        go to the device as one ordered sequence (dlp_lp_apply_ops; each pair ends as its last
+        operation leaves it, no host-side merge), in calls of at most 2^19 operations (a call
+        that overflows a row is regrown before the next one).  Each batch is checked as
+        apply_and_solve checks it; a pair repeated within a batch is fine."""
+        st = Stats(path="fp64-push" if self.fp64 else "push")
+        t0 = time.perf_counter()
+        for b in batches:
+            check_batch(b, self.n)
+        u, v, w = pack_ops(batches)
+        nops = int(u.shape[0])
+        self._batch_edges = nops
+        _lib.call("dlp_lp_begin_batch", self.h, stream())
+        lo = 0
+        ops = None
+        while True:
+            hi = min(nops, lo + _OPS_PER_CALL)
+            du, dv, dw = self._stage_arrays(u[lo:hi], v[lo:hi], w[lo:hi].view(np.int32))
+            ops = (du, dv, dw.view(torch.float32))
+            _lib.call("dlp_lp_apply_ops", self.h, hi - lo, ptr(ops[0]), ptr(ops[1]), ptr(ops[2]), stream())
+            ins_rows = (u[lo:hi][w[lo:hi] > 0], v[lo:hi][w[lo:hi] > 0])
+            if hi >= nops:
+                break
+            s = self.status()      # also: the staging buffer is free again
+            if s["overflow"]:
+                self._regrow(ops, self._regrow_extra(ins_rows))
+            lo = hi
+        if self.incremental_ground and nops <= _OPS_PER_CALL:
+            # every endpoint is checked (a superset of the deletions' endpoints)
+            _lib.call("dlp_lp_ground_incremental", self.h, nops, ptr(ops[0]), ptr(ops[1]), nops, stream())
+        else:
+            _lib.call("dlp_lp_ground", self.h, stream())
+        _lib.call("dlp_lp_refresh", self.h, 0, 1 if self.fp64 else 0, stream())
+        s = None if (self.fp64 or nops > self.push_max_rows) else self.status()
+        return {"st": st, "t0": t0, "ins": ops, "ins_rows": ins_rows, "status": s}
+
+    def apply_group_and_solve(self, batches) -> Stats:
+        """Several batches applied in order and repaired once (begin_group + end_request)."""
+        return self.end_request(self.begin_group(batches))
+
+    @staticmethod
+    def _regrow_extra(ins_rows):
+        """Slack per row that holds every insertion of the call in any one row."""
+        r = np.concatenate([np.asarray(ins_rows[0]), np.asarray(ins_rows[1])])
+        return max(8, int(np.bincount(r).max()) if r.size else 0)
 
     def end_request(self, req, pushed=None):
         """Second half: the repair (with pushed = (rounds, converged) of a push already run on
         this batch), a regrow after overflowed insertions, and the request's statistics."""
-        st, b = req["st"], req["b"]
+        st = req["st"]
         prec = 1 if self.fp64 else 0
         s = self._repair(st, pushed=pushed, s0=req["status"])
         if s["invalid"] or s["duplicates"]:
             raise RuntimeError(f"device apply refused {s['invalid']} pairs and merged {s['duplicates']} "
                                "repeated entries that the host checks had passed")
         if s["overflow"]:
-            # room for every insertion of this batch in any one row, so one regrow suffices
-            per_row = np.bincount(np.concatenate([b.ins_u, b.ins_v]), minlength=1).max()
-            self._regrow(req["ins"], max(8, int(per_row)))
+            # room for every insertion of this call in any one row, so one regrow suffices
+            self._regrow(req["ins"], self._regrow_extra(req["ins_rows"]))
             _lib.call("dlp_lp_ground", self.h, stream())   # re-inserted edges can re-ground rows
             _lib.call("dlp_lp_refresh", self.h, 0, prec, stream())
             s = self._repair(st)
@@ -574,6 +626,20 @@ class EdgeStreamLP:
 
     def stored_entries(self):
         return int(self.rowlen.sum().item())
+
+    def edge_graph(self) -> Graph:
+        """The graph as the device store holds it now (every live entry u < v with its weight),
+        with this graph's labels: a checkpoint from which a new state can be built."""
+        import dataclasses
+        ip = self.indptr.cpu().numpy().astype(np.int64)
+        rl = self.rowlen.cpu().numpy().astype(np.int64)
+        col, w = self.col.cpu().numpy(), self.w.cpu().numpy()
+        caps = np.diff(ip)
+        rows = np.repeat(np.arange(self.n, dtype=np.int64), caps)
+        live = (np.arange(ip[-1], dtype=np.int64) - np.repeat(ip[:-1], caps)) < np.repeat(rl, caps)
+        r, c, x = rows[live], col[live].astype(np.int64), w[live]
+        keep = c > r
+        return dataclasses.replace(self.g, u=r[keep], v=c[keep], w=x[keep].astype(np.float32))
 
     def full_check(self):
         """FP64 residual of every row: (rho per column, violators) — the periodic full refresh

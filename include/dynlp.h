@@ -362,6 +362,17 @@ int dlp_lp_begin_batch(void* handle, void* stream);
  * (stream.py:202-228) and the weight/row-sum/class-weight updates (stream.py:221-235). */
 int dlp_lp_apply(void* handle, int nd, const int32_t* du, const int32_t* dv, int ni,
                  const int32_t* iu, const int32_t* iv, const float* iw, void* stream);
+/* A sequence of nops <= 2^19 pair operations (u[s], v[s], w[s]) applied as if one after the
+ * other: w[s] == 0 deletes the pair, w[s] > 0 inserts it (or replaces its weight).  Each
+ * unordered pair ends as its last operation in the sequence leaves it, so the requests of a
+ * group are applied in arrival order by one call (their deletions and insertions interleaved)
+ * with no host-side merge.  Invalid operations (ids outside [0, n), a weight that is negative,
+ * NaN or infinite) are skipped and counted (out[16]); a deletion that is its pair's last
+ * operation and finds no stored edge counts as missing; superseded operations are not counted.
+ * Overflowed insertions are re-inserted by dlp_lp_regrow_commit with ni = nops and (u, v, w).
+ * Successive calls in one batch apply in order.  Replaces the same part of the reference as
+ * dlp_lp_apply, for a group of update batches (stream.py:202-235 applied per batch in order). */
+int dlp_lp_apply_ops(void* handle, int nops, const int32_t* u, const int32_t* v, const float* w, void* stream);
 /* New row offsets with room to grow (rowlen + max(extra, growth*rowlen)); *total = capacity
  * (host; synchronises). */
 int dlp_lp_regrow_plan(void* handle, int extra, double growth, int32_t* new_indptr, int64_t* total,

@@ -124,17 +124,17 @@ def test_failed_builds_fail_their_requests_and_the_service_drains():
     assert svc.resident == 0
 
 
-def test_fusion_stops_at_the_insertion_limit(monkeypatch):
-    """Requests queued together are fused only while their insertions fit one device apply call
-    (the limit lowered here so that two of these requests fit); every request is served and the
-    graph ends at the exact solution of all of them."""
+def test_fusion_stops_at_the_group_limit(monkeypatch):
+    """Requests queued together are fused only while their operations fit one group (the limit
+    lowered here so that two of these requests fit); every request is served and the graph ends
+    at the exact solution of all of them."""
     from dynlp_amd import serve
     gs = _graphs()[:1]
     reqs = serve.poisson_requests(gs, rate=1e9, duration=1.0, size_lo=600, size_hi=601, seed=7, max_requests=6)
-    ins = [len(r.batch.ins_u) for r in reqs]
-    limit = max(a + b for a, b in zip(ins, ins[1:]))
-    assert limit < min(ins) * 3
-    monkeypatch.setattr(serve, "_MAX_INSERTIONS", limit)
+    ops = [r.batch.size for r in reqs]
+    limit = max(a + b for a, b in zip(ops, ops[1:]))
+    assert limit < min(ops) * 3
+    monkeypatch.setattr(serve, "_MAX_GROUP_OPS", limit)
     svc = serve.Service(gs, serve.footprint_bytes(gs[0]) + (1 << 20), slots=1, eps=EPS)
     svc.run(reqs)
     rep = svc.report()
@@ -144,6 +144,36 @@ def test_fusion_stops_at_the_insertion_limit(monkeypatch):
     Z = svc.graphs[0].lp.scores_with_h()
     assert np.abs(Z[:, : sysm.C] - Fs[:, : sysm.C]).max() <= EPS * Fs[:, sysm.C].max() * 1.05
 
+
+def test_evicted_graph_returns_with_its_updates():
+    """Two graphs under a budget that holds one: requests alternate between them, so each
+    re-admission rebuilds a graph from its checkpoint (lowered so that checkpoints are taken)
+    plus the batches served since; both end at the exact solution of all their updates."""
+    from dynlp_amd import serve
+    gs = _graphs()[:2]
+    reqs = serve.poisson_requests(gs, rate=300.0, duration=10.0, size_lo=50, size_hi=300, seed=11,
+                                  max_requests=16)
+    sizes = [serve.footprint_bytes(g) for g in gs]
+    svc = serve.Service(gs, max(sizes) + (1 << 20), slots=1, eps=EPS)
+    orig = svc._served
+
+    def served(s, batches):            # checkpoint after every repair
+        orig(s, batches)
+        if s.lp is not None and s.history:
+            s.base = s.lp.edge_graph()
+            s.history, s.hist_ops = [], 0
+    svc._served = served
+    svc.run(reqs, preload=False)
+    rep = svc.report()
+    assert rep["errors"] == 0 and rep["requests"] == len(reqs) and rep["evictions"] >= 2
+    for gid, g in enumerate(gs):
+        mine = [r for r in reqs if r.gid == gid]
+        sysm, Fs = _oracle_hmax(g, mine)
+        slot = svc.graphs[gid]
+        if slot.lp is None:
+            slot.lp = svc._new_state(slot)
+        Z = slot.lp.scores_with_h()
+        assert np.abs(Z[:, : sysm.C] - Fs[:, : sysm.C]).max() <= EPS * Fs[:, sysm.C].max() * 1.05, gid
 
 def test_push_across_graphs_matches_oracle():
     """dlp_lp_push_multi through the C ABI: three graphs of different column counts (so more

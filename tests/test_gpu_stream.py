@@ -619,3 +619,64 @@ def test_fp64_pcg_repair_matches_rebuilt_system(K):
         O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
                            zip(b.del_u.tolist(), b.del_v.tolist()))
         check_against_oracle(lp, g, es, eps, f"batch {t}")
+
+
+def _conflicting_batches(g, es, rng):
+    """Batches whose operations conflict across and within batches: new pairs inserted and later
+    deleted, present edges deleted and later re-inserted with new weights, a pair inserted in
+    both orientations, a deletion repeated, a labeled vertex's edges removed and restored."""
+    from dynlp_amd.graphs import EdgeBatch
+    i32, f32 = (lambda x: np.asarray(x, dtype=np.int32)), (lambda x: np.asarray(x, dtype=np.float32))
+    keys = sorted(es)
+    new = []
+    while len(new) < 60:
+        a, b = sorted(rng.integers(0, g.n, 2).tolist())
+        if a != b and (a, b) not in es and (a, b) not in new:
+            new.append((a, b))
+    lab = int(np.flatnonzero(g.labeled)[0])
+    nbrs = [k for k in keys if lab in k]
+    A = EdgeBatch(i32([a for a, _ in new]), i32([b for _, b in new]), f32(rng.uniform(0.5, 2, 60)),
+                  i32([a for a, _ in keys[:30]]), i32([b for _, b in keys[:30]]))
+    B = EdgeBatch(i32([b for _, b in keys[:15]] + [new[0][1]]), i32([a for a, _ in keys[:15]] + [new[0][0]]),
+                  f32(list(rng.uniform(0.5, 2, 15)) + [3.0]),
+                  i32([a for a, _ in new[:20]] + [new[5][1]]), i32([b for _, b in new[:20]] + [new[5][0]]))
+    C = EdgeBatch(i32([a for a, _ in new[10:20]] + [new[1][0], new[1][1]]),
+                  i32([b for _, b in new[10:20]] + [new[1][1], new[1][0]]),
+                  f32(list(rng.uniform(0.5, 2, 10)) + [1.25, 2.5]),
+                  i32([a for a, _ in nbrs] + [new[30][0]]), i32([b for _, b in nbrs] + [new[30][1]]))
+    D = EdgeBatch(i32([a for a, _ in nbrs[:2]]), i32([b for _, b in nbrs[:2]]), f32([0.9, 1.1]),
+                  i32([new[40][0], new[40][0]]), i32([new[40][1], new[40][1]]))
+    return [A, B, C, D]
+
+
+@pytest.mark.parametrize("ops_per_call", [None, 37])
+def test_request_group_matches_sequential_oracle(monkeypatch, ops_per_call):
+    """A group of conflicting batches applied by dlp_lp_apply_ops (one ordered sequence, or
+    calls of 37 operations) and repaired once: the device edge set equals the oracle's after
+    applying the batches one after the other (weights bit-exact), and the solution, solved set
+    and CC labels match the oracle.  With small calls and no slack the rows overflow and are
+    regrown between calls."""
+    from dynlp_amd import stream as S
+    from dynlp_amd.stream import EdgeStreamLP
+    if ops_per_call:
+        monkeypatch.setattr(S, "_OPS_PER_CALL", ops_per_call)
+    g = graphs.sbm(3000, 4, 8, 0.8, 4, 0.03, seed=12)
+    eps = 1e-6
+    lp = EdgeStreamLP(g, eps=eps, slack_min=0 if ops_per_call else 4, slack_frac=0.0 if ops_per_call else 0.25)
+    lp.initial_solve()
+    cap0 = lp.col.numel()
+    es = edge_dict(g)
+    rng = np.random.default_rng(12)
+    for rep in range(2):
+        bs = _conflicting_batches(g, es, rng)
+        st = lp.apply_group_and_solve(bs)
+        for b in bs:
+            O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
+                               zip(b.del_u.tolist(), b.del_v.tolist()))
+        assert st.rho <= eps * 1.0001
+        ge = lp.edge_graph()
+        got = {(int(a), int(b)): float(w) for a, b, w in zip(ge.u, ge.v, ge.w)}
+        assert got == {k: float(np.float32(v)) for k, v in es.items()}, rep
+        check_against_oracle(lp, g, es, eps, f"group {rep}")
+    if ops_per_call:
+        assert lp.col.numel() > cap0      # the store was regrown

@@ -1,5 +1,5 @@
-"""Host logic of the update service (no device): fusing request batches and replaying a fused
-history in chunks."""
+"""Host logic of the update service (no device): fusing request batches, the ordered operation
+sequence of a request group, the serving request stream."""
 import numpy as np
 
 from oracle import lp as O
@@ -37,21 +37,78 @@ def test_fused_history_equals_sequential_application():
         assert seq == fused, trial
 
 
-def test_replay_chunks_equal_the_fused_batch():
-    from dynlp_amd.serve import fuse, replay_chunks
-    rng = np.random.default_rng(4)
-    for trial in range(20):
-        base = {(a, b): 2.0 for a, b in np.unique(np.sort(rng.integers(0, 50, (80, 2)), axis=1), axis=0).tolist()
-                if a != b}
-        f = fuse(_random_batches(50, 40, 4, rng))
-        one, chunked = dict(base), dict(base)
-        _apply(one, f)
-        pieces = replay_chunks(f, chunk=int(rng.integers(1, 7)))
-        assert sum(p.size for p in pieces) == f.size
-        for p in pieces:
-            _apply(chunked, p)
-        assert one == chunked, trial
+def _last_operation(es, u, v, w):
+    """The ordered operation sequence as dlp_lp_apply_ops defines it: each unordered pair takes
+    only its last operation (w == 0 deletes, w > 0 inserts with that weight)."""
+    last = {}
+    for s_, (a, b, x) in enumerate(zip(u.tolist(), v.tolist(), w.tolist())):
+        if a != b:
+            last[(min(a, b), max(a, b))] = x
+    for k, x in last.items():
+        if x == 0:
+            es.pop(k, None)
+        else:
+            es[k] = float(x)
 
+
+def test_ordered_operations_equal_sequential_application():
+    """pack_ops lays the batches out so that each pair's last operation is what applying the
+    batches one after the other leaves: per batch its deletions, then its insertions (a batch
+    that deletes and re-inserts a pair keeps it), batches in order.  Pairs repeat within and
+    across batches, in both orientations."""
+    from dynlp_amd.graphs import EdgeBatch, pack_ops
+    rng = np.random.default_rng(4)
+    for trial in range(30):
+        base = {(a, b): 2.0 for a, b in np.unique(np.sort(rng.integers(0, 30, (60, 2)), axis=1), axis=0).tolist()
+                if a != b}
+        bs = []
+        for _ in range(int(rng.integers(1, 7))):
+            ni, nd = int(rng.integers(0, 25)), int(rng.integers(0, 25))
+            iu, iv = rng.integers(0, 30, ni), rng.integers(0, 30, ni)
+            du, dv = rng.integers(0, 30, nd), rng.integers(0, 30, nd)
+            bs.append(EdgeBatch(iu, iv, rng.integers(1, 5, ni).astype(np.float32), du, dv))
+        seq, ops = dict(base), dict(base)
+        for b in bs:
+            # one batch: deletions first, then the last insertion of each pair
+            for a, c in zip(b.del_u.tolist(), b.del_v.tolist()):
+                seq.pop((min(a, c), max(a, c)), None)
+            for a, c, x in zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()):
+                if a != c:
+                    seq[(min(a, c), max(a, c))] = float(x)
+        u, v, w = pack_ops(bs)
+        assert u.dtype == np.int32 and v.dtype == np.int32 and w.dtype == np.float32
+        assert len(u) == sum(b.size for b in bs)
+        _last_operation(ops, u, v, w)
+        assert seq == ops, trial
+
+
+def test_stream_requests_delete_present_edges():
+    """The serving stream: arrivals sorted, sizes within bounds, half insertions of pairs u != v,
+    and every deletion names an edge present at that point of the graph's stream (checked by
+    replaying the stream per graph on a host edge set)."""
+    from dynlp_amd import graphs
+    from dynlp_amd.serve import stream_requests
+    gs = [graphs.erdos_renyi(3000, 8, 4, 0.02, seed=1), graphs.sbm(2000, 4, 10, 0.8, 4, 0.02, seed=2)]
+    reqs = stream_requests(gs, rate=500, duration=1.0, size_lo=10, size_hi=3000, seed=3)
+    assert len(reqs) > 300
+    t = [r.arrival for r in reqs]
+    assert t == sorted(t)
+    sets = [{(int(a), int(b)) for a, b in zip(g.u, g.v)} for g in gs]
+    missing = 0
+    for r in reqs:
+        b = r.batch
+        assert 10 <= b.size <= 3000 and len(b.ins_u) == b.size // 2
+        assert np.all(b.ins_u != b.ins_v) and np.all(b.ins_w > 0)
+        es = sets[r.gid]
+        for a, c in zip(b.del_u.tolist(), b.del_v.tolist()):
+            k = (min(a, c), max(a, c))
+            if k in es:
+                es.remove(k)
+            else:
+                missing += 1
+        for a, c in zip(b.ins_u.tolist(), b.ins_v.tolist()):
+            es.add((min(a, c), max(a, c)))
+    assert missing <= 0.01 * sum(len(r.batch.del_u) for r in reqs)
 
 def test_check_batch_refuses_ragged_and_out_of_range():
     import pytest
