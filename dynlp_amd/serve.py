@@ -240,13 +240,18 @@ class Service:
 
     # -- completion ---------------------------------------------------------------------------
     def _complete(self, s: Slot, good, bad, st, err):
+        held = None
         if err is None and good:
             try:
                 self._served(s, [r.batch for r in good])
+                held = s.lp.resident_bytes()     # a regrown store holds more than the estimate
             except Exception as e:  # noqa: BLE001  (checkpoint failed: the state is suspect)
                 err = f"checkpoint failed: {type(e).__name__}: {e}"
         now = time.perf_counter() - self._t0
         with self.cv:
+            if held is not None and s.reserved and held > s.bytes:
+                self.resident += held - s.bytes
+                s.bytes = held
             for r in bad:
                 r.done = now
             if err is not None:

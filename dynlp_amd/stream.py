@@ -385,7 +385,21 @@ class EdgeStreamLP:
             self.scratch3 = torch.empty(3 << 20, dtype=torch.int32, device=device())
         _lib.call("dlp_lp_regrow_commit", self.h, ptr(new_ptr), ptr(col), ptr(w), int(iu.shape[0]), ptr(iu),
                   ptr(iv), ptr(iw), ptr(self.scratch3), stream())
+        old = (self.indptr, self.col, self.w)
         self.indptr, self.col, self.w = new_ptr, col, w
+        # The copy above still reads the old store.  Its memory returns to the caching allocator
+        # of the stream that allocated it, which may be another thread's: that thread could take
+        # it before the copy has run.  Free it only once this stream is done with it.
+        stream_obj = torch.cuda.current_stream()
+        stream_obj.synchronize()
+        del old
+
+    def resident_bytes(self):
+        """Device bytes this state holds now: its arrays (the store as regrown) and the
+        library handle's buffers."""
+        arrs = [self.indptr, self.rowlen, self.col, self.w, self.labeled, self.y, self.solved, self.labcnt,
+                self.Z32, self.R32, self.diag64, self.diag32, self.Z64, self.R64, self._stage_dev, self.scratch3]
+        return sum(int(a.numel() * a.element_size()) for a in arrs if a is not None) + self.device_bytes()
 
     # -- public ---------------------------------------------------------------------------------
     def initial_solve(self):

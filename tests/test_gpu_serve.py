@@ -239,3 +239,26 @@ def test_service_pushes_graphs_together():
         sysm, Fs = _oracle_hmax(g, mine)
         Z = svc.graphs[gid].lp.scores_with_h()
         assert np.abs(Z[:, : sysm.C] - Fs[:, : sysm.C]).max() <= EPS * Fs[:, sysm.C].max() * 1.05, gid
+
+
+def test_service_regrows_stores_on_many_streams():
+    """Stores with no slack, so that nearly every repair regrows its graph's store, served by
+    four workers (global solves) and the scheduler, each on its own stream: a regrown store's
+    old memory is released only after the copy that reads it (the caching allocator hands
+    memory back to the stream that allocated it, which may be another thread's).  Every graph
+    ends at the exact solution of its updates."""
+    from dynlp_amd.serve import Service, footprint_bytes, stream_requests
+    kw = dict(slack_min=0, slack_frac=0.0)
+    gs = [graphs.sbm(4000 + 1000 * i, 4, 10, 0.8, 4, 0.02, seed=60 + i) for i in range(6)]
+    reqs = stream_requests(gs, rate=3000, duration=0.2, size_lo=200, size_hi=3000, seed=13)
+    svc = Service(gs, 4 * sum(footprint_bytes(g, **kw) for g in gs), slots=4, eps=EPS, lp_kwargs=kw)
+    svc.run(reqs)
+    rep = svc.report()
+    assert rep["errors"] == 0 and rep["requests"] == len(reqs)
+    for gid, g in enumerate(gs):
+        mine = [r for r in reqs if r.gid == gid]
+        sysm, Fs = _oracle_hmax(g, mine)
+        lp = svc.graphs[gid].lp
+        assert lp.col.numel() > 2 * g.m          # regrown past the slack-free store
+        Z = lp.scores_with_h()
+        assert np.abs(Z[:, : sysm.C] - Fs[:, : sysm.C]).max() <= EPS * Fs[:, sysm.C].max() * 1.05, gid
