@@ -1303,6 +1303,7 @@ struct LP {
   int64_t dev_bytes = 0;        // device memory this handle holds (its own buffers)
   int last_refresh_kernel = -1; // kernel of the last refresh: 0 k_refresh, 1 k_refresh_v4, 2 k_verify
   int applies = 0;              // apply calls since the batch began (the changed-row list grows)
+  long long batch_ops = 0;      // pair operations applied since the batch began
   double push_ms = 0.0;         // summed duration of push-round launches (events)
   int64_t push_launched = 0;    // rounds launched
 };
@@ -1605,6 +1606,7 @@ extern "C" int dlp_lp_begin_batch(void* handle, void* stream) {
   LP* h = (LP*)handle;
   h->stampv += 1;
   h->applies = 0;
+  h->batch_ops = 0;
   hipLaunchKernelGGL(k_begin_batch, dim3(1), dim3(64), 0, (hipStream_t)stream, h->cnt,
                      (int32_t*)h->S.bstamp, h->S.bcount, h->S.edges);
   DLP_CHECK_LAUNCH();
@@ -1646,6 +1648,7 @@ extern "C" int dlp_lp_apply(void* handle, int nd, const int32_t* du, const int32
     DLP_CHECK_LAUNCH();
   }
   h->applies += 1;
+  h->batch_ops += (long long)nd + ni;
   return DLP_OK;
 }
 
@@ -1682,6 +1685,7 @@ extern "C" int dlp_lp_apply_ops(void* handle, int nops, const int32_t* u, const 
                      h->stampv, h->rows, h->cnt + CNT_ROWS, h->overflow, h->cnt + CNT_OVERFLOW);
   DLP_CHECK_LAUNCH();
   h->applies += 1;
+  h->batch_ops += nops;
   return DLP_OK;
 }
 
@@ -1818,7 +1822,11 @@ extern "C" int dlp_lp_ground(void* handle, void* stream) {
 extern "C" int dlp_lp_ground_incremental(void* handle, int nd, const int32_t* du, const int32_t* dv, int ni,
                                          void* stream) {
   DLP_CHECK_ARG(handle && nd >= 0 && ni >= 0 && (nd == 0 || (du && dv)), "lp_ground_incremental: bad argument");
-  return ground_impl((LP*)handle, 0, nd, du, dv, ni, (hipStream_t)stream);
+  LP* h = (LP*)handle;
+  // Past n/16 endpoints the bounded searches (one wavefront each) cost more than the full
+  // connected-components pass: run that instead.
+  const int force = 2LL * nd > (long long)h->d.n / 16 ? 1 : 0;
+  return ground_impl(h, force, nd, du, dv, ni, (hipStream_t)stream);
 }
 
 // [0] full pass requested by the last grounding, [1] inert unlabeled vertices, [2] local checks run.
@@ -1871,8 +1879,10 @@ extern "C" int dlp_lp_refresh(void* handle, int which, int prec, void* stream) {
   }
   RefreshArgs A = refresh_args(h, which);
   const RowMap& m = h->map;
-  // listed rows: the changed list is O(batch); the touched list can reach n
-  const unsigned cap = which == 0 ? 512u : (which == 1 ? 2048u : 4096u);
+  // listed rows: the changed list is O(batch) (two rows per pair operation, four rows a block
+  // in the one-row-per-wavefront kernel); the touched list can reach n
+  unsigned cap = which == 0 ? 512u : (which == 1 ? 2048u : 4096u);
+  if (which == 0) cap = (unsigned)std::max<long long>(cap, std::min<long long>(1LL << 20, (2 * h->batch_ops + 3) / 4));
   unsigned grid = persistent_grid(h->d.n, m.g_shift);
   if (grid > cap) grid = cap;
   if (prec == 0 && (h->d.ld % 4) == 0 && tuning().vec4 && h->d.C1 <= 256) {

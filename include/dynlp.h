@@ -391,7 +391,7 @@ int dlp_lp_ground(void* handle, void* stream);
  * endpoint of a deleted edge, so a bounded BFS from every unlabeled deletion endpoint to a
  * vertex with a labeled neighbour decides it exactly; otherwise (or when a BFS exhausts its
  * budget, or insertions meet inert vertices) the full pass of dlp_lp_ground runs, decided on
- * the device.  Replaces the ball check of stream.py:237-273 and the full check :275-280. */
+ * the device; with more than n/16 endpoints (2*nd > n/16) the full pass runs directly.  Replaces the ball check of stream.py:237-273 and the full check :275-280. */
 int dlp_lp_ground_incremental(void* handle, int nd, const int32_t* du, const int32_t* dv, int ni,
                               void* stream);
 /* out[0] = full pass requested by the last grounding, out[1] = inert unlabeled vertices (after

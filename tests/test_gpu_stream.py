@@ -20,7 +20,10 @@ def oracle_system(g, es):
     return O.build_system(g.n, u, v, w, g.labeled, g.y, g.K), (u, v, w)
 
 
-def check_against_oracle(lp, g, es, eps, label):
+def check_against_oracle(lp, g, es, eps, label, iterative=False):
+    """Solved set, diagonal, certificate and CC labels against the oracle; F* by the oracle's
+    direct solve, or (iterative: graphs without community structure, whose factors fill in)
+    by its CG to rho <= 1e-13."""
     sysm, (u, v, w) = oracle_system(g, es)
     assert np.array_equal(lp.solved_mask(), sysm.solved), label
     np.testing.assert_allclose(lp.diag64.cpu().numpy(), sysm.diag, rtol=1e-12, err_msg=label)
@@ -28,7 +31,15 @@ def check_against_oracle(lp, g, es, eps, label):
     R = O.residual(sysm, Z)
     rc = O.rho_cols(sysm.diag, R)
     assert rc[: sysm.C].max() <= eps * 1.0001, (label, rc[: sysm.C].max())
-    Fs = O.solve_system(sysm)
+    if iterative:
+        Wl, dl, rl = sysm.local()
+        cols = np.concatenate([rl, dl[:, None]], axis=1)
+        Xs, _ = O.pcg(Wl, dl, np.zeros_like(cols), cols, [1e-13] * cols.shape[1])
+        Fs = np.zeros_like(Z)
+        Fs[:, sysm.C] = 1.0
+        Fs[sysm.U] = Xs
+    else:
+        Fs = O.solve_system(sysm)
     hmax = Fs[:, sysm.C].max()
     err = np.abs(Z[:, : sysm.C] - Fs[:, : sysm.C]).max()
     assert err <= rc[: sysm.C].max() * hmax * 1.05 + 1e-12, (label, err)
@@ -324,6 +335,41 @@ def test_fragmenting_stream_regrounds():
                            zip(b.del_u.tolist(), b.del_v.tolist()))
         check_against_oracle(lp, g, es, eps, f"batch {t}")
     assert toggles > 0
+
+
+def test_local_grounding_checks_match_oracle():
+    """Deletion-only batches small enough for the local grounding checks (2 * deletions <=
+    n/16): the bounded searches from the deletion endpoints decide the solved set, and when one
+    finds no labeled contact the full pass runs; the solved set and solution match the oracle
+    after every batch, and the local checks alone decide some batches.  Then a vertex is cut off
+    (all its edges deleted): it leaves the solved set."""
+    from dynlp_amd.graphs import EdgeBatch
+    from dynlp_amd.stream import EdgeStreamLP
+    g = graphs.erdos_renyi(20000, 6, 4, 0.01, seed=8)
+    eps = 1e-6
+    lp = EdgeStreamLP(g, eps=eps)
+    lp.initial_solve()
+    es = edge_dict(g)
+    batches, _ = graphs.update_stream(g, 5, 120, ins_frac=0.0, seed=8)
+    local_only = 0
+    for t, b in enumerate(batches):
+        c0 = lp.ground_flags()["calls"]
+        lp.apply_and_solve(b)
+        f = lp.ground_flags()
+        local_only += int(f["calls"] > c0 and not f["full_pass"])
+        O.apply_edge_batch(es, [], zip(b.del_u.tolist(), b.del_v.tolist()))
+        check_against_oracle(lp, g, es, eps, f"batch {t}", iterative=True)
+    assert local_only >= 1
+    solved = lp.solved_mask()
+    deg = np.bincount(np.array([k[0] for k in es] + [k[1] for k in es]), minlength=g.n)
+    x = int(np.flatnonzero(solved & ~g.labeled & (deg >= 2))[0])
+    cut = [k for k in es if x in k]
+    i32 = lambda z: np.asarray(z, dtype=np.int32)  # noqa: E731
+    b = EdgeBatch(i32([]), i32([]), np.zeros(0, np.float32), i32([a for a, _ in cut]), i32([c for _, c in cut]))
+    lp.apply_and_solve(b)
+    O.apply_edge_batch(es, [], cut)
+    check_against_oracle(lp, g, es, eps, "vertex cut off", iterative=True)
+    assert not lp.solved_mask()[x]
 
 
 def test_verify_pass_matches_full_refresh():
