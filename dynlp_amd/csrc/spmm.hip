@@ -231,6 +231,9 @@ struct PcgEpilogue {
   double* sm;  // workgroup column sums of P .* AP (LDS)
   float xr[W];
   float ar;
+  // this lane's partial sums of P .* AP over the rows it finishes (FP32: a few hundred terms
+  // of FP16-held directions), added to the workgroup's FP64 column sums once, at the end
+  float dacc[W] = {};
   __device__ __forceinline__ void pre(int, int r) {
     if (owner) {
       unpack<K>(load_word<K>(X + (int64_t)r * row_bytes + WordT<K>::bytes * L.l), xr);
@@ -240,9 +243,8 @@ struct PcgEpilogue {
     }
     ar = a[r];
   }
-  // out[r, W*l ..] = b*acc + ar*P (columns < C), sm += P .* out (FP64 LDS atomics: no
-  // per-lane FP64 accumulators, which would cost a wave of occupancy); rows outside the mask
-  // get AP = diag*P (no neighbour sum), as the plain operator writes them
+  // out[r, W*l ..] = b*acc + ar*P (columns < C), dacc += P .* out; rows outside the mask get
+  // AP = diag*P (no neighbour sum), as the plain operator writes them
   __device__ __forceinline__ void post(int, int r, bool, const float (&acc)[W]) {
     if (!owner) return;
     const int c0 = W * L.l;
@@ -255,21 +257,28 @@ struct PcgEpilogue {
       for (int k = 0; k < W / 4; ++k)
         reinterpret_cast<float4*>(op)[k] = make_float4(y[4 * k], y[4 * k + 1], y[4 * k + 2], y[4 * k + 3]);
 #pragma unroll
-      for (int q = 0; q < W; ++q) atomicAdd(&sm[c0 + q], (double)xr[q] * (double)y[q]);
+      for (int q = 0; q < W; ++q) dacc[q] = __builtin_fmaf(xr[q], y[q], dacc[q]);
     } else {
 #pragma unroll
       for (int q = 0; q < W; ++q) {
         if (c0 + q < C) {
           op[q] = y[q];
-          atomicAdd(&sm[c0 + q], (double)xr[q] * (double)y[q]);
+          dacc[q] = __builtin_fmaf(xr[q], y[q], dacc[q]);
         }
       }
     }
   }
+  // the lane's sums into the workgroup's FP64 column sums (LDS)
+  __device__ __forceinline__ void flush() {
+    if (!owner) return;
+#pragma unroll
+    for (int q = 0; q < W; ++q)
+      if (W * L.l + q < C) atomicAdd(&sm[W * L.l + q], (double)dacc[q]);
+  }
 };
 
 template <int S, Word K>
-__global__ __launch_bounds__(256) void k_pcg_operator(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_pcg_operator(
     int nrows, int C, const int32_t* __restrict__ indptr, const int32_t* __restrict__ rowlen,
     const int32_t* __restrict__ indices, const float* __restrict__ w, const uint8_t* __restrict__ rowmask,
     const void* __restrict__ X, unsigned row_bytes, int xbytes, const float* __restrict__ a, float b,
@@ -282,6 +291,7 @@ __global__ __launch_bounds__(256) void k_pcg_operator(
   const RowPipe G(nrows, C, W, WordT<K>::bytes, xcd, rowmask, indptr, rowlen, indices, w, X, row_bytes, xbytes);
   PcgEpilogue<K> E{G.L, C, G.L.e == 0 && W * G.L.l < C, (const char*)X, row_bytes, a, b, out, ldo, sm};
   run_rows<S, K, float>(G, E);
+  E.flush();
   __syncthreads();
   for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&dot[c], sm[c]);
 }
