@@ -297,9 +297,17 @@ def gpu_case_run(g, b):
 
 
 def pmc_probe(g):
-    """Launch sequence for hardware-counter runs (rocprofv3 --pmc ...): the counters of the 5
-    A*P launches are read against those of one plain copy of known size."""
+    """Launch sequence for hardware-counter runs (rocprofv3 --pmc ...): the counters of each
+    kernel's launches are read against those of one plain copy of known size.  After the initial
+    solve: 5 A*P launches (k_pcg_operator), the copy (512 MiB), 3 FP64 verification passes
+    (k_verify, refresh mode 3), 3 connected-components passes of the store, 3 reference-API
+    SpMMs (n x C1 FP32) and 3 push scatters from every 10th row; each kernel's algorithmic
+    bytes per launch are printed for the summary."""
+    import numpy as np
     import torch
+    from dynlp_amd import _lib
+    from dynlp_amd import backend as B
+    from dynlp_amd.kernels import FrontierOps, spmm_axpy
     from dynlp_amd.stream import EdgeStreamLP
     lp = EdgeStreamLP(g, eps=1e-6)
     lp.initial_solve()
@@ -313,7 +321,88 @@ def pmc_probe(g):
     torch.cuda.synchronize()
     log(f"pmc probe: 5 A*P launches ({op_ms:.3f} ms each, {nbytes} algorithmic bytes each); "
         f"copy of {src.numel() * 4} bytes")
+    del src, dst
+    solved = lp.solved_mask()
+    ip_s, rl_s = lp.indptr.cpu().numpy(), lp.rowlen.cpu().numpy()
+    vb = verify_bytes(g.n, int(solved.sum()), lp.C1, int(rl_s[solved].sum()))
+    for _ in range(3):
+        _lib.call("dlp_lp_refresh", lp.h, 3, 0, B.stream())
+    torch.cuda.synchronize()
+    log(f"pmc probe: 3 verification passes (k_verify), {vb} algorithmic bytes each")
+    n, nnz = g.n, lp.stored_entries()
+    for _ in range(3):
+        lp.components()
+    torch.cuda.synchronize()
+    log(f"pmc probe: 3 connected-components passes, {4 * (n + 1) + 4 * nnz + 8 * n + 4 * 4 * n} algorithmic bytes each")
+    ip, ix, d = g.csr()
+    W = B.csr_from_host(ip, ix, d, n, torch.float32)
+    C = lp.C1
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    X = torch.rand((n, C), dtype=torch.float32, device="cuda", generator=gen)
+    a = torch.rand(n, dtype=torch.float32, device="cuda", generator=gen) + 20
+    for _ in range(3):
+        spmm_axpy(W, X, a=a, b=-1.0)
+    torch.cuda.synchronize()
+    e = int(ix.shape[0])
+    log(f"pmc probe: 3 SpMMs (n x {C}), {4 * (n + 1) + 8 * e + 2 * 4 * n * C + 4 * n} algorithmic bytes each")
+    f_h = np.arange(0, n, 10, dtype=np.int32)
+    f = torch.as_tensor(f_h, device="cuda")
+    nf = f_h.size
+    e_f = int(np.diff(ip)[f_h].sum())
+    nbrs = np.unique(ix[np.concatenate([np.arange(ip[r], ip[r + 1]) for r in f_h])])
+    ops = FrontierOps(W, C)
+    D = torch.rand((nf, C), dtype=torch.float32, device="cuda", generator=gen)
+    R = torch.zeros((n, C), dtype=torch.float32, device="cuda")
+    mark = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    for _ in range(3):
+        ops.push(f, D, R, mark)
+    torch.cuda.synchronize()
+    log(f"pmc probe: 3 push scatters ({nf} rows), {nf * (12 + 4 * C) + 8 * e_f + nbrs.size * (2 * 4 * C + 1)} "
+        f"algorithmic bytes each")
     return 0
+
+
+def trace_summary(path, start=None):
+    """Markdown summary of a rocprofv3 kernel trace (`--kernel-trace --output-format csv`):
+    from the first launch of a kernel whose name contains `start` (whole trace when None), the
+    span, the summed kernel time, the time with 0, 1, 2, 3 and 4 or more kernels running at once
+    (launches on different streams overlapping on the device), streams and hardware queues
+    seen, and the kernels by total time."""
+    import collections
+    import csv
+    ev = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            ev.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"], r["Stream_Id"],
+                       r["Queue_Id"]))
+    ev.sort()
+    if start:
+        first = min(e[0] for e in ev if start in e[2])
+        ev = [e for e in ev if e[0] >= first]
+    t0, t1 = ev[0][0], max(e[1] for e in ev)
+    pts = sorted([(e[0], 1) for e in ev] + [(e[1], -1) for e in ev])
+    act, last, by = 0, pts[0][0], collections.Counter()
+    for t, d in pts:
+        by[min(act, 4)] += t - last
+        act += d
+        last = t
+    tot = sum(e[1] - e[0] for e in ev)
+    span = t1 - t0
+    out = [f"Launches: {len(ev)}; span {span / 1e9:.3f} s; summed kernel time {tot / 1e9:.3f} s "
+           f"({tot / span:.2f} x the span); streams {len({e[3] for e in ev})}, "
+           f"hardware queues {len({e[4] for e in ev})}.", "",
+           "| Kernels running at once | Time (s) | Share of the span |", "|---|---|---|"]
+    for k in range(5):
+        out.append(f"| {k if k < 4 else '4 or more'} | {by[k] / 1e9:.3f} | {100 * by[k] / span:.1f} % |")
+    agg = collections.defaultdict(lambda: [0, 0])
+    for s_, e_, name, *_ in ev:
+        short = name.split("(")[0].replace("void ", "")
+        agg[short][0] += 1
+        agg[short][1] += e_ - s_
+    out += ["", "| Kernel | Calls | Total ms | Avg us | % of kernel time |", "|---|---|---|---|---|"]
+    for name, (c, t) in sorted(agg.items(), key=lambda x: -x[1][1])[:20]:
+        out.append(f"| `{name[:90]}` | {c} | {t / 1e6:.1f} | {t / c / 1e3:.1f} | {100 * t / tot:.1f} |")
+    return "\n".join(out)
 
 
 def serve_run(a):
@@ -364,7 +453,7 @@ def serve_run(a):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20, help="timed requests")
+    ap.add_argument("--steps", type=int, default=100, help="timed requests")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch-edges", type=int, default=10_000)
     ap.add_argument("--scale", type=float, default=1.0, help="multiplies n (1.0 = SBM-1M)")
@@ -393,7 +482,13 @@ def main(argv=None):
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="arrival window")
     ap.add_argument("--serve-budget-gb", type=float, default=32.0, help="HBM budget of the resident graphs")
     ap.add_argument("--serve-slots", type=int, default=4, help="worker streams for global solves")
+    ap.add_argument("--trace-summary", metavar="CSV", default=None,
+                    help="print a markdown summary of a rocprofv3 kernel-trace CSV and exit (no GPU)")
+    ap.add_argument("--trace-from", default=None, help="with --trace-summary: start at this kernel's first launch")
     a = ap.parse_args(argv)
+    if a.trace_summary:
+        print(trace_summary(a.trace_summary, a.trace_from))
+        return 0
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: the package has no multi-GPU path (asked for {a.gpus}); "
               "the per-GPU workload is the whole graph", file=sys.stderr)

@@ -355,22 +355,32 @@ class EdgeSet:
         return (self.key.shape[0] > 0) & (self.key[pos] == key)
 
     def apply(self, b: EdgeBatch):
+        """Deletions, then insertions (upserts; of a pair inserted twice the last weight), as
+        merges into the sorted keys: O(m) per batch plus a sort of the batch."""
         n = self.n
-        dk = np.minimum(b.del_u, b.del_v) * n + np.maximum(b.del_u, b.del_v)
-        if dk.shape[0]:
-            keep = ~np.isin(self.key, dk)
-            self.key, self.w = self.key[keep], self.w[keep]
-        ik = np.minimum(b.ins_u, b.ins_v) * n + np.maximum(b.ins_u, b.ins_v)
+        dk = np.minimum(b.del_u, b.del_v).astype(np.int64) * n + np.maximum(b.del_u, b.del_v)
+        if dk.shape[0] and self.key.shape[0]:
+            pos = np.searchsorted(self.key, dk)
+            pos = pos[pos < self.key.shape[0]]
+            pos = pos[np.isin(self.key[pos], dk)]
+            if pos.shape[0]:
+                keep = np.ones(self.key.shape[0], dtype=bool)
+                keep[pos] = False
+                self.key, self.w = self.key[keep], self.w[keep]
+        ik = np.minimum(b.ins_u, b.ins_v).astype(np.int64) * n + np.maximum(b.ins_u, b.ins_v)
         if ik.shape[0]:
+            iw = np.asarray(b.ins_w, dtype=self.w.dtype)
+            # the last occurrence of each inserted pair
+            last = ik.shape[0] - 1 - np.unique(ik[::-1], return_index=True)[1]
+            ik, iw = ik[last], iw[last]          # sorted by key
             have = self.contains(ik)
             if have.any():   # upsert: an existing pair takes the new weight
-                pos = np.searchsorted(self.key, ik[have])
-                self.w[pos] = b.ins_w[have]
-            nk, nw = ik[~have], b.ins_w[~have]
-            key = np.concatenate([self.key, nk])
-            w = np.concatenate([self.w, nw])
-            order = np.argsort(key, kind="stable")
-            self.key, self.w = key[order], w[order]
+                self.w[np.searchsorted(self.key, ik[have])] = iw[have]
+            nk, nw = ik[~have], iw[~have]
+            if nk.shape[0]:
+                at = np.searchsorted(self.key, nk)
+                self.key = np.insert(self.key, at, nk)
+                self.w = np.insert(self.w, at, nw)
 
     def pairs(self):
         return self.key // self.n, self.key % self.n, self.w
