@@ -296,6 +296,45 @@ def gpu_case_run(g, b):
             "batch_path": sb.path, "edge_updates_per_s": round(1000 / (gpu_batch_ms / 1e3), 1), "rho": sb.rho}
 
 
+def full_solves(g):
+    """The full solve of this graph from the prior, by the configuration's stated method (the
+    reference API's certified DynLP+ with the aggregation-AMG preconditioner, solvers.py:244-343,
+    error bound tol = 1e-6) and its Jacobi form, and by the resident path's initial solve (two-level
+    column PCG; block PCG with the FP64 Gram on the matrix cores), each to rho <= 1e-6 at least:
+    wall time, iterations and the certified bound on |F - F*|."""
+    import torch
+    from dynlp_amd.solvers import DynLPPlus, build_system
+    from dynlp_amd.stream import EdgeStreamLP
+    sysd = build_system(g, "float32")
+    rows = []
+    for method in ("amg", "pcg"):
+        solver = DynLPPlus(g.n, g.K, "float32", method=method, tol=1e-6)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st = solver.solve(sysd)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) * 1e3
+        rows.append({"solver": f"DynLP+ ({'aggregation-AMG' if method == 'amg' else 'Jacobi'} PCG), reference API",
+                     "ms": round(ms, 1), "setup_ms": round(st.setup_ms, 1), "iters": st.iters,
+                     "cert_bound": st.cert, "hmax": st.hmax, "rho": st.cert / st.hmax if st.hmax else None})
+        log(f"full solve, {rows[-1]['solver']}: {ms:.0f} ms, {st.iters} iterations, bound {st.cert:.2e}")
+    del sysd
+    for mode, name in (("column", "resident state: column PCG, two-level preconditioner, FP16 directions"),
+                       ("block", "resident state: block PCG, FP64 Gram on the matrix cores, Jacobi")):
+        lp = EdgeStreamLP(g, eps=1e-6, pcg_mode=mode)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st = lp.initial_solve()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) * 1e3
+        rows.append({"solver": name, "ms": round(ms, 1), "iters": lp.pcg_stats()["pcg_iters"],
+                     "cert_bound": st.rho * st.hmax, "hmax": st.hmax, "rho": st.rho})
+        log(f"full solve, {name}: {ms:.0f} ms, rho {st.rho:.2e}")
+        del lp
+        torch.cuda.empty_cache()
+    return rows
+
+
 def pmc_probe(g):
     """Launch sequence for hardware-counter runs (rocprofv3 --pmc ...): the counters of each
     kernel's launches are read against those of one plain copy of known size.  After the initial
@@ -466,6 +505,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu-case", action="store_true",
                     help="skip the ER-100k (k = 10) CPU case: Jacobi to rho <= 1e-6 and one push batch, CPU and GPU")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel roofline table")
+    ap.add_argument("--no-full-solve", action="store_true",
+                    help="skip the full solves from the prior (DynLP+ with AMG and Jacobi PCG, resident column and "
+                         "block PCG)")
     ap.add_argument("--out", default=None,
                     help="also append one row per timed request to this CSV in the reference's schema "
                          "(bench.py FIELDS), with a .json record of the run next to it")
@@ -610,6 +652,12 @@ def main(argv=None):
                        "p99_ms": round(percentile(lat, 99), 2)})
     fp64 = ladder[0] if ladder else None
 
+    # ---- the configuration's full solve by its stated method ------------------------------------
+    full = None
+    if not a.no_full_solve:
+        full = full_solves(g)
+        torch.cuda.empty_cache()
+
     # ---- the other hot kernels at this graph's shape ------------------------------------------
     ktab = None
     if not a.no_kernels:
@@ -673,7 +721,7 @@ def main(argv=None):
         "speedup_vs_fp64_path": round(value / fp64["value"], 2) if fp64 else None,
         "path_ladder": ladder or None,
         "roofline": roofline, "verify_pass": verify, "cpu_baseline": cpu, "cpu_agreement": agreement,
-        "cpu_case": case, "kernels": ktab,
+        "cpu_case": case, "full_solve": full, "kernels": ktab,
         "wall_s": round(time.perf_counter() - t_all, 1),
     }
     if a.out:

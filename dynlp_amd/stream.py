@@ -461,6 +461,8 @@ class EdgeStreamLP:
         _lib.call("dlp_lp_begin_batch", self.h, stream())
         _lib.call("dlp_lp_apply", self.h, int(du.shape[0]), ptr(du), ptr(dv), int(iu.shape[0]), ptr(iu), ptr(iv),
                   ptr(iw), stream())
+        if iu.shape[0]:
+            self._regrow_if_full((iu, iv, iw), (b.ins_u, b.ins_v))
         if self.incremental_ground:
             _lib.call("dlp_lp_ground_incremental", self.h, int(du.shape[0]), ptr(du), ptr(dv), int(iu.shape[0]),
                       stream())
@@ -492,11 +494,10 @@ class EdgeStreamLP:
             ops = (du, dv, dw.view(torch.float32))
             _lib.call("dlp_lp_apply_ops", self.h, hi - lo, ptr(ops[0]), ptr(ops[1]), ptr(ops[2]), stream())
             ins_rows = (u[lo:hi][w[lo:hi] > 0], v[lo:hi][w[lo:hi] > 0])
+            # also: the staging buffer is free again once the status has been read
+            self._regrow_if_full(ops, ins_rows)
             if hi >= nops:
                 break
-            s = self.status()      # also: the staging buffer is free again
-            if s["overflow"]:
-                self._regrow(ops, self._regrow_extra(ins_rows))
             lo = hi
         if self.incremental_ground and nops <= _OPS_PER_CALL:
             # every endpoint is checked (a superset of the deletions' endpoints)
@@ -510,6 +511,12 @@ class EdgeStreamLP:
     def apply_group_and_solve(self, batches) -> Stats:
         """Several batches applied in order and repaired once (begin_group + end_request)."""
         return self.end_request(self.begin_group(batches))
+
+    def _regrow_if_full(self, ins, ins_rows):
+        """After an apply call: when an insertion found its row full, regrow the store now (the
+        overflowed entries re-inserted), before the grounding and the repair see the graph."""
+        if self.status()["overflow"]:
+            self._regrow(ins, self._regrow_extra(ins_rows))
 
     @staticmethod
     def _regrow_extra(ins_rows):
@@ -527,7 +534,7 @@ class EdgeStreamLP:
             raise RuntimeError(f"device apply refused {s['invalid']} pairs and merged {s['duplicates']} "
                                "repeated entries that the host checks had passed")
         if s["overflow"]:
-            # room for every insertion of this call in any one row, so one regrow suffices
+            # (the apply's overflow is regrown before the repair; this only guards the counter)
             self._regrow(req["ins"], self._regrow_extra(req["ins_rows"]))
             _lib.call("dlp_lp_ground", self.h, stream())   # re-inserted edges can re-ground rows
             _lib.call("dlp_lp_refresh", self.h, 0, prec, stream())

@@ -221,19 +221,24 @@ def test_push_across_graphs_matches_oracle():
         assert np.abs(Z[:, : sysm.C] - Fs[:, : sysm.C]).max() <= EPS * Fs[:, sysm.C].max() * 1.05, i
 
 
-def test_service_pushes_graphs_together():
+@pytest.mark.parametrize("budget", [1e9, 1e-4])
+def test_service_pushes_graphs_together(budget):
     """Eight small graphs whose requests arrive at once: the scheduler repairs them with pushes
     that cover several graphs per call (the repair policy set to push whatever the residual
-    mass), and every graph ends certified at the exact solution of its updates."""
+    mass), and every graph ends certified at the exact solution of its updates.  With an edge
+    budget of 1e-4 nnz the pushes stop early and every graph goes on to a global solve in a
+    worker."""
     from dynlp_amd.serve import Service, footprint_bytes, poisson_requests
+    kw = dict(_ALWAYS_PUSH, push_budget=budget)
     gs = [graphs.sbm(1500 + 300 * i, 4, 8, 0.8, 4, 0.03, seed=40 + i) for i in range(8)]
     reqs = poisson_requests(gs, rate=1e9, duration=1.0, size_lo=10, size_hi=40, seed=9, max_requests=32)
-    svc = Service(gs, sum(footprint_bytes(g, **_ALWAYS_PUSH) for g in gs) + (1 << 20), slots=2, eps=EPS,
-                  lp_kwargs=_ALWAYS_PUSH)
+    svc = Service(gs, sum(footprint_bytes(g, **kw) for g in gs) + (1 << 20), slots=2, eps=EPS, lp_kwargs=kw)
     svc.run(reqs)
     rep = svc.report()
     assert rep["errors"] == 0 and rep["requests"] == len(reqs)
     assert rep["push_launches"] >= 1 and rep["graphs_per_push_launch"] > 1, rep
+    if budget < 1:
+        assert rep["global_repairs"] >= 1, rep
     for gid, g in enumerate(gs):
         mine = [r for r in reqs if r.gid == gid]
         sysm, Fs = _oracle_hmax(g, mine)
