@@ -523,7 +523,7 @@ def main(argv=None):
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="requests per second")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="arrival window")
     ap.add_argument("--serve-budget-gb", type=float, default=32.0, help="HBM budget of the resident graphs")
-    ap.add_argument("--serve-slots", type=int, default=4, help="worker streams for global solves")
+    ap.add_argument("--serve-slots", type=int, default=8, help="worker streams for global solves")
     ap.add_argument("--trace-summary", metavar="CSV", default=None,
                     help="print a markdown summary of a rocprofv3 kernel-trace CSV and exit (no GPU)")
     ap.add_argument("--trace-from", default=None, help="with --trace-summary: start at this kernel's first launch")
@@ -617,7 +617,8 @@ def main(argv=None):
     n_solved = int(solved.sum().item())
     stored_solved = int(lp.rowlen[solved].sum().item())
     vbytes = verify_bytes(g.n, n_solved, lp.C1, stored_solved)
-    vname = {2: "k_verify<4> (FP64 residual of every solved row, column maxima, h maximum)",
+    vname = {3: "k_refresh_pipe<4> (listed rows: the pipelined verification did not apply)",
+             2: "k_verify<4> (FP64 residual of every solved row, column maxima, h maximum)",
              1: "k_refresh_v4 (FP64 refresh of every row: the pipelined verification did not apply)",
              0: "k_refresh (FP64 refresh of every row: the pipelined verification did not apply)"}[ver_kernel]
     verify = {"kernel": vname, "bound": "hbm", "ms": round(ver_ms, 4), "algorithmic_bytes": int(vbytes),

@@ -1848,6 +1848,120 @@ __global__ void k_refresh_reset(int32_t* __restrict__ nviol, double* __restrict_
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// Refresh of listed rows on the pipelined gathers (modes 0 and 1).  The parking trick of the
+// verification pass applies once every row outside the solved set holds its parked state: a
+// first pass parks the listed rows that left it (or are labeled), then the pipelined pass
+// computes R = rhs - diag*Z + W Z over all neighbours of the listed solved rows, with the
+// row's diagonal recomputed (mode 0: the sum of its weights to labeled and solved neighbours,
+// plus eta) or kept (mode 1: the push changed no equation).
+
+__global__ void k_park_listed(RefreshArgs A, float* __restrict__ Z, float* __restrict__ R) {
+  const int nr = *A.rows_count;
+  const int lane = threadIdx.x & 63;
+  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < nr;
+       i += ((int64_t)gridDim.x * blockDim.x) >> 6) {
+    const int r = A.rows[i];
+    if (A.solved[r]) continue;   // wave-uniform
+    const int pc = parked_class(A, r);
+    const int64_t zo = (int64_t)r * A.ld;
+    for (int c = lane; c < A.C1; c += 64) {
+      Z[zo + c] = c == pc ? 1.f : 0.f;
+      R[zo + c] = 0.f;
+    }
+    if (lane == 0) {
+      A.diag64[r] = 1.0;
+      A.diag32[r] = 1.f;
+    }
+  }
+}
+
+struct ListedRefreshEpilogue {
+  const LanesP& L;
+  const RefreshArgs& A;
+  const float* __restrict__ Z;
+  float* __restrict__ R;
+  bool eqs;                // recompute the diagonal (mode 0)
+  double rmax[4] = {0.0, 0.0, 0.0, 0.0};
+  double mass = 0.0;       // lane 0: this wave's rows' push mass
+  float4 zr;
+  double dg = 1.0;
+  __device__ __forceinline__ bool owner() const { return L.e == 0 && 4 * L.l < A.C1; }
+  __device__ __forceinline__ void pre(int, int r) {
+    zr = owner() ? *(reinterpret_cast<const float4*>(Z + (int64_t)r * A.ld) + L.l) : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!A.solved[r]) return;   // wave-uniform; parked by k_park_listed
+    if (!eqs) {
+      dg = A.diag64[r];
+      return;
+    }
+    const int beg = __builtin_amdgcn_readfirstlane(A.indptr[r]);
+    const int end = beg + __builtin_amdgcn_readfirstlane(A.rowlen[r]);
+    double sw = 0.0;
+    for (int j = beg + L.lane; j < end; j += 64) {
+      const int v = A.col[j];
+      if (v >= 0 && (A.labeled[v] || A.solved[v])) sw += (double)A.w[j];
+    }
+    for (int off = 32; off > 0; off >>= 1) sw += __shfl_xor(sw, off, 64);
+    dg = sw + A.eta;
+  }
+  __device__ __forceinline__ void post(int, int r, bool live, const double (&acc)[4]) {
+    if (!live) return;  // wave-uniform
+    bool viol = false;
+    double m = 0.0;
+    if (owner()) {
+      if (eqs && L.l == 0) {
+        A.diag64[r] = dg;
+        A.diag32[r] = (float)dg;
+      }
+      const int C = A.C1 - 1;
+      const float zv[4] = {zr.x, zr.y, zr.z, zr.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c = 4 * L.l + q;
+        if (c < A.C1) {
+          const double rh = c == C ? dg : A.eta * A.prior;
+          const double res = rh - dg * (double)zv[q] + acc[q];
+          R[(int64_t)r * A.ld + c] = (float)res;
+          const double ratio = ratio_or_inf(res, dg);
+          viol |= ratio > A.eps[c];
+          rmax[q] = fmax(rmax[q], ratio);
+          m = fmax(m, ratio / A.eps[c]);
+        }
+      }
+    }
+    const unsigned long long bal = __ballot(viol);
+    if (L.lane == 0 && bal && A.viol) A.viol[atomicAdd(A.nviol, 1)] = r;
+    if (A.mass) {
+      for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+      if (L.lane == 0 && m > 1.0) mass += m;
+    }
+  }
+};
+
+template <int S>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_refresh_pipe(
+    RefreshArgs A, int eqs, int xcd, int xbytes, float* __restrict__ Z, float* __restrict__ R) {
+  const int nr = *A.rows_count;
+  RowPipe G(nr, A.C1, 4, 16, xcd, A.solved, A.indptr, A.rowlen, A.col, A.w, Z, (unsigned)(A.ld * 4), xbytes);
+  G.rows = A.rows;
+  ListedRefreshEpilogue E{G.L, A, Z, R, eqs != 0};
+  run_rows<S, Word::F32x4, double>(G, E);
+  flush_mass(A.mass, E.mass);
+  if (!A.rho) return;
+  __shared__ double sm[256];
+  for (int c = threadIdx.x; c < A.C1; c += blockDim.x) sm[c] = 0.0;
+  __syncthreads();
+  if (E.owner()) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = 4 * G.L.l + q;
+      if (c < A.C1) atomic_max_nonneg(&sm[c], E.rmax[q]);
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < A.C1; c += blockDim.x) atomic_max_nonneg(A.rho + c, sm[c]);
+}
+
 extern "C" int dlp_lp_refresh(void* handle, int which, int prec, void* stream) {
   DLP_CHECK_ARG(handle && which >= 0 && which <= 3 && (prec == 0 || prec == 1), "lp_refresh: bad argument");
   LP* h = (LP*)handle;
@@ -1876,6 +1990,26 @@ extern "C" int dlp_lp_refresh(void* handle, int which, int prec, void* stream) {
       return DLP_OK;
     }
     which = 2;
+  }
+  if ((which == 0 || which == 1) && prec == 0 && (h->d.ld % 4) == 0 && h->d.C1 <= 256 && tuning().vec4 &&
+      tuning().pipe && h->d.n < (1 << 24) && xbytes < (1LL << 31)) {
+    // listed rows on the pipelined gathers: park the listed rows outside the solved set, then
+    // the residual (and, for the changed rows, the diagonal) of the listed solved rows
+    RefreshArgs A = refresh_args(h, which);
+    if (which == 0) {
+      hipLaunchKernelGGL(k_park_listed, dim3(1024), dim3(kBlock), 0, st, A, h->d.Z32, h->d.R32);
+      DLP_CHECK_LAUNCH();
+    }
+    const int xcd = tuning().xcd != 0;
+    long long g = resident_grid(reinterpret_cast<const void*>(&k_refresh_pipe<4>));
+    const long long need = (which == 0 ? 2 * h->batch_ops : (long long)h->d.n) / 4 + 8;
+    if (g > need) g = need;
+    if (xcd) g = g < 8 ? 8 : g & ~7LL;
+    hipLaunchKernelGGL((k_refresh_pipe<4>), dim3((unsigned)g), dim3(kBlock), 0, st, A, which == 0 ? 1 : 0, xcd,
+                       (int)xbytes, h->d.Z32, h->d.R32);
+    DLP_CHECK_LAUNCH();
+    h->last_refresh_kernel = 3;
+    return DLP_OK;
   }
   RefreshArgs A = refresh_args(h, which);
   const RowMap& m = h->map;

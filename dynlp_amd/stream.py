@@ -461,16 +461,29 @@ class EdgeStreamLP:
         _lib.call("dlp_lp_begin_batch", self.h, stream())
         _lib.call("dlp_lp_apply", self.h, int(du.shape[0]), ptr(du), ptr(dv), int(iu.shape[0]), ptr(iu), ptr(iv),
                   ptr(iw), stream())
-        if iu.shape[0]:
-            self._regrow_if_full((iu, iv, iw), (b.ins_u, b.ins_v))
         if self.incremental_ground:
             _lib.call("dlp_lp_ground_incremental", self.h, int(du.shape[0]), ptr(du), ptr(dv), int(iu.shape[0]),
                       stream())
         else:
             _lib.call("dlp_lp_ground", self.h, stream())
         _lib.call("dlp_lp_refresh", self.h, 0, 1 if self.fp64 else 0, stream())
-        s = None if (self.fp64 or self._batch_edges > self.push_max_rows) else self.status()
-        return {"st": st, "t0": t0, "ins": (iu, iv, iw), "ins_rows": (b.ins_u, b.ins_v), "status": s}
+        req = {"st": st, "t0": t0, "ins": (iu, iv, iw), "ins_rows": (b.ins_u, b.ins_v), "status": None}
+        if not (self.fp64 or self._batch_edges > self.push_max_rows):
+            req["status"] = self._settle(req)
+        return req
+
+    def _settle(self, req):
+        """The status after the first half, with an overflowed store regrown first: the
+        overflowed insertions re-inserted, the grounding and the changed-row refresh redone, so
+        that the repair sees the whole batch (one regrow makes room for every insertion of the
+        call in any one row)."""
+        s = self.status()
+        if s["overflow"]:
+            self._regrow(req["ins"], self._regrow_extra(req["ins_rows"]))
+            _lib.call("dlp_lp_ground", self.h, stream())
+            _lib.call("dlp_lp_refresh", self.h, 0, 1 if self.fp64 else 0, stream())
+            s = self.status()
+        return s
 
     def begin_group(self, batches):
         """begin_request for several batches applied in order, as one repair: their operations
@@ -494,10 +507,13 @@ class EdgeStreamLP:
             ops = (du, dv, dw.view(torch.float32))
             _lib.call("dlp_lp_apply_ops", self.h, hi - lo, ptr(ops[0]), ptr(ops[1]), ptr(ops[2]), stream())
             ins_rows = (u[lo:hi][w[lo:hi] > 0], v[lo:hi][w[lo:hi] > 0])
-            # also: the staging buffer is free again once the status has been read
-            self._regrow_if_full(ops, ins_rows)
             if hi >= nops:
                 break
+            # the next call's operations may touch this call's overflowed pairs, and the overflow
+            # list names entries of this call's arrays: regrow before going on (the status read
+            # also frees the staging buffer)
+            if self.status()["overflow"]:
+                self._regrow(ops, self._regrow_extra(ins_rows))
             lo = hi
         if self.incremental_ground and nops <= _OPS_PER_CALL:
             # every endpoint is checked (a superset of the deletions' endpoints)
@@ -505,18 +521,14 @@ class EdgeStreamLP:
         else:
             _lib.call("dlp_lp_ground", self.h, stream())
         _lib.call("dlp_lp_refresh", self.h, 0, 1 if self.fp64 else 0, stream())
-        s = None if (self.fp64 or nops > self.push_max_rows) else self.status()
-        return {"st": st, "t0": t0, "ins": ops, "ins_rows": ins_rows, "status": s}
+        req = {"st": st, "t0": t0, "ins": ops, "ins_rows": ins_rows, "status": None}
+        if not (self.fp64 or nops > self.push_max_rows):
+            req["status"] = self._settle(req)
+        return req
 
     def apply_group_and_solve(self, batches) -> Stats:
         """Several batches applied in order and repaired once (begin_group + end_request)."""
         return self.end_request(self.begin_group(batches))
-
-    def _regrow_if_full(self, ins, ins_rows):
-        """After an apply call: when an insertion found its row full, regrow the store now (the
-        overflowed entries re-inserted), before the grounding and the repair see the graph."""
-        if self.status()["overflow"]:
-            self._regrow(ins, self._regrow_extra(ins_rows))
 
     @staticmethod
     def _regrow_extra(ins_rows):
@@ -529,6 +541,10 @@ class EdgeStreamLP:
         this batch), a regrow after overflowed insertions, and the request's statistics."""
         st = req["st"]
         prec = 1 if self.fp64 else 0
+        if req["status"] is None:
+            # a large batch: settled here, in the thread that repairs it (no status was read
+            # after the first half); the repair goes straight to the global solve
+            self._settle(req)
         s = self._repair(st, pushed=pushed, s0=req["status"])
         if s["invalid"] or s["duplicates"]:
             raise RuntimeError(f"device apply refused {s['invalid']} pairs and merged {s['duplicates']} "

@@ -523,7 +523,8 @@ int dlp_lp_violators(void* handle, int32_t** list, int32_t** count);
  * would take), [16] pairs of this batch refused by dlp_lp_apply (ids outside [0, n) or
  * insertion weights not finite and positive), [17] earlier occurrences of pairs repeated in its
  * insertions (not applied), [18] the kernel of the last dlp_lp_refresh (0 the generic refresh,
- * 1 the 16-byte-row refresh, 2 the pipelined verification of mode 3; -1 none yet);
+ * 1 the 16-byte-row refresh, 2 the pipelined verification of mode 3, 3 the pipelined refresh of
+ * listed rows of modes 0 and 1; -1 none yet);
  * rho (C1, may be NULL): column maxima of the last refresh, in which a NaN residual counts as
  * +inf (so it never certifies).  `out` holds 19 entries. */
 int dlp_lp_status(void* handle, int64_t* out, double* rho, void* stream);
