@@ -315,11 +315,12 @@ class Service:
                     stream.synchronize()
                 except Exception as e:  # noqa: BLE001  (reported on the requests, service goes on)
                     err = f"repair failed: {type(e).__name__}: {e}"
-                self.times["build" if kind == "build" else "global"] += time.perf_counter() - t_job
-                if kind != "build":
-                    self.stats["global_repairs"] += 1
+                dt = time.perf_counter() - t_job
                 self._complete(s, good, bad, st, err)
-                with self.cv:
+                with self.cv:   # workers share these counters
+                    self.times["build" if kind == "build" else "global"] += dt
+                    if kind != "build":
+                        self.stats["global_repairs"] += 1
                     self._running -= 1
                     self.cv.notify_all()
 
