@@ -1053,7 +1053,10 @@ int coarse_build(int n, int na, const int32_t* bounds, const uint8_t* aggmap, co
                  const int32_t* rowlen, const int32_t* col, const float* w, const uint8_t* solved,
                  const double* diag, double* Ac, double* Ainv, hipStream_t st) {
   DLP_CHECK_ARG(n >= 1 && na >= 1 && na <= kCoarseMax && bounds && aggmap, "coarse_build: na=%d n=%d", na, n);
-  const int parts = 8;
+  // each workgroup (4 wavefronts) takes about 256 rows of one aggregate: a wavefront's rows
+  // are walked one after another, each a chain of dependent loads
+  int parts = (int)(((long long)n / na + 255) / 256);
+  parts = parts < 8 ? 8 : (parts > 256 ? 256 : parts);
   hipLaunchKernelGGL(k_zero_f64, dim3(grid_for((long long)na * na)), dim3(kBlock), 0, st, na * na, Ac);
   hipLaunchKernelGGL(k_coarse_build, dim3(na * parts), dim3(kBlock), 0, st, n, na, parts, bounds, aggmap, indptr,
                      rowlen, col, w, solved, diag, Ac);

@@ -1278,6 +1278,7 @@ struct LP {
   // only), rebuilt every coarse_refresh global solves; cg_* = the current layout and buffers
   int coarse_na = 0, coarse_refresh = 64;
   int cg_na = 0, cg_min = 0, cg_age = -1;
+  int cg_builds = 0;            // coarse operators built; the boundaries are searched every 4th
   double* cg_Ac = nullptr;      // buffers sized for kCoarseMax aggregates
   double* cg_Ainv = nullptr;
   double* cg_rc = nullptr;
@@ -2555,8 +2556,18 @@ static int coarse_prepare(LP* h, hipStream_t st) {
     h->cg_age += 1;
     return DLP_OK;
   }
-  // boundaries: targets k*S0, each moved to the lowest-cut row boundary near it
   const int min_agg = chunk > 32 ? chunk : 32;
+  if (h->cg_age >= 0 && h->cg_na > 0 && min_agg == h->cg_min && (h->cg_builds % 4) != 0) {
+    // a batch moves few edges across the row ranges: keep them, rebuild A_c from the current
+    // equations (the boundary search, a pass with an atomic per edge, runs every 4th build)
+    int rc = coarse_build(n, h->cg_na, h->cg_bounds, h->cg_aggmap, h->d.indptr, h->d.rowlen, h->d.col, h->d.w,
+                          h->d.solved, h->d.diag64, h->cg_Ac, h->cg_Ainv, st);
+    if (rc) return rc;
+    h->cg_builds += 1;
+    h->cg_age = 1;
+    return DLP_OK;
+  }
+  // boundaries: targets k*S0, each moved to the lowest-cut row boundary near it
   const int want = h->coarse_na < kCoarseMax ? h->coarse_na : kCoarseMax;
   int S0 = (n + want - 1) / want;
   if (S0 < min_agg) S0 = min_agg;
@@ -2604,6 +2615,7 @@ static int coarse_prepare(LP* h, hipStream_t st) {
   h->cg_na = na;
   h->cg_min = min_agg;
   h->cg_age = 1;
+  h->cg_builds += 1;
   return DLP_OK;
 }
 

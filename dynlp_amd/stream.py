@@ -47,7 +47,7 @@ class Stats:
     extra: dict = field(default_factory=dict)
 
 
-def slack_layout(indptr, indices, data, slack_min=4, slack_frac=0.25):
+def slack_layout(indptr, indices, data, slack_min=8, slack_frac=0.25):
     """Host ingest: a CSR expanded to per-row capacity deg + max(slack_min, slack_frac*deg)."""
     n = indptr.shape[0] - 1
     deg = np.diff(indptr).astype(np.int64)
@@ -81,7 +81,7 @@ class EdgeStreamLP:
     convergence return at once; with 16-bit directions each group adds its directions to Z once,
     at its end), 7 by default: a 10,000-edge repair of SBM-1M takes 6-7."""
 
-    def __init__(self, g: Graph, eps=1e-6, eps_h=0.05, fp64=False, eta=0.0, slack_min=4,
+    def __init__(self, g: Graph, eps=1e-6, eps_h=0.05, fp64=False, eta=0.0, slack_min=8,
                  slack_frac=0.25, push_scale=0.9, rounds_per_launch=4, use_graph=True, max_rounds=200_000,
                  push_frac=0.05, push_max_rows=4096, push_budget=1.0, push_mass=0.15, pcg_iters_per_launch=7,
                  pcg_max_iter=20_000, pcg_mode="column", incremental_ground=True, pcg_directions="fp16", pcg_coarse=128,
@@ -158,7 +158,7 @@ class EdgeStreamLP:
         self.nnz = int(ix.shape[0])
 
     @staticmethod
-    def footprint(g: Graph, fp64=False, slack_min=4, slack_frac=0.25, pcg_directions="fp16",
+    def footprint(g: Graph, fp64=False, slack_min=8, slack_frac=0.25, pcg_directions="fp16",
                   pcg_coarse=128, pcg_iters_per_launch=7, pcg_mode="column", **_):
         """Device bytes of a resident graph built with these settings, before its first
         request: the arrays this object allocates (slack store, masks, labels, FP32 state, FP64
