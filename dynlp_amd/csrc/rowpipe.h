@@ -5,7 +5,8 @@
 //
 // Lanes are packed (LanesP): ceil(C/W) column lanes times LE edge lanes.  While row
 // i's neighbour rows are gathered (S gathers per lane issued before the first multiply-add), row
-// i+1's range and its first 64 (column, weight) pairs are already in flight.  Neighbour rows are
+// i+1's first 64 (column, weight) pairs are already in flight; row ranges come from descriptors
+// loaded 64 row steps ahead (one lane per row).  Neighbour rows are
 // read with raw buffer loads at 32-bit byte offsets v*row_bytes + word*l (24-bit multiply-add);
 // lanes without an edge get an out-of-range offset, for which the buffer load returns zeros, so
 // no lane masking or select sits on the gather path.  Rows outside rowmask are skipped; neighbour
@@ -210,6 +211,43 @@ __device__ __forceinline__ void fmaw(TA (&acc)[WordT<K>::W], float wv, const typ
   }
 }
 
+// Row descriptors of 64 consecutive steps of this wavefront, one lane each: lane j describes
+// ordinal base + j*nwx.  mrow = row id, with bit 30 set when the row is dead (outside rowmask) or
+// past the end (row 0 then); [mbeg, mend) its entries (empty when dead).  The loads are issued a
+// batch ahead of use, so a row step reads its descriptor with a lane read instead of waiting on
+// dependent loads of the mask, the row start and the length.
+constexpr int kDead = 1 << 30;
+__device__ __forceinline__ void meta_load(const RowPipe& G, int64_t base, int& mrow, int& mbeg, int& mend) {
+  const int64_t ii = base + (int64_t)(threadIdx.x & 63) * G.nwx;
+  mrow = kDead;
+  mbeg = mend = 0;
+  if (ii < G.hi) {
+    const int ri = G.rows ? G.rows[ii] : (int)ii;
+    const bool live = G.rowmask ? G.rowmask[ri] != 0 : true;
+    mrow = live ? ri : (ri | kDead);
+    if (live) {
+      mbeg = G.indptr[ri];
+      mend = G.rowlen ? mbeg + G.rowlen[ri] : G.indptr[ri + 1];
+    }
+  }
+}
+
+// Lane l of the wavefront: v, wv = the (column, weight) pair beg + l when beg + l < end, else
+// (0, 0).  The loads go through buffer resources spanning [beg, min(end, beg + 64)), so lanes past
+// the end read zeros without a branch; every row step then issues the same number of vector
+// memory operations, which lets the compiler wait for exactly the loads a step consumes instead
+// of for everything in flight (stores of the previous row included).
+__device__ __forceinline__ void load_pairs(const RowPipe& G, int beg, int end, int& v, float& wv) {
+  const int cnt = __builtin_amdgcn_readfirstlane(max(0, min(64, end - beg)));
+  const __amdgpu_buffer_rsrc_t ri =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<int32_t*>(G.indices + beg), (short)0, cnt * 4, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rw =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(G.w + beg), (short)0, cnt * 4, 0x00020000);
+  const int off = (threadIdx.x & 63) * 4;
+  v = (int)__builtin_amdgcn_raw_buffer_load_b32(ri, off, 0, 0);
+  wv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rw, off, 0, 0));
+}
+
 // Runs every row of the pass: epi.pre(g, r) before row r = rows[g]'s gathers are issued
 // (own-row loads), epi.post(g, r, live, acc) with live = rowmask[r] and acc = sum over the row's
 // entries of w_j * X[col_j, W*l .. W*l+W-1] (zero when !live; valid in edge lane 0), called by
@@ -220,22 +258,20 @@ __device__ __forceinline__ void run_rows(const RowPipe& G, E& epi) {
   using raw = typename WordT<K>::raw;
   const LanesP& L = G.L;
   constexpr unsigned kOOB = 0x80000000u;
-  auto range = [&](int64_t ii, int& row, int& beg, int& end, bool& live) {
-    beg = end = 0;
-    live = false;
-    row = 0;
-    if (ii < G.hi) {
-      const int gi = __builtin_amdgcn_readfirstlane((int)ii);
-      const int ri = G.rows ? __builtin_amdgcn_readfirstlane(G.rows[gi]) : gi;
-      row = ri;
-      live = G.rowmask ? G.rowmask[ri] != 0 : true;
-      if (live) {
-        beg = __builtin_amdgcn_readfirstlane(G.indptr[ri]);
-        end = G.rowlen ? beg + __builtin_amdgcn_readfirstlane(G.rowlen[ri])
-                       : __builtin_amdgcn_readfirstlane(G.indptr[ri + 1]);
-      }
-    }
+  // descriptors: the current batch (c*) and the next one (n*), slot j of the current batch is
+  // the row of step i
+  int crow, cbeg, cend, nrow, nbeg, nend;
+  const int64_t bstep = 64 * G.nwx;
+  meta_load(G, G.lo + G.wx, crow, cbeg, cend);
+  meta_load(G, G.lo + G.wx + bstep, nrow, nbeg, nend);
+  auto take = [&](int vrow, int vbeg, int vend, int j, int& row, int& beg, int& end, bool& live) {
+    const int m = __builtin_amdgcn_readlane(vrow, j);
+    live = (m & kDead) == 0;
+    row = m & (kDead - 1);
+    beg = __builtin_amdgcn_readlane(vbeg, j);
+    end = __builtin_amdgcn_readlane(vend, j);
   };
+  int j = 0;
   // cnt_l: pairs this lane may read (0 on idle edge lanes); the select is branch-free
   auto gather = [&](int v, float wv, int t, int cnt_l, raw& xv, float& ww) {
     const int vv = __shfl(v, t & 63, 64);
@@ -257,17 +293,18 @@ __device__ __forceinline__ void run_rows(const RowPipe& G, E& epi) {
   int64_t i = G.lo + G.wx;
   int row0, beg0, end0;
   bool live0;
-  range(i, row0, beg0, end0, live0);
-  int v0 = 0;
-  float w0 = 0.f;
-  if (beg0 + L.lane < end0) {
-    v0 = G.indices[beg0 + L.lane];
-    w0 = G.w[beg0 + L.lane];
-  }
+  take(crow, cbeg, cend, 0, row0, beg0, end0, live0);
+  int v0, v1;
+  float w0, w1;
+  load_pairs(G, beg0, end0, v0, w0);
+  // everything issued so far has landed before the loop: otherwise the compiler's wait for the
+  // first row's pairs (a load still in flight only on entry) is repeated on every row step
+  __builtin_amdgcn_s_waitcnt(0);
   for (; i < G.hi; i += G.nwx) {
     int row1, beg1, end1;
     bool live1;
-    range(i + G.nwx, row1, beg1, end1, live1);
+    if (j < 63) take(crow, cbeg, cend, j + 1, row1, beg1, end1, live1);
+    else take(nrow, nbeg, nend, 0, row1, beg1, end1, live1);
     const int g = __builtin_amdgcn_readfirstlane((int)i);
     const int r = row0;
     epi.pre(g, r);
@@ -278,12 +315,7 @@ __device__ __forceinline__ void run_rows(const RowPipe& G, E& epi) {
 #pragma unroll
     for (int u = 0; u < S; ++u) gather(v0, w0, u * L.LE + L.e, cnt_l, xv[u], ww[u]);
     // next row's pairs go out while this row's gathers are in flight
-    int v1 = 0;
-    float w1 = 0.f;
-    if (beg1 + L.lane < end1) {
-      v1 = G.indices[beg1 + L.lane];
-      w1 = G.w[beg1 + L.lane];
-    }
+    load_pairs(G, beg1, end1, v1, w1);
     TA acc[W];
 #pragma unroll
     for (int q = 0; q < W; ++q) acc[q] = 0;
@@ -292,12 +324,9 @@ __device__ __forceinline__ void run_rows(const RowPipe& G, E& epi) {
     steps(v0, w0, S * L.LE, cnt, acc);
     // rows longer than 64 entries: further chunks of 64 pairs
     for (int base = beg0 + 64; base < end0; base += 64) {
-      int vc = 0;
-      float wc = 0.f;
-      if (base + L.lane < end0) {
-        vc = G.indices[base + L.lane];
-        wc = G.w[base + L.lane];
-      }
+      int vc;
+      float wc;
+      load_pairs(G, base, end0, vc, wc);
       steps(vc, wc, 0, min(64, end0 - base), acc);
     }
     // edge-lane partials of column lane l are at lanes l + k*LC
@@ -316,6 +345,14 @@ __device__ __forceinline__ void run_rows(const RowPipe& G, E& epi) {
     beg0 = beg1;
     end0 = end1;
     live0 = live1;
+    if (++j == 64) {
+      // the next batch becomes current; the one after it is loaded now (first used 63 steps on)
+      j = 0;
+      crow = nrow;
+      cbeg = nbeg;
+      cend = nend;
+      meta_load(G, i + G.nwx + bstep, nrow, nbeg, nend);
+    }
   }
 }
 

@@ -229,24 +229,24 @@ struct PcgEpilogue {
   float* __restrict__ out;
   int64_t ldo;
   double* sm;  // workgroup column sums of P .* AP (LDS)
-  float xr[W];
+  // own P row's lane word, kept raw until post: converting it here would make the row's
+  // gathers wait for this load
+  typename WordT<K>::raw xw;
   float ar;
   // this lane's partial sums of P .* AP over the rows it finishes (FP32: a few hundred terms
   // of FP16-held directions), added to the workgroup's FP64 column sums once, at the end
   float dacc[W] = {};
   __device__ __forceinline__ void pre(int, int r) {
-    if (owner) {
-      unpack<K>(load_word<K>(X + (int64_t)r * row_bytes + WordT<K>::bytes * L.l), xr);
-    } else {
-#pragma unroll
-      for (int q = 0; q < W; ++q) xr[q] = 0.f;
-    }
+    if (owner) xw = load_word<K>(X + (int64_t)r * row_bytes + WordT<K>::bytes * L.l);
+    else xw = typename WordT<K>::raw{};
     ar = a[r];
   }
   // out[r, W*l ..] = b*acc + ar*P (columns < C), dacc += P .* out; rows outside the mask get
   // AP = diag*P (no neighbour sum), as the plain operator writes them
   __device__ __forceinline__ void post(int, int r, bool, const float (&acc)[W]) {
     if (!owner) return;
+    float xr[W];
+    unpack<K>(xw, xr);
     const int c0 = W * L.l;
     float y[W];
 #pragma unroll
