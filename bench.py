@@ -444,6 +444,43 @@ def trace_summary(path, start=None):
     return "\n".join(out)
 
 
+def open_loop(g, batches, rates, seconds, seed):
+    """The same request stream under open-loop Poisson arrivals (seed S+3) at each rate (requests
+    per second), served by the continuous-batching service on the one resident graph: requests
+    that arrive while a repair runs are fused into the next one.  Each rate starts from a fresh
+    state of the graph and serves the stream's first requests; latency is arrival -> certified,
+    throughput = edge updates served / wall time from the first arrival to the last result."""
+    import numpy as np
+    import torch
+
+    from dynlp_amd import serve
+
+    pts = []
+    for rate in rates:
+        rng = np.random.default_rng(seed + 3)
+        arr = np.cumsum(rng.exponential(1.0 / rate, size=len(batches)))
+        k = int(np.searchsorted(arr, seconds, side="right"))
+        if k < 2:
+            continue
+        reqs = [serve.Request(0, b, float(t)) for b, t in zip(batches[:k], arr[:k])]
+        svc = serve.Service([g], 1 << 62, slots=1, eps=1e-6)
+        svc.run(reqs)
+        torch.cuda.synchronize()
+        rep = svc.report()
+        p = {"rate_req_s": rate, "offered_edge_updates_per_s": round(rate * batches[0].size, 1),
+             "requests": rep["requests"], "errors": rep["errors"],
+             "edge_updates_per_s": round(rep["edge_updates_per_s"], 1),
+             "p50_ms": round(rep["p50_ms"], 2), "p99_ms": round(rep["p99_ms"], 2),
+             "mean_fused": round(rep["mean_fused"], 2), "repairs": rep["repairs"], "max_rho": rep["max_rho"]}
+        log(f"open loop {rate:g} req/s ({p['offered_edge_updates_per_s']:,.0f} offered): "
+            f"{p['edge_updates_per_s']:,.0f} edge-updates/s, p50 {p['p50_ms']:.1f} ms p99 {p['p99_ms']:.1f} ms, "
+            f"{p['requests']} requests in {p['repairs']} repairs, max rho {p['max_rho']:.2e}")
+        pts.append(p)
+        del svc
+        torch.cuda.empty_cache()
+    return pts
+
+
 def serve_run(a):
     """The serving workload: open-loop Poisson arrivals over resident graphs, each request
     certified at rho <= 1e-6 in FP64 (Service: per-graph fusion, one push across graphs per
@@ -505,6 +542,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu-case", action="store_true",
                     help="skip the ER-100k (k = 10) CPU case: Jacobi to rho <= 1e-6 and one push batch, CPU and GPU")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel roofline table")
+    ap.add_argument("--open-loop-rates", default="25,50,100,200",
+                    help="Poisson arrival rates (requests/s) of the open-loop leg; empty: skip it")
+    ap.add_argument("--open-loop-seconds", type=float, default=1.5, help="arrival window of each open-loop rate")
     ap.add_argument("--no-full-solve", action="store_true",
                     help="skip the full solves from the prior (DynLP+ with AMG and Jacobi PCG, resident column and "
                          "block PCG)")
@@ -554,8 +594,12 @@ def main(argv=None):
         return pmc_probe(g)
     t = time.perf_counter()
     nreq = a.warmup + a.steps
-    batches, es_final = graphs.update_stream(g, nreq, a.batch_edges, seed=a.seed)
-    log(f"update stream: {nreq} requests x {a.batch_edges} edges ({time.perf_counter() - t:.1f}s)")
+    rates = [float(x) for x in a.open_loop_rates.split(",") if x.strip()]
+    # the open-loop leg serves the stream's first requests at each rate: enough for the largest
+    n_open = int(max(rates, default=0) * a.open_loop_seconds * 1.3) + 8 if rates else 0
+    batches, _, es_final = graphs.update_stream(g, max(nreq, n_open), a.batch_edges, seed=a.seed,
+                                                snapshot_after=nreq)
+    log(f"update stream: {len(batches)} requests x {a.batch_edges} edges ({time.perf_counter() - t:.1f}s)")
 
     # ---- mixed-precision service path -----------------------------------------------------
     t = time.perf_counter()
@@ -573,7 +617,7 @@ def main(argv=None):
     pcg0 = lp.pcg_stats()
     paused = 0.0
     t0 = time.perf_counter()
-    for t, b in enumerate(batches[a.warmup:]):
+    for t, b in enumerate(batches[a.warmup:nreq]):
         ts = time.perf_counter()
         st = lp.apply_and_solve(b)
         lat.append((time.perf_counter() - ts) * 1e3)
@@ -639,6 +683,15 @@ def main(argv=None):
     pcg_ms = (pcg1["pcg_us"] - pcg0["pcg_us"]) / 1e3
     del lp
     torch.cuda.empty_cache()
+
+    # ---- the same requests under open-loop Poisson arrivals (continuous batching) --------------
+    openl = None
+    if rates:
+        openl = {"workload": f"the same {a.batch_edges}-edge requests, Poisson arrivals (seed S+3) over "
+                             f"{a.open_loop_seconds:g} s per rate, served by the continuous-batching service on "
+                             "the one resident graph (requests arriving during a repair are fused into the "
+                             "next); latency = arrival -> certified",
+                 "points": open_loop(g, batches, rates, a.open_loop_seconds, a.seed)}
 
     # ---- the same requests on simpler paths: the plain FP64 per-batch path (the denominator of
     # the 5x target) and the ladder from it to the service, one change at a time ------------------
@@ -717,7 +770,7 @@ def main(argv=None):
         "p50_ms": round(percentile(lat, 50), 3), "p99_ms": round(percentile(lat, 99), 3),
         "rho_max": max(rhos), "paths": paths,
         "pcg_iters_per_request": round(pcg_iters / a.steps, 2), "pcg_ms_per_request": round(pcg_ms / a.steps, 3),
-        "error": err, "cc_bit_exact": cc_exact,
+        "error": err, "cc_bit_exact": cc_exact, "open_loop": openl,
         "fp64_path": fp64,
         "speedup_vs_fp64_path": round(value / fp64["value"], 2) if fp64 else None,
         "path_ladder": ladder or None,

@@ -424,14 +424,17 @@ def draw_batch(es: EdgeSet, size, ins_frac=0.5, rng=None, wlo=0.5, whi=1.5, unit
 
 
 def update_stream(g: Graph, n_batches, batch_size, ins_frac=0.5, seed=0, unit=None,
-                  inter_first=False):
+                  inter_first=False, snapshot_after=None):
     """Pre-drawn batches for graph g (seed S+2).  inter_first: deletions are drawn from
-    inter-block edges while any remain, then from all edges (fragmenting stream)."""
+    inter-block edges while any remain, then from all edges (fragmenting stream).  Returns the
+    batches and the edge set after the last one; with snapshot_after = k, also a copy of the
+    edge set after the first k batches."""
+    import copy
     rng = np.random.default_rng(seed + 2)
     es = EdgeSet(g)
     unit = bool(np.all(g.w == 1)) if unit is None else unit
-    out = []
-    for _ in range(n_batches):
+    out, snap = [], None
+    for i in range(n_batches):
         pool = None
         if inter_first and g.block is not None:
             uu, vv = es.key // es.n, es.key % es.n
@@ -439,4 +442,9 @@ def update_stream(g: Graph, n_batches, batch_size, ins_frac=0.5, seed=0, unit=No
         b = draw_batch(es, batch_size, ins_frac, rng, unit=unit, del_pool=pool)
         es.apply(b)
         out.append(b)
+        if snapshot_after == i + 1:
+            snap = copy.copy(es)
+            snap.key, snap.w = es.key.copy(), es.w.copy()
+    if snapshot_after is not None:
+        return out, es, snap
     return out, es

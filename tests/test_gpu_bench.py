@@ -14,7 +14,8 @@ def test_bench_line_and_csv_rows(tmp_path, capsys):
     from dynlp_amd import report
     out = tmp_path / "rows.csv"
     rc = bench.main(["--scale", "0.02", "--steps", "3", "--warmup", "1", "--batch-edges", "500",
-                     "--fp64-steps", "1", "--cpu-batches", "1", "--out", str(out), "--tag", "small"])
+                     "--fp64-steps", "1", "--cpu-batches", "1", "--out", str(out), "--tag", "small",
+                     "--open-loop-rates", "100,400", "--open-loop-seconds", "0.1"])
     assert rc == 0
     line = json.loads(capsys.readouterr().out.strip().splitlines()[-1])
     assert line["metric"].startswith("edge-updates/s") and line["value"] > 0 and line["n_gpus"] == 1
@@ -28,6 +29,12 @@ def test_bench_line_and_csv_rows(tmp_path, capsys):
     cpu = line["cpu_baseline"]
     assert cpu["kind"] == "port" and cpu["cores"] == 1 and cpu["value"] > 0
     assert line["fp64_path"]["value"] > 0 and line["rho_max"] <= 1e-6 * 1.0001
+    # open loop: every request served and certified; fused repairs never outnumber requests
+    pts = line["open_loop"]["points"]
+    assert [p["rate_req_s"] for p in pts] == [100.0, 400.0]
+    for p in pts:
+        assert p["requests"] >= 2 and p["errors"] == 0 and p["repairs"] <= p["requests"]
+        assert 0 < p["p50_ms"] <= p["p99_ms"] and p["max_rho"] <= 1e-6 * 1.0001 and p["edge_updates_per_s"] > 0
     assert {k["kernel"].split()[0] for k in line["kernels"]} >= {"spmm_axpy", "frontier_jacobi", "push_scatter",
                                                                    "amg"}
     assert all(k["achieved"] > 0 and 0 < k["frac"] < 1 for k in line["kernels"])
