@@ -39,8 +39,8 @@ def percentile(xs, q):
 
 
 def pmc_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed counter summary (FETCH_SIZE and
-    WRITE_SIZE runs scaled by a copy of known size), or None when it covers another kernel."""
+    """Memory traffic per launch of `kernel` from the committed counter summary (the L2's read
+    and write requests to memory, counted by size), or None when it covers another kernel."""
     import re
     path = os.path.join(ROOT, "profiles", "pcg_operator_sbm1m_k64_pmc.md")
     try:
@@ -647,7 +647,7 @@ def main(argv=None):
     roofline = {"kernel": kname,
                 "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "traffic_source": "profiles/pcg_operator_sbm1m_k64_pmc.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)"
+                "traffic_source": "profiles/pcg_operator_sbm1m_k64_pmc.md (rocprofv3: the L2's memory read and write requests by size)"
                 if traffic else None,
                 "avg_launch_ms": round(op_ms, 4), "algorithmic_bytes": int(nbytes)}
     log(f"A*P launch: {op_ms:.3f} ms, {nbytes / 1e6:.0f} MB -> {achieved:.0f} GB/s")
