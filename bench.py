@@ -340,8 +340,9 @@ def pmc_probe(g):
     kernel's launches are read against those of one plain copy of known size.  After the initial
     solve: 5 A*P launches (k_pcg_operator), the copy (512 MiB), 3 FP64 verification passes
     (k_verify, refresh mode 3), 3 connected-components passes of the store, 3 reference-API
-    SpMMs (n x C1 FP32) and 3 push scatters from every 10th row; each kernel's algorithmic
-    bytes per launch are printed for the summary."""
+    SpMMs (n x C1 FP32), 3 push scatters and 3 frontier Jacobi sweeps from every 10th row and 3
+    FP64 Gram products on the matrix cores; each kernel's algorithmic bytes (or flops) per launch
+    are printed for the summary."""
     import numpy as np
     import torch
     from dynlp_amd import _lib
@@ -398,6 +399,19 @@ def pmc_probe(g):
     torch.cuda.synchronize()
     log(f"pmc probe: 3 push scatters ({nf} rows), {nf * (12 + 4 * C) + 8 * e_f + nbrs.size * (2 * 4 * C + 1)} "
         f"algorithmic bytes each")
+    rhs = torch.rand((n, C), dtype=torch.float32, device="cuda", generator=gen)
+    for _ in range(3):
+        ops.jacobi(f, X, rhs, a, 1e-4)
+    torch.cuda.synchronize()
+    log(f"pmc probe: 3 frontier Jacobi sweeps ({nf} rows), "
+        f"{nf * (4 + 8 + 4 + 3 * C * 4 + 1) + 8 * e_f + 4 * C * nbrs.size} algorithmic bytes each")
+    Xg = torch.rand((n, (C + 3) // 4 * 4), dtype=torch.float32, device="cuda", generator=gen)
+    G = torch.empty((C, C), dtype=torch.float64, device="cuda")
+    for _ in range(3):
+        _lib.call("dlp_gram", n, C, B.ptr(Xg), Xg.shape[1], B.ptr(Xg), Xg.shape[1], None, B.ptr(G), _lib.F32,
+                  B.stream())
+    torch.cuda.synchronize()
+    log(f"pmc probe: 3 Gram products X^T X (n x {C}, FP64 MFMA), {2 * n * C * C} flops each")
     return 0
 
 
