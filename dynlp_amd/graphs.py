@@ -324,9 +324,10 @@ def normalize_batch(b: EdgeBatch, n: int, max_insertions: int | None = None) -> 
             continue
         u64, v64 = np.asarray(u, dtype=np.int64), np.asarray(v, dtype=np.int64)
         key = np.minimum(u64, v64) * n + np.maximum(u64, v64)
-        keep = _last_unique(key)
-        if keep.shape[0] != key.shape[0]:
-            out[kind] = np.sort(keep)
+        s = np.sort(key)
+        if not (s[1:] == s[:-1]).any():    # the common case, a sort cheaper than the stable one below
+            continue
+        out[kind] = np.sort(_last_unique(key))
     if not out:
         return b
     dk = out.get("del")
