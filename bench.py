@@ -298,23 +298,26 @@ def gpu_case_run(g, b):
 
 def full_solves(g):
     """The full solve of this graph from the prior, by the configuration's stated method (the
-    reference API's certified DynLP+ with the aggregation-AMG preconditioner, solvers.py:244-343,
-    error bound tol = 1e-6) and its Jacobi form, and by the resident path's initial solve (two-level
-    column PCG; block PCG with the FP64 Gram on the matrix cores), each to rho <= 1e-6 at least:
-    wall time, iterations and the certified bound on |F - F*|."""
+    reference API's certified DynLP+, solvers.py:244-343, error bound tol = 1e-6, with block PCG
+    and the aggregation-AMG preconditioner), its column-PCG and Jacobi forms, and by the
+    resident path's initial solve (two-level column PCG; block PCG with the FP64 Gram on the
+    matrix cores), each to rho <= 1e-6 at least: wall time, iterations and the certified bound
+    on |F - F*|."""
     import torch
     from dynlp_amd.solvers import DynLPPlus, build_system
     from dynlp_amd.stream import EdgeStreamLP
     sysd = build_system(g, "float32")
     rows = []
-    for method in ("amg", "pcg"):
-        solver = DynLPPlus(g.n, g.K, "float32", method=method, tol=1e-6)
+    for method, block in (("amg", True), ("amg", False), ("pcg", False)):
+        solver = DynLPPlus(g.n, g.K, "float32", method=method, tol=1e-6, block=block)
         torch.cuda.synchronize()
         t = time.perf_counter()
         st = solver.solve(sysd)
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t) * 1e3
-        rows.append({"solver": f"DynLP+ ({'aggregation-AMG' if method == 'amg' else 'Jacobi'} PCG), reference API",
+        kind = ("block PCG (FP64 Grams on the matrix cores, FP64 Cholesky)" if block else "column PCG")
+        rows.append({"solver": f"DynLP+ ({kind}, {'aggregation-AMG' if method == 'amg' else 'Jacobi'} "
+                               "preconditioner), reference API",
                      "ms": round(ms, 1), "setup_ms": round(st.setup_ms, 1), "iters": st.iters,
                      "cert_bound": st.cert, "hmax": st.hmax, "rho": st.cert / st.hmax if st.hmax else None})
         log(f"full solve, {rows[-1]['solver']}: {ms:.0f} ms, {st.iters} iterations, bound {st.cert:.2e}")

@@ -88,6 +88,7 @@ _SIGS = {
     "dlp_lp_bpcg": (ct.c_int, [P, I32, I32, I32, I32, P, P, P]),
     "dlp_gram": (ct.c_int, [I32, I32, P, I64, P, I64, P, P, I32, P]),
     "dlp_chol_solve": (ct.c_int, [I32, P, P, P, DBL, P]),
+    "dlp_block_update": (ct.c_int, [I32, I32, I32, P, P, P, P, P, I64, P, P, P, P, P]),
     "dlp_lp_time_operator": (ct.c_int, [P, I32, P, P]),
     "dlp_lp_scores": (ct.c_int, [P, P, I64, P]),
     "dlp_lp_set_pcg_directions": (ct.c_int, [P, I32]),

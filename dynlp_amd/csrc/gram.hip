@@ -279,13 +279,15 @@ __global__ __launch_bounds__(256) void k_chol_solve(int k, const double* __restr
 }
 
 // Row-local block updates with the k x k matrix in LDS (FP32 copy of FP64 S):
-// mode 0: Z += P S, R -= AP S, viol = max |R|/(eps*diag);  mode 1: P = R/diag + P S.
+// mode 0: Z += P S, R -= AP S, viol = max |R|/(eps*diag);  mode 1: P = Y + P S, with Y the
+// preconditioned residual (Y null: Y = R/diag, the Jacobi preconditioner).
 template <int CPL>
 __global__ __launch_bounds__(256) void k_block_update(int n, int C, int lc_shift, int mode, float* __restrict__ Z,
                                                       float* __restrict__ R, float* __restrict__ P,
-                                                      const float* __restrict__ AP, int64_t ld,
-                                                      const double* __restrict__ S, const float* __restrict__ diag,
-                                                      const float* __restrict__ eps, float* __restrict__ viol) {
+                                                      const float* __restrict__ AP, const float* __restrict__ Y,
+                                                      int64_t ld, const double* __restrict__ S,
+                                                      const float* __restrict__ diag, const float* __restrict__ eps,
+                                                      float* __restrict__ viol) {
   __shared__ float s[kGramMaxT * 16 * kGramMaxT * 16];
   __shared__ float vmax;
   for (int t = threadIdx.x; t < C * C; t += blockDim.x) s[t] = (float)S[t];
@@ -353,7 +355,7 @@ __global__ __launch_bounds__(256) void k_block_update(int n, int C, int lc_shift
             R[o + c] = rv;
             vm = fmaxf(vm, fabsf(rv) * inv * ie[q]);
           } else {
-            P[o + c] = R[o + c] * inv + outp[q];
+            P[o + c] = (Y ? Y[o + c] : R[o + c] * inv) + outp[q];
           }
         }
       }
@@ -462,8 +464,8 @@ int copy_f64(int n, const double* a, double* b, hipStream_t st) {
   return DLP_OK;
 }
 
-int block_update(int n, int C, int mode, float* Z, float* R, float* P, const float* AP, int64_t ld, const double* S,
-                 const float* diag, const float* eps, float* viol, hipStream_t st) {
+int block_update(int n, int C, int mode, float* Z, float* R, float* P, const float* AP, const float* Y, int64_t ld,
+                 const double* S, const float* diag, const float* eps, float* viol, hipStream_t st) {
   int lc = C <= 1 ? 0 : ceil_log2(C);
   if (lc > 6) lc = 6;
   const int cpl = (C + (1 << lc) - 1) >> lc;
@@ -472,7 +474,7 @@ int block_update(int n, int C, int mode, float* Z, float* R, float* P, const flo
   if (g > 2048) g = 2048;
   if (mode == 0) hipLaunchKernelGGL(k_zero_f32b, dim3(1), dim3(1), 0, st, viol);
 #define K(CPLV)                                                                                     \
-  hipLaunchKernelGGL((k_block_update<CPLV>), dim3(g), dim3(kBlock), 0, st, n, C, lc, mode, Z, R, P, AP, ld, S, \
+  hipLaunchKernelGGL((k_block_update<CPLV>), dim3(g), dim3(kBlock), 0, st, n, C, lc, mode, Z, R, P, AP, Y, ld, S, \
                      diag, eps, viol);
   DLP_DISPATCH_CPL(cpl, K)
 #undef K
@@ -480,3 +482,16 @@ int block_update(int n, int C, int mode, float* Z, float* R, float* P, const flo
   return DLP_OK;
 }
 }  // namespace dlp
+
+// The block PCG's row-local updates (FP32 n x C blocks of leading dimension ld, S = FP64 C x C
+// on the device): mode 0: Z += P S, R -= AP S and viol[0] = max |R|/(eps*diag) over the rows
+// (cleared first); mode 1: P = Y + P S (Y = NULL: Y = R/diag).
+extern "C" int dlp_block_update(int n, int C, int mode, float* Z, float* R, float* P, const float* AP, const float* Y,
+                                int64_t ld, const double* S, const float* diag, const float* eps, float* viol,
+                                void* stream) {
+  DLP_CHECK_ARG(n >= 0 && C >= 1 && C <= 16 * kGramMaxT && ld >= C && (mode == 0 || mode == 1),
+                "block_update: bad n=%d, C=%d, ld=%lld or mode=%d", n, C, (long long)ld, mode);
+  DLP_CHECK_ARG(R && P && S && diag && (mode == 1 || (Z && AP && eps && viol)), "block_update: null pointer");
+  if (n == 0) return DLP_OK;
+  return block_update(n, C, mode, Z, R, P, AP, Y, ld, S, diag, eps, viol, (hipStream_t)stream);
+}

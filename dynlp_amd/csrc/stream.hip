@@ -2989,8 +2989,8 @@ extern "C" int dlp_gram(int n, int C, const void* X, int64_t ldx, const void* Y,
                         double* G, int dtype, void* stream);
 extern "C" int dlp_chol_solve(int k, const double* A, const double* B, double* S, double tol, void* stream);
 namespace dlp {
-int block_update(int n, int C, int mode, float* Z, float* R, float* P, const float* AP, int64_t ld, const double* S,
-                 const float* diag, const float* eps, float* viol, hipStream_t st);
+int block_update(int n, int C, int mode, float* Z, float* R, float* P, const float* AP, const float* Y, int64_t ld,
+                 const double* S, const float* diag, const float* eps, float* viol, hipStream_t st);
 int copy_f64(int n, const double* a, double* b, hipStream_t st);
 }
 
@@ -3005,10 +3005,10 @@ static int bpcg_iteration(LP* h, const PcgBufs& b, hipStream_t st) {
                             b.P, b.ld, DLP_F32, b.diag, -1.0, nullptr, 0, b.AP, b.ld, DLP_F32, 0, 20.0, st);
   if (!rc) rc = dlp_gram(b.n, C, b.P, b.ld, b.AP, b.ld, nullptr, PAP, DLP_F32, st);
   if (!rc) rc = dlp_chol_solve(C, PAP, G, alpha, 1e-8, st);
-  if (!rc) rc = block_update(b.n, C, 0, b.Z, b.R, b.P, b.AP, b.ld, alpha, b.diag, b.eps, b.viol, st);
+  if (!rc) rc = block_update(b.n, C, 0, b.Z, b.R, b.P, b.AP, nullptr, b.ld, alpha, b.diag, b.eps, b.viol, st);
   if (!rc) rc = dlp_gram(b.n, C, b.R, b.ld, b.R, b.ld, b.diag, Gn, DLP_F32, st);
   if (!rc) rc = dlp_chol_solve(C, G, Gn, beta, 1e-8, st);
-  if (!rc) rc = block_update(b.n, C, 1, b.Z, b.R, b.P, b.AP, b.ld, beta, b.diag, b.eps, b.viol, st);
+  if (!rc) rc = block_update(b.n, C, 1, b.Z, b.R, b.P, b.AP, nullptr, b.ld, beta, b.diag, b.eps, b.viol, st);
   if (!rc) rc = copy_f64(C * C, Gn, G, st);
   return rc;
 }

@@ -1,8 +1,8 @@
 """Label-propagation solvers for one batch system, on the device.
 
 The public surface is the reference's (dynlp/solvers.py): `Stats`, `spmm`, `apply_A`,
-`rho_cols`, `violation`, `jacobi_M`, `pcg` (:66-97), `push` (:100-125), `supernode_init` /
-`init_h` (:128-163), `predict`, `ItLP` (:190-210), `DynLP` (:213-241), `DynLPPlus`
+`rho_cols`, `violation`, `jacobi_M`, `pcg` (:66-97; `block_pcg` is its block form),
+`push` (:100-125), `supernode_init` / `init_h` (:128-163), `predict`, `ItLP` (:190-210), `DynLP` (:213-241), `DynLPPlus`
 (:244-343), `IncrementalLP` (:346-434), `Reference` (:437-445), `make` (:448-458) and
 `SOLVERS`.  Behind it the work is device work:
 
@@ -345,6 +345,60 @@ def pcg(sys: System, Z, rhs, eps, M, max_iter, st: Stats, work_M=0, patience=400
     return Z
 
 
+def block_pcg(sys: System, Z, rhs, eps, M, max_iter, st: Stats, work_M=0, patience=400):
+    """Block PCG of A Z = rhs: one search space for all C columns (the block form of
+    solvers.py:66-97), the k x k Grams G = R^T M(R) and P^T A P on the FP64 matrix cores
+    (dlp_gram), their Cholesky solves in FP64 (dlp_chol_solve; collapsed directions deflate),
+    the row-local updates Z += P S, R -= AP S, P = M(R) + P B on FP32 blocks (dlp_block_update).
+    Same stop tests as `pcg`.  FP32 blocks of at most 80 columns; anything else goes to `pcg`."""
+    Z = Z.contiguous()
+    if Z.dtype != torch.float32 or not 1 <= Z.shape[1] <= 80:
+        return pcg(sys, Z, rhs, eps, M, max_iter, st, work_M, patience)
+    cg = _ColumnCG(sys, Z, rhs.contiguous(), eps)     # R, P and the true-residual pass
+    st.edges += sys.nnz
+    v = cg.residual()
+    if v <= 1:
+        return Z
+    n, C, ld = cg.n, cg.C, cg.ld
+    dev = Z.device
+    mats = torch.zeros((4, C, C), dtype=torch.float64, device=dev)   # G, P^T A P, G_new, step
+    G, PAP, Gn, S = mats[0], mats[1], mats[2], mats[3]
+    viol = torch.zeros(1, dtype=torch.float32, device=dev)
+    diag = sys.diag.to(torch.float32).contiguous()
+    eps32 = cg.eps.to(torch.float32)
+    f32 = _lib.F32
+
+    def gram(X, ldx, Y, ldy, out):
+        _lib.call("dlp_gram", n, C, ptr(X), ldx, ptr(Y), ldy, None, ptr(out), f32, stream())
+
+    Y = M(cg.R).contiguous()
+    cg.P.copy_(Y)
+    gram(cg.R, ld, Y, C, G)
+    track = _Progress(v, 0.9, patience)
+    for _ in range(max_iter):
+        AP = apply_A(sys, cg.P)
+        gram(cg.P, ld, AP, C, PAP)
+        _lib.call("dlp_chol_solve", C, ptr(PAP), ptr(G), ptr(S), 1e-8, stream())
+        _lib.call("dlp_block_update", n, C, 0, ptr(cg.Z), ptr(cg.R), ptr(cg.P), ptr(AP), None, ld, ptr(S), ptr(diag),
+                  ptr(eps32), ptr(viol), stream())
+        st.iters, st.edges = st.iters + 1, st.edges + sys.nnz + work_M
+        v = float(viol.item())
+        if v <= 1:
+            st.edges += sys.nnz
+            v = cg.residual()              # the recurrence converged: check the true residual
+            if v <= 1:
+                break
+        if track.stalled(v):
+            break
+        Y = M(cg.R).contiguous()
+        gram(cg.R, ld, Y, C, Gn)
+        _lib.call("dlp_chol_solve", C, ptr(G), ptr(Gn), ptr(S), 1e-8, stream())
+        _lib.call("dlp_block_update", n, C, 1, None, ptr(cg.R), ptr(cg.P), None, ptr(Y), ld, ptr(S), ptr(diag), None,
+                  None, stream())
+        G.copy_(Gn)
+    return Z
+
+
 def push(sys: System, Z, R, eps, ops: FrontierOps, st: Stats, budget_edges=None, max_iter=10**7):
     """Residual push on the violating set (solvers.py:100-125) in device rounds (FusedPush):
     candidates take D = R/diag into Z and scatter W D into R, the next candidates are the
@@ -534,12 +588,15 @@ class DynLPPlus(Solver):
        column-scaled residual refines the FP64 solution (10 rounds FP32, 2 FP64)."""
 
     def __init__(self, n_total, K, dtype="float32", method="auto", tol=1e-3, eps_h=0.05, group="auto",
-                 push_frac=0.05, push_budget=30.0, amg_hmax=2000.0, max_iter=100_000, amg_opts=None):
+                 push_frac=0.05, push_budget=30.0, amg_hmax=2000.0, max_iter=100_000, amg_opts=None,
+                 block=False):
         super().__init__(n_total, K, dtype, store_dtype="float64")
         self.method, self.tol, self.eps_h, self.group = method, tol, eps_h, group
         self.push_frac, self.push_budget, self.amg_hmax = push_frac, push_budget, amg_hmax
         self.max_iter, self.amg_opts = max_iter, amg_opts or {}
-        self.name, self.H, self.hmax = f"dynlp+{method}", None, None
+        # block: the global solves run as one block PCG over all columns (block_pcg)
+        self.block = bool(block)
+        self.name, self.H, self.hmax = f"dynlp+{method}" + ("-block" if block else ""), None, None
 
     # -- preconditioner and working repair --------------------------------------------------
     def _wants_amg(self):
@@ -561,7 +618,8 @@ class DynLPPlus(Solver):
 
     def _global_solve(self, ctx, Z, rhs, eps):
         M, work = self._preconditioner(ctx)
-        return pcg(ctx["sys"], Z, rhs, eps, M, self.max_iter, ctx["st"], work)
+        solve = block_pcg if self.block else pcg
+        return solve(ctx["sys"], Z, rhs, eps, M, self.max_iter, ctx["st"], work)
 
     def _repair(self, ctx, Z, rhs, eps):
         """Working-precision repair of A Z = rhs to eps (solvers.py:272-288)."""

@@ -442,6 +442,13 @@ int dlp_gram(int n, int C, const void* X, int64_t ldx, const void* Y, int64_t ld
 /* S = A^{-1} B for SPD A (k x k FP64, k <= 128) by Cholesky in one workgroup, scaled to unit
  * diagonal; pivots below tol deflate their direction (row of S = 0). */
 int dlp_chol_solve(int k, const double* A, const double* B, double* S, double tol, void* stream);
+/* Row-local updates of the block PCG (the block form of solvers.py:66-97, its Z/R/P updates
+ * :79-80, :93-94 with the k x k step matrices): FP32 n x C blocks of leading dimension ld, S the
+ * C x C FP64 step matrix on the device (C <= 80).  mode 0: Z += P S, R -= AP S and viol[0] =
+ * max |R_uc| / (eps_c diag_u) (viol cleared first); mode 1: P = Y + P S, Y the preconditioned
+ * residual M(R) (Y = NULL: R/diag, the Jacobi preconditioner). */
+int dlp_block_update(int n, int C, int mode, float* Z, float* R, float* P, const float* AP, const float* Y,
+                     int64_t ld, const double* S, const float* diag, const float* eps, float* viol, void* stream);
 /* Takes now every device buffer the global solve needs for captured groups of
  * iters_per_launch iterations (block != 0: the block PCG of dlp_lp_bpcg), with the current
  * direction storage and coarse setting, so that a service can allocate at admission and a

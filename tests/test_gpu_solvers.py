@@ -126,3 +126,48 @@ def test_refinement_certifies_beyond_float32(method):
     st = solver.solve(s)
     assert st.cert <= 1e-3
     assert np.abs(solver.scores(s).cpu().numpy() - Fd).max() <= st.cert * 1.05 + 1e-9
+
+
+@pytest.mark.parametrize("precond", ["jacobi", "amg"])
+def test_block_pcg_matches_oracle_block_pcg(precond):
+    """The device block PCG (FP64 Grams on the matrix cores, FP64 Cholesky steps, FP32 blocks)
+    against the oracle's FP64 block PCG with the same preconditioner: both reach the stopping
+    test, and the device solution is within the stopping bound of the direct solve."""
+    import torch
+    from dynlp_amd.amg import AMG
+    from dynlp_amd.solvers import Stats, block_pcg, jacobi_M
+    g = graphs.sbm(3000, 8, 10, 0.8, 16, 0.04, seed=11)
+    s = _sys(g, "float32")
+    o = O.build_system(g.n, g.u, g.v, g.w, g.labeled, g.y, g.K)
+    Wl, dl, rl = o.local()
+    C = rl.shape[1]
+    eps = [1e-5] * C
+    M = jacobi_M(s) if precond == "jacobi" else AMG(s.W, s.s)
+    Mo = None if precond == "jacobi" else O.AMG(Wl, o.s[o.U])
+    st = Stats()
+    Z = block_pcg(s, torch.zeros((s.n_u, C), dtype=torch.float32, device="cuda"), s.rhs.clone(), eps, M, 500, st)
+    Xo, it_o = O.block_pcg(Wl, dl, np.zeros((s.n_u, C)), rl.copy(), eps, M=Mo)
+    A, rhs, d = _dense(s.astype("float64"))
+    Fd = np.linalg.solve(A, rhs)
+    R = rhs - A @ Z.double().cpu().numpy()
+    assert (np.abs(R) / d[:, None]).max() <= 1e-5 * 1.0001          # the true residual passes the test
+    hmax = np.linalg.solve(A, d).max()
+    assert np.abs(Z.double().cpu().numpy() - Fd).max() <= 1e-5 * hmax * 1.05
+    assert np.abs(Xo - Fd).max() <= 1e-5 * hmax * 1.05
+    assert 1 <= st.iters <= 1.5 * it_o + 3, (st.iters, it_o)           # FP32 blocks: about the FP64 count
+
+
+def test_dynlp_plus_block_amg_certificate_holds_k64():
+    """DynLP+ with the aggregation-AMG preconditioner and the block PCG (the mixed-precision
+    configuration's stated method) at k = 64: the certified bound holds against the direct
+    solve."""
+    from dynlp_amd.solvers import DynLPPlus
+    g = graphs.sbm(4000, 64, 12, 0.8, 64, 0.05, seed=12)
+    s = _sys(g, "float32")
+    solver = DynLPPlus(g.n, 64, "float32", method="amg", tol=1e-6, block=True)
+    st = solver.solve(s)
+    A, rhs, _ = _dense(s.astype("float64"))
+    err = np.abs(solver.scores(s).cpu().numpy() - np.linalg.solve(A, rhs)).max()
+    assert solver.name == "dynlp+amg-block" and st.iters >= 1
+    assert st.cert <= 1e-6 * 1.0001
+    assert err <= st.cert * 1.05 + 1e-9, (err, st.cert)
