@@ -369,6 +369,97 @@ __global__ __launch_bounds__(256) void k_block_update(int n, int C, int lc_shift
   }
 }
 
+// The same updates on 64-row tiles: the step matrix (FP32, row pitch 80) and the tile's rows of
+// P (and AP) are staged in LDS with coalesced loads; a thread computes 20 consecutive output
+// columns of one row (four threads a row), the products read S as 16-byte words; the results go
+// back through the LDS tiles so that Z, R and P are read and written row-contiguously.
+constexpr int kBuRows = 64, kBuPitchS = 80, kBuCols = 20;
+template <int MODE>
+__global__ __launch_bounds__(256) void k_block_update_tiled(int n, int C, float* __restrict__ Z, float* __restrict__ R,
+                                                            float* __restrict__ P, const float* __restrict__ AP,
+                                                            const float* __restrict__ Y, int64_t ld,
+                                                            const double* __restrict__ S,
+                                                            const float* __restrict__ diag,
+                                                            const float* __restrict__ eps, float* __restrict__ viol) {
+  __shared__ __attribute__((aligned(16))) float st[kGramMaxT * 16 * kBuPitchS];
+  __shared__ float pt[kBuRows * (kGramMaxT * 16 + 1)];
+  __shared__ float at[MODE == 0 ? kBuRows * (kGramMaxT * 16 + 1) : 1];
+  __shared__ float ie[kGramMaxT * 16];
+  __shared__ float vmax;
+  const int PT = C + 1;   // tile row pitch: the 16 rows a wavefront reads fall in distinct banks
+  for (int t = threadIdx.x; t < C * kBuPitchS; t += blockDim.x) {
+    const int i = t / kBuPitchS, j = t - i * kBuPitchS;
+    st[t] = j < C ? (float)S[i * C + j] : 0.f;
+  }
+  if (MODE == 0) {
+    for (int j = threadIdx.x; j < C; j += blockDim.x) ie[j] = 1.f / eps[j];
+    if (threadIdx.x == 0) vmax = 0.f;
+  }
+  const int row = threadIdx.x >> 2, j0 = (threadIdx.x & 3) * kBuCols;
+  float vm = 0.f;
+  for (int64_t r0 = (int64_t)blockIdx.x * kBuRows; r0 < n; r0 += (int64_t)gridDim.x * kBuRows) {
+    const int rows = (int)min<int64_t>(kBuRows, n - r0);
+    __syncthreads();   // st and ie ready; the previous tile's write-back has finished with pt/at
+    for (int t = threadIdx.x; t < rows * C; t += blockDim.x) {
+      const int rr = t / C, c = t - rr * C;
+      const int64_t g = (r0 + rr) * ld + c;
+      pt[rr * PT + c] = P[g];
+      if (MODE == 0) at[rr * PT + c] = AP[g];
+    }
+    __syncthreads();
+    float az[kBuCols], ar[kBuCols];
+#pragma unroll
+    for (int k = 0; k < kBuCols; ++k) az[k] = ar[k] = 0.f;
+    if (row < rows && j0 < C) {
+      for (int i = 0; i < C; ++i) {
+        const float pv = pt[row * PT + i];
+        const float av = MODE == 0 ? at[row * PT + i] : 0.f;
+        const float4* sr = reinterpret_cast<const float4*>(st + i * kBuPitchS + j0);
+#pragma unroll
+        for (int q = 0; q < kBuCols / 4; ++q) {
+          const float4 s4 = sr[q];
+          const float sv[4] = {s4.x, s4.y, s4.z, s4.w};
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            az[4 * q + u] = __builtin_fmaf(pv, sv[u], az[4 * q + u]);
+            if (MODE == 0) ar[4 * q + u] = __builtin_fmaf(av, sv[u], ar[4 * q + u]);
+          }
+        }
+      }
+    }
+    __syncthreads();   // every thread has read the tiles: they now take the products
+    if (row < rows) {
+#pragma unroll
+      for (int k = 0; k < kBuCols; ++k) {
+        const int j = j0 + k;
+        if (j < C) {
+          pt[row * PT + j] = az[k];
+          if (MODE == 0) at[row * PT + j] = ar[k];
+        }
+      }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < rows * C; t += blockDim.x) {
+      const int rr = t / C, c = t - rr * C;
+      const int64_t g = (r0 + rr) * ld + c;
+      if (MODE == 0) {
+        Z[g] += pt[rr * PT + c];
+        const float rv = R[g] - at[rr * PT + c];
+        R[g] = rv;
+        vm = fmaxf(vm, fabsf(rv) / diag[r0 + rr] * ie[c]);
+      } else {
+        P[g] = (Y ? Y[g] : R[g] / diag[r0 + rr]) + pt[rr * PT + c];
+      }
+    }
+  }
+  if (MODE == 0) {
+    for (int off = 32; off > 0; off >>= 1) vm = fmaxf(vm, __shfl_xor(vm, off, 64));
+    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&vmax, vm);
+    __syncthreads();
+    if (threadIdx.x == 0) atomic_max_nonneg(viol, vmax);
+  }
+}
+
 __global__ void k_zero_f64b(int n, double* __restrict__ p) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0.0;
 }
@@ -473,6 +564,19 @@ int block_update(int n, int C, int mode, float* Z, float* R, float* P, const flo
   unsigned g = grid_for(threads);
   if (g > 2048) g = 2048;
   if (mode == 0) hipLaunchKernelGGL(k_zero_f32b, dim3(1), dim3(1), 0, st, viol);
+  if (C <= 4 * kBuCols && C <= kGramMaxT * 16) {
+    long long tiles = ((long long)n + kBuRows - 1) / kBuRows;
+    if (tiles < 1) return DLP_OK;
+    if (tiles > 4096) tiles = 4096;
+    if (mode == 0)
+      hipLaunchKernelGGL(k_block_update_tiled<0>, dim3((unsigned)tiles), dim3(kBlock), 0, st, n, C, Z, R, P, AP, Y, ld,
+                         S, diag, eps, viol);
+    else
+      hipLaunchKernelGGL(k_block_update_tiled<1>, dim3((unsigned)tiles), dim3(kBlock), 0, st, n, C, Z, R, P, AP, Y, ld,
+                         S, diag, eps, viol);
+    DLP_CHECK_LAUNCH();
+    return DLP_OK;
+  }
 #define K(CPLV)                                                                                     \
   hipLaunchKernelGGL((k_block_update<CPLV>), dim3(g), dim3(kBlock), 0, st, n, C, lc, mode, Z, R, P, AP, Y, ld, S, \
                      diag, eps, viol);
