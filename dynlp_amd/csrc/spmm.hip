@@ -629,6 +629,54 @@ __global__ __launch_bounds__(256) void k_spmm_rows(int nrows, int C, const int32
   run_rows<S, K, float>(G, E);
 }
 
+// One pass of an AMG level on the pipelined row gathers (A = diag - W, FP32 rows of ld floats):
+// mode 0 the residual out = R - A X, mode 1 a weighted-Jacobi sweep out = X + omega (R - A X) /
+// diag (the smoother's residual and correction in one pass).
+template <Word K>
+struct LevelEpilogue {
+  const LanesP& L;
+  int C;
+  bool owner;
+  const float* __restrict__ X;
+  const float* __restrict__ Rr;
+  int64_t ld;
+  const float* __restrict__ d;
+  float omega;
+  int mode;
+  float* __restrict__ out;
+  float xr[4], rr[4];
+  float dr;
+  __device__ __forceinline__ void pre(int, int r) {
+    if (owner) {
+      load4(X + (int64_t)r * ld, 4 * L.l, C, xr);
+      load4(Rr + (int64_t)r * ld, 4 * L.l, C, rr);
+    }
+    dr = d[r];
+  }
+  __device__ __forceinline__ void post(int, int r, bool, const float (&acc)[4]) {
+    if (!owner) return;
+    float y[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float res = rr[q] - dr * xr[q] + acc[q];
+      y[q] = mode == 0 ? res : xr[q] + omega * res / dr;
+    }
+    store4(out + (int64_t)r * ld, 4 * L.l, C, true, y);
+  }
+};
+
+template <int S>
+__global__ __launch_bounds__(256) void k_amg_level_rows(int nrows, int C, const int32_t* __restrict__ indptr,
+                                                        const int32_t* __restrict__ indices,
+                                                        const float* __restrict__ w, const float* __restrict__ X,
+                                                        int64_t ld, int xbytes, const float* __restrict__ R,
+                                                        const float* __restrict__ diag, float omega, int mode,
+                                                        float* __restrict__ out, int xcd) {
+  const RowPipe G(nrows, C, 4, 16, xcd, nullptr, indptr, nullptr, indices, w, X, (unsigned)(ld * 4), xbytes);
+  LevelEpilogue<Word::F32x4> E{G.L, C, G.L.e == 0 && 4 * G.L.l < C, X, R, ld, diag, omega, mode, out};
+  run_rows<S, Word::F32x4, float>(G, E);
+}
+
 // 0 when the packed form does not apply (the caller falls back to the generic row kernel).
 static int spmm_rows_f32(int n, int C, const int32_t* indptr, const int32_t* indices, const float* w,
                          const float* X, int64_t ldx, const float* a, float b, float* Y, int64_t ldy,
@@ -670,6 +718,33 @@ extern "C" int dlp_spmm_axpy(int n, int C, const int32_t* indptr, const int32_t*
   }
   return dlp_row_operator(n, nullptr, C, indptr, nullptr, indices, w, dtype, nullptr, X, ldx,
                           dtype, a, b, nullptr, 0, Y, ldy, dtype, 0, avg_deg, stream);
+}
+
+// An AMG level pass (amg.py:114-134's smoother and residual, A = diag - W): mode 0 out = R - A X,
+// mode 1 out = X + omega (R - A X) / diag; FP32 n x C blocks of leading dimension ld (a multiple
+// of 4, 16-byte aligned, n * ld * 4 < 2^31), plain CSR W.  DLP_ERR_ARG when the layout does not
+// fit (the caller then uses dlp_row_operator and dlp_colop).
+extern "C" int dlp_amg_level(int n, int C, const int32_t* indptr, const int32_t* indices, const float* w,
+                             const float* diag, const float* X, const float* R, int64_t ld, double omega, int mode,
+                             float* out, void* stream) {
+  DLP_CHECK_ARG(n >= 0 && C >= 1 && C <= 256 && ld >= C && (mode == 0 || mode == 1), "amg_level: bad arguments");
+  if (n == 0) return DLP_OK;
+  const long long xbytes = (long long)n * ld * 4;
+  DLP_CHECK_ARG(indptr && indices && w && diag && X && R && out && (ld % 4) == 0 && (((uintptr_t)X) & 15) == 0 &&
+                    (((uintptr_t)R) & 15) == 0 && (((uintptr_t)out) & 15) == 0 && n < (1 << 24) &&
+                    xbytes < (1LL << 31) && X != out,
+                "amg_level: layout (ld %% 4, 16-byte alignment, n * ld * 4 < 2^31, out apart from X)");
+  hipStream_t st = (hipStream_t)stream;
+  const void* kfn = reinterpret_cast<const void*>(&k_amg_level_rows<4>);
+  long long grid = resident_grid(kfn);
+  const long long need = ((long long)n + 3) / 4;
+  if (grid > need) grid = need;
+  const int xcd = tuning().xcd != 0;
+  if (xcd) grid = grid < 8 ? 8 : grid & ~7LL;
+  hipLaunchKernelGGL(k_amg_level_rows<4>, dim3((unsigned)grid), dim3(kBlock), 0, st, n, C, indptr, indices, w, X, ld,
+                     (int)xbytes, R, diag, (float)omega, mode, out, xcd);
+  DLP_CHECK_LAUNCH();
+  return DLP_OK;
 }
 
 // rho[c] = max_r |R[r,c]| / diag[r] over all rows of a 16-byte-aligned FP32 block (flat quads).

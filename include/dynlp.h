@@ -139,6 +139,13 @@ int dlp_amg_restrict(int n, int C, const int32_t* agg, const void* X, int64_t ld
                      int dtype, void* stream);
 int dlp_amg_prolong_add(int n, int C, const int32_t* agg, const void* Xc, int64_t ldc, void* X,
                         int64_t ldx, int dtype, void* stream);
+/* One level pass of the V-cycle (amg.py:114-134, A = diag - W on a plain FP32 CSR): mode 0 the
+ * residual out = R - A X (the Level residual of the restriction step), mode 1 one weighted-
+ * Jacobi sweep out = X + omega (R - A X) / diag (amg.py:114-117's smoother with its residual in
+ * the same pass).  FP32 n x C blocks of leading dimension ld, a multiple of 4, 16-byte aligned,
+ * n * ld * 4 < 2^31, out a separate block; DLP_ERR_ARG otherwise. */
+int dlp_amg_level(int n, int C, const int32_t* indptr, const int32_t* indices, const float* w, const float* diag,
+                  const float* X, const float* R, int64_t ld, double omega, int mode, float* out, void* stream);
 
 /* Coarsest AMG level in FP64 on the device (amg.py:104-108, where the reference inverts with
  * cuSOLVER and casts the inverse to the working dtype): A = diag(diag) - W as a dense FP64
