@@ -177,19 +177,23 @@ def kernel_table(g, C, reps=10):
 def amg_cycle_bytes(amg, C, s=4):
     """Algorithmic bytes of one V-cycle (nu = 1): its passes depend on each other, so each pass
     counts its distinct elements once.  Per level (n rows, nnz entries, nc aggregates): first
-    sweep 3nCs + ns, two residuals 2(3nCs + 4(n+1) + 8nnz + ns), restriction nCs + 2ncCs + 4n,
-    prolongation 2nCs + ncCs + 4n, correction 3nCs + ns; coarsest level: its inverse (FP32 n^2)
-    applied to an n x C block."""
+    sweep 3nCs + ns, two residual passes 2(3nCs + 4(n+1) + 8nnz + ns), restriction
+    nCs + 2ncCs + 4n, prolongation 2nCs + ncCs + 4n, and on levels whose sweep is not one fused
+    pass (Level.pipelined) the separate correction 3nCs + ns; coarsest level: its FP64 inverse
+    (8 n^2) applied to an n x C block."""
+    import torch
+    probe = torch.empty((1, (C + 3) // 4 * 4), dtype=torch.float32)
     total = 0
     for L in amg.levels:
         n, nnz, nc = L.W.shape[0], L.W.nnz, L.nc
         total += 3 * n * C * s + n * s                                   # first sweep
-        total += 2 * (3 * n * C * s + 4 * (n + 1) + 8 * nnz + n * s)    # residuals
+        total += 2 * (3 * n * C * s + 4 * (n + 1) + 8 * nnz + n * s)    # residual passes
         total += n * C * s + 2 * nc * C * s + 4 * n                      # restriction
         total += 2 * n * C * s + nc * C * s + 4 * n                      # prolongation
-        total += 3 * n * C * s + n * s                                   # correction
+        if not (s == 4 and L.pipelined(probe)):
+            total += 3 * n * C * s + n * s                               # correction
     nc = amg.sizes[-1]
-    total += (nc * nc + 2 * nc * C) * s
+    total += 8 * nc * nc + 2 * nc * C * s
     return total
 
 
