@@ -278,99 +278,11 @@ __global__ __launch_bounds__(256) void k_chol_solve(int k, const double* __restr
   }
 }
 
-// Row-local block updates with the k x k matrix in LDS (FP32 copy of FP64 S):
+// Row-local block updates on 64-row tiles (S = FP64 C x C step matrix, C <= 80):
 // mode 0: Z += P S, R -= AP S, viol = max |R|/(eps*diag);  mode 1: P = Y + P S, with Y the
-// preconditioned residual (Y null: Y = R/diag, the Jacobi preconditioner).
-template <int CPL>
-__global__ __launch_bounds__(256) void k_block_update(int n, int C, int lc_shift, int mode, float* __restrict__ Z,
-                                                      float* __restrict__ R, float* __restrict__ P,
-                                                      const float* __restrict__ AP, const float* __restrict__ Y,
-                                                      int64_t ld, const double* __restrict__ S,
-                                                      const float* __restrict__ diag, const float* __restrict__ eps,
-                                                      float* __restrict__ viol) {
-  __shared__ float s[kGramMaxT * 16 * kGramMaxT * 16];
-  __shared__ float vmax;
-  for (int t = threadIdx.x; t < C * C; t += blockDim.x) s[t] = (float)S[t];
-  if (threadIdx.x == 0) vmax = 0.f;
-  __syncthreads();
-  const int LC = 1 << lc_shift;
-  const int l = threadIdx.x & (LC - 1);
-  const int lane = threadIdx.x & 63;
-  const int gbase = lane & ~(LC - 1);     // first lane of this row group in the wavefront
-  float ie[CPL];
-#pragma unroll
-  for (int q = 0; q < CPL; ++q) {
-    const int c = l + q * LC;
-    ie[q] = (c < C && eps) ? 1.f / eps[c] : 0.f;
-  }
-  float vm = 0.f;
-  const int64_t groups = ((int64_t)gridDim.x * blockDim.x) >> lc_shift;
-  const int64_t wave_first = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) >> lc_shift;
-  const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> lc_shift;
-  for (int64_t base = wave_first; base < n; base += groups) {
-    const int64_t r = base + (g0 - wave_first);
-    const bool valid = r < n;
-    const int64_t o = (valid ? r : 0) * ld;
-    float p[CPL], ap[CPL], outp[CPL], outa[CPL];
-#pragma unroll
-    for (int q = 0; q < CPL; ++q) {
-      const int c = l + q * LC;
-      p[q] = (valid && c < C) ? P[o + c] : 0.f;
-      ap[q] = (valid && c < C && mode == 0) ? AP[o + c] : 0.f;
-      outp[q] = outa[q] = 0.f;
-    }
-    // out[j] = sum_i in[i] * S[i][j]: broadcast in[i] from the lane that holds column i
-    for (int i = 0; i < C; ++i) {
-      const int src = gbase + (i & (LC - 1));
-      const int qi = i >> lc_shift;
-      float pi = 0.f, ai = 0.f;
-#pragma unroll
-      for (int q = 0; q < CPL; ++q) {
-        const float tp = __shfl(p[q], src, 64);
-        const float ta = __shfl(ap[q], src, 64);
-        if (q == qi) {
-          pi = tp;
-          ai = ta;
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < CPL; ++q) {
-        const int j = l + q * LC;
-        if (j < C) {
-          const float sij = s[i * C + j];
-          outp[q] += pi * sij;
-          outa[q] += ai * sij;
-        }
-      }
-    }
-    if (valid) {
-      const float inv = 1.f / diag[r];
-#pragma unroll
-      for (int q = 0; q < CPL; ++q) {
-        const int c = l + q * LC;
-        if (c < C) {
-          if (mode == 0) {
-            Z[o + c] += outp[q];
-            const float rv = R[o + c] - outa[q];
-            R[o + c] = rv;
-            vm = fmaxf(vm, fabsf(rv) * inv * ie[q]);
-          } else {
-            P[o + c] = (Y ? Y[o + c] : R[o + c] * inv) + outp[q];
-          }
-        }
-      }
-    }
-  }
-  if (mode == 0) {
-    for (int off = 32; off > 0; off >>= 1) vm = fmaxf(vm, __shfl_xor(vm, off, 64));
-    if (lane == 0) atomic_max_nonneg(&vmax, vm);
-    __syncthreads();
-    if (threadIdx.x == 0) atomic_max_nonneg(viol, vmax);
-  }
-}
-
-// The same updates on 64-row tiles: the step matrix (FP32, row pitch 80) and the tile's rows of
-// P (and AP) are staged in LDS with coalesced loads; a thread computes 20 consecutive output
+// preconditioned residual (Y null: Y = R/diag, the Jacobi preconditioner).  The step matrix
+// (FP32 copy, row pitch 80) and the tile's rows of P (and AP) are staged in LDS with coalesced
+// loads; a thread computes 20 consecutive output
 // columns of one row (four threads a row), the products read S as 16-byte words; the results go
 // back through the LDS tiles so that Z, R and P are read and written row-contiguously.
 constexpr int kBuRows = 64, kBuPitchS = 80, kBuCols = 20;
@@ -557,35 +469,21 @@ int copy_f64(int n, const double* a, double* b, hipStream_t st) {
 
 int block_update(int n, int C, int mode, float* Z, float* R, float* P, const float* AP, const float* Y, int64_t ld,
                  const double* S, const float* diag, const float* eps, float* viol, hipStream_t st) {
-  int lc = C <= 1 ? 0 : ceil_log2(C);
-  if (lc > 6) lc = 6;
-  const int cpl = (C + (1 << lc) - 1) >> lc;
-  long long threads = (long long)n << lc;
-  unsigned g = grid_for(threads);
-  if (g > 2048) g = 2048;
+  static_assert(4 * kBuCols == kGramMaxT * 16 && kBuPitchS >= kGramMaxT * 16, "tile covers every column count");
+  DLP_CHECK_ARG(C >= 1 && C <= kGramMaxT * 16, "block_update: C=%d outside [1, %d]", C, kGramMaxT * 16);
   if (mode == 0) hipLaunchKernelGGL(k_zero_f32b, dim3(1), dim3(1), 0, st, viol);
-  if (C <= 4 * kBuCols && C <= kGramMaxT * 16) {
-    long long tiles = ((long long)n + kBuRows - 1) / kBuRows;
-    if (tiles < 1) return DLP_OK;
-    if (tiles > 4096) tiles = 4096;
-    if (mode == 0)
-      hipLaunchKernelGGL(k_block_update_tiled<0>, dim3((unsigned)tiles), dim3(kBlock), 0, st, n, C, Z, R, P, AP, Y, ld,
-                         S, diag, eps, viol);
-    else
-      hipLaunchKernelGGL(k_block_update_tiled<1>, dim3((unsigned)tiles), dim3(kBlock), 0, st, n, C, Z, R, P, AP, Y, ld,
-                         S, diag, eps, viol);
-    DLP_CHECK_LAUNCH();
-    return DLP_OK;
-  }
-#define K(CPLV)                                                                                     \
-  hipLaunchKernelGGL((k_block_update<CPLV>), dim3(g), dim3(kBlock), 0, st, n, C, lc, mode, Z, R, P, AP, Y, ld, S, \
-                     diag, eps, viol);
-  DLP_DISPATCH_CPL(cpl, K)
-#undef K
+  long long tiles = ((long long)n + kBuRows - 1) / kBuRows;
+  if (tiles < 1) return DLP_OK;
+  if (tiles > 4096) tiles = 4096;
+  if (mode == 0)
+    hipLaunchKernelGGL(k_block_update_tiled<0>, dim3((unsigned)tiles), dim3(kBlock), 0, st, n, C, Z, R, P, AP, Y, ld,
+                       S, diag, eps, viol);
+  else
+    hipLaunchKernelGGL(k_block_update_tiled<1>, dim3((unsigned)tiles), dim3(kBlock), 0, st, n, C, Z, R, P, AP, Y, ld,
+                       S, diag, eps, viol);
   DLP_CHECK_LAUNCH();
   return DLP_OK;
-}
-}  // namespace dlp
+}}  // namespace dlp
 
 // The block PCG's row-local updates (FP32 n x C blocks of leading dimension ld, S = FP64 C x C
 // on the device): mode 0: Z += P S, R -= AP S and viol[0] = max |R|/(eps*diag) over the rows
