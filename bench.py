@@ -54,10 +54,14 @@ def pmc_traffic(kernel):
     return int(b.group(1))
 
 
+# bytes per stored direction value
+DIRECTION_BYTES = {"fp32": 4, "bf16": 2, "fp16": 2, "fp8": 1}
+
+
 def operator_bytes(n, C1, stored, p_bytes=4):
     """Algorithmic bytes of one A*P launch (each distinct element once): row starts and
     lengths, column/weight of every stored entry, the solved mask, P read (p_bytes per value:
-    2 for 16-bit directions), diag, AP written (FP32)."""
+    2 for 16-bit directions, 1 for FP8), diag, AP written (FP32)."""
     return 4 * (n + 1) + 4 * n + 8 * stored + n + p_bytes * n * C1 + 4 * n + 4 * n * C1
 
 
@@ -342,7 +346,7 @@ def full_solves(g):
     return rows
 
 
-def pmc_probe(g):
+def pmc_probe(g, directions="fp16"):
     """Launch sequence for hardware-counter runs (rocprofv3 --pmc ...): the counters of each
     kernel's launches are read against those of one plain copy of known size.  After the initial
     solve: 5 A*P launches (k_pcg_operator), the copy (512 MiB), 3 FP64 verification passes
@@ -356,11 +360,11 @@ def pmc_probe(g):
     from dynlp_amd import backend as B
     from dynlp_amd.kernels import FrontierOps, spmm_axpy
     from dynlp_amd.stream import EdgeStreamLP
-    lp = EdgeStreamLP(g, eps=1e-6)
+    lp = EdgeStreamLP(g, eps=1e-6, pcg_directions=directions)
     lp.initial_solve()
     torch.cuda.synchronize()
     op_ms = lp.time_operator(reps=4)           # 1 + 4 launches
-    nbytes = operator_bytes(g.n, lp.C1, lp.stored_entries(), 4 if lp.pcg_directions == "fp32" else 2)
+    nbytes = operator_bytes(g.n, lp.C1, lp.stored_entries(), DIRECTION_BYTES[lp.pcg_directions])
     src = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
@@ -465,7 +469,7 @@ def trace_summary(path, start=None):
     return "\n".join(out)
 
 
-def open_loop(g, batches, rates, seconds, seed):
+def open_loop(g, batches, rates, seconds, seed, directions="fp16"):
     """The same request stream under open-loop Poisson arrivals (seed S+3) at each rate (requests
     per second), served by the continuous-batching service on the one resident graph: requests
     that arrive while a repair runs are fused into the next one.  Each rate starts from a fresh
@@ -484,7 +488,7 @@ def open_loop(g, batches, rates, seconds, seed):
         if k < 2:
             continue
         reqs = [serve.Request(0, b, float(t)) for b, t in zip(batches[:k], arr[:k])]
-        svc = serve.Service([g], 1 << 62, slots=1, eps=1e-6)
+        svc = serve.Service([g], 1 << 62, slots=1, eps=1e-6, lp_kwargs=dict(pcg_directions=directions))
         svc.run(reqs)
         torch.cuda.synchronize()
         rep = svc.report()
@@ -555,6 +559,8 @@ def main(argv=None):
     ap.add_argument("--batch-edges", type=int, default=10_000)
     ap.add_argument("--scale", type=float, default=1.0, help="multiplies n (1.0 = SBM-1M)")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--directions", default="fp16", choices=["fp32", "bf16", "fp16", "fp8"],
+                    help="storage of the global solve's search directions on the service path")
     ap.add_argument("--fp64-steps", type=int, default=4,
                     help="requests timed on each path of the ladder from the plain FP64 path to the service")
     ap.add_argument("--cpu-batches", type=int, default=1, help="requests timed on the CPU port")
@@ -612,7 +618,7 @@ def main(argv=None):
     g = graphs.config_graph(3, scale=a.scale, seed=a.seed)
     log(f"graph n={g.n:,} m={g.m:,} labelled={int(g.labeled.sum()):,} ({time.perf_counter() - t:.1f}s)")
     if a.pmc_probe:
-        return pmc_probe(g)
+        return pmc_probe(g, a.directions)
     t = time.perf_counter()
     nreq = a.warmup + a.steps
     rates = [float(x) for x in a.open_loop_rates.split(",") if x.strip()]
@@ -624,7 +630,7 @@ def main(argv=None):
 
     # ---- mixed-precision service path -----------------------------------------------------
     t = time.perf_counter()
-    lp = EdgeStreamLP(g, eps=1e-6)
+    lp = EdgeStreamLP(g, eps=1e-6, pcg_directions=a.directions)
     st0 = lp.initial_solve()
     Z0 = lp.scores_with_h().astype(np.float32)
     log(f"initial solve: {st0.ms:.0f} ms rho={st0.rho:.2e} (setup {time.perf_counter() - t:.1f}s)")
@@ -660,9 +666,9 @@ def main(argv=None):
     op_ms = lp.time_operator(reps=20)
     stored = lp.stored_entries()
     dirs = lp.pcg_directions
-    nbytes = operator_bytes(g.n, lp.C1, stored, 4 if dirs == "fp32" else 2)
+    nbytes = operator_bytes(g.n, lp.C1, stored, DIRECTION_BYTES[dirs])
     achieved = nbytes / (op_ms * 1e-3) / 1e9
-    word, wid = {"fp32": ("F32x4", 0), "bf16": ("BF16x4", 2), "fp16": ("F16x4", 3)}[dirs]
+    word, wid = {"fp32": ("F32x4", 0), "bf16": ("BF16x4", 2), "fp16": ("F16x4", 3), "fp8": ("F8x4", 5)}[dirs]
     kname = f"k_pcg_operator<4, {word}> (A*P on {dirs.upper()} directions with fused p^T A p, the global solve)"
     traffic = pmc_traffic(f"dlp::k_pcg_operator<4, (dlp::Word){wid}>")
     roofline = {"kernel": kname,
@@ -712,7 +718,7 @@ def main(argv=None):
                              f"{a.open_loop_seconds:g} s per rate, served by the continuous-batching service on "
                              "the one resident graph (requests arriving during a repair are fused into the "
                              "next); latency = arrival -> certified",
-                 "points": open_loop(g, batches, rates, a.open_loop_seconds, a.seed)}
+                 "points": open_loop(g, batches, rates, a.open_loop_seconds, a.seed, a.directions)}
 
     # ---- the same requests on simpler paths: the plain FP64 per-batch path (the denominator of
     # the 5x target) and the ladder from it to the service, one change at a time ------------------

@@ -22,13 +22,19 @@ struct PcgBufs {
   // [0] done (set once the violation is <= 1: every later pass of a captured group returns at
   // once), [1] iterations completed; null disables the device-side exit
   int32_t* ctl;
-  // 16-bit directions (null: P in FP32): n x ldh values, ldh % 8 == 0; rp = r.p per column;
-  // p16 = 1 BF16, 2 FP16 held scaled by sig[c] (a power of two per column, C floats)
+  // low-precision directions (null: P in FP32): rows of ldh 16-bit units; rp = r.p per column;
+  // p16 = 1 BF16 or 2 FP16 (ldh % 8 == 0), 3 FP8 E4M3 (ldh even: 4-byte groups of 4 values);
+  // FP16 and FP8 are held scaled by sig[c] (a power of two per column, C floats).  FP8 scales
+  // come from the column maxima rmax (max |r|/diag), pmax (max |P| as held) and emax (max |e|
+  // of the start's coarse correction), C floats each, zero between uses
   uint16_t* P16;
   int64_t ldh;
   double* rp;
   int p16;
   float* sig;
+  float* rmax;
+  float* pmax;
+  float* emax;
   // deferred solution updates (16-bit directions on the flat passes; slot_stride 0: Z is updated
   // by every iteration): a group of `group` iterations keeps its directions in slots of
   // slot_stride values (slot s at P16 + s*slot_stride); iteration `slot` gathers slot `slot`,

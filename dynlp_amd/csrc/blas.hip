@@ -151,8 +151,9 @@ __global__ __launch_bounds__(256) void k_pcg_dir(int n, int C, int lc_shift, flo
 
 // The direction block P of the flat passes (PK): 0 = FP32 (the n x Q4 float4 layout of Z and
 // R); 1 = BF16 or 2 = FP16 (rows of H4 groups of 4 values, 8 bytes each; stores round to
-// nearest even, FP16 saturating).  FP16 directions are held scaled by a power of two per column
-// (the caller passes the scaled values), which keeps them inside FP16's range.
+// nearest even, FP16 saturating); 3 = FP8 E4M3 (rows of H4 groups of 4 values, 4 bytes each;
+// nearest even, saturating).  FP16 and FP8 directions are held scaled by a power of two per
+// column (the caller passes the scaled values), which keeps them inside the format's range.
 template <int PK>
 struct PBlock {
   void* base;
@@ -164,6 +165,8 @@ struct PBlock {
     } else if constexpr (PK == 2) {
       const uint2 u = reinterpret_cast<const uint2*>(base)[r * H4 + q4];
       return make_float4(f16_lo(u.x), f16_hi(u.x), f16_lo(u.y), f16_hi(u.y));
+    } else if constexpr (PK == 3) {
+      return f8_unpack4(reinterpret_cast<const unsigned*>(base)[r * H4 + q4]);
     } else {
       return reinterpret_cast<const float4*>(base)[i];
     }
@@ -182,6 +185,10 @@ struct PBlock {
       u.y = f16_pack(v.z, v.w);
       reinterpret_cast<uint2*>(base)[r * H4 + q4] = u;
       return make_float4(f16_lo(u.x), f16_hi(u.x), f16_lo(u.y), f16_hi(u.y));
+    } else if constexpr (PK == 3) {
+      const unsigned u = f8_pack4(v.x, v.y, v.z, v.w);
+      reinterpret_cast<unsigned*>(base)[r * H4 + q4] = u;
+      return f8_unpack4(u);
     } else {
       reinterpret_cast<float4*>(base)[i] = v;
       return v;
@@ -197,6 +204,17 @@ __device__ __forceinline__ void flush_quads(const Quad& Qd, int Q4, int C, const
     if (j < Qd.cnt) lds_add(&sm[4 * Qd.q4 + j], acc[j]);
   __syncthreads();
   for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&out[c], sm[c]);
+}
+
+// Column maxima of non-negative per-thread quad values: LDS max (smx cleared beforehand), then,
+// after a barrier the caller places, one atomic per column per workgroup (quad_max_out).
+__device__ __forceinline__ void quad_max_lds(int q4, int cnt, const float (&m)[4], float* smx) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if (j < cnt) atomic_max_nonneg(&smx[4 * q4 + j], m[j]);
+}
+__device__ __forceinline__ void quad_max_out(int C, const float* smx, float* __restrict__ out) {
+  for (int c = threadIdx.x; c < C; c += blockDim.x) atomic_max_nonneg(&out[c], smx[c]);
 }
 
 // Loads and stores of the FP32 vector streams of the flat passes; NT marks them non-temporal so
@@ -299,25 +317,31 @@ __device__ __forceinline__ float4 coarse_quad(const CoarseArg& K, int64_t r, int
 }
 
 // pcg start: P = M R (M R = R/diag, plus the coarse correction on solved rows), rz[c] += sum
-// R*(M R); with a 16-bit block also rp[c] += sum R*P as held.  FP16 blocks hold sig[c] * M R
-// (sig from the rz of a prior k_pcg_rz4 pass).
+// R*(M R); with a 16-bit or 8-bit block also rp[c] += sum R*P as held.  FP16 and FP8 blocks hold
+// sig[c] * M R (sig from a prior k_pcg_rz4 pass); FP8 blocks also record pmax[c] = max |P| as
+// held.
 template <int PK>
 __global__ __launch_bounds__(256) void k_pcg_start4(int n, int Q4, int C, PBlock<PK> P, const float4* __restrict__ R,
                                                     const float* __restrict__ diag, double* __restrict__ rz,
                                                     double* __restrict__ rp, const float* __restrict__ sig,
-                                                    CoarseArg K) {
+                                                    CoarseArg K, float* __restrict__ pmax) {
   __shared__ double sm[kMaxCols + 4];
   __shared__ double sm2[kMaxCols + 4];
-  for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) sm[c] = sm2[c] = 0.0;
+  __shared__ float smx[PK == 3 ? kMaxCols + 4 : 1];
+  for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) {
+    sm[c] = sm2[c] = 0.0;
+    if constexpr (PK == 3) smx[c] = 0.f;
+  }
   __syncthreads();
   const Quad Qd = quad_of(Q4, C);
   float sg[4] = {1.f, 1.f, 1.f, 1.f};
-  if constexpr (PK == 2) {
+  if constexpr (PK >= 2) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) sg[j] = j < Qd.cnt ? sig[4 * Qd.q4 + j] : 1.f;
   }
   const int64_t nq = (int64_t)n * Q4;
   double acc[4] = {0.0, 0.0, 0.0, 0.0}, acc2[4] = {0.0, 0.0, 0.0, 0.0};
+  float pm[4] = {0.f, 0.f, 0.f, 0.f};
   if (Qd.active && Qd.r0 < n) {
     int64_t r = Qd.r0;
     for (int64_t i = Qd.i0; i < nq; i += Qd.step, r += Qd.rstep) {
@@ -335,26 +359,36 @@ __global__ __launch_bounds__(256) void k_pcg_start4(int n, int Q4, int C, PBlock
         if (j < Qd.cnt) {
           acc[j] += (double)quad_comp(rv, j) * (double)quad_comp(y, j);
           if constexpr (PK != 0) acc2[j] += (double)quad_comp(rv, j) * (double)quad_comp(p, j);
+          if constexpr (PK == 3) pm[j] = fmaxf(pm[j], fabsf(quad_comp(p, j)));
         }
       }
     }
   }
-  flush_quads(Qd, Q4, C, acc, sm, rz);
+  if constexpr (PK == 3)
+    if (Qd.active) quad_max_lds(Qd.q4, Qd.cnt, pm, smx);
+  flush_quads(Qd, Q4, C, acc, sm, rz);   // (its barrier also orders the LDS maxima)
   if constexpr (PK != 0) flush_quads(Qd, Q4, C, acc2, sm2, rp);
+  if constexpr (PK == 3) quad_max_out(C, smx, pmax);
 }
 
 // rz[c] += sum R*R/diag, and with K the restriction rc += P^T R (the pass before a start that
-// needs the column scales or the coarse correction).
+// needs the column scales or the coarse correction); with rmax also rmax[c] = max |R|/diag.
 __global__ __launch_bounds__(256) void k_pcg_rz4(int n, int Q4, int C, const float4* __restrict__ R,
-                                                 const float* __restrict__ diag, double* __restrict__ rz, CoarseArg K) {
+                                                 const float* __restrict__ diag, double* __restrict__ rz, CoarseArg K,
+                                                 float* __restrict__ rmax) {
   __shared__ double sm[kMaxCols + 4];
   __shared__ double sm2[2 * (kMaxCols + 4)];
-  for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) sm[c] = sm2[c] = sm2[4 * Q4 + c] = 0.0;
+  __shared__ float smx[kMaxCols + 4];
+  for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) {
+    sm[c] = sm2[c] = sm2[4 * Q4 + c] = 0.0;
+    smx[c] = 0.f;
+  }
   __syncthreads();
   const QuadRange q = quad_range(Q4, C, n);
   Restrict rs;
   rs.init(q.r_lo, n, K);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  float rm[4] = {0.f, 0.f, 0.f, 0.f};
   if (q.active) {
     for (int64_t r = q.r_lo + q.roff; r < q.r_hi; r += q.rstep) {
       const double inv = 1.0 / (double)diag[r];
@@ -364,6 +398,7 @@ __global__ __launch_bounds__(256) void k_pcg_rz4(int n, int Q4, int C, const flo
         if (j < q.cnt) {
           const double x = (double)quad_comp(rv, j);
           acc[j] += x * x * inv;
+          rm[j] = fmaxf(rm[j], (float)(fabs(x) * inv));
           if (K.na) rs.add(r, j, x);
         }
       }
@@ -372,15 +407,18 @@ __global__ __launch_bounds__(256) void k_pcg_rz4(int n, int Q4, int C, const flo
 #pragma unroll
   for (int j = 0; j < 4; ++j)
     if (j < q.cnt) lds_add(&sm[4 * q.q4 + j], acc[j]);
+  if (rmax) quad_max_lds(q.q4, q.cnt, rm, smx);
   if (K.na) rs.to_lds(q, Q4, sm2);
   __syncthreads();
   for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&rz[c], sm[c]);
+  if (rmax) quad_max_out(C, smx, rmax);
   if (K.na) rs.to_global(Q4, C, K, sm2);
 }
 
 // Z += a P, R -= a AP with the step length a = num/pAp of every column (num = r.p for 16-bit
 // directions, r.z otherwise; 0 where pAp <= 0); rz_new[c] += sum R*R/diag; viol = max
-// |R|/(eps_c*diag); with K also rc += P^T R (the new residual restricted to the aggregates).
+// |R|/(eps_c*diag); with K also rc += P^T R (the new residual restricted to the aggregates);
+// with FP8 directions (PK 3) also rmax[c] = max |R|/diag (the next direction's scale bound).
 // Rows in a contiguous split.
 // DEFER: Z and P are not touched; the step lengths go to alpha_out (C floats, written by the
 // first workgroup; zeros when the iteration is skipped) for the group's deferred update.
@@ -392,7 +430,7 @@ __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float
                                                      const float* __restrict__ diag, const float* __restrict__ eps,
                                                      double* __restrict__ rz_new, float* __restrict__ viol,
                                                      const int32_t* __restrict__ skip, CoarseArg K,
-                                                     float* __restrict__ alpha_out) {
+                                                     float* __restrict__ alpha_out, float* __restrict__ rmax) {
   if (skip && *skip) {
     if constexpr (DEFER) {
       if (blockIdx.x == 0)
@@ -403,13 +441,17 @@ __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float
   __shared__ double sm[kMaxCols + 4];
   __shared__ double sm2[2 * (kMaxCols + 4)];
   __shared__ float vmax;
-  for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) sm[c] = sm2[c] = sm2[4 * Q4 + c] = 0.0;
+  __shared__ float smx[PK == 3 ? kMaxCols + 4 : 1];
+  for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) {
+    sm[c] = sm2[c] = sm2[4 * Q4 + c] = 0.0;
+    if constexpr (PK == 3) smx[c] = 0.f;
+  }
   if (threadIdx.x == 0) vmax = 0.f;
   __syncthreads();
   const QuadRange q = quad_range(Q4, C, n);
   Restrict rs;
   rs.init(q.r_lo, n, K);
-  float a[4], ie[4];
+  float a[4], ie[4], rm[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int c = 4 * q.q4 + j;
@@ -452,6 +494,7 @@ __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float
           const float x = quad_comp(rv, j);
           acc[j] += (double)x * (double)x * (double)inv;
           vm = fmaxf(vm, fabsf(x) * inv * ie[j]);
+          if constexpr (PK == 3) rm[j] = fmaxf(rm[j], fabsf(x) * inv);
           if (K.na) rs.add(r, j, (double)x);
         }
       }
@@ -461,40 +504,49 @@ __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float
   for (int j = 0; j < 4; ++j)
     if (j < q.cnt) lds_add(&sm[4 * q.q4 + j], acc[j]);
   if (K.na) rs.to_lds(q, Q4, sm2);
+  if constexpr (PK == 3) quad_max_lds(q.q4, q.cnt, rm, smx);
   for (int off = 32; off > 0; off >>= 1) vm = fmaxf(vm, __shfl_xor(vm, off, 64));
   if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&vmax, vm);
   __syncthreads();
   for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&rz_new[c], sm[c]);
+  if constexpr (PK == 3) quad_max_out(C, smx, rmax);
   if (K.na) rs.to_global(Q4, C, K, sm2);
   if (threadIdx.x == 0) atomic_max_nonneg(viol, vmax);
 }
 
 // Pout = M R + beta P (M R = R/diag plus the coarse correction on solved rows; Pout may be P);
 // with a 16-bit block also rp[c] += sum R*Pout as held (the next step length's numerator, exact
-// for the rounded direction).  FP16 blocks: Pout = sig*M R + beta*P with sig the new column
-// scale and beta already rescaled to the held direction.
+// for the rounded direction).  FP16 and FP8 blocks: Pout = sig*M R + beta*P with sig the new
+// column scale and beta already rescaled to the held direction; FP8 blocks also record
+// pmax[c] = max |Pout| as held (cleared beforehand).
 template <int PK, bool NT = false>
 __global__ __launch_bounds__(256) void k_pcg_dir4(int n, int Q4, int C, PBlock<PK> P, PBlock<PK> Pout,
                                                   const float4* __restrict__ R,
                                                   const float* __restrict__ diag, const double* __restrict__ beta,
                                                   double* __restrict__ rp, const int32_t* __restrict__ skip,
-                                                  const float* __restrict__ sig, CoarseArg K) {
+                                                  const float* __restrict__ sig, CoarseArg K,
+                                                  float* __restrict__ pmax) {
   if (skip && *skip) return;
   __shared__ double sm[kMaxCols + 4];
+  __shared__ float smx[PK == 3 ? kMaxCols + 4 : 1];
   if constexpr (PK != 0) {
-    for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) sm[c] = 0.0;
+    for (int c = threadIdx.x; c < 4 * Q4; c += blockDim.x) {
+      sm[c] = 0.0;
+      if constexpr (PK == 3) smx[c] = 0.f;
+    }
     __syncthreads();
   }
   const Quad Qd = quad_of(Q4, C);
   float b[4], sg[4] = {1.f, 1.f, 1.f, 1.f};
 #pragma unroll
   for (int j = 0; j < 4; ++j) b[j] = j < Qd.cnt ? (float)beta[4 * Qd.q4 + j] : 0.f;
-  if constexpr (PK == 2) {
+  if constexpr (PK >= 2) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) sg[j] = j < Qd.cnt ? sig[4 * Qd.q4 + j] : 1.f;
   }
   const int64_t nq = (int64_t)n * Q4;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  float pm[4] = {0.f, 0.f, 0.f, 0.f};
   if (Qd.active && Qd.r0 < n) {
     int64_t r = Qd.r0;
     for (int64_t i = Qd.i0; i < nq; i += Qd.step, r += Qd.rstep) {
@@ -502,7 +554,7 @@ __global__ __launch_bounds__(256) void k_pcg_dir4(int n, int Q4, int C, PBlock<P
       const float4 rv = ld4<NT>(R + i);
       const float4 ev = coarse_quad(K, r, Qd.q4);
       float4 p = P.load(i, r, Qd.q4);
-      if constexpr (PK == 2) {
+      if constexpr (PK >= 2) {
         if (Qd.cnt > 0) p.x = (rv.x * inv + ev.x) * sg[0] + b[0] * p.x;
         if (Qd.cnt > 1) p.y = (rv.y * inv + ev.y) * sg[1] + b[1] * p.y;
         if (Qd.cnt > 2) p.z = (rv.z * inv + ev.z) * sg[2] + b[2] * p.z;
@@ -516,12 +568,19 @@ __global__ __launch_bounds__(256) void k_pcg_dir4(int n, int Q4, int C, PBlock<P
       const float4 held = Pout.store(i, r, Qd.q4, p);
       if constexpr (PK != 0) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-          if (j < Qd.cnt) acc[j] += (double)quad_comp(rv, j) * (double)quad_comp(held, j);
+        for (int j = 0; j < 4; ++j) {
+          if (j < Qd.cnt) {
+            acc[j] += (double)quad_comp(rv, j) * (double)quad_comp(held, j);
+            if constexpr (PK == 3) pm[j] = fmaxf(pm[j], fabsf(quad_comp(held, j)));
+          }
+        }
       }
     }
   }
-  if constexpr (PK != 0) flush_quads(Qd, Q4, C, acc, sm, rp);
+  if constexpr (PK == 3)
+    if (Qd.active) quad_max_lds(Qd.q4, Qd.cnt, pm, smx);
+  if constexpr (PK != 0) flush_quads(Qd, Q4, C, acc, sm, rp);   // (its barrier also orders the maxima)
+  if constexpr (PK == 3) quad_max_out(C, smx, pmax);
 }
 
 // The deferred solution update of a group of G <= kMaxGroup iterations: Z += sum_s alpha[s] *
@@ -617,16 +676,41 @@ __device__ __forceinline__ float pcg_scale(double rz) {
   return (float)ldexp(1.0, (int)e);
 }
 
+// Power-of-two scale of an FP8 direction column from a bound on the largest magnitude of the
+// unscaled direction: the held values stay within 224, half of E4M3's largest finite value, so
+// no value saturates (FP8 has too few exponent bits for the rz-based FP16 scale); 1 when the
+// bound is 0 (the column is converged).
+__device__ __forceinline__ float f8_scale(double bound) {
+  if (!(bound > 0.0)) return 1.f;
+  double e = floor(log2(224.0 / bound));
+  e = fmin(fmax(e, -100.0), 100.0);
+  return (float)ldexp(1.0, (int)e);
+}
+
+// The new scale of an FP8 direction column, P_new = M r + beta P_old: max |M r| <= rmax + emax
+// (rmax = max |r|/diag, emax = max |coarse correction|) and max |P_old| = pmax / sig_old (pmax =
+// the held maximum); the maxima are cleared for the next iteration.
+__device__ __forceinline__ float f8_next_scale(int c, double beta, float emax, float* rmax, float* pmax,
+                                               const float* sig) {
+  const double bound = (double)rmax[c] + (double)emax + fabs(beta) * (double)pmax[c] / (double)sig[c];
+  rmax[c] = 0.f;
+  pmax[c] = 0.f;
+  return f8_scale(bound);
+}
+
 // Scalars of one iteration (one workgroup): mode 0: alpha = rz/pAp (0 where pAp <= 0), then
 // clear rz_new and viol for the update pass; mode 1: beta = rz_new/rz (0 where rz <= 0),
 // rz = rz_new, then clear rz_new, pAp, rp and (after the convergence test, keeping its value in
-// viol[1] for the host) viol for the next iteration; with sig (FP16 directions) also the new column scale, and beta rescaled to the
-// held direction (beta * sig_new / sig_old).  Mode 0 (the generic passes) also clears rc (nrc
-// doubles).  Mode 2 (before a start that had a scale pass): sig from rz (when sig), then rz = 0.
+// viol[1] for the host) viol for the next iteration; with sig (FP16 or FP8 directions) also the
+// new column scale, and beta rescaled to the held direction (beta * sig_new / sig_old).  Mode 0
+// (the generic passes) also clears rc (nrc doubles).  Mode 2 (before a start that had a scale
+// pass): sig from rz (when sig), then rz = 0.  FP8 directions (rmax non-null) take their scale
+// from the maxima instead (rmax, pmax; emax in mode 2, the start's coarse correction).
 __global__ void k_pcg_scalars(int C, int mode, double* __restrict__ rz, double* __restrict__ rz_new,
                               double* __restrict__ pAp, double* __restrict__ alpha,
                               double* __restrict__ beta, float* __restrict__ viol, int32_t* ctl,
-                              double* __restrict__ rp, float* __restrict__ sig, double* __restrict__ rc, int nrc) {
+                              double* __restrict__ rp, float* __restrict__ sig, double* __restrict__ rc, int nrc,
+                              float* __restrict__ rmax, float* __restrict__ pmax, float* __restrict__ emax) {
   // every thread reads the flag before thread 0 may set it below (after the barrier)
   if (ctl && ctl[0]) return;
   if (mode == 0 && rc)
@@ -641,7 +725,7 @@ __global__ void k_pcg_scalars(int C, int mode, double* __restrict__ rz, double* 
       const double r = rz[c];
       double bt = r > 0.0 ? rz_new[c] / r : 0.0;
       if (sig) {
-        const float s = pcg_scale(rz_new[c]);
+        const float s = rmax ? f8_next_scale(c, bt, 0.f, rmax, pmax, sig) : pcg_scale(rz_new[c]);
         bt *= (double)s / (double)sig[c];
         sig[c] = s;
       }
@@ -651,7 +735,14 @@ __global__ void k_pcg_scalars(int C, int mode, double* __restrict__ rz, double* 
       pAp[c] = 0.0;
       if (rp) rp[c] = 0.0;
     } else {
-      if (sig) sig[c] = pcg_scale(rz[c]);
+      if (sig && rmax) {
+        sig[c] = f8_scale((double)rmax[c] + (double)(emax ? emax[c] : 0.f));
+        rmax[c] = 0.f;
+        pmax[c] = 0.f;
+        if (emax) emax[c] = 0.f;
+      } else if (sig) {
+        sig[c] = pcg_scale(rz[c]);
+      }
       rz[c] = 0.0;
     }
   }
@@ -673,47 +764,65 @@ __global__ void k_pcg_scalars(int C, int mode, double* __restrict__ rz, double* 
 
 // ec[I][c] = sum_J Ainv[J][I] rcs[J] for thread I < na (Ainv symmetric, rcs = rc[:, c] in LDS);
 // returns sum_I rcs[I] ec[I][c] in thread 0 (the workgroup's threads all call it).
+// With emax, thread 0 also gets max_I |ec[I][c]| there.
 __device__ __forceinline__ double coarse_column(int na, int c, int64_t ldc, const double* __restrict__ Ainv,
-                                                const double* rcs, float* __restrict__ ec, double* part) {
+                                                const double* rcs, float* __restrict__ ec, double* part,
+                                                float* partm = nullptr, float* emax = nullptr) {
   const int I = threadIdx.x;
   double v = 0.0;
+  float m = 0.f;
   if (I < na) {
     double e = 0.0;
 #pragma unroll 8
     for (int J = 0; J < na; ++J) e += Ainv[(int64_t)J * na + I] * rcs[J];
-    ec[(int64_t)I * ldc + c] = (float)e;
+    const float ef = (float)e;
+    ec[(int64_t)I * ldc + c] = ef;
     v = rcs[I] * e;
+    m = fabsf(ef);
   }
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  if (emax) {
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) partm[threadIdx.x >> 6] = m;
+  }
   __syncthreads();
   double t = 0.0;
-  if (threadIdx.x == 0)
+  if (threadIdx.x == 0) {
     for (int k = 0; k < ((int)blockDim.x >> 6); ++k) t += part[k];
+    if (emax) {
+      float mm = 0.f;
+      for (int k = 0; k < ((int)blockDim.x >> 6); ++k) mm = fmaxf(mm, partm[k]);
+      *emax = mm;
+    }
+  }
   return t;
 }
 
 // ec[:, c] = Ainv rc[:, c] (one workgroup of kCoarseMax threads per column c < C, thread I =
 // aggregate, rc[:, c] staged in LDS, then cleared), and rz[c] += rc[:, c] . ec[:, c] (the
-// coarse part of r^T M r) -- the start's correction.
+// coarse part of r^T M r) -- the start's correction; with emax also emax[c] = max_I |ec[I][c]|.
 __global__ __launch_bounds__(kCoarseMax) void k_coarse_apply(int na, int C, int64_t ldc,
                                                              const double* __restrict__ Ainv,
                                                              double* __restrict__ rc, float* __restrict__ ec,
-                                                             double* __restrict__ rz, const int32_t* __restrict__ ctl) {
+                                                             double* __restrict__ rz, const int32_t* __restrict__ ctl,
+                                                             float* __restrict__ emax) {
   if (ctl && ctl[0]) return;
   __shared__ double rcs[kCoarseMax];
   __shared__ double part[kCoarseMax / 64];
+  __shared__ float partm[kCoarseMax / 64];
   const int c = blockIdx.x, I = threadIdx.x;
   rcs[I] = I < na ? rc[(int64_t)I * ldc + c] : 0.0;
   __syncthreads();
   if (I < na) rc[(int64_t)I * ldc + c] = 0.0;   // ready for the next restriction
-  const double v = coarse_column(na, c, ldc, Ainv, rcs, ec, part);
+  const double v = coarse_column(na, c, ldc, Ainv, rcs, ec, part, partm, emax ? emax + c : nullptr);
   if (threadIdx.x == 0) atomicAdd(&rz[c], v);
 }
 
 // One iteration's scalars with the coarse correction (one workgroup of kCoarseMax threads per
 // column c): ec[:, c] = Ainv rc[:, c]; rz_c = rz_new[c] + rc . ec (r^T M r); beta = rz_c/rz[c]
-// (with FP16 directions the new column scale and beta rescaled to the held direction);
+// (with FP16 or FP8 directions the new column scale and beta rescaled to the held direction;
+// FP8 scales come from the maxima rmax and pmax and the correction's own maximum);
 // rz[c] = rz_c; rz_new, pAp, rp and rc of the column cleared for the next iteration.
 // Workgroup 0 counts the iteration and sets the done flag when viol <= 1, then moves viol to
 // viol[1] (read by the host) and clears it.
@@ -723,21 +832,24 @@ __global__ __launch_bounds__(kCoarseMax) void k_coarse_scalars(int na, int C, in
                                                                double* __restrict__ rz, double* __restrict__ rz_new,
                                                                double* __restrict__ pAp, double* __restrict__ rp,
                                                                double* __restrict__ beta, float* __restrict__ sig,
-                                                               float* __restrict__ viol, int32_t* ctl) {
+                                                               float* __restrict__ viol, int32_t* ctl,
+                                                               float* __restrict__ rmax, float* __restrict__ pmax) {
   if (ctl && ctl[0]) return;
   __shared__ double rcs[kCoarseMax];
   __shared__ double part[kCoarseMax / 64];
+  __shared__ float partm[kCoarseMax / 64];
   const int c = blockIdx.x, I = threadIdx.x;
   rcs[I] = I < na ? rc[(int64_t)I * ldc + c] : 0.0;
   __syncthreads();
   if (I < na) rc[(int64_t)I * ldc + c] = 0.0;
-  const double v = coarse_column(na, c, ldc, Ainv, rcs, ec, part);
+  float em = 0.f;
+  const double v = coarse_column(na, c, ldc, Ainv, rcs, ec, part, partm, rmax ? &em : nullptr);
   if (threadIdx.x == 0) {
     const double rzc = rz_new[c] + v;
     const double r = rz[c];
     double bt = r > 0.0 ? rzc / r : 0.0;
     if (sig) {
-      const float sn = pcg_scale(rzc);
+      const float sn = rmax ? f8_next_scale(c, bt, em, rmax, pmax, sig) : pcg_scale(rzc);
       bt *= (double)sn / (double)sig[c];
       sig[c] = sn;
     }
@@ -961,6 +1073,12 @@ int pcg_flat_chunk(const PcgBufs& b) {
   return (int)(((int64_t)b.n + f.grid - 1) / f.grid);
 }
 
+// Lane words per row of the low-precision direction block: 8-byte groups for BF16/FP16, 4-byte
+// groups for FP8.
+static int64_t pwords(const PcgBufs& b) {
+  return !b.P16 ? 0 : b.p16 == 3 ? b.ldh / 2 : b.ldh / 4;
+}
+
 static CoarseArg coarse_arg(const PcgBufs& b) {
   if (!b.na) return CoarseArg{0, 0, nullptr, nullptr, nullptr, nullptr, nullptr};
   return CoarseArg{b.na, b.ldc, b.rc, b.ec, b.solved, b.bounds, b.aggmap};
@@ -987,14 +1105,20 @@ int pcg_start(const PcgBufs& b, hipStream_t st) {
   if (b.ctl) hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, 1, (double*)b.ctl);
   const FlatLaunch f = flat_launch(b);
   DLP_CHECK_ARG(f.ok || !b.P16, "pcg: 16-bit directions need 16-byte FP32 rows");
-  DLP_CHECK_ARG(!b.P16 || b.p16 == 1 || (b.p16 == 2 && b.sig), "pcg: 16-bit direction kind %d", b.p16);
+  DLP_CHECK_ARG(!b.P16 || b.p16 == 1 || (b.p16 == 2 && b.sig) || (b.p16 == 3 && b.sig && b.rmax && b.pmax && b.emax),
+                "pcg: low-precision direction kind %d", b.p16);
   DLP_CHECK_ARG(f.ok || !b.na, "pcg: the coarse correction needs 16-byte FP32 rows");
   int rc = check_coarse(b, f);
   if (rc) return rc;
   if (f.ok) {
     const CoarseArg K = coarse_arg(b);
-    float* sig = (b.P16 && b.p16 == 2) ? b.sig : nullptr;
+    float* sig = (b.P16 && b.p16 >= 2) ? b.sig : nullptr;
+    const bool f8 = b.P16 && b.p16 == 3;
     if (b.P16) hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, b.C, b.rp);
+    if (f8) {   // the maxima accumulate from zero (C floats each fit in C/2 doubles)
+      for (float* m : {b.rmax, b.pmax, b.emax})
+        hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, (b.C + 1) / 2, reinterpret_cast<double*>(m));
+    }
     // step lengths of the deferred updates (held as doubles' worth of float pairs)
     if (b.P16 && b.slot_stride > 0 && b.alpha_hist)
       hipLaunchKernelGGL(k_zero_f64, dim3(grid_for((b.group * b.C + 1) / 2)), dim3(kBlock), 0, st,
@@ -1007,22 +1131,26 @@ int pcg_start(const PcgBufs& b, hipStream_t st) {
         hipLaunchKernelGGL(k_zero_f64, dim3(grid_for(nrc)), dim3(kBlock), 0, st, nrc, b.rc);
       }
       hipLaunchKernelGGL(k_pcg_rz4, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (const float4*)b.R, b.diag,
-                         b.rz, K);
+                         b.rz, K, f8 ? b.rmax : nullptr);
       if (b.na)
         hipLaunchKernelGGL(k_coarse_apply, dim3(b.C), dim3(kCoarseMax), 0, st, b.na, b.C, b.ldc, b.Ainv, b.rc, b.ec,
-                           b.rz, nullptr);
+                           b.rz, nullptr, f8 ? b.emax : nullptr);
       hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 2, b.rz, b.rz_new, b.pAp, b.alpha,
-                         b.beta, b.viol, nullptr, nullptr, sig, nullptr, 0);
+                         b.beta, b.viol, nullptr, nullptr, sig, nullptr, 0, f8 ? b.rmax : nullptr,
+                         f8 ? b.pmax : nullptr, f8 ? b.emax : nullptr);
     }
     if (!b.P16)
       hipLaunchKernelGGL(k_pcg_start4<0>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
-                         PBlock<0>{b.P, 0}, (const float4*)b.R, b.diag, b.rz, nullptr, nullptr, K);
+                         PBlock<0>{b.P, 0}, (const float4*)b.R, b.diag, b.rz, nullptr, nullptr, K, nullptr);
+    else if (b.p16 == 3)
+      hipLaunchKernelGGL(k_pcg_start4<3>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
+                         PBlock<3>{b.P16, pwords(b)}, (const float4*)b.R, b.diag, b.rz, b.rp, b.sig, K, b.pmax);
     else if (b.p16 == 2)
       hipLaunchKernelGGL(k_pcg_start4<2>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
-                         PBlock<2>{b.P16, b.ldh / 4}, (const float4*)b.R, b.diag, b.rz, b.rp, b.sig, K);
+                         PBlock<2>{b.P16, pwords(b)}, (const float4*)b.R, b.diag, b.rz, b.rp, b.sig, K, nullptr);
     else
       hipLaunchKernelGGL(k_pcg_start4<1>, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,
-                         PBlock<1>{b.P16, b.ldh / 4}, (const float4*)b.R, b.diag, b.rz, b.rp, nullptr, K);
+                         PBlock<1>{b.P16, pwords(b)}, (const float4*)b.R, b.diag, b.rz, b.rp, nullptr, K, nullptr);
     DLP_CHECK_LAUNCH();
     return DLP_OK;
   }
@@ -1104,7 +1232,10 @@ int pcg_finish_iteration(const PcgBufs& b, hipStream_t st, bool have_pap) {
     if (rc) return rc;
     const CoarseArg K = coarse_arg(b);
     double* rp = b.P16 ? b.rp : nullptr;
-    float* sig = (b.P16 && b.p16 == 2) ? b.sig : nullptr;
+    float* sig = (b.P16 && b.p16 >= 2) ? b.sig : nullptr;
+    const bool f8 = b.P16 && b.p16 == 3;
+    float* rmax = f8 ? b.rmax : nullptr;
+    float* pmax = f8 ? b.pmax : nullptr;
     // the update pass takes its step lengths from pAp and r.p (16-bit directions) or r.z
     const double* num = b.P16 ? b.rp : b.rz;
     const bool nt = tuning().nt_streams != 0;
@@ -1118,8 +1249,8 @@ int pcg_finish_iteration(const PcgBufs& b, hipStream_t st, bool have_pap) {
     float* aout = defer ? b.alpha_hist + (int64_t)b.slot * b.C : nullptr;
 #define DLP_UPDATE_NT(PKV, PB, NTV, DV)                                                                      \
   hipLaunchKernelGGL((k_pcg_update4<PKV, NTV, DV>), dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,     \
-                     (float4*)b.Z, (float4*)b.R, PBlock<PKV>{PB, b.P16 ? b.ldh / 4 : 0}, (const float4*)b.AP,  \
-                     b.pAp, num, b.diag, b.eps, b.rz_new, b.viol, b.ctl, K, aout)
+                     (float4*)b.Z, (float4*)b.R, PBlock<PKV>{PB, pwords(b)}, (const float4*)b.AP,            \
+                     b.pAp, num, b.diag, b.eps, b.rz_new, b.viol, b.ctl, K, aout, rmax)
 #define DLP_UPDATE(PKV, PB)                                   \
   if (defer && nt) DLP_UPDATE_NT(PKV, PB, true, true);       \
   else if (defer) DLP_UPDATE_NT(PKV, PB, false, true);       \
@@ -1127,23 +1258,25 @@ int pcg_finish_iteration(const PcgBufs& b, hipStream_t st, bool have_pap) {
   else DLP_UPDATE_NT(PKV, PB, false, false)
 #define DLP_DIR_NT(PKV, PB, PO, NTV)                                                                         \
   hipLaunchKernelGGL((k_pcg_dir4<PKV, NTV>), dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C,           \
-                     PBlock<PKV>{PB, b.P16 ? b.ldh / 4 : 0}, PBlock<PKV>{PO, b.P16 ? b.ldh / 4 : 0},           \
-                     (const float4*)b.R, b.diag, b.beta, rp, b.ctl, sig, K)
+                     PBlock<PKV>{PB, pwords(b)}, PBlock<PKV>{PO, pwords(b)},                                 \
+                     (const float4*)b.R, b.diag, b.beta, rp, b.ctl, sig, K, pmax)
 #define DLP_DIR(PKV, PB, PO)              \
   if (nt) DLP_DIR_NT(PKV, PB, PO, true); \
   else DLP_DIR_NT(PKV, PB, PO, false)
     if (!b.P16) { DLP_UPDATE(0, b.P); }
+    else if (b.p16 == 3) { DLP_UPDATE(3, Pcur); }
     else if (b.p16 == 2) { DLP_UPDATE(2, Pcur); }
     else { DLP_UPDATE(1, Pcur); }
     // beta (from r^T M r: the update's r^T D^-1 r plus the coarse part), the convergence test,
     // and the coarse correction the direction uses
     if (b.na)
       hipLaunchKernelGGL(k_coarse_scalars, dim3(b.C), dim3(kCoarseMax), 0, st, b.na, b.C, b.ldc, b.Ainv, b.rc, b.ec,
-                         b.rz, b.rz_new, b.pAp, rp, b.beta, sig, b.viol, b.ctl);
+                         b.rz, b.rz_new, b.pAp, rp, b.beta, sig, b.viol, b.ctl, rmax, pmax);
     else
       hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 1, b.rz, b.rz_new, b.pAp, b.alpha,
-                         b.beta, b.viol, b.ctl, rp, sig, nullptr, 0);
+                         b.beta, b.viol, b.ctl, rp, sig, nullptr, 0, rmax, pmax, nullptr);
     if (!b.P16) { DLP_DIR(0, b.P, b.P); }
+    else if (b.p16 == 3) { DLP_DIR(3, Pcur, Pnext); }
     else if (b.p16 == 2) { DLP_DIR(2, Pcur, Pnext); }
     else { DLP_DIR(1, Pcur, Pnext); }
 #undef DLP_UPDATE
@@ -1159,11 +1292,11 @@ int pcg_finish_iteration(const PcgBufs& b, hipStream_t st, bool have_pap) {
     hipLaunchKernelGGL((k_coldot<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.P, b.AP, \
                        b.ld, b.pAp);                                                                 \
   hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 0, b.rz, b.rz_new, b.pAp,        \
-                     b.alpha, b.beta, b.viol, b.ctl, nullptr, nullptr, nullptr, 0);                  \
+                     b.alpha, b.beta, b.viol, b.ctl, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr); \
   hipLaunchKernelGGL((k_pcg_update<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.Z, b.R, \
                      b.P, b.AP, b.ld, b.alpha, b.diag, b.eps, b.rz_new, b.viol, b.ctl);              \
   hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(kBlock), 0, st, b.C, 1, b.rz, b.rz_new, b.pAp,        \
-                     b.alpha, b.beta, b.viol, b.ctl, nullptr, nullptr, nullptr, 0);                  \
+                     b.alpha, b.beta, b.viol, b.ctl, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr); \
   hipLaunchKernelGGL((k_pcg_dir<CPLV>), dim3(g), dim3(kBlock), 0, st, b.n, b.C, m.lc_shift, b.P, b.R,   \
                      b.ld, b.diag, b.beta, 0, b.ctl);
   DLP_DISPATCH_CPL(m.cpl, K)
@@ -1180,8 +1313,11 @@ int pcg_flush(const PcgBufs& b, hipStream_t st) {
   const bool nt = tuning().nt_streams != 0;
 #define DLP_FLUSH(PKV, NTV)                                                                                 \
   hipLaunchKernelGGL((k_pcg_flush4<PKV, NTV>), dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (float4*)b.Z, \
-                     PBlock<PKV>{b.P16, b.ldh / 4}, b.slot_stride, b.group, b.alpha_hist, b.ctl)
-  if (b.p16 == 2) {
+                     PBlock<PKV>{b.P16, pwords(b)}, b.slot_stride, b.group, b.alpha_hist, b.ctl)
+  if (b.p16 == 3) {
+    if (nt) DLP_FLUSH(3, true);
+    else DLP_FLUSH(3, false);
+  } else if (b.p16 == 2) {
     if (nt) DLP_FLUSH(2, true);
     else DLP_FLUSH(2, false);
   } else {

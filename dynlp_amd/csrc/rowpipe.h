@@ -48,10 +48,26 @@ __device__ __forceinline__ unsigned f16_pack(float lo, float hi) {
   return __builtin_bit_cast(unsigned, h);
 }
 
+// FP8 E4M3 (OCP: bias 7, largest finite value 448, no infinity), four values per 32-bit word,
+// first value in the low byte.  The hardware conversion rounds to nearest even and turns
+// magnitudes past 448 into NaN, so values saturate at +-448 first.
+__device__ __forceinline__ unsigned f8_pack4(float a, float b, float c, float d) {
+  const float m = 448.f;
+  int w = __builtin_amdgcn_cvt_pk_fp8_f32(fminf(fmaxf(a, -m), m), fminf(fmaxf(b, -m), m), 0, false);
+  w = __builtin_amdgcn_cvt_pk_fp8_f32(fminf(fmaxf(c, -m), m), fminf(fmaxf(d, -m), m), w, true);
+  return (unsigned)w;
+}
+__device__ __forceinline__ float4 f8_unpack4(unsigned u) {
+  const auto lo = __builtin_amdgcn_cvt_pk_f32_fp8((int)u, false);
+  const auto hi = __builtin_amdgcn_cvt_pk_f32_fp8((int)u, true);
+  return make_float4(lo[0], lo[1], hi[0], hi[1]);
+}
+
 // Lane words of the gathered rows: FP32 x 4 (16 bytes), BF16 x 8 (16 bytes), BF16 x 4 or
-// FP16 x 4 (8 bytes), and FP32 x 4 at 4-byte alignment (F32x4u: four dword loads, for
-// contiguous rows whose length is not a multiple of 4 columns).  W values per lane, column order.
-enum class Word { F32x4, BF16x8, BF16x4, F16x4, F32x4u };
+// FP16 x 4 (8 bytes), FP32 x 4 at 4-byte alignment (F32x4u: four dword loads, for contiguous
+// rows whose length is not a multiple of 4 columns), and FP8 (E4M3) x 4 (4 bytes, converted by
+// the hardware's packed FP8 -> FP32 conversion).  W values per lane, column order.
+enum class Word { F32x4, BF16x8, BF16x4, F16x4, F32x4u, F8x4 };
 
 template <Word K> struct WordT;
 template <> struct WordT<Word::F32x4> {
@@ -74,6 +90,10 @@ template <> struct WordT<Word::F32x4u> {
   static constexpr int W = 4, bytes = 16;
   using raw = u4v;
 };
+template <> struct WordT<Word::F8x4> {
+  static constexpr int W = 4, bytes = 4;
+  using raw = unsigned int;
+};
 
 template <Word K>
 __device__ __forceinline__ typename WordT<K>::raw load_word(__amdgpu_buffer_rsrc_t rs, unsigned off) {
@@ -87,6 +107,8 @@ __device__ __forceinline__ typename WordT<K>::raw load_word(__amdgpu_buffer_rsrc
     return r;
   } else if constexpr (WordT<K>::bytes == 16) {
     return __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0);
+  } else if constexpr (WordT<K>::bytes == 4) {
+    return __builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, 0);
   } else {
     return __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, 0);
   }
@@ -117,6 +139,12 @@ __device__ __forceinline__ void unpack(const typename WordT<K>::raw& u, float (&
     x[1] = bf16_hi(u.x);
     x[2] = bf16_lo(u.y);
     x[3] = bf16_hi(u.y);
+  } else if constexpr (K == Word::F8x4) {
+    const float4 v = f8_unpack4(u);
+    x[0] = v.x;
+    x[1] = v.y;
+    x[2] = v.z;
+    x[3] = v.w;
   } else {
     x[0] = f16_lo(u.x);
     x[1] = f16_hi(u.x);

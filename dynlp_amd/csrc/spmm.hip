@@ -363,7 +363,8 @@ int row_operator_dot(int n, int C, const int32_t* indptr, const int32_t* rowlen,
 
 // The global solve's operator (k_pcg_operator): rowmask = the solved set, rowlen required,
 // 16-byte rows, C <= 256.  X is FP32 (ldx floats per row) or, with X16, BF16 (x16_kind 1) or
-// FP16 (x16_kind 2), ldh values per row, a multiple of 8.  The FP32 form falls back to row_operator_dot when tuning disables the
+// FP16 (x16_kind 2), ldh values per row, a multiple of 8, or FP8 E4M3 (x16_kind 3), rows of
+// 2*ldh bytes (ldh even: 4-byte lane words of 4 columns).  The FP32 form falls back to row_operator_dot when tuning disables the
 // pipelined kernel (key 6 = 0) or the block exceeds its 32-bit offsets.
 int pcg_operator(int n, int C, const int32_t* indptr, const int32_t* rowlen, const int32_t* indices, const float* w,
                  const uint8_t* rowmask, const float* X, int64_t ldx, const uint16_t* X16, int64_t ldh, int x16_kind,
@@ -371,20 +372,23 @@ int pcg_operator(int n, int C, const int32_t* indptr, const int32_t* rowlen, con
   const Tuning& T = tuning();
   const bool bf = X16 != nullptr;
   const bool h16 = bf && x16_kind == 2;
-  DLP_CHECK_ARG(!bf || x16_kind == 1 || x16_kind == 2, "pcg_operator: 16-bit kind %d", x16_kind);
+  const bool f8 = bf && x16_kind == 3;
+  DLP_CHECK_ARG(!bf || (x16_kind >= 1 && x16_kind <= 3), "pcg_operator: low-precision kind %d", x16_kind);
   const long long xbytes = bf ? (long long)n * ldh * 2 : (long long)n * ldx * 4;
   const bool fits = rowlen && rowmask && a && n < (1 << 24) && xbytes < (1LL << 31);
   if (!bf && (!T.pipe || !fits))
     return row_operator_dot(n, C, indptr, rowlen, indices, w, rowmask, X, ldx, a, b, out, ldo, dot, st, skip);
   DLP_CHECK_ARG(fits && (ldo % 4) == 0 && (((uintptr_t)out) & 15) == 0 && C <= 256 &&
-                    (bf ? ((ldh % 8) == 0 && (((uintptr_t)X16) & 15) == 0)
-                        : ((ldx % 4) == 0 && (((uintptr_t)X) & 15) == 0)),
+                    (f8 ? ((ldh % 2) == 0 && ldh * 2 >= 4 * ((C + 3) / 4) && (((uintptr_t)X16) & 15) == 0)
+                     : bf ? ((ldh % 8) == 0 && (((uintptr_t)X16) & 15) == 0)
+                          : ((ldx % 4) == 0 && (((uintptr_t)X) & 15) == 0)),
                 "pcg_operator: layout");
   if (n <= 0) return DLP_OK;
   const int S = T.pipe == 8 ? 8 : T.pipe == 6 ? 6 : 4;
   const bool w8 = !h16 && T.bf16_word == 8;
 #define DLP_PIPE_FN(SV, KV) reinterpret_cast<const void*>(&k_pcg_operator<SV, KV>)
-  const void* kfn = h16 ? (S == 8 ? DLP_PIPE_FN(8, Word::F16x4)
+  const void* kfn = f8 ? DLP_PIPE_FN(4, Word::F8x4)
+                  : h16 ? (S == 8 ? DLP_PIPE_FN(8, Word::F16x4)
                            : S == 6 ? DLP_PIPE_FN(6, Word::F16x4) : DLP_PIPE_FN(4, Word::F16x4))
                   : bf ? (w8 ? DLP_PIPE_FN(4, Word::BF16x8)
                              : S == 8 ? DLP_PIPE_FN(8, Word::BF16x4)
@@ -402,7 +406,8 @@ int pcg_operator(int n, int C, const int32_t* indptr, const int32_t* rowlen, con
 #define DLP_PIPE(SV, KV)                                                                                      \
   hipLaunchKernelGGL((k_pcg_operator<SV, KV>), dim3((unsigned)grid), dim3(kBlock), 0, st, n, C, indptr, rowlen, \
                      indices, w, rowmask, Xp, rb, (int)xbytes, a, b, out, ldo, dot, xcd, skip)
-  if (h16 && S == 8) DLP_PIPE(8, Word::F16x4);
+  if (f8) DLP_PIPE(4, Word::F8x4);
+  else if (h16 && S == 8) DLP_PIPE(8, Word::F16x4);
   else if (h16 && S == 6) DLP_PIPE(6, Word::F16x4);
   else if (h16) DLP_PIPE(4, Word::F16x4);
   else if (bf && w8) DLP_PIPE(4, Word::BF16x8);

@@ -1266,14 +1266,16 @@ struct LP {
   // global solve (column-wise PCG on the FP32 state)
   float* P = nullptr;
   float* AP = nullptr;
-  uint16_t* P16 = nullptr;      // 16-bit directions of the column PCG (pcg_p16 != 0)
-  int64_t ldh = 0;              // its row length: C1 rounded up to 8
+  uint16_t* P16 = nullptr;      // low-precision directions of the column PCG (pcg_p16 != 0)
+  int64_t ldh = 0;              // its row length in 16-bit units (direction_ldh)
   int p16_slots = 0;            // n x ldh blocks allocated in P16
   int pcg_group = 0;            // iterations per deferred solution update (0: every iteration)
   float* pcg_ahist = nullptr;   // their step lengths, pcg_group x C1 (rounded up to even)
   int ahist_len = 0;
-  int pcg_p16 = 0;              // 0 FP32 directions, 1 BF16, 2 FP16 with a per-column scale
-  double* pcg_scal = nullptr;   // rz, rz_new, pAp, alpha, beta, rp (C1 doubles each), sig (C1 floats)
+  int pcg_p16 = 0;              // 0 FP32 directions, 1 BF16, 2 FP16 / 3 FP8 with a per-column scale
+  // rz, rz_new, pAp, alpha, beta, rp (C1 doubles each), then sig, rmax, pmax, emax (C1 floats
+  // each, in slots of C1 doubles)
+  double* pcg_scal = nullptr;
   // two-level preconditioner of the column PCG: coarse_na aggregates requested (0: Jacobi
   // only), rebuilt every coarse_refresh global solves; cg_* = the current layout and buffers
   int coarse_na = 0, coarse_refresh = 64;
@@ -2482,6 +2484,9 @@ static PcgBufs pcg_bufs(LP* h) {
   b.rp = h->pcg_scal + 5 * C1;
   b.p16 = h->pcg_p16;
   b.sig = reinterpret_cast<float*>(h->pcg_scal + 6 * C1);
+  b.rmax = reinterpret_cast<float*>(h->pcg_scal + 7 * C1);
+  b.pmax = reinterpret_cast<float*>(h->pcg_scal + 8 * C1);
+  b.emax = reinterpret_cast<float*>(h->pcg_scal + 9 * C1);
   if (b.P16 && h->pcg_group > 0) {
     b.slot_stride = (int64_t)h->d.n * h->ldh;
     b.group = h->pcg_group;
@@ -2669,6 +2674,16 @@ static int pcg_group_for(const LP* h, int iters_per_launch) {
              ? iters_per_launch : 0;
 }
 
+// Slots of C1 doubles in pcg_scal.
+constexpr int kScalSlots = 10;
+
+// Row length (16-bit units) of the low-precision direction block for leading dimension ld: BF16
+// and FP16 rows of 8-value groups covering ld columns, FP8 rows of exactly ld bytes (the flat
+// passes address column quad q < ld/4 of every row).
+static int64_t direction_ldh(int kind, int64_t ld) {
+  return kind == 3 ? ld / 2 : (ld + 7) / 8 * 8;
+}
+
 // Buffers of the coarse correction (sized for kCoarseMax aggregates), whole or not at all.
 static int coarse_alloc(LP* h, hipStream_t st) {
   if (h->cg_Ac) return DLP_OK;
@@ -2704,7 +2719,7 @@ static int pcg_reserve(LP* h, int iters_per_launch, bool block, hipStream_t st) 
     AllocGroup g;
     g.add((void**)&h->P, nz * sizeof(float));
     g.add((void**)&h->AP, nz * sizeof(float));
-    g.add((void**)&h->pcg_scal, 7 * C1 * sizeof(double));
+    g.add((void**)&h->pcg_scal, kScalSlots * C1 * sizeof(double));
     g.add((void**)&h->pcg_viol, 4 * sizeof(float));   // viol, last viol, done, iterations
     int rc = g.commit(&h->dev_bytes, "global solve");
     if (rc) return rc;
@@ -2722,8 +2737,8 @@ static int pcg_reserve(LP* h, int iters_per_launch, bool block, hipStream_t st) 
   // deferred solution updates keep a group's directions: iters_per_launch + 1 slots
   const int group = pcg_group_for(h, iters_per_launch);
   const int slots = group > 0 ? group + 1 : 1;
-  if (h->pcg_p16 && (!h->P16 || h->p16_slots < slots)) {
-    const int64_t ldh = (C1 + 7) / 8 * 8;
+  const int64_t ldh = direction_ldh(h->pcg_p16, h->d.ld);
+  if (h->pcg_p16 && (!h->P16 || h->p16_slots < slots || h->ldh != ldh)) {
     const size_t nh = (size_t)h->d.n * ldh * slots;
     uint16_t* fresh = nullptr;
     AllocGroup g;
@@ -2883,7 +2898,7 @@ extern "C" int dlp_lp_set_coarse(void* handle, int na, int refresh) {
 
 extern "C" int dlp_lp_set_pcg_directions(void* handle, int kind) {
   DLP_CHECK_ARG(handle, "lp_set_pcg_directions: null handle");
-  DLP_CHECK_ARG(kind >= 0 && kind <= 2, "lp_set_pcg_directions: kind %d (0 FP32, 1 BF16, 2 FP16)", kind);
+  DLP_CHECK_ARG(kind >= 0 && kind <= 3, "lp_set_pcg_directions: kind %d (0 FP32, 1 BF16, 2 FP16, 3 FP8)", kind);
   LP* h = (LP*)handle;
   DLP_CHECK_ARG(!kind || ((h->d.ld % 4) == 0 && h->d.C1 <= 256), "lp_set_pcg_directions: needs 16-byte FP32 rows");
   if (h->pcg_p16 != kind && h->pcg_exec) {   // captured iterations use the other layout
@@ -2925,12 +2940,12 @@ extern "C" int64_t dlp_lp_handle_bytes(int n, int C1, int64_t ld, int fp64, int 
   b += (int64_t)dlp_scan_workspace_bytes(n) + 8 * kStatWords;
   if (fp64) return b;   // the plain FP64 path repairs by push only: no global-solve buffers
   // global solve
-  b += 2 * 4 * nz + 7 * 8 * C1 + 16;
+  b += 2 * 4 * nz + kScalSlots * 8 * C1 + 16;
   if (block) return b + 5 * 8 * (int64_t)C1 * C1;
   const bool p16 = dirs != 0;
   const int group =
       (p16 && tuning().defer_z && (ld % 4) == 0 && iters_per_launch <= kMaxGroup) ? iters_per_launch : 0;
-  if (p16) b += 2 * N * ((C1 + 7) / 8 * 8) * (group > 0 ? group + 1 : 1);
+  if (p16) b += 2 * N * direction_ldh(dirs, ld) * (group > 0 ? group + 1 : 1);
   if (group > 0) b += 4 * ((group * C1 + 2) & ~1);
   if (coarse > 0 && n >= 64)
     b += 2 * 8 * kCoarseMax * kCoarseMax + (8 + 4) * kCoarseMax * ld + 4 * (kCoarseMax + 1) + N + 8 * N +

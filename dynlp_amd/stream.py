@@ -27,6 +27,8 @@ from .graphs import EdgeBatch, Graph, check_batch, normalize_batch, pack_ops
 from .solvers import System  # noqa: F401  (the batch system, reference stream.py:17-58)
 
 _OPS_PER_CALL = 1 << 19     # operations one dlp_lp_apply_ops call takes (its pair table's capacity / 2)
+# storage of the global solve's search directions -> dlp_lp_set_pcg_directions kind
+DIRECTION_KINDS = {"fp32": 0, "bf16": 1, "fp16": 2, "fp8": 3}
 
 
 @dataclass
@@ -74,7 +76,8 @@ class EdgeStreamLP:
     Z/R in FP64 and repairs with the host-driven push rounds of the reference.  pcg_directions:
     storage of the global solve's search directions, "fp16" (default: half the gather bytes, a
     power-of-two scale per column, 11 significant bits, step lengths exact for the rounded
-    direction), "bf16" (half the bytes, 8 significant bits) or "fp32".  pcg_coarse: aggregates of
+    direction), "fp8" (E4M3: a quarter of the bytes, 4 significant bits, the scale from a bound
+    on each column's largest value), "bf16" (half the bytes, 8 significant bits) or "fp32".  pcg_coarse: aggregates of
     the two-level preconditioner of the global solve (runs of consecutive rows; 0 = Jacobi only),
     its coarse operator rebuilt from the current equations every coarse_refresh global solves.
     pcg_iters_per_launch: global-solve iterations per replayed graph (iterations past
@@ -140,9 +143,9 @@ class EdgeStreamLP:
         _lib.call("dlp_lp_create", ct.byref(self._desc), ct.byref(h))
         self.h = h
         self._set_eps()
-        kinds = {"fp32": 0, "bf16": 1, "fp16": 2}
+        kinds = DIRECTION_KINDS
         if pcg_directions not in kinds:
-            raise ValueError(f"pcg_directions {pcg_directions!r}: 'fp32', 'bf16' or 'fp16'")
+            raise ValueError(f"pcg_directions {pcg_directions!r}: 'fp32', 'bf16', 'fp16' or 'fp8'")
         self.pcg_directions = pcg_directions
         _lib.call("dlp_lp_set_pcg_directions", self.h, 0 if self.fp64 else kinds[pcg_directions])
         self.pcg_coarse = int(pcg_coarse)
@@ -177,7 +180,7 @@ class EdgeStreamLP:
         b += 4 * (1 << 16)                                 # batch staging
         if fp64:
             b += 2 * 8 * n * ld + 4 * n + 4 + 8 * C1       # Z64, R64, candidates, rho
-        kinds = {"fp32": 0, "bf16": 1, "fp16": 2}
+        kinds = DIRECTION_KINDS
         hb = _lib.load().dlp_lp_handle_bytes(n, C1, ld, int(fp64), kinds[pcg_directions], int(pcg_iters_per_launch),
                                               int(pcg_coarse), int(pcg_mode == "block"))
         if hb < 0:

@@ -23,7 +23,7 @@ def test_spmm_fixture():
 
 
 @pytest.mark.parametrize("name", ["sys_er_k2", "sys_er_k16", "sys_sbm_k64"])
-@pytest.mark.parametrize("dirs", ["fp32", "bf16", "fp16"])
+@pytest.mark.parametrize("dirs", ["fp32", "bf16", "fp16", "fp8"])
 def test_system_fixture_initial_solve(name, dirs):
     """Initial solve of the stored graph: solved set exact, diag to 1e-12, every label column
     within the certificate rho * hmax of the stored F*."""

@@ -52,7 +52,8 @@ def check_against_oracle(lp, g, es, eps, label, iterative=False):
     return err, rc[: sysm.C].max()
 
 
-@pytest.mark.parametrize("fp64,dirs", [(False, "fp32"), (True, "fp32"), (False, "bf16"), (False, "fp16")])
+@pytest.mark.parametrize("fp64,dirs", [(False, "fp32"), (True, "fp32"), (False, "bf16"), (False, "fp16"),
+                                      (False, "fp8")])
 @pytest.mark.parametrize("K", [2, 8])
 def test_stream_matches_rebuilt_system(fp64, dirs, K):
     from dynlp_amd.stream import EdgeStreamLP
@@ -103,11 +104,11 @@ def test_stream_lane_layouts_and_row_orders(pack, xcd, vec4, pipe):
         _lib.call("dlp_tune_set", 3, 1)
 
 
-@pytest.mark.parametrize("dirs,word", [("fp32", 4), ("bf16", 4), ("bf16", 8), ("fp16", 4)])
+@pytest.mark.parametrize("dirs,word", [("fp32", 4), ("bf16", 4), ("bf16", 8), ("fp16", 4), ("fp8", 4)])
 def test_global_solve_from_zero_certifies(dirs, word):
     """A global solve from a zero state (every row violating) under FP32, BF16 (8- and 16-byte
-    lane words) and scaled FP16 directions: certified against the rebuilt oracle, on a
-    fragmenting graph with inert rows."""
+    lane words), scaled FP16 and scaled FP8 directions: certified against the rebuilt oracle, on
+    a fragmenting graph with inert rows."""
     from dynlp_amd import _lib
     from dynlp_amd.stream import EdgeStreamLP
     _lib.call("dlp_tune_set", 7, word)
@@ -128,7 +129,7 @@ def test_global_solve_from_zero_certifies(dirs, word):
     check_against_oracle(lp, g, es, 1e-6, f"{dirs}/{word}")
 
 
-@pytest.mark.parametrize("dirs", ["fp32", "bf16", "fp16"])
+@pytest.mark.parametrize("dirs", ["fp32", "bf16", "fp16", "fp8"])
 def test_two_level_preconditioner_certifies_and_cuts_iterations(dirs):
     """The coarse correction (runs of consecutive rows as aggregates, Ac = P^T A P factored and
     inverted on the device) certifies against the rebuilt oracle like the Jacobi-preconditioned
@@ -220,7 +221,7 @@ def test_pcg_group_length_does_not_change_the_solve():
     np.testing.assert_allclose(Z1[:, :C], Z7[:, :C], rtol=0, atol=2.1e-6 * hmax)
 
 
-@pytest.mark.parametrize("dirs", ["fp16", "bf16"])
+@pytest.mark.parametrize("dirs", ["fp16", "bf16", "fp8"])
 @pytest.mark.parametrize("per", [1, 4, 8])
 def test_deferred_solution_updates_match_per_iteration_updates(dirs, per):
     """Deferred solution updates (Z += sum_s alpha_s P_s once per captured group of iterations,
@@ -618,7 +619,7 @@ def test_device_apply_refuses_bad_pairs_and_keeps_last_occurrence():
 
 
 @pytest.mark.parametrize("kw", [dict(), dict(pcg_directions="fp32", pcg_coarse=0), dict(fp64=True),
-                                dict(pcg_mode="block"), dict(pcg_iters_per_launch=3)])
+                                dict(pcg_mode="block"), dict(pcg_iters_per_launch=3), dict(pcg_directions="fp8")])
 def test_footprint_is_what_the_graph_holds(kw):
     """The admission estimate equals what the handle holds once built (its own count) and,
     within allocation granularity, the device memory a resident graph takes; a request
@@ -634,7 +635,7 @@ def test_footprint_is_what_the_graph_holds(kw):
     est = EdgeStreamLP.footprint(g, **kw)
     from dynlp_amd import _lib
     n, C1, ld = lp.n, lp.C1, lp.ld
-    kinds = {"fp32": 0, "bf16": 1, "fp16": 2}
+    from dynlp_amd.stream import DIRECTION_KINDS as kinds
     hb = _lib.load().dlp_lp_handle_bytes(n, C1, ld, int(lp.fp64), kinds[lp.pcg_directions],
                                          lp.pcg_iters_per_launch, lp.pcg_coarse, int(lp.pcg_mode == "block"))
     assert lp.device_bytes() == hb
@@ -726,3 +727,31 @@ def test_request_group_matches_sequential_oracle(monkeypatch, ops_per_call):
         check_against_oracle(lp, g, es, eps, f"group {rep}")
     if ops_per_call:
         assert lp.col.numel() > cap0      # the store was regrown
+
+
+def test_fp8_directions_take_about_the_fp16_iterations():
+    """FP8 (E4M3) directions scaled from a bound on each column's largest value keep the
+    iteration count of FP16 directions (the step length is exact for the stored direction, so
+    coarser directions only cost conjugacy), and the repaired states are certified, over an
+    initial solve from zero and three global repairs on an SBM with 64 label columns (the FP8
+    state checked against the oracle's CG solution: the direct solve of this graph fills in)."""
+    from dynlp_amd.stream import EdgeStreamLP
+    g = graphs.sbm(20000, 16, 12, 0.8, 64, 0.01, seed=21)
+    batches, _ = graphs.update_stream(g, 3, 1000, seed=21)
+    its = {}
+    for dirs in ("fp16", "fp8"):
+        lp = EdgeStreamLP(g, eps=1e-6, pcg_directions=dirs, push_max_rows=0)
+        st0 = lp.initial_solve()
+        assert st0.rho <= 1e-6 * 1.0001
+        es = edge_dict(g)
+        s0 = lp.pcg_stats()["pcg_iters"]
+        for b in batches:
+            st = lp.apply_and_solve(b)
+            assert "pcg" in st.path and st.rho <= 1e-6 * 1.0001, (dirs, st.path, st.rho)
+            O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
+                               zip(b.del_u.tolist(), b.del_v.tolist()))
+        its[dirs] = lp.pcg_stats()["pcg_iters"] - s0
+        if dirs == "fp8":
+            check_against_oracle(lp, g, es, 1e-6, dirs, iterative=True)
+        del lp
+    assert its["fp8"] <= 1.25 * its["fp16"] + 3, its
