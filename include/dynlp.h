@@ -481,12 +481,15 @@ int dlp_lp_pcg_stats(void* handle, int64_t* out);
 int dlp_lp_time_operator(void* handle, int reps, double* ms_out, void* stream);
 /* Storage of the column PCG directions: kind 0 = FP32 (default), 1 = BF16, 2 = FP16 held
  * scaled by a power of two per column (chosen from r.z so the values sit near 1; saturating
- * stores).  Z, R and A*P stay FP32 with FP64 column dots; the step length is r.p / p.Ap for
- * the direction as held, so the residual recurrence stays that of the FP32 path and the FP64
- * verification certifies as before.  16-bit directions halve the bytes of every neighbour-row
- * gather of the global solve; FP16 keeps 11 significant bits (BF16: 8) and its values enter the
- * multiply-adds without unpacking.  No reference counterpart (solvers.py:66-97 keeps P in the
- * working dtype). */
+ * stores), 3 = FP8 E4M3 held scaled by a power of two per column (chosen from a bound on the
+ * new direction's largest magnitude, max |r|/diag + max |coarse correction| + |beta| max |P|,
+ * so every value stays within 224; saturating stores), rows of ld bytes.  Z, R and A*P stay
+ * FP32 with FP64 column dots; the step length is r.p / p.Ap for the direction as held, so the
+ * residual recurrence stays that of the FP32 path and the FP64 verification certifies as
+ * before.  16-bit directions halve the bytes of every neighbour-row gather of the global
+ * solve, FP8 quarters them; FP16 keeps 11 significant bits (BF16: 8, FP8: 4), and coarser
+ * directions can cost iterations, never accuracy.  No reference counterpart (solvers.py:66-97
+ * keeps P in the working dtype). */
 int dlp_lp_set_pcg_directions(void* handle, int kind);
 /* Two-level preconditioner of the column PCG, M = D^-1 + P Ac^-1 P^T: P aggregates the solved
  * rows into up to na (<= 128) runs of consecutive rows (at least the rows one workgroup of the vector
