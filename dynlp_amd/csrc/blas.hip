@@ -1093,16 +1093,35 @@ static int check_coarse(const PcgBufs& b, const FlatLaunch& f) {
   return DLP_OK;
 }
 
+// The arrays a solve starts from zero, cleared by one launch (a launch per array costs more
+// than its work): up to kClearMax (pointer, count of doubles) pairs.
+constexpr int kClearMax = 12;
+struct ClearList {
+  double* p[kClearMax];
+  int n[kClearMax];
+  int k;
+  void add(void* ptr, int count) {
+    p[k] = reinterpret_cast<double*>(ptr);
+    n[k++] = count;
+  }
+};
+
+__global__ void k_clear_many(ClearList L) {
+  for (int j = 0; j < L.k; ++j)
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < L.n[j]; i += gridDim.x * blockDim.x) L.p[j][i] = 0.0;
+}
+
+static void clear_many(const ClearList& L, hipStream_t st) {
+  int most = 1;
+  for (int j = 0; j < L.k; ++j) most = L.n[j] > most ? L.n[j] : most;
+  const int grid = (most + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_clear_many, dim3(grid < 16 ? grid : 16), dim3(kBlock), 0, st, L);
+}
+
 int pcg_start(const PcgBufs& b, hipStream_t st) {
   DLP_CHECK_ARG(b.C >= 1 && b.C <= kMaxCols, "pcg: C=%d outside [1, %d]", b.C, kMaxCols);
   const ColMap m = col_map(b.C);
   const unsigned g = pass_grid(b.n, m.lc_shift);
-  hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, b.C, b.rz);
-  hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, b.C, b.rz_new);
-  hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, b.C, b.pAp);
-  // viol and the pad word after it (the iterations clear viol after each convergence test)
-  hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, 1, (double*)b.viol);
-  if (b.ctl) hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, 1, (double*)b.ctl);
   const FlatLaunch f = flat_launch(b);
   DLP_CHECK_ARG(f.ok || !b.P16, "pcg: 16-bit directions need 16-byte FP32 rows");
   DLP_CHECK_ARG(!b.P16 || b.p16 == 1 || (b.p16 == 2 && b.sig) || (b.p16 == 3 && b.sig && b.rmax && b.pmax && b.emax),
@@ -1110,26 +1129,26 @@ int pcg_start(const PcgBufs& b, hipStream_t st) {
   DLP_CHECK_ARG(f.ok || !b.na, "pcg: the coarse correction needs 16-byte FP32 rows");
   int rc = check_coarse(b, f);
   if (rc) return rc;
+  const bool f8 = f.ok && b.P16 && b.p16 == 3;
+  ClearList L{};
+  L.add(b.rz, b.C);
+  L.add(b.rz_new, b.C);
+  L.add(b.pAp, b.C);
+  L.add(b.viol, 1);   // viol and the pad word after it (the iterations clear viol after each test)
+  if (b.ctl) L.add(b.ctl, 1);
+  if (f.ok && b.P16) L.add(b.rp, b.C);
+  if (f8)   // the maxima accumulate from zero (C floats each fit in C/2 doubles)
+    for (float* mx : {b.rmax, b.pmax, b.emax}) L.add(mx, (b.C + 1) / 2);
+  // step lengths of the deferred updates (held as doubles' worth of float pairs)
+  if (f.ok && b.P16 && b.slot_stride > 0 && b.alpha_hist) L.add(b.alpha_hist, (b.group * b.C + 1) / 2);
+  if (f.ok && b.na) L.add(b.rc, b.na * (int)b.ldc);
+  clear_many(L, st);
   if (f.ok) {
     const CoarseArg K = coarse_arg(b);
     float* sig = (b.P16 && b.p16 >= 2) ? b.sig : nullptr;
-    const bool f8 = b.P16 && b.p16 == 3;
-    if (b.P16) hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, b.C, b.rp);
-    if (f8) {   // the maxima accumulate from zero (C floats each fit in C/2 doubles)
-      for (float* m : {b.rmax, b.pmax, b.emax})
-        hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(kBlock), 0, st, (b.C + 1) / 2, reinterpret_cast<double*>(m));
-    }
-    // step lengths of the deferred updates (held as doubles' worth of float pairs)
-    if (b.P16 && b.slot_stride > 0 && b.alpha_hist)
-      hipLaunchKernelGGL(k_zero_f64, dim3(grid_for((b.group * b.C + 1) / 2)), dim3(kBlock), 0, st,
-                         (b.group * b.C + 1) / 2, reinterpret_cast<double*>(b.alpha_hist));
     if (sig || b.na) {
-      // scale pass: rz (for the FP16 column scales) and the restricted residual, then the
-      // coarse correction of the start
-      if (b.na) {
-        const int nrc = b.na * (int)b.ldc;
-        hipLaunchKernelGGL(k_zero_f64, dim3(grid_for(nrc)), dim3(kBlock), 0, st, nrc, b.rc);
-      }
+      // scale pass: rz (for the FP16 column scales), the FP8 maxima and the restricted
+      // residual, then the coarse correction of the start
       hipLaunchKernelGGL(k_pcg_rz4, dim3(f.grid), dim3(f.block), 0, st, b.n, f.Q4, b.C, (const float4*)b.R, b.diag,
                          b.rz, K, f8 ? b.rmax : nullptr);
       if (b.na)
