@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void k_pcg_update(int n, int C, int lc_shift, 
   if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&vmax, vm);
   __syncthreads();
   for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&rz_new[c], sm[c]);
-  if (threadIdx.x == 0) atomic_max_nonneg(viol, vmax);
+  if (threadIdx.x == 0) global_max_nonneg(viol, vmax);
 }
 
 // P = R/diag + beta P   (P = R/diag when first != 0)
@@ -207,14 +207,14 @@ __device__ __forceinline__ void flush_quads(const Quad& Qd, int Q4, int C, const
 }
 
 // Column maxima of non-negative per-thread quad values: LDS max (smx cleared beforehand), then,
-// after a barrier the caller places, one atomic per column per workgroup (quad_max_out).
+// after a barrier the caller places, the workgroup's maxima into out (quad_max_out).
 __device__ __forceinline__ void quad_max_lds(int q4, int cnt, const float (&m)[4], float* smx) {
 #pragma unroll
   for (int j = 0; j < 4; ++j)
     if (j < cnt) atomic_max_nonneg(&smx[4 * q4 + j], m[j]);
 }
 __device__ __forceinline__ void quad_max_out(int C, const float* smx, float* __restrict__ out) {
-  for (int c = threadIdx.x; c < C; c += blockDim.x) atomic_max_nonneg(&out[c], smx[c]);
+  for (int c = threadIdx.x; c < C; c += blockDim.x) global_max_nonneg(&out[c], smx[c]);
 }
 
 // Loads and stores of the FP32 vector streams of the flat passes; NT marks them non-temporal so
@@ -511,7 +511,7 @@ __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float
   for (int c = threadIdx.x; c < C; c += blockDim.x) atomicAdd(&rz_new[c], sm[c]);
   if constexpr (PK == 3) quad_max_out(C, smx, rmax);
   if (K.na) rs.to_global(Q4, C, K, sm2);
-  if (threadIdx.x == 0) atomic_max_nonneg(viol, vmax);
+  if (threadIdx.x == 0) global_max_nonneg(viol, vmax);
 }
 
 // Pout = M R + beta P (M R = R/diag plus the coarse correction on solved rows; Pout may be P);

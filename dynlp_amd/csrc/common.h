@@ -119,4 +119,15 @@ __device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
   atomicMax((unsigned long long*)p, (unsigned long long)__double_as_longlong(v));
 }
 
+// The same for a word in global memory that many workgroups send their maxima to: the atomic is
+// skipped when the value already there is at least v (most workgroups, once the first few have
+// written), which keeps thousands of atomics off the same L2 line.  A stale read only costs an
+// atomic that changes nothing; NaN (any sign bit clear) still goes through, as before.
+__device__ __forceinline__ void global_max_nonneg(float* p, float v) {
+  if (!(v <= __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) atomic_max_nonneg(p, v);
+}
+__device__ __forceinline__ void global_max_nonneg(double* p, double v) {
+  if (!(v <= __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) atomic_max_nonneg(p, v);
+}
+
 }  // namespace dlp

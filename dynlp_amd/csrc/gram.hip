@@ -368,7 +368,7 @@ __global__ __launch_bounds__(256) void k_block_update_tiled(int n, int C, float*
     for (int off = 32; off > 0; off >>= 1) vm = fmaxf(vm, __shfl_xor(vm, off, 64));
     if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&vmax, vm);
     __syncthreads();
-    if (threadIdx.x == 0) atomic_max_nonneg(viol, vmax);
+    if (threadIdx.x == 0) global_max_nonneg(viol, vmax);
   }
 }
 

@@ -519,7 +519,7 @@ __global__ __launch_bounds__(256) void k_rho_cols(int nrows, const int32_t* __re
     atomic_max_nonneg(&sm[c0], local);
   }
   __syncthreads();
-  for (int c = threadIdx.x; c < C; c += blockDim.x) atomic_max_nonneg(&rho[c], sm[c]);
+  for (int c = threadIdx.x; c < C; c += blockDim.x) global_max_nonneg(&rho[c], sm[c]);
 }
 
 // Rows with any |R[r,c]| > eps[c]*diag[r] appended to out.  Group of LC lanes per row.
@@ -775,7 +775,7 @@ __global__ __launch_bounds__(256) void k_rho_cols4(int n, int Q4, int C, const f
   for (int j = 0; j < 4; ++j)
     if (j < Qd.cnt) atomic_max_nonneg(&sm[4 * Qd.q4 + j], m[j]);
   __syncthreads();
-  for (int c = threadIdx.x; c < C; c += blockDim.x) atomic_max_nonneg(rho + c, sm[c]);
+  for (int c = threadIdx.x; c < C; c += blockDim.x) global_max_nonneg(rho + c, sm[c]);
 }
 
 extern "C" int dlp_rho_cols(int nrows, const int32_t* rows, int C, const void* R, int64_t ldr,

@@ -844,7 +844,7 @@ __global__ __launch_bounds__(256) void k_refresh_v4(RefreshArgs A, int pow2, int
     }
   }
   __syncthreads();
-  for (int c = threadIdx.x; c < A.C1; c += blockDim.x) atomic_max_nonneg(A.rho + c, sm[c]);
+  for (int c = threadIdx.x; c < A.C1; c += blockDim.x) global_max_nonneg(A.rho + c, sm[c]);
 }
 
 // Residual verification of every solved row after a global solve (refresh mode 3): the row
@@ -918,8 +918,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     }
   }
   __syncthreads();
-  for (int c = threadIdx.x; c < A.C1; c += blockDim.x) atomic_max_nonneg(A.rho + c, sm[c]);
-  if (A.hmax && threadIdx.x == 0) atomic_max_nonneg(A.hmax, shm);
+  for (int c = threadIdx.x; c < A.C1; c += blockDim.x) global_max_nonneg(A.rho + c, sm[c]);
+  if (A.hmax && threadIdx.x == 0) global_max_nonneg(A.hmax, shm);
 }
 
 // Grid-stride over the listed rows in wave-uniform steps (the row count may live on the device).
@@ -956,7 +956,7 @@ __global__ __launch_bounds__(256) void k_refresh(RefreshArgs A, int g_shift, int
     }
   }
   __syncthreads();
-  for (int c = threadIdx.x; c < A.C1; c += blockDim.x) atomic_max_nonneg(A.rho + c, sm[c]);
+  for (int c = threadIdx.x; c < A.C1; c += blockDim.x) global_max_nonneg(A.rho + c, sm[c]);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1962,7 +1962,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     }
   }
   __syncthreads();
-  for (int c = threadIdx.x; c < A.C1; c += blockDim.x) atomic_max_nonneg(A.rho + c, sm[c]);
+  for (int c = threadIdx.x; c < A.C1; c += blockDim.x) global_max_nonneg(A.rho + c, sm[c]);
 }
 
 extern "C" int dlp_lp_refresh(void* handle, int which, int prec, void* stream) {
