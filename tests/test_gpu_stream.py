@@ -755,3 +755,23 @@ def test_fp8_directions_take_about_the_fp16_iterations():
             check_against_oracle(lp, g, es, 1e-6, dirs, iterative=True)
         del lp
     assert its["fp8"] <= 1.25 * its["fp16"] + 3, its
+
+
+@pytest.mark.parametrize("dirs", ["fp16", "fp8"])
+def test_low_precision_directions_certify_with_weights_over_six_decades(dirs):
+    """Edge weights spread over six decades give direction values of very different sizes within
+    a column, the case where a few significant bits and FP8's narrow range (values below 2^-9 of
+    the column's bound become zero) cost most: the repairs still end certified against the
+    rebuilt oracle."""
+    from dynlp_amd.stream import EdgeStreamLP
+    g = graphs.sbm(4000, 8, 10, 0.8, 8, 0.02, seed=31, wlo=1e-3, whi=1e3)
+    lp = EdgeStreamLP(g, eps=1e-6, pcg_directions=dirs, push_max_rows=0)
+    st = lp.initial_solve()
+    assert st.rho <= 1e-6 * 1.0001
+    es = edge_dict(g)
+    for b in graphs.update_stream(g, 3, 400, seed=31)[0]:
+        st = lp.apply_and_solve(b)
+        assert "pcg" in st.path and st.rho <= 1e-6 * 1.0001, (st.path, st.rho)
+        O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
+                           zip(b.del_u.tolist(), b.del_v.tolist()))
+    check_against_oracle(lp, g, es, 1e-6, dirs)
