@@ -673,8 +673,12 @@ def main(argv=None):
     nbytes = operator_bytes(g.n, lp.C1, stored, DIRECTION_BYTES[dirs])
     achieved = nbytes / (op_ms * 1e-3) / 1e9
     word, wid = {"fp32": ("F32x4", 0), "bf16": ("BF16x4", 2), "fp16": ("F16x4", 3), "fp8": ("F8x4", 5)}[dirs]
-    kname = f"k_pcg_operator<4, {word}> (A*P on {dirs.upper()} directions with fused p^T A p, the global solve)"
-    traffic = pmc_traffic(f"dlp::k_pcg_operator<4, (dlp::Word){wid}>")
+    # template arguments: gathers in flight per lane, lane word, waves per SIMD it is built for
+    # (the library's defaults: FP8 8 and 6, the others 4 and 7)
+    depth, waves = (8, 6) if dirs == "fp8" else (4, 7)
+    kname = (f"k_pcg_operator<{depth}, {word}, {waves}> (A*P on {dirs.upper()} directions with fused p^T A p, "
+             "the global solve)")
+    traffic = pmc_traffic(f"dlp::k_pcg_operator<{depth}, (dlp::Word){wid}, {waves}>")
     roofline = {"kernel": kname,
                 "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

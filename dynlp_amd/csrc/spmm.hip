@@ -277,8 +277,8 @@ struct PcgEpilogue {
   }
 };
 
-template <int S, Word K>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_pcg_operator(
+template <int S, Word K, int WPE = 7>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void k_pcg_operator(
     int nrows, int C, const int32_t* __restrict__ indptr, const int32_t* __restrict__ rowlen,
     const int32_t* __restrict__ indices, const float* __restrict__ w, const uint8_t* __restrict__ rowmask,
     const void* __restrict__ X, unsigned row_bytes, int xbytes, const float* __restrict__ a, float b,
@@ -387,7 +387,15 @@ int pcg_operator(int n, int C, const int32_t* indptr, const int32_t* rowlen, con
   const int S = T.pipe == 8 ? 8 : T.pipe == 6 ? 6 : 4;
   const bool w8 = !h16 && T.bf16_word == 8;
 #define DLP_PIPE_FN(SV, KV) reinterpret_cast<const void*>(&k_pcg_operator<SV, KV>)
-  const void* kfn = f8 ? DLP_PIPE_FN(4, Word::F8x4)
+  // FP8 words take one register per gather: 8 in flight per lane at 6 waves per SIMD measured
+  // 0.396 against 0.409 ms at 4 and 7 (SBM-1M, k = 64); 16-bit words spill at depth 6 or 8
+  const int S8 = T.f8_pipe;
+  const int f8w = T.f8_waves;
+  const void* kfn = f8 ? (S8 == 8   ? (f8w == 5 ? reinterpret_cast<const void*>(&k_pcg_operator<8, Word::F8x4, 5>)
+                                               : reinterpret_cast<const void*>(&k_pcg_operator<8, Word::F8x4, 6>))
+                          : S8 == 6 ? (f8w == 5 ? reinterpret_cast<const void*>(&k_pcg_operator<6, Word::F8x4, 5>)
+                                               : reinterpret_cast<const void*>(&k_pcg_operator<6, Word::F8x4, 6>))
+                                   : DLP_PIPE_FN(4, Word::F8x4))
                   : h16 ? (S == 8 ? DLP_PIPE_FN(8, Word::F16x4)
                            : S == 6 ? DLP_PIPE_FN(6, Word::F16x4) : DLP_PIPE_FN(4, Word::F16x4))
                   : bf ? (w8 ? DLP_PIPE_FN(4, Word::BF16x8)
@@ -403,10 +411,17 @@ int pcg_operator(int n, int C, const int32_t* indptr, const int32_t* rowlen, con
   if (xcd) grid = grid < 8 ? 8 : grid & ~7LL;
   const void* Xp = bf ? (const void*)X16 : (const void*)X;
   const unsigned rb = bf ? (unsigned)(ldh * 2) : (unsigned)(ldx * 4);
+#define DLP_PIPE3(SV, KV, WV)                                                                                 \
+  hipLaunchKernelGGL((k_pcg_operator<SV, KV, WV>), dim3((unsigned)grid), dim3(kBlock), 0, st, n, C, indptr,      \
+                     rowlen, indices, w, rowmask, Xp, rb, (int)xbytes, a, b, out, ldo, dot, xcd, skip)
 #define DLP_PIPE(SV, KV)                                                                                      \
   hipLaunchKernelGGL((k_pcg_operator<SV, KV>), dim3((unsigned)grid), dim3(kBlock), 0, st, n, C, indptr, rowlen, \
                      indices, w, rowmask, Xp, rb, (int)xbytes, a, b, out, ldo, dot, xcd, skip)
-  if (f8) DLP_PIPE(4, Word::F8x4);
+  if (f8 && S8 == 8 && f8w == 5) DLP_PIPE3(8, Word::F8x4, 5);
+  else if (f8 && S8 == 8) DLP_PIPE3(8, Word::F8x4, 6);
+  else if (f8 && S8 == 6 && f8w == 5) DLP_PIPE3(6, Word::F8x4, 5);
+  else if (f8 && S8 == 6) DLP_PIPE3(6, Word::F8x4, 6);
+  else if (f8) DLP_PIPE(4, Word::F8x4);
   else if (h16 && S == 8) DLP_PIPE(8, Word::F16x4);
   else if (h16 && S == 6) DLP_PIPE(6, Word::F16x4);
   else if (h16) DLP_PIPE(4, Word::F16x4);
@@ -418,6 +433,7 @@ int pcg_operator(int n, int C, const int32_t* indptr, const int32_t* rowlen, con
   else if (S == 6) DLP_PIPE(6, Word::F32x4);
   else DLP_PIPE(4, Word::F32x4);
 #undef DLP_PIPE
+#undef DLP_PIPE3
   DLP_CHECK_LAUNCH();
   return DLP_OK;
 }
@@ -841,7 +857,9 @@ extern "C" int dlp_tune_set(int key, int value) {
   // key 6: gathers in flight per lane in the pipelined global-solve operator (4, 6 or 8; 0 =
   // the plain fused operator); key 7: BF16 direction values per lane (4: 8-byte words, 8: 16);
   // key 8: workgroups of the flat PCG vector passes (0 = 2048); key 9: non-temporal FP32 streams
-  // in the PCG update and direction passes (1 default)
+  // in the PCG update and direction passes (1 default); key 10: deferred solution updates; key
+  // 11: waves per SIMD the FP8 operator is compiled for at depth 6 or 8 (5 or 6); key 12:
+  // gathers per lane in flight in the FP8 operator (4, 6 or 8; default 8)
   if (key == 0) {
     DLP_CHECK_ARG(value == 0 || value == 4 || value == 8, "tune: unroll must be 0, 4 or 8");
     tuning().unroll = value;
@@ -870,8 +888,12 @@ extern "C" int dlp_tune_set(int key, int value) {
     tuning().nt_streams = value ? 1 : 0;
   } else if (key == 10) {
     tuning().defer_z = value ? 1 : 0;
-
-
+  } else if (key == 11) {
+    DLP_CHECK_ARG(value == 5 || value == 6, "tune: FP8 operator waves per SIMD at depth 6/8 must be 5 or 6");
+    tuning().f8_waves = value;
+  } else if (key == 12) {
+    DLP_CHECK_ARG(value == 4 || value == 6 || value == 8, "tune: FP8 operator depth must be 4, 6 or 8");
+    tuning().f8_pipe = value;
   } else {
     DLP_CHECK_ARG(false, "tune: unknown key %d", key);
   }

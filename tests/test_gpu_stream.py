@@ -775,3 +775,30 @@ def test_low_precision_directions_certify_with_weights_over_six_decades(dirs):
         O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
                            zip(b.del_u.tolist(), b.del_v.tolist()))
     check_against_oracle(lp, g, es, 1e-6, dirs)
+
+
+@pytest.mark.parametrize("depth,waves", [(4, 6), (6, 5), (6, 6), (8, 5)])
+def test_fp8_operator_depths_certify(depth, waves):
+    """The FP8 operator at every depth (gathers per lane in flight) and register budget it is
+    built for (the default, depth 8 at 6 waves per SIMD, runs in the other FP8 tests): a global
+    solve from zero and two repairs, certified against the rebuilt oracle."""
+    from dynlp_amd import _lib
+    from dynlp_amd.stream import EdgeStreamLP
+    _lib.call("dlp_tune_set", 12, depth)
+    _lib.call("dlp_tune_set", 11, waves)
+    try:
+        g = graphs.sbm(3000, 8, 10, 0.8, 8, 0.02, seed=41)
+        lp = EdgeStreamLP(g, eps=1e-6, pcg_directions="fp8", push_max_rows=0)
+        lp.initial_solve()
+        es = edge_dict(g)
+        for b in graphs.update_stream(g, 2, 300, seed=41)[0]:
+            st = lp.apply_and_solve(b)
+            assert "pcg" in st.path and st.rho <= 1e-6 * 1.0001
+            O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
+                               zip(b.del_u.tolist(), b.del_v.tolist()))
+        st = lp.load_state(np.zeros((g.n, lp.C1), np.float32))
+        assert "pcg" in st.path
+        check_against_oracle(lp, g, es, 1e-6, f"depth {depth} waves {waves}")
+    finally:
+        _lib.call("dlp_tune_set", 12, 8)
+        _lib.call("dlp_tune_set", 11, 6)
