@@ -106,6 +106,7 @@ _SIGS = {
     "dlp_amg_restrict": (ct.c_int, [I32, I32, P, P, I64, P, I64, I32, P]),
     "dlp_amg_prolong_add": (ct.c_int, [I32, I32, P, P, I64, P, I64, I32, P]),
     "dlp_amg_level": (ct.c_int, [I32, I32, P, P, P, P, P, P, I64, DBL, I32, P, P]),
+    "dlp_check_edge_batch": (ct.c_int, [I64, I64, P, P, I64, P, P, P, I32, P]),
 }
 
 

@@ -15,6 +15,7 @@ All generators use numpy's PCG64 (`numpy.random.default_rng`).
 """
 from __future__ import annotations
 
+import ctypes as ct
 import os
 from dataclasses import dataclass, field
 
@@ -268,27 +269,49 @@ class EdgeBatch:
         return int(self.ins_u.shape[0] + self.del_u.shape[0])
 
 
-def check_batch(b: EdgeBatch, n: int, max_insertions: int | None = None) -> None:
-    """Refuses a batch whose arrays differ in length, that holds more than max_insertions
-    insertions or an insertion weight that is not finite and positive (ValueError), or whose
-    vertex ids leave [0, n) (IndexError): the device apply indexes rows by these ids."""
+def _id_arrays(b: EdgeBatch):
+    """The four id arrays, C-contiguous, of one integer width (int32 kept, anything else int64)."""
+    arrs = [np.asarray(a) for a in (b.del_u, b.del_v, b.ins_u, b.ins_v)]
+    dt = np.int32 if all(a.dtype == np.int32 for a in arrs) else np.int64
+    return [np.ascontiguousarray(a, dtype=dt) for a in arrs], np.dtype(dt).itemsize
+
+
+def _scan_batch(b: EdgeBatch, n: int, max_insertions: int | None) -> int:
+    """The checks of check_batch; returns a mask of 1 (a pair repeats within the deletions) and
+    2 (within the insertions).  Ids, weights and repeats are scanned in one pass of the library's
+    host-side check (dlp_check_edge_batch), no device work."""
+    from . import _lib
     nd, ni = len(b.del_u), len(b.ins_u)
     if len(b.del_v) != nd or len(b.ins_v) != ni or len(b.ins_w) != ni:
         raise ValueError(f"batch arrays differ in length: del {nd}/{len(b.del_v)}, "
                          f"ins {ni}/{len(b.ins_v)}/{len(b.ins_w)}")
     if max_insertions is not None and ni > max_insertions:
         raise ValueError(f"{ni} insertions in one batch: at most {max_insertions} (split the batch)")
-    for a in (b.del_u, b.del_v, b.ins_u, b.ins_v):
-        if len(a) and (int(a.min()) < 0 or int(a.max()) >= n):
-            raise IndexError(f"batch vertex ids must lie in [0, {n}): found {int(a.min())}..{int(a.max())}")
-    if ni:
-        w = np.asarray(b.ins_w, dtype=np.float64)
-        with np.errstate(over="ignore"):
-            w32 = w.astype(np.float32)
-        bad = ~(np.isfinite(w32) & (w32 > 0))
-        if bad.any():
-            i = int(np.flatnonzero(bad)[0])
-            raise ValueError(f"insertion weights must be finite and positive (as FP32): ins_w[{i}] = {w[i]!r}")
+    ids, width = _id_arrays(b)
+    w = np.asarray(b.ins_w)
+    with np.errstate(over="ignore", invalid="ignore"):
+        w32 = np.ascontiguousarray(w, dtype=np.float32)
+    where = ct.c_int64(-1)
+    P = ct.c_void_p
+    rc = _lib.load().dlp_check_edge_batch(int(n), nd, P(ids[0].ctypes.data), P(ids[1].ctypes.data), ni,
+                                           P(ids[2].ctypes.data), P(ids[3].ctypes.data), P(w32.ctypes.data),
+                                           width, ct.byref(where))
+    if rc == -1:
+        a = ids[[0, 0, 1, 2, 3][np.searchsorted(np.cumsum([0, nd, nd, ni, ni]), where.value, side="right")]]
+        raise IndexError(f"batch vertex ids must lie in [0, {n}): found {int(a.min())}..{int(a.max())}")
+    if rc == -2:
+        i = int(where.value)
+        raise ValueError(f"insertion weights must be finite and positive (as FP32): ins_w[{i}] = {w[i]!r}")
+    if rc < 0:
+        raise ValueError(f"batch check refused its arguments (n = {n})")
+    return int(rc)
+
+
+def check_batch(b: EdgeBatch, n: int, max_insertions: int | None = None) -> None:
+    """Refuses a batch whose arrays differ in length, that holds more than max_insertions
+    insertions or an insertion weight that is not finite and positive (ValueError), or whose
+    vertex ids leave [0, n) (IndexError): the device apply indexes rows by these ids."""
+    _scan_batch(b, n, max_insertions)
 
 
 def pack_ops(batches):
@@ -317,19 +340,15 @@ def normalize_batch(b: EdgeBatch, n: int, max_insertions: int | None = None) -> 
     """check_batch, then one entry per unordered pair: repeated deletions collapse to one,
     repeated insertions (either orientation) to their last occurrence.  Returns b itself when
     nothing repeats.  Self-loops pass through (the apply ignores them)."""
-    check_batch(b, n, max_insertions)
-    out = {}
-    for kind, (u, v) in (("del", (b.del_u, b.del_v)), ("ins", (b.ins_u, b.ins_v))):
-        if len(u) < 2:
-            continue
-        u64, v64 = np.asarray(u, dtype=np.int64), np.asarray(v, dtype=np.int64)
-        key = np.minimum(u64, v64) * n + np.maximum(u64, v64)
-        s = np.sort(key)
-        if not (s[1:] == s[:-1]).any():    # the common case, a sort cheaper than the stable one below
-            continue
-        out[kind] = np.sort(_last_unique(key))
-    if not out:
+    rep = _scan_batch(b, n, max_insertions)
+    if not rep:
         return b
+    out = {}
+    for bit, kind, (u, v) in ((1, "del", (b.del_u, b.del_v)), (2, "ins", (b.ins_u, b.ins_v))):
+        if rep & bit:
+            u64, v64 = np.asarray(u, dtype=np.int64), np.asarray(v, dtype=np.int64)
+            key = np.minimum(u64, v64) * n + np.maximum(u64, v64)
+            out[kind] = np.sort(_last_unique(key))
     dk = out.get("del")
     ik = out.get("ins")
     return EdgeBatch(b.ins_u if ik is None else np.asarray(b.ins_u)[ik],

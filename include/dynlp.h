@@ -491,6 +491,17 @@ int dlp_lp_time_operator(void* handle, int reps, double* ms_out, void* stream);
  * directions can cost iterations, never accuracy.  No reference counterpart (solvers.py:66-97
  * keeps P in the working dtype). */
 int dlp_lp_set_pcg_directions(void* handle, int kind);
+/* Host-side check of an edge batch before it is applied (no device work): ids (id_bytes 4 =
+ * int32, 8 = int64) in [0, n), insertion weights finite and > 0.  Returns 0 (valid, no pair
+ * repeated), a mask of 1 (an unordered pair repeats within the deletions) and 2 (within the
+ * insertions, either orientation) -- the caller collapses repeats before dlp_lp_apply, which
+ * takes distinct pairs -- or -1 (an id out of range; *where = its position in del_u, del_v,
+ * ins_u, ins_v taken one after the other), -2 (a bad weight; *where = its index), -3 (bad
+ * arguments).  The batch contract of the reference's update path (stream.py:180-230 builds
+ * each batch's system from distinct edges; check_batch / normalize_batch in graphs.py). */
+int dlp_check_edge_batch(int64_t n, int64_t nd, const void* del_u, const void* del_v, int64_t ni,
+                         const void* ins_u, const void* ins_v, const float* ins_w, int id_bytes,
+                         int64_t* where);
 /* Two-level preconditioner of the column PCG, M = D^-1 + P Ac^-1 P^T: P aggregates the solved
  * rows into up to na (<= 128) runs of consecutive rows (at least the rows one workgroup of the vector
  * passes spans; each of the na evenly spaced boundaries moves to the row boundary with the fewest

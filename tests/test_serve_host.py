@@ -196,3 +196,49 @@ def test_fuse_of_one_batch_collapses_repeats():
     one = O.apply_edge_batch(dict(base), zip(f.ins_u.tolist(), f.ins_v.tolist(), f.ins_w.tolist()),
                              zip(f.del_u.tolist(), f.del_v.tolist()))
     assert seq == one == {(0, 1): 1.0, (1, 2): 5.0, (1, 3): 2.0}
+
+
+def test_library_batch_check_matches_brute_force():
+    """dlp_check_edge_batch (one hashed pass in the library, no device work) against a brute
+    force over random batches: the repeat mask (pairs unordered, deletions and insertions
+    separately), the first id out of range and the first bad weight, for int32 and int64 ids and
+    vertex counts up to 2^31 (keys near 2^62)."""
+    import ctypes as ct
+    from dynlp_amd import _lib
+    lib = _lib.load()
+    rs = np.random.RandomState(9)
+    P = ct.c_void_p
+    for trial in range(400):
+        n = int(rs.choice([5, 50, 1 << 20, (1 << 31) - 1]))
+        dt = np.int32 if rs.rand() < 0.5 else np.int64
+        nd, ni = rs.randint(0, 40), rs.randint(0, 40)
+        du, dv, iu, iv = (rs.randint(0, min(n, 1 << 30), m).astype(dt) for m in (nd, nd, ni, ni))
+        if n > 100 and nd > 2 and rs.rand() < 0.5:      # plant a repeat (either orientation)
+            du[-1], dv[-1] = (dv[0], du[0]) if rs.rand() < 0.5 else (du[0], dv[0])
+        if n > 100 and ni > 2 and rs.rand() < 0.5:
+            iu[-1], iv[-1] = iv[1], iu[1]
+        iw = rs.uniform(0.1, 2, ni).astype(np.float32)
+        expect_err = 0
+        if rs.rand() < 0.15 and nd + ni:
+            arrs = [a for a in (du, dv, iu, iv) if len(a)]
+            a = arrs[rs.randint(len(arrs))]
+            a[rs.randint(len(a))] = -1 if rs.rand() < 0.5 else (n if dt == np.int64 or n < (1 << 31) - 1 else -2)
+            expect_err = -1
+        elif rs.rand() < 0.15 and ni:
+            iw[rs.randint(ni)] = [np.nan, np.inf, 0.0, -1.0][rs.randint(4)]
+            expect_err = -2
+        where = ct.c_int64(-7)
+        rc = lib.dlp_check_edge_batch(n, nd, P(du.ctypes.data), P(dv.ctypes.data), ni, P(iu.ctypes.data),
+                                      P(iv.ctypes.data), P(iw.ctypes.data), np.dtype(dt).itemsize, ct.byref(where))
+        flat = np.concatenate([du, dv, iu, iv]).astype(np.int64)
+        bad_id = np.flatnonzero((flat < 0) | (flat >= n))
+        bad_w = np.flatnonzero(~(np.isfinite(iw) & (iw > 0)))
+        if len(bad_id):
+            assert rc == -1 and where.value == bad_id[0], (trial, rc, where.value)
+        elif len(bad_w):
+            assert rc == -2 and where.value == bad_w[0], (trial, rc, where.value)
+        else:
+            assert expect_err == 0
+            rep = lambda u, v: len({(min(a, b), max(a, b)) for a, b in zip(u.tolist(), v.tolist())}) < len(u)  # noqa: E731
+            assert rc == (1 if rep(du, dv) else 0) | (2 if rep(iu, iv) else 0), trial
+    assert lib.dlp_check_edge_batch(0, 0, None, None, 0, None, None, None, 4, None) == -3
