@@ -57,6 +57,7 @@ _SIGS = {
     "dlp_lp_violators": (ct.c_int, [P, P, P]),
     "dlp_lp_status": (ct.c_int, [P, P, P, P]),
     "dlp_lp_pcg": (ct.c_int, [P, I32, I32, I32, I32, P, P, P]),
+    "dlp_lp_pcg_verify": (ct.c_int, [P, I32, I32, I32, I32, P, P, P]),
     "dlp_lp_pcg_stats": (ct.c_int, [P, P]),
     "dlp_lp_reserve": (ct.c_int, [P, I32, I32, P]),
     "dlp_push_plan_create": (ct.c_int, [P, I32, P]),

@@ -2784,11 +2784,12 @@ static int pcg_reserve(LP* h, int iters_per_launch, bool block, hipStream_t st) 
 // recurrence residual meets the push thresholds (max |R|/(eps*diag) <= 1), the violation
 // stalls for `patience` launches, or max_iter.  *iters_out: iterations run; *viol_out: last
 // violation ratio.
-extern "C" int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int use_graph, int patience,
-                          int64_t* iters_out, double* viol_out, void* stream) {
-  DLP_CHECK_ARG(handle && max_iter >= 0 && iters_per_launch >= 1, "lp_pcg: bad argument");
-  LP* h = (LP*)handle;
-  hipStream_t st = (hipStream_t)stream;
+// The column PCG; with verify, the FP64 verification pass (refresh mode 3) is enqueued behind
+// every group of iterations, before the host reads the group's status, so a solve that ends
+// after a group has its certificate without another host round trip (a group that does not
+// end the solve costs one extra pass, which also replaces the kept residual by the FP64 one).
+static int lp_pcg(LP* h, int max_iter, int iters_per_launch, int use_graph, int patience, int64_t* iters_out,
+                  double* viol_out, hipStream_t st, bool verify) {
   int rc = pcg_reserve(h, iters_per_launch, false, st);
   if (rc) return rc;
   const int group = pcg_group_for(h, iters_per_launch);
@@ -2821,6 +2822,10 @@ extern "C" int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int 
     }
     launched += iters_per_launch;
     DLP_CHECK_HIP(hipEventRecord(h->ev1, st));
+    if (verify) {
+      rc = dlp_lp_refresh(h, 3, 0, st);
+      if (rc) return rc;
+    }
     DLP_CHECK_HIP(hipMemcpyAsync(h->host, h->pcg_viol, 4 * sizeof(float), hipMemcpyDeviceToHost, st));
     DLP_CHECK_HIP(hipStreamSynchronize(st));
     float ms = 0.f;
@@ -2843,6 +2848,20 @@ extern "C" int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int 
   if (iters_out) *iters_out = it;
   if (viol_out) *viol_out = v;
   return DLP_OK;
+}
+
+extern "C" int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int use_graph, int patience,
+                          int64_t* iters_out, double* viol_out, void* stream) {
+  DLP_CHECK_ARG(handle && max_iter >= 0 && iters_per_launch >= 1, "lp_pcg: bad argument");
+  return lp_pcg((LP*)handle, max_iter, iters_per_launch, use_graph, patience, iters_out, viol_out,
+                (hipStream_t)stream, false);
+}
+
+extern "C" int dlp_lp_pcg_verify(void* handle, int max_iter, int iters_per_launch, int use_graph, int patience,
+                                 int64_t* iters_out, double* viol_out, void* stream) {
+  DLP_CHECK_ARG(handle && max_iter >= 0 && iters_per_launch >= 1, "lp_pcg_verify: bad argument");
+  return lp_pcg((LP*)handle, max_iter, iters_per_launch, use_graph, patience, iters_out, viol_out,
+                (hipStream_t)stream, true);
 }
 
 // Reported scores [F, h] in FP64: solved rows as held, rows outside the solved set as F = 0,

@@ -222,12 +222,16 @@ class EdgeStreamLP:
                   int(budget), ct.byref(r), ct.byref(ok), stream())
         return int(r.value), bool(ok.value)
 
-    def _pcg(self):
+    def _pcg(self, verify=False):
+        """The global solve; with verify (column PCG only) the library also runs the FP64
+        verification pass behind its last group of iterations, before it reads their status.
+        Returns (iterations, last violation, whether the verification ran)."""
         it, v = ct.c_int64(0), ct.c_double(0.0)
-        fn = "dlp_lp_bpcg" if self.pcg_mode == "block" else "dlp_lp_pcg"
+        verify = verify and self.pcg_mode != "block"
+        fn = "dlp_lp_bpcg" if self.pcg_mode == "block" else ("dlp_lp_pcg_verify" if verify else "dlp_lp_pcg")
         _lib.call(fn, self.h, self.pcg_max_iter, self.pcg_iters_per_launch, int(self.use_graph), 50,
                   ct.byref(it), ct.byref(v), stream())
-        return int(it.value), float(v.value)
+        return int(it.value), float(v.value), verify
 
     def coarse_info(self):
         """The global solve's coarse operator: aggregates (0: none), their minimum rows, solves
@@ -329,11 +333,12 @@ class EdgeStreamLP:
                 st.iters += r
         if not ok:
             st.path = "push+pcg" if tried else "pcg"
-            it, _ = self._pcg()
+            it, _, verified = self._pcg(verify=True)
             st.extra["pcg_iters"] = st.extra.get("pcg_iters", 0) + it
             # FP64 residual of every solved row; their equations are those of the changed-row
-            # refresh, so the verification pass (mode 3) suffices
-            _lib.call("dlp_lp_refresh", self.h, 3, 0, stream())
+            # refresh, so the verification pass (mode 3) suffices (the column PCG ran it already)
+            if not verified:
+                _lib.call("dlp_lp_refresh", self.h, 3, 0, stream())
             s = self.status()
             if s["viol"] == 0:
                 # certified on every row: the pass's column maxima are rho over all rows

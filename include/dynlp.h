@@ -436,6 +436,14 @@ int dlp_lp_push(void* handle, int max_rounds, int rounds_per_launch, int use_gra
  * DynLPPlus._global (solvers.py:268-270). */
 int dlp_lp_pcg(void* handle, int max_iter, int iters_per_launch, int use_graph, int patience,
                int64_t* iters_out, double* viol_out, void* stream);
+/* dlp_lp_pcg with the FP64 verification pass of every solved row (dlp_lp_refresh mode 3) enqueued
+ * behind each launch of iterations, before the host reads their status: a solve that ends after
+ * a launch returns with its certificate computed (dlp_lp_status) and one host round trip fewer.
+ * A launch that does not end the solve costs one extra pass, which replaces the kept residual by
+ * its FP64 value.  The certification step of DynLPPlus._global (solvers.py:322-337) fused with
+ * the solve. */
+int dlp_lp_pcg_verify(void* handle, int max_iter, int iters_per_launch, int use_graph, int patience,
+                      int64_t* iters_out, double* viol_out, void* stream);
 /* Block PCG variant of dlp_lp_pcg: one search space for all C1 (<= 80) columns, alpha =
  * (P^T A P)^{-1} (R^T D^{-1} R), beta from the ratio of successive R^T D^{-1} R, the k x k
  * Grams in FP64 on the matrix cores and their Cholesky solves in FP64 (collapsed directions

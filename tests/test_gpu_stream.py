@@ -802,3 +802,39 @@ def test_fp8_operator_depths_certify(depth, waves):
     finally:
         _lib.call("dlp_tune_set", 12, 8)
         _lib.call("dlp_tune_set", 11, 6)
+
+
+def test_solve_with_enqueued_verification_matches_separate_verification():
+    """The global solve with the FP64 verification enqueued behind its launches
+    (dlp_lp_pcg_verify, the service's path) against the solve followed by a separate
+    verification call: the same paths and iteration counts (to one), both certified, each state
+    within the other's certificate, and the enqueued one against the rebuilt oracle.  One case
+    with launches of 2 iterations, so that verification also runs behind launches that do not
+    end the solve."""
+    from dynlp_amd.stream import EdgeStreamLP
+    g = graphs.sbm(4000, 8, 10, 0.8, 8, 0.02, seed=51)
+    batches, _ = graphs.update_stream(g, 3, 400, seed=51)
+    for per in (7, 2):
+        res = []
+        for fused in (False, True):
+            lp = EdgeStreamLP(g, eps=1e-6, push_max_rows=0, pcg_iters_per_launch=per)
+            inner = lp._pcg
+            lp._pcg = lambda verify=False, _inner=inner, _fused=fused: _inner(verify=_fused)
+            lp.initial_solve()
+            es = edge_dict(g)
+            its = []
+            for b in batches:
+                st = lp.apply_and_solve(b)
+                assert "pcg" in st.path and st.rho <= 1e-6 * 1.0001, (per, fused, st.path, st.rho)
+                its.append(st.extra.get("pcg_iters", 0))
+                O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
+                                   zip(b.del_u.tolist(), b.del_v.tolist()))
+            if fused:
+                check_against_oracle(lp, g, es, 1e-6, f"enqueued verification, {per} per launch")
+            res.append((its, lp.scores_with_h()))
+            del lp
+        (i0, Z0), (i1, Z1) = res
+        assert all(abs(a - b) <= 1 for a, b in zip(i0, i1)), (per, i0, i1)
+        C = Z0.shape[1] - 1
+        hmax = max(Z0[:, C].max(), Z1[:, C].max())
+        np.testing.assert_allclose(Z0[:, :C], Z1[:, :C], rtol=0, atol=2.1e-6 * hmax)
