@@ -458,11 +458,17 @@ def test_batch_edge_cases_match_oracle():
     check_against_oracle(lp, g, es, eps, "after refused batches")
 
 
-def test_overflow_regrow_keeps_results():
+@pytest.mark.parametrize("push_max_rows", [None, 0])
+def test_overflow_regrow_keeps_results(push_max_rows):
+    """Rows without slack overflow on insertion and the store is regrown: before the repair for
+    batches a push may repair (status read after the apply), after the global solve for large
+    batches (push_max_rows 0: the overflow shows in the status the repair ends with, then the
+    regrow, re-grounding, refresh and a second repair)."""
     from dynlp_amd.stream import EdgeStreamLP
     g = graphs.sbm(2000, 4, 6, 0.8, 4, 0.03, seed=3)
     eps = 1e-6
-    lp = EdgeStreamLP(g, eps=eps, slack_min=0, slack_frac=0.0)
+    kw = {} if push_max_rows is None else dict(push_max_rows=push_max_rows)
+    lp = EdgeStreamLP(g, eps=eps, slack_min=0, slack_frac=0.0, **kw)
     lp.initial_solve()
     es = edge_dict(g)
     batches, _ = graphs.update_stream(g, 3, 400, ins_frac=0.8, seed=3)
