@@ -529,8 +529,9 @@ def serve_run(a):
     offered = sum(r.batch.size for r in reqs)
     log(f"{len(reqs)} requests, {offered:,} edge updates ({time.perf_counter() - t:.1f}s)")
     budget = int(a.serve_budget_gb * 1e9)
-    need = sum(serve.footprint_bytes(g) for g in gs)
-    svc = serve.Service(gs, budget, slots=a.serve_slots, eps=1e-6, progress=lambda k, d: log(f"{k}: {d}"))
+    need = sum(serve.footprint_bytes(g, pcg_directions=a.directions) for g in gs)
+    svc = serve.Service(gs, budget, slots=a.serve_slots, eps=1e-6, lp_kwargs=dict(pcg_directions=a.directions),
+                        progress=lambda k, d: log(f"{k}: {d}"))
     t = time.perf_counter()
     # preload: every graph that fits is admitted and solved before the clock starts
     done = svc.run(reqs)
@@ -547,7 +548,7 @@ def serve_run(a):
                                   f"{a.serve_seconds:g} s, batches log-uniform [100, 100k] (50/50)",
                       "graphs": len(gs), "n_total": int(sum(g.n for g in gs)),
                       "hbm_budget_gb": a.serve_budget_gb, "footprint_gb": round(need / 1e9, 2),
-                      "slots": a.serve_slots},
+                      "slots": a.serve_slots, "pcg_directions": a.directions},
            "offered_edge_updates": int(offered),
            "service": {k: (float(f"{v:.4g}") if isinstance(v, float) else v) for k, v in rep.items()}}
     print(json.dumps(out), flush=True)

@@ -1268,7 +1268,7 @@ struct LP {
   float* AP = nullptr;
   uint16_t* P16 = nullptr;      // low-precision directions of the column PCG (pcg_p16 != 0)
   int64_t ldh = 0;              // its row length in 16-bit units (direction_ldh)
-  int p16_slots = 0;            // n x ldh blocks allocated in P16
+  int p16_slots = 0;            // direction slots allocated in P16 (slot_units each)
   int pcg_group = 0;            // iterations per deferred solution update (0: every iteration)
   float* pcg_ahist = nullptr;   // their step lengths, pcg_group x C1 (rounded up to even)
   int ahist_len = 0;
@@ -2460,6 +2460,12 @@ extern "C" int dlp_lp_status(void* handle, int64_t* out, double* rho, void* stre
 // Global solve: column-wise PCG with the Jacobi preconditioner on the FP32 state, from the
 // kept residual R32 (solvers.py:66-97, the DynLP+ global path solvers.py:268-288).
 
+// 16-bit units per direction slot: n rows of ldh, rounded up to 256 bytes, so that every slot
+// starts as aligned as the block (FP8 rows of ld bytes alone leave n*ld a multiple of 4 only).
+static int64_t slot_units(int64_t n, int64_t ldh) {
+  return (n * ldh + 127) / 128 * 128;
+}
+
 static PcgBufs pcg_bufs(LP* h) {
   PcgBufs b{};
   b.n = h->d.n;
@@ -2488,7 +2494,7 @@ static PcgBufs pcg_bufs(LP* h) {
   b.pmax = reinterpret_cast<float*>(h->pcg_scal + 8 * C1);
   b.emax = reinterpret_cast<float*>(h->pcg_scal + 9 * C1);
   if (b.P16 && h->pcg_group > 0) {
-    b.slot_stride = (int64_t)h->d.n * h->ldh;
+    b.slot_stride = slot_units(h->d.n, h->ldh);
     b.group = h->pcg_group;
     b.alpha_hist = h->pcg_ahist;
   }
@@ -2658,7 +2664,11 @@ static int capture_pcg(LP* h, int iters) {
     (void)hipGraphDestroy(graph);
   }
   (void)hipStreamDestroy(cs);
-  if (e != hipSuccess || rc != DLP_OK) {
+  if (rc != DLP_OK) {   // a launch refused during capture: its own message stands
+    h->pcg_exec = nullptr;
+    return rc;
+  }
+  if (e != hipSuccess) {
     set_error("pcg graph capture failed: %s", hipGetErrorString(e));
     h->pcg_exec = nullptr;
     return DLP_ERR_HIP;
@@ -2739,7 +2749,7 @@ static int pcg_reserve(LP* h, int iters_per_launch, bool block, hipStream_t st) 
   const int slots = group > 0 ? group + 1 : 1;
   const int64_t ldh = direction_ldh(h->pcg_p16, h->d.ld);
   if (h->pcg_p16 && (!h->P16 || h->p16_slots < slots || h->ldh != ldh)) {
-    const size_t nh = (size_t)h->d.n * ldh * slots;
+    const size_t nh = (size_t)slot_units(h->d.n, ldh) * slots;
     uint16_t* fresh = nullptr;
     AllocGroup g;
     g.add((void**)&fresh, nh * sizeof(uint16_t));
@@ -2747,7 +2757,7 @@ static int pcg_reserve(LP* h, int iters_per_launch, bool block, hipStream_t st) 
     if (rc) return rc;
     if (h->P16) {
       (void)hipFree(h->P16);
-      h->dev_bytes -= (int64_t)((size_t)h->d.n * h->ldh * h->p16_slots * sizeof(uint16_t));
+      h->dev_bytes -= (int64_t)((size_t)slot_units(h->d.n, h->ldh) * h->p16_slots * sizeof(uint16_t));
     }
     h->P16 = fresh;
     h->ldh = ldh;
@@ -2964,7 +2974,7 @@ extern "C" int64_t dlp_lp_handle_bytes(int n, int C1, int64_t ld, int fp64, int 
   const bool p16 = dirs != 0;
   const int group =
       (p16 && tuning().defer_z && (ld % 4) == 0 && iters_per_launch <= kMaxGroup) ? iters_per_launch : 0;
-  if (p16) b += 2 * N * direction_ldh(dirs, ld) * (group > 0 ? group + 1 : 1);
+  if (p16) b += 2 * slot_units(N, direction_ldh(dirs, ld)) * (group > 0 ? group + 1 : 1);
   if (group > 0) b += 4 * ((group * C1 + 2) & ~1);
   if (coarse > 0 && n >= 64)
     b += 2 * 8 * kCoarseMax * kCoarseMax + (8 + 4) * kCoarseMax * ld + 4 * (kCoarseMax + 1) + N + 8 * N +

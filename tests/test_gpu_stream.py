@@ -844,3 +844,27 @@ def test_solve_with_enqueued_verification_matches_separate_verification():
         C = Z0.shape[1] - 1
         hmax = max(Z0[:, C].max(), Z1[:, C].max())
         np.testing.assert_allclose(Z0[:, :C], Z1[:, :C], rtol=0, atol=2.1e-6 * hmax)
+
+
+@pytest.mark.parametrize("n,K", [(3001, 8), (2999, 16), (4003, 64)])
+def test_fp8_directions_any_row_count(n, K):
+    """FP8 direction slots of n rows of ld bytes: for n*ld not a multiple of 16 the slots are
+    padded so every slot the operator gathers from stays aligned; solves with deferred updates
+    (several slots) certify against the rebuilt oracle, and the footprint equals what the
+    handle holds."""
+    from dynlp_amd import _lib
+    from dynlp_amd.stream import DIRECTION_KINDS, EdgeStreamLP
+    g = graphs.sbm(n, 8, 10, 0.8, K, 0.02, seed=61)
+    lp = EdgeStreamLP(g, eps=1e-6, pcg_directions="fp8", push_max_rows=0)
+    hb = _lib.load().dlp_lp_handle_bytes(lp.n, lp.C1, lp.ld, 0, DIRECTION_KINDS["fp8"], lp.pcg_iters_per_launch,
+                                         lp.pcg_coarse, 0)
+    assert lp.device_bytes() == hb
+    lp.initial_solve()
+    es = edge_dict(g)
+    for b in graphs.update_stream(g, 2, 300, seed=61)[0]:
+        st = lp.apply_and_solve(b)
+        assert "pcg" in st.path and st.rho <= 1e-6 * 1.0001
+        O.apply_edge_batch(es, zip(b.ins_u.tolist(), b.ins_v.tolist(), b.ins_w.tolist()),
+                           zip(b.del_u.tolist(), b.del_v.tolist()))
+    check_against_oracle(lp, g, es, 1e-6, f"fp8 n={n} K={K}")
+    assert lp.device_bytes() == hb
