@@ -349,7 +349,7 @@ def full_solves(g):
     return rows
 
 
-def pmc_probe(g, directions="fp8"):
+def pmc_probe(g, directions="auto"):
     """Launch sequence for hardware-counter runs (rocprofv3 --pmc ...): the counters of each
     kernel's launches are read against those of one plain copy of known size.  After the initial
     solve: 5 A*P launches (k_pcg_operator), the copy (512 MiB), 3 FP64 verification passes
@@ -472,7 +472,7 @@ def trace_summary(path, start=None):
     return "\n".join(out)
 
 
-def open_loop(g, batches, rates, seconds, seed, directions="fp8"):
+def open_loop(g, batches, rates, seconds, seed, directions="auto"):
     """The same request stream under open-loop Poisson arrivals (seed S+3) at each rate (requests
     per second), served by the continuous-batching service on the one resident graph: requests
     that arrive while a repair runs are fused into the next one.  Each rate starts from a fresh
@@ -563,9 +563,10 @@ def main(argv=None):
     ap.add_argument("--batch-edges", type=int, default=10_000)
     ap.add_argument("--scale", type=float, default=1.0, help="multiplies n (1.0 = SBM-1M)")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--directions", default="fp8", choices=["fp32", "bf16", "fp16", "fp8"],
-                    help="storage of the global solve's search directions on the service path "
-                         "(FP8: 6 %% more edge updates per second than FP16 on this workload)")
+    ap.add_argument("--directions", default="auto", choices=["auto", "fp32", "bf16", "fp16", "fp8"],
+                    help="storage of the global solve's search directions on the service path (auto, "
+                         "the library's default: FP8 for this workload's 65 columns, 6 %% more edge "
+                         "updates per second than FP16)")
     ap.add_argument("--fp64-steps", type=int, default=4,
                     help="requests timed on each path of the ladder from the plain FP64 path to the service")
     ap.add_argument("--cpu-batches", type=int, default=1, help="requests timed on the CPU port")
@@ -675,8 +676,9 @@ def main(argv=None):
     achieved = nbytes / (op_ms * 1e-3) / 1e9
     word, wid = {"fp32": ("F32x4", 0), "bf16": ("BF16x4", 2), "fp16": ("F16x4", 3), "fp8": ("F8x4", 5)}[dirs]
     # template arguments: gathers in flight per lane, lane word, waves per SIMD it is built for
-    # (the library's defaults: FP8 8 and 6, the others 4 and 7)
-    depth, waves = (8, 6) if dirs == "fp8" else (4, 7)
+    # (the library's defaults: FP8 on wide rows, at most 4 edge lanes, 8 and 6; else 4 and 7)
+    wide = 64 // ((lp.C1 + 3) // 4) <= 4
+    depth, waves = (8, 6) if dirs == "fp8" and wide else (4, 7)
     kname = (f"k_pcg_operator<{depth}, {word}, {waves}> (A*P on {dirs.upper()} directions with fused p^T A p, "
              "the global solve)")
     traffic = pmc_traffic(f"dlp::k_pcg_operator<{depth}, (dlp::Word){wid}, {waves}>")
@@ -736,7 +738,7 @@ def main(argv=None):
         for name, kw in LADDER:
             ladder.append(run_path(g, batches, Z0, name, kw, a.fp64_steps, a.batch_edges))
             torch.cuda.empty_cache()
-        ladder.append({"path": f"the service (above): FP32 state, {a.directions.upper()} directions, two-level "
+        ladder.append({"path": f"the service (above): FP32 state, {pcg_dirs.upper()} directions, two-level "
                                "preconditioner, hipGraph replay, push for small repairs", "value": round(value, 1),
                        "unit": "edge-updates/s", "requests": a.steps, "p50_ms": round(percentile(lat, 50), 2),
                        "p99_ms": round(percentile(lat, 99), 2)})

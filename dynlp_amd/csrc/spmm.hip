@@ -388,8 +388,11 @@ int pcg_operator(int n, int C, const int32_t* indptr, const int32_t* rowlen, con
   const bool w8 = !h16 && T.bf16_word == 8;
 #define DLP_PIPE_FN(SV, KV) reinterpret_cast<const void*>(&k_pcg_operator<SV, KV>)
   // FP8 words take one register per gather: 8 in flight per lane at 6 waves per SIMD measured
-  // 0.396 against 0.409 ms at 4 and 7 (SBM-1M, k = 64); 16-bit words spill at depth 6 or 8
-  const int S8 = T.f8_pipe;
+  // 0.396 against 0.409 ms at 4 and 7 on wide rows (SBM-1M, 65 columns: 3 edge lanes); on narrow
+  // rows (ER-1M, 17 columns: 12 edge lanes) 8 gathers overshoot most rows and depth 4 is
+  // faster (1.36M against 1.23M edge-updates/s).  16-bit words spill at depth 6 or 8
+  const int f8_lanes_e = 64 / ((C + 3) / 4);
+  const int S8 = T.f8_pipe ? T.f8_pipe : (f8_lanes_e <= 4 ? 8 : 4);
   const int f8w = T.f8_waves;
   const void* kfn = f8 ? (S8 == 8   ? (f8w == 5 ? reinterpret_cast<const void*>(&k_pcg_operator<8, Word::F8x4, 5>)
                                                : reinterpret_cast<const void*>(&k_pcg_operator<8, Word::F8x4, 6>))
@@ -859,7 +862,7 @@ extern "C" int dlp_tune_set(int key, int value) {
   // key 8: workgroups of the flat PCG vector passes (0 = 2048); key 9: non-temporal FP32 streams
   // in the PCG update and direction passes (1 default); key 10: deferred solution updates; key
   // 11: waves per SIMD the FP8 operator is compiled for at depth 6 or 8 (5 or 6); key 12:
-  // gathers per lane in flight in the FP8 operator (4, 6 or 8; default 8)
+  // gathers per lane in flight in the FP8 operator (4, 6 or 8; 0, the default: by row width)
   if (key == 0) {
     DLP_CHECK_ARG(value == 0 || value == 4 || value == 8, "tune: unroll must be 0, 4 or 8");
     tuning().unroll = value;
@@ -892,7 +895,8 @@ extern "C" int dlp_tune_set(int key, int value) {
     DLP_CHECK_ARG(value == 5 || value == 6, "tune: FP8 operator waves per SIMD at depth 6/8 must be 5 or 6");
     tuning().f8_waves = value;
   } else if (key == 12) {
-    DLP_CHECK_ARG(value == 4 || value == 6 || value == 8, "tune: FP8 operator depth must be 4, 6 or 8");
+    DLP_CHECK_ARG(value == 0 || value == 4 || value == 6 || value == 8,
+                  "tune: FP8 operator depth must be 0 (by row width), 4, 6 or 8");
     tuning().f8_pipe = value;
   } else {
     DLP_CHECK_ARG(false, "tune: unknown key %d", key);

@@ -29,6 +29,20 @@ from .solvers import System  # noqa: F401  (the batch system, reference stream.p
 _OPS_PER_CALL = 1 << 19     # operations one dlp_lp_apply_ops call takes (its pair table's capacity / 2)
 # storage of the global solve's search directions -> dlp_lp_set_pcg_directions kind
 DIRECTION_KINDS = {"fp32": 0, "bf16": 1, "fp16": 2, "fp8": 3}
+# "auto": FP8 from this many columns (label columns + h) up, FP16 below.  Measured on one MI355X:
+# 65 columns (SBM-1M, k = 64) FP8 6-7 % more edge updates per second; 17 and 33 columns (ER-1M,
+# k = 16; SBM-300k, k = 32) FP16 as fast or 6-7 % faster (narrow rows gain little from fewer
+# bytes per gather and FP8's step takes a few more iterations)
+FP8_MIN_COLUMNS = 48
+
+
+def resolve_directions(kind: str, C1: int) -> str:
+    """The concrete direction storage for a kind name ("auto" by column count)."""
+    if kind == "auto":
+        return "fp8" if C1 >= FP8_MIN_COLUMNS else "fp16"
+    if kind not in DIRECTION_KINDS:
+        raise ValueError(f"pcg_directions {kind!r}: 'auto', 'fp32', 'bf16', 'fp16' or 'fp8'")
+    return kind
 
 
 @dataclass
@@ -74,10 +88,11 @@ class EdgeStreamLP:
     eps_h: the h column's (solvers.py:247 default 0.05).  fp64=False is the mixed-precision
     service path (FP32 Z/R and push, FP64 residual verification and refinement); fp64=True keeps
     Z/R in FP64 and repairs with the host-driven push rounds of the reference.  pcg_directions:
-    storage of the global solve's search directions, "fp16" (default: half the gather bytes, a
-    power-of-two scale per column, 11 significant bits, step lengths exact for the rounded
-    direction), "fp8" (E4M3: a quarter of the bytes, 4 significant bits, the scale from a bound
-    on each column's largest value), "bf16" (half the bytes, 8 significant bits) or "fp32".  pcg_coarse: aggregates of
+    storage of the global solve's search directions: "auto" (default: "fp8" from 48 columns up,
+    "fp16" below), "fp16" (half the gather bytes, a power-of-two scale per column, 11 significant
+    bits, step lengths exact for the rounded direction), "fp8" (E4M3: a quarter of the bytes, 4
+    significant bits, the scale from a bound on each column's largest value), "bf16" (half the
+    bytes, 8 significant bits) or "fp32".  pcg_coarse: aggregates of
     the two-level preconditioner of the global solve (runs of consecutive rows; 0 = Jacobi only),
     its coarse operator rebuilt from the current equations every coarse_refresh global solves.
     pcg_iters_per_launch: global-solve iterations per replayed graph (iterations past
@@ -87,7 +102,7 @@ class EdgeStreamLP:
     def __init__(self, g: Graph, eps=1e-6, eps_h=0.05, fp64=False, eta=0.0, slack_min=8,
                  slack_frac=0.25, push_scale=0.9, rounds_per_launch=4, use_graph=True, max_rounds=200_000,
                  push_frac=0.05, push_max_rows=4096, push_budget=1.0, push_mass=0.15, pcg_iters_per_launch=7,
-                 pcg_max_iter=20_000, pcg_mode="column", incremental_ground=True, pcg_directions="fp16", pcg_coarse=128,
+                 pcg_max_iter=20_000, pcg_mode="column", incremental_ground=True, pcg_directions="auto", pcg_coarse=128,
                  coarse_refresh=64, fp64_repair="push"):
         self.g, self.n, self.K = g, g.n, g.K
         self.C = 1 if g.K == 2 else g.K
@@ -143,11 +158,9 @@ class EdgeStreamLP:
         _lib.call("dlp_lp_create", ct.byref(self._desc), ct.byref(h))
         self.h = h
         self._set_eps()
-        kinds = DIRECTION_KINDS
-        if pcg_directions not in kinds:
-            raise ValueError(f"pcg_directions {pcg_directions!r}: 'fp32', 'bf16', 'fp16' or 'fp8'")
+        pcg_directions = resolve_directions(pcg_directions, self.C1)
         self.pcg_directions = pcg_directions
-        _lib.call("dlp_lp_set_pcg_directions", self.h, 0 if self.fp64 else kinds[pcg_directions])
+        _lib.call("dlp_lp_set_pcg_directions", self.h, 0 if self.fp64 else DIRECTION_KINDS[pcg_directions])
         self.pcg_coarse = int(pcg_coarse)
         _lib.call("dlp_lp_set_coarse", self.h, self.pcg_coarse, int(coarse_refresh))
         if not self.fp64:
@@ -161,7 +174,7 @@ class EdgeStreamLP:
         self.nnz = int(ix.shape[0])
 
     @staticmethod
-    def footprint(g: Graph, fp64=False, slack_min=8, slack_frac=0.25, pcg_directions="fp16",
+    def footprint(g: Graph, fp64=False, slack_min=8, slack_frac=0.25, pcg_directions="auto",
                   pcg_coarse=128, pcg_iters_per_launch=7, pcg_mode="column", **_):
         """Device bytes of a resident graph built with these settings, before its first
         request: the arrays this object allocates (slack store, masks, labels, FP32 state, FP64
@@ -181,6 +194,7 @@ class EdgeStreamLP:
         if fp64:
             b += 2 * 8 * n * ld + 4 * n + 4 + 8 * C1       # Z64, R64, candidates, rho
         kinds = DIRECTION_KINDS
+        pcg_directions = resolve_directions(pcg_directions, C1)
         hb = _lib.load().dlp_lp_handle_bytes(n, C1, ld, int(fp64), kinds[pcg_directions], int(pcg_iters_per_launch),
                                               int(pcg_coarse), int(pcg_mode == "block"))
         if hb < 0:

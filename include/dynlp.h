@@ -496,8 +496,9 @@ int dlp_lp_time_operator(void* handle, int reps, double* ms_out, void* stream);
  * residual recurrence stays that of the FP32 path and the FP64 verification certifies as
  * before.  16-bit directions halve the bytes of every neighbour-row gather of the global
  * solve, FP8 quarters them; FP16 keeps 11 significant bits (BF16: 8, FP8: 4), and coarser
- * directions can cost iterations, never accuracy.  No reference counterpart (solvers.py:66-97
- * keeps P in the working dtype). */
+ * directions can cost iterations, never accuracy (the Python layer's default picks FP8 from 48
+ * columns up, FP16 below).  No reference counterpart (solvers.py:66-97 keeps P in the working
+ * dtype). */
 int dlp_lp_set_pcg_directions(void* handle, int kind);
 /* Host-side check of an edge batch before it is applied (no device work): ids (id_bytes 4 =
  * int32, 8 = int64) in [0, n), insertion weights finite and > 0.  Returns 0 (valid, no pair

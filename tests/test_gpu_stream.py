@@ -786,7 +786,7 @@ def test_low_precision_directions_certify_with_weights_over_six_decades(dirs):
 @pytest.mark.parametrize("depth,waves", [(4, 6), (6, 5), (6, 6), (8, 5)])
 def test_fp8_operator_depths_certify(depth, waves):
     """The FP8 operator at every depth (gathers per lane in flight) and register budget it is
-    built for (the default, depth 8 at 6 waves per SIMD, runs in the other FP8 tests): a global
+    built for (the default, depth by row width, runs in the other FP8 tests): a global
     solve from zero and two repairs, certified against the rebuilt oracle."""
     from dynlp_amd import _lib
     from dynlp_amd.stream import EdgeStreamLP
@@ -806,7 +806,7 @@ def test_fp8_operator_depths_certify(depth, waves):
         assert "pcg" in st.path
         check_against_oracle(lp, g, es, 1e-6, f"depth {depth} waves {waves}")
     finally:
-        _lib.call("dlp_tune_set", 12, 8)
+        _lib.call("dlp_tune_set", 12, 0)
         _lib.call("dlp_tune_set", 11, 6)
 
 
