@@ -1280,6 +1280,10 @@ struct LP {
   // only), rebuilt every coarse_refresh global solves; cg_* = the current layout and buffers
   int coarse_na = 0, coarse_refresh = 64;
   int cg_na = 0, cg_min = 0, cg_age = -1;
+  // fewest iterations of a global solve since the coarse operator was built, and whether a later
+  // solve needed a quarter more (the operator has gone stale: rebuild before the next solve)
+  int64_t cg_base_iters = 0;
+  int cg_stale = 0;
   int cg_builds = 0;            // coarse operators built; the boundaries are searched every 4th
   double* cg_Ac = nullptr;      // buffers sized for kCoarseMax aggregates
   double* cg_Ainv = nullptr;
@@ -2529,8 +2533,10 @@ static int pcg_iteration(LP* h, const PcgBufs& b, hipStream_t st) {
 }
 
 // The coarse operator of the two-level preconditioner, rebuilt from the current graph and
-// equations every coarse_refresh global solves (a stale Ac still gives an SPD preconditioner,
-// only a weaker one: the equations of a few percent of the rows change between rebuilds).
+// equations every coarse_refresh global solves, or before the next one once a solve needed a
+// quarter more iterations than the fewest since the build (a stale Ac still gives an SPD
+// preconditioner, only a weaker one: the equations of a few percent of the rows change between
+// rebuilds, more on a graph that fragments).
 // Aggregates are row ranges: coarse_na targets spaced S0 rows apart, each moved to the row
 // boundary with the fewest crossing edges within half a spacing; a moved boundary is kept when it
 // cuts fewer edges than the rows half a spacing to either side of it (on a graph whose ids follow
@@ -2563,10 +2569,11 @@ static int coarse_prepare(LP* h, hipStream_t st) {
     const int rc = coarse_alloc(h, st);
     if (rc) return rc;
   }
-  if (h->cg_age >= 0 && h->cg_age < h->coarse_refresh) {
+  if (h->cg_age >= 0 && h->cg_age < h->coarse_refresh && !h->cg_stale) {
     h->cg_age += 1;
     return DLP_OK;
   }
+  h->cg_stale = 0;
   const int min_agg = chunk > 32 ? chunk : 32;
   if (h->cg_age >= 0 && h->cg_na > 0 && min_agg == h->cg_min && (h->cg_builds % 4) != 0) {
     // a batch moves few edges across the row ranges: keep them, rebuild A_c from the current
@@ -2854,6 +2861,15 @@ static int lp_pcg(LP* h, int max_iter, int iters_per_launch, int use_graph, int 
     } else if (++stall >= patience) {
       break;
     }
+  }
+  // a coarse operator that has gone stale (on a fragmenting graph the equations drift from the
+  // ones it was built from) shows as a solve needing a quarter more iterations (and 2) than the
+  // fewest any solve needed since its build: rebuild it before the next solve instead of
+  // waiting for its age
+  if (h->cg_na && (h->cg_age == 1 || it < h->cg_base_iters)) {
+    h->cg_base_iters = it;
+  } else if (h->cg_na && h->cg_age > 1 && h->cg_base_iters > 0 && it >= h->cg_base_iters * 5 / 4 + 2) {
+    h->cg_stale = 1;
   }
   if (iters_out) *iters_out = it;
   if (viol_out) *viol_out = v;
