@@ -247,7 +247,7 @@ struct Tuning {
   int bf16_word = 4;  // BF16 direction values per lane in the global-solve operator (4 or 8)
   int flat_grid = 0;  // workgroups of the flat PCG vector passes (0: 2048)
   int nt_streams = 1; // non-temporal FP32 streams (Z, R, AP) in the PCG update and direction passes
-  int defer_z = 1;    // 16-bit directions: Z updated once per captured group of PCG iterations
+  int defer_z = 1;    // low-precision directions: Z updated once per captured group of PCG iterations
   int f8_pipe = 0;    // gathers per lane in flight in the FP8 global-solve operator (4, 6 or 8; 0:
                       // 8 when at most 4 edge lanes share a gather, i.e. wide rows, else 4)
   int f8_waves = 6;   // FP8 operator at depth 6 or 8: register budget for 5 or 6 waves per SIMD
