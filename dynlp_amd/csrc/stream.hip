@@ -1946,7 +1946,7 @@ struct ListedRefreshEpilogue {
 };
 
 template <int S>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_refresh_pipe(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_refresh_pipe(
     RefreshArgs A, int eqs, int xcd, int xbytes, float* __restrict__ Z, float* __restrict__ R) {
   const int nr = *A.rows_count;
   RowPipe G(nr, A.C1, 4, 16, xcd, A.solved, A.indptr, A.rowlen, A.col, A.w, Z, (unsigned)(A.ld * 4), xbytes);
