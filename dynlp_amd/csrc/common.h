@@ -119,6 +119,22 @@ __device__ __forceinline__ void atomic_max_nonneg(double* p, double v) {
   atomicMax((unsigned long long*)p, (unsigned long long)__double_as_longlong(v));
 }
 
+// Appends val to list for every active lane whose add is set, with one atomic per wavefront on
+// the shared counter (the lanes' slots follow their order in the wavefront) instead of one per
+// lane: lists that most rows of a pass join (changed rows of a batch, violators right after it)
+// otherwise serialise every wavefront on the one counter.  Every active lane must call it.
+__device__ __forceinline__ void wave_append(bool add, int val, int32_t* __restrict__ list,
+                                            int32_t* __restrict__ count) {
+  const unsigned long long m = __ballot(add);
+  if (m == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1);
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, (int)__popcll(m));
+  base = __builtin_amdgcn_readlane(base, leader);
+  if (add) list[base + (int)__popcll(m & ((1ull << lane) - 1ull))] = val;
+}
+
 // The same for a word in global memory that many workgroups send their maxima to: the atomic is
 // skipped when the value already there is at least v (most workgroups, once the first few have
 // written), which keeps thousands of atomics off the same L2 line.  A stale read only costs an

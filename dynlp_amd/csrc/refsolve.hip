@@ -114,7 +114,7 @@ static unsigned flat_grid(int64_t total) {
 
 __device__ __forceinline__ void first_touch(int v, int32_t* __restrict__ stamp, int32_t* __restrict__ touched,
                                             int32_t* __restrict__ ntouched) {
-  if (atomicExch(stamp + v, 1) == 0) touched[atomicAdd(ntouched, 1)] = v;
+  wave_append(atomicExch(stamp + v, 1) == 0, v, touched, ntouched);
 }
 
 template <typename T>
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void k_round_select(const int32_t* __restrict_
     const double dv = (double)diag[v];
     bool viol = false;
     for (int c = 0; c < C && !viol; ++c) viol = !(fabs((double)R[(int64_t)v * ld + c]) <= eps[c] * dv);
-    if (viol) next[atomicAdd(nnext, 1)] = v;
+    wave_append(viol, v, next, nnext);
   }
 }
 

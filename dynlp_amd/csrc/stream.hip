@@ -22,7 +22,7 @@ namespace dlp {
 
 __device__ __forceinline__ void stamp_append(int v, int32_t* __restrict__ stamp, int stampv,
                                              int32_t* __restrict__ list, int32_t* __restrict__ count) {
-  if (atomicExch(stamp + v, stampv) != stampv) list[atomicAdd(count, 1)] = v;
+  wave_append(atomicExch(stamp + v, stampv) != stampv, v, list, count);
 }
 
 __device__ __forceinline__ bool id_ok(int v, int n) { return (unsigned)v < (unsigned)n; }
@@ -660,7 +660,7 @@ __device__ __forceinline__ void refresh_row(const RefreshArgs& A, const Lanes& L
   const unsigned long long bal = __ballot(viol);
   const int wl = threadIdx.x & 63;
   const unsigned long long gm = (L.G == 64 ? ~0ull : ((1ull << L.G) - 1)) << (wl & ~(L.G - 1));
-  if (valid && L.lane == 0 && (bal & gm) && A.viol) A.viol[atomicAdd(A.nviol, 1)] = r;
+  wave_append(valid && L.lane == 0 && (bal & gm) && A.viol, r, A.viol, A.nviol);
   if (A.mass) {
     for (int off = 1; off < L.G; off <<= 1) m = fmax(m, __shfl_xor(m, off, 64));
     if (valid && L.lane == 0 && m > 1.0) mass += m;
@@ -803,7 +803,7 @@ __device__ __forceinline__ void refresh_row_v4(const RefreshArgs& A, const Lanes
     }
   }
   const unsigned long long bal = __ballot(viol);
-  if (valid && L.lane == 0 && bal && A.viol) A.viol[atomicAdd(A.nviol, 1)] = r;
+  wave_append(valid && L.lane == 0 && bal && A.viol, r, A.viol, A.nviol);
   if (A.mass) {
     for (int off = 1; off < 64; off <<= 1) m = fmax(m, __shfl_xor(m, off, 64));
     if (valid && L.lane == 0 && m > 1.0) mass += m;
@@ -996,19 +996,6 @@ struct PushState {
   unsigned long long* edges;
   unsigned long long* stats;  // [0] candidates, [1] pushes, [2] scatter entries, [3] unused
 };
-
-// Append val for every active lane with pred, one atomic per wavefront.
-__device__ __forceinline__ void wave_append(bool pred, int val, int32_t* __restrict__ list,
-                                            int32_t* __restrict__ count) {
-  const unsigned long long m = __ballot(pred);
-  if (m == 0ull) return;
-  const int lane = threadIdx.x & 63;
-  const int leader = __ffsll((long long)m) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(count, __popcll(m));
-  base = __shfl(base, leader, 64);
-  if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = val;
-}
 
 template <int CPL>
 __device__ __forceinline__ void push_round_body(const PushState& S, int parity, int g_shift, int lc_shift) {
@@ -1937,11 +1924,26 @@ struct ListedRefreshEpilogue {
       }
     }
     const unsigned long long bal = __ballot(viol);
-    if (L.lane == 0 && bal && A.viol) A.viol[atomicAdd(A.nviol, 1)] = r;
+    if (bal && A.viol) {
+      // most listed rows violate right after a batch: the wavefront keeps its violating rows in
+      // a register (lane k: the k-th) and reserves list space once per 64, instead of one
+      // atomic per row on the one counter every wavefront shares
+      if (L.lane == nbuf) vbuf = r;
+      if (++nbuf == 64) flush_viol();
+    }
     if (A.mass) {
       for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
       if (L.lane == 0 && m > 1.0) mass += m;
     }
+  }
+  int vbuf = 0, nbuf = 0;  // violating rows held for the list (nbuf wave-uniform)
+  __device__ __forceinline__ void flush_viol() {
+    if (nbuf == 0) return;
+    int base = 0;
+    if (L.lane == 0) base = atomicAdd(A.nviol, nbuf);
+    base = __shfl(base, 0, 64);
+    if (L.lane < nbuf) A.viol[base + L.lane] = vbuf;
+    nbuf = 0;
   }
 };
 
@@ -1953,6 +1955,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   G.rows = A.rows;
   ListedRefreshEpilogue E{G.L, A, Z, R, eqs != 0};
   run_rows<S, Word::F32x4, double>(G, E);
+  E.flush_viol();
   flush_mass(A.mass, E.mass);
   if (!A.rho) return;
   __shared__ double sm[256];
@@ -2412,7 +2415,7 @@ __global__ void k_filter64(const int32_t* __restrict__ idx, const int32_t* __res
   const double d = diag[v];
   bool viol = false;
   for (int c = 0; c < C1; ++c) viol |= !(fabs(R[(int64_t)v * ld + c]) <= eps[c] * d);
-  if (viol) out[atomicAdd(nout, 1)] = v;
+  wave_append(viol, v, out, nout);
 }
 
 extern "C" int dlp_lp_push64_next(void* handle, int32_t* cand_out, int32_t* count_out_dev, void* stream) {
