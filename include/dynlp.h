@@ -84,10 +84,13 @@ int dlp_row_operator_dot(int n, int C, const int32_t* indptr, const int32_t* row
  *   key 8 = workgroups of the flat PCG vector passes (0 default = 2048);
  *   key 9 = non-temporal FP32 streams (Z, R, AP) in the PCG update and direction passes
  *           (1 default: the caches keep the direction block the next operator gathers);
- *   key 10 = deferred solution updates with 16-bit directions (1 default: each captured group
- *           of at most 8 global-solve iterations keeps its directions, one per slot, and adds
- *           sum alpha_s P_s to Z once at its end; 0, or longer groups: Z updated by every
- *           iteration). */
+ *   key 10 = deferred solution updates with 16-bit or FP8 directions (1 default: each captured
+ *           group of at most 8 global-solve iterations keeps its directions, one per slot, and
+ *           adds sum alpha_s P_s to Z once at its end; 0, or longer groups: Z updated by every
+ *           iteration);
+ *   key 11 = waves per SIMD the FP8 operator is built for at depth 6 or 8 (6 default; 5);
+ *   key 12 = gathers in flight per lane in the FP8 operator (0 default: 8 when at most 4 edge
+ *           lanes share a gather instruction, i.e. wide rows, else 4; or 4, 6, 8). */
 int dlp_tune_set(int key, int value);
 
 /* ---- Column residual ratios ------------------------------------------------------------
