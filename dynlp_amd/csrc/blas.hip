@@ -493,8 +493,9 @@ __global__ __launch_bounds__(256) void k_pcg_update4(int n, int Q4, int C, float
         if (j < q.cnt) {
           const float x = quad_comp(rv, j);
           acc[j] += (double)x * (double)x * (double)inv;
-          vm = fmaxf(vm, fabsf(x) * inv * ie[j]);
-          if constexpr (PK == 3) rm[j] = fmaxf(rm[j], fabsf(x) * inv);
+          const float t = fabsf(x) * inv;
+          vm = fmaxf(vm, t * ie[j]);
+          if constexpr (PK == 3) rm[j] = fmaxf(rm[j], t);
           if (K.na) rs.add(r, j, (double)x);
         }
       }

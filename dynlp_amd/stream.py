@@ -563,15 +563,17 @@ class EdgeStreamLP:
         this batch), a regrow after overflowed insertions, and the request's statistics."""
         st = req["st"]
         prec = 1 if self.fp64 else 0
-        # a large batch (no status read after the first half) goes straight to the global
-        # solve; an overflow of the store (rare) shows in the status the repair ends with and
-        # is regrown and repaired below, so no host round trip waits for it here
+        if req["status"] is None:
+            # a large batch: settled here, in the thread that repairs it (no status was read
+            # after the first half): an overflowed store is regrown before the repair, which
+            # then goes straight to the global solve
+            self._settle(req)
         s = self._repair(st, pushed=pushed, s0=req["status"])
         if s["invalid"] or s["duplicates"]:
             raise RuntimeError(f"device apply refused {s['invalid']} pairs and merged {s['duplicates']} "
                                "repeated entries that the host checks had passed")
         if s["overflow"]:
-            # an overflow of a large batch's apply (a small batch's is regrown before its repair)
+            # (the apply's overflow is regrown before the repair; this only guards the counter)
             self._regrow(req["ins"], self._regrow_extra(req["ins_rows"]))
             _lib.call("dlp_lp_ground", self.h, stream())   # re-inserted edges can re-ground rows
             _lib.call("dlp_lp_refresh", self.h, 0, prec, stream())

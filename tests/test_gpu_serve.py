@@ -55,7 +55,8 @@ def test_alone_equals_mixed(tight):
     svc = Service(gs, budget, slots=3, eps=EPS)
     done = svc.run(reqs)
     rep = svc.report()
-    assert rep["requests"] == len(reqs)
+    errs = [(r.gid, r.error) for r in done if r.error is not None]
+    assert rep["requests"] == len(reqs), (len(done), errs[:5], {k: rep[k] for k in ("evictions", "failed_repairs")})
     assert rep["max_rho"] <= EPS * 1.0001
     if tight:
         assert rep["evictions"] > 0

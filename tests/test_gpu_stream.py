@@ -460,10 +460,9 @@ def test_batch_edge_cases_match_oracle():
 
 @pytest.mark.parametrize("push_max_rows", [None, 0])
 def test_overflow_regrow_keeps_results(push_max_rows):
-    """Rows without slack overflow on insertion and the store is regrown: before the repair for
-    batches a push may repair (status read after the apply), after the global solve for large
-    batches (push_max_rows 0: the overflow shows in the status the repair ends with, then the
-    regrow, re-grounding, refresh and a second repair)."""
+    """Rows without slack overflow on insertion and the store is regrown before the repair, for
+    batches a push may repair (status read after the apply) and for large batches that go
+    straight to the global solve (push_max_rows 0: settled in the repairing step)."""
     from dynlp_amd.stream import EdgeStreamLP
     g = graphs.sbm(2000, 4, 6, 0.8, 4, 0.03, seed=3)
     eps = 1e-6
