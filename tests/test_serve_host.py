@@ -242,3 +242,17 @@ def test_library_batch_check_matches_brute_force():
             rep = lambda u, v: len({(min(a, b), max(a, b)) for a, b in zip(u.tolist(), v.tolist())}) < len(u)  # noqa: E731
             assert rc == (1 if rep(du, dv) else 0) | (2 if rep(iu, iv) else 0), trial
     assert lib.dlp_check_edge_batch(0, 0, None, None, 0, None, None, None, 4, None) == -3
+
+
+def test_direction_kind_by_column_count():
+    """The default direction storage ("auto") is FP8 from 48 columns (label columns + h) up and
+    FP16 below; explicit kinds pass through and unknown ones are refused."""
+    import pytest
+    from dynlp_amd.stream import DIRECTION_KINDS, FP8_MIN_COLUMNS, resolve_directions
+    assert FP8_MIN_COLUMNS == 48
+    assert resolve_directions("auto", 65) == "fp8" and resolve_directions("auto", 48) == "fp8"
+    assert resolve_directions("auto", 47) == "fp16" and resolve_directions("auto", 17) == "fp16"
+    for k in DIRECTION_KINDS:
+        assert resolve_directions(k, 9) == k
+    with pytest.raises(ValueError):
+        resolve_directions("fp4", 65)
